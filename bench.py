@@ -1,0 +1,196 @@
+#!/usr/bin/env python3
+"""Throughput of the plain FLIP transfer + advection step on one GPU.
+
+One step = cell sort -> P2G gather (u, v, w with weights, normalisation,
+snapshot, body force on fluid-bordering faces) -> G2P with PIC/FLIP blend and
+RK4 advection, over every particle of the scene, inputs resident in HBM.
+The default workload is the 256^3 pool-plus-spheres scene (~43M particles,
+dx = 1/32, dt = 1/240, curl noise at 3 m/s, seed 1003), trilinear stencil.
+
+Prints ONE JSON line (rank 0) with the throughput in particle-steps/s, the
+roofline of the dominant kernel (HIP events on the library's stream), the
+whole-step roofline, and a CPU baseline: the reference's own CPU code
+(oracle/_ref, single thread) run on a bounded x-slab of the same scene.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "FLIP transfer+advect particle-steps/s at 256³; batched & fwd+bwd vs plain"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def face_count(dims):
+    ni, nj, nk = dims
+    return (ni + 1) * nj * nk + ni * (nj + 1) * nk + ni * nj * (nk + 1)
+
+
+def algorithmic_bytes(P, F, C):
+    """Bytes each stage must move at minimum (FP32 / uint32):
+    sort: read positions (12P) + write the permutation (4P) + one pass over
+          the cell counts (8C);
+    p2g : read x, v once (24P) + write one value per face (4F);
+    g2p : read x, v (24P) + write x', v' (24P) + read one value per face (4F);
+    step: the whole step, 48P + 8F."""
+    return dict(sort=16 * P + 8 * C, p2g=24 * P + 4 * F, g2p=48 * P + 4 * F,
+                step=48 * P + 8 * F)
+
+
+def cpu_baseline(scene, stencil, target_s=15.0):
+    """Time the reference's CPU path (its own compiled sources, one thread)
+    on an x-slab of the scene sized for ~target_s seconds."""
+    import numpy as np
+    from oracle import oracle
+    lib = oracle.load_reference()
+    kind = "reference"
+    if lib is None:
+        lib = oracle.load_oracle()
+        kind = "port"
+    ni, nj, nk = scene.dims
+
+    def run(cells):
+        keep = scene.pos[:, 0] < cells * scene.dx
+        pos = np.ascontiguousarray(scene.pos[keep])
+        vel = np.ascontiguousarray(scene.vel[keep])
+        t0 = time.perf_counter()
+        lib.flip_step(pos, vel, (cells, nj, nk), scene.dx, scene.dt, scene.body_force,
+                      0.05, stencil)
+        return pos.shape[0], time.perf_counter() - t0
+
+    cells = max(1, ni // 64)
+    p, t = run(cells)
+    if t > 0 and cells < ni:
+        cells = int(min(ni, max(cells, round(cells * target_s / t))))
+        p, t = run(cells)
+    return dict(value=p / t, unit="particle-steps/s", cores=1, kind=kind,
+                sample="one plain step on the x-slab [0,%d) x %d x %d of the same scene "
+                       "(%d particles), %.1f s" % (cells, nj, nk, p, t),
+                nproc=os.cpu_count(), cpu=_cpu_model())
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="pool256",
+                    choices=["pool256", "dambreak128", "sphere64"])
+    ap.add_argument("--stencil", default="trilinear", choices=["trilinear", "tricubic"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(requested --gpus %d, WORLD_SIZE %d)" % (args.gpus, world), file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import gridfluidsim3d_amd as gf
+    from gridfluidsim3d_amd import scenes
+
+    stencil = gf.TRILINEAR if args.stencil == "trilinear" else gf.TRICUBIC
+    t0 = time.time()
+    scene = scenes.CONFIGS[args.workload]()
+    P = scene.n
+    F = face_count(scene.dims)
+    C = (scene.dims[0] + 2) * (scene.dims[1] + 2) * (scene.dims[2] + 2)
+    log("scene %s: %d particles, %d faces, built in %.1f s" % (scene.name, P, F, time.time() - t0))
+
+    ctx = gf.Context(0)
+    log(ctx.device_info)
+    st = gf.FlipState(scene.pos, scene.vel, scene.dims, scene.dx)
+    stream = torch.cuda.current_stream()
+
+    def step():
+        ctx.flip_step(st, scene.dt, scene.body_force, 0.05, stencil, gf.ORDER_SORTED)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    log("warmup done (%d steps)" % args.warmup)
+
+    ctx.timing_begin()
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record(stream)
+    for _ in range(args.steps):
+        step()
+    end.record(stream)
+    torch.cuda.synchronize()
+    elapsed_ms = start.elapsed_time(end)
+    nsteps, stage_ms = ctx.timing_end()
+    assert nsteps == args.steps
+    ms_per_step = elapsed_ms / args.steps
+    value = P * args.steps / (elapsed_ms / 1e3)
+    stage_avg = {k: v / nsteps for k, v in stage_ms.items()}
+    log("ms/step %.3f  stages %s" % (ms_per_step, {k: round(v, 3) for k, v in stage_avg.items()}))
+
+    alg = algorithmic_bytes(P, F, C)
+    dom = max(("sort", "p2g", "g2p"), key=lambda k: stage_avg[k])
+    achieved = alg[dom] / (stage_avg[dom] / 1e3) / 1e9
+    roofline = dict(bound="hbm", kernel=dict(sort="cell sort (5 kernels)", p2g="k_flip_p2g",
+                                             g2p="k_flip_g2p")[dom],
+                    achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                    algorithmic_bytes=alg[dom], avg_ms=round(stage_avg[dom], 4))
+    step_gbs = alg["step"] / (ms_per_step / 1e3) / 1e9
+    step_roofline = dict(achieved=round(step_gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                         frac=round(step_gbs / HBM_PEAK_GBS, 4),
+                         algorithmic_bytes=alg["step"])
+
+    cpu = None
+    if not args.no_cpu:
+        log("CPU baseline ...")
+        cpu = cpu_baseline(scene, stencil, args.cpu_seconds)
+        log("CPU baseline: %s" % cpu)
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": dict(scene.describe(), stencil=args.stencil, form="plain",
+                       pic_ratio=0.05, body_force=list(scene.body_force)),
+        "roofline": roofline,
+        "step_roofline": step_roofline,
+        "stages_ms": {k: round(v, 4) for k, v in stage_avg.items()},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,554 @@
+// gfsx_api.hip -- the C ABI declared in include/gfsx.h.
+//
+// Error convention of the reference's C bindings (cbindings.cpp:9-19): a
+// trailing int* err receives GFSX_SUCCESS/GFSX_FAIL and the message of the
+// last failure is kept per thread.  Nothing here aborts: HIP errors and bad
+// arguments become GFSX_FAIL with a message, where the reference would call
+// FLUIDSIM_ASSERT (clscalarfield.cpp:519-524, particleadvector.cpp:467-472).
+
+#include "gfsx.h"
+#include "gfsx_internal.h"
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <vector>
+#include <cstring>
+#include <sstream>
+#include <string>
+
+using gfsx::Error;
+
+namespace {
+
+thread_local std::string g_error = "";
+
+void fail(int *err, const std::string &msg) {
+    g_error = msg;
+    if (err) *err = GFSX_FAIL;
+}
+
+void ok(int *err) {
+    if (err) *err = GFSX_SUCCESS;
+}
+
+enum Slot {
+    S_COUNT, S_START, S_KEY, S_SLOT, S_PERM, S_BOUT, S_BOUTS, S_SCAN,
+    S_OLD0, S_OLD1, S_OLD2,
+    S_D_POS, S_D_VAL, S_D_S, S_D_W, S_D_U, S_D_V, S_D_WW, S_D_OUT,
+    NSLOT
+};
+
+void require(bool cond, const char *msg) {
+    if (!cond) throw Error(msg);
+}
+
+}  // namespace
+
+struct gfsx_ctx {
+    int device = -1;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    std::string info;
+    void *buf[NSLOT] = {};
+    size_t bytes[NSLOT] = {};
+    bool timing = false;
+    std::vector<std::array<hipEvent_t, 4>> events;
+    size_t events_used = 0;
+
+    hipEvent_t *next_events() {
+        if (events_used == events.size()) {
+            std::array<hipEvent_t, 4> e{};
+            for (auto &x : e) GFSX_HIP(hipEventCreate(&x));
+            events.push_back(e);
+        }
+        return events[events_used++].data();
+    }
+
+    // Device workspace, grown on demand and kept until gfsx_destroy.
+    void *get(int slot, size_t nbytes) {
+        nbytes = std::max<size_t>(nbytes, 256);
+        if (bytes[slot] < nbytes) {
+            if (buf[slot]) GFSX_HIP(hipFree(buf[slot]));
+            buf[slot] = nullptr;
+            bytes[slot] = 0;
+            GFSX_HIP(hipMalloc(&buf[slot], nbytes));
+            bytes[slot] = nbytes;
+        }
+        return buf[slot];
+    }
+
+    uint32_t *u32(int slot, int64_t count) {
+        return (uint32_t *)get(slot, sizeof(uint32_t) * (size_t)std::max<int64_t>(count, 1));
+    }
+    float *f32(int slot, int64_t count) {
+        return (float *)get(slot, sizeof(float) * (size_t)std::max<int64_t>(count, 1));
+    }
+
+    gfsx::SortBuffers sort_buffers(int64_t n, uint32_t nbin) {
+        gfsx::SortBuffers sb;
+        int64_t nb = gfsx::sort_blocks(n);
+        sb.count = u32(S_COUNT, (int64_t)nbin + 1);
+        sb.start = u32(S_START, (int64_t)nbin + 2);
+        sb.key = u32(S_KEY, n);
+        sb.slot = u32(S_SLOT, n);
+        sb.perm = u32(S_PERM, n);
+        sb.block_out = u32(S_BOUT, nb);
+        sb.block_out_scan = u32(S_BOUTS, nb + 1);
+        int64_t words = std::max(gfsx::scan_scratch_words((int64_t)nbin + 1),
+                                 gfsx::scan_scratch_words(nb));
+        sb.scan_scratch = u32(S_SCAN, words);
+        return sb;
+    }
+
+    ~gfsx_ctx() {
+        hipSetDevice(device);
+        for (int s = 0; s < NSLOT; s++)
+            if (buf[s]) hipFree(buf[s]);
+        for (auto &e : events)
+            for (auto &x : e) hipEventDestroy(x);
+        if (own_stream) hipStreamDestroy(own_stream);
+    }
+};
+
+namespace {
+
+template <class F>
+void guarded(gfsx_ctx *ctx, int *err, F &&f) {
+    try {
+        if (!ctx) throw Error("gfsx: null context");
+        GFSX_HIP(hipSetDevice(ctx->device));
+        f();
+        ok(err);
+    } catch (const std::exception &e) {
+        fail(err, e.what());
+    } catch (...) {
+        fail(err, "gfsx: unknown error");
+    }
+}
+
+void check_grid(int ni, int nj, int nk, double dx) {
+    require(ni > 0 && nj > 0 && nk > 0, "gfsx: grid dimensions must be positive");
+    require(std::isfinite(dx) && dx > 0.0, "gfsx: dx must be positive and finite");
+    int64_t faces = (int64_t)(ni + 1) * (nj + 1) * (nk + 1);
+    require(faces < (int64_t)1 << 31, "gfsx: grid too large (more than 2^31 entries per array)");
+}
+
+void check_particles(int64_t n, const void *p) {
+    require(n >= 0, "gfsx: negative particle count");
+    require(n < ((int64_t)1 << 32) - 1, "gfsx: more than 2^32-2 particles in one call");
+    require(n == 0 || p != nullptr, "gfsx: null particle array");
+}
+
+gfsx::MacGrid mac_grid(const float *u, const float *v, const float *w, int ni,
+                       int nj, int nk, double dx) {
+    gfsx::MacGrid g;
+    g.f[0] = u;
+    g.f[1] = v;
+    g.f[2] = w;
+    g.ni = ni;
+    g.nj = nj;
+    g.nk = nk;
+    g.dx = dx;
+    g.inv_dx = 1.0 / dx;
+    g.lim[0] = dx * ni;
+    g.lim[1] = dx * nj;
+    g.lim[2] = dx * nk;
+    return g;
+}
+
+int64_t mac_count(int c, int ni, int nj, int nk) {
+    return (int64_t)(ni + (c == 0)) * (nj + (c == 1)) * (nk + (c == 2));
+}
+
+// P2G on device arrays: bin by node cell in the frame shifted by `offset`,
+// then gather.  Bins cover every node cell a contributing point can lie in:
+// node n gathers cells [n + floor(-R/dx), n + ceil(R/dx) - 1].
+void p2g_device_impl(gfsx_ctx *c, const float *pos, const float *val,
+                     int64_t n, double radius, const float offset[3],
+                     double dx, int ni, int nj, int nk, float *S, float *W) {
+    check_grid(ni, nj, nk, dx);
+    check_particles(n, pos);
+    require(std::isfinite(radius) && radius > 0.0, "gfsx: radius must be positive and finite");
+    require(S != nullptr, "gfsx: null scalar field");
+    require(offset != nullptr, "gfsx: null offset");
+    const double r = radius / dx;
+    require(r <= 64.0, "gfsx: radius larger than 64 cells");
+    const int pad = (int)(-std::floor(-r));
+    const int up = (int)std::ceil(r) - 1;
+    gfsx::BinGrid bg;
+    gfsx::NodeGrid ng;
+    const int dims[3] = {ni, nj, nk};
+    int64_t nbin = 1;
+    for (int a = 0; a < 3; a++) {
+        bg.b[a] = dims[a] + pad + up;
+        bg.pad[a] = pad;
+        bg.off[a] = offset[a];
+        ng.n[a] = dims[a];
+        ng.off[a] = offset[a];
+        ng.dlo[a] = (int)std::floor(pad - r);
+        ng.dhi[a] = (int)std::ceil(pad + r) - 1;
+        nbin *= bg.b[a];
+    }
+    require(nbin < ((int64_t)1 << 32) - 2, "gfsx: too many bins");
+    bg.inv_dx = 1.0 / dx;
+    bg.nbin = (uint32_t)nbin;
+    ng.dx_f = (float)dx;
+    gfsx::SortBuffers sb = c->sort_buffers(n, bg.nbin);
+    gfsx::cell_sort(pos, nullptr, nullptr, nullptr, n, bg, sb, c->stream);
+    gfsx::p2g_gather(pos, val, n, bg, ng, radius, dx, sb, S, W, c->stream);
+}
+
+void copy_in(void *dst, const void *src, size_t bytes, hipStream_t s) {
+    if (bytes) GFSX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+}
+
+void copy_out(void *dst, const void *src, size_t bytes, hipStream_t s) {
+    if (bytes) GFSX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *gfsx_get_error_message(void) { return g_error.c_str(); }
+
+int gfsx_get_device_count(int *err) {
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) {
+        fail(err, std::string("hipGetDeviceCount: ") + hipGetErrorString(e));
+        return 0;
+    }
+    ok(err);
+    return n;
+}
+
+gfsx_ctx *gfsx_create(int device, int *err) {
+    gfsx_ctx *c = nullptr;
+    try {
+        int n = 0;
+        GFSX_HIP(hipGetDeviceCount(&n));
+        require(device >= 0 && device < n, "gfsx_create: no such device");
+        c = new gfsx_ctx();
+        c->device = device;
+        GFSX_HIP(hipSetDevice(device));
+        GFSX_HIP(hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking));
+        c->stream = c->own_stream;
+        hipDeviceProp_t prop;
+        GFSX_HIP(hipGetDeviceProperties(&prop, device));
+        std::ostringstream ss;
+        ss << "device " << device << ": " << prop.name << " (" << prop.gcnArchName
+           << "), " << prop.multiProcessorCount << " CUs, "
+           << (prop.totalGlobalMem >> 20) << " MiB";
+        c->info = ss.str();
+        ok(err);
+        return c;
+    } catch (const std::exception &e) {
+        delete c;
+        fail(err, e.what());
+        return nullptr;
+    }
+}
+
+void gfsx_destroy(gfsx_ctx *ctx, int *err) {
+    if (!ctx) {
+        ok(err);
+        return;
+    }
+    try {
+        hipSetDevice(ctx->device);
+        hipStreamSynchronize(ctx->stream);
+        delete ctx;
+        ok(err);
+    } catch (const std::exception &e) {
+        fail(err, e.what());
+    }
+}
+
+void gfsx_set_stream(gfsx_ctx *ctx, void *stream, int *err) {
+    guarded(ctx, err, [&] {
+        ctx->stream = (hipStream_t)stream;
+    });
+}
+
+void gfsx_synchronize(gfsx_ctx *ctx, int *err) {
+    guarded(ctx, err, [&] { GFSX_HIP(hipStreamSynchronize(ctx->stream)); });
+}
+
+const char *gfsx_get_device_info(gfsx_ctx *ctx, int *err) {
+    if (!ctx) {
+        fail(err, "gfsx: null context");
+        return "";
+    }
+    ok(err);
+    return ctx->info.c_str();
+}
+
+// ---------------------------------------------------------------- P2G
+
+void gfsx_p2g_device(gfsx_ctx *ctx, const float *pos, const float *val,
+                     int64_t n, double radius, const float offset[3],
+                     double dx, int ni, int nj, int nk, float *S, float *W,
+                     int *err) {
+    guarded(ctx, err, [&] {
+        p2g_device_impl(ctx, pos, val, n, radius, offset, dx, ni, nj, nk, S, W);
+    });
+}
+
+void gfsx_p2g(gfsx_ctx *ctx, const float *pos, const float *val, int64_t n,
+              double radius, const float offset[3], double dx, int ni, int nj,
+              int nk, float *S, float *W, int *err) {
+    guarded(ctx, err, [&] {
+        check_grid(ni, nj, nk, dx);
+        check_particles(n, pos);
+        require(S != nullptr, "gfsx: null scalar field");
+        const int64_t nn = (int64_t)ni * nj * nk;
+        hipStream_t s = ctx->stream;
+        float *dpos = ctx->f32(S_D_POS, 3 * n);
+        float *dval = val ? ctx->f32(S_D_VAL, n) : nullptr;
+        float *dS = ctx->f32(S_D_S, nn);
+        float *dW = W ? ctx->f32(S_D_W, nn) : nullptr;
+        copy_in(dpos, pos, sizeof(float) * 3 * n, s);
+        if (val) copy_in(dval, val, sizeof(float) * n, s);
+        copy_in(dS, S, sizeof(float) * nn, s);
+        if (W) copy_in(dW, W, sizeof(float) * nn, s);
+        p2g_device_impl(ctx, dpos, dval, n, radius, offset, dx, ni, nj, nk, dS, dW);
+        copy_out(S, dS, sizeof(float) * nn, s);
+        if (W) copy_out(W, dW, sizeof(float) * nn, s);
+        GFSX_HIP(hipStreamSynchronize(s));
+    });
+}
+
+void gfsx_apply_weight_device(gfsx_ctx *ctx, float *S, const float *W,
+                              int64_t count, int *err) {
+    guarded(ctx, err, [&] {
+        require(count >= 0, "gfsx: negative count");
+        require(count == 0 || (S && W), "gfsx: null field");
+        gfsx::apply_weight(S, W, count, ctx->stream);
+    });
+}
+
+// ---------------------------------------------------------------- G2P
+
+void gfsx_g2p_device(gfsx_ctx *ctx, const float *pos, int64_t n,
+                     const float *u, const float *v, const float *w, int ni,
+                     int nj, int nk, double dx, int stencil, float *out,
+                     int *err) {
+    guarded(ctx, err, [&] {
+        check_grid(ni, nj, nk, dx);
+        check_particles(n, pos);
+        require(stencil == GFSX_TRILINEAR || stencil == GFSX_TRICUBIC, "gfsx: unknown stencil");
+        require(u && v && w, "gfsx: null velocity grid");
+        require(n == 0 || out, "gfsx: null output");
+        gfsx::g2p(pos, n, mac_grid(u, v, w, ni, nj, nk, dx), stencil, out, ctx->stream);
+    });
+}
+
+void gfsx_g2p(gfsx_ctx *ctx, const float *pos, int64_t n, const float *u,
+              const float *v, const float *w, int ni, int nj, int nk,
+              double dx, int stencil, float *out, int *err) {
+    guarded(ctx, err, [&] {
+        check_grid(ni, nj, nk, dx);
+        check_particles(n, pos);
+        require(stencil == GFSX_TRILINEAR || stencil == GFSX_TRICUBIC, "gfsx: unknown stencil");
+        require(u && v && w, "gfsx: null velocity grid");
+        require(n == 0 || out, "gfsx: null output");
+        hipStream_t s = ctx->stream;
+        float *dpos = ctx->f32(S_D_POS, 3 * n);
+        float *dout = ctx->f32(S_D_OUT, 3 * n);
+        float *dg[3];
+        const float *hg[3] = {u, v, w};
+        const int slots[3] = {S_D_U, S_D_V, S_D_WW};
+        for (int c = 0; c < 3; c++) {
+            int64_t m = mac_count(c, ni, nj, nk);
+            dg[c] = ctx->f32(slots[c], m);
+            copy_in(dg[c], hg[c], sizeof(float) * m, s);
+        }
+        copy_in(dpos, pos, sizeof(float) * 3 * n, s);
+        gfsx::g2p(dpos, n, mac_grid(dg[0], dg[1], dg[2], ni, nj, nk, dx), stencil, dout, s);
+        copy_out(out, dout, sizeof(float) * 3 * n, s);
+        GFSX_HIP(hipStreamSynchronize(s));
+    });
+}
+
+// ---------------------------------------------------------------- RK
+
+void gfsx_advect_device(gfsx_ctx *ctx, int order, const float *pos, int64_t n,
+                        const float *u, const float *v, const float *w, int ni,
+                        int nj, int nk, double dx, double dt, int stencil,
+                        float *out, int *err) {
+    guarded(ctx, err, [&] {
+        check_grid(ni, nj, nk, dx);
+        check_particles(n, pos);
+        require(order >= 1 && order <= 4, "gfsx: Runge-Kutta order must be 1..4");
+        require(stencil == GFSX_TRILINEAR || stencil == GFSX_TRICUBIC, "gfsx: unknown stencil");
+        require(std::isfinite(dt), "gfsx: dt must be finite");
+        require(u && v && w, "gfsx: null velocity grid");
+        require(n == 0 || out, "gfsx: null output");
+        gfsx::advect(pos, n, mac_grid(u, v, w, ni, nj, nk, dx), order, dt, stencil, out,
+                     ctx->stream);
+    });
+}
+
+void gfsx_advect(gfsx_ctx *ctx, int order, const float *pos, int64_t n,
+                 const float *u, const float *v, const float *w, int ni,
+                 int nj, int nk, double dx, double dt, int stencil, float *out,
+                 int *err) {
+    guarded(ctx, err, [&] {
+        check_grid(ni, nj, nk, dx);
+        check_particles(n, pos);
+        require(order >= 1 && order <= 4, "gfsx: Runge-Kutta order must be 1..4");
+        require(stencil == GFSX_TRILINEAR || stencil == GFSX_TRICUBIC, "gfsx: unknown stencil");
+        require(std::isfinite(dt), "gfsx: dt must be finite");
+        require(u && v && w, "gfsx: null velocity grid");
+        require(n == 0 || out, "gfsx: null output");
+        hipStream_t s = ctx->stream;
+        float *dpos = ctx->f32(S_D_POS, 3 * n);
+        float *dout = ctx->f32(S_D_OUT, 3 * n);
+        float *dg[3];
+        const float *hg[3] = {u, v, w};
+        const int slots[3] = {S_D_U, S_D_V, S_D_WW};
+        for (int c = 0; c < 3; c++) {
+            int64_t m = mac_count(c, ni, nj, nk);
+            dg[c] = ctx->f32(slots[c], m);
+            copy_in(dg[c], hg[c], sizeof(float) * m, s);
+        }
+        copy_in(dpos, pos, sizeof(float) * 3 * n, s);
+        gfsx::advect(dpos, n, mac_grid(dg[0], dg[1], dg[2], ni, nj, nk, dx), order, dt,
+                     stencil, dout, s);
+        copy_out(out, dout, sizeof(float) * 3 * n, s);
+        GFSX_HIP(hipStreamSynchronize(s));
+    });
+}
+
+// ---------------------------------------------------------------- step
+
+void gfsx_flip_step(gfsx_ctx *ctx, const gfsx_scene *sc,
+                    const gfsx_step_params *pr, int *err) {
+    guarded(ctx, err, [&] {
+        require(sc && pr, "gfsx_flip_step: null scene or params");
+        check_grid(sc->ni, sc->nj, sc->nk, sc->dx);
+        check_particles(sc->n, sc->x_in);
+        require(pr->stencil == GFSX_TRILINEAR || pr->stencil == GFSX_TRICUBIC,
+                "gfsx_flip_step: unknown stencil");
+        require(pr->order == GFSX_ORDER_SORTED || pr->order == GFSX_ORDER_KEEP,
+                "gfsx_flip_step: unknown output order");
+        require(std::isfinite(pr->dt), "gfsx_flip_step: dt must be finite");
+        require(sc->u && sc->v && sc->w, "gfsx_flip_step: null grid output");
+        if (sc->n > 0) {
+            require(sc->x_in && sc->y_in && sc->z_in && sc->vx_in && sc->vy_in && sc->vz_in,
+                    "gfsx_flip_step: null particle input");
+            require(sc->x_out && sc->y_out && sc->z_out && sc->vx_out && sc->vy_out && sc->vz_out,
+                    "gfsx_flip_step: null particle output");
+            const void *ins[6] = {sc->x_in, sc->y_in, sc->z_in, sc->vx_in, sc->vy_in, sc->vz_in};
+            const void *outs[6] = {sc->x_out, sc->y_out, sc->z_out, sc->vx_out, sc->vy_out, sc->vz_out};
+            for (int a = 0; a < 6; a++)
+                for (int b = 0; b < 6; b++)
+                    require(ins[a] != outs[b], "gfsx_flip_step: outputs must not alias inputs");
+        }
+        const int ni = sc->ni, nj = sc->nj, nk = sc->nk;
+        int64_t nbin = (int64_t)(ni + 2) * (nj + 2) * (nk + 2);
+        require(nbin < ((int64_t)1 << 32) - 2, "gfsx_flip_step: too many cells");
+        gfsx::StepArgs a;
+        a.n = sc->n;
+        a.ni = ni;
+        a.nj = nj;
+        a.nk = nk;
+        a.dx = sc->dx;
+        a.xin[0] = sc->x_in;
+        a.xin[1] = sc->y_in;
+        a.xin[2] = sc->z_in;
+        a.vin[0] = sc->vx_in;
+        a.vin[1] = sc->vy_in;
+        a.vin[2] = sc->vz_in;
+        a.xout[0] = sc->x_out;
+        a.xout[1] = sc->y_out;
+        a.xout[2] = sc->z_out;
+        a.vout[0] = sc->vx_out;
+        a.vout[1] = sc->vy_out;
+        a.vout[2] = sc->vz_out;
+        a.id_in = sc->id_in;
+        a.id_out = sc->id_out;
+        float *grids[3] = {sc->u, sc->v, sc->w};
+        float *olds[3] = {sc->u_old, sc->v_old, sc->w_old};
+        float *wts[3] = {sc->wu, sc->wv, sc->ww};
+        const int old_slots[3] = {S_OLD0, S_OLD1, S_OLD2};
+        for (int c = 0; c < 3; c++) {
+            a.grid[c] = grids[c];
+            a.grid_old[c] = olds[c] ? olds[c] : ctx->f32(old_slots[c], mac_count(c, ni, nj, nk));
+            a.wgt[c] = wts[c];
+            a.grav_on[c] = std::fabs((double)pr->body_force[c]) > 0.0;
+            a.grav_delta[c] = (float)((double)pr->body_force[c] * pr->dt);
+        }
+        a.h_half = (float)(0.5 * pr->dt);
+        a.h_full = (float)pr->dt;
+        a.h_sixth = (float)(pr->dt / 6.0f);
+        a.r_pic = (float)pr->pic_ratio;
+        a.r_flip = (float)(1 - pr->pic_ratio);
+        a.stencil = pr->stencil;
+        a.keep_order = pr->order == GFSX_ORDER_KEEP;
+        gfsx::SortBuffers sb = ctx->sort_buffers(sc->n, (uint32_t)nbin);
+        gfsx::flip_step(a, sb, ctx->stream, ctx->timing ? ctx->next_events() : nullptr);
+    });
+}
+
+void gfsx_timing_begin(gfsx_ctx *ctx, int *err) {
+    guarded(ctx, err, [&] {
+        ctx->timing = true;
+        ctx->events_used = 0;
+    });
+}
+
+int gfsx_timing_end(gfsx_ctx *ctx, double stage_ms[4], int *err) {
+    int steps = 0;
+    guarded(ctx, err, [&] {
+        require(stage_ms != nullptr, "gfsx_timing_end: null output");
+        GFSX_HIP(hipStreamSynchronize(ctx->stream));
+        for (int a = 0; a < 4; a++) stage_ms[a] = 0.0;
+        for (size_t e = 0; e < ctx->events_used; e++) {
+            const auto &ev = ctx->events[e];
+            for (int a = 0; a < 3; a++) {
+                float ms = 0.0f;
+                GFSX_HIP(hipEventElapsedTime(&ms, ev[a], ev[a + 1]));
+                stage_ms[a] += ms;
+            }
+            float tot = 0.0f;
+            GFSX_HIP(hipEventElapsedTime(&tot, ev[0], ev[3]));
+            stage_ms[3] += tot;
+        }
+        steps = (int)ctx->events_used;
+        ctx->events_used = 0;
+        ctx->timing = false;
+    });
+    return steps;
+}
+
+void gfsx_cell_sort_device(gfsx_ctx *ctx, const float *x, const float *y,
+                           const float *z, int64_t n, double dx, int ni,
+                           int nj, int nk, uint32_t *perm,
+                           uint32_t *cell_start, int *err) {
+    guarded(ctx, err, [&] {
+        check_grid(ni, nj, nk, dx);
+        check_particles(n, x);
+        require(n == 0 || (y && z && perm), "gfsx: null array");
+        require(cell_start != nullptr, "gfsx: null cell_start");
+        gfsx::BinGrid bg;
+        bg.b[0] = ni;
+        bg.b[1] = nj;
+        bg.b[2] = nk;
+        bg.pad[0] = bg.pad[1] = bg.pad[2] = 0;
+        bg.off[0] = bg.off[1] = bg.off[2] = 0.0f;
+        bg.inv_dx = 1.0 / dx;
+        bg.nbin = (uint32_t)((int64_t)ni * nj * nk);
+        gfsx::SortBuffers sb = ctx->sort_buffers(n, bg.nbin);
+        gfsx::cell_sort(nullptr, x, y, z, n, bg, sb, ctx->stream);
+        if (n > 0)
+            GFSX_HIP(hipMemcpyAsync(perm, sb.perm, sizeof(uint32_t) * n,
+                                    hipMemcpyDeviceToDevice, ctx->stream));
+        GFSX_HIP(hipMemcpyAsync(cell_start, sb.start, sizeof(uint32_t) * ((size_t)bg.nbin + 1),
+                                hipMemcpyDeviceToDevice, ctx->stream));
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,899 @@
+// gfsx_kernels.hip -- gfx950 kernels of the FLIP transfer + advection path.
+//
+//  * cell_sort   : stable counting sort of particle indices by cell (integer
+//                  work, bit-identical to a sequential stable counting sort).
+//  * p2g gather  : every node sums the kernel-weighted values of the
+//                  particles in its candidate cells, in a fixed order (no
+//                  float atomics), then normalises, snapshots and adds the
+//                  body force on faces bordering fluid.
+//  * g2p / RK    : gather-only trilinear or tricubic sampling of the MAC
+//                  grid, fused with the PIC/FLIP blend and the RK4 stages.
+//
+// Reference behaviour followed: ScalarField::addPointValue / applyWeightField
+// (scalarfield.cpp:90-106,167-201), MACVelocityField::_interpolate{,Linear}
+// U/V/W (macvelocityfield.cpp:350-575), ParticleAdvector::_RK1.._RK4
+// (particleadvector.cpp:1045-1078), FluidSimulation stages
+// (fluidsimulation.cpp:2010-2017,2526-2595,2766-2805,3104-3129,3181-3209).
+
+#include "gfsx_internal.h"
+
+#include <algorithm>
+
+namespace gfsx {
+
+// ===========================================================================
+// Utilities
+// ===========================================================================
+
+static inline unsigned grid_for(int64_t n, int64_t per_block) {
+    int64_t b = (n + per_block - 1) / per_block;
+    return (unsigned)std::max<int64_t>(b, 1);
+}
+
+__global__ void k_fill_u32(uint32_t *p, uint32_t v, int64_t n) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+
+__global__ void k_fill_f32(float *p, float v, int64_t n) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+
+void fill_u32(uint32_t *p, uint32_t value, int64_t count, hipStream_t s) {
+    if (count <= 0) return;
+    unsigned g = std::min<unsigned>(grid_for(count, kBlock), 8192u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(g), dim3(kBlock), 0, s, p, value, count);
+    GFSX_LAUNCH_CHECK();
+}
+
+void fill_f32(float *p, float value, int64_t count, hipStream_t s) {
+    if (count <= 0) return;
+    unsigned g = std::min<unsigned>(grid_for(count, kBlock), 8192u);
+    hipLaunchKernelGGL(k_fill_f32, dim3(g), dim3(kBlock), 0, s, p, value, count);
+    GFSX_LAUNCH_CHECK();
+}
+
+// ===========================================================================
+// Device-wide exclusive scan of uint32 (tiles of 4096, three phases)
+// ===========================================================================
+
+__device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t t = __shfl_up(v, d, 64);
+        if (lane >= d) v += t;
+    }
+    return v;
+}
+
+// Exclusive prefix of v over the 256-thread block; total receives the sum.
+__device__ __forceinline__ uint32_t block_exclusive_sum(uint32_t v,
+                                                        uint32_t *wsum,
+                                                        uint32_t &total) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    uint32_t inc = wave_inclusive_sum(v);
+    if (lane == 63) wsum[wid] = inc;
+    __syncthreads();
+    uint32_t woff = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; w++) {
+        uint32_t s = wsum[w];
+        woff += (w < wid) ? s : 0u;
+        tot += s;
+    }
+    __syncthreads();
+    total = tot;
+    return woff + inc - v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_tile_reduce(const uint32_t *in,
+                                                       int64_t m,
+                                                       uint32_t *sums) {
+    __shared__ uint32_t wsum[kBlock / 64];
+    int64_t base = (int64_t)blockIdx.x * kScanTile;
+    uint32_t s = 0;
+#pragma unroll
+    for (int it = 0; it < kScanItems; it++) {
+        int64_t idx = base + it * kBlock + threadIdx.x;
+        if (idx < m) s += in[idx];
+    }
+    uint32_t total;
+    block_exclusive_sum(s, wsum, total);
+    if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+__device__ __forceinline__ int padded(int e) { return e + (e >> 5); }
+
+// out[i] = sum(in[0..i)), out[m] = sum(in[0..m)).  tile_off (nullable) holds
+// the exclusive prefix of the tile sums.
+__global__ __launch_bounds__(kBlock) void k_tile_scan(const uint32_t *in,
+                                                     int64_t m,
+                                                     const uint32_t *tile_off,
+                                                     uint32_t *out) {
+    __shared__ uint32_t tile[kScanTile + kScanTile / 32];
+    __shared__ uint32_t wsum[kBlock / 64];
+    int64_t base = (int64_t)blockIdx.x * kScanTile;
+#pragma unroll
+    for (int it = 0; it < kScanItems; it++) {
+        int e = it * kBlock + threadIdx.x;
+        int64_t idx = base + e;
+        tile[padded(e)] = idx < m ? in[idx] : 0u;
+    }
+    __syncthreads();
+    uint32_t sum = 0;
+    const int first = threadIdx.x * kScanItems;
+#pragma unroll
+    for (int i = 0; i < kScanItems; i++) sum += tile[padded(first + i)];
+    uint32_t total;
+    uint32_t toff = tile_off ? tile_off[blockIdx.x] : 0u;
+    uint32_t run = block_exclusive_sum(sum, wsum, total) + toff;
+#pragma unroll
+    for (int i = 0; i < kScanItems; i++) {
+        uint32_t v = tile[padded(first + i)];
+        tile[padded(first + i)] = run;
+        run += v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < kScanItems; it++) {
+        int e = it * kBlock + threadIdx.x;
+        int64_t idx = base + e;
+        if (idx < m) out[idx] = tile[padded(e)];
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[m] = toff + total;
+}
+
+int64_t scan_scratch_words(int64_t m) {
+    int64_t nb = (m + kScanTile - 1) / kScanTile;
+    if (nb <= 1) return 0;
+    return 2 * nb + 1 + scan_scratch_words(nb);
+}
+
+void exclusive_scan(const uint32_t *in, uint32_t *out, int64_t m,
+                    uint32_t *scratch, hipStream_t s) {
+    if (m <= 0) {
+        fill_u32(out, 0u, 1, s);
+        return;
+    }
+    int64_t nb = (m + kScanTile - 1) / kScanTile;
+    if (nb == 1) {
+        hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(kBlock), 0, s, in, m,
+                           (const uint32_t *)nullptr, out);
+        GFSX_LAUNCH_CHECK();
+        return;
+    }
+    uint32_t *sums = scratch;
+    uint32_t *sums_scan = scratch + nb;
+    hipLaunchKernelGGL(k_tile_reduce, dim3((unsigned)nb), dim3(kBlock), 0, s,
+                       in, m, sums);
+    GFSX_LAUNCH_CHECK();
+    exclusive_scan(sums, sums_scan, nb, scratch + 2 * nb + 1, s);
+    hipLaunchKernelGGL(k_tile_scan, dim3((unsigned)nb), dim3(kBlock), 0, s,
+                       in, m, (const uint32_t *)sums_scan, out);
+    GFSX_LAUNCH_CHECK();
+}
+
+// ===========================================================================
+// Stable counting sort by bin
+// ===========================================================================
+
+int64_t sort_blocks(int64_t n) { return (n + kBlock - 1) / kBlock; }
+
+// Bin every particle.  In-grid particles take a slot in their bin with an
+// integer atomic (the slot order is fixed up by k_bucket_sort); overflow
+// particles get their rank among the block's overflow particles from a
+// wave ballot, so that bin is filled in index order directly.
+template <bool AOS>
+__global__ __launch_bounds__(kBlock) void k_bin(const float *aos,
+                                               const float *x, const float *y,
+                                               const float *z, int64_t n,
+                                               BinGrid g, uint32_t *count,
+                                               uint32_t *key, uint32_t *slot,
+                                               uint32_t *block_out) {
+    __shared__ uint32_t wcnt[kBlock / 64];
+    int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    bool valid = i < n;
+    uint32_t b = g.nbin;
+    if (valid) {
+        float px, py, pz;
+        if (AOS) {
+            px = aos[3 * i];
+            py = aos[3 * i + 1];
+            pz = aos[3 * i + 2];
+        } else {
+            px = x[i];
+            py = y[i];
+            pz = z[i];
+        }
+        b = bin_of(px, py, pz, g);
+    }
+    bool over = valid && b == g.nbin;
+    uint32_t sl = 0;
+    if (valid && !over) sl = atomicAdd(&count[b], 1u);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    uint64_t mask = __ballot(over);
+    uint32_t lrank = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+    if (lane == 0) wcnt[wid] = (uint32_t)__popcll(mask);
+    __syncthreads();
+    uint32_t woff = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; w++) {
+        woff += (w < wid) ? wcnt[w] : 0u;
+        tot += wcnt[w];
+    }
+    if (over) sl = woff + lrank;
+    if (valid) {
+        key[i] = b;
+        slot[i] = sl;
+    }
+    if (threadIdx.x == 0) block_out[blockIdx.x] = tot;
+}
+
+__global__ void k_copy_one(uint32_t *dst, const uint32_t *src) { *dst = *src; }
+
+__global__ __launch_bounds__(kBlock) void k_scatter(
+    const uint32_t *key, const uint32_t *slot, int64_t n,
+    const uint32_t *start, const uint32_t *block_out_scan, uint32_t nbin,
+    uint32_t *perm) {
+    int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    if (i >= n) return;
+    uint32_t b = key[i], s = slot[i];
+    uint32_t dst = (b == nbin) ? start[nbin] + block_out_scan[blockIdx.x] + s
+                               : start[b] + s;
+    perm[dst] = (uint32_t)i;
+}
+
+// Put each in-grid bin's indices into ascending (= input) order.  Bins hold
+// a handful of particles, so a per-thread insertion sort is enough; a bin
+// with more than 128 entries uses a Shell sort (gaps 1, 4, 13, 40, ...).
+__global__ __launch_bounds__(kBlock) void k_bucket_sort(const uint32_t *start,
+                                                       uint32_t nbin,
+                                                       uint32_t *perm) {
+    uint32_t b = blockIdx.x * kBlock + threadIdx.x;
+    if (b >= nbin) return;
+    uint32_t s = start[b], e = start[b + 1];
+    uint32_t len = e - s;
+    if (len <= 1) return;
+    uint32_t *a = perm + s;
+    uint32_t gap = 1;
+    if (len > 128)
+        while (gap < len / 3) gap = 3 * gap + 1;
+    for (; gap >= 1; gap /= 3) {
+        for (uint32_t q = gap; q < len; q++) {
+            uint32_t v = a[q];
+            uint32_t r = q;
+            while (r >= gap && a[r - gap] > v) {
+                a[r] = a[r - gap];
+                r -= gap;
+            }
+            a[r] = v;
+        }
+        if (gap == 1) break;
+    }
+}
+
+void cell_sort(const float *aos, const float *x, const float *y,
+               const float *z, int64_t n, const BinGrid &g,
+               const SortBuffers &sb, hipStream_t s) {
+    fill_u32(sb.count, 0u, (int64_t)g.nbin + 1, s);
+    if (n <= 0) {
+        fill_u32(sb.start, 0u, (int64_t)g.nbin + 2, s);
+        return;
+    }
+    unsigned nb = (unsigned)sort_blocks(n);
+    if (aos)
+        hipLaunchKernelGGL(k_bin<true>, dim3(nb), dim3(kBlock), 0, s, aos, x,
+                           y, z, n, g, sb.count, sb.key, sb.slot, sb.block_out);
+    else
+        hipLaunchKernelGGL(k_bin<false>, dim3(nb), dim3(kBlock), 0, s, aos, x,
+                           y, z, n, g, sb.count, sb.key, sb.slot, sb.block_out);
+    GFSX_LAUNCH_CHECK();
+    exclusive_scan(sb.block_out, sb.block_out_scan, nb, sb.scan_scratch, s);
+    hipLaunchKernelGGL(k_copy_one, dim3(1), dim3(1), 0, s, sb.count + g.nbin,
+                       (const uint32_t *)(sb.block_out_scan + nb));
+    GFSX_LAUNCH_CHECK();
+    exclusive_scan(sb.count, sb.start, (int64_t)g.nbin + 1, sb.scan_scratch, s);
+    hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(kBlock), 0, s,
+                       (const uint32_t *)sb.key, (const uint32_t *)sb.slot, n,
+                       (const uint32_t *)sb.start,
+                       (const uint32_t *)sb.block_out_scan, g.nbin, sb.perm);
+    GFSX_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_bucket_sort, dim3(grid_for(g.nbin, kBlock)),
+                       dim3(kBlock), 0, s, (const uint32_t *)sb.start, g.nbin,
+                       sb.perm);
+    GFSX_LAUNCH_CHECK();
+}
+
+// ===========================================================================
+// P2G gather
+// ===========================================================================
+
+// Sum K and K*val over the candidate bins of node (i,j,k).  Bins along x in
+// one (y,z) row are contiguous in perm, so each row is one index range; the
+// order of the sum is fixed: rows in (k,j) order, then perm order.
+template <bool HAS_VAL, class Load>
+__device__ __forceinline__ void gather_node(int i, int j, int k,
+                                            const NodeGrid &ng,
+                                            const BinGrid &bg, double dx,
+                                            float inv_r2,
+                                            const uint32_t *__restrict__ start,
+                                            const uint32_t *__restrict__ perm,
+                                            Load load, float &S, float &W) {
+    // node position in the shifted frame, rounded as
+    // Grid3d::GridIndexToPosition does (grid3d.h:79-81)
+    const float gx = (float)((float)i * dx);
+    const float gy = (float)((float)j * dx);
+    const float gz = (float)((float)k * dx);
+    const int ilo = max(i + ng.dlo[0], 0), ihi = min(i + ng.dhi[0], bg.b[0] - 1);
+    const int jlo = max(j + ng.dlo[1], 0), jhi = min(j + ng.dhi[1], bg.b[1] - 1);
+    const int klo = max(k + ng.dlo[2], 0), khi = min(k + ng.dhi[2], bg.b[2] - 1);
+    if (ilo > ihi) return;
+    for (int bk = klo; bk <= khi; bk++) {
+        for (int bj = jlo; bj <= jhi; bj++) {
+            const uint32_t row = (uint32_t)bg.b[0] * ((uint32_t)bj + (uint32_t)bg.b[1] * (uint32_t)bk);
+            const uint32_t q0 = start[row + ilo], q1 = start[row + ihi + 1];
+            for (uint32_t q = q0; q < q1; q++) {
+                const uint32_t p = perm[q];
+                float px, py, pz, val;
+                load(p, px, py, pz, val);
+                // p -= offset, then v = gpos - p (scalarfield.cpp:168,186-188)
+                px -= ng.off[0];
+                py -= ng.off[1];
+                pz -= ng.off[2];
+                const float vx = gx - px, vy = gy - py, vz = gz - pz;
+                const float d2 = vx * vx + vy * vy + vz * vz;
+                const float K = wyvill(d2 * inv_r2);
+                if (HAS_VAL) S = fmaf(K, val, S);
+                else S += K;
+                W += K;
+            }
+        }
+    }
+}
+
+struct AosLoad {
+    const float *__restrict__ pos;
+    const float *__restrict__ val;
+    __device__ __forceinline__ void operator()(uint32_t p, float &x, float &y,
+                                               float &z, float &v) const {
+        x = pos[3 * (size_t)p];
+        y = pos[3 * (size_t)p + 1];
+        z = pos[3 * (size_t)p + 2];
+        v = val ? val[p] : 1.0f;
+    }
+};
+
+template <bool HAS_VAL>
+__global__ __launch_bounds__(kBlock) void k_p2g_gather(
+    AosLoad ld, BinGrid bg, NodeGrid ng, double dx, float inv_r2,
+    const uint32_t *__restrict__ start, const uint32_t *__restrict__ perm,
+    float *__restrict__ S, float *__restrict__ W) {
+    const int64_t f = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    const int64_t nnode = (int64_t)ng.n[0] * ng.n[1] * ng.n[2];
+    if (f >= nnode) return;
+    const int i = (int)(f % ng.n[0]);
+    const int j = (int)((f / ng.n[0]) % ng.n[1]);
+    const int k = (int)(f / ((int64_t)ng.n[0] * ng.n[1]));
+    float s = 0.0f, w = 0.0f;
+    gather_node<HAS_VAL>(i, j, k, ng, bg, dx, inv_r2, start, perm, ld, s, w);
+    // accumulate into the caller's arrays (clscalarfield.cpp:1448-1449)
+    S[f] += s;
+    if (W) W[f] += w;
+}
+
+void p2g_gather(const float *aos, const float *val, int64_t n,
+                const BinGrid &bins, const NodeGrid &nodes, double radius,
+                double dx, const SortBuffers &sb, float *S, float *W,
+                hipStream_t s) {
+    (void)n;
+    int64_t nnode = (int64_t)nodes.n[0] * nodes.n[1] * nodes.n[2];
+    if (nnode <= 0) return;
+    float inv_r2 = (float)(1.0 / (radius * radius));
+    AosLoad ld{aos, val};
+    if (val)
+        hipLaunchKernelGGL(k_p2g_gather<true>, dim3(grid_for(nnode, kBlock)),
+                           dim3(kBlock), 0, s, ld, bins, nodes, dx, inv_r2,
+                           (const uint32_t *)sb.start, (const uint32_t *)sb.perm,
+                           S, W);
+    else
+        hipLaunchKernelGGL(k_p2g_gather<false>, dim3(grid_for(nnode, kBlock)),
+                           dim3(kBlock), 0, s, ld, bins, nodes, dx, inv_r2,
+                           (const uint32_t *)sb.start, (const uint32_t *)sb.perm,
+                           S, W);
+    GFSX_LAUNCH_CHECK();
+}
+
+__global__ void k_apply_weight(float *S, const float *W, int64_t count) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    float w = W[i];
+    if (w > 0.0f) S[i] = S[i] / w;
+}
+
+void apply_weight(float *S, const float *W, int64_t count, hipStream_t s) {
+    if (count <= 0) return;
+    hipLaunchKernelGGL(k_apply_weight, dim3(grid_for(count, kBlock)),
+                       dim3(kBlock), 0, s, S, W, count);
+    GFSX_LAUNCH_CHECK();
+}
+
+// ===========================================================================
+// G2P sampling
+// ===========================================================================
+
+// Per-axis cell coordinates for the unshifted (a) and half-shifted (b)
+// variants: u samples (a,b,b), v (b,a,b), w (b,b,a)
+// (macvelocityfield.cpp:355-366,389-400,423-434).
+struct AxisCoords {
+    int ia, ib;
+    float ta, tb;
+};
+
+__device__ __forceinline__ AxisCoords axis_coords(double x, double inv_dx) {
+    AxisCoords c;
+    const double ga = x * inv_dx;
+    const double gb = ga - 0.5;
+    const double fa = floor(ga), fb = floor(gb);
+    c.ia = (int)fa;
+    c.ib = (int)fb;
+    c.ta = (float)(ga - fa);
+    c.tb = (float)(gb - fb);
+    return c;
+}
+
+__device__ __forceinline__ float ld_or_zero(const float *__restrict__ f,
+                                            bool ok, uint32_t idx) {
+    float v = f[idx];
+    return ok ? v : 0.0f;
+}
+
+// Trilinear sample of one component array with dims (w,h,d); out-of-range
+// corners read 0 (macvelocityfield.cpp:470-478).
+__device__ __forceinline__ float trilinear(const float *__restrict__ f, int w,
+                                           int h, int d, int i, int j, int k,
+                                           float tx, float ty, float tz) {
+    const bool vi0 = (unsigned)i < (unsigned)w, vi1 = (unsigned)(i + 1) < (unsigned)w;
+    const bool vj0 = (unsigned)j < (unsigned)h, vj1 = (unsigned)(j + 1) < (unsigned)h;
+    const bool vk0 = (unsigned)k < (unsigned)d, vk1 = (unsigned)(k + 1) < (unsigned)d;
+    const uint32_t i0 = (uint32_t)min(max(i, 0), w - 1), i1 = (uint32_t)min(max(i + 1, 0), w - 1);
+    const uint32_t j0 = (uint32_t)min(max(j, 0), h - 1), j1 = (uint32_t)min(max(j + 1, 0), h - 1);
+    const uint32_t k0 = (uint32_t)min(max(k, 0), d - 1), k1 = (uint32_t)min(max(k + 1, 0), d - 1);
+    const uint32_t r00 = (uint32_t)w * (j0 + (uint32_t)h * k0);
+    const uint32_t r10 = (uint32_t)w * (j1 + (uint32_t)h * k0);
+    const uint32_t r01 = (uint32_t)w * (j0 + (uint32_t)h * k1);
+    const uint32_t r11 = (uint32_t)w * (j1 + (uint32_t)h * k1);
+    const float v000 = ld_or_zero(f, vi0 && vj0 && vk0, r00 + i0);
+    const float v100 = ld_or_zero(f, vi1 && vj0 && vk0, r00 + i1);
+    const float v010 = ld_or_zero(f, vi0 && vj1 && vk0, r10 + i0);
+    const float v110 = ld_or_zero(f, vi1 && vj1 && vk0, r10 + i1);
+    const float v001 = ld_or_zero(f, vi0 && vj0 && vk1, r01 + i0);
+    const float v101 = ld_or_zero(f, vi1 && vj0 && vk1, r01 + i1);
+    const float v011 = ld_or_zero(f, vi0 && vj1 && vk1, r11 + i0);
+    const float v111 = ld_or_zero(f, vi1 && vj1 && vk1, r11 + i1);
+    const float c00 = fmaf(tx, v100 - v000, v000);
+    const float c10 = fmaf(tx, v110 - v010, v010);
+    const float c01 = fmaf(tx, v101 - v001, v001);
+    const float c11 = fmaf(tx, v111 - v011, v011);
+    const float c0 = fmaf(ty, c10 - c00, c00);
+    const float c1 = fmaf(ty, c11 - c01, c01);
+    return fmaf(tz, c1 - c0, c0);
+}
+
+// Catmull-Rom cubic (Keys a = -1/2) between p1 and p2, the formula of
+// Interpolation::cubicInterpolate (interpolation.cpp:44-46).
+__device__ __forceinline__ float cubic(float p0, float p1, float p2, float p3,
+                                       float t) {
+    return p1 + 0.5f * t *
+                    (p2 - p0 +
+                     t * (2.0f * p0 - 5.0f * p1 + 4.0f * p2 - p3 +
+                          t * (3.0f * (p1 - p2) + p3 - p0)));
+}
+
+// Tricubic sample over the 4x4x4 stencil at (i-1..i+2, j-1..j+2,
+// k-1..k+2), x first, then y, then z (interpolation.cpp:26-42); samples
+// outside the array read 0 (macvelocityfield.cpp:372-378).
+__device__ __forceinline__ float tricubic(const float *__restrict__ f, int w,
+                                          int h, int d, int i, int j, int k,
+                                          float tx, float ty, float tz) {
+    uint32_t ic[4], jr[4], kr[4];
+    bool vi[4], vj[4], vk[4];
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+        const int a = i - 1 + m, b = j - 1 + m, c = k - 1 + m;
+        vi[m] = (unsigned)a < (unsigned)w;
+        vj[m] = (unsigned)b < (unsigned)h;
+        vk[m] = (unsigned)c < (unsigned)d;
+        ic[m] = (uint32_t)min(max(a, 0), w - 1);
+        jr[m] = (uint32_t)min(max(b, 0), h - 1);
+        kr[m] = (uint32_t)min(max(c, 0), d - 1);
+    }
+    float zs[4];
+#pragma unroll
+    for (int mk = 0; mk < 4; mk++) {
+        float ys[4];
+#pragma unroll
+        for (int mj = 0; mj < 4; mj++) {
+            const uint32_t row = (uint32_t)w * (jr[mj] + (uint32_t)h * kr[mk]);
+            const bool okr = vj[mj] && vk[mk];
+            float p[4];
+#pragma unroll
+            for (int mi = 0; mi < 4; mi++) p[mi] = ld_or_zero(f, okr && vi[mi], row + ic[mi]);
+            ys[mj] = cubic(p[0], p[1], p[2], p[3], tx);
+        }
+        zs[mk] = cubic(ys[0], ys[1], ys[2], ys[3], ty);
+    }
+    return cubic(zs[0], zs[1], zs[2], zs[3], tz);
+}
+
+template <int ST>
+__device__ __forceinline__ float sample_comp(const float *__restrict__ f,
+                                             int w, int h, int d, int i, int j,
+                                             int k, float tx, float ty,
+                                             float tz) {
+    if (ST == 0) return trilinear(f, w, h, d, i, j, k, tx, ty, tz);
+    return tricubic(f, w, h, d, i, j, k, tx, ty, tz);
+}
+
+// Velocity of grid g (and, when OLD, of the same-shaped grid o0,o1,o2) at
+// (x,y,z): MACVelocityField::evaluateVelocityAtPosition
+// {,Linear} (macvelocityfield.cpp:549-575), (0,0,0) outside the grid.
+template <int ST, bool OLD>
+__device__ __forceinline__ void sample_mac(const MacGrid &g, const float *o0,
+                                           const float *o1, const float *o2,
+                                           float x, float y, float z,
+                                           float out[3], float out_old[3]) {
+    const double dxp = x, dyp = y, dzp = z;
+    if (!(dxp >= 0.0 && dyp >= 0.0 && dzp >= 0.0 && dxp < g.lim[0] &&
+          dyp < g.lim[1] && dzp < g.lim[2])) {
+        out[0] = out[1] = out[2] = 0.0f;
+        if (OLD) out_old[0] = out_old[1] = out_old[2] = 0.0f;
+        return;
+    }
+    const AxisCoords cx = axis_coords(dxp, g.inv_dx);
+    const AxisCoords cy = axis_coords(dyp, g.inv_dx);
+    const AxisCoords cz = axis_coords(dzp, g.inv_dx);
+    const int ni = g.ni, nj = g.nj, nk = g.nk;
+    out[0] = sample_comp<ST>(g.f[0], ni + 1, nj, nk, cx.ia, cy.ib, cz.ib, cx.ta, cy.tb, cz.tb);
+    out[1] = sample_comp<ST>(g.f[1], ni, nj + 1, nk, cx.ib, cy.ia, cz.ib, cx.tb, cy.ta, cz.tb);
+    out[2] = sample_comp<ST>(g.f[2], ni, nj, nk + 1, cx.ib, cy.ib, cz.ia, cx.tb, cy.tb, cz.ta);
+    if (OLD) {
+        out_old[0] = sample_comp<ST>(o0, ni + 1, nj, nk, cx.ia, cy.ib, cz.ib, cx.ta, cy.tb, cz.tb);
+        out_old[1] = sample_comp<ST>(o1, ni, nj + 1, nk, cx.ib, cy.ia, cz.ib, cx.tb, cy.ta, cz.tb);
+        out_old[2] = sample_comp<ST>(o2, ni, nj, nk + 1, cx.ib, cy.ib, cz.ia, cx.tb, cy.tb, cz.ta);
+    }
+}
+
+// ParticleAdvector::_validateOutput (particleadvector.cpp:1139-1149)
+__device__ __forceinline__ void validate3(float v[3]) {
+    if (!(isfinite(v[0]) && isfinite(v[1]) && isfinite(v[2])))
+        v[0] = v[1] = v[2] = 0.0f;
+}
+
+template <int ST>
+__global__ __launch_bounds__(kBlock) void k_g2p(const float *pos, int64_t n,
+                                               MacGrid g, float *out) {
+    const int64_t p = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    if (p >= n) return;
+    const float x = pos[3 * p], y = pos[3 * p + 1], z = pos[3 * p + 2];
+    float v[3];
+    sample_mac<ST, false>(g, nullptr, nullptr, nullptr, x, y, z, v, nullptr);
+    validate3(v);
+    out[3 * p] = v[0];
+    out[3 * p + 1] = v[1];
+    out[3 * p + 2] = v[2];
+}
+
+void g2p(const float *pos, int64_t n, const MacGrid &g, int stencil,
+         float *out, hipStream_t s) {
+    if (n <= 0) return;
+    if (stencil == 0)
+        hipLaunchKernelGGL(k_g2p<0>, dim3(grid_for(n, kBlock)), dim3(kBlock),
+                           0, s, pos, n, g, out);
+    else
+        hipLaunchKernelGGL(k_g2p<1>, dim3(grid_for(n, kBlock)), dim3(kBlock),
+                           0, s, pos, n, g, out);
+    GFSX_LAUNCH_CHECK();
+}
+
+struct RkConsts {
+    float half, three_q, full, sixth, ninth;
+};
+
+// ParticleAdvector::_RK1.._RK4 (particleadvector.cpp:1045-1078) in FP32
+// vec3 arithmetic; the step sizes are the reference's FP32 casts.
+template <int ST>
+__device__ __forceinline__ void rk_step(const MacGrid &g, int order,
+                                        const RkConsts &c, const float x0[3],
+                                        const float k1[3], float x1[3]) {
+    float q[3], k2[3], k3[3], k4[3], unused[3];
+    if (order == 1) {
+#pragma unroll
+        for (int a = 0; a < 3; a++) x1[a] = x0[a] + c.full * k1[a];
+        return;
+    }
+#pragma unroll
+    for (int a = 0; a < 3; a++) q[a] = x0[a] + c.half * k1[a];
+    sample_mac<ST, false>(g, nullptr, nullptr, nullptr, q[0], q[1], q[2], k2, unused);
+    if (order == 2) {
+#pragma unroll
+        for (int a = 0; a < 3; a++) x1[a] = x0[a] + c.full * k2[a];
+        return;
+    }
+    if (order == 3) {
+#pragma unroll
+        for (int a = 0; a < 3; a++) q[a] = x0[a] + c.three_q * k2[a];
+        sample_mac<ST, false>(g, nullptr, nullptr, nullptr, q[0], q[1], q[2], k3, unused);
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+            x1[a] = x0[a] + c.ninth * ((2.0f * k1[a] + 3.0f * k2[a]) + 4.0f * k3[a]);
+        return;
+    }
+#pragma unroll
+    for (int a = 0; a < 3; a++) q[a] = x0[a] + c.half * k2[a];
+    sample_mac<ST, false>(g, nullptr, nullptr, nullptr, q[0], q[1], q[2], k3, unused);
+#pragma unroll
+    for (int a = 0; a < 3; a++) q[a] = x0[a] + c.full * k3[a];
+    sample_mac<ST, false>(g, nullptr, nullptr, nullptr, q[0], q[1], q[2], k4, unused);
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+        x1[a] = x0[a] + c.sixth * (((k1[a] + 2.0f * k2[a]) + 2.0f * k3[a]) + k4[a]);
+}
+
+template <int ST>
+__global__ __launch_bounds__(kBlock) void k_advect(const float *pos, int64_t n,
+                                                  MacGrid g, int order,
+                                                  RkConsts c, float *out) {
+    const int64_t p = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    if (p >= n) return;
+    const float x0[3] = {pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]};
+    float k1[3], unused[3], x1[3];
+    sample_mac<ST, false>(g, nullptr, nullptr, nullptr, x0[0], x0[1], x0[2], k1, unused);
+    rk_step<ST>(g, order, c, x0, k1, x1);
+    out[3 * p] = x1[0];
+    out[3 * p + 1] = x1[1];
+    out[3 * p + 2] = x1[2];
+}
+
+static RkConsts rk_consts(double dt) {
+    RkConsts c;
+    c.half = (float)(0.5 * dt);
+    c.three_q = (float)(0.75 * dt);
+    c.full = (float)dt;
+    c.sixth = (float)(dt / 6.0f);
+    c.ninth = (float)(dt / 9.0f);
+    return c;
+}
+
+void advect(const float *pos, int64_t n, const MacGrid &g, int order,
+            double dt, int stencil, float *out, hipStream_t s) {
+    if (n <= 0) return;
+    RkConsts c = rk_consts(dt);
+    if (stencil == 0)
+        hipLaunchKernelGGL(k_advect<0>, dim3(grid_for(n, kBlock)), dim3(kBlock),
+                           0, s, pos, n, g, order, c, out);
+    else
+        hipLaunchKernelGGL(k_advect<1>, dim3(grid_for(n, kBlock)), dim3(kBlock),
+                           0, s, pos, n, g, order, c, out);
+    GFSX_LAUNCH_CHECK();
+}
+
+// ===========================================================================
+// The fused plain step
+// ===========================================================================
+
+struct SoaLoad {
+    const float *__restrict__ x;
+    const float *__restrict__ y;
+    const float *__restrict__ z;
+    const float *__restrict__ v;
+    __device__ __forceinline__ void operator()(uint32_t p, float &px, float &py,
+                                               float &pz, float &pv) const {
+        px = x[p];
+        py = y[p];
+        pz = z[p];
+        pv = v[p];
+    }
+};
+
+struct StepP2G {
+    const float *x, *y, *z;
+    const float *vel[3];
+    float *grid[3], *grid_old[3], *wgt[3];
+    NodeGrid ng[3];
+    int64_t nface[3];
+    float grav_delta[3];
+    int grav_on[3];
+    int ni, nj, nk;
+};
+
+// One thread per face of u, v or w: gather, normalise
+// (scalarfield.cpp:90-106), snapshot (fluidsimulation.cpp:3306), then add
+// the body force where the face borders a cell that holds a particle
+// (fluidsimulation.cpp:2766-2805, fluidmaterialgrid.cpp:119-143).
+__global__ __launch_bounds__(kBlock) void k_flip_p2g(
+    StepP2G a, BinGrid bg, double dx, float inv_r2,
+    const uint32_t *__restrict__ count, const uint32_t *__restrict__ start,
+    const uint32_t *__restrict__ perm) {
+    int64_t f = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    int c = 0;
+    if (f >= a.nface[0]) {
+        f -= a.nface[0];
+        c = 1;
+        if (f >= a.nface[1]) {
+            f -= a.nface[1];
+            c = 2;
+            if (f >= a.nface[2]) return;
+        }
+    }
+    const NodeGrid &ng = a.ng[c];
+    const int i = (int)(f % ng.n[0]);
+    const int j = (int)((f / ng.n[0]) % ng.n[1]);
+    const int k = (int)(f / ((int64_t)ng.n[0] * ng.n[1]));
+    float S = 0.0f, W = 0.0f;
+    SoaLoad ld{a.x, a.y, a.z, a.vel[c]};
+    gather_node<true>(i, j, k, ng, bg, dx, inv_r2, start, perm, ld, S, W);
+    float val = W > 0.0f ? S / W : S;
+    if (a.grid_old[c]) a.grid_old[c][f] = val;
+    if (a.wgt[c]) a.wgt[c][f] = W;
+    if (a.grav_on[c]) {
+        // the two cells on either side of the face along axis c
+        const int ax = c == 0 ? i : (c == 1 ? j : k);
+        const int nax = c == 0 ? a.ni : (c == 1 ? a.nj : a.nk);
+        bool fluid = false;
+        // bin of in-domain cell (ci,cj,ck) is (ci+1, cj+1, ck+1)
+        const uint32_t bhi = (uint32_t)(i + 1) + (uint32_t)bg.b[0] *
+                             ((uint32_t)(j + 1) + (uint32_t)bg.b[1] * (uint32_t)(k + 1));
+        const uint32_t step = c == 0 ? 1u : (c == 1 ? (uint32_t)bg.b[0]
+                                                    : (uint32_t)bg.b[0] * (uint32_t)bg.b[1]);
+        if (ax < nax) fluid = fluid || count[bhi] > 0u;
+        if (ax > 0) fluid = fluid || count[bhi - step] > 0u;
+        if (fluid) val += a.grav_delta[c];
+    }
+    a.grid[c][f] = val;
+}
+
+struct StepG2P {
+    const float *xin[3], *vin[3];
+    float *xout[3], *vout[3];
+    const uint32_t *id_in;
+    uint32_t *id_out;
+    const float *old[3];
+    RkConsts rk;
+    float r_pic, r_flip;
+    int keep_order;
+    int64_t n;
+};
+
+// One thread per particle, in cell order: PIC/FLIP blend
+// (fluidsimulation.cpp:3115-3127) and RK4 (particleadvector.cpp:1045-1054)
+// through the new field.
+template <int ST>
+__global__ __launch_bounds__(kBlock) void k_flip_g2p(
+    StepG2P a, MacGrid g, const uint32_t *__restrict__ perm) {
+    const int64_t jdx = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    if (jdx >= a.n) return;
+    const uint32_t p = perm[jdx];
+    const float x0[3] = {a.xin[0][p], a.xin[1][p], a.xin[2][p]};
+    const float v0[3] = {a.vin[0][p], a.vin[1][p], a.vin[2][p]};
+    float k1[3], vo[3];
+    sample_mac<ST, true>(g, a.old[0], a.old[1], a.old[2], x0[0], x0[1], x0[2], k1, vo);
+    float vn[3] = {k1[0], k1[1], k1[2]};
+    validate3(vn);
+    validate3(vo);
+    float vnew[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const float vflip = (v0[c] + vn[c]) - vo[c];
+        vnew[c] = a.r_pic * vn[c] + a.r_flip * vflip;
+    }
+    float x1[3];
+    rk_step<ST>(g, 4, a.rk, x0, k1, x1);
+    const int64_t o = a.keep_order ? (int64_t)p : jdx;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        a.xout[c][o] = x1[c];
+        a.vout[c][o] = vnew[c];
+    }
+    if (a.id_out) a.id_out[o] = a.id_in ? a.id_in[p] : p;
+}
+
+void flip_step(const StepArgs &s, const SortBuffers &sb, hipStream_t st,
+               hipEvent_t *ev) {
+    if (ev) GFSX_HIP(hipEventRecord(ev[0], st));
+    BinGrid bg;
+    bg.b[0] = s.ni + 2;
+    bg.b[1] = s.nj + 2;
+    bg.b[2] = s.nk + 2;
+    bg.pad[0] = bg.pad[1] = bg.pad[2] = 1;
+    bg.off[0] = bg.off[1] = bg.off[2] = 0.0f;
+    bg.inv_dx = 1.0 / s.dx;
+    bg.nbin = (uint32_t)bg.b[0] * (uint32_t)bg.b[1] * (uint32_t)bg.b[2];
+    cell_sort(nullptr, s.xin[0], s.xin[1], s.xin[2], s.n, bg, sb, st);
+    if (ev) GFSX_HIP(hipEventRecord(ev[1], st));
+
+    StepP2G a;
+    a.x = s.xin[0];
+    a.y = s.xin[1];
+    a.z = s.xin[2];
+    const float half = (float)(0.5 * s.dx);
+    for (int c = 0; c < 3; c++) {
+        a.vel[c] = s.vin[c];
+        a.grid[c] = s.grid[c];
+        a.grid_old[c] = s.grid_old[c];
+        a.wgt[c] = s.wgt[c];
+        a.grav_delta[c] = s.grav_delta[c];
+        a.grav_on[c] = s.grav_on[c];
+        NodeGrid &ng = a.ng[c];
+        ng.n[0] = s.ni + (c == 0);
+        ng.n[1] = s.nj + (c == 1);
+        ng.n[2] = s.nk + (c == 2);
+        ng.dx_f = (float)s.dx;
+        for (int ax = 0; ax < 3; ax++) {
+            ng.off[ax] = (ax == c) ? 0.0f : half;
+            // node n sits at bin coordinate n + off/dx + pad; its candidate
+            // bins cover (n + base - R/dx, n + base + R/dx) with R = dx
+            double base = (double)ng.off[ax] / s.dx + 1.0;
+            ng.dlo[ax] = (int)floor(base - 1.0);
+            ng.dhi[ax] = (int)ceil(base + 1.0) - 1;
+        }
+        a.nface[c] = (int64_t)ng.n[0] * ng.n[1] * ng.n[2];
+    }
+    a.ni = s.ni;
+    a.nj = s.nj;
+    a.nk = s.nk;
+    const int64_t F = a.nface[0] + a.nface[1] + a.nface[2];
+    const float inv_r2 = (float)(1.0 / (s.dx * s.dx));
+    hipLaunchKernelGGL(k_flip_p2g, dim3(grid_for(F, kBlock)), dim3(kBlock), 0,
+                       st, a, bg, s.dx, inv_r2, (const uint32_t *)sb.count,
+                       (const uint32_t *)sb.start, (const uint32_t *)sb.perm);
+    GFSX_LAUNCH_CHECK();
+    if (ev) GFSX_HIP(hipEventRecord(ev[2], st));
+
+    if (s.n <= 0) {
+        if (ev) GFSX_HIP(hipEventRecord(ev[3], st));
+        return;
+    }
+    MacGrid g;
+    for (int c = 0; c < 3; c++) g.f[c] = s.grid[c];
+    g.ni = s.ni;
+    g.nj = s.nj;
+    g.nk = s.nk;
+    g.dx = s.dx;
+    g.inv_dx = 1.0 / s.dx;
+    g.lim[0] = s.dx * s.ni;
+    g.lim[1] = s.dx * s.nj;
+    g.lim[2] = s.dx * s.nk;
+    StepG2P b;
+    for (int c = 0; c < 3; c++) {
+        b.xin[c] = s.xin[c];
+        b.vin[c] = s.vin[c];
+        b.xout[c] = s.xout[c];
+        b.vout[c] = s.vout[c];
+        b.old[c] = s.grid_old[c];
+    }
+    b.id_in = s.id_in;
+    b.id_out = s.id_out;
+    b.rk.half = s.h_half;
+    b.rk.three_q = 0.0f;
+    b.rk.full = s.h_full;
+    b.rk.sixth = s.h_sixth;
+    b.rk.ninth = 0.0f;
+    b.r_pic = s.r_pic;
+    b.r_flip = s.r_flip;
+    b.keep_order = s.keep_order;
+    b.n = s.n;
+    if (s.stencil == 0)
+        hipLaunchKernelGGL(k_flip_g2p<0>, dim3(grid_for(s.n, kBlock)),
+                           dim3(kBlock), 0, st, b, g, (const uint32_t *)sb.perm);
+    else
+        hipLaunchKernelGGL(k_flip_g2p<1>, dim3(grid_for(s.n, kBlock)),
+                           dim3(kBlock), 0, st, b, g, (const uint32_t *)sb.perm);
+    GFSX_LAUNCH_CHECK();
+    if (ev) GFSX_HIP(hipEventRecord(ev[3], st));
+}
+
+}  // namespace gfsx

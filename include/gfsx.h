@@ -1,0 +1,191 @@
+/*
+ * gfsx.h -- C ABI of the MI355X (gfx950) FLIP transfer + advection library.
+ *
+ * Conventions follow the reference's own C bindings
+ * (src/c_bindings/cbindings.cpp:9-19, cbindings.h:25-131): every entry point
+ * takes a trailing `int *err` that receives GFSX_SUCCESS (1) or GFSX_FAIL (0),
+ * and the text of the last failure on the calling thread is returned by
+ * gfsx_get_error_message().  No entry point aborts or throws.
+ *
+ * Layouts are the reference's: positions/velocities for the drop-in entry
+ * points are AoS float triples (std::vector<vmath::vec3>, vmath.h:33-57);
+ * grids are flat i-fastest float arrays (array3d.h:394-397); MAC grids have
+ * u:(ni+1)*nj*nk, v:ni*(nj+1)*nk, w:ni*nj*(nk+1) entries
+ * (macvelocityfield.cpp:37-45).
+ *
+ * Host-pointer entry points copy their inputs to the GPU, run, copy back and
+ * return when the result is in host memory (the reference's calls are
+ * synchronous too, clscalarfield.cpp:1388-1393).  *_device entry points take
+ * device pointers, enqueue on the context's stream and return without waiting.
+ */
+#ifndef GFSX_H
+#define GFSX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GFSX_SUCCESS 1
+#define GFSX_FAIL 0
+
+/* G2P stencils.  TRICUBIC is the reference's ParticleAdvector stencil
+ * (tricubicinterpolate.cl:46-67, macvelocityfield.cpp:350-450); TRILINEAR is
+ * MACVelocityField::evaluateVelocityAtPositionLinear (macvelocityfield.cpp:
+ * 452-575). */
+#define GFSX_TRILINEAR 0
+#define GFSX_TRICUBIC 1
+
+/* Output order of gfsx_flip_step. */
+#define GFSX_ORDER_SORTED 0 /* particles come out grouped by cell */
+#define GFSX_ORDER_KEEP 1   /* particle i stays at index i */
+
+typedef struct gfsx_ctx gfsx_ctx;
+
+/* Last error on this thread; replaces the reference's
+ * CBindings_get_error_message (cbindings.cpp:11-19). */
+const char *gfsx_get_error_message(void);
+
+/* Number of visible HIP devices (0 on a machine without a GPU: *err is
+ * GFSX_FAIL then, with the runtime's message). */
+int gfsx_get_device_count(int *err);
+
+/* Context: device, stream and device workspace.  Replaces
+ * CLScalarField::initialize / ParticleAdvector::initialize
+ * (clscalarfield.cpp:31-66, particleadvector.cpp:30-58): returns NULL and
+ * GFSX_FAIL instead of aborting when no device is usable. */
+gfsx_ctx *gfsx_create(int device, int *err);
+void gfsx_destroy(gfsx_ctx *ctx, int *err);
+
+/* Run on the caller's stream (a hipStream_t) from now on; NULL selects the
+ * default (null) stream.  A new context runs on a stream of its own. */
+void gfsx_set_stream(gfsx_ctx *ctx, void *stream, int *err);
+void gfsx_synchronize(gfsx_ctx *ctx, int *err);
+
+/* Device name / library build text, for CLScalarField::getDeviceInfo-style
+ * reporting (clscalarfield.cpp:580-637). */
+const char *gfsx_get_device_info(gfsx_ctx *ctx, int *err);
+
+/* ------------------------------------------------------------------------ */
+/* P2G: CLScalarField::addPointValues / addPoints.                          */
+/* ------------------------------------------------------------------------ */
+/* Splat n points (pos: n*3 floats) onto the node grid ni*nj*nk whose node
+ * (i,j,k) sits at offset + (i,j,k)*dx, with the Wyvill kernel of the given
+ * radius (scalarfield.cpp:40-46,167-201, scalarfield.cl:202-268).
+ * S[node] += sum_p val[p]*K and, when W is not NULL, W[node] += sum_p K.
+ * val == NULL splats the weight itself (CLScalarField::addPoints,
+ * clscalarfield.cpp:68-122, scalarfield.cl:79-138).  Accumulates into S/W
+ * like the reference (clscalarfield.cpp:1448-1449); the sum for each node is
+ * taken in a fixed order, so results are bit-identical run to run.
+ * Replaces CLScalarField::addPointValues (clscalarfield.cpp:143-301). */
+void gfsx_p2g(gfsx_ctx *ctx, const float *pos, const float *val, int64_t n,
+              double radius, const float offset[3], double dx, int ni, int nj,
+              int nk, float *S, float *W, int *err);
+void gfsx_p2g_device(gfsx_ctx *ctx, const float *pos, const float *val,
+                     int64_t n, double radius, const float offset[3],
+                     double dx, int ni, int nj, int nk, float *S, float *W,
+                     int *err);
+
+/* ScalarField::applyWeightField (scalarfield.cpp:90-106): S = S/W where W>0. */
+void gfsx_apply_weight_device(gfsx_ctx *ctx, float *S, const float *W,
+                              int64_t count, int *err);
+
+/* ------------------------------------------------------------------------ */
+/* G2P: ParticleAdvector::tricubicInterpolate.                              */
+/* ------------------------------------------------------------------------ */
+/* out[p] = velocity of the MAC field (u,v,w on an ni*nj*nk grid of spacing
+ * dx) at pos[p]; (0,0,0) outside [0,ni*dx)x[0,nj*dx)x[0,nk*dx); NaN/Inf
+ * results are zeroed (particleadvector.cpp:1139-1149).  out may alias pos
+ * (the reference's in-place overload, particleadvector.cpp:457-465).
+ * Replaces ParticleAdvector::tricubicInterpolate (particleadvector.cpp:
+ * 401-455). */
+void gfsx_g2p(gfsx_ctx *ctx, const float *pos, int64_t n, const float *u,
+              const float *v, const float *w, int ni, int nj, int nk,
+              double dx, int stencil, float *out, int *err);
+void gfsx_g2p_device(gfsx_ctx *ctx, const float *pos, int64_t n,
+                     const float *u, const float *v, const float *w, int ni,
+                     int nj, int nk, double dx, int stencil, float *out,
+                     int *err);
+
+/* ------------------------------------------------------------------------ */
+/* Advection: ParticleAdvector::advectParticlesRK1..RK4.                    */
+/* ------------------------------------------------------------------------ */
+/* out[p] = one Runge-Kutta step of the given order (1..4) from pos[p]
+ * through the MAC field (particleadvector.cpp:209-399, 1045-1078).
+ * Replaces ParticleAdvector::advectParticlesRK{1,2,3,4}. */
+void gfsx_advect(gfsx_ctx *ctx, int order, const float *pos, int64_t n,
+                 const float *u, const float *v, const float *w, int ni,
+                 int nj, int nk, double dx, double dt, int stencil, float *out,
+                 int *err);
+void gfsx_advect_device(gfsx_ctx *ctx, int order, const float *pos, int64_t n,
+                        const float *u, const float *v, const float *w, int ni,
+                        int nj, int nk, double dx, double dt, int stencil,
+                        float *out, int *err);
+
+/* ------------------------------------------------------------------------ */
+/* The fused plain FLIP step on device memory.                              */
+/* ------------------------------------------------------------------------ */
+/* One step of: fluid cells from particle positions -> P2G of u,v,w with the
+ * weight field, R = dx, face offsets -> normalisation -> snapshot -> constant
+ * body force on faces bordering fluid -> PIC/FLIP velocity update -> RK4
+ * (the FluidSimulation stages at fluidsimulation.cpp:2010-2017, 2526-2595,
+ * 3306, 2766-2805, 3104-3129, 3181-3209).  Particles are structure-of-arrays
+ * device buffers; the *_in arrays are read, the *_out arrays written (they
+ * must not alias the inputs).  With GFSX_ORDER_SORTED the outputs come out
+ * grouped by cell (the reference reorders its particles every step as well,
+ * fluidsimulation.cpp:3211-3223); id_in/id_out (optional) carry a particle
+ * label through that permutation.  u,v,w receive the grid after the body
+ * force; the snapshot (u_old..) and the weights (wu..) are written when not
+ * NULL. */
+typedef struct gfsx_scene {
+    int ni, nj, nk;
+    double dx;
+    int64_t n;
+    const float *x_in, *y_in, *z_in;
+    const float *vx_in, *vy_in, *vz_in;
+    float *x_out, *y_out, *z_out;
+    float *vx_out, *vy_out, *vz_out;
+    const uint32_t *id_in; /* optional */
+    uint32_t *id_out;      /* optional */
+    float *u, *v, *w;
+    float *u_old, *v_old, *w_old; /* optional */
+    float *wu, *wv, *ww;          /* optional */
+} gfsx_scene;
+
+typedef struct gfsx_step_params {
+    double dt;
+    float body_force[3]; /* e.g. (0, -9.8, 0) */
+    double pic_ratio;    /* 0.05 in the reference (fluidsimulation.h:1161) */
+    int stencil;         /* GFSX_TRILINEAR or GFSX_TRICUBIC */
+    int order;           /* GFSX_ORDER_SORTED or GFSX_ORDER_KEEP */
+} gfsx_step_params;
+
+void gfsx_flip_step(gfsx_ctx *ctx, const gfsx_scene *scene,
+                    const gfsx_step_params *params, int *err);
+
+/* Per-stage timing of gfsx_flip_step with HIP events recorded on the
+ * context's stream.  After gfsx_timing_begin every step records four events:
+ * before the cell sort, after the sort, after the P2G gather and after the
+ * G2P/PIC-FLIP/RK4 kernel.  gfsx_timing_end waits for the stream, writes the
+ * summed milliseconds of the three stages to stage_ms[0..2] and of whole
+ * steps to stage_ms[3], stops recording and returns the number of steps
+ * timed.  (The reference only has wall-clock StopWatch timers per stage,
+ * fluidsimulation.cpp:3273-3389.) */
+void gfsx_timing_begin(gfsx_ctx *ctx, int *err);
+int gfsx_timing_end(gfsx_ctx *ctx, double stage_ms[4], int *err);
+
+/* Stable counting sort of particle indices by cell (the binning the P2G
+ * gather uses), exposed for testing: perm[n] receives the indices ordered by
+ * cell, then by index; cell_start[ni*nj*nk+1] the cell offsets.  Particles
+ * outside the grid are placed after cell_start[ni*nj*nk], in index order.
+ * pos is SoA (x, y, z device arrays). */
+void gfsx_cell_sort_device(gfsx_ctx *ctx, const float *x, const float *y,
+                           const float *z, int64_t n, double dx, int ni,
+                           int nj, int nk, uint32_t *perm,
+                           uint32_t *cell_start, int *err);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GFSX_H */

@@ -52,6 +52,7 @@ struct gfsx_ctx {
     std::string info;
     void *buf[NSLOT] = {};
     size_t bytes[NSLOT] = {};
+    int p2g_variant = 0;
     bool timing = false;
     std::vector<std::array<hipEvent_t, 4>> events;
     size_t events_used = 0;
@@ -101,13 +102,14 @@ struct gfsx_ctx {
         return sb;
     }
 
+    // Teardown cannot report failures; the runtime's status is discarded.
     ~gfsx_ctx() {
-        hipSetDevice(device);
+        (void)hipSetDevice(device);
         for (int s = 0; s < NSLOT; s++)
-            if (buf[s]) hipFree(buf[s]);
+            if (buf[s]) (void)hipFree(buf[s]);
         for (auto &e : events)
-            for (auto &x : e) hipEventDestroy(x);
-        if (own_stream) hipStreamDestroy(own_stream);
+            for (auto &x : e) (void)hipEventDestroy(x);
+        if (own_stream) (void)hipStreamDestroy(own_stream);
     }
 };
 
@@ -257,9 +259,11 @@ void gfsx_destroy(gfsx_ctx *ctx, int *err) {
         return;
     }
     try {
-        hipSetDevice(ctx->device);
-        hipStreamSynchronize(ctx->stream);
+        GFSX_HIP(hipSetDevice(ctx->device));
+        hipError_t e = hipStreamSynchronize(ctx->stream);
         delete ctx;
+        if (e != hipSuccess)
+            throw Error(std::string("gfsx_destroy: ") + hipGetErrorString(e));
         ok(err);
     } catch (const std::exception &e) {
         fail(err, e.what());
@@ -488,9 +492,32 @@ void gfsx_flip_step(gfsx_ctx *ctx, const gfsx_scene *sc,
         a.r_flip = (float)(1 - pr->pic_ratio);
         a.stencil = pr->stencil;
         a.keep_order = pr->order == GFSX_ORDER_KEEP;
+        a.p2g_variant = ctx->p2g_variant;
         gfsx::SortBuffers sb = ctx->sort_buffers(sc->n, (uint32_t)nbin);
         gfsx::flip_step(a, sb, ctx->stream, ctx->timing ? ctx->next_events() : nullptr);
     });
+}
+
+void gfsx_set_option(gfsx_ctx *ctx, const char *name, int value, int *err) {
+    guarded(ctx, err, [&] {
+        require(name != nullptr, "gfsx_set_option: null name");
+        if (std::strcmp(name, "p2g_variant") == 0) {
+            require(value >= 0 && value <= 3, "gfsx_set_option: p2g_variant must be 0..3");
+            ctx->p2g_variant = value;
+        } else {
+            throw Error(std::string("gfsx_set_option: unknown option ") + name);
+        }
+    });
+}
+
+int gfsx_get_option(gfsx_ctx *ctx, const char *name, int *err) {
+    int v = 0;
+    guarded(ctx, err, [&] {
+        require(name != nullptr, "gfsx_get_option: null name");
+        if (std::strcmp(name, "p2g_variant") == 0) v = ctx->p2g_variant;
+        else throw Error(std::string("gfsx_get_option: unknown option ") + name);
+    });
+    return v;
 }
 
 void gfsx_timing_begin(gfsx_ctx *ctx, int *err) {

@@ -87,11 +87,6 @@ __device__ __forceinline__ float wyvill(float t) {
 
 // ---- launches (implemented in gfsx_kernels.hip) ------------------------
 
-struct ScanScratch {
-    uint32_t *level[3];   // per-level tile sums
-    uint32_t *level_out[3];
-};
-
 void fill_u32(uint32_t *p, uint32_t value, int64_t count, hipStream_t s);
 void fill_f32(float *p, float value, int64_t count, hipStream_t s);
 int64_t scan_scratch_words(int64_t m);
@@ -147,6 +142,7 @@ struct StepArgs {
     float r_pic, r_flip;             // (float)ratio, (float)(1-ratio)
     int stencil;
     int keep_order;
+    int p2g_variant;   // 0: brick-staged gather, 1: direct global gather
 };
 
 // The fused plain step.  Needs SortBuffers sized for n particles and

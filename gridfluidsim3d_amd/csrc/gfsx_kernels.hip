@@ -756,6 +756,451 @@ __global__ __launch_bounds__(kBlock) void k_flip_p2g(
     a.grid[c][f] = val;
 }
 
+// ---------------------------------------------------------------------------
+// Brick-staged P2G gather for the fused step.
+//
+// One work-group per 4x4x4-cell brick.  It stages every particle of the
+// brick plus a one-cell halo (6x6x6 cells) into LDS, re-laid out in
+// half-cell-major order: half-cell (2c+o) per axis, where o says in which
+// half of its cell the particle lies.  A face then has all its candidate
+// particles in 16 contiguous LDS ranges -- the 4x4x4 half-cells whose
+// union is the 2x2x2 "shifted cells" around it -- instead of 18 whole
+// cells: 64 candidates instead of 144 at 8 particles per cell.  The order
+// inside a half-cell is the order of the sorted particle list, so the sum
+// for every face is taken in a fixed order.
+// ---------------------------------------------------------------------------
+namespace brick {
+constexpr int BX = 4, BY = 4, BZ = 4;
+constexpr int SX = BX + 2, SY = BY + 2, SZ = BZ + 2;  // staged cells
+constexpr int NROW = SY * SZ;
+constexpr int HX = 2 * SX, HY = 2 * SY, HZ = 2 * SZ;  // half-cells
+constexpr int NH = HX * HY * HZ;
+constexpr int NCELL = BX * BY * BZ;       // 64 cells: one wave per component
+constexpr int NT = 3 * NCELL;             // 192 threads
+// Particles staged per brick (6 floats each).  With the half-cell table
+// this keeps a work-group under 54 KB of LDS, so three fit in a CU's
+// 160 KB; a dense brick (6x6x6 cells at 8 per cell) stages 1728.
+constexpr int CAP = 2016;
+constexpr int PER = (CAP + NT - 1) / NT;
+static_assert(NCELL == 64, "one wave per component");
+static_assert(NH + 1 <= CAP, "half-cell counts live in the x array");
+}  // namespace brick
+
+__device__ __forceinline__ void face_epilogue(const StepP2G &a,
+                                              const BinGrid &bg,
+                                              const uint32_t *__restrict__ count,
+                                              int c, int i, int j, int k,
+                                              float S, float W) {
+    const NodeGrid &ng = a.ng[c];
+    const int64_t f = (int64_t)i + (int64_t)ng.n[0] * ((int64_t)j + (int64_t)ng.n[1] * k);
+    float val = W > 0.0f ? S / W : S;
+    if (a.grid_old[c]) a.grid_old[c][f] = val;
+    if (a.wgt[c]) a.wgt[c][f] = W;
+    if (a.grav_on[c]) {
+        const int ax = c == 0 ? i : (c == 1 ? j : k);
+        const int nax = c == 0 ? a.ni : (c == 1 ? a.nj : a.nk);
+        bool fluid = false;
+        const uint32_t bhi = (uint32_t)(i + 1) + (uint32_t)bg.b[0] *
+                             ((uint32_t)(j + 1) + (uint32_t)bg.b[1] * (uint32_t)(k + 1));
+        const uint32_t step = c == 0 ? 1u : (c == 1 ? (uint32_t)bg.b[0]
+                                                    : (uint32_t)bg.b[0] * (uint32_t)bg.b[1]);
+        if (ax < nax) fluid = fluid || count[bhi] > 0u;
+        if (ax > 0) fluid = fluid || count[bhi - step] > 0u;
+        if (fluid) val += a.grav_delta[c];
+    }
+    a.grid[c][f] = val;
+}
+
+// Range r (0..15) of the 4x4 (y,z) rows of half-cells around a face; each
+// row covers 4 consecutive half-cells along x, contiguous in LDS.
+__device__ __forceinline__ void brick_range(const uint16_t *__restrict__ hs,
+                                            int hx0, int hy0, int hz0, int r,
+                                            uint32_t &q, uint32_t &e) {
+    using namespace brick;
+    const int base = hx0 + HX * ((hy0 + (r & 3)) + HY * (hz0 + (r >> 2)));
+    q = hs[base];
+    e = hs[base + 4];
+}
+
+// Gather over the 4x4x4 half-cells around face (li,lj,lk) (staged cell
+// coordinates) of component C, as one stream of candidates per lane: the
+// lane steps to its next non-empty range when it exhausts one, so a wave
+// runs for the longest lane's total, not for the sum of per-range maxima.
+template <int C>
+__device__ __forceinline__ void brick_gather(const float *__restrict__ sx,
+                                             const float *__restrict__ sy,
+                                             const float *__restrict__ sz,
+                                             const float *__restrict__ sv,
+                                             const uint16_t *__restrict__ hs,
+                                             int li, int lj, int lk, float gx,
+                                             float gy, float gz, float offh,
+                                             float inv_r2, float &S, float &W) {
+    const int hx0 = C == 0 ? 2 * li - 2 : 2 * li - 1;
+    const int hy0 = C == 1 ? 2 * lj - 2 : 2 * lj - 1;
+    const int hz0 = C == 2 ? 2 * lk - 2 : 2 * lk - 1;
+    int r = 0;
+    uint32_t q, e;
+    brick_range(hs, hx0, hy0, hz0, 0, q, e);
+    while (true) {
+        while (q >= e && r < 15) {
+            r++;
+            brick_range(hs, hx0, hy0, hz0, r, q, e);
+        }
+        if (q >= e) break;
+        float px = sx[q], py = sy[q], pz = sz[q];
+        const float pv = sv[q];
+        // p -= offset (scalarfield.cpp:168); the offset is zero on the
+        // component's own axis
+        if (C != 0) px -= offh;
+        if (C != 1) py -= offh;
+        if (C != 2) pz -= offh;
+        const float vx = gx - px, vy = gy - py, vz = gz - pz;
+        const float d2 = vx * vx + vy * vy + vz * vz;
+        const float K = wyvill(d2 * inv_r2);
+        S = fmaf(K, pv, S);
+        W += K;
+        q++;
+    }
+}
+
+// The same gather as 16 nested range loops.
+template <int C>
+__device__ __forceinline__ void brick_gather_nested(const float *__restrict__ sx,
+                                                    const float *__restrict__ sy,
+                                                    const float *__restrict__ sz,
+                                                    const float *__restrict__ sv,
+                                                    const uint16_t *__restrict__ hs,
+                                                    int li, int lj, int lk, float gx,
+                                                    float gy, float gz, float offh,
+                                                    float inv_r2, float &S, float &W) {
+    const int hx0 = C == 0 ? 2 * li - 2 : 2 * li - 1;
+    const int hy0 = C == 1 ? 2 * lj - 2 : 2 * lj - 1;
+    const int hz0 = C == 2 ? 2 * lk - 2 : 2 * lk - 1;
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        uint32_t q, e;
+        brick_range(hs, hx0, hy0, hz0, r, q, e);
+        for (; q < e; q++) {
+            float px = sx[q], py = sy[q], pz = sz[q];
+            const float pv = sv[q];
+            if (C != 0) px -= offh;
+            if (C != 1) py -= offh;
+            if (C != 2) pz -= offh;
+            const float vx = gx - px, vy = gy - py, vz = gz - pz;
+            const float d2 = vx * vx + vy * vy + vz * vz;
+            const float K = wyvill(d2 * inv_r2);
+            S = fmaf(K, pv, S);
+            W += K;
+        }
+    }
+}
+
+// The same gather with all 16 range bounds read up front and candidates
+// taken four at a time (the tail of a range is masked to zero weight), so
+// one LDS round trip serves four candidates.
+template <int C>
+__device__ __forceinline__ void brick_gather_chunked(const float *__restrict__ sx,
+                                                     const float *__restrict__ sy,
+                                                     const float *__restrict__ sz,
+                                                     const float *__restrict__ sv,
+                                                     const uint16_t *__restrict__ hs,
+                                                     int li, int lj, int lk, float gx,
+                                                     float gy, float gz, float offh,
+                                                     float inv_r2, float &S, float &W) {
+    const int hx0 = C == 0 ? 2 * li - 2 : 2 * li - 1;
+    const int hy0 = C == 1 ? 2 * lj - 2 : 2 * lj - 1;
+    const int hz0 = C == 2 ? 2 * lk - 2 : 2 * lk - 1;
+    uint32_t rq[16], re[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++) brick_range(hs, hx0, hy0, hz0, r, rq[r], re[r]);
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const uint32_t e = re[r];
+        for (uint32_t q = rq[r]; q < e; q += 4) {
+            float px[4], py[4], pz[4], pv[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const uint32_t qq = min(q + u, e - 1);
+                px[u] = sx[qq];
+                py[u] = sy[qq];
+                pz[u] = sz[qq];
+                pv[u] = sv[qq];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                if (C != 0) px[u] -= offh;
+                if (C != 1) py[u] -= offh;
+                if (C != 2) pz[u] -= offh;
+                const float vx = gx - px[u], vy = gy - py[u], vz = gz - pz[u];
+                const float d2 = vx * vx + vy * vy + vz * vz;
+                float K = wyvill(d2 * inv_r2);
+                K = (q + u < e) ? K : 0.0f;
+                S = fmaf(K, pv[u], S);
+                W += K;
+            }
+        }
+    }
+}
+
+// Row of staged slot s: the number of row starts (after the first) that
+// are <= s.  All lanes read the same LDS words, so the reads broadcast and
+// do not depend on one another.
+__device__ __forceinline__ int brick_row_of(const uint32_t *row_s0, uint32_t s) {
+    int r = 0;
+#pragma unroll
+    for (int q = 1; q < brick::NROW; q++) r += row_s0[q] <= s;
+    return r;
+}
+
+template <int C, int V>
+__device__ __forceinline__ void brick_face(const StepP2G &a, const BinGrid &bg,
+                                           const uint32_t *__restrict__ count,
+                                           const float *sx, const float *sy,
+                                           const float *sz, const float *sv,
+                                           const uint16_t *hs, int i0, int j0,
+                                           int k0, int li, int lj, int lk,
+                                           double dx, float inv_r2) {
+    const int fi = i0 + li, fj = j0 + lj, fk = k0 + lk;
+    if (fi >= a.ni + (C == 0) || fj >= a.nj + (C == 1) || fk >= a.nk + (C == 2)) return;
+    // face position in the component's shifted frame (grid3d.h:79-81)
+    const float gx = (float)((float)fi * dx);
+    const float gy = (float)((float)fj * dx);
+    const float gz = (float)((float)fk * dx);
+    const float offh = a.ng[C].off[C == 0 ? 1 : 0];
+    float S = 0.0f, W = 0.0f;
+    if (V == 0)
+        brick_gather_chunked<C>(sx, sy, sz, sv, hs, li + 1, lj + 1, lk + 1, gx, gy, gz, offh, inv_r2, S, W);
+    else if (V == 2)
+        brick_gather_nested<C>(sx, sy, sz, sv, hs, li + 1, lj + 1, lk + 1, gx, gy, gz, offh, inv_r2, S, W);
+    else
+        brick_gather<C>(sx, sy, sz, sv, hs, li + 1, lj + 1, lk + 1, gx, gy, gz, offh, inv_r2, S, W);
+    face_epilogue(a, bg, count, C, fi, fj, fk, S, W);
+}
+
+__device__ __forceinline__ void brick_zero_face(const StepP2G &a, int c, int fi, int fj, int fk) {
+    if (fi >= a.ni + (c == 0) || fj >= a.nj + (c == 1) || fk >= a.nk + (c == 2)) return;
+    const NodeGrid &ng = a.ng[c];
+    const int64_t f = (int64_t)fi + (int64_t)ng.n[0] * ((int64_t)fj + (int64_t)ng.n[1] * fk);
+    if (a.grid_old[c]) a.grid_old[c][f] = 0.0f;
+    if (a.wgt[c]) a.wgt[c][f] = 0.0f;
+    a.grid[c][f] = 0.0f;
+}
+
+template <int V>
+__global__ __launch_bounds__(brick::NT) void k_flip_p2g_brick(
+    StepP2G a, BinGrid bg, double dx, float inv_r2, int nbx, int nby, int nbz,
+    int per_xcd, const uint32_t *__restrict__ count,
+    const uint32_t *__restrict__ start, const uint32_t *__restrict__ perm) {
+    using namespace brick;
+    __shared__ float s_x[CAP], s_y[CAP], s_z[CAP];
+    __shared__ float s_v[3][CAP];
+    __shared__ uint16_t s_hs[NH + 1];
+    __shared__ uint8_t s_oct[CAP];
+    __shared__ uint32_t s_rowq[NROW], s_rows0[NROW + 1];
+    __shared__ uint32_t s_wsum[NT / 64];
+    // half-cell counts (32-bit, for LDS atomics) borrow the x array, which
+    // is only written once the layout is known
+    uint32_t *s_cnt = reinterpret_cast<uint32_t *>(s_x);
+
+    // consecutive bricks share halo particles: keep them on one XCD (work-
+    // groups are dealt round-robin over the 8 XCDs)
+    const int xcd = blockIdx.x & 7;
+    const int brick_id = xcd * per_xcd + (blockIdx.x >> 3);
+    const int nbrick = nbx * nby * nbz;
+    if (brick_id >= nbrick) return;
+    const int bxi = brick_id % nbx, byi = (brick_id / nbx) % nby, bzi = brick_id / (nbx * nby);
+    const int i0 = bxi * BX, j0 = byi * BY, k0 = bzi * BZ;
+    const int t = threadIdx.x;
+    const int wave = t >> 6, lane = t & 63;
+    const uint32_t row_stride = (uint32_t)bg.b[0];
+    const int bx_hi = min(i0 + BX + 1, bg.b[0] - 1);
+
+    // rows of staged cells: world cells [i0-1, i0+BX] along x are bins
+    // [i0, i0+BX+1] (pad 1), contiguous in perm; wave 0 scans the lengths
+    if (wave == 0) {
+        uint32_t q0 = 0, len = 0;
+        if (lane < NROW) {
+            const int cy = j0 - 1 + lane % SY, cz = k0 - 1 + lane / SY;
+            if (cy >= -1 && cy <= a.nj && cz >= -1 && cz <= a.nk) {
+                const uint32_t row = row_stride * ((uint32_t)(cy + 1) + (uint32_t)bg.b[1] * (uint32_t)(cz + 1));
+                q0 = start[row + i0];
+                len = start[row + bx_hi + 1] - q0;
+            }
+            s_rowq[lane] = q0;
+        }
+        const uint32_t inc = wave_inclusive_sum(len);
+        if (lane < NROW) s_rows0[lane + 1] = inc;
+        if (lane == 0) s_rows0[0] = 0u;
+    }
+    for (int h = t; h <= NH; h += NT) s_cnt[h] = 0u;
+    __syncthreads();
+    const uint32_t N = s_rows0[NROW];
+
+    // this thread's cell; wave w computes component w
+    const int li = lane % BX, lj = (lane / BX) % BY, lk = lane / (BX * BY);
+    // the faces on the domain's upper planes belong to the last brick
+    const bool xtra = (wave == 0 && i0 + BX == a.ni) || (wave == 1 && j0 + BY == a.nj) ||
+                      (wave == 2 && k0 + BZ == a.nk);
+    const int e0 = lane & 3, e1 = (lane >> 2) & 3;   // lanes 0..15: the 4x4 plane
+
+    if (N == 0) {
+        // nothing within reach of any face of this brick: all values are 0
+        // and no face borders fluid
+        brick_zero_face(a, wave, i0 + li, j0 + lj, k0 + lk);
+        if (xtra && lane < 16) {
+            if (wave == 0) brick_zero_face(a, 0, i0 + BX, j0 + e0, k0 + e1);
+            else if (wave == 1) brick_zero_face(a, 1, i0 + e0, j0 + BY, k0 + e1);
+            else brick_zero_face(a, 2, i0 + e0, j0 + e1, k0 + BZ);
+        }
+        return;
+    }
+    if (N > (uint32_t)CAP) {
+        // too many particles to stage: gather straight from global memory
+        SoaLoad ld{a.x, a.y, a.z, a.vel[wave]};
+        for (int pass = 0; pass < 2; pass++) {
+            int fi = i0 + li, fj = j0 + lj, fk = k0 + lk;
+            if (pass == 1) {
+                if (!(xtra && lane < 16)) break;
+                if (wave == 0) { fi = i0 + BX; fj = j0 + e0; fk = k0 + e1; }
+                else if (wave == 1) { fi = i0 + e0; fj = j0 + BY; fk = k0 + e1; }
+                else { fi = i0 + e0; fj = j0 + e1; fk = k0 + BZ; }
+            }
+            if (fi < a.ni + (wave == 0) && fj < a.nj + (wave == 1) && fk < a.nk + (wave == 2)) {
+                float S = 0.0f, W = 0.0f;
+                gather_node<true>(fi, fj, fk, a.ng[wave], bg, dx, inv_r2, start, perm, ld, S, W);
+                face_epilogue(a, bg, count, wave, fi, fj, fk, S, W);
+            }
+        }
+        return;
+    }
+
+    // stage 1: positions -> half-cell of every staged particle; count them.
+    // Every slot's loads are issued before any is used (slots past N load
+    // particle 0 and are masked), so the global latency is paid once per
+    // batch instead of once per slot.
+    uint32_t rp[PER], rh[PER], rfirst[PER];
+    int rrow[PER];
+#pragma unroll
+    for (int m = 0; m < PER; m++) {
+        const uint32_t s = (uint32_t)t + (uint32_t)m * NT;
+        const int r = brick_row_of(s_rows0, s);
+        rrow[m] = r;
+        const uint32_t q = s_rowq[r] + (s - s_rows0[r]);
+        rp[m] = perm[s < N ? q : 0u];
+    }
+    float rx[PER], ry[PER], rz[PER];
+#pragma unroll
+    for (int m = 0; m < PER; m++) {
+        const uint32_t s = (uint32_t)t + (uint32_t)m * NT;
+        const uint32_t p = s < N ? rp[m] : rp[0];
+        rx[m] = a.x[p];
+        ry[m] = a.y[p];
+        rz[m] = a.z[p];
+    }
+    uint32_t rbin[PER];
+#pragma unroll
+    for (int m = 0; m < PER; m++) {
+        // the same FP64 floor as the binning, so the cell agrees
+        const double gxd = (double)rx[m] * bg.inv_dx;
+        const double gyd = (double)ry[m] * bg.inv_dx;
+        const double gzd = (double)rz[m] * bg.inv_dx;
+        const double fx = floor(gxd), fy = floor(gyd), fz = floor(gzd);
+        const int lx = (int)fx - (i0 - 1), ly = (int)fy - (j0 - 1), lz = (int)fz - (k0 - 1);
+        const int ox = (gxd - fx) >= 0.5, oy = (gyd - fy) >= 0.5, oz = (gzd - fz) >= 0.5;
+        rh[m] = (uint32_t)((2 * lx + ox) + HX * ((2 * ly + oy) + HY * (2 * lz + oz)));
+        const uint32_t s = (uint32_t)t + (uint32_t)m * NT;
+        // the particle's bin (pad 1); slots past N use the brick's first row
+        rbin[m] = s < N ? (uint32_t)(fx + 1.0) + row_stride * ((uint32_t)(fy + 1.0) +
+                                                 (uint32_t)bg.b[1] * (uint32_t)(fz + 1.0))
+                        : (uint32_t)i0;
+        if (s < N) {
+            s_oct[s] = (uint8_t)(ox + 2 * oy + 4 * oz);
+            atomicAdd(&s_cnt[rh[m]], 1u);
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < PER; m++) {
+        // first staged slot of the particle's cell: its bin's start,
+        // relative to the row
+        const int r = rrow[m];
+        rfirst[m] = s_rows0[r] + (start[rbin[m]] - s_rowq[r]);
+    }
+    __syncthreads();
+
+    // stage 2: exclusive prefix over the half-cells
+    {
+        constexpr int ITEMS = (NH + NT - 1) / NT;
+        uint32_t loc[ITEMS];
+        uint32_t sum = 0;
+#pragma unroll
+        for (int m = 0; m < ITEMS; m++) {
+            const int h = t * ITEMS + m;
+            loc[m] = h < NH ? s_cnt[h] : 0u;
+            sum += loc[m];
+        }
+        // block-wide exclusive sum over the 3 waves
+        const uint32_t inc = wave_inclusive_sum(sum);
+        if (lane == 63) s_wsum[wave] = inc;
+        __syncthreads();
+        uint32_t run = inc - sum;
+        for (int w = 0; w < wave; w++) run += s_wsum[w];
+#pragma unroll
+        for (int m = 0; m < ITEMS; m++) {
+            const int h = t * ITEMS + m;
+            if (h < NH) s_hs[h] = (uint16_t)run;
+            run += loc[m];
+        }
+        if (t == 0) s_hs[NH] = (uint16_t)N;
+    }
+    __syncthreads();
+
+    // stage 3: place every particle at its half-cell's start plus its rank
+    // among the earlier particles of its cell in the same half-cell (a
+    // half-cell lies in one cell, whose particles are contiguous)
+    float rvx[PER], rvy[PER], rvz[PER];
+#pragma unroll
+    for (int m = 0; m < PER; m++) {
+        const uint32_t s = (uint32_t)t + (uint32_t)m * NT;
+        const uint32_t p = s < N ? rp[m] : rp[0];
+        rvx[m] = a.vel[0][p];
+        rvy[m] = a.vel[1][p];
+        rvz[m] = a.vel[2][p];
+    }
+#pragma unroll
+    for (int m = 0; m < PER; m++) {
+        const uint32_t s = (uint32_t)t + (uint32_t)m * NT;
+        if (s < N) {
+            const uint8_t oct = s_oct[s];
+            uint32_t rank = 0;
+            for (uint32_t q = rfirst[m]; q < s; q++) rank += s_oct[q] == oct;
+            const uint32_t d = (uint32_t)s_hs[rh[m]] + rank;
+            s_x[d] = rx[m];
+            s_y[d] = ry[m];
+            s_z[d] = rz[m];
+            s_v[0][d] = rvx[m];
+            s_v[1][d] = rvy[m];
+            s_v[2][d] = rvz[m];
+        }
+    }
+    __syncthreads();
+
+    // gather: wave w computes component w for its 64 cells, then the
+    // domain-boundary plane if this brick has one
+    if (wave == 0) {
+        brick_face<0, V>(a, bg, count, s_x, s_y, s_z, s_v[0], s_hs, i0, j0, k0, li, lj, lk, dx, inv_r2);
+        if (xtra && lane < 16)
+            brick_face<0, V>(a, bg, count, s_x, s_y, s_z, s_v[0], s_hs, i0, j0, k0, BX, e0, e1, dx, inv_r2);
+    } else if (wave == 1) {
+        brick_face<1, V>(a, bg, count, s_x, s_y, s_z, s_v[1], s_hs, i0, j0, k0, li, lj, lk, dx, inv_r2);
+        if (xtra && lane < 16)
+            brick_face<1, V>(a, bg, count, s_x, s_y, s_z, s_v[1], s_hs, i0, j0, k0, e0, BY, e1, dx, inv_r2);
+    } else {
+        brick_face<2, V>(a, bg, count, s_x, s_y, s_z, s_v[2], s_hs, i0, j0, k0, li, lj, lk, dx, inv_r2);
+        if (xtra && lane < 16)
+            brick_face<2, V>(a, bg, count, s_x, s_y, s_z, s_v[2], s_hs, i0, j0, k0, e0, e1, BZ, dx, inv_r2);
+    }
+}
+
 struct StepG2P {
     const float *xin[3], *vin[3];
     float *xout[3], *vout[3];
@@ -845,11 +1290,25 @@ void flip_step(const StepArgs &s, const SortBuffers &sb, hipStream_t st,
     a.ni = s.ni;
     a.nj = s.nj;
     a.nk = s.nk;
-    const int64_t F = a.nface[0] + a.nface[1] + a.nface[2];
     const float inv_r2 = (float)(1.0 / (s.dx * s.dx));
-    hipLaunchKernelGGL(k_flip_p2g, dim3(grid_for(F, kBlock)), dim3(kBlock), 0,
-                       st, a, bg, s.dx, inv_r2, (const uint32_t *)sb.count,
-                       (const uint32_t *)sb.start, (const uint32_t *)sb.perm);
+    if (s.p2g_variant != 1) {
+        const int nbx = (s.ni + brick::BX - 1) / brick::BX;
+        const int nby = (s.nj + brick::BY - 1) / brick::BY;
+        const int nbz = (s.nk + brick::BZ - 1) / brick::BZ;
+        const int64_t nbrick = (int64_t)nbx * nby * nbz;
+        const int per_xcd = (int)((nbrick + 7) / 8);
+        const dim3 grid((unsigned)(8 * (int64_t)per_xcd)), block(brick::NT);
+        auto *kern = s.p2g_variant == 2 ? k_flip_p2g_brick<2>
+                   : s.p2g_variant == 3 ? k_flip_p2g_brick<3> : k_flip_p2g_brick<0>;
+        hipLaunchKernelGGL(kern, grid, block, 0, st, a, bg, s.dx, inv_r2, nbx, nby, nbz,
+                           per_xcd, (const uint32_t *)sb.count,
+                           (const uint32_t *)sb.start, (const uint32_t *)sb.perm);
+    } else {
+        const int64_t F = a.nface[0] + a.nface[1] + a.nface[2];
+        hipLaunchKernelGGL(k_flip_p2g, dim3(grid_for(F, kBlock)), dim3(kBlock), 0,
+                           st, a, bg, s.dx, inv_r2, (const uint32_t *)sb.count,
+                           (const uint32_t *)sb.start, (const uint32_t *)sb.perm);
+    }
     GFSX_LAUNCH_CHECK();
     if (ev) GFSX_HIP(hipEventRecord(ev[2], st));
 

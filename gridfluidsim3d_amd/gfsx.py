@@ -103,7 +103,11 @@ class Context:
         self.device = device
 
     def __del__(self):
-        self.close()
+        try:
+            self.close()
+        except Exception:
+            # interpreter shutdown may already have torn down ctypes
+            pass
 
     def close(self):
         h = getattr(self, "_h", None)
@@ -125,6 +129,15 @@ class Context:
     def synchronize(self):
         f = pb.lib_func("gfsx_synchronize", [c_void_p])
         pb.execute_lib_func(f, [self()])
+
+    def set_option(self, name, value):
+        """Set a tunable of the fused step (see gfsx_set_option)."""
+        f = pb.lib_func("gfsx_set_option", [c_void_p, ctypes.c_char_p, c_int])
+        pb.execute_lib_func(f, [self(), name.encode(), int(value)])
+
+    def get_option(self, name):
+        f = pb.lib_func("gfsx_get_option", [c_void_p, ctypes.c_char_p], c_int)
+        return pb.execute_lib_func(f, [self(), name.encode()])
 
     def timing_begin(self):
         """Record per-stage HIP events for subsequent flip_step calls."""

@@ -164,6 +164,12 @@ typedef struct gfsx_step_params {
 void gfsx_flip_step(gfsx_ctx *ctx, const gfsx_scene *scene,
                     const gfsx_step_params *params, int *err);
 
+/* Tunables of the fused step, by name (an unknown name fails):
+ *   "p2g_variant": 0 = brick-staged LDS gather (default),
+ *                  1 = direct gather from global memory. */
+void gfsx_set_option(gfsx_ctx *ctx, const char *name, int value, int *err);
+int gfsx_get_option(gfsx_ctx *ctx, const char *name, int *err);
+
 /* Per-stage timing of gfsx_flip_step with HIP events recorded on the
  * context's stream.  After gfsx_timing_begin every step records four events:
  * before the cell sort, after the sort, after the P2G gather and after the

@@ -320,3 +320,53 @@ def test_bad_arguments_raise(ctx):
                    (2, 2, 2), 0.1, 0.01, order=5)
     with pytest.raises(RuntimeError, match="radius"):
         ctx.p2g(np.zeros((1, 3), np.float32), None, -1.0, (0, 0, 0), 0.1, (2, 2, 2))
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+def test_step_p2g_variants(ctx, orc, variant):
+    """Brick-staged (0) and direct (1) P2G gathers both match the oracle."""
+    from gridfluidsim3d_amd import scenes
+    sc = scenes.random_blobs(21, (27, 18, 13), 0.1, amplitude=2.0)
+    exp = orc.flip_step(sc.pos, sc.vel, sc.dims, 0.1, sc.dt, (0.0, -9.8, 0.0), 0.05, TRILINEAR)
+    old = ctx.get_option("p2g_variant")
+    ctx.set_option("p2g_variant", variant)
+    try:
+        res = _run_step(ctx, sc.pos, sc.vel, sc.dims, 0.1, sc.dt, (0.0, -9.8, 0.0), 0.05,
+                        TRILINEAR, 0)
+    finally:
+        ctx.set_option("p2g_variant", old)
+    _check_step(res, exp, False)
+
+
+def test_step_dense_cells_use_fallback(ctx, orc):
+    """Bricks holding more particles than fit in LDS take the global-memory
+    gather; results still match the oracle."""
+    rng = np.random.default_rng(5)
+    dims = (12, 12, 12)
+    dx = 0.1
+    dense = rng.uniform(0.3, 0.7, size=(6000, 3)).astype(np.float32)   # ~94 per cell
+    sparse = rng.uniform(0.0, 1.2, size=(2000, 3)).astype(np.float32)
+    pos = np.concatenate([dense, sparse])
+    vel = rng.normal(size=pos.shape).astype(np.float32)
+    exp = orc.flip_step(pos, vel, dims, dx, 1 / 240, (0.0, -9.8, 0.0), 0.05, TRILINEAR)
+    res = _run_step(ctx, pos, vel, dims, dx, 1 / 240, (0.0, -9.8, 0.0), 0.05, TRILINEAR, 0)
+    _check_step(res, exp, False)
+
+
+def test_step_is_deterministic(ctx):
+    from gridfluidsim3d_amd import scenes
+    sc = scenes.random_blobs(8, (30, 22, 17), 0.05, amplitude=3.0)
+    r1 = _run_step(ctx, sc.pos, sc.vel, sc.dims, 0.05, sc.dt, (0.0, -9.8, 0.0), 0.05,
+                   TRILINEAR, 0)
+    r2 = _run_step(ctx, sc.pos, sc.vel, sc.dims, 0.05, sc.dt, (0.0, -9.8, 0.0), 0.05,
+                   TRILINEAR, 0)
+    for key in r1:
+        assert np.array_equal(r1[key], r2[key]), key
+
+
+def test_options(ctx):
+    assert ctx.get_option("p2g_variant") == 0
+    with pytest.raises(RuntimeError, match="unknown option"):
+        ctx.set_option("no_such_option", 1)
+    with pytest.raises(RuntimeError, match="p2g_variant"):
+        ctx.set_option("p2g_variant", 7)

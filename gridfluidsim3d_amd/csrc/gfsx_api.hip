@@ -33,7 +33,7 @@ void ok(int *err) {
 }
 
 enum Slot {
-    S_COUNT, S_START, S_KEY, S_SLOT, S_PERM, S_BOUT, S_BOUTS, S_SCAN,
+    S_COUNT, S_START, S_KEY, S_SLOT, S_PERM, S_BOUT, S_BOUTS, S_SCAN, S_OCT, S_POCT,
     S_OLD0, S_OLD1, S_OLD2,
     S_D_POS, S_D_VAL, S_D_S, S_D_W, S_D_U, S_D_V, S_D_WW, S_D_OUT,
     NSLOT
@@ -99,6 +99,8 @@ struct gfsx_ctx {
         int64_t words = std::max(gfsx::scan_scratch_words((int64_t)nbin + 1),
                                  gfsx::scan_scratch_words(nb));
         sb.scan_scratch = u32(S_SCAN, words);
+        sb.oct = (uint8_t *)get(S_OCT, (size_t)std::max<int64_t>(n, 1));
+        sb.poct = (uint8_t *)get(S_POCT, (size_t)std::max<int64_t>(n, 1));
         return sb;
     }
 

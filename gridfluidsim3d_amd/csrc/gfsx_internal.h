@@ -102,16 +102,21 @@ struct SortBuffers {
     uint32_t *block_out;  // blocks(n)
     uint32_t *block_out_scan;  // blocks(n)+1
     uint32_t *scan_scratch;
+    uint8_t *oct;         // n: half of its cell each particle lies in
+    uint8_t *poct;        // n: the same, in perm order
 };
 int64_t sort_blocks(int64_t n);
 
 // Stable counting sort of particle indices by bin: perm lists bin 0's
 // particles in index order, then bin 1's, ..., then the overflow bin's.
 // start[b] is the first perm slot of bin b; start[nbin+1] == n.
+// With octant_order, particles inside a bin are ordered by (octant, index)
+// instead, the octant being the 3-bit index of the half of the cell the
+// particle lies in along x, y, z (so each half-cell is a contiguous run).
 // aos != nullptr selects AoS input (x,y,z interleaved); otherwise SoA.
 void cell_sort(const float *aos, const float *x, const float *y,
                const float *z, int64_t n, const BinGrid &g,
-               const SortBuffers &sb, hipStream_t s);
+               const SortBuffers &sb, hipStream_t s, bool octant_order = false);
 
 // Generic P2G gather for one component (ScalarField semantics).
 void p2g_gather(const float *aos, const float *val, int64_t n,

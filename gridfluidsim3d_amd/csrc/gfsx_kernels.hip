@@ -184,20 +184,26 @@ void exclusive_scan(const uint32_t *in, uint32_t *out, int64_t m,
 int64_t sort_blocks(int64_t n) { return (n + kBlock - 1) / kBlock; }
 
 // Bin every particle.  In-grid particles take a slot in their bin with an
-// integer atomic (the slot order is fixed up by k_bucket_sort); overflow
-// particles get their rank among the block's overflow particles from a
-// wave ballot, so that bin is filled in index order directly.
+// integer atomic; lanes of a wave that hold a run of equal bins (the input
+// is usually close to cell order) take one atomic per run and number
+// themselves within it in lane order.  The order inside a bin is then
+// fixed by k_bucket_sort.  Overflow particles get their rank among the
+// block's overflow particles from a wave ballot, so that bin is filled in
+// index order directly.  With an octant array, each particle also records
+// which half of its cell it occupies along each axis.
 template <bool AOS>
 __global__ __launch_bounds__(kBlock) void k_bin(const float *aos,
                                                const float *x, const float *y,
                                                const float *z, int64_t n,
                                                BinGrid g, uint32_t *count,
                                                uint32_t *key, uint32_t *slot,
-                                               uint32_t *block_out) {
+                                               uint32_t *block_out,
+                                               uint8_t *oct) {
     __shared__ uint32_t wcnt[kBlock / 64];
-    int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x;
-    bool valid = i < n;
+    const int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    const bool valid = i < n;
     uint32_t b = g.nbin;
+    uint8_t o = 0;
     if (valid) {
         float px, py, pz;
         if (AOS) {
@@ -210,13 +216,37 @@ __global__ __launch_bounds__(kBlock) void k_bin(const float *aos,
             pz = z[i];
         }
         b = bin_of(px, py, pz, g);
+        if (oct) {
+            const double gx = (double)(px - g.off[0]) * g.inv_dx;
+            const double gy = (double)(py - g.off[1]) * g.inv_dx;
+            const double gz = (double)(pz - g.off[2]) * g.inv_dx;
+            o = (uint8_t)(((gx - floor(gx)) >= 0.5) | (((gy - floor(gy)) >= 0.5) << 1) |
+                          (((gz - floor(gz)) >= 0.5) << 2));
+        }
     }
-    bool over = valid && b == g.nbin;
-    uint32_t sl = 0;
-    if (valid && !over) sl = atomicAdd(&count[b], 1u);
+    const bool over = valid && b == g.nbin;
+    const bool in = valid && !over;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    uint64_t mask = __ballot(over);
-    uint32_t lrank = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+
+    // runs of equal in-grid bins among consecutive lanes
+    const uint32_t kk = in ? b : 0xFFFFFFFFu;
+    const uint32_t kprev = __shfl_up(kk, 1, 64);
+    const uint32_t knext = __shfl_down(kk, 1, 64);
+    const bool rstart = in && (lane == 0 || kprev != kk);
+    const bool rend = in && (lane == 63 || knext != kk);
+    const uint64_t S = __ballot(rstart), E = __ballot(rend);
+    const uint64_t le = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+    const int s0 = in ? 63 - __clzll(S & le) : 0;
+    uint32_t base = 0;
+    if (rstart) {
+        const int e0 = __ffsll((unsigned long long)(E & ~((1ull << lane) - 1ull))) - 1;
+        base = atomicAdd(&count[b], (uint32_t)(e0 - lane + 1));
+    }
+    base = __shfl(base, s0, 64);
+    uint32_t sl = base + (uint32_t)(lane - s0);
+
+    const uint64_t mask = __ballot(over);
+    const uint32_t lrank = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
     if (lane == 0) wcnt[wid] = (uint32_t)__popcll(mask);
     __syncthreads();
     uint32_t woff = 0, tot = 0;
@@ -229,6 +259,7 @@ __global__ __launch_bounds__(kBlock) void k_bin(const float *aos,
     if (valid) {
         key[i] = b;
         slot[i] = sl;
+        if (oct) oct[i] = o;
     }
     if (threadIdx.x == 0) block_out[blockIdx.x] = tot;
 }
@@ -238,39 +269,50 @@ __global__ void k_copy_one(uint32_t *dst, const uint32_t *src) { *dst = *src; }
 __global__ __launch_bounds__(kBlock) void k_scatter(
     const uint32_t *key, const uint32_t *slot, int64_t n,
     const uint32_t *start, const uint32_t *block_out_scan, uint32_t nbin,
-    uint32_t *perm) {
+    uint32_t *perm, const uint8_t *oct, uint8_t *poct) {
     int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x;
     if (i >= n) return;
     uint32_t b = key[i], s = slot[i];
     uint32_t dst = (b == nbin) ? start[nbin] + block_out_scan[blockIdx.x] + s
                                : start[b] + s;
     perm[dst] = (uint32_t)i;
+    if (poct) poct[dst] = oct[i];
 }
 
-// Put each in-grid bin's indices into ascending (= input) order.  Bins hold
-// a handful of particles, so a per-thread insertion sort is enough; a bin
-// with more than 128 entries uses a Shell sort (gaps 1, 4, 13, 40, ...).
+// Put each in-grid bin's entries into ascending order of key(q): the index
+// itself, or (octant, index) when poct is given (the octants then move
+// with the indices).  Bins hold a handful of particles, so a per-thread
+// insertion sort is enough; a bin with more than 128 entries uses a Shell
+// sort (gaps 1, 4, 13, 40, ...).
 __global__ __launch_bounds__(kBlock) void k_bucket_sort(const uint32_t *start,
                                                        uint32_t nbin,
-                                                       uint32_t *perm) {
-    uint32_t b = blockIdx.x * kBlock + threadIdx.x;
+                                                       uint32_t *perm,
+                                                       uint8_t *poct) {
+    const uint32_t b = blockIdx.x * kBlock + threadIdx.x;
     if (b >= nbin) return;
-    uint32_t s = start[b], e = start[b + 1];
-    uint32_t len = e - s;
+    const uint32_t s = start[b], e = start[b + 1];
+    const uint32_t len = e - s;
     if (len <= 1) return;
     uint32_t *a = perm + s;
+    uint8_t *o = poct ? poct + s : nullptr;
     uint32_t gap = 1;
     if (len > 128)
         while (gap < len / 3) gap = 3 * gap + 1;
     for (; gap >= 1; gap /= 3) {
         for (uint32_t q = gap; q < len; q++) {
-            uint32_t v = a[q];
+            const uint32_t v = a[q];
+            const uint8_t vo = o ? o[q] : 0;
+            const uint64_t kv = ((uint64_t)vo << 32) | v;
             uint32_t r = q;
-            while (r >= gap && a[r - gap] > v) {
+            while (r >= gap) {
+                const uint64_t kr = ((uint64_t)(o ? o[r - gap] : 0) << 32) | a[r - gap];
+                if (kr <= kv) break;
                 a[r] = a[r - gap];
+                if (o) o[r] = o[r - gap];
                 r -= gap;
             }
             a[r] = v;
+            if (o) o[r] = vo;
         }
         if (gap == 1) break;
     }
@@ -278,19 +320,21 @@ __global__ __launch_bounds__(kBlock) void k_bucket_sort(const uint32_t *start,
 
 void cell_sort(const float *aos, const float *x, const float *y,
                const float *z, int64_t n, const BinGrid &g,
-               const SortBuffers &sb, hipStream_t s) {
+               const SortBuffers &sb, hipStream_t s, bool octant_order) {
     fill_u32(sb.count, 0u, (int64_t)g.nbin + 1, s);
     if (n <= 0) {
         fill_u32(sb.start, 0u, (int64_t)g.nbin + 2, s);
         return;
     }
+    uint8_t *oct = octant_order ? sb.oct : nullptr;
+    uint8_t *poct = octant_order ? sb.poct : nullptr;
     unsigned nb = (unsigned)sort_blocks(n);
     if (aos)
         hipLaunchKernelGGL(k_bin<true>, dim3(nb), dim3(kBlock), 0, s, aos, x,
-                           y, z, n, g, sb.count, sb.key, sb.slot, sb.block_out);
+                           y, z, n, g, sb.count, sb.key, sb.slot, sb.block_out, oct);
     else
         hipLaunchKernelGGL(k_bin<false>, dim3(nb), dim3(kBlock), 0, s, aos, x,
-                           y, z, n, g, sb.count, sb.key, sb.slot, sb.block_out);
+                           y, z, n, g, sb.count, sb.key, sb.slot, sb.block_out, oct);
     GFSX_LAUNCH_CHECK();
     exclusive_scan(sb.block_out, sb.block_out_scan, nb, sb.scan_scratch, s);
     hipLaunchKernelGGL(k_copy_one, dim3(1), dim3(1), 0, s, sb.count + g.nbin,
@@ -300,11 +344,12 @@ void cell_sort(const float *aos, const float *x, const float *y,
     hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(kBlock), 0, s,
                        (const uint32_t *)sb.key, (const uint32_t *)sb.slot, n,
                        (const uint32_t *)sb.start,
-                       (const uint32_t *)sb.block_out_scan, g.nbin, sb.perm);
+                       (const uint32_t *)sb.block_out_scan, g.nbin, sb.perm,
+                       (const uint8_t *)oct, poct);
     GFSX_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_bucket_sort, dim3(grid_for(g.nbin, kBlock)),
                        dim3(kBlock), 0, s, (const uint32_t *)sb.start, g.nbin,
-                       sb.perm);
+                       sb.perm, poct);
     GFSX_LAUNCH_CHECK();
 }
 
@@ -995,7 +1040,6 @@ __global__ __launch_bounds__(brick::NT) void k_flip_p2g_brick(
     __shared__ float s_x[CAP], s_y[CAP], s_z[CAP];
     __shared__ float s_v[3][CAP];
     __shared__ uint16_t s_hs[NH + 1];
-    __shared__ uint8_t s_oct[CAP];
     __shared__ uint32_t s_rowq[NROW], s_rows0[NROW + 1];
     __shared__ uint32_t s_wsum[NT / 64];
     // half-cell counts (32-bit, for LDS atomics) borrow the x array, which
@@ -1078,7 +1122,7 @@ __global__ __launch_bounds__(brick::NT) void k_flip_p2g_brick(
     // Every slot's loads are issued before any is used (slots past N load
     // particle 0 and are masked), so the global latency is paid once per
     // batch instead of once per slot.
-    uint32_t rp[PER], rh[PER], rfirst[PER];
+    uint32_t rp[PER], rh[PER], rfirst[PER], ro[PER];
     int rrow[PER];
 #pragma unroll
     for (int m = 0; m < PER; m++) {
@@ -1108,15 +1152,13 @@ __global__ __launch_bounds__(brick::NT) void k_flip_p2g_brick(
         const int lx = (int)fx - (i0 - 1), ly = (int)fy - (j0 - 1), lz = (int)fz - (k0 - 1);
         const int ox = (gxd - fx) >= 0.5, oy = (gyd - fy) >= 0.5, oz = (gzd - fz) >= 0.5;
         rh[m] = (uint32_t)((2 * lx + ox) + HX * ((2 * ly + oy) + HY * (2 * lz + oz)));
+        ro[m] = (uint32_t)(ox + 2 * oy + 4 * oz);
         const uint32_t s = (uint32_t)t + (uint32_t)m * NT;
         // the particle's bin (pad 1); slots past N use the brick's first row
         rbin[m] = s < N ? (uint32_t)(fx + 1.0) + row_stride * ((uint32_t)(fy + 1.0) +
                                                  (uint32_t)bg.b[1] * (uint32_t)(fz + 1.0))
                         : (uint32_t)i0;
-        if (s < N) {
-            s_oct[s] = (uint8_t)(ox + 2 * oy + 4 * oz);
-            atomicAdd(&s_cnt[rh[m]], 1u);
-        }
+        if (s < N) atomicAdd(&s_cnt[rh[m]], 1u);
     }
 #pragma unroll
     for (int m = 0; m < PER; m++) {
@@ -1154,9 +1196,10 @@ __global__ __launch_bounds__(brick::NT) void k_flip_p2g_brick(
     }
     __syncthreads();
 
-    // stage 3: place every particle at its half-cell's start plus its rank
-    // among the earlier particles of its cell in the same half-cell (a
-    // half-cell lies in one cell, whose particles are contiguous)
+    // stage 3: place every particle.  Inside its cell the particles are
+    // ordered by (octant, index), so the half-cell (cell, o) is the run that
+    // starts after the cell's runs of octants below o; the particle's rank
+    // in its half-cell is its distance from that start.
     float rvx[PER], rvy[PER], rvz[PER];
 #pragma unroll
     for (int m = 0; m < PER; m++) {
@@ -1170,10 +1213,16 @@ __global__ __launch_bounds__(brick::NT) void k_flip_p2g_brick(
     for (int m = 0; m < PER; m++) {
         const uint32_t s = (uint32_t)t + (uint32_t)m * NT;
         if (s < N) {
-            const uint8_t oct = s_oct[s];
-            uint32_t rank = 0;
-            for (uint32_t q = rfirst[m]; q < s; q++) rank += s_oct[q] == oct;
-            const uint32_t d = (uint32_t)s_hs[rh[m]] + rank;
+            const uint32_t o = ro[m];
+            const uint32_t h0 = rh[m] - ((o & 1u) + HX * (((o >> 1) & 1u) + HY * (o >> 2)));
+            uint32_t run = rfirst[m];
+#pragma unroll
+            for (uint32_t oo = 0; oo < 7; oo++) {
+                const uint32_t ho = h0 + (oo & 1u) + HX * (((oo >> 1) & 1u) + HY * (oo >> 2));
+                const uint32_t c = (uint32_t)s_hs[ho + 1] - (uint32_t)s_hs[ho];
+                run += oo < o ? c : 0u;
+            }
+            const uint32_t d = (uint32_t)s_hs[rh[m]] + (s - run);
             s_x[d] = rx[m];
             s_y[d] = ry[m];
             s_z[d] = rz[m];
@@ -1257,7 +1306,8 @@ void flip_step(const StepArgs &s, const SortBuffers &sb, hipStream_t st,
     bg.off[0] = bg.off[1] = bg.off[2] = 0.0f;
     bg.inv_dx = 1.0 / s.dx;
     bg.nbin = (uint32_t)bg.b[0] * (uint32_t)bg.b[1] * (uint32_t)bg.b[2];
-    cell_sort(nullptr, s.xin[0], s.xin[1], s.xin[2], s.n, bg, sb, st);
+    // cells ordered inside by (octant, index): each half-cell is a run
+    cell_sort(nullptr, s.xin[0], s.xin[1], s.xin[2], s.n, bg, sb, st, true);
     if (ev) GFSX_HIP(hipEventRecord(ev[1], st));
 
     StepP2G a;

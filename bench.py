@@ -13,6 +13,8 @@ whole-step roofline, and a CPU baseline: the reference's own CPU code
 (oracle/_ref, single thread) run on a bounded x-slab of the same scene.
 """
 import argparse
+import glob
+import hashlib
 import json
 import os
 import sys
@@ -43,6 +45,46 @@ def algorithmic_bytes(P, F, C):
     step: the whole step, 48P + 8F."""
     return dict(sort=16 * P + 8 * C, p2g=24 * P + 4 * F, g2p=48 * P + 4 * F,
                 step=48 * P + 8 * F)
+
+
+def kernel_source_hash():
+    """sha1 over the HIP sources the library is built from."""
+    h = hashlib.sha1()
+    for f in sorted(glob.glob(os.path.join(ROOT, "gridfluidsim3d_amd", "csrc", "*"))):
+        with open(f, "rb") as fh:
+            h.update(os.path.basename(f).encode())
+            h.update(fh.read())
+    return h.hexdigest()
+
+
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` from profiles/traffic.json, which
+    holds rocprofv3 FETCH_SIZE + WRITE_SIZE per dispatch scaled by the same
+    counters' reading for a plain copy of known size.  Only used when it was
+    measured on the current kernel sources."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if rec.get("src_sha1") != kernel_source_hash():
+        return None
+    ent = rec.get("workloads", {}).get(workload, {}).get(kernel)
+    return None if ent is None else ent.get("hbm_bytes")
+
+
+def copy_calibration(nbytes, reps=5):
+    """A plain device-to-device copy of nbytes, for scaling the FETCH_SIZE /
+    WRITE_SIZE counters (each copy reads and writes nbytes once)."""
+    import torch
+    src = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": src.numel() * 4, "reps": reps}), flush=True)
 
 
 def cpu_baseline(scene, stencil, target_s=15.0):
@@ -98,6 +140,8 @@ def main():
     ap.add_argument("--stencil", default="trilinear", choices=["trilinear", "tricubic"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="only run a plain copy of this many bytes (counter calibration)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -105,6 +149,10 @@ def main():
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(requested --gpus %d, WORLD_SIZE %d)" % (args.gpus, world), file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_bytes:
+        copy_calibration(args.copy_bytes)
+        return
 
     import torch
 
@@ -152,10 +200,12 @@ def main():
     alg = algorithmic_bytes(P, F, C)
     dom = max(("sort", "p2g", "g2p"), key=lambda k: stage_avg[k])
     achieved = alg[dom] / (stage_avg[dom] / 1e3) / 1e9
-    roofline = dict(bound="hbm", kernel=dict(sort="cell sort (5 kernels)", p2g="k_flip_p2g",
+    roofline = dict(bound="hbm", kernel=dict(sort="cell sort (7 kernels)", p2g="k_flip_p2g_brick",
                                              g2p="k_flip_g2p")[dom],
                     achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                    frac=round(achieved / HBM_PEAK_GBS, 4),
+                    traffic=measured_traffic(args.workload, dict(sort="sort", p2g="k_flip_p2g_brick",
+                                                                 g2p="k_flip_g2p")[dom]),
                     algorithmic_bytes=alg[dom], avg_ms=round(stage_avg[dom], 4))
     step_gbs = alg["step"] / (ms_per_step / 1e3) / 1e9
     step_roofline = dict(achieved=round(step_gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",

@@ -32,7 +32,8 @@ def build(verbose=False):
     """Compile every HIP source of the library into lib/libgfsx.so."""
     os.makedirs(LIBDIR, exist_ok=True)
     tmp = LIB + ".tmp"
-    cmd = ([HIPCC] + HIP_FLAGS + ["-shared", "-I", os.path.join(ROOT, "include"),
+    cmd = ([HIPCC] + HIP_FLAGS + ["-shared", "-Wl,-soname," + os.path.basename(LIB),
+                                  "-I", os.path.join(ROOT, "include"),
                                   "-I", CSRC, "-o", tmp] +
            [os.path.join(CSRC, s) for s in SOURCES])
     if verbose:

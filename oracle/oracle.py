@@ -34,12 +34,16 @@ _int = ctypes.c_int
 _dbl = ctypes.c_double
 
 
+DROPIN_SO = os.path.join(HERE, "_ref", "libgfsx_dropin_check.so")
+
+
 def build(with_reference=None):
-    """Compile liboracle.so, and _ref/libgfsx_ref.so when the reference tree
-    is present (or when ``with_reference`` is True)."""
+    """Compile liboracle.so, and -- when the reference tree is present (or
+    ``with_reference`` is True) -- _ref/libgfsx_ref.so and the drop-in
+    checker _ref/libgfsx_dropin_check.so (which links the built libgfsx.so)."""
     if with_reference is None:
         with_reference = os.path.isdir(REF_SRC)
-    targets = ["all"] + (["ref"] if with_reference else [])
+    targets = ["all"] + (["ref", "dropin"] if with_reference else [])
     subprocess.run(["make", "-s", "-C", HERE] + targets, check=True)
 
 

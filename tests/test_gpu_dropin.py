@@ -1,0 +1,76 @@
+"""The drop-in CLScalarField / ParticleAdvector (gridfluidsim3d_amd/dropin),
+compiled against the reference's own headers and driven with the reference's
+own ScalarField and MACVelocityField objects (oracle/dropin_harness.cpp),
+against the reference CPU answer for the same call."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+from conftest import NORM_REL_TOL, norm_rel
+
+pytestmark = pytest.mark.gpu
+
+_f32p = ctypes.POINTER(ctypes.c_float)
+
+
+def _lib():
+    import gridfluidsim3d_amd as g
+    from oracle import oracle
+    g.load_library()   # the HIP runtime and libgfsx.so, loaded the usual way
+    if not os.path.exists(oracle.DROPIN_SO):
+        pytest.skip("drop-in checker not built (needs the reference tree at build time)")
+    lib = ctypes.CDLL(oracle.DROPIN_SO)
+    lib.dropin_p2g.restype = ctypes.c_int
+    lib.dropin_p2g.argtypes = [_f32p, _f32p, ctypes.c_int64, ctypes.c_double, _f32p,
+                               ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                               _f32p, _f32p, _f32p, _f32p]
+    lib.dropin_g2p_rk4.restype = ctypes.c_int
+    lib.dropin_g2p_rk4.argtypes = [_f32p, ctypes.c_int64, _f32p, _f32p, _f32p, ctypes.c_int,
+                                   ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                   ctypes.c_double, ctypes.c_int, _f32p, _f32p, _f32p, _f32p]
+    return lib
+
+
+def _p(a):
+    return a.ctypes.data_as(_f32p)
+
+
+@pytest.mark.parametrize("radius_cells,offset", [(1.0, (0.0, 0.05, 0.05)),
+                                                 (1.6, (0.02, -0.01, 0.0))])
+def test_clscalarfield_add_point_values(radius_cells, offset):
+    lib = _lib()
+    rng = np.random.default_rng(17)
+    dims = (21, 17, 13)
+    dx = 0.1
+    n = 20000
+    pos = rng.uniform(-0.1, np.array(dims) * dx + 0.1, size=(n, 3)).astype(np.float32)
+    val = rng.normal(size=n).astype(np.float32)
+    off = np.array(offset, np.float32)
+    shape = dims[::-1]
+    Sg, Wg, Sc, Wc = (np.zeros(shape, np.float32) for _ in range(4))
+    ok = lib.dropin_p2g(_p(pos), _p(val), n, radius_cells * dx, _p(off), dx, *dims,
+                        _p(Sg), _p(Wg), _p(Sc), _p(Wc))
+    assert ok == 1
+    assert Wc.max() > 0
+    assert norm_rel(Wg, Wc) <= NORM_REL_TOL
+    assert norm_rel(Sg, Sc) <= NORM_REL_TOL
+
+
+@pytest.mark.parametrize("stencil", [0, 1])
+def test_particleadvector_interpolate_and_rk4(stencil):
+    lib = _lib()
+    rng = np.random.default_rng(23)
+    dims = (19, 14, 16)
+    dx = 0.05
+    shapes = [(16, 14, 20), (16, 15, 19), (17, 14, 19)]
+    u, v, w = (rng.normal(size=s).astype(np.float32) for s in shapes)
+    n = 15000
+    pos = rng.uniform(-0.02, np.array(dims) * dx + 0.02, size=(n, 3)).astype(np.float32)
+    g_gpu, g_cpu, r_gpu, r_cpu = (np.zeros((n, 3), np.float32) for _ in range(4))
+    ok = lib.dropin_g2p_rk4(_p(pos), n, _p(u), _p(v), _p(w), *dims, dx, 0.01, stencil,
+                            _p(g_gpu), _p(g_cpu), _p(r_gpu), _p(r_cpu))
+    assert ok == 1
+    assert norm_rel(g_gpu, g_cpu) <= NORM_REL_TOL
+    assert norm_rel(r_gpu - pos, r_cpu - pos) <= NORM_REL_TOL

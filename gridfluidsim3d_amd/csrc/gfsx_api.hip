@@ -53,6 +53,7 @@ struct gfsx_ctx {
     void *buf[NSLOT] = {};
     size_t bytes[NSLOT] = {};
     int p2g_variant = 0;
+    int g2p_variant = 0;
     bool timing = false;
     std::vector<std::array<hipEvent_t, 4>> events;
     size_t events_used = 0;
@@ -495,6 +496,7 @@ void gfsx_flip_step(gfsx_ctx *ctx, const gfsx_scene *sc,
         a.stencil = pr->stencil;
         a.keep_order = pr->order == GFSX_ORDER_KEEP;
         a.p2g_variant = ctx->p2g_variant;
+        a.g2p_variant = ctx->g2p_variant;
         gfsx::SortBuffers sb = ctx->sort_buffers(sc->n, (uint32_t)nbin);
         gfsx::flip_step(a, sb, ctx->stream, ctx->timing ? ctx->next_events() : nullptr);
     });
@@ -506,6 +508,9 @@ void gfsx_set_option(gfsx_ctx *ctx, const char *name, int value, int *err) {
         if (std::strcmp(name, "p2g_variant") == 0) {
             require(value >= 0 && value <= 3, "gfsx_set_option: p2g_variant must be 0..3");
             ctx->p2g_variant = value;
+        } else if (std::strcmp(name, "g2p_variant") == 0) {
+            require(value >= 0 && value <= 1, "gfsx_set_option: g2p_variant must be 0..1");
+            ctx->g2p_variant = value;
         } else {
             throw Error(std::string("gfsx_set_option: unknown option ") + name);
         }
@@ -517,6 +522,7 @@ int gfsx_get_option(gfsx_ctx *ctx, const char *name, int *err) {
     guarded(ctx, err, [&] {
         require(name != nullptr, "gfsx_get_option: null name");
         if (std::strcmp(name, "p2g_variant") == 0) v = ctx->p2g_variant;
+        else if (std::strcmp(name, "g2p_variant") == 0) v = ctx->g2p_variant;
         else throw Error(std::string("gfsx_get_option: unknown option ") + name);
     });
     return v;

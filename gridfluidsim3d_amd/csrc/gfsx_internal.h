@@ -148,6 +148,7 @@ struct StepArgs {
     int stencil;
     int keep_order;
     int p2g_variant;   // 0: brick-staged gather, 1: direct global gather
+    int g2p_variant;   // 0: brick-staged LDS boxes, 1: direct global sampling
 };
 
 // The fused plain step.  Needs SortBuffers sized for n particles and

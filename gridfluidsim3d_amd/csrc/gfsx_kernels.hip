@@ -1295,6 +1295,291 @@ __global__ __launch_bounds__(kBlock) void k_flip_g2p(
     if (a.id_out) a.id_out[o] = a.id_in ? a.id_in[p] : p;
 }
 
+// ---------------------------------------------------------------------------
+// Brick-staged G2P: one work-group per brick of GB cells per axis.  The
+// work-group copies the box of the new field around its brick (the stencil
+// footprint of every position inside the brick plus a halo of GH cells for
+// the RK stage positions) and the box of the pre-force snapshot (no halo:
+// it is only sampled at the start position) into LDS, with 0 wherever the
+// box leaves the arrays -- the value the reference reads there
+// (macvelocityfield.cpp:372-378,470-478).  A sample whose stencil leaves
+// the box reads global memory instead, so every position is handled and
+// the result is bit-identical to k_flip_g2p.
+// ---------------------------------------------------------------------------
+namespace gbrick {
+constexpr int GB = 8;        // cells per brick edge
+constexpr int GH = 1;        // halo cells for RK stage samples
+constexpr int NT = 256;      // threads per work-group
+constexpr int MAXROW = (GB + 2) * (GB + 2);   // (y,z) bin rows, pads included
+constexpr int NEXTRA = 64;   // work-groups for particles outside the bin grid
+template <int ST> struct Box {
+    static constexpr int S = ST;                        // stencil reach below the cell: 0 or 1
+    static constexpr int E = GB + 2 + 2 * GH + 2 * S;   // new-field box edge
+    static constexpr int LO = 1 + GH + S;               // new-field box origin: C0 - LO
+    static constexpr int EO = GB + 2 + 2 * S;           // snapshot box edge
+    static constexpr int LOO = 1 + S;                   // snapshot box origin: C0 - LOO
+};
+static_assert(MAXROW <= 2 * 64, "two waves scan the rows");
+}  // namespace gbrick
+
+// Box sample of one component: (i,j,k) is the stencil's base cell relative
+// to the box origin; the stencil (i-S..i+1+S per axis) lies inside the box.
+template <int ST, int E>
+__device__ __forceinline__ float box_comp(const float *__restrict__ box, int i,
+                                          int j, int k, float tx, float ty,
+                                          float tz) {
+    if (ST == 0) {
+        const float *b = box + (i + E * (j + E * k));
+        const float v000 = b[0], v100 = b[1], v010 = b[E], v110 = b[E + 1];
+        const float v001 = b[E * E], v101 = b[E * E + 1];
+        const float v011 = b[E * E + E], v111 = b[E * E + E + 1];
+        const float c00 = fmaf(tx, v100 - v000, v000);
+        const float c10 = fmaf(tx, v110 - v010, v010);
+        const float c01 = fmaf(tx, v101 - v001, v001);
+        const float c11 = fmaf(tx, v111 - v011, v011);
+        const float c0 = fmaf(ty, c10 - c00, c00);
+        const float c1 = fmaf(ty, c11 - c01, c01);
+        return fmaf(tz, c1 - c0, c0);
+    }
+    const float *b = box + ((i - 1) + E * ((j - 1) + E * (k - 1)));
+    float zs[4];
+#pragma unroll
+    for (int mk = 0; mk < 4; mk++) {
+        float ys[4];
+#pragma unroll
+        for (int mj = 0; mj < 4; mj++) {
+            const float *r = b + E * (mj + E * mk);
+            ys[mj] = cubic(r[0], r[1], r[2], r[3], tx);
+        }
+        zs[mk] = cubic(ys[0], ys[1], ys[2], ys[3], ty);
+    }
+    return cubic(zs[0], zs[1], zs[2], zs[3], tz);
+}
+
+// True when the stencils of all three components at cell coordinates
+// (cx,cy,cz) lie in the box of edge E whose origin is at cell lo[].  Per
+// axis the components use base cells ia (own axis) and ib <= ia (others),
+// so the lowest cell read is ib - S and the highest ia + 1 + S.
+template <int ST, int E>
+__device__ __forceinline__ bool in_box(const AxisCoords &cx, const AxisCoords &cy,
+                                       const AxisCoords &cz, int lx, int ly, int lz) {
+    constexpr int S = ST;
+    return (unsigned)(cx.ib - S - lx) < (unsigned)E && (unsigned)(cx.ia + 1 + S - lx) < (unsigned)E &&
+           (unsigned)(cy.ib - S - ly) < (unsigned)E && (unsigned)(cy.ia + 1 + S - ly) < (unsigned)E &&
+           (unsigned)(cz.ib - S - lz) < (unsigned)E && (unsigned)(cz.ia + 1 + S - lz) < (unsigned)E;
+}
+
+// sample_mac with the LDS boxes as the first choice.  cur[c] / old[c] are
+// the staged boxes (edges E, EO; origins l*, lo*); g and o* are the global
+// arrays used when a stencil leaves its box.
+template <int ST, bool OLD>
+__device__ __forceinline__ void brick_sample(const MacGrid &g, const float *o0,
+                                             const float *o1, const float *o2,
+                                             const float *__restrict__ cur0,
+                                             const float *__restrict__ cur1,
+                                             const float *__restrict__ cur2,
+                                             const float *__restrict__ old0,
+                                             const float *__restrict__ old1,
+                                             const float *__restrict__ old2,
+                                             int lx, int ly, int lz,
+                                             int lox, int loy, int loz,
+                                             float x, float y, float z,
+                                             float out[3], float out_old[3]) {
+    using B = gbrick::Box<ST>;
+    const double dxp = x, dyp = y, dzp = z;
+    if (!(dxp >= 0.0 && dyp >= 0.0 && dzp >= 0.0 && dxp < g.lim[0] &&
+          dyp < g.lim[1] && dzp < g.lim[2])) {
+        out[0] = out[1] = out[2] = 0.0f;
+        if (OLD) out_old[0] = out_old[1] = out_old[2] = 0.0f;
+        return;
+    }
+    const AxisCoords cx = axis_coords(dxp, g.inv_dx);
+    const AxisCoords cy = axis_coords(dyp, g.inv_dx);
+    const AxisCoords cz = axis_coords(dzp, g.inv_dx);
+    const int ni = g.ni, nj = g.nj, nk = g.nk;
+    if (in_box<ST, B::E>(cx, cy, cz, lx, ly, lz)) {
+        out[0] = box_comp<ST, B::E>(cur0, cx.ia - lx, cy.ib - ly, cz.ib - lz, cx.ta, cy.tb, cz.tb);
+        out[1] = box_comp<ST, B::E>(cur1, cx.ib - lx, cy.ia - ly, cz.ib - lz, cx.tb, cy.ta, cz.tb);
+        out[2] = box_comp<ST, B::E>(cur2, cx.ib - lx, cy.ib - ly, cz.ia - lz, cx.tb, cy.tb, cz.ta);
+    } else {
+        out[0] = sample_comp<ST>(g.f[0], ni + 1, nj, nk, cx.ia, cy.ib, cz.ib, cx.ta, cy.tb, cz.tb);
+        out[1] = sample_comp<ST>(g.f[1], ni, nj + 1, nk, cx.ib, cy.ia, cz.ib, cx.tb, cy.ta, cz.tb);
+        out[2] = sample_comp<ST>(g.f[2], ni, nj, nk + 1, cx.ib, cy.ib, cz.ia, cx.tb, cy.tb, cz.ta);
+    }
+    if (OLD) {
+        if (in_box<ST, B::EO>(cx, cy, cz, lox, loy, loz)) {
+            out_old[0] = box_comp<ST, B::EO>(old0, cx.ia - lox, cy.ib - loy, cz.ib - loz, cx.ta, cy.tb, cz.tb);
+            out_old[1] = box_comp<ST, B::EO>(old1, cx.ib - lox, cy.ia - loy, cz.ib - loz, cx.tb, cy.ta, cz.tb);
+            out_old[2] = box_comp<ST, B::EO>(old2, cx.ib - lox, cy.ib - loy, cz.ia - loz, cx.tb, cy.tb, cz.ta);
+        } else {
+            out_old[0] = sample_comp<ST>(o0, ni + 1, nj, nk, cx.ia, cy.ib, cz.ib, cx.ta, cy.tb, cz.tb);
+            out_old[1] = sample_comp<ST>(o1, ni, nj + 1, nk, cx.ib, cy.ia, cz.ib, cx.tb, cy.ta, cz.tb);
+            out_old[2] = sample_comp<ST>(o2, ni, nj, nk + 1, cx.ib, cy.ib, cz.ia, cx.tb, cy.tb, cz.ta);
+        }
+    }
+}
+
+// Copy the box of edge E with origin (ox,oy,oz) of a component array with
+// dims (w,h,d) into LDS; cells outside the array are 0.
+template <int E>
+__device__ __forceinline__ void stage_box(float *__restrict__ box,
+                                          const float *__restrict__ f, int w,
+                                          int h, int d, int ox, int oy, int oz) {
+    for (int e = threadIdx.x; e < E * E * E; e += gbrick::NT) {
+        const int ii = e % E, jj = (e / E) % E, kk = e / (E * E);
+        const int gi = ox + ii, gj = oy + jj, gk = oz + kk;
+        float v = 0.0f;
+        if ((unsigned)gi < (unsigned)w && (unsigned)gj < (unsigned)h && (unsigned)gk < (unsigned)d)
+            v = f[(int64_t)gi + (int64_t)w * ((int64_t)gj + (int64_t)h * gk)];
+        box[e] = v;
+    }
+}
+
+// PIC/FLIP blend and RK4 for one particle, given its new-field and
+// snapshot velocities at the start position.
+__device__ __forceinline__ void flip_blend(const StepG2P &a, const float v0[3],
+                                           const float k1[3], const float vo_in[3],
+                                           float vnew[3]) {
+    float vn[3] = {k1[0], k1[1], k1[2]};
+    float vo[3] = {vo_in[0], vo_in[1], vo_in[2]};
+    validate3(vn);
+    validate3(vo);
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const float vflip = (v0[c] + vn[c]) - vo[c];
+        vnew[c] = a.r_pic * vn[c] + a.r_flip * vflip;
+    }
+}
+
+template <int ST>
+__global__ __launch_bounds__(gbrick::NT) void k_flip_g2p_brick(
+    StepG2P a, MacGrid g, BinGrid bg, int nbx, int nby, int nbz, int per_xcd,
+    const uint32_t *__restrict__ start, const uint32_t *__restrict__ perm) {
+    using namespace gbrick;
+    using B = Box<ST>;
+    __shared__ float s_cur[3][B::E * B::E * B::E];
+    __shared__ float s_old[3][B::EO * B::EO * B::EO];
+    __shared__ uint32_t s_rowq[MAXROW], s_rows0[MAXROW + 1];
+    __shared__ uint32_t s_wsum;
+    const int t = threadIdx.x;
+    const int nslot = 8 * per_xcd;
+
+    if ((int)blockIdx.x >= nslot) {
+        // particles beyond the padded bin grid: outside the domain, so every
+        // sample is 0 and the global path returns at once
+        const uint32_t q0 = start[bg.nbin], q1 = start[bg.nbin + 1];
+        for (uint32_t q = q0 + (uint32_t)((blockIdx.x - nslot) * NT + t); q < q1;
+             q += (uint32_t)(NEXTRA * NT)) {
+            const uint32_t p = perm[q];
+            const float x0[3] = {a.xin[0][p], a.xin[1][p], a.xin[2][p]};
+            const float v0[3] = {a.vin[0][p], a.vin[1][p], a.vin[2][p]};
+            float k1[3], vo[3], vnew[3], x1[3];
+            sample_mac<ST, true>(g, a.old[0], a.old[1], a.old[2], x0[0], x0[1], x0[2], k1, vo);
+            flip_blend(a, v0, k1, vo, vnew);
+            rk_step<ST>(g, 4, a.rk, x0, k1, x1);
+            const int64_t o = a.keep_order ? (int64_t)p : (int64_t)q;
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                a.xout[c][o] = x1[c];
+                a.vout[c][o] = vnew[c];
+            }
+            if (a.id_out) a.id_out[o] = a.id_in ? a.id_in[p] : p;
+        }
+        return;
+    }
+
+    // consecutive bricks share box halos: keep them on one XCD (work-groups
+    // are dealt round-robin over the 8 XCDs)
+    const int brick_id = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
+    if (brick_id >= nbx * nby * nbz) return;
+    const int bxi = brick_id % nbx, byi = (brick_id / nbx) % nby, bzi = brick_id / (nbx * nby);
+    const int c0x = bxi * GB, c0y = byi * GB, c0z = bzi * GB;
+    // bins of the brick (bin = cell + 1); the first and last bricks of an
+    // axis also take the pad bin beyond the domain
+    const int bx_lo = bxi == 0 ? 0 : c0x + 1, bx_hi = bxi == nbx - 1 ? bg.b[0] : c0x + GB + 1;
+    const int by_lo = byi == 0 ? 0 : c0y + 1, by_hi = byi == nby - 1 ? bg.b[1] : c0y + GB + 1;
+    const int bz_lo = bzi == 0 ? 0 : c0z + 1, bz_hi = bzi == nbz - 1 ? bg.b[2] : c0z + GB + 1;
+    const int ny = by_hi - by_lo;
+    const int nrow = ny * (bz_hi - bz_lo);
+
+    // (y,z) rows of bins: each is one contiguous range of perm
+    if (t < 128) {
+        uint32_t q0 = 0, len = 0;
+        if (t < nrow) {
+            const uint32_t by = (uint32_t)(by_lo + t % ny), bz = (uint32_t)(bz_lo + t / ny);
+            const uint32_t row = (uint32_t)bg.b[0] * (by + (uint32_t)bg.b[1] * bz);
+            q0 = start[row + (uint32_t)bx_lo];
+            len = start[row + (uint32_t)bx_hi] - q0;
+            s_rowq[t] = q0;
+        }
+        const uint32_t inc = wave_inclusive_sum(len);
+        if (t == 63) s_wsum = inc;
+        if (t < nrow) s_rows0[t + 1] = inc;   // the second wave adds the first's total below
+        if (t == 0) s_rows0[0] = 0u;
+    }
+    __syncthreads();
+    if (t >= 64 && t < nrow) s_rows0[t + 1] += s_wsum;
+    __syncthreads();
+    const uint32_t N = s_rows0[nrow];
+    if (N == 0) return;
+
+    const int lx = c0x - B::LO, ly = c0y - B::LO, lz = c0z - B::LO;
+    const int lox = c0x - B::LOO, loy = c0y - B::LOO, loz = c0z - B::LOO;
+    stage_box<B::E>(s_cur[0], g.f[0], g.ni + 1, g.nj, g.nk, lx, ly, lz);
+    stage_box<B::E>(s_cur[1], g.f[1], g.ni, g.nj + 1, g.nk, lx, ly, lz);
+    stage_box<B::E>(s_cur[2], g.f[2], g.ni, g.nj, g.nk + 1, lx, ly, lz);
+    stage_box<B::EO>(s_old[0], a.old[0], g.ni + 1, g.nj, g.nk, lox, loy, loz);
+    stage_box<B::EO>(s_old[1], a.old[1], g.ni, g.nj + 1, g.nk, lox, loy, loz);
+    stage_box<B::EO>(s_old[2], a.old[2], g.ni, g.nj, g.nk + 1, lox, loy, loz);
+    __syncthreads();
+
+    for (uint32_t s = (uint32_t)t; s < N; s += NT) {
+        // row of slot s: the last r with rows0[r] <= s
+        int lo = 0, hi = nrow - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (s_rows0[mid] <= s) lo = mid;
+            else hi = mid - 1;
+        }
+        const uint32_t q = s_rowq[lo] + (s - s_rows0[lo]);
+        const uint32_t p = perm[q];
+        const float x0[3] = {a.xin[0][p], a.xin[1][p], a.xin[2][p]};
+        const float v0[3] = {a.vin[0][p], a.vin[1][p], a.vin[2][p]};
+        float k1[3], vo[3], vnew[3];
+        brick_sample<ST, true>(g, a.old[0], a.old[1], a.old[2], s_cur[0], s_cur[1], s_cur[2],
+                               s_old[0], s_old[1], s_old[2], lx, ly, lz, lox, loy, loz,
+                               x0[0], x0[1], x0[2], k1, vo);
+        flip_blend(a, v0, k1, vo, vnew);
+        // RK4 (particleadvector.cpp:1045-1054) through the staged field
+        float xq[3], k2[3], k3[3], k4[3], unused[3], x1[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) xq[c] = x0[c] + a.rk.half * k1[c];
+        brick_sample<ST, false>(g, nullptr, nullptr, nullptr, s_cur[0], s_cur[1], s_cur[2],
+                                nullptr, nullptr, nullptr, lx, ly, lz, 0, 0, 0,
+                                xq[0], xq[1], xq[2], k2, unused);
+#pragma unroll
+        for (int c = 0; c < 3; c++) xq[c] = x0[c] + a.rk.half * k2[c];
+        brick_sample<ST, false>(g, nullptr, nullptr, nullptr, s_cur[0], s_cur[1], s_cur[2],
+                                nullptr, nullptr, nullptr, lx, ly, lz, 0, 0, 0,
+                                xq[0], xq[1], xq[2], k3, unused);
+#pragma unroll
+        for (int c = 0; c < 3; c++) xq[c] = x0[c] + a.rk.full * k3[c];
+        brick_sample<ST, false>(g, nullptr, nullptr, nullptr, s_cur[0], s_cur[1], s_cur[2],
+                                nullptr, nullptr, nullptr, lx, ly, lz, 0, 0, 0,
+                                xq[0], xq[1], xq[2], k4, unused);
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+            x1[c] = x0[c] + a.rk.sixth * (((k1[c] + 2.0f * k2[c]) + 2.0f * k3[c]) + k4[c]);
+        const int64_t o = a.keep_order ? (int64_t)p : (int64_t)q;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            a.xout[c][o] = x1[c];
+            a.vout[c][o] = vnew[c];
+        }
+        if (a.id_out) a.id_out[o] = a.id_in ? a.id_in[p] : p;
+    }
+}
+
 void flip_step(const StepArgs &s, const SortBuffers &sb, hipStream_t st,
                hipEvent_t *ev) {
     if (ev) GFSX_HIP(hipEventRecord(ev[0], st));
@@ -1395,12 +1680,22 @@ void flip_step(const StepArgs &s, const SortBuffers &sb, hipStream_t st,
     b.r_flip = s.r_flip;
     b.keep_order = s.keep_order;
     b.n = s.n;
-    if (s.stencil == 0)
+    if (s.g2p_variant == 0) {
+        const int nbx = (s.ni + gbrick::GB - 1) / gbrick::GB;
+        const int nby = (s.nj + gbrick::GB - 1) / gbrick::GB;
+        const int nbz = (s.nk + gbrick::GB - 1) / gbrick::GB;
+        const int per_xcd = (int)(((int64_t)nbx * nby * nbz + 7) / 8);
+        const dim3 grid((unsigned)(8 * per_xcd + gbrick::NEXTRA)), block(gbrick::NT);
+        auto *kern = s.stencil == 0 ? k_flip_g2p_brick<0> : k_flip_g2p_brick<1>;
+        hipLaunchKernelGGL(kern, grid, block, 0, st, b, g, bg, nbx, nby, nbz, per_xcd,
+                           (const uint32_t *)sb.start, (const uint32_t *)sb.perm);
+    } else if (s.stencil == 0) {
         hipLaunchKernelGGL(k_flip_g2p<0>, dim3(grid_for(s.n, kBlock)),
                            dim3(kBlock), 0, st, b, g, (const uint32_t *)sb.perm);
-    else
+    } else {
         hipLaunchKernelGGL(k_flip_g2p<1>, dim3(grid_for(s.n, kBlock)),
                            dim3(kBlock), 0, st, b, g, (const uint32_t *)sb.perm);
+    }
     GFSX_LAUNCH_CHECK();
     if (ev) GFSX_HIP(hipEventRecord(ev[3], st));
 }

@@ -164,9 +164,15 @@ typedef struct gfsx_step_params {
 void gfsx_flip_step(gfsx_ctx *ctx, const gfsx_scene *scene,
                     const gfsx_step_params *params, int *err);
 
-/* Tunables of the fused step, by name (an unknown name fails):
- *   "p2g_variant": 0 = brick-staged LDS gather (default),
- *                  1 = direct gather from global memory. */
+/* Tunables of the fused step, by name (an unknown name or value fails).
+ * Every variant gives bit-identical results; they differ in speed only.
+ *   "p2g_variant": 0 = brick-staged LDS gather, candidates streamed in
+ *                      chunks (default),
+ *                  1 = direct gather from global memory,
+ *                  2 = brick-staged, candidates range by range,
+ *                  3 = brick-staged, candidates as one flattened stream.
+ *   "g2p_variant": 0 = grid boxes staged in LDS per brick (default),
+ *                  1 = direct sampling from global memory. */
 void gfsx_set_option(gfsx_ctx *ctx, const char *name, int value, int *err);
 int gfsx_get_option(gfsx_ctx *ctx, const char *name, int *err);
 

@@ -338,6 +338,48 @@ def test_step_p2g_variants(ctx, orc, variant):
     _check_step(res, exp, False)
 
 
+def _g2p_variant_scene(kind):
+    from gridfluidsim3d_amd import scenes
+    rng = np.random.default_rng(31)
+    if kind == "blobs":
+        sc = scenes.random_blobs(22, (27, 18, 13), 0.1, amplitude=2.0)
+        return sc.pos, sc.vel, sc.dims, 0.1, sc.dt
+    if kind == "tiny":
+        # one brick per axis, holding both pad layers
+        dims = (5, 6, 7)
+        pos = rng.uniform(-0.12, 0.8, size=(3000, 3)).astype(np.float32)
+        return pos, rng.normal(size=pos.shape).astype(np.float32), dims, 0.1, 0.01
+    # fast particles whose RK stages leave the staged boxes, particles in the
+    # pad layers and beyond them, and a NaN position
+    dims = (19, 17, 21)
+    W = np.array(dims) * 0.1
+    pos = rng.uniform(-0.4, W + 0.4, size=(20000, 3)).astype(np.float32)
+    pos[7] = np.nan
+    vel = (15.0 * rng.normal(size=pos.shape)).astype(np.float32)
+    return pos, vel, dims, 0.1, 0.01
+
+
+@pytest.mark.parametrize("kind", ["blobs", "tiny", "wild"])
+@pytest.mark.parametrize("stencil", [TRILINEAR, TRICUBIC])
+@pytest.mark.parametrize("keep", [False, True])
+def test_step_g2p_variants_bit_identical(ctx, kind, stencil, keep):
+    """The LDS-staged G2P (0) and the direct one (1) give identical bits,
+    including samples that fall back to global memory and particles outside
+    the bin grid."""
+    pos, vel, dims, dx, dt = _g2p_variant_scene(kind)
+    old = ctx.get_option("g2p_variant")
+    out = []
+    try:
+        for variant in (0, 1):
+            ctx.set_option("g2p_variant", variant)
+            out.append(_run_step(ctx, pos, vel, dims, dx, dt, (0.0, -9.8, 0.0), 0.05,
+                                 stencil, int(keep)))
+    finally:
+        ctx.set_option("g2p_variant", old)
+    for key in out[0]:
+        assert np.array_equal(out[0][key], out[1][key], equal_nan=True), key
+
+
 def test_step_dense_cells_use_fallback(ctx, orc):
     """Bricks holding more particles than fit in LDS take the global-memory
     gather; results still match the oracle."""
@@ -366,6 +408,9 @@ def test_step_is_deterministic(ctx):
 
 def test_options(ctx):
     assert ctx.get_option("p2g_variant") == 0
+    assert ctx.get_option("g2p_variant") == 0
+    with pytest.raises(RuntimeError, match="g2p_variant"):
+        ctx.set_option("g2p_variant", 2)
     with pytest.raises(RuntimeError, match="unknown option"):
         ctx.set_option("no_such_option", 1)
     with pytest.raises(RuntimeError, match="p2g_variant"):

@@ -33,8 +33,9 @@ void ok(int *err) {
 }
 
 enum Slot {
-    S_COUNT, S_START, S_KEY, S_SLOT, S_PERM, S_BOUT, S_BOUTS, S_SCAN, S_OCT, S_POCT,
-    S_OLD0, S_OLD1, S_OLD2,
+    S_COUNT, S_START, S_KEY, S_SLOT, S_PERM, S_PERM_TMP, S_HRANK, S_BIG, S_BIGN,
+    S_BOUT, S_BOUTS, S_SCAN, S_OCT, S_POCT,
+    S_OLD0, S_OLD1, S_OLD2, S_XS0, S_XS1, S_XS2, S_VS0, S_VS1, S_VS2, S_DENSE, S_DENSEN,
     S_D_POS, S_D_VAL, S_D_S, S_D_W, S_D_U, S_D_V, S_D_WW, S_D_OUT,
     NSLOT
 };
@@ -95,6 +96,11 @@ struct gfsx_ctx {
         sb.key = u32(S_KEY, n);
         sb.slot = u32(S_SLOT, n);
         sb.perm = u32(S_PERM, n);
+        sb.perm_tmp = u32(S_PERM_TMP, n);
+        sb.hrank = u32(S_HRANK, n);
+        // every big bin holds more than kSmallBin particles
+        sb.big_list = u32(S_BIG, std::min<int64_t>((int64_t)nbin, n / 16 + 1));
+        sb.big_count = u32(S_BIGN, 1);
         sb.block_out = u32(S_BOUT, nb);
         sb.block_out_scan = u32(S_BOUTS, nb + 1);
         int64_t words = std::max(gfsx::scan_scratch_words((int64_t)nbin + 1),
@@ -481,7 +487,15 @@ void gfsx_flip_step(gfsx_ctx *ctx, const gfsx_scene *sc,
         float *olds[3] = {sc->u_old, sc->v_old, sc->w_old};
         float *wts[3] = {sc->wu, sc->wv, sc->ww};
         const int old_slots[3] = {S_OLD0, S_OLD1, S_OLD2};
+        {
+            // one entry per P2G brick of 4x4x4 cells, at most
+            const int64_t nbrick = (int64_t)((ni + 3) / 4) * ((nj + 3) / 4) * ((nk + 3) / 4);
+            a.dense = ctx->u32(S_DENSE, nbrick);
+            a.dense_count = ctx->u32(S_DENSEN, 1);
+        }
         for (int c = 0; c < 3; c++) {
+            a.xs[c] = ctx->f32(S_XS0 + c, sc->n);
+            a.vs[c] = ctx->f32(S_VS0 + c, sc->n);
             a.grid[c] = grids[c];
             a.grid_old[c] = olds[c] ? olds[c] : ctx->f32(old_slots[c], mac_count(c, ni, nj, nk));
             a.wgt[c] = wts[c];

@@ -98,7 +98,11 @@ struct SortBuffers {
     uint32_t *start;      // nbin+2
     uint32_t *key;        // n
     uint32_t *slot;       // n
-    uint32_t *perm;       // n
+    uint32_t *perm;       // n: the result
+    uint32_t *perm_tmp;   // n: bins filled, not yet ordered inside
+    uint32_t *hrank;      // n (octant order only): rank inside the half-cell
+    uint32_t *big_list;   // bins too large to order in registers
+    uint32_t *big_count;  // 1
     uint32_t *block_out;  // blocks(n)
     uint32_t *block_out_scan;  // blocks(n)+1
     uint32_t *scan_scratch;
@@ -112,7 +116,8 @@ int64_t sort_blocks(int64_t n);
 // start[b] is the first perm slot of bin b; start[nbin+1] == n.
 // With octant_order, particles inside a bin are ordered by (octant, index)
 // instead, the octant being the 3-bit index of the half of the cell the
-// particle lies in along x, y, z (so each half-cell is a contiguous run).
+// particle lies in along x, y, z (so each half-cell is a contiguous run),
+// and hrank[q] receives the rank of perm[q] inside its half-cell run.
 // aos != nullptr selects AoS input (x,y,z interleaved); otherwise SoA.
 void cell_sort(const float *aos, const float *x, const float *y,
                const float *z, int64_t n, const BinGrid &g,
@@ -138,6 +143,8 @@ struct StepArgs {
     double dx;
     const float *xin[3], *vin[3];
     float *xout[3], *vout[3];
+    float *xs[3], *vs[3];            // workspace: the input in sorted order
+    uint32_t *dense, *dense_count;   // workspace: P2G bricks over capacity
     const uint32_t *id_in;
     uint32_t *id_out;
     float *grid[3], *grid_old[3], *wgt[3];

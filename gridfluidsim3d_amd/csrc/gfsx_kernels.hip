@@ -187,7 +187,7 @@ int64_t sort_blocks(int64_t n) { return (n + kBlock - 1) / kBlock; }
 // integer atomic; lanes of a wave that hold a run of equal bins (the input
 // is usually close to cell order) take one atomic per run and number
 // themselves within it in lane order.  The order inside a bin is then
-// fixed by k_bucket_sort.  Overflow particles get their rank among the
+// fixed by k_order_bins / k_order_big.  Overflow particles get their rank among the
 // block's overflow particles from a wave ballot, so that bin is filled in
 // index order directly.  With an octant array, each particle also records
 // which half of its cell it occupies along each axis.
@@ -279,42 +279,141 @@ __global__ __launch_bounds__(kBlock) void k_scatter(
     if (poct) poct[dst] = oct[i];
 }
 
-// Put each in-grid bin's entries into ascending order of key(q): the index
-// itself, or (octant, index) when poct is given (the octants then move
-// with the indices).  Bins hold a handful of particles, so a per-thread
-// insertion sort is enough; a bin with more than 128 entries uses a Shell
-// sort (gaps 1, 4, 13, 40, ...).
-__global__ __launch_bounds__(kBlock) void k_bucket_sort(const uint32_t *start,
-                                                       uint32_t nbin,
-                                                       uint32_t *perm,
-                                                       uint8_t *poct) {
-    const uint32_t b = blockIdx.x * kBlock + threadIdx.x;
-    if (b >= nbin) return;
-    const uint32_t s = start[b], e = start[b + 1];
-    const uint32_t len = e - s;
-    if (len <= 1) return;
-    uint32_t *a = perm + s;
-    uint8_t *o = poct ? poct + s : nullptr;
-    uint32_t gap = 1;
-    if (len > 128)
-        while (gap < len / 3) gap = 3 * gap + 1;
-    for (; gap >= 1; gap /= 3) {
-        for (uint32_t q = gap; q < len; q++) {
-            const uint32_t v = a[q];
-            const uint8_t vo = o ? o[q] : 0;
-            const uint64_t kv = ((uint64_t)vo << 32) | v;
-            uint32_t r = q;
-            while (r >= gap) {
-                const uint64_t kr = ((uint64_t)(o ? o[r - gap] : 0) << 32) | a[r - gap];
-                if (kr <= kv) break;
-                a[r] = a[r - gap];
-                if (o) o[r] = o[r - gap];
-                r -= gap;
+// Sorting-network compare-exchange: v[i] <= v[l] afterwards when up.
+template <int N>
+__device__ __forceinline__ void bitonic_sort(uint64_t (&v)[N]) {
+#pragma unroll
+    for (int k = 2; k <= N; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+#pragma unroll
+            for (int i = 0; i < N; i++) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const bool up = (i & k) == 0;
+                    const uint64_t a = v[i], b = v[l];
+                    const bool sw = up ? (a > b) : (a < b);
+                    v[i] = sw ? b : a;
+                    v[l] = sw ? a : b;
+                }
             }
-            a[r] = v;
-            if (o) o[r] = vo;
         }
-        if (gap == 1) break;
+    }
+}
+
+// The sort key of slot q: the particle index, or (octant, index) when
+// octants are given.  Keys of one bin are distinct.
+__device__ __forceinline__ uint64_t order_key(const uint32_t *perm,
+                                              const uint8_t *poct, uint32_t q) {
+    const uint64_t o = poct ? (uint64_t)poct[q] : 0ull;
+    return (o << 32) | (uint64_t)perm[q];
+}
+
+// Write a bin's sorted keys: perm_out gets the indices; hrank (when given)
+// the rank of each particle among its bin's particles of the same octant.
+template <int N>
+__device__ __forceinline__ void write_sorted(const uint64_t (&v)[N], uint32_t s,
+                                             uint32_t len, uint32_t *perm_out,
+                                             uint32_t *hrank) {
+    uint32_t rs = 0;
+#pragma unroll
+    for (int m = 0; m < N; m++) {
+        if (m > 0 && (v[m] >> 32) != (v[m - 1] >> 32)) rs = (uint32_t)m;
+        if ((uint32_t)m < len) {
+            perm_out[s + m] = (uint32_t)v[m];
+            if (hrank) hrank[s + m] = (uint32_t)m - rs;
+        }
+    }
+}
+
+constexpr uint32_t kSmallBin = 16;   // bins up to this size sort in registers
+
+// Order every in-grid bin's entries by key (see order_key), out of place
+// from perm_in to perm_out.  A thread owns a bin: bins of up to kSmallBin
+// entries are sorted in registers by a bitonic network (the network size is
+// the largest bin of the wave, rounded up to 8 or 16); larger bins are
+// appended to big_list for k_order_big.
+__global__ __launch_bounds__(kBlock) void k_order_bins(
+    const uint32_t *__restrict__ start, uint32_t nbin,
+    const uint32_t *__restrict__ perm_in, const uint8_t *__restrict__ poct,
+    uint32_t *__restrict__ perm_out, uint32_t *__restrict__ hrank,
+    uint32_t *__restrict__ big_list, uint32_t *__restrict__ big_count) {
+    const uint32_t b = blockIdx.x * kBlock + threadIdx.x;
+    uint32_t s = 0, len = 0;
+    if (b < nbin) {
+        s = start[b];
+        len = start[b + 1] - s;
+    }
+    if (len > kSmallBin) {
+        big_list[atomicAdd(big_count, 1u)] = b;
+        len = 0;
+    }
+    if (len == 1) {
+        perm_out[s] = perm_in[s];
+        if (hrank) hrank[s] = 0u;
+        return;
+    }
+    // wave-uniform network size
+    const bool need16 = __any(len > 8u), need8 = __any(len > 1u);
+    if (need16) {
+        uint64_t v[16];
+#pragma unroll
+        for (int m = 0; m < 16; m++)
+            v[m] = (uint32_t)m < len ? order_key(perm_in, poct, s + m) : ~0ull;
+        bitonic_sort<16>(v);
+        write_sorted<16>(v, s, len, perm_out, hrank);
+    } else if (need8) {
+        uint64_t v[8];
+#pragma unroll
+        for (int m = 0; m < 8; m++)
+            v[m] = (uint32_t)m < len ? order_key(perm_in, poct, s + m) : ~0ull;
+        bitonic_sort<8>(v);
+        write_sorted<8>(v, s, len, perm_out, hrank);
+    }
+}
+
+constexpr int kBigTile = 2048;   // keys per LDS tile in k_order_big
+constexpr int kOrderBigBlocks = 1024;
+
+// The bins k_order_bins left: one wave per bin ranks every entry by
+// counting the smaller keys of its bin (tiles of the keys pass through LDS,
+// so any size works).  Also copies the overflow bin, which k_bin already
+// filled in index order.
+__global__ __launch_bounds__(64) void k_order_big(
+    const uint32_t *__restrict__ start, uint32_t nbin,
+    const uint32_t *__restrict__ perm_in, const uint8_t *__restrict__ poct,
+    uint32_t *__restrict__ perm_out, uint32_t *__restrict__ hrank,
+    const uint32_t *__restrict__ big_list, const uint32_t *__restrict__ big_count) {
+    __shared__ uint64_t tile[kBigTile];
+    const uint32_t lane = threadIdx.x;
+    const uint32_t o0 = start[nbin], o1 = start[nbin + 1];
+    for (uint32_t q = o0 + blockIdx.x * 64u + lane; q < o1; q += gridDim.x * 64u)
+        perm_out[q] = perm_in[q];
+    const uint32_t nbig = *big_count;
+    for (uint32_t c = blockIdx.x; c < nbig; c += gridDim.x) {
+        const uint32_t b = big_list[c];
+        const uint32_t s = start[b], len = start[b + 1] - s;
+        for (uint32_t e0 = 0; e0 < len; e0 += 64u) {
+            const uint32_t e = e0 + lane;
+            const uint64_t mine = e < len ? order_key(perm_in, poct, s + e) : ~0ull;
+            uint32_t rank = 0, hr = 0;
+            for (uint32_t t0 = 0; t0 < len; t0 += kBigTile) {
+                const uint32_t tl = min(len - t0, (uint32_t)kBigTile);
+                __syncthreads();
+                for (uint32_t k = lane; k < tl; k += 64u) tile[k] = order_key(perm_in, poct, s + t0 + k);
+                __syncthreads();
+                for (uint32_t k = 0; k < tl; k++) {
+                    const uint64_t kk = tile[k];
+                    const bool lt = kk < mine;
+                    rank += lt;
+                    hr += lt && (kk >> 32) == (mine >> 32);
+                }
+            }
+            if (e < len) {
+                perm_out[s + rank] = (uint32_t)mine;
+                if (hrank) hrank[s + rank] = hr;
+            }
+        }
     }
 }
 
@@ -344,12 +443,19 @@ void cell_sort(const float *aos, const float *x, const float *y,
     hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(kBlock), 0, s,
                        (const uint32_t *)sb.key, (const uint32_t *)sb.slot, n,
                        (const uint32_t *)sb.start,
-                       (const uint32_t *)sb.block_out_scan, g.nbin, sb.perm,
+                       (const uint32_t *)sb.block_out_scan, g.nbin, sb.perm_tmp,
                        (const uint8_t *)oct, poct);
     GFSX_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_bucket_sort, dim3(grid_for(g.nbin, kBlock)),
-                       dim3(kBlock), 0, s, (const uint32_t *)sb.start, g.nbin,
-                       sb.perm, poct);
+    fill_u32(sb.big_count, 0u, 1, s);
+    uint32_t *hrank = octant_order ? sb.hrank : nullptr;
+    hipLaunchKernelGGL(k_order_bins, dim3(grid_for(g.nbin, kBlock)), dim3(kBlock), 0, s,
+                       (const uint32_t *)sb.start, g.nbin, (const uint32_t *)sb.perm_tmp,
+                       (const uint8_t *)poct, sb.perm, hrank, sb.big_list, sb.big_count);
+    GFSX_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_order_big, dim3(kOrderBigBlocks), dim3(64), 0, s,
+                       (const uint32_t *)sb.start, g.nbin, (const uint32_t *)sb.perm_tmp,
+                       (const uint8_t *)poct, sb.perm, hrank,
+                       (const uint32_t *)sb.big_list, (const uint32_t *)sb.big_count);
     GFSX_LAUNCH_CHECK();
 }
 
@@ -382,7 +488,7 @@ __device__ __forceinline__ void gather_node(int i, int j, int k,
             const uint32_t row = (uint32_t)bg.b[0] * ((uint32_t)bj + (uint32_t)bg.b[1] * (uint32_t)bk);
             const uint32_t q0 = start[row + ilo], q1 = start[row + ihi + 1];
             for (uint32_t q = q0; q < q1; q++) {
-                const uint32_t p = perm[q];
+                const uint32_t p = perm ? perm[q] : q;
                 float px, py, pz, val;
                 load(p, px, py, pz, val);
                 // p -= offset, then v = gpos - p (scalarfield.cpp:168,186-188)
@@ -822,13 +928,11 @@ constexpr int HX = 2 * SX, HY = 2 * SY, HZ = 2 * SZ;  // half-cells
 constexpr int NH = HX * HY * HZ;
 constexpr int NCELL = BX * BY * BZ;       // 64 cells: one wave per component
 constexpr int NT = 3 * NCELL;             // 192 threads
-// Particles staged per brick (6 floats each).  With the half-cell table
-// this keeps a work-group under 54 KB of LDS, so three fit in a CU's
-// 160 KB; a dense brick (6x6x6 cells at 8 per cell) stages 1728.
+// Particles staged per chunk (6 floats each); a brick at 8 particles per
+// cell stages 6x6x6 cells * 8 = 1728.
 constexpr int CAP = 2016;
-constexpr int PER = (CAP + NT - 1) / NT;
+constexpr int DENSE_BLOCKS = 2048;   // work-groups of the pass over dense bricks
 static_assert(NCELL == 64, "one wave per component");
-static_assert(NH + 1 <= CAP, "half-cell counts live in the x array");
 }  // namespace brick
 
 __device__ __forceinline__ void face_epilogue(const StepP2G &a,
@@ -997,29 +1101,26 @@ __device__ __forceinline__ int brick_row_of(const uint32_t *row_s0, uint32_t s) 
     return r;
 }
 
+// Add the staged candidates of face (li,lj,lk) (brick-local) of component C
+// to S and W.
 template <int C, int V>
-__device__ __forceinline__ void brick_face(const StepP2G &a, const BinGrid &bg,
-                                           const uint32_t *__restrict__ count,
-                                           const float *sx, const float *sy,
-                                           const float *sz, const float *sv,
-                                           const uint16_t *hs, int i0, int j0,
-                                           int k0, int li, int lj, int lk,
-                                           double dx, float inv_r2) {
-    const int fi = i0 + li, fj = j0 + lj, fk = k0 + lk;
-    if (fi >= a.ni + (C == 0) || fj >= a.nj + (C == 1) || fk >= a.nk + (C == 2)) return;
+__device__ __forceinline__ void brick_accum(const StepP2G &a, const float *sx,
+                                            const float *sy, const float *sz,
+                                            const float *sv, const uint16_t *hs,
+                                            int i0, int j0, int k0, int li, int lj,
+                                            int lk, double dx, float inv_r2,
+                                            float &S, float &W) {
     // face position in the component's shifted frame (grid3d.h:79-81)
-    const float gx = (float)((float)fi * dx);
-    const float gy = (float)((float)fj * dx);
-    const float gz = (float)((float)fk * dx);
+    const float gx = (float)((float)(i0 + li) * dx);
+    const float gy = (float)((float)(j0 + lj) * dx);
+    const float gz = (float)((float)(k0 + lk) * dx);
     const float offh = a.ng[C].off[C == 0 ? 1 : 0];
-    float S = 0.0f, W = 0.0f;
     if (V == 0)
         brick_gather_chunked<C>(sx, sy, sz, sv, hs, li + 1, lj + 1, lk + 1, gx, gy, gz, offh, inv_r2, S, W);
     else if (V == 2)
         brick_gather_nested<C>(sx, sy, sz, sv, hs, li + 1, lj + 1, lk + 1, gx, gy, gz, offh, inv_r2, S, W);
     else
         brick_gather<C>(sx, sy, sz, sv, hs, li + 1, lj + 1, lk + 1, gx, gy, gz, offh, inv_r2, S, W);
-    face_epilogue(a, bg, count, C, fi, fj, fk, S, W);
 }
 
 __device__ __forceinline__ void brick_zero_face(const StepP2G &a, int c, int fi, int fj, int fk) {
@@ -1031,27 +1132,28 @@ __device__ __forceinline__ void brick_zero_face(const StepP2G &a, int c, int fi,
     a.grid[c][f] = 0.0f;
 }
 
-template <int V>
-__global__ __launch_bounds__(brick::NT) void k_flip_p2g_brick(
-    StepP2G a, BinGrid bg, double dx, float inv_r2, int nbx, int nby, int nbz,
-    int per_xcd, const uint32_t *__restrict__ count,
-    const uint32_t *__restrict__ start, const uint32_t *__restrict__ perm) {
+// One work-group per brick of BX*BY*BZ cells; wave w computes the faces of
+// component w.  The particles of the staged cells (the brick plus one cell
+// around it) are copied into LDS in half-cell-major order, CAP at a time.
+// With CHUNKED false a brick holding more than CAP is appended to dense[]
+// and left for the CHUNKED pass, which processes it in chunks of
+// consecutive sort slots, the faces accumulating over the chunks (a
+// separate kernel, so that the common case keeps its register budget).
+template <int V, int CAP, bool CHUNKED>
+__device__ __forceinline__ void p2g_brick(
+    const StepP2G &a, const BinGrid &bg, double dx, float inv_r2, int nbx, int nby,
+    int brick_id, const uint32_t *__restrict__ count,
+    const uint32_t *__restrict__ start, const uint32_t *__restrict__ hrank,
+    uint32_t *__restrict__ dense, uint32_t *__restrict__ dense_count,
+    float *s_buf, uint16_t *s_hs, uint32_t *s_rowq, uint32_t *s_rows0, uint32_t *s_wsum) {
     using namespace brick;
-    __shared__ float s_x[CAP], s_y[CAP], s_z[CAP];
-    __shared__ float s_v[3][CAP];
-    __shared__ uint16_t s_hs[NH + 1];
-    __shared__ uint32_t s_rowq[NROW], s_rows0[NROW + 1];
-    __shared__ uint32_t s_wsum[NT / 64];
-    // half-cell counts (32-bit, for LDS atomics) borrow the x array, which
-    // is only written once the layout is known
-    uint32_t *s_cnt = reinterpret_cast<uint32_t *>(s_x);
+    constexpr int PER = (CAP + NT - 1) / NT;
+    float *s_x = s_buf, *s_y = s_buf + CAP, *s_z = s_buf + 2 * CAP;
+    float *s_v[3] = {s_buf + 3 * CAP, s_buf + 4 * CAP, s_buf + 5 * CAP};
+    // half-cell counts (32-bit, for LDS atomics) borrow the particle
+    // arrays, which are only written once the layout is known
+    uint32_t *s_cnt = reinterpret_cast<uint32_t *>(s_buf);
 
-    // consecutive bricks share halo particles: keep them on one XCD (work-
-    // groups are dealt round-robin over the 8 XCDs)
-    const int xcd = blockIdx.x & 7;
-    const int brick_id = xcd * per_xcd + (blockIdx.x >> 3);
-    const int nbrick = nbx * nby * nbz;
-    if (brick_id >= nbrick) return;
     const int bxi = brick_id % nbx, byi = (brick_id / nbx) % nby, bzi = brick_id / (nbx * nby);
     const int i0 = bxi * BX, j0 = byi * BY, k0 = bzi * BZ;
     const int t = threadIdx.x;
@@ -1060,7 +1162,8 @@ __global__ __launch_bounds__(brick::NT) void k_flip_p2g_brick(
     const int bx_hi = min(i0 + BX + 1, bg.b[0] - 1);
 
     // rows of staged cells: world cells [i0-1, i0+BX] along x are bins
-    // [i0, i0+BX+1] (pad 1), contiguous in perm; wave 0 scans the lengths
+    // [i0, i0+BX+1] (pad 1), contiguous in sort order; wave 0 scans the
+    // lengths
     if (wave == 0) {
         uint32_t q0 = 0, len = 0;
         if (lane < NROW) {
@@ -1079,174 +1182,176 @@ __global__ __launch_bounds__(brick::NT) void k_flip_p2g_brick(
     for (int h = t; h <= NH; h += NT) s_cnt[h] = 0u;
     __syncthreads();
     const uint32_t N = s_rows0[NROW];
+    if (!CHUNKED && N > (uint32_t)CAP) {
+        if (t == 0) dense[atomicAdd(dense_count, 1u)] = (uint32_t)brick_id;
+        return;
+    }
 
-    // this thread's cell; wave w computes component w
+    // this thread's face: cell (li,lj,lk) of the brick, component = wave
     const int li = lane % BX, lj = (lane / BX) % BY, lk = lane / (BX * BY);
-    // the faces on the domain's upper planes belong to the last brick
+    const bool own_ok = i0 + li < a.ni + (wave == 0) && j0 + lj < a.nj + (wave == 1) &&
+                        k0 + lk < a.nk + (wave == 2);
+    // the faces on the domain's upper planes belong to the last brick:
+    // lanes 0..15 of the wave take its 4x4 plane
+    const int e0 = lane & 3, e1 = (lane >> 2) & 3;
     const bool xtra = (wave == 0 && i0 + BX == a.ni) || (wave == 1 && j0 + BY == a.nj) ||
                       (wave == 2 && k0 + BZ == a.nk);
-    const int e0 = lane & 3, e1 = (lane >> 2) & 3;   // lanes 0..15: the 4x4 plane
+    int eli, elj, elk;
+    if (wave == 0) { eli = BX; elj = e0; elk = e1; }
+    else if (wave == 1) { eli = e0; elj = BY; elk = e1; }
+    else { eli = e0; elj = e1; elk = BZ; }
+    const bool ext_ok = xtra && lane < 16 && i0 + eli < a.ni + (wave == 0) &&
+                        j0 + elj < a.nj + (wave == 1) && k0 + elk < a.nk + (wave == 2);
 
     if (N == 0) {
         // nothing within reach of any face of this brick: all values are 0
         // and no face borders fluid
-        brick_zero_face(a, wave, i0 + li, j0 + lj, k0 + lk);
-        if (xtra && lane < 16) {
-            if (wave == 0) brick_zero_face(a, 0, i0 + BX, j0 + e0, k0 + e1);
-            else if (wave == 1) brick_zero_face(a, 1, i0 + e0, j0 + BY, k0 + e1);
-            else brick_zero_face(a, 2, i0 + e0, j0 + e1, k0 + BZ);
-        }
-        return;
-    }
-    if (N > (uint32_t)CAP) {
-        // too many particles to stage: gather straight from global memory
-        SoaLoad ld{a.x, a.y, a.z, a.vel[wave]};
-        for (int pass = 0; pass < 2; pass++) {
-            int fi = i0 + li, fj = j0 + lj, fk = k0 + lk;
-            if (pass == 1) {
-                if (!(xtra && lane < 16)) break;
-                if (wave == 0) { fi = i0 + BX; fj = j0 + e0; fk = k0 + e1; }
-                else if (wave == 1) { fi = i0 + e0; fj = j0 + BY; fk = k0 + e1; }
-                else { fi = i0 + e0; fj = j0 + e1; fk = k0 + BZ; }
-            }
-            if (fi < a.ni + (wave == 0) && fj < a.nj + (wave == 1) && fk < a.nk + (wave == 2)) {
-                float S = 0.0f, W = 0.0f;
-                gather_node<true>(fi, fj, fk, a.ng[wave], bg, dx, inv_r2, start, perm, ld, S, W);
-                face_epilogue(a, bg, count, wave, fi, fj, fk, S, W);
-            }
-        }
+        if (own_ok) brick_zero_face(a, wave, i0 + li, j0 + lj, k0 + lk);
+        if (ext_ok) brick_zero_face(a, wave, i0 + eli, j0 + elj, k0 + elk);
         return;
     }
 
-    // stage 1: positions -> half-cell of every staged particle; count them.
-    // Every slot's loads are issued before any is used (slots past N load
-    // particle 0 and are masked), so the global latency is paid once per
-    // batch instead of once per slot.
-    uint32_t rp[PER], rh[PER], rfirst[PER], ro[PER];
-    int rrow[PER];
-#pragma unroll
-    for (int m = 0; m < PER; m++) {
-        const uint32_t s = (uint32_t)t + (uint32_t)m * NT;
-        const int r = brick_row_of(s_rows0, s);
-        rrow[m] = r;
-        const uint32_t q = s_rowq[r] + (s - s_rows0[r]);
-        rp[m] = perm[s < N ? q : 0u];
-    }
-    float rx[PER], ry[PER], rz[PER];
-#pragma unroll
-    for (int m = 0; m < PER; m++) {
-        const uint32_t s = (uint32_t)t + (uint32_t)m * NT;
-        const uint32_t p = s < N ? rp[m] : rp[0];
-        rx[m] = a.x[p];
-        ry[m] = a.y[p];
-        rz[m] = a.z[p];
-    }
-    uint32_t rbin[PER];
-#pragma unroll
-    for (int m = 0; m < PER; m++) {
-        // the same FP64 floor as the binning, so the cell agrees
-        const double gxd = (double)rx[m] * bg.inv_dx;
-        const double gyd = (double)ry[m] * bg.inv_dx;
-        const double gzd = (double)rz[m] * bg.inv_dx;
-        const double fx = floor(gxd), fy = floor(gyd), fz = floor(gzd);
-        const int lx = (int)fx - (i0 - 1), ly = (int)fy - (j0 - 1), lz = (int)fz - (k0 - 1);
-        const int ox = (gxd - fx) >= 0.5, oy = (gyd - fy) >= 0.5, oz = (gzd - fz) >= 0.5;
-        rh[m] = (uint32_t)((2 * lx + ox) + HX * ((2 * ly + oy) + HY * (2 * lz + oz)));
-        ro[m] = (uint32_t)(ox + 2 * oy + 4 * oz);
-        const uint32_t s = (uint32_t)t + (uint32_t)m * NT;
-        // the particle's bin (pad 1); slots past N use the brick's first row
-        rbin[m] = s < N ? (uint32_t)(fx + 1.0) + row_stride * ((uint32_t)(fy + 1.0) +
-                                                 (uint32_t)bg.b[1] * (uint32_t)(fz + 1.0))
-                        : (uint32_t)i0;
-        if (s < N) atomicAdd(&s_cnt[rh[m]], 1u);
-    }
-#pragma unroll
-    for (int m = 0; m < PER; m++) {
-        // first staged slot of the particle's cell: its bin's start,
-        // relative to the row
-        const int r = rrow[m];
-        rfirst[m] = s_rows0[r] + (start[rbin[m]] - s_rowq[r]);
-    }
-    __syncthreads();
-
-    // stage 2: exclusive prefix over the half-cells
-    {
-        constexpr int ITEMS = (NH + NT - 1) / NT;
-        uint32_t loc[ITEMS];
-        uint32_t sum = 0;
-#pragma unroll
-        for (int m = 0; m < ITEMS; m++) {
-            const int h = t * ITEMS + m;
-            loc[m] = h < NH ? s_cnt[h] : 0u;
-            sum += loc[m];
+    float S = 0.0f, W = 0.0f, SE = 0.0f, WE = 0.0f;
+    for (uint32_t c0 = 0; c0 < N; c0 += (uint32_t)CAP) {
+        const uint32_t ce = min(N, c0 + (uint32_t)CAP);
+        if (c0 > 0) {
+            __syncthreads();   // the last chunk's gather is done with LDS
+            for (int h = t; h <= NH; h += NT) s_cnt[h] = 0u;
+            __syncthreads();
         }
-        // block-wide exclusive sum over the 3 waves
-        const uint32_t inc = wave_inclusive_sum(sum);
-        if (lane == 63) s_wsum[wave] = inc;
+
+        // stage 1: positions -> half-cell of every particle of the chunk;
+        // count them.  Every slot's loads (position, velocity, rank inside
+        // its half-cell) are issued before any is used (slots past the
+        // chunk load the brick's first slot and are masked), so the global
+        // latency is paid once per batch.  The particle arrays are in sort
+        // order: slot q of the sort is index q.
+        uint32_t rq[PER], rh[PER], rk[PER];
+#pragma unroll
+        for (int m = 0; m < PER; m++) {
+            const uint32_t s = c0 + (uint32_t)t + (uint32_t)m * NT;
+            const int r = brick_row_of(s_rows0, s);
+            rq[m] = s < ce ? s_rowq[r] + (s - s_rows0[r]) : s_rowq[0];
+        }
+        float rx[PER], ry[PER], rz[PER], rvx[PER], rvy[PER], rvz[PER];
+#pragma unroll
+        for (int m = 0; m < PER; m++) {
+            const uint32_t q = rq[m];
+            rx[m] = a.x[q];
+            ry[m] = a.y[q];
+            rz[m] = a.z[q];
+            rvx[m] = a.vel[0][q];
+            rvy[m] = a.vel[1][q];
+            rvz[m] = a.vel[2][q];
+            rk[m] = hrank[q];
+        }
+#pragma unroll
+        for (int m = 0; m < PER; m++) {
+            // the same FP64 floor as the binning, so the cell agrees
+            const double gxd = (double)rx[m] * bg.inv_dx;
+            const double gyd = (double)ry[m] * bg.inv_dx;
+            const double gzd = (double)rz[m] * bg.inv_dx;
+            const double fx = floor(gxd), fy = floor(gyd), fz = floor(gzd);
+            const int lx = (int)fx - (i0 - 1), ly = (int)fy - (j0 - 1), lz = (int)fz - (k0 - 1);
+            const int ox = (gxd - fx) >= 0.5, oy = (gyd - fy) >= 0.5, oz = (gzd - fz) >= 0.5;
+            rh[m] = (uint32_t)((2 * lx + ox) + HX * ((2 * ly + oy) + HY * (2 * lz + oz)));
+            const uint32_t s = c0 + (uint32_t)t + (uint32_t)m * NT;
+            if (s < ce) atomicAdd(&s_cnt[rh[m]], 1u);
+        }
         __syncthreads();
-        uint32_t run = inc - sum;
-        for (int w = 0; w < wave; w++) run += s_wsum[w];
-#pragma unroll
-        for (int m = 0; m < ITEMS; m++) {
-            const int h = t * ITEMS + m;
-            if (h < NH) s_hs[h] = (uint16_t)run;
-            run += loc[m];
-        }
-        if (t == 0) s_hs[NH] = (uint16_t)N;
-    }
-    __syncthreads();
 
-    // stage 3: place every particle.  Inside its cell the particles are
-    // ordered by (octant, index), so the half-cell (cell, o) is the run that
-    // starts after the cell's runs of octants below o; the particle's rank
-    // in its half-cell is its distance from that start.
-    float rvx[PER], rvy[PER], rvz[PER];
+        // stage 2: exclusive prefix over the half-cells
+        {
+            constexpr int ITEMS = (NH + NT - 1) / NT;
+            uint32_t loc[ITEMS];
+            uint32_t sum = 0;
 #pragma unroll
-    for (int m = 0; m < PER; m++) {
-        const uint32_t s = (uint32_t)t + (uint32_t)m * NT;
-        const uint32_t p = s < N ? rp[m] : rp[0];
-        rvx[m] = a.vel[0][p];
-        rvy[m] = a.vel[1][p];
-        rvz[m] = a.vel[2][p];
-    }
-#pragma unroll
-    for (int m = 0; m < PER; m++) {
-        const uint32_t s = (uint32_t)t + (uint32_t)m * NT;
-        if (s < N) {
-            const uint32_t o = ro[m];
-            const uint32_t h0 = rh[m] - ((o & 1u) + HX * (((o >> 1) & 1u) + HY * (o >> 2)));
-            uint32_t run = rfirst[m];
-#pragma unroll
-            for (uint32_t oo = 0; oo < 7; oo++) {
-                const uint32_t ho = h0 + (oo & 1u) + HX * (((oo >> 1) & 1u) + HY * (oo >> 2));
-                const uint32_t c = (uint32_t)s_hs[ho + 1] - (uint32_t)s_hs[ho];
-                run += oo < o ? c : 0u;
+            for (int m = 0; m < ITEMS; m++) {
+                const int h = t * ITEMS + m;
+                loc[m] = h < NH ? s_cnt[h] : 0u;
+                sum += loc[m];
             }
-            const uint32_t d = (uint32_t)s_hs[rh[m]] + (s - run);
-            s_x[d] = rx[m];
-            s_y[d] = ry[m];
-            s_z[d] = rz[m];
-            s_v[0][d] = rvx[m];
-            s_v[1][d] = rvy[m];
-            s_v[2][d] = rvz[m];
+            // block-wide exclusive sum over the 3 waves
+            const uint32_t inc = wave_inclusive_sum(sum);
+            if (lane == 63) s_wsum[wave] = inc;
+            __syncthreads();
+            uint32_t run = inc - sum;
+            for (int w = 0; w < wave; w++) run += s_wsum[w];
+#pragma unroll
+            for (int m = 0; m < ITEMS; m++) {
+                const int h = t * ITEMS + m;
+                if (h < NH) s_hs[h] = (uint16_t)run;
+                run += loc[m];
+            }
+            if (t == 0) s_hs[NH] = (uint16_t)(ce - c0);
         }
-    }
-    __syncthreads();
+        __syncthreads();
 
-    // gather: wave w computes component w for its 64 cells, then the
-    // domain-boundary plane if this brick has one
-    if (wave == 0) {
-        brick_face<0, V>(a, bg, count, s_x, s_y, s_z, s_v[0], s_hs, i0, j0, k0, li, lj, lk, dx, inv_r2);
-        if (xtra && lane < 16)
-            brick_face<0, V>(a, bg, count, s_x, s_y, s_z, s_v[0], s_hs, i0, j0, k0, BX, e0, e1, dx, inv_r2);
-    } else if (wave == 1) {
-        brick_face<1, V>(a, bg, count, s_x, s_y, s_z, s_v[1], s_hs, i0, j0, k0, li, lj, lk, dx, inv_r2);
-        if (xtra && lane < 16)
-            brick_face<1, V>(a, bg, count, s_x, s_y, s_z, s_v[1], s_hs, i0, j0, k0, e0, BY, e1, dx, inv_r2);
+        // stage 3: place every particle at its half-cell's first LDS slot
+        // plus its rank among the half-cell's particles in this chunk (a
+        // half-cell's particles are consecutive sort slots, the run
+        // starting rank slots before the particle)
+#pragma unroll
+        for (int m = 0; m < PER; m++) {
+            const uint32_t s = c0 + (uint32_t)t + (uint32_t)m * NT;
+            if (s < ce) {
+                const uint32_t d = (uint32_t)s_hs[rh[m]] + min(rk[m], s - c0);
+                s_x[d] = rx[m];
+                s_y[d] = ry[m];
+                s_z[d] = rz[m];
+                s_v[0][d] = rvx[m];
+                s_v[1][d] = rvy[m];
+                s_v[2][d] = rvz[m];
+            }
+        }
+        __syncthreads();
+
+        // gather: wave w accumulates component w for its 64 faces, then the
+        // domain-boundary plane if this brick has one
+        if (wave == 0) {
+            if (own_ok) brick_accum<0, V>(a, s_x, s_y, s_z, s_v[0], s_hs, i0, j0, k0, li, lj, lk, dx, inv_r2, S, W);
+            if (ext_ok) brick_accum<0, V>(a, s_x, s_y, s_z, s_v[0], s_hs, i0, j0, k0, eli, elj, elk, dx, inv_r2, SE, WE);
+        } else if (wave == 1) {
+            if (own_ok) brick_accum<1, V>(a, s_x, s_y, s_z, s_v[1], s_hs, i0, j0, k0, li, lj, lk, dx, inv_r2, S, W);
+            if (ext_ok) brick_accum<1, V>(a, s_x, s_y, s_z, s_v[1], s_hs, i0, j0, k0, eli, elj, elk, dx, inv_r2, SE, WE);
+        } else {
+            if (own_ok) brick_accum<2, V>(a, s_x, s_y, s_z, s_v[2], s_hs, i0, j0, k0, li, lj, lk, dx, inv_r2, S, W);
+            if (ext_ok) brick_accum<2, V>(a, s_x, s_y, s_z, s_v[2], s_hs, i0, j0, k0, eli, elj, elk, dx, inv_r2, SE, WE);
+        }
+        if (!CHUNKED) break;
+    }
+    if (own_ok) face_epilogue(a, bg, count, wave, i0 + li, j0 + lj, k0 + lk, S, W);
+    if (ext_ok) face_epilogue(a, bg, count, wave, i0 + eli, j0 + elj, k0 + elk, SE, WE);
+}
+
+template <int V, int CAP, bool CHUNKED>
+__global__ __launch_bounds__(brick::NT) void k_flip_p2g_brick(
+    StepP2G a, BinGrid bg, double dx, float inv_r2, int nbx, int nby, int nbz,
+    int per_xcd, const uint32_t *__restrict__ count,
+    const uint32_t *__restrict__ start, const uint32_t *__restrict__ hrank,
+    uint32_t *__restrict__ dense, uint32_t *__restrict__ dense_count) {
+    using namespace brick;
+    static_assert(NH + 1 <= 6 * CAP, "half-cell counts live in the particle arrays");
+    static_assert(CAP <= 65535, "LDS slots are 16-bit");
+    __shared__ float s_buf[6 * CAP];
+    __shared__ uint16_t s_hs[NH + 1];
+    __shared__ uint32_t s_rowq[NROW], s_rows0[NROW + 1];
+    __shared__ uint32_t s_wsum[NT / 64];
+    if (!CHUNKED) {
+        // consecutive bricks share halo particles: keep them on one XCD
+        // (work-groups are dealt round-robin over the 8 XCDs)
+        const int brick_id = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
+        if (brick_id >= nbx * nby * nbz) return;
+        p2g_brick<V, CAP, false>(a, bg, dx, inv_r2, nbx, nby, brick_id, count, start, hrank,
+                                 dense, dense_count, s_buf, s_hs, s_rowq, s_rows0, s_wsum);
     } else {
-        brick_face<2, V>(a, bg, count, s_x, s_y, s_z, s_v[2], s_hs, i0, j0, k0, li, lj, lk, dx, inv_r2);
-        if (xtra && lane < 16)
-            brick_face<2, V>(a, bg, count, s_x, s_y, s_z, s_v[2], s_hs, i0, j0, k0, e0, e1, BZ, dx, inv_r2);
+        const uint32_t nd = *dense_count;
+        for (uint32_t d = blockIdx.x; d < nd; d += gridDim.x) {
+            __syncthreads();   // the previous brick is done with LDS
+            p2g_brick<V, CAP, true>(a, bg, dx, inv_r2, nbx, nby, (int)dense[d], count, start,
+                                    hrank, dense, dense_count, s_buf, s_hs, s_rowq, s_rows0,
+                                    s_wsum);
+        }
     }
 }
 
@@ -1262,6 +1367,43 @@ struct StepG2P {
     int64_t n;
 };
 
+// Write the new state of the particle in sorted slot q: at q (sorted
+// order) or at its original index perm[q] (kept order), with its id.
+__device__ __forceinline__ void store_particle(const StepG2P &a,
+                                               const uint32_t *__restrict__ perm,
+                                               uint32_t q, const float x1[3],
+                                               const float vnew[3]) {
+    uint32_t o = q, p = q;
+    if (a.keep_order || a.id_out) p = perm[q];
+    if (a.keep_order) o = p;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        a.xout[c][o] = x1[c];
+        a.vout[c][o] = vnew[c];
+    }
+    if (a.id_out) a.id_out[o] = a.id_in ? a.id_in[p] : p;
+}
+
+// Positions and velocities copied into sorted order, so that the transfer
+// kernels read them by sort slot with no indirection.
+struct SortedCopy {
+    const float *src[6];
+    float *dst[6];
+};
+
+__global__ __launch_bounds__(kBlock) void k_sorted_copy(SortedCopy c,
+                                                       const uint32_t *__restrict__ perm,
+                                                       int64_t n) {
+    const int64_t q = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    if (q >= n) return;
+    const uint32_t p = perm[q];
+    float v[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) v[k] = c.src[k][p];
+#pragma unroll
+    for (int k = 0; k < 6; k++) c.dst[k][q] = v[k];
+}
+
 // One thread per particle, in cell order: PIC/FLIP blend
 // (fluidsimulation.cpp:3115-3127) and RK4 (particleadvector.cpp:1045-1054)
 // through the new field.
@@ -1270,9 +1412,9 @@ __global__ __launch_bounds__(kBlock) void k_flip_g2p(
     StepG2P a, MacGrid g, const uint32_t *__restrict__ perm) {
     const int64_t jdx = blockIdx.x * (int64_t)kBlock + threadIdx.x;
     if (jdx >= a.n) return;
-    const uint32_t p = perm[jdx];
-    const float x0[3] = {a.xin[0][p], a.xin[1][p], a.xin[2][p]};
-    const float v0[3] = {a.vin[0][p], a.vin[1][p], a.vin[2][p]};
+    const uint32_t q = (uint32_t)jdx;
+    const float x0[3] = {a.xin[0][q], a.xin[1][q], a.xin[2][q]};
+    const float v0[3] = {a.vin[0][q], a.vin[1][q], a.vin[2][q]};
     float k1[3], vo[3];
     sample_mac<ST, true>(g, a.old[0], a.old[1], a.old[2], x0[0], x0[1], x0[2], k1, vo);
     float vn[3] = {k1[0], k1[1], k1[2]};
@@ -1286,13 +1428,7 @@ __global__ __launch_bounds__(kBlock) void k_flip_g2p(
     }
     float x1[3];
     rk_step<ST>(g, 4, a.rk, x0, k1, x1);
-    const int64_t o = a.keep_order ? (int64_t)p : jdx;
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-        a.xout[c][o] = x1[c];
-        a.vout[c][o] = vnew[c];
-    }
-    if (a.id_out) a.id_out[o] = a.id_in ? a.id_in[p] : p;
+    store_particle(a, perm, q, x1, vnew);
 }
 
 // ---------------------------------------------------------------------------
@@ -1470,20 +1606,13 @@ __global__ __launch_bounds__(gbrick::NT) void k_flip_g2p_brick(
         const uint32_t q0 = start[bg.nbin], q1 = start[bg.nbin + 1];
         for (uint32_t q = q0 + (uint32_t)((blockIdx.x - nslot) * NT + t); q < q1;
              q += (uint32_t)(NEXTRA * NT)) {
-            const uint32_t p = perm[q];
-            const float x0[3] = {a.xin[0][p], a.xin[1][p], a.xin[2][p]};
-            const float v0[3] = {a.vin[0][p], a.vin[1][p], a.vin[2][p]};
+            const float x0[3] = {a.xin[0][q], a.xin[1][q], a.xin[2][q]};
+            const float v0[3] = {a.vin[0][q], a.vin[1][q], a.vin[2][q]};
             float k1[3], vo[3], vnew[3], x1[3];
             sample_mac<ST, true>(g, a.old[0], a.old[1], a.old[2], x0[0], x0[1], x0[2], k1, vo);
             flip_blend(a, v0, k1, vo, vnew);
             rk_step<ST>(g, 4, a.rk, x0, k1, x1);
-            const int64_t o = a.keep_order ? (int64_t)p : (int64_t)q;
-#pragma unroll
-            for (int c = 0; c < 3; c++) {
-                a.xout[c][o] = x1[c];
-                a.vout[c][o] = vnew[c];
-            }
-            if (a.id_out) a.id_out[o] = a.id_in ? a.id_in[p] : p;
+            store_particle(a, perm, q, x1, vnew);
         }
         return;
     }
@@ -1542,9 +1671,8 @@ __global__ __launch_bounds__(gbrick::NT) void k_flip_g2p_brick(
             else hi = mid - 1;
         }
         const uint32_t q = s_rowq[lo] + (s - s_rows0[lo]);
-        const uint32_t p = perm[q];
-        const float x0[3] = {a.xin[0][p], a.xin[1][p], a.xin[2][p]};
-        const float v0[3] = {a.vin[0][p], a.vin[1][p], a.vin[2][p]};
+        const float x0[3] = {a.xin[0][q], a.xin[1][q], a.xin[2][q]};
+        const float v0[3] = {a.vin[0][q], a.vin[1][q], a.vin[2][q]};
         float k1[3], vo[3], vnew[3];
         brick_sample<ST, true>(g, a.old[0], a.old[1], a.old[2], s_cur[0], s_cur[1], s_cur[2],
                                s_old[0], s_old[1], s_old[2], lx, ly, lz, lox, loy, loz,
@@ -1570,13 +1698,7 @@ __global__ __launch_bounds__(gbrick::NT) void k_flip_g2p_brick(
 #pragma unroll
         for (int c = 0; c < 3; c++)
             x1[c] = x0[c] + a.rk.sixth * (((k1[c] + 2.0f * k2[c]) + 2.0f * k3[c]) + k4[c]);
-        const int64_t o = a.keep_order ? (int64_t)p : (int64_t)q;
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-            a.xout[c][o] = x1[c];
-            a.vout[c][o] = vnew[c];
-        }
-        if (a.id_out) a.id_out[o] = a.id_in ? a.id_in[p] : p;
+        store_particle(a, perm, q, x1, vnew);
     }
 }
 
@@ -1593,15 +1715,27 @@ void flip_step(const StepArgs &s, const SortBuffers &sb, hipStream_t st,
     bg.nbin = (uint32_t)bg.b[0] * (uint32_t)bg.b[1] * (uint32_t)bg.b[2];
     // cells ordered inside by (octant, index): each half-cell is a run
     cell_sort(nullptr, s.xin[0], s.xin[1], s.xin[2], s.n, bg, sb, st, true);
+    if (s.n > 0) {
+        SortedCopy sc;
+        for (int c = 0; c < 3; c++) {
+            sc.src[c] = s.xin[c];
+            sc.src[3 + c] = s.vin[c];
+            sc.dst[c] = s.xs[c];
+            sc.dst[3 + c] = s.vs[c];
+        }
+        hipLaunchKernelGGL(k_sorted_copy, dim3(grid_for(s.n, kBlock)), dim3(kBlock), 0, st,
+                           sc, (const uint32_t *)sb.perm, s.n);
+        GFSX_LAUNCH_CHECK();
+    }
     if (ev) GFSX_HIP(hipEventRecord(ev[1], st));
 
     StepP2G a;
-    a.x = s.xin[0];
-    a.y = s.xin[1];
-    a.z = s.xin[2];
+    a.x = s.xs[0];
+    a.y = s.xs[1];
+    a.z = s.xs[2];
     const float half = (float)(0.5 * s.dx);
     for (int c = 0; c < 3; c++) {
-        a.vel[c] = s.vin[c];
+        a.vel[c] = s.vs[c];
         a.grid[c] = s.grid[c];
         a.grid_old[c] = s.grid_old[c];
         a.wgt[c] = s.wgt[c];
@@ -1633,16 +1767,30 @@ void flip_step(const StepArgs &s, const SortBuffers &sb, hipStream_t st,
         const int64_t nbrick = (int64_t)nbx * nby * nbz;
         const int per_xcd = (int)((nbrick + 7) / 8);
         const dim3 grid((unsigned)(8 * (int64_t)per_xcd)), block(brick::NT);
-        auto *kern = s.p2g_variant == 2 ? k_flip_p2g_brick<2>
-                   : s.p2g_variant == 3 ? k_flip_p2g_brick<3> : k_flip_p2g_brick<0>;
+        constexpr int cap = brick::CAP;
+        auto *kern = s.p2g_variant == 2 ? k_flip_p2g_brick<2, cap, false>
+                   : s.p2g_variant == 3 ? k_flip_p2g_brick<3, cap, false>
+                                        : k_flip_p2g_brick<0, cap, false>;
+        auto *kern_dense = s.p2g_variant == 2 ? k_flip_p2g_brick<2, cap, true>
+                         : s.p2g_variant == 3 ? k_flip_p2g_brick<3, cap, true>
+                                              : k_flip_p2g_brick<0, cap, true>;
+        fill_u32(s.dense_count, 0u, 1, st);
         hipLaunchKernelGGL(kern, grid, block, 0, st, a, bg, s.dx, inv_r2, nbx, nby, nbz,
                            per_xcd, (const uint32_t *)sb.count,
-                           (const uint32_t *)sb.start, (const uint32_t *)sb.perm);
+                           (const uint32_t *)sb.start, (const uint32_t *)sb.hrank,
+                           s.dense, s.dense_count);
+        GFSX_LAUNCH_CHECK();
+        // bricks holding more than cap particles, from the dense list
+        const unsigned ndense = (unsigned)std::min<int64_t>(nbrick, brick::DENSE_BLOCKS);
+        hipLaunchKernelGGL(kern_dense, dim3(ndense), block, 0, st, a, bg, s.dx, inv_r2, nbx,
+                           nby, nbz, per_xcd, (const uint32_t *)sb.count,
+                           (const uint32_t *)sb.start, (const uint32_t *)sb.hrank,
+                           s.dense, s.dense_count);
     } else {
         const int64_t F = a.nface[0] + a.nface[1] + a.nface[2];
         hipLaunchKernelGGL(k_flip_p2g, dim3(grid_for(F, kBlock)), dim3(kBlock), 0,
                            st, a, bg, s.dx, inv_r2, (const uint32_t *)sb.count,
-                           (const uint32_t *)sb.start, (const uint32_t *)sb.perm);
+                           (const uint32_t *)sb.start, (const uint32_t *)nullptr);
     }
     GFSX_LAUNCH_CHECK();
     if (ev) GFSX_HIP(hipEventRecord(ev[2], st));
@@ -1663,8 +1811,8 @@ void flip_step(const StepArgs &s, const SortBuffers &sb, hipStream_t st,
     g.lim[2] = s.dx * s.nk;
     StepG2P b;
     for (int c = 0; c < 3; c++) {
-        b.xin[c] = s.xin[c];
-        b.vin[c] = s.vin[c];
+        b.xin[c] = s.xs[c];
+        b.vin[c] = s.vs[c];
         b.xout[c] = s.xout[c];
         b.vout[c] = s.vout[c];
         b.old[c] = s.grid_old[c];

@@ -99,8 +99,9 @@ struct gfsx_ctx {
         sb.perm_tmp = u32(S_PERM_TMP, n);
         sb.hrank = u32(S_HRANK, n);
         // every big bin holds more than kSmallBin particles
-        sb.big_list = u32(S_BIG, std::min<int64_t>((int64_t)nbin, n / 16 + 1));
-        sb.big_count = u32(S_BIGN, 1);
+        sb.big_cap = (uint32_t)std::min<int64_t>((int64_t)nbin, n / 16 + 1);
+        sb.big_list = u32(S_BIG, sb.big_cap);
+        sb.big_count = u32(S_BIGN, 2);
         sb.block_out = u32(S_BOUT, nb);
         sb.block_out_scan = u32(S_BOUTS, nb + 1);
         int64_t words = std::max(gfsx::scan_scratch_words((int64_t)nbin + 1),

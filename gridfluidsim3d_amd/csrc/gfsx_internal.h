@@ -101,8 +101,9 @@ struct SortBuffers {
     uint32_t *perm;       // n: the result
     uint32_t *perm_tmp;   // n: bins filled, not yet ordered inside
     uint32_t *hrank;      // n (octant order only): rank inside the half-cell
-    uint32_t *big_list;   // bins too large to order in registers
-    uint32_t *big_count;  // 1
+    uint32_t *big_list;   // big_cap entries: bins too large for one thread
+    uint32_t big_cap;
+    uint32_t *big_count;  // 2
     uint32_t *block_out;  // blocks(n)
     uint32_t *block_out_scan;  // blocks(n)+1
     uint32_t *scan_scratch;

@@ -326,18 +326,21 @@ __device__ __forceinline__ void write_sorted(const uint64_t (&v)[N], uint32_t s,
     }
 }
 
-constexpr uint32_t kSmallBin = 16;   // bins up to this size sort in registers
+constexpr uint32_t kSmallBin = 16;   // bins up to this size: a thread, in registers
+constexpr uint32_t kMidBin = 256;    // bins up to this size: a wave, in registers
 
 // Order every in-grid bin's entries by key (see order_key), out of place
 // from perm_in to perm_out.  A thread owns a bin: bins of up to kSmallBin
 // entries are sorted in registers by a bitonic network (the network size is
 // the largest bin of the wave, rounded up to 8 or 16); larger bins are
-// appended to big_list for k_order_big.
+// listed for k_order_mid (up to kMidBin entries, from the front of
+// big_list, count in big_count[0]) and k_order_big (from the back, count in
+// big_count[1]).
 __global__ __launch_bounds__(kBlock) void k_order_bins(
     const uint32_t *__restrict__ start, uint32_t nbin,
     const uint32_t *__restrict__ perm_in, const uint8_t *__restrict__ poct,
     uint32_t *__restrict__ perm_out, uint32_t *__restrict__ hrank,
-    uint32_t *__restrict__ big_list, uint32_t *__restrict__ big_count) {
+    uint32_t *__restrict__ big_list, uint32_t big_cap, uint32_t *__restrict__ big_count) {
     const uint32_t b = blockIdx.x * kBlock + threadIdx.x;
     uint32_t s = 0, len = 0;
     if (b < nbin) {
@@ -345,7 +348,10 @@ __global__ __launch_bounds__(kBlock) void k_order_bins(
         len = start[b + 1] - s;
     }
     if (len > kSmallBin) {
-        big_list[atomicAdd(big_count, 1u)] = b;
+        // the wave-sorted sizes go to the front of the list, the rest to
+        // the back
+        if (len <= kMidBin) big_list[atomicAdd(&big_count[0], 1u)] = b;
+        else big_list[big_cap - 1u - atomicAdd(&big_count[1], 1u)] = b;
         len = 0;
     }
     if (len == 1) {
@@ -372,10 +378,117 @@ __global__ __launch_bounds__(kBlock) void k_order_bins(
     }
 }
 
+// Bitonic sort of the 64*K keys held by a wave, element e = lane*K + r in
+// v[r]: pairs inside a lane swap registers, pairs across lanes exchange
+// through a lane shuffle and keep the minimum or the maximum.
+template <int K>
+__device__ __forceinline__ void wave_bitonic(uint64_t (&v)[K]) {
+    constexpr int N = 64 * K;
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int k = 2; k <= N; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            if (j < K) {
+#pragma unroll
+                for (int r = 0; r < K; r++) {
+                    const int l = r ^ j;
+                    if (l > r) {
+                        const int e = lane * K + r;
+                        const bool up = (e & k) == 0;
+                        const uint64_t a = v[r], b = v[l];
+                        const bool sw = up ? (a > b) : (a < b);
+                        v[r] = sw ? b : a;
+                        v[l] = sw ? a : b;
+                    }
+                }
+            } else {
+                const int lx = j / K;
+#pragma unroll
+                for (int r = 0; r < K; r++) {
+                    const int e = lane * K + r;
+                    const uint32_t lo = __shfl_xor((uint32_t)v[r], lx, 64);
+                    const uint32_t hi = __shfl_xor((uint32_t)(v[r] >> 32), lx, 64);
+                    const uint64_t o = ((uint64_t)hi << 32) | lo;
+                    const bool up = (e & k) == 0, lower = (e & j) == 0;
+                    const bool keep_min = up == lower;
+                    const uint64_t mn = v[r] < o ? v[r] : o, mx = v[r] < o ? o : v[r];
+                    v[r] = keep_min ? mn : mx;
+                }
+            }
+        }
+    }
+}
+
+// Sort one bin of 16 < len <= 64*K entries with the calling wave and write
+// it out (perm_out, and hrank when given).
+template <int K>
+__device__ __forceinline__ void wave_order_bin(uint32_t s, uint32_t len,
+                                               const uint32_t *__restrict__ perm_in,
+                                               const uint8_t *__restrict__ poct,
+                                               uint32_t *__restrict__ perm_out,
+                                               uint32_t *__restrict__ hrank) {
+    const int lane = threadIdx.x & 63;
+    uint64_t v[K];
+#pragma unroll
+    for (int r = 0; r < K; r++) {
+        const uint32_t e = (uint32_t)(lane * K + r);
+        v[r] = e < len ? order_key(perm_in, poct, s + e) : ~0ull;
+    }
+    wave_bitonic<K>(v);
+    // first element of each octant run: the running maximum of the run
+    // starts seen so far, over registers then lanes
+    const uint32_t prev_hi = __shfl_up((uint32_t)(v[K - 1] >> 32), 1, 64);
+    uint32_t rs[K];
+    uint32_t m = 0;
+#pragma unroll
+    for (int r = 0; r < K; r++) {
+        const uint32_t e = (uint32_t)(lane * K + r);
+        const uint32_t before = r > 0 ? (uint32_t)(v[r - 1] >> 32) : prev_hi;
+        const bool first = e == 0 || before != (uint32_t)(v[r] >> 32);
+        m = first ? e : m;
+        rs[r] = m;
+    }
+    // carry the last run start of lower lanes into this lane's registers
+    uint32_t carry = m;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t t = __shfl_up(carry, d, 64);
+        if (lane >= d) carry = max(carry, t);
+    }
+    const uint32_t from_below = lane > 0 ? __shfl_up(carry, 1, 64) : 0u;
+#pragma unroll
+    for (int r = 0; r < K; r++) {
+        const uint32_t e = (uint32_t)(lane * K + r);
+        if (e < len) {
+            perm_out[s + e] = (uint32_t)v[r];
+            if (hrank) hrank[s + e] = e - max(rs[r], from_below);
+        }
+    }
+}
+
+// Bins of kSmallBin < len <= kMidBin entries: one wave per bin, sorted in
+// registers (64 or 256 keys per wave).
+__global__ __launch_bounds__(kBlock) void k_order_mid(
+    const uint32_t *__restrict__ start, const uint32_t *__restrict__ perm_in,
+    const uint8_t *__restrict__ poct, uint32_t *__restrict__ perm_out,
+    uint32_t *__restrict__ hrank, const uint32_t *__restrict__ big_list,
+    const uint32_t *__restrict__ big_count) {
+    const uint32_t nmid = big_count[0];
+    const uint32_t waves = gridDim.x * (kBlock / 64);
+    for (uint32_t c = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); c < nmid; c += waves) {
+        const uint32_t b = big_list[c];
+        const uint32_t s = start[b], len = start[b + 1] - s;
+        if (len <= 64u) wave_order_bin<1>(s, len, perm_in, poct, perm_out, hrank);
+        else wave_order_bin<4>(s, len, perm_in, poct, perm_out, hrank);
+    }
+}
+
 constexpr int kBigTile = 2048;   // keys per LDS tile in k_order_big
 constexpr int kOrderBigBlocks = 1024;
+constexpr int kOrderMidBlocks = 4096;
 
-// The bins k_order_bins left: one wave per bin ranks every entry by
+// Bins of more than kMidBin entries: one wave per bin ranks every entry by
 // counting the smaller keys of its bin (tiles of the keys pass through LDS,
 // so any size works).  Also copies the overflow bin, which k_bin already
 // filled in index order.
@@ -383,15 +496,16 @@ __global__ __launch_bounds__(64) void k_order_big(
     const uint32_t *__restrict__ start, uint32_t nbin,
     const uint32_t *__restrict__ perm_in, const uint8_t *__restrict__ poct,
     uint32_t *__restrict__ perm_out, uint32_t *__restrict__ hrank,
-    const uint32_t *__restrict__ big_list, const uint32_t *__restrict__ big_count) {
+    const uint32_t *__restrict__ big_list, uint32_t big_cap,
+    const uint32_t *__restrict__ big_count) {
     __shared__ uint64_t tile[kBigTile];
     const uint32_t lane = threadIdx.x;
     const uint32_t o0 = start[nbin], o1 = start[nbin + 1];
     for (uint32_t q = o0 + blockIdx.x * 64u + lane; q < o1; q += gridDim.x * 64u)
         perm_out[q] = perm_in[q];
-    const uint32_t nbig = *big_count;
+    const uint32_t nbig = big_count[1];
     for (uint32_t c = blockIdx.x; c < nbig; c += gridDim.x) {
-        const uint32_t b = big_list[c];
+        const uint32_t b = big_list[big_cap - 1u - c];
         const uint32_t s = start[b], len = start[b + 1] - s;
         for (uint32_t e0 = 0; e0 < len; e0 += 64u) {
             const uint32_t e = e0 + lane;
@@ -446,16 +560,23 @@ void cell_sort(const float *aos, const float *x, const float *y,
                        (const uint32_t *)sb.block_out_scan, g.nbin, sb.perm_tmp,
                        (const uint8_t *)oct, poct);
     GFSX_LAUNCH_CHECK();
-    fill_u32(sb.big_count, 0u, 1, s);
+    fill_u32(sb.big_count, 0u, 2, s);
     uint32_t *hrank = octant_order ? sb.hrank : nullptr;
     hipLaunchKernelGGL(k_order_bins, dim3(grid_for(g.nbin, kBlock)), dim3(kBlock), 0, s,
                        (const uint32_t *)sb.start, g.nbin, (const uint32_t *)sb.perm_tmp,
-                       (const uint8_t *)poct, sb.perm, hrank, sb.big_list, sb.big_count);
+                       (const uint8_t *)poct, sb.perm, hrank, sb.big_list, sb.big_cap,
+                       sb.big_count);
+    GFSX_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_order_mid, dim3(kOrderMidBlocks), dim3(kBlock), 0, s,
+                       (const uint32_t *)sb.start, (const uint32_t *)sb.perm_tmp,
+                       (const uint8_t *)poct, sb.perm, hrank, (const uint32_t *)sb.big_list,
+                       (const uint32_t *)sb.big_count);
     GFSX_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_order_big, dim3(kOrderBigBlocks), dim3(64), 0, s,
                        (const uint32_t *)sb.start, g.nbin, (const uint32_t *)sb.perm_tmp,
                        (const uint8_t *)poct, sb.perm, hrank,
-                       (const uint32_t *)sb.big_list, (const uint32_t *)sb.big_count);
+                       (const uint32_t *)sb.big_list, sb.big_cap,
+                       (const uint32_t *)sb.big_count);
     GFSX_LAUNCH_CHECK();
 }
 
@@ -932,6 +1053,7 @@ constexpr int NT = 3 * NCELL;             // 192 threads
 // cell stages 6x6x6 cells * 8 = 1728.
 constexpr int CAP = 2016;
 constexpr int DENSE_BLOCKS = 2048;   // work-groups of the pass over dense bricks
+constexpr int DENSE_CAP = 1344;      // its chunk size (smaller: more work-groups per CU)
 static_assert(NCELL == 64, "one wave per component");
 }  // namespace brick
 
@@ -1307,16 +1429,23 @@ __device__ __forceinline__ void p2g_brick(
         __syncthreads();
 
         // gather: wave w accumulates component w for its 64 faces, then the
-        // domain-boundary plane if this brick has one
+        // domain-boundary plane if this brick has one.  (In the chunk loop
+        // the face coordinates are hidden from the optimiser, so that it does
+        // not keep every range address of both faces live across chunks.)
+        int fli = li, flj = lj, flk = lk, feli = eli, felj = elj, felk = elk;
+        if (CHUNKED) {
+            asm volatile("" : "+v"(fli), "+v"(flj), "+v"(flk));
+            asm volatile("" : "+v"(feli), "+v"(felj), "+v"(felk));
+        }
         if (wave == 0) {
-            if (own_ok) brick_accum<0, V>(a, s_x, s_y, s_z, s_v[0], s_hs, i0, j0, k0, li, lj, lk, dx, inv_r2, S, W);
-            if (ext_ok) brick_accum<0, V>(a, s_x, s_y, s_z, s_v[0], s_hs, i0, j0, k0, eli, elj, elk, dx, inv_r2, SE, WE);
+            if (own_ok) brick_accum<0, V>(a, s_x, s_y, s_z, s_v[0], s_hs, i0, j0, k0, fli, flj, flk, dx, inv_r2, S, W);
+            if (ext_ok) brick_accum<0, V>(a, s_x, s_y, s_z, s_v[0], s_hs, i0, j0, k0, feli, felj, felk, dx, inv_r2, SE, WE);
         } else if (wave == 1) {
-            if (own_ok) brick_accum<1, V>(a, s_x, s_y, s_z, s_v[1], s_hs, i0, j0, k0, li, lj, lk, dx, inv_r2, S, W);
-            if (ext_ok) brick_accum<1, V>(a, s_x, s_y, s_z, s_v[1], s_hs, i0, j0, k0, eli, elj, elk, dx, inv_r2, SE, WE);
+            if (own_ok) brick_accum<1, V>(a, s_x, s_y, s_z, s_v[1], s_hs, i0, j0, k0, fli, flj, flk, dx, inv_r2, S, W);
+            if (ext_ok) brick_accum<1, V>(a, s_x, s_y, s_z, s_v[1], s_hs, i0, j0, k0, feli, felj, felk, dx, inv_r2, SE, WE);
         } else {
-            if (own_ok) brick_accum<2, V>(a, s_x, s_y, s_z, s_v[2], s_hs, i0, j0, k0, li, lj, lk, dx, inv_r2, S, W);
-            if (ext_ok) brick_accum<2, V>(a, s_x, s_y, s_z, s_v[2], s_hs, i0, j0, k0, eli, elj, elk, dx, inv_r2, SE, WE);
+            if (own_ok) brick_accum<2, V>(a, s_x, s_y, s_z, s_v[2], s_hs, i0, j0, k0, fli, flj, flk, dx, inv_r2, S, W);
+            if (ext_ok) brick_accum<2, V>(a, s_x, s_y, s_z, s_v[2], s_hs, i0, j0, k0, feli, felj, felk, dx, inv_r2, SE, WE);
         }
         if (!CHUNKED) break;
     }
@@ -1324,7 +1453,7 @@ __device__ __forceinline__ void p2g_brick(
     if (ext_ok) face_epilogue(a, bg, count, wave, i0 + eli, j0 + elj, k0 + elk, SE, WE);
 }
 
-template <int V, int CAP, bool CHUNKED>
+template <int V, int CAP>
 __global__ __launch_bounds__(brick::NT) void k_flip_p2g_brick(
     StepP2G a, BinGrid bg, double dx, float inv_r2, int nbx, int nby, int nbz,
     int per_xcd, const uint32_t *__restrict__ count,
@@ -1337,21 +1466,34 @@ __global__ __launch_bounds__(brick::NT) void k_flip_p2g_brick(
     __shared__ uint16_t s_hs[NH + 1];
     __shared__ uint32_t s_rowq[NROW], s_rows0[NROW + 1];
     __shared__ uint32_t s_wsum[NT / 64];
-    if (!CHUNKED) {
-        // consecutive bricks share halo particles: keep them on one XCD
-        // (work-groups are dealt round-robin over the 8 XCDs)
-        const int brick_id = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
-        if (brick_id >= nbx * nby * nbz) return;
-        p2g_brick<V, CAP, false>(a, bg, dx, inv_r2, nbx, nby, brick_id, count, start, hrank,
-                                 dense, dense_count, s_buf, s_hs, s_rowq, s_rows0, s_wsum);
-    } else {
-        const uint32_t nd = *dense_count;
-        for (uint32_t d = blockIdx.x; d < nd; d += gridDim.x) {
-            __syncthreads();   // the previous brick is done with LDS
-            p2g_brick<V, CAP, true>(a, bg, dx, inv_r2, nbx, nby, (int)dense[d], count, start,
-                                    hrank, dense, dense_count, s_buf, s_hs, s_rowq, s_rows0,
-                                    s_wsum);
-        }
+    // consecutive bricks share halo particles: keep them on one XCD
+    // (work-groups are dealt round-robin over the 8 XCDs)
+    const int brick_id = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
+    if (brick_id >= nbx * nby * nbz) return;
+    p2g_brick<V, CAP, false>(a, bg, dx, inv_r2, nbx, nby, brick_id, count, start, hrank,
+                             dense, dense_count, s_buf, s_hs, s_rowq, s_rows0, s_wsum);
+}
+
+// The bricks listed by k_flip_p2g_brick as over its capacity, in chunks.
+// Three waves per SIMD (a register budget of 168).
+template <int V, int CAP>
+__global__ __launch_bounds__(brick::NT, 3) void k_flip_p2g_dense(
+    StepP2G a, BinGrid bg, double dx, float inv_r2, int nbx, int nby,
+    const uint32_t *__restrict__ count, const uint32_t *__restrict__ start,
+    const uint32_t *__restrict__ hrank, uint32_t *__restrict__ dense,
+    uint32_t *__restrict__ dense_count) {
+    using namespace brick;
+    static_assert(NH + 1 <= 6 * CAP, "half-cell counts live in the particle arrays");
+    static_assert(CAP <= 65535, "LDS slots are 16-bit");
+    __shared__ float s_buf[6 * CAP];
+    __shared__ uint16_t s_hs[NH + 1];
+    __shared__ uint32_t s_rowq[NROW], s_rows0[NROW + 1];
+    __shared__ uint32_t s_wsum[NT / 64];
+    const uint32_t nd = *dense_count;
+    for (uint32_t d = blockIdx.x; d < nd; d += gridDim.x) {
+        __syncthreads();   // the previous brick is done with LDS
+        p2g_brick<V, CAP, true>(a, bg, dx, inv_r2, nbx, nby, (int)dense[d], count, start,
+                                hrank, dense, dense_count, s_buf, s_hs, s_rowq, s_rows0, s_wsum);
     }
 }
 
@@ -1768,12 +1910,12 @@ void flip_step(const StepArgs &s, const SortBuffers &sb, hipStream_t st,
         const int per_xcd = (int)((nbrick + 7) / 8);
         const dim3 grid((unsigned)(8 * (int64_t)per_xcd)), block(brick::NT);
         constexpr int cap = brick::CAP;
-        auto *kern = s.p2g_variant == 2 ? k_flip_p2g_brick<2, cap, false>
-                   : s.p2g_variant == 3 ? k_flip_p2g_brick<3, cap, false>
-                                        : k_flip_p2g_brick<0, cap, false>;
-        auto *kern_dense = s.p2g_variant == 2 ? k_flip_p2g_brick<2, cap, true>
-                         : s.p2g_variant == 3 ? k_flip_p2g_brick<3, cap, true>
-                                              : k_flip_p2g_brick<0, cap, true>;
+        auto *kern = s.p2g_variant == 2 ? k_flip_p2g_brick<2, cap>
+                   : s.p2g_variant == 3 ? k_flip_p2g_brick<3, cap> : k_flip_p2g_brick<0, cap>;
+        constexpr int dcap = brick::DENSE_CAP;
+        auto *kern_dense = s.p2g_variant == 2 ? k_flip_p2g_dense<2, dcap>
+                         : s.p2g_variant == 3 ? k_flip_p2g_dense<3, dcap>
+                                              : k_flip_p2g_dense<0, dcap>;
         fill_u32(s.dense_count, 0u, 1, st);
         hipLaunchKernelGGL(kern, grid, block, 0, st, a, bg, s.dx, inv_r2, nbx, nby, nbz,
                            per_xcd, (const uint32_t *)sb.count,
@@ -1783,9 +1925,8 @@ void flip_step(const StepArgs &s, const SortBuffers &sb, hipStream_t st,
         // bricks holding more than cap particles, from the dense list
         const unsigned ndense = (unsigned)std::min<int64_t>(nbrick, brick::DENSE_BLOCKS);
         hipLaunchKernelGGL(kern_dense, dim3(ndense), block, 0, st, a, bg, s.dx, inv_r2, nbx,
-                           nby, nbz, per_xcd, (const uint32_t *)sb.count,
-                           (const uint32_t *)sb.start, (const uint32_t *)sb.hrank,
-                           s.dense, s.dense_count);
+                           nby, (const uint32_t *)sb.count, (const uint32_t *)sb.start,
+                           (const uint32_t *)sb.hrank, s.dense, s.dense_count);
     } else {
         const int64_t F = a.nface[0] + a.nface[1] + a.nface[2];
         hipLaunchKernelGGL(k_flip_p2g, dim3(grid_for(F, kBlock)), dim3(kBlock), 0,

@@ -56,6 +56,48 @@ def test_cell_sort_bit_exact(ctx, orc, seed, n):
     assert np.array_equal(perm[m:], outside)
 
 
+def _clustered(rng, dims, dx, sizes, background):
+    """Positions with one cluster per entry of sizes (all in one cell, at
+    cells spread over the grid) plus uniform background particles."""
+    parts = [rng.uniform(-0.15, np.array(dims) * dx + 0.15, size=(background, 3))]
+    for c, m in enumerate(sizes):
+        cell = np.array([(3 * c + 1) % dims[0], (5 * c + 2) % dims[1], (7 * c + 3) % dims[2]])
+        parts.append((cell + rng.uniform(0.01, 0.99, size=(m, 3))) * dx)
+    pos = np.concatenate(parts).astype(np.float32)
+    return pos[rng.permutation(len(pos))]
+
+
+def test_cell_sort_bin_size_classes(ctx, orc):
+    """Bins ordered by a thread (up to 16), a wave (17..256) and a wave over
+    LDS tiles (beyond, including more than one 2048-key tile)."""
+    rng = np.random.default_rng(41)
+    dims = (11, 9, 10)
+    dx = 0.1
+    pos = _clustered(rng, dims, dx, [17, 40, 64, 65, 130, 256, 257, 1500, 4500], 3000)
+    perm_ref, start_ref = orc.cell_sort(pos, dx, dims)
+    x, y, z = (_dev(pos[:, a].copy()) for a in range(3))
+    perm, start = ctx.cell_sort_device(x, y, z, dx, dims)
+    perm, start = _np(perm), _np(start)
+    assert np.array_equal(start, start_ref.astype(np.int64))
+    m = start_ref[dims[0] * dims[1] * dims[2]]
+    assert np.array_equal(perm[:m], perm_ref.astype(np.int64))
+
+
+def test_step_bin_size_classes(ctx, orc):
+    """The fused step on cells of every ordering class and bricks that need
+    several LDS chunks, against the oracle."""
+    rng = np.random.default_rng(43)
+    dims = (14, 12, 13)
+    dx = 0.1
+    pos = _clustered(rng, dims, dx, [20, 70, 300, 2600, 5000], 6000)
+    vel = rng.normal(size=pos.shape).astype(np.float32)
+    exp = orc.flip_step(pos, vel, dims, dx, 1 / 240, (0.0, -9.8, 0.0), 0.05, TRILINEAR)
+    for keep in (0, 1):
+        res = _run_step(ctx, pos, vel, dims, dx, 1 / 240, (0.0, -9.8, 0.0), 0.05, TRILINEAR,
+                        keep)
+        _check_step(res, exp, bool(keep))
+
+
 def test_cell_sort_nan_and_empty(ctx):
     torch = _torch()
     dims = (4, 4, 4)

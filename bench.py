@@ -38,13 +38,26 @@ def face_count(dims):
 
 def algorithmic_bytes(P, F, C):
     """Bytes each stage must move at minimum (FP32 / uint32):
-    sort: read positions (12P) + write the permutation (4P) + one pass over
-          the cell counts (8C);
-    p2g : read x, v once (24P) + write one value per face (4F);
-    g2p : read x, v (24P) + write x', v' (24P) + read one value per face (4F);
-    step: the whole step, 48P + 8F."""
-    return dict(sort=16 * P + 8 * C, p2g=24 * P + 4 * F, g2p=48 * P + 4 * F,
+    sort: read positions and velocities (24P), write them in cell order
+          (24P) with the permutation and the in-half-cell ranks (8P), one
+          pass over the cell counts and starts (8C);
+    p2g : read x, v once (24P) + write the field and its pre-force
+          snapshot, one value per face each (8F);
+    g2p : read x, v (24P) + write x', v' (24P) + read the field and the
+          snapshot once (8F);
+    step: the whole step: read x, v and write x', v' (48P), write the field
+          and read it back (8F)."""
+    return dict(sort=56 * P + 8 * C, p2g=24 * P + 8 * F, g2p=48 * P + 8 * F,
                 step=48 * P + 8 * F)
+
+
+# The kernels each stage of gfsx_flip_step launches (HIP events bracket the
+# stage on the library's stream).
+STAGE_KERNELS = {
+    "sort": "cell sort: k_bin, scans, k_scatter, k_order_bins/mid/big, k_sorted_copy",
+    "p2g": "k_flip_p2g_brick + k_flip_p2g_dense",
+    "g2p": "k_flip_g2p_brick",
+}
 
 
 def kernel_source_hash():
@@ -133,7 +146,7 @@ def _cpu_model():
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="pool256",
                     choices=["pool256", "dambreak128", "sphere64"])
@@ -169,14 +182,19 @@ def main():
 
     ctx = gf.Context(0)
     log(ctx.device_info)
-    st = gf.FlipState(scene.pos, scene.vel, scene.dims, scene.dx)
     stream = torch.cuda.current_stream()
 
-    def step():
+    def step(st):
         ctx.flip_step(st, scene.dt, scene.body_force, 0.05, stencil, gf.ORDER_SORTED)
 
+    # warm-up on a scratch copy; the timed run starts from the scene's
+    # initial state and advances it args.steps times
+    st = gf.FlipState(scene.pos, scene.vel, scene.dims, scene.dx)
     for _ in range(args.warmup):
-        step()
+        step(st)
+    torch.cuda.synchronize()
+    del st
+    st = gf.FlipState(scene.pos, scene.vel, scene.dims, scene.dx)
     torch.cuda.synchronize()
     log("warmup done (%d steps)" % args.warmup)
 
@@ -186,7 +204,7 @@ def main():
     torch.cuda.synchronize()
     start.record(stream)
     for _ in range(args.steps):
-        step()
+        step(st)
     end.record(stream)
     torch.cuda.synchronize()
     elapsed_ms = start.elapsed_time(end)
@@ -200,12 +218,10 @@ def main():
     alg = algorithmic_bytes(P, F, C)
     dom = max(("sort", "p2g", "g2p"), key=lambda k: stage_avg[k])
     achieved = alg[dom] / (stage_avg[dom] / 1e3) / 1e9
-    roofline = dict(bound="hbm", kernel=dict(sort="cell sort (7 kernels)", p2g="k_flip_p2g_brick",
-                                             g2p="k_flip_g2p")[dom],
+    roofline = dict(bound="hbm", kernel=STAGE_KERNELS[dom],
                     achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 4),
-                    traffic=measured_traffic(args.workload, dict(sort="sort", p2g="k_flip_p2g_brick",
-                                                                 g2p="k_flip_g2p")[dom]),
+                    traffic=measured_traffic(args.workload, dom),
                     algorithmic_bytes=alg[dom], avg_ms=round(stage_avg[dom], 4))
     step_gbs = alg["step"] / (ms_per_step / 1e3) / 1e9
     step_roofline = dict(achieved=round(step_gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",

@@ -33,7 +33,7 @@ void ok(int *err) {
 }
 
 enum Slot {
-    S_COUNT, S_START, S_KEY, S_SLOT, S_PERM, S_PERM_TMP, S_HRANK, S_BIG, S_BIGN,
+    S_COUNT, S_START, S_KEY, S_SLOT, S_PERM, S_PERM_TMP, S_HRANK, S_SOCT, S_BFLAG, S_BIG, S_BIGN,
     S_BOUT, S_BOUTS, S_SCAN, S_OCT, S_POCT,
     S_OLD0, S_OLD1, S_OLD2, S_XS0, S_XS1, S_XS2, S_VS0, S_VS1, S_VS2, S_DENSE, S_DENSEN,
     S_D_POS, S_D_VAL, S_D_S, S_D_W, S_D_U, S_D_V, S_D_WW, S_D_OUT,
@@ -51,6 +51,7 @@ struct gfsx_ctx {
     hipStream_t own_stream = nullptr;
     hipStream_t stream = nullptr;
     std::string info;
+    int num_cu = 1;
     void *buf[NSLOT] = {};
     size_t bytes[NSLOT] = {};
     int p2g_variant = 0;
@@ -98,6 +99,8 @@ struct gfsx_ctx {
         sb.perm = u32(S_PERM, n);
         sb.perm_tmp = u32(S_PERM_TMP, n);
         sb.hrank = u32(S_HRANK, n);
+        sb.soct = (uint8_t *)get(S_SOCT, (size_t)std::max<int64_t>(n, 1));
+        sb.brick_flag = nullptr;
         // every big bin holds more than kSmallBin particles
         sb.big_cap = (uint32_t)std::min<int64_t>((int64_t)nbin, n / 16 + 1);
         sb.big_list = u32(S_BIG, sb.big_cap);
@@ -163,9 +166,7 @@ gfsx::MacGrid mac_grid(const float *u, const float *v, const float *w, int ni,
     g.nk = nk;
     g.dx = dx;
     g.inv_dx = 1.0 / dx;
-    g.lim[0] = dx * ni;
-    g.lim[1] = dx * nj;
-    g.lim[2] = dx * nk;
+    gfsx::set_limits(g);
     return g;
 }
 
@@ -254,6 +255,7 @@ gfsx_ctx *gfsx_create(int device, int *err) {
            << "), " << prop.multiProcessorCount << " CUs, "
            << (prop.totalGlobalMem >> 20) << " MiB";
         c->info = ss.str();
+        c->num_cu = std::max(prop.multiProcessorCount, 1);
         ok(err);
         return c;
     } catch (const std::exception &e) {
@@ -512,7 +514,10 @@ void gfsx_flip_step(gfsx_ctx *ctx, const gfsx_scene *sc,
         a.keep_order = pr->order == GFSX_ORDER_KEEP;
         a.p2g_variant = ctx->p2g_variant;
         a.g2p_variant = ctx->g2p_variant;
+        a.num_cu = ctx->num_cu;
         gfsx::SortBuffers sb = ctx->sort_buffers(sc->n, (uint32_t)nbin);
+        // P2G brick flags, one byte per 4x4x4-cell brick
+        sb.brick_flag = (uint8_t *)ctx->get(S_BFLAG, (size_t)((ni + 3) / 4) * ((nj + 3) / 4) * ((nk + 3) / 4));
         gfsx::flip_step(a, sb, ctx->stream, ctx->timing ? ctx->next_events() : nullptr);
     });
 }

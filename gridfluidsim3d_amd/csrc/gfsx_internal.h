@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdint>
 #include <stdexcept>
 #include <string>
@@ -62,7 +63,21 @@ struct MacGrid {
     int ni, nj, nk;
     double dx, inv_dx;
     double lim[3];   // dx*ni, dx*nj, dx*nk (Grid3d::isPositionInGrid)
+    float limf[3];   // for float x, x < limf[a] exactly when (double)x < lim[a]
 };
+
+// Fill lim/limf of g from dx and the cell counts.  limf is the smallest
+// float not below lim: every float under it is under lim, every float at
+// or above it is at or above lim.
+inline void set_limits(MacGrid &g) {
+    const int n[3] = {g.ni, g.nj, g.nk};
+    for (int a = 0; a < 3; a++) {
+        g.lim[a] = g.dx * n[a];
+        float f = (float)g.lim[a];
+        if ((double)f < g.lim[a]) f = std::nextafter(f, INFINITY);
+        g.limf[a] = f;
+    }
+}
 
 __device__ __forceinline__ uint32_t bin_of(float x, float y, float z,
                                            const BinGrid &g) {
@@ -89,6 +104,7 @@ __device__ __forceinline__ float wyvill(float t) {
 
 void fill_u32(uint32_t *p, uint32_t value, int64_t count, hipStream_t s);
 void fill_f32(float *p, float value, int64_t count, hipStream_t s);
+void fill_u8(uint8_t *p, uint8_t value, int64_t count, hipStream_t s);
 int64_t scan_scratch_words(int64_t m);
 void exclusive_scan(const uint32_t *in, uint32_t *out, int64_t m,
                     uint32_t *scratch, hipStream_t s);
@@ -101,6 +117,8 @@ struct SortBuffers {
     uint32_t *perm;       // n: the result
     uint32_t *perm_tmp;   // n: bins filled, not yet ordered inside
     uint32_t *hrank;      // n (octant order only): rank inside the half-cell
+    uint8_t *soct;        // n (octant order only): octant, in perm order
+    uint8_t *brick_flag;  // (octant order only, nullable): P2G bricks with particles in reach
     uint32_t *big_list;   // big_cap entries: bins too large for one thread
     uint32_t big_cap;
     uint32_t *big_count;  // 2
@@ -118,7 +136,8 @@ int64_t sort_blocks(int64_t n);
 // With octant_order, particles inside a bin are ordered by (octant, index)
 // instead, the octant being the 3-bit index of the half of the cell the
 // particle lies in along x, y, z (so each half-cell is a contiguous run),
-// and hrank[q] receives the rank of perm[q] inside its half-cell run.
+// hrank[q] receives the rank of perm[q] inside its half-cell run and
+// soct[q] its octant.
 // aos != nullptr selects AoS input (x,y,z interleaved); otherwise SoA.
 void cell_sort(const float *aos, const float *x, const float *y,
                const float *z, int64_t n, const BinGrid &g,
@@ -157,6 +176,7 @@ struct StepArgs {
     int keep_order;
     int p2g_variant;   // 0: brick-staged gather, 1: direct global gather
     int g2p_variant;   // 0: brick-staged LDS boxes, 1: direct global sampling
+    int num_cu;        // compute units of the device (persistent launches)
 };
 
 // The fused plain step.  Needs SortBuffers sized for n particles and

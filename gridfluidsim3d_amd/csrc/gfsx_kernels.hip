@@ -18,6 +18,7 @@
 #include "gfsx_internal.h"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace gfsx {
 
@@ -46,6 +47,19 @@ void fill_u32(uint32_t *p, uint32_t value, int64_t count, hipStream_t s) {
     if (count <= 0) return;
     unsigned g = std::min<unsigned>(grid_for(count, kBlock), 8192u);
     hipLaunchKernelGGL(k_fill_u32, dim3(g), dim3(kBlock), 0, s, p, value, count);
+    GFSX_LAUNCH_CHECK();
+}
+
+__global__ void k_fill_u8(uint8_t *p, uint8_t v, int64_t n) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+
+void fill_u8(uint8_t *p, uint8_t value, int64_t count, hipStream_t s) {
+    if (count <= 0) return;
+    unsigned g = std::min<unsigned>(grid_for(count, kBlock), 8192u);
+    hipLaunchKernelGGL(k_fill_u8, dim3(g), dim3(kBlock), 0, s, p, value, count);
     GFSX_LAUNCH_CHECK();
 }
 
@@ -181,16 +195,24 @@ void exclusive_scan(const uint32_t *in, uint32_t *out, int64_t m,
 // Stable counting sort by bin
 // ===========================================================================
 
-int64_t sort_blocks(int64_t n) { return (n + kBlock - 1) / kBlock; }
+// The binning and scatter kernels take kBinItems particles per thread
+// (item m of a block is the consecutive range [m*kBlock, (m+1)*kBlock) of
+// the block's kBinItems*kBlock particles), so that every wave has that many
+// independent loads and atomics in flight.
+constexpr int kBinItems = 4;
+constexpr int64_t kBinTile = (int64_t)kBinItems * kBlock;
+
+int64_t sort_blocks(int64_t n) { return (n + kBinTile - 1) / kBinTile; }
 
 // Bin every particle.  In-grid particles take a slot in their bin with an
 // integer atomic; lanes of a wave that hold a run of equal bins (the input
 // is usually close to cell order) take one atomic per run and number
 // themselves within it in lane order.  The order inside a bin is then
-// fixed by k_order_bins / k_order_big.  Overflow particles get their rank among the
-// block's overflow particles from a wave ballot, so that bin is filled in
-// index order directly.  With an octant array, each particle also records
-// which half of its cell it occupies along each axis.
+// fixed by k_order_bins / k_order_big.  Overflow particles get their rank
+// among the block's overflow particles (in index order) from wave ballots,
+// so that bin is filled in index order directly.  With an octant array,
+// each particle also records which half of its cell it occupies along
+// each axis.
 template <bool AOS>
 __global__ __launch_bounds__(kBlock) void k_bin(const float *aos,
                                                const float *x, const float *y,
@@ -199,67 +221,82 @@ __global__ __launch_bounds__(kBlock) void k_bin(const float *aos,
                                                uint32_t *key, uint32_t *slot,
                                                uint32_t *block_out,
                                                uint8_t *oct) {
-    __shared__ uint32_t wcnt[kBlock / 64];
-    const int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x;
-    const bool valid = i < n;
-    uint32_t b = g.nbin;
-    uint8_t o = 0;
-    if (valid) {
-        float px, py, pz;
-        if (AOS) {
-            px = aos[3 * i];
-            py = aos[3 * i + 1];
-            pz = aos[3 * i + 2];
-        } else {
-            px = x[i];
-            py = y[i];
-            pz = z[i];
-        }
-        b = bin_of(px, py, pz, g);
-        if (oct) {
-            const double gx = (double)(px - g.off[0]) * g.inv_dx;
-            const double gy = (double)(py - g.off[1]) * g.inv_dx;
-            const double gz = (double)(pz - g.off[2]) * g.inv_dx;
-            o = (uint8_t)(((gx - floor(gx)) >= 0.5) | (((gy - floor(gy)) >= 0.5) << 1) |
-                          (((gz - floor(gz)) >= 0.5) << 2));
-        }
-    }
-    const bool over = valid && b == g.nbin;
-    const bool in = valid && !over;
+    __shared__ uint32_t wcnt[kBinItems][kBlock / 64];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-
-    // runs of equal in-grid bins among consecutive lanes
-    const uint32_t kk = in ? b : 0xFFFFFFFFu;
-    const uint32_t kprev = __shfl_up(kk, 1, 64);
-    const uint32_t knext = __shfl_down(kk, 1, 64);
-    const bool rstart = in && (lane == 0 || kprev != kk);
-    const bool rend = in && (lane == 63 || knext != kk);
-    const uint64_t S = __ballot(rstart), E = __ballot(rend);
-    const uint64_t le = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
-    const int s0 = in ? 63 - __clzll(S & le) : 0;
-    uint32_t base = 0;
-    if (rstart) {
-        const int e0 = __ffsll((unsigned long long)(E & ~((1ull << lane) - 1ull))) - 1;
-        base = atomicAdd(&count[b], (uint32_t)(e0 - lane + 1));
-    }
-    base = __shfl(base, s0, 64);
-    uint32_t sl = base + (uint32_t)(lane - s0);
-
-    const uint64_t mask = __ballot(over);
-    const uint32_t lrank = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-    if (lane == 0) wcnt[wid] = (uint32_t)__popcll(mask);
-    __syncthreads();
-    uint32_t woff = 0, tot = 0;
+    const int64_t base = (int64_t)blockIdx.x * kBinTile + threadIdx.x;
+    float px[kBinItems], py[kBinItems], pz[kBinItems];
 #pragma unroll
-    for (int w = 0; w < kBlock / 64; w++) {
-        woff += (w < wid) ? wcnt[w] : 0u;
-        tot += wcnt[w];
+    for (int m = 0; m < kBinItems; m++) {
+        const int64_t i = base + (int64_t)m * kBlock;
+        const int64_t ii = i < n ? i : 0;
+        if (AOS) {
+            px[m] = aos[3 * ii];
+            py[m] = aos[3 * ii + 1];
+            pz[m] = aos[3 * ii + 2];
+        } else {
+            px[m] = x[ii];
+            py[m] = y[ii];
+            pz[m] = z[ii];
+        }
     }
-    if (over) sl = woff + lrank;
-    if (valid) {
-        key[i] = b;
-        slot[i] = sl;
-        if (oct) oct[i] = o;
+    uint32_t b[kBinItems], sl[kBinItems];
+    uint8_t o[kBinItems];
+    bool over[kBinItems];
+#pragma unroll
+    for (int m = 0; m < kBinItems; m++) {
+        const bool valid = base + (int64_t)m * kBlock < n;
+        b[m] = valid ? bin_of(px[m], py[m], pz[m], g) : g.nbin;
+        o[m] = 0;
+        if (valid && oct) {
+            const double gx = (double)(px[m] - g.off[0]) * g.inv_dx;
+            const double gy = (double)(py[m] - g.off[1]) * g.inv_dx;
+            const double gz = (double)(pz[m] - g.off[2]) * g.inv_dx;
+            o[m] = (uint8_t)(((gx - floor(gx)) >= 0.5) | (((gy - floor(gy)) >= 0.5) << 1) |
+                             (((gz - floor(gz)) >= 0.5) << 2));
+        }
+        over[m] = valid && b[m] == g.nbin;
+        // runs of equal in-grid bins among consecutive lanes
+        const bool in = valid && !over[m];
+        const uint32_t kk = in ? b[m] : 0xFFFFFFFFu;
+        const uint32_t kprev = __shfl_up(kk, 1, 64);
+        const uint32_t knext = __shfl_down(kk, 1, 64);
+        const bool rstart = in && (lane == 0 || kprev != kk);
+        const bool rend = in && (lane == 63 || knext != kk);
+        const uint64_t S = __ballot(rstart), E = __ballot(rend);
+        const uint64_t le = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+        const int s0 = in ? 63 - __clzll(S & le) : 0;
+        uint32_t rbase = 0;
+        if (rstart) {
+            const int e0 = __ffsll((unsigned long long)(E & ~((1ull << lane) - 1ull))) - 1;
+            rbase = atomicAdd(&count[b[m]], (uint32_t)(e0 - lane + 1));
+        }
+        rbase = __shfl(rbase, s0, 64);
+        sl[m] = rbase + (uint32_t)(lane - s0);
+        const uint64_t mask = __ballot(over[m]);
+        if (over[m]) sl[m] = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+        if (lane == 0) wcnt[m][wid] = (uint32_t)__popcll(mask);
+    }
+    __syncthreads();
+    // overflow ranks: all earlier items, then the earlier waves of this item
+    uint32_t tot = 0;
+#pragma unroll
+    for (int m = 0; m < kBinItems; m++) {
+        uint32_t woff = tot;
+#pragma unroll
+        for (int w = 0; w < kBlock / 64; w++) {
+            woff += (w < wid) ? wcnt[m][w] : 0u;
+            tot += wcnt[m][w];
+        }
+        if (over[m]) sl[m] += woff;
+    }
+#pragma unroll
+    for (int m = 0; m < kBinItems; m++) {
+        const int64_t i = base + (int64_t)m * kBlock;
+        if (i < n) {
+            key[i] = b[m];
+            slot[i] = sl[m];
+            if (oct) oct[i] = o[m];
+        }
     }
     if (threadIdx.x == 0) block_out[blockIdx.x] = tot;
 }
@@ -270,18 +307,31 @@ __global__ __launch_bounds__(kBlock) void k_scatter(
     const uint32_t *key, const uint32_t *slot, int64_t n,
     const uint32_t *start, const uint32_t *block_out_scan, uint32_t nbin,
     uint32_t *perm, const uint8_t *oct, uint8_t *poct) {
-    int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x;
-    if (i >= n) return;
-    uint32_t b = key[i], s = slot[i];
-    uint32_t dst = (b == nbin) ? start[nbin] + block_out_scan[blockIdx.x] + s
-                               : start[b] + s;
-    perm[dst] = (uint32_t)i;
-    if (poct) poct[dst] = oct[i];
+    const int64_t base = (int64_t)blockIdx.x * kBinTile + threadIdx.x;
+    uint32_t b[kBinItems], sl[kBinItems], st[kBinItems];
+#pragma unroll
+    for (int m = 0; m < kBinItems; m++) {
+        const int64_t i = base + (int64_t)m * kBlock;
+        b[m] = i < n ? key[i] : nbin;
+        sl[m] = i < n ? slot[i] : 0u;
+    }
+    const uint32_t over0 = start[nbin] + block_out_scan[blockIdx.x];
+#pragma unroll
+    for (int m = 0; m < kBinItems; m++) st[m] = b[m] == nbin ? over0 : start[b[m]];
+#pragma unroll
+    for (int m = 0; m < kBinItems; m++) {
+        const int64_t i = base + (int64_t)m * kBlock;
+        if (i < n) {
+            const uint32_t dst = st[m] + sl[m];
+            perm[dst] = (uint32_t)i;
+            if (poct) poct[dst] = oct[i];
+        }
+    }
 }
 
-// Sorting-network compare-exchange: v[i] <= v[l] afterwards when up.
-template <int N>
-__device__ __forceinline__ void bitonic_sort(uint64_t (&v)[N]) {
+// Bitonic sorting network over N keys in registers, ascending.
+template <int N, class T>
+__device__ __forceinline__ void bitonic_sort(T (&v)[N]) {
 #pragma unroll
     for (int k = 2; k <= N; k <<= 1) {
 #pragma unroll
@@ -291,10 +341,10 @@ __device__ __forceinline__ void bitonic_sort(uint64_t (&v)[N]) {
                 const int l = i ^ j;
                 if (l > i) {
                     const bool up = (i & k) == 0;
-                    const uint64_t a = v[i], b = v[l];
-                    const bool sw = up ? (a > b) : (a < b);
-                    v[i] = sw ? b : a;
-                    v[l] = sw ? a : b;
+                    const T a = v[i], b = v[l];
+                    const T lo = a < b ? a : b, hi = a < b ? b : a;
+                    v[i] = up ? lo : hi;
+                    v[l] = up ? hi : lo;
                 }
             }
         }
@@ -309,21 +359,78 @@ __device__ __forceinline__ uint64_t order_key(const uint32_t *perm,
     return (o << 32) | (uint64_t)perm[q];
 }
 
-// Write a bin's sorted keys: perm_out gets the indices; hrank (when given)
-// the rank of each particle among its bin's particles of the same octant.
-template <int N>
-__device__ __forceinline__ void write_sorted(const uint64_t (&v)[N], uint32_t s,
-                                             uint32_t len, uint32_t *perm_out,
-                                             uint32_t *hrank) {
+// Where ordered bins go: the indices (perm), and with octant order each
+// particle's rank inside its half-cell run (hrank) and its octant (soct).
+struct OrderOut {
+    uint32_t *perm;
+    uint32_t *hrank;   // nullable
+    uint8_t *soct;     // nullable
+};
+
+// Write entry e of a bin starting at slot s, given its key and the first
+// entry of its octant run.
+__device__ __forceinline__ void put_sorted(const OrderOut &out, uint32_t s, uint32_t e,
+                                           uint64_t key, uint32_t run_start) {
+    out.perm[s + e] = (uint32_t)key;
+    if (out.hrank) out.hrank[s + e] = e - run_start;
+    if (out.soct) out.soct[s + e] = (uint8_t)(key >> 32);
+}
+
+// Keys in 32 bits when every index fits in kNarrowBits: (octant, index)
+// as octant << kNarrowBits | index.
+constexpr int kNarrowBits = 29;
+
+__device__ __forceinline__ uint32_t order_key32(const uint32_t *perm, const uint8_t *poct,
+                                                uint32_t q) {
+    const uint32_t o = poct ? (uint32_t)poct[q] : 0u;
+    return (o << kNarrowBits) | perm[q];
+}
+
+__device__ __forceinline__ uint64_t widen_key(uint64_t k) { return k; }
+__device__ __forceinline__ uint64_t widen_key(uint32_t k) {
+    return ((uint64_t)(k >> kNarrowBits) << 32) | (k & ((1u << kNarrowBits) - 1u));
+}
+
+// Write a bin's sorted keys.
+template <int N, class T>
+__device__ __forceinline__ void write_sorted(const T (&v)[N], uint32_t s,
+                                             uint32_t len, const OrderOut &out) {
     uint32_t rs = 0;
 #pragma unroll
     for (int m = 0; m < N; m++) {
-        if (m > 0 && (v[m] >> 32) != (v[m - 1] >> 32)) rs = (uint32_t)m;
-        if ((uint32_t)m < len) {
-            perm_out[s + m] = (uint32_t)v[m];
-            if (hrank) hrank[s + m] = (uint32_t)m - rs;
-        }
+        const uint64_t w = widen_key(v[m]);
+        if (m > 0 && (w >> 32) != (widen_key(v[m - 1]) >> 32)) rs = (uint32_t)m;
+        if ((uint32_t)m < len) put_sorted(out, s, (uint32_t)m, w, rs);
     }
+}
+
+// P2G bricks (edge kMarkBrick cells) whose staged cells (the brick plus one
+// cell around it) hold a particle: flag[brick] = 1.
+constexpr int kMarkBrick = 4;
+struct BrickMarks {
+    uint8_t *flag;   // nullable
+    int b0, b1;      // bin grid dims x, y
+    int nb[3];       // bricks per axis
+};
+
+// Bin b (cell = bin - 1 per axis) is staged by the bricks bx with
+// 4bx - 1 <= cell <= 4bx + 4, i.e. 4bx <= bin <= 4bx + 5: from
+// ceil((bin - 5)/4) = floor((bin - 2)/4) to floor(bin/4); every one of them
+// is marked (plain stores of 1).
+__device__ __forceinline__ void mark_bricks(const BrickMarks &m, uint32_t b) {
+    const int ix = (int)(b % (uint32_t)m.b0), iy = (int)((b / (uint32_t)m.b0) % (uint32_t)m.b1);
+    const int iz = (int)(b / ((uint32_t)m.b0 * (uint32_t)m.b1));
+    const int idx[3] = {ix, iy, iz};
+    int lo[3], hi[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        lo[a] = idx[a] >= 2 ? (idx[a] - 2) / kMarkBrick : 0;
+        hi[a] = min(idx[a] / kMarkBrick, m.nb[a] - 1);
+    }
+    for (int z = lo[2]; z <= hi[2]; z++)
+        for (int y = lo[1]; y <= hi[1]; y++)
+            for (int x = lo[0]; x <= hi[0]; x++)
+                m.flag[x + m.nb[0] * (y + m.nb[1] * z)] = 1;
 }
 
 constexpr uint32_t kSmallBin = 16;   // bins up to this size: a thread, in registers
@@ -336,17 +443,19 @@ constexpr uint32_t kMidBin = 256;    // bins up to this size: a wave, in registe
 // listed for k_order_mid (up to kMidBin entries, from the front of
 // big_list, count in big_count[0]) and k_order_big (from the back, count in
 // big_count[1]).
+template <bool NARROW>
 __global__ __launch_bounds__(kBlock) void k_order_bins(
     const uint32_t *__restrict__ start, uint32_t nbin,
-    const uint32_t *__restrict__ perm_in, const uint8_t *__restrict__ poct,
-    uint32_t *__restrict__ perm_out, uint32_t *__restrict__ hrank,
-    uint32_t *__restrict__ big_list, uint32_t big_cap, uint32_t *__restrict__ big_count) {
+    const uint32_t *__restrict__ perm_in, const uint8_t *__restrict__ poct, OrderOut out,
+    uint32_t *__restrict__ big_list, uint32_t big_cap, uint32_t *__restrict__ big_count,
+    BrickMarks marks) {
     const uint32_t b = blockIdx.x * kBlock + threadIdx.x;
     uint32_t s = 0, len = 0;
     if (b < nbin) {
         s = start[b];
         len = start[b + 1] - s;
     }
+    if (len > 0 && marks.flag) mark_bricks(marks, b);
     if (len > kSmallBin) {
         // the wave-sorted sizes go to the front of the list, the rest to
         // the back
@@ -355,26 +464,28 @@ __global__ __launch_bounds__(kBlock) void k_order_bins(
         len = 0;
     }
     if (len == 1) {
-        perm_out[s] = perm_in[s];
-        if (hrank) hrank[s] = 0u;
+        put_sorted(out, s, 0u, order_key(perm_in, poct, s), 0u);
         return;
     }
     // wave-uniform network size
+    using KT = typename std::conditional<NARROW, uint32_t, uint64_t>::type;
+    auto key = [&](uint32_t q) -> KT {
+        if (NARROW) return (KT)order_key32(perm_in, poct, q);
+        return (KT)order_key(perm_in, poct, q);
+    };
     const bool need16 = __any(len > 8u), need8 = __any(len > 1u);
     if (need16) {
-        uint64_t v[16];
+        KT v[16];
 #pragma unroll
-        for (int m = 0; m < 16; m++)
-            v[m] = (uint32_t)m < len ? order_key(perm_in, poct, s + m) : ~0ull;
+        for (int m = 0; m < 16; m++) v[m] = (uint32_t)m < len ? key(s + m) : (KT)~(KT)0;
         bitonic_sort<16>(v);
-        write_sorted<16>(v, s, len, perm_out, hrank);
+        write_sorted<16>(v, s, len, out);
     } else if (need8) {
-        uint64_t v[8];
+        KT v[8];
 #pragma unroll
-        for (int m = 0; m < 8; m++)
-            v[m] = (uint32_t)m < len ? order_key(perm_in, poct, s + m) : ~0ull;
+        for (int m = 0; m < 8; m++) v[m] = (uint32_t)m < len ? key(s + m) : (KT)~(KT)0;
         bitonic_sort<8>(v);
-        write_sorted<8>(v, s, len, perm_out, hrank);
+        write_sorted<8>(v, s, len, out);
     }
 }
 
@@ -421,13 +532,12 @@ __device__ __forceinline__ void wave_bitonic(uint64_t (&v)[K]) {
 }
 
 // Sort one bin of 16 < len <= 64*K entries with the calling wave and write
-// it out (perm_out, and hrank when given).
+// it out.
 template <int K>
 __device__ __forceinline__ void wave_order_bin(uint32_t s, uint32_t len,
                                                const uint32_t *__restrict__ perm_in,
                                                const uint8_t *__restrict__ poct,
-                                               uint32_t *__restrict__ perm_out,
-                                               uint32_t *__restrict__ hrank) {
+                                               const OrderOut &out) {
     const int lane = threadIdx.x & 63;
     uint64_t v[K];
 #pragma unroll
@@ -460,10 +570,7 @@ __device__ __forceinline__ void wave_order_bin(uint32_t s, uint32_t len,
 #pragma unroll
     for (int r = 0; r < K; r++) {
         const uint32_t e = (uint32_t)(lane * K + r);
-        if (e < len) {
-            perm_out[s + e] = (uint32_t)v[r];
-            if (hrank) hrank[s + e] = e - max(rs[r], from_below);
-        }
+        if (e < len) put_sorted(out, s, e, v[r], max(rs[r], from_below));
     }
 }
 
@@ -471,16 +578,15 @@ __device__ __forceinline__ void wave_order_bin(uint32_t s, uint32_t len,
 // registers (64 or 256 keys per wave).
 __global__ __launch_bounds__(kBlock) void k_order_mid(
     const uint32_t *__restrict__ start, const uint32_t *__restrict__ perm_in,
-    const uint8_t *__restrict__ poct, uint32_t *__restrict__ perm_out,
-    uint32_t *__restrict__ hrank, const uint32_t *__restrict__ big_list,
+    const uint8_t *__restrict__ poct, OrderOut out, const uint32_t *__restrict__ big_list,
     const uint32_t *__restrict__ big_count) {
     const uint32_t nmid = big_count[0];
     const uint32_t waves = gridDim.x * (kBlock / 64);
     for (uint32_t c = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); c < nmid; c += waves) {
         const uint32_t b = big_list[c];
         const uint32_t s = start[b], len = start[b + 1] - s;
-        if (len <= 64u) wave_order_bin<1>(s, len, perm_in, poct, perm_out, hrank);
-        else wave_order_bin<4>(s, len, perm_in, poct, perm_out, hrank);
+        if (len <= 64u) wave_order_bin<1>(s, len, perm_in, poct, out);
+        else wave_order_bin<4>(s, len, perm_in, poct, out);
     }
 }
 
@@ -494,15 +600,14 @@ constexpr int kOrderMidBlocks = 4096;
 // filled in index order.
 __global__ __launch_bounds__(64) void k_order_big(
     const uint32_t *__restrict__ start, uint32_t nbin,
-    const uint32_t *__restrict__ perm_in, const uint8_t *__restrict__ poct,
-    uint32_t *__restrict__ perm_out, uint32_t *__restrict__ hrank,
+    const uint32_t *__restrict__ perm_in, const uint8_t *__restrict__ poct, OrderOut out,
     const uint32_t *__restrict__ big_list, uint32_t big_cap,
     const uint32_t *__restrict__ big_count) {
     __shared__ uint64_t tile[kBigTile];
     const uint32_t lane = threadIdx.x;
     const uint32_t o0 = start[nbin], o1 = start[nbin + 1];
     for (uint32_t q = o0 + blockIdx.x * 64u + lane; q < o1; q += gridDim.x * 64u)
-        perm_out[q] = perm_in[q];
+        out.perm[q] = perm_in[q];
     const uint32_t nbig = big_count[1];
     for (uint32_t c = blockIdx.x; c < nbig; c += gridDim.x) {
         const uint32_t b = big_list[big_cap - 1u - c];
@@ -523,10 +628,7 @@ __global__ __launch_bounds__(64) void k_order_big(
                     hr += lt && (kk >> 32) == (mine >> 32);
                 }
             }
-            if (e < len) {
-                perm_out[s + rank] = (uint32_t)mine;
-                if (hrank) hrank[s + rank] = hr;
-            }
+            if (e < len) put_sorted(out, s, rank, mine, rank - hr);
         }
     }
 }
@@ -561,21 +663,32 @@ void cell_sort(const float *aos, const float *x, const float *y,
                        (const uint8_t *)oct, poct);
     GFSX_LAUNCH_CHECK();
     fill_u32(sb.big_count, 0u, 2, s);
-    uint32_t *hrank = octant_order ? sb.hrank : nullptr;
-    hipLaunchKernelGGL(k_order_bins, dim3(grid_for(g.nbin, kBlock)), dim3(kBlock), 0, s,
+    OrderOut out;
+    out.perm = sb.perm;
+    out.hrank = octant_order ? sb.hrank : nullptr;
+    out.soct = octant_order ? sb.soct : nullptr;
+    BrickMarks marks;
+    marks.flag = nullptr;
+    if (octant_order && sb.brick_flag) {
+        marks.flag = sb.brick_flag;
+        marks.b0 = g.b[0];
+        marks.b1 = g.b[1];
+        for (int a = 0; a < 3; a++) marks.nb[a] = (g.b[a] - 2 + kMarkBrick - 1) / kMarkBrick;
+        fill_u8(sb.brick_flag, 0, (int64_t)marks.nb[0] * marks.nb[1] * marks.nb[2], s);
+    }
+    auto *order = n < ((int64_t)1 << kNarrowBits) ? k_order_bins<true> : k_order_bins<false>;
+    hipLaunchKernelGGL(order, dim3(grid_for(g.nbin, kBlock)), dim3(kBlock), 0, s,
                        (const uint32_t *)sb.start, g.nbin, (const uint32_t *)sb.perm_tmp,
-                       (const uint8_t *)poct, sb.perm, hrank, sb.big_list, sb.big_cap,
-                       sb.big_count);
+                       (const uint8_t *)poct, out, sb.big_list, sb.big_cap, sb.big_count, marks);
     GFSX_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_order_mid, dim3(kOrderMidBlocks), dim3(kBlock), 0, s,
                        (const uint32_t *)sb.start, (const uint32_t *)sb.perm_tmp,
-                       (const uint8_t *)poct, sb.perm, hrank, (const uint32_t *)sb.big_list,
+                       (const uint8_t *)poct, out, (const uint32_t *)sb.big_list,
                        (const uint32_t *)sb.big_count);
     GFSX_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_order_big, dim3(kOrderBigBlocks), dim3(64), 0, s,
                        (const uint32_t *)sb.start, g.nbin, (const uint32_t *)sb.perm_tmp,
-                       (const uint8_t *)poct, sb.perm, hrank,
-                       (const uint32_t *)sb.big_list, sb.big_cap,
+                       (const uint8_t *)poct, out, (const uint32_t *)sb.big_list, sb.big_cap,
                        (const uint32_t *)sb.big_count);
     GFSX_LAUNCH_CHECK();
 }
@@ -705,15 +818,23 @@ struct AxisCoords {
     float ta, tb;
 };
 
-__device__ __forceinline__ AxisCoords axis_coords(double x, double inv_dx) {
+// Cell index and fraction along one axis for the unshifted (a) and the
+// half-shifted (b) samples, x >= 0.  With ga = x/dx and r = ga - floor(ga)
+// (exact), gb = ga - 1/2 has floor(gb) = floor(ga) - (r < 1/2) and
+// gb - floor(gb) = r - 1/2 or r + 1/2.  For ga >= 1/4 this is the FP64 value
+// the reference computes (ga - 1/2 is exact there); below 1/4 its two
+// roundings (ga - 1/2, then + 1) may differ from this one by an FP64 ulp,
+// far under FP32 resolution.
+__device__ __forceinline__ AxisCoords axis_coords(float x, double inv_dx) {
     AxisCoords c;
-    const double ga = x * inv_dx;
-    const double gb = ga - 0.5;
-    const double fa = floor(ga), fb = floor(gb);
+    const double ga = (double)x * inv_dx;
+    const double fa = floor(ga);
+    const double r = ga - fa;
+    const bool low = r < 0.5;
     c.ia = (int)fa;
-    c.ib = (int)fb;
-    c.ta = (float)(ga - fa);
-    c.tb = (float)(gb - fb);
+    c.ib = c.ia - (int)low;
+    c.ta = (float)r;
+    c.tb = (float)(low ? r + 0.5 : r - 0.5);
     return c;
 }
 
@@ -818,16 +939,15 @@ __device__ __forceinline__ void sample_mac(const MacGrid &g, const float *o0,
                                            const float *o1, const float *o2,
                                            float x, float y, float z,
                                            float out[3], float out_old[3]) {
-    const double dxp = x, dyp = y, dzp = z;
-    if (!(dxp >= 0.0 && dyp >= 0.0 && dzp >= 0.0 && dxp < g.lim[0] &&
-          dyp < g.lim[1] && dzp < g.lim[2])) {
+    if (!(x >= 0.0f && y >= 0.0f && z >= 0.0f && x < g.limf[0] && y < g.limf[1] &&
+          z < g.limf[2])) {
         out[0] = out[1] = out[2] = 0.0f;
         if (OLD) out_old[0] = out_old[1] = out_old[2] = 0.0f;
         return;
     }
-    const AxisCoords cx = axis_coords(dxp, g.inv_dx);
-    const AxisCoords cy = axis_coords(dyp, g.inv_dx);
-    const AxisCoords cz = axis_coords(dzp, g.inv_dx);
+    const AxisCoords cx = axis_coords(x, g.inv_dx);
+    const AxisCoords cy = axis_coords(y, g.inv_dx);
+    const AxisCoords cz = axis_coords(z, g.inv_dx);
     const int ni = g.ni, nj = g.nj, nk = g.nk;
     out[0] = sample_comp<ST>(g.f[0], ni + 1, nj, nk, cx.ia, cy.ib, cz.ib, cx.ta, cy.tb, cz.tb);
     out[1] = sample_comp<ST>(g.f[1], ni, nj + 1, nk, cx.ib, cy.ia, cz.ib, cx.tb, cy.ta, cz.tb);
@@ -1048,13 +1168,15 @@ constexpr int NROW = SY * SZ;
 constexpr int HX = 2 * SX, HY = 2 * SY, HZ = 2 * SZ;  // half-cells
 constexpr int NH = HX * HY * HZ;
 constexpr int NCELL = BX * BY * BZ;       // 64 cells: one wave per component
-constexpr int NT = 3 * NCELL;             // 192 threads
+constexpr int NT = 3 * NCELL;             // 192 threads (dense pass)
+constexpr int NTC = 4 * NCELL;            // 256 threads (common pass)
 // Particles staged per chunk (6 floats each); a brick at 8 particles per
 // cell stages 6x6x6 cells * 8 = 1728.
 constexpr int CAP = 2016;
 constexpr int DENSE_BLOCKS = 2048;   // work-groups of the pass over dense bricks
 constexpr int DENSE_CAP = 1344;      // its chunk size (smaller: more work-groups per CU)
 static_assert(NCELL == 64, "one wave per component");
+static_assert(BX == kMarkBrick && BY == kMarkBrick && BZ == kMarkBrick, "the sort marks these bricks");
 }  // namespace brick
 
 __device__ __forceinline__ void face_epilogue(const StepP2G &a,
@@ -1077,6 +1199,32 @@ __device__ __forceinline__ void face_epilogue(const StepP2G &a,
                                                     : (uint32_t)bg.b[0] * (uint32_t)bg.b[1]);
         if (ax < nax) fluid = fluid || count[bhi] > 0u;
         if (ax > 0) fluid = fluid || count[bhi - step] > 0u;
+        if (fluid) val += a.grav_delta[c];
+    }
+    a.grid[c][f] = val;
+}
+
+// face_epilogue for a brick face: whether the cells on either side hold a
+// particle comes from the brick's occupancy table (staged cells, x fastest)
+// instead of the global counts.
+__device__ __forceinline__ void brick_epilogue(const StepP2G &a, const uint8_t *occ,
+                                               int c, int i, int j, int k, int i0, int j0,
+                                               int k0, float S, float W) {
+    using namespace brick;
+    const NodeGrid &ng = a.ng[c];
+    const int64_t f = (int64_t)i + (int64_t)ng.n[0] * ((int64_t)j + (int64_t)ng.n[1] * k);
+    float val = W > 0.0f ? S / W : S;
+    if (a.grid_old[c]) a.grid_old[c][f] = val;
+    if (a.wgt[c]) a.wgt[c][f] = W;
+    if (a.grav_on[c]) {
+        const int ax = c == 0 ? i : (c == 1 ? j : k);
+        const int nax = c == 0 ? a.ni : (c == 1 ? a.nj : a.nk);
+        // staged cell of (i,j,k); the cell below along c is one step back
+        const int hi = (i - i0 + 1) + SX * ((j - j0 + 1) + SY * (k - k0 + 1));
+        const int step = c == 0 ? 1 : (c == 1 ? SX : SX * SY);
+        bool fluid = false;
+        if (ax < nax) fluid = fluid || occ[hi] != 0;
+        if (ax > 0) fluid = fluid || occ[hi - step] != 0;
         if (fluid) val += a.grav_delta[c];
     }
     a.grid[c][f] = val;
@@ -1166,9 +1314,12 @@ __device__ __forceinline__ void brick_gather_nested(const float *__restrict__ sx
     }
 }
 
-// The same gather with all 16 range bounds read up front and candidates
-// taken four at a time (the tail of a range is masked to zero weight), so
-// one LDS round trip serves four candidates.
+// The same gather range by range, candidates four at a time.  The four
+// reads of a chunk are consecutive (merged into paired LDS reads); reads past
+// the range's end land on other staged data (or the next LDS array) and are
+// masked: their weight and value are selected to zero, so nothing of them
+// reaches the sums.  The order of the sum per face is fixed (ranges in
+// order, then candidates in order).
 template <int C>
 __device__ __forceinline__ void brick_gather_chunked(const float *__restrict__ sx,
                                                      const float *__restrict__ sy,
@@ -1181,32 +1332,33 @@ __device__ __forceinline__ void brick_gather_chunked(const float *__restrict__ s
     const int hx0 = C == 0 ? 2 * li - 2 : 2 * li - 1;
     const int hy0 = C == 1 ? 2 * lj - 2 : 2 * lj - 1;
     const int hz0 = C == 2 ? 2 * lk - 2 : 2 * lk - 1;
-    uint32_t rq[16], re[16];
-#pragma unroll
-    for (int r = 0; r < 16; r++) brick_range(hs, hx0, hy0, hz0, r, rq[r], re[r]);
 #pragma unroll
     for (int r = 0; r < 16; r++) {
-        const uint32_t e = re[r];
-        for (uint32_t q = rq[r]; q < e; q += 4) {
+        uint32_t q, e;
+        brick_range(hs, hx0, hy0, hz0, r, q, e);
+        for (; q < e; q += 4u) {
             float px[4], py[4], pz[4], pv[4];
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
-                const uint32_t qq = min(q + u, e - 1);
-                px[u] = sx[qq];
-                py[u] = sy[qq];
-                pz[u] = sz[qq];
-                pv[u] = sv[qq];
+            for (int c = 0; c < 4; c++) {
+                px[c] = sx[q + c];
+                py[c] = sy[q + c];
+                pz[c] = sz[q + c];
+                pv[c] = sv[q + c];
             }
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
-                if (C != 0) px[u] -= offh;
-                if (C != 1) py[u] -= offh;
-                if (C != 2) pz[u] -= offh;
-                const float vx = gx - px[u], vy = gy - py[u], vz = gz - pz[u];
+            for (int c = 0; c < 4; c++) {
+                const bool valid = q + (uint32_t)c < e;
+                // p -= offset (scalarfield.cpp:168); the offset is zero on
+                // the component's own axis
+                if (C != 0) px[c] -= offh;
+                if (C != 1) py[c] -= offh;
+                if (C != 2) pz[c] -= offh;
+                const float vx = gx - px[c], vy = gy - py[c], vz = gz - pz[c];
                 const float d2 = vx * vx + vy * vy + vz * vz;
                 float K = wyvill(d2 * inv_r2);
-                K = (q + u < e) ? K : 0.0f;
-                S = fmaf(K, pv[u], S);
+                K = valid ? K : 0.0f;
+                const float v = valid ? pv[c] : 0.0f;
+                S = fmaf(K, v, S);
                 W += K;
             }
         }
@@ -1267,7 +1419,8 @@ __device__ __forceinline__ void p2g_brick(
     int brick_id, const uint32_t *__restrict__ count,
     const uint32_t *__restrict__ start, const uint32_t *__restrict__ hrank,
     uint32_t *__restrict__ dense, uint32_t *__restrict__ dense_count,
-    float *s_buf, uint16_t *s_hs, uint32_t *s_rowq, uint32_t *s_rows0, uint32_t *s_wsum) {
+    float *s_buf, uint16_t *s_hs, uint32_t *s_rowq, uint32_t *s_rows0, uint32_t *s_wsum,
+    uint8_t *s_occ) {
     using namespace brick;
     constexpr int PER = (CAP + NT - 1) / NT;
     float *s_x = s_buf, *s_y = s_buf + CAP, *s_z = s_buf + 2 * CAP;
@@ -1302,6 +1455,7 @@ __device__ __forceinline__ void p2g_brick(
         if (lane == 0) s_rows0[0] = 0u;
     }
     for (int h = t; h <= NH; h += NT) s_cnt[h] = 0u;
+    for (int h = t; h < SX * SY * SZ; h += NT) s_occ[h] = 0;
     __syncthreads();
     const uint32_t N = s_rows0[NROW];
     if (!CHUNKED && N > (uint32_t)CAP) {
@@ -1378,7 +1532,10 @@ __device__ __forceinline__ void p2g_brick(
             const int ox = (gxd - fx) >= 0.5, oy = (gyd - fy) >= 0.5, oz = (gzd - fz) >= 0.5;
             rh[m] = (uint32_t)((2 * lx + ox) + HX * ((2 * ly + oy) + HY * (2 * lz + oz)));
             const uint32_t s = c0 + (uint32_t)t + (uint32_t)m * NT;
-            if (s < ce) atomicAdd(&s_cnt[rh[m]], 1u);
+            if (s < ce) {
+                atomicAdd(&s_cnt[rh[m]], 1u);
+                s_occ[lx + SX * (ly + SY * lz)] = 1;
+            }
         }
         __syncthreads();
 
@@ -1449,29 +1606,224 @@ __device__ __forceinline__ void p2g_brick(
         }
         if (!CHUNKED) break;
     }
-    if (own_ok) face_epilogue(a, bg, count, wave, i0 + li, j0 + lj, k0 + lk, S, W);
-    if (ext_ok) face_epilogue(a, bg, count, wave, i0 + eli, j0 + elj, k0 + elk, SE, WE);
+    if (own_ok) brick_epilogue(a, s_occ, wave, i0 + li, j0 + lj, k0 + lk, i0, j0, k0, S, W);
+    if (ext_ok) brick_epilogue(a, s_occ, wave, i0 + eli, j0 + elj, k0 + elk, i0, j0, k0, SE, WE);
 }
 
+// The common P2G kernel: one work-group of four waves per brick.  A brick
+// whose staged cells hold no particle (flag 0, set by the sort) writes zeros
+// and leaves at once.  Otherwise the staged cells (the brick and one cell
+// around it) form 36 rows along x, each one contiguous range of the sort;
+// wave w stages rows w, w+4, ..., 64 consecutive sort slots per lane step
+// (coalesced loads).  A particle's cell in its row comes from the row's bin
+// starts, its octant from the sort, and its LDS slot is its half-cell's run
+// start (prefix of the half-cell counts) plus its rank in the run (from the
+// sort).  Waves 0..2 then gather the faces of components 0..2, wave 3 the
+// faces on the domain's upper planes when the brick has them.  A brick
+// holding more than CAP particles is appended to dense[] for
+// k_flip_p2g_dense.
 template <int V, int CAP>
-__global__ __launch_bounds__(brick::NT) void k_flip_p2g_brick(
+__global__ __launch_bounds__(brick::NTC) void k_flip_p2g_brick(
     StepP2G a, BinGrid bg, double dx, float inv_r2, int nbx, int nby, int nbz,
-    int per_xcd, const uint32_t *__restrict__ count,
-    const uint32_t *__restrict__ start, const uint32_t *__restrict__ hrank,
+    int per_xcd, const uint32_t *__restrict__ start, const uint32_t *__restrict__ hrank,
+    const uint8_t *__restrict__ soct, const uint8_t *__restrict__ brick_flag,
     uint32_t *__restrict__ dense, uint32_t *__restrict__ dense_count) {
     using namespace brick;
+    constexpr int NCS = SX * SY * SZ;          // staged cells
+    constexpr int NB = SX + 1;                 // bin starts per row
+    constexpr int RPW = (NROW + 3) / 4;        // rows per wave
+    static_assert(NROW * NB <= NTC, "one thread per row bin start");
     static_assert(NH + 1 <= 6 * CAP, "half-cell counts live in the particle arrays");
     static_assert(CAP <= 65535, "LDS slots are 16-bit");
-    __shared__ float s_buf[6 * CAP];
+    __shared__ float s_buf[6 * CAP + 4];   // +4: masked reads past a range end
     __shared__ uint16_t s_hs[NH + 1];
-    __shared__ uint32_t s_rowq[NROW], s_rows0[NROW + 1];
-    __shared__ uint32_t s_wsum[NT / 64];
+    __shared__ uint32_t s_bs[NROW][NB];
+    __shared__ uint32_t s_wsum[NTC / 64];
+    __shared__ uint8_t s_occ[NCS];
+    float *const s_x = s_buf, *const s_y = s_buf + CAP, *const s_z = s_buf + 2 * CAP;
+    float *const s_v0 = s_buf + 3 * CAP, *const s_v1 = s_buf + 4 * CAP;
+    float *const s_v2 = s_buf + 5 * CAP;
+    // half-cell counts borrow the particle arrays, which are only written
+    // once the layout is known
+    uint32_t *s_cnt = reinterpret_cast<uint32_t *>(s_buf);
+
     // consecutive bricks share halo particles: keep them on one XCD
     // (work-groups are dealt round-robin over the 8 XCDs)
     const int brick_id = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
     if (brick_id >= nbx * nby * nbz) return;
-    p2g_brick<V, CAP, false>(a, bg, dx, inv_r2, nbx, nby, brick_id, count, start, hrank,
-                             dense, dense_count, s_buf, s_hs, s_rowq, s_rows0, s_wsum);
+    const int bxi = brick_id % nbx, byi = (brick_id / nbx) % nby, bzi = brick_id / (nbx * nby);
+    const int i0 = bxi * BX, j0 = byi * BY, k0 = bzi * BZ;
+    const int t = threadIdx.x;
+    const int wave = t >> 6, lane = t & 63;
+
+    // faces of this thread: waves 0..2 own the brick's faces of component
+    // `wave`; wave 3's lanes 16c..16c+15 the upper-plane faces of component c
+    const int c = wave < 3 ? wave : (lane >> 4);
+    int fi, fj, fk;
+    bool ok;
+    if (wave < 3) {
+        fi = lane % BX;
+        fj = (lane / BX) % BY;
+        fk = lane / (BX * BY);
+        ok = true;
+    } else {
+        const int e0 = lane & 3, e1 = (lane >> 2) & 3;
+        if (c == 0) { fi = BX; fj = e0; fk = e1; ok = i0 + BX == a.ni; }
+        else if (c == 1) { fi = e0; fj = BY; fk = e1; ok = j0 + BY == a.nj; }
+        else { fi = e0; fj = e1; fk = BZ; ok = k0 + BZ == a.nk; }
+        ok = ok && lane < 48;
+    }
+    ok = ok && i0 + fi < a.ni + (c == 0) && j0 + fj < a.nj + (c == 1) && k0 + fk < a.nk + (c == 2);
+
+    // row bin starts: row (cy, cz) holds world cells i0-1 .. i0+4 at
+    // (j0-1+cy, k0-1+cz), bins i0 .. i0+5 (pad 1); a row outside the bin
+    // grid, or cells beyond its end, are empty.  Loaded together with the
+    // brick's flag.
+    uint32_t v = 0;
+    if (t < NROW * NB) {
+        const int r = t / NB, k = t % NB;
+        const int wy = j0 - 1 + r % SY, wz = k0 - 1 + r / SY;
+        if (wy <= a.nj && wz <= a.nk) {
+            const uint32_t row = (uint32_t)bg.b[0] * ((uint32_t)(wy + 1) + (uint32_t)bg.b[1] * (uint32_t)(wz + 1));
+            v = start[row + (uint32_t)min(i0 + k, bg.b[0])];
+        }
+    }
+    if (brick_flag[brick_id] == 0) {
+        // no particle within reach of any face of this brick: all values
+        // are 0 and no face borders fluid
+        if (ok) brick_zero_face(a, c, i0 + fi, j0 + fj, k0 + fk);
+        return;
+    }
+    if (t < NROW * NB) s_bs[t / NB][t % NB] = v;
+    for (int h = t; h <= NH; h += NTC) s_cnt[h] = 0u;
+    __syncthreads();
+    uint32_t total = 0;
+#pragma unroll 4
+    for (int r = 0; r < NROW; r++) total += s_bs[r][NB - 1] - s_bs[r][0];
+    if (total > (uint32_t)CAP) {
+        if (t == 0) dense[atomicAdd(dense_count, 1u)] = (uint32_t)brick_id;
+        return;
+    }
+    if (total == 0) {
+        if (ok) brick_zero_face(a, c, i0 + fi, j0 + fj, k0 + fk);
+        return;
+    }
+    if (t < NCS) {
+        const int r = t / SX, k = t % SX;
+        s_occ[t] = s_bs[r][k + 1] > s_bs[r][k];
+    }
+
+    // the half-cell of sort slot q in row r (staged row coordinates cy, cz)
+    auto half_cell = [&](int r, uint32_t q, uint32_t o) {
+        int ix = 0;
+#pragma unroll
+        for (int k = 1; k < SX; k++) ix += q >= s_bs[r][k];
+        const int cy = r % SY, cz = r / SY;
+        return (uint32_t)((2 * ix + (int)(o & 1u)) +
+                          HX * ((2 * cy + (int)((o >> 1) & 1u)) + HY * (2 * cz + (int)(o >> 2))));
+    };
+
+    // count: each wave its rows; the first 64 particles of a row stay in
+    // registers for placing, the rest are read again then
+    float rx[RPW], ry[RPW], rz[RPW], rvx[RPW], rvy[RPW], rvz[RPW];
+    uint32_t rd[RPW];   // half-cell | rank << 16
+#pragma unroll
+    for (int m = 0; m < RPW; m++) {
+        const int r = wave + 4 * m;
+        uint32_t base = 0, len = 0;
+        if (r < NROW) {
+            base = s_bs[r][0];
+            len = s_bs[r][NB - 1] - base;
+        }
+        const bool valid = (uint32_t)lane < len;
+        const uint32_t q = valid ? base + (uint32_t)lane : 0u;
+        rx[m] = a.x[q];
+        ry[m] = a.y[q];
+        rz[m] = a.z[q];
+        rvx[m] = a.vel[0][q];
+        rvy[m] = a.vel[1][q];
+        rvz[m] = a.vel[2][q];
+        const uint32_t o = soct[q], rk = hrank[q];
+        rd[m] = 0xFFFFFFFFu;
+        if (valid) {
+            const uint32_t h = half_cell(r, q, o);
+            atomicAdd(&s_cnt[h], 1u);
+            rd[m] = h | (rk << 16);
+        }
+        for (uint32_t off = 64; off < len; off += 64) {
+            if (off + (uint32_t)lane < len) {
+                const uint32_t q2 = base + off + (uint32_t)lane;
+                atomicAdd(&s_cnt[half_cell(r, q2, soct[q2])], 1u);
+            }
+        }
+    }
+    __syncthreads();
+
+    // exclusive prefix over the half-cells
+    {
+        constexpr int ITEMS = (NH + NTC - 1) / NTC;
+        uint32_t loc[ITEMS];
+        uint32_t sum = 0;
+#pragma unroll
+        for (int m = 0; m < ITEMS; m++) {
+            const int h = t * ITEMS + m;
+            loc[m] = h < NH ? s_cnt[h] : 0u;
+            sum += loc[m];
+        }
+        const uint32_t inc = wave_inclusive_sum(sum);
+        if (lane == 63) s_wsum[wave] = inc;
+        __syncthreads();
+        uint32_t run = inc - sum;
+#pragma unroll
+        for (int w = 0; w < NTC / 64; w++) run += w < wave ? s_wsum[w] : 0u;
+#pragma unroll
+        for (int m = 0; m < ITEMS; m++) {
+            const int h = t * ITEMS + m;
+            if (h < NH) s_hs[h] = (uint16_t)run;
+            run += loc[m];
+        }
+        if (t == 0) s_hs[NH] = (uint16_t)total;
+    }
+    __syncthreads();
+
+    // place: half-cell run start + rank in the run
+#pragma unroll
+    for (int m = 0; m < RPW; m++) {
+        if (rd[m] != 0xFFFFFFFFu) {
+            const uint32_t d = (uint32_t)s_hs[rd[m] & 0xFFFFu] + (rd[m] >> 16);
+            s_x[d] = rx[m];
+            s_y[d] = ry[m];
+            s_z[d] = rz[m];
+            s_v0[d] = rvx[m];
+            s_v1[d] = rvy[m];
+            s_v2[d] = rvz[m];
+        }
+        const int r = wave + 4 * m;
+        if (r < NROW) {
+            const uint32_t base = s_bs[r][0], len = s_bs[r][NB - 1] - base;
+            for (uint32_t off = 64; off < len; off += 64) {
+                if (off + (uint32_t)lane < len) {
+                    const uint32_t q2 = base + off + (uint32_t)lane;
+                    const uint32_t d = (uint32_t)s_hs[half_cell(r, q2, soct[q2])] + hrank[q2];
+                    s_x[d] = a.x[q2];
+                    s_y[d] = a.y[q2];
+                    s_z[d] = a.z[q2];
+                    s_v0[d] = a.vel[0][q2];
+                    s_v1[d] = a.vel[1][q2];
+                    s_v2[d] = a.vel[2][q2];
+                }
+            }
+        }
+    }
+    __syncthreads();
+
+    float S = 0.0f, W = 0.0f;
+    if (ok) {
+        if (c == 0) brick_accum<0, V>(a, s_x, s_y, s_z, s_v0, s_hs, i0, j0, k0, fi, fj, fk, dx, inv_r2, S, W);
+        else if (c == 1) brick_accum<1, V>(a, s_x, s_y, s_z, s_v1, s_hs, i0, j0, k0, fi, fj, fk, dx, inv_r2, S, W);
+        else brick_accum<2, V>(a, s_x, s_y, s_z, s_v2, s_hs, i0, j0, k0, fi, fj, fk, dx, inv_r2, S, W);
+        brick_epilogue(a, s_occ, c, i0 + fi, j0 + fj, k0 + fk, i0, j0, k0, S, W);
+    }
 }
 
 // The bricks listed by k_flip_p2g_brick as over its capacity, in chunks.
@@ -1485,15 +1837,17 @@ __global__ __launch_bounds__(brick::NT, 3) void k_flip_p2g_dense(
     using namespace brick;
     static_assert(NH + 1 <= 6 * CAP, "half-cell counts live in the particle arrays");
     static_assert(CAP <= 65535, "LDS slots are 16-bit");
-    __shared__ float s_buf[6 * CAP];
+    __shared__ float s_buf[6 * CAP + 4];   // +4: masked reads past a range end
     __shared__ uint16_t s_hs[NH + 1];
     __shared__ uint32_t s_rowq[NROW], s_rows0[NROW + 1];
     __shared__ uint32_t s_wsum[NT / 64];
+    __shared__ uint8_t s_occ[SX * SY * SZ];
     const uint32_t nd = *dense_count;
     for (uint32_t d = blockIdx.x; d < nd; d += gridDim.x) {
         __syncthreads();   // the previous brick is done with LDS
         p2g_brick<V, CAP, true>(a, bg, dx, inv_r2, nbx, nby, (int)dense[d], count, start,
-                                hrank, dense, dense_count, s_buf, s_hs, s_rowq, s_rows0, s_wsum);
+                                hrank, dense, dense_count, s_buf, s_hs, s_rowq, s_rows0, s_wsum,
+                                s_occ);
     }
 }
 
@@ -1664,16 +2018,15 @@ __device__ __forceinline__ void brick_sample(const MacGrid &g, const float *o0,
                                              float x, float y, float z,
                                              float out[3], float out_old[3]) {
     using B = gbrick::Box<ST>;
-    const double dxp = x, dyp = y, dzp = z;
-    if (!(dxp >= 0.0 && dyp >= 0.0 && dzp >= 0.0 && dxp < g.lim[0] &&
-          dyp < g.lim[1] && dzp < g.lim[2])) {
+    if (!(x >= 0.0f && y >= 0.0f && z >= 0.0f && x < g.limf[0] && y < g.limf[1] &&
+          z < g.limf[2])) {
         out[0] = out[1] = out[2] = 0.0f;
         if (OLD) out_old[0] = out_old[1] = out_old[2] = 0.0f;
         return;
     }
-    const AxisCoords cx = axis_coords(dxp, g.inv_dx);
-    const AxisCoords cy = axis_coords(dyp, g.inv_dx);
-    const AxisCoords cz = axis_coords(dzp, g.inv_dx);
+    const AxisCoords cx = axis_coords(x, g.inv_dx);
+    const AxisCoords cy = axis_coords(y, g.inv_dx);
+    const AxisCoords cz = axis_coords(z, g.inv_dx);
     const int ni = g.ni, nj = g.nj, nk = g.nk;
     if (in_box<ST, B::E>(cx, cy, cz, lx, ly, lz)) {
         out[0] = box_comp<ST, B::E>(cur0, cx.ia - lx, cy.ib - ly, cz.ib - lz, cx.ta, cy.tb, cz.tb);
@@ -1908,7 +2261,8 @@ void flip_step(const StepArgs &s, const SortBuffers &sb, hipStream_t st,
         const int nbz = (s.nk + brick::BZ - 1) / brick::BZ;
         const int64_t nbrick = (int64_t)nbx * nby * nbz;
         const int per_xcd = (int)((nbrick + 7) / 8);
-        const dim3 grid((unsigned)(8 * (int64_t)per_xcd)), block(brick::NT);
+        const dim3 block(brick::NT);
+        const dim3 grid((unsigned)(8 * (int64_t)per_xcd));
         constexpr int cap = brick::CAP;
         auto *kern = s.p2g_variant == 2 ? k_flip_p2g_brick<2, cap>
                    : s.p2g_variant == 3 ? k_flip_p2g_brick<3, cap> : k_flip_p2g_brick<0, cap>;
@@ -1917,10 +2271,10 @@ void flip_step(const StepArgs &s, const SortBuffers &sb, hipStream_t st,
                          : s.p2g_variant == 3 ? k_flip_p2g_dense<3, dcap>
                                               : k_flip_p2g_dense<0, dcap>;
         fill_u32(s.dense_count, 0u, 1, st);
-        hipLaunchKernelGGL(kern, grid, block, 0, st, a, bg, s.dx, inv_r2, nbx, nby, nbz,
-                           per_xcd, (const uint32_t *)sb.count,
-                           (const uint32_t *)sb.start, (const uint32_t *)sb.hrank,
-                           s.dense, s.dense_count);
+        hipLaunchKernelGGL(kern, grid, dim3(brick::NTC), 0, st, a, bg, s.dx, inv_r2, nbx, nby,
+                           nbz, per_xcd, (const uint32_t *)sb.start,
+                           (const uint32_t *)sb.hrank, (const uint8_t *)sb.soct,
+                           (const uint8_t *)sb.brick_flag, s.dense, s.dense_count);
         GFSX_LAUNCH_CHECK();
         // bricks holding more than cap particles, from the dense list
         const unsigned ndense = (unsigned)std::min<int64_t>(nbrick, brick::DENSE_BLOCKS);
@@ -1947,9 +2301,7 @@ void flip_step(const StepArgs &s, const SortBuffers &sb, hipStream_t st,
     g.nk = s.nk;
     g.dx = s.dx;
     g.inv_dx = 1.0 / s.dx;
-    g.lim[0] = s.dx * s.ni;
-    g.lim[1] = s.dx * s.nj;
-    g.lim[2] = s.dx * s.nk;
+    set_limits(g);
     StepG2P b;
     for (int c = 0; c < 3; c++) {
         b.xin[c] = s.xs[c];

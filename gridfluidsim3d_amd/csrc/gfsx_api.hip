@@ -35,7 +35,7 @@ void ok(int *err) {
 enum Slot {
     S_COUNT, S_START, S_KEY, S_SLOT, S_PERM, S_PERM_TMP, S_HRANK, S_SOCT, S_BFLAG, S_BIG, S_BIGN,
     S_BOUT, S_BOUTS, S_SCAN, S_OCT, S_POCT,
-    S_OLD0, S_OLD1, S_OLD2, S_XS0, S_XS1, S_XS2, S_VS0, S_VS1, S_VS2, S_DENSE, S_DENSEN,
+    S_OLD0, S_XS0, S_XS1, S_XS2, S_VS0, S_VS1, S_VS2, S_DENSE, S_DENSEN,
     S_D_POS, S_D_VAL, S_D_S, S_D_W, S_D_U, S_D_V, S_D_WW, S_D_OUT,
     NSLOT
 };
@@ -89,11 +89,65 @@ struct gfsx_ctx {
         return (float *)get(slot, sizeof(float) * (size_t)std::max<int64_t>(count, 1));
     }
 
-    gfsx::SortBuffers sort_buffers(int64_t n, uint32_t nbin) {
+    // Scene tables on the device, kept by content: a simulation loop
+    // alternates between a few tables (its buffers swap every step), so
+    // after the first steps nothing is uploaded, and a step captured into a
+    // graph holds no copy.
+    struct Table {
+        std::vector<gfsx::SceneDesc> host;
+        gfsx::SceneDesc *dev = nullptr;
+        size_t cap = 0;
+        uint64_t used = 0;
+    };
+    std::array<Table, 4> tables;
+    uint64_t table_clock = 0;
+
+    const gfsx::SceneDesc *table(const std::vector<gfsx::SceneDesc> &t) {
+        const size_t bytes = sizeof(gfsx::SceneDesc) * t.size();
+        for (auto &e : tables) {
+            if (e.dev && e.host.size() == t.size() &&
+                std::memcmp(e.host.data(), t.data(), bytes) == 0) {
+                e.used = ++table_clock;
+                return e.dev;
+            }
+        }
+        hipStreamCaptureStatus cap_status = hipStreamCaptureStatusNone;
+        GFSX_HIP(hipStreamIsCapturing(stream, &cap_status));
+        require(cap_status == hipStreamCaptureStatusNone,
+                "gfsx: the scenes of a captured step must have run once before capture "
+                "(their table is uploaded then)");
+        Table *slot = &tables[0];
+        for (auto &e : tables)
+            if (e.used < slot->used) slot = &e;
+        if (slot->cap < t.size()) {
+            // the old table may still be read by queued work
+            GFSX_HIP(hipStreamSynchronize(stream));
+            if (slot->dev) GFSX_HIP(hipFree(slot->dev));
+            slot->dev = nullptr;
+            slot->cap = 0;
+            GFSX_HIP(hipMalloc(&slot->dev, std::max<size_t>(bytes, 1)));
+            slot->cap = t.size();
+        } else if (slot->used) {
+            // queued work may still read the table being replaced
+            GFSX_HIP(hipStreamSynchronize(stream));
+        }
+        slot->host = t;
+        GFSX_HIP(hipMemcpyAsync(slot->dev, slot->host.data(), bytes, hipMemcpyHostToDevice, stream));
+        slot->used = ++table_clock;
+        return slot->dev;
+    }
+
+    gfsx::SortBuffers sort_buffers(const std::vector<gfsx::SceneDesc> &t, int64_t n,
+                                   uint32_t nbin_total, uint32_t nblk) {
         gfsx::SortBuffers sb;
-        int64_t nb = gfsx::sort_blocks(n);
-        sb.count = u32(S_COUNT, (int64_t)nbin + 1);
-        sb.start = u32(S_START, (int64_t)nbin + 2);
+        sb.scenes = table(t);
+        sb.nscene = (int)t.size();
+        sb.nbin_total = nbin_total;
+        sb.nblk_total = nblk;
+        const int64_t nb = nblk;
+        const uint32_t nbin = nbin_total;
+        sb.count = u32(S_COUNT, (int64_t)nbin);
+        sb.start = u32(S_START, (int64_t)nbin + 1);
         sb.key = u32(S_KEY, n);
         sb.slot = u32(S_SLOT, n);
         sb.perm = u32(S_PERM, n);
@@ -107,7 +161,7 @@ struct gfsx_ctx {
         sb.big_count = u32(S_BIGN, 2);
         sb.block_out = u32(S_BOUT, nb);
         sb.block_out_scan = u32(S_BOUTS, nb + 1);
-        int64_t words = std::max(gfsx::scan_scratch_words((int64_t)nbin + 1),
+        int64_t words = std::max(gfsx::scan_scratch_words((int64_t)nbin),
                                  gfsx::scan_scratch_words(nb));
         sb.scan_scratch = u32(S_SCAN, words);
         sb.oct = (uint8_t *)get(S_OCT, (size_t)std::max<int64_t>(n, 1));
@@ -122,6 +176,8 @@ struct gfsx_ctx {
             if (buf[s]) (void)hipFree(buf[s]);
         for (auto &e : events)
             for (auto &x : e) (void)hipEventDestroy(x);
+        for (auto &t : tables)
+            if (t.dev) (void)hipFree(t.dev);
         if (own_stream) (void)hipStreamDestroy(own_stream);
     }
 };
@@ -207,9 +263,120 @@ void p2g_device_impl(gfsx_ctx *c, const float *pos, const float *val,
     bg.inv_dx = 1.0 / dx;
     bg.nbin = (uint32_t)nbin;
     ng.dx_f = (float)dx;
-    gfsx::SortBuffers sb = c->sort_buffers(n, bg.nbin);
-    gfsx::cell_sort(pos, nullptr, nullptr, nullptr, n, bg, sb, c->stream);
+    std::vector<gfsx::SceneDesc> t(1);
+    std::memset(t.data(), 0, sizeof(gfsx::SceneDesc));
+    t[0].aos = pos;
+    t[0].n = n;
+    t[0].bg = bg;
+    gfsx::SortBuffers sb = c->sort_buffers(t, n, bg.nbin + 1, (uint32_t)gfsx::sort_blocks(n));
+    gfsx::cell_sort(sb, n, c->stream, false);
     gfsx::p2g_gather(pos, val, n, bg, ng, radius, dx, sb, S, W, c->stream);
+}
+
+// Checks one scene of a fused step and fills its table entry.
+gfsx::SceneDesc scene_desc(const gfsx_scene *sc, const gfsx_step_params *pr, const char *who) {
+    auto req = [&](bool c, const char *what) {
+        if (!c) throw gfsx::Error(std::string(who) + ": " + what);
+    };
+    check_grid(sc->ni, sc->nj, sc->nk, sc->dx);
+    check_particles(sc->n, sc->x_in);
+    req(pr->stencil == GFSX_TRILINEAR || pr->stencil == GFSX_TRICUBIC, "unknown stencil");
+    req(pr->order == GFSX_ORDER_SORTED || pr->order == GFSX_ORDER_KEEP, "unknown output order");
+    req(std::isfinite(pr->dt), "dt must be finite");
+    req(sc->u && sc->v && sc->w, "null grid output");
+    if (sc->n > 0) {
+        req(sc->x_in && sc->y_in && sc->z_in && sc->vx_in && sc->vy_in && sc->vz_in,
+            "null particle input");
+        req(sc->x_out && sc->y_out && sc->z_out && sc->vx_out && sc->vy_out && sc->vz_out,
+            "null particle output");
+        const void *ins[6] = {sc->x_in, sc->y_in, sc->z_in, sc->vx_in, sc->vy_in, sc->vz_in};
+        const void *outs[6] = {sc->x_out, sc->y_out, sc->z_out, sc->vx_out, sc->vy_out, sc->vz_out};
+        for (int a = 0; a < 6; a++)
+            for (int b = 0; b < 6; b++) req(ins[a] != outs[b], "outputs must not alias inputs");
+    }
+    const int64_t nbin = (int64_t)(sc->ni + 2) * (sc->nj + 2) * (sc->nk + 2);
+    req(nbin < ((int64_t)1 << 31), "too many cells");
+    gfsx::SceneDesc d;
+    std::memset(&d, 0, sizeof d);   // padding too: tables are compared bytewise
+    d.n = sc->n;
+    d.xin[0] = sc->x_in;
+    d.xin[1] = sc->y_in;
+    d.xin[2] = sc->z_in;
+    d.vin[0] = sc->vx_in;
+    d.vin[1] = sc->vy_in;
+    d.vin[2] = sc->vz_in;
+    d.xout[0] = sc->x_out;
+    d.xout[1] = sc->y_out;
+    d.xout[2] = sc->z_out;
+    d.vout[0] = sc->vx_out;
+    d.vout[1] = sc->vy_out;
+    d.vout[2] = sc->vz_out;
+    d.id_in = sc->id_in;
+    d.id_out = sc->id_out;
+    d.ni = sc->ni;
+    d.nj = sc->nj;
+    d.nk = sc->nk;
+    d.dx = sc->dx;
+    float *grids[3] = {sc->u, sc->v, sc->w};
+    float *olds[3] = {sc->u_old, sc->v_old, sc->w_old};
+    float *wts[3] = {sc->wu, sc->wv, sc->ww};
+    for (int c = 0; c < 3; c++) {
+        d.grid[c] = grids[c];
+        d.grid_old[c] = olds[c];
+        d.wgt[c] = wts[c];
+        d.grav_on[c] = std::fabs((double)pr->body_force[c]) > 0.0;
+        d.grav_delta[c] = (float)((double)pr->body_force[c] * pr->dt);
+    }
+    d.h_half = (float)(0.5 * pr->dt);
+    d.h_full = (float)pr->dt;
+    d.h_sixth = (float)(pr->dt / 6.0f);
+    d.r_pic = (float)pr->pic_ratio;
+    d.r_flip = (float)(1 - pr->pic_ratio);
+    return d;
+}
+
+void flip_step_impl(gfsx_ctx *ctx, const gfsx_scene *scenes, const gfsx_step_params *params,
+                    int nscenes, const char *who) {
+    gfsx::StepBatch s;
+    s.scenes.resize(nscenes);
+    for (int i = 0; i < nscenes; i++) {
+        s.scenes[i] = scene_desc(&scenes[i], &params[i], who);
+        if (params[i].stencil != params[0].stencil || params[i].order != params[0].order)
+            throw gfsx::Error(std::string(who) + ": every scene needs the same stencil and order");
+    }
+    gfsx::layout_scenes(s.scenes.data(), nscenes, s.tot);
+    require(s.tot.n < ((int64_t)1 << 32) - 1, "gfsx: too many particles in one step");
+    require(s.tot.nbin < ((int64_t)1 << 32) - 2, "gfsx: too many cells in one step");
+    require(nscenes == 1 || (ctx->p2g_variant != 1 && ctx->g2p_variant != 1),
+            "gfsx: the direct P2G and G2P variants take one scene");
+    // snapshots the caller does not keep live in one workspace buffer
+    int64_t old_faces = 0;
+    for (auto &d : s.scenes)
+        for (int c = 0; c < 3; c++)
+            if (!d.grid_old[c]) old_faces += mac_count(c, d.ni, d.nj, d.nk);
+    float *old_ws = ctx->f32(S_OLD0, old_faces);
+    for (auto &d : s.scenes)
+        for (int c = 0; c < 3; c++)
+            if (!d.grid_old[c]) {
+                d.grid_old[c] = old_ws;
+                old_ws += mac_count(c, d.ni, d.nj, d.nk);
+            }
+    for (int c = 0; c < 3; c++) {
+        s.xs[c] = ctx->f32(S_XS0 + c, s.tot.n);
+        s.vs[c] = ctx->f32(S_VS0 + c, s.tot.n);
+    }
+    s.dense = ctx->u32(S_DENSE, s.tot.np2g);
+    s.dense_count = ctx->u32(S_DENSEN, 1);
+    s.stencil = params[0].stencil;
+    s.keep_order = params[0].order == GFSX_ORDER_KEEP;
+    s.p2g_variant = ctx->p2g_variant;
+    s.g2p_variant = ctx->g2p_variant;
+    gfsx::SortBuffers sb = ctx->sort_buffers(s.scenes, s.tot.n, (uint32_t)s.tot.nbin,
+                                             (uint32_t)s.tot.nblk);
+    s.scenes_dev = sb.scenes;
+    // P2G brick flags, one byte per 4x4x4-cell brick of every scene
+    sb.brick_flag = (uint8_t *)ctx->get(S_BFLAG, (size_t)s.tot.np2g);
+    gfsx::flip_step(s, sb, ctx->stream, ctx->timing ? ctx->next_events() : nullptr);
 }
 
 void copy_in(void *dst, const void *src, size_t bytes, hipStream_t s) {
@@ -444,81 +611,16 @@ void gfsx_flip_step(gfsx_ctx *ctx, const gfsx_scene *sc,
                     const gfsx_step_params *pr, int *err) {
     guarded(ctx, err, [&] {
         require(sc && pr, "gfsx_flip_step: null scene or params");
-        check_grid(sc->ni, sc->nj, sc->nk, sc->dx);
-        check_particles(sc->n, sc->x_in);
-        require(pr->stencil == GFSX_TRILINEAR || pr->stencil == GFSX_TRICUBIC,
-                "gfsx_flip_step: unknown stencil");
-        require(pr->order == GFSX_ORDER_SORTED || pr->order == GFSX_ORDER_KEEP,
-                "gfsx_flip_step: unknown output order");
-        require(std::isfinite(pr->dt), "gfsx_flip_step: dt must be finite");
-        require(sc->u && sc->v && sc->w, "gfsx_flip_step: null grid output");
-        if (sc->n > 0) {
-            require(sc->x_in && sc->y_in && sc->z_in && sc->vx_in && sc->vy_in && sc->vz_in,
-                    "gfsx_flip_step: null particle input");
-            require(sc->x_out && sc->y_out && sc->z_out && sc->vx_out && sc->vy_out && sc->vz_out,
-                    "gfsx_flip_step: null particle output");
-            const void *ins[6] = {sc->x_in, sc->y_in, sc->z_in, sc->vx_in, sc->vy_in, sc->vz_in};
-            const void *outs[6] = {sc->x_out, sc->y_out, sc->z_out, sc->vx_out, sc->vy_out, sc->vz_out};
-            for (int a = 0; a < 6; a++)
-                for (int b = 0; b < 6; b++)
-                    require(ins[a] != outs[b], "gfsx_flip_step: outputs must not alias inputs");
-        }
-        const int ni = sc->ni, nj = sc->nj, nk = sc->nk;
-        int64_t nbin = (int64_t)(ni + 2) * (nj + 2) * (nk + 2);
-        require(nbin < ((int64_t)1 << 32) - 2, "gfsx_flip_step: too many cells");
-        gfsx::StepArgs a;
-        a.n = sc->n;
-        a.ni = ni;
-        a.nj = nj;
-        a.nk = nk;
-        a.dx = sc->dx;
-        a.xin[0] = sc->x_in;
-        a.xin[1] = sc->y_in;
-        a.xin[2] = sc->z_in;
-        a.vin[0] = sc->vx_in;
-        a.vin[1] = sc->vy_in;
-        a.vin[2] = sc->vz_in;
-        a.xout[0] = sc->x_out;
-        a.xout[1] = sc->y_out;
-        a.xout[2] = sc->z_out;
-        a.vout[0] = sc->vx_out;
-        a.vout[1] = sc->vy_out;
-        a.vout[2] = sc->vz_out;
-        a.id_in = sc->id_in;
-        a.id_out = sc->id_out;
-        float *grids[3] = {sc->u, sc->v, sc->w};
-        float *olds[3] = {sc->u_old, sc->v_old, sc->w_old};
-        float *wts[3] = {sc->wu, sc->wv, sc->ww};
-        const int old_slots[3] = {S_OLD0, S_OLD1, S_OLD2};
-        {
-            // one entry per P2G brick of 4x4x4 cells, at most
-            const int64_t nbrick = (int64_t)((ni + 3) / 4) * ((nj + 3) / 4) * ((nk + 3) / 4);
-            a.dense = ctx->u32(S_DENSE, nbrick);
-            a.dense_count = ctx->u32(S_DENSEN, 1);
-        }
-        for (int c = 0; c < 3; c++) {
-            a.xs[c] = ctx->f32(S_XS0 + c, sc->n);
-            a.vs[c] = ctx->f32(S_VS0 + c, sc->n);
-            a.grid[c] = grids[c];
-            a.grid_old[c] = olds[c] ? olds[c] : ctx->f32(old_slots[c], mac_count(c, ni, nj, nk));
-            a.wgt[c] = wts[c];
-            a.grav_on[c] = std::fabs((double)pr->body_force[c]) > 0.0;
-            a.grav_delta[c] = (float)((double)pr->body_force[c] * pr->dt);
-        }
-        a.h_half = (float)(0.5 * pr->dt);
-        a.h_full = (float)pr->dt;
-        a.h_sixth = (float)(pr->dt / 6.0f);
-        a.r_pic = (float)pr->pic_ratio;
-        a.r_flip = (float)(1 - pr->pic_ratio);
-        a.stencil = pr->stencil;
-        a.keep_order = pr->order == GFSX_ORDER_KEEP;
-        a.p2g_variant = ctx->p2g_variant;
-        a.g2p_variant = ctx->g2p_variant;
-        a.num_cu = ctx->num_cu;
-        gfsx::SortBuffers sb = ctx->sort_buffers(sc->n, (uint32_t)nbin);
-        // P2G brick flags, one byte per 4x4x4-cell brick
-        sb.brick_flag = (uint8_t *)ctx->get(S_BFLAG, (size_t)((ni + 3) / 4) * ((nj + 3) / 4) * ((nk + 3) / 4));
-        gfsx::flip_step(a, sb, ctx->stream, ctx->timing ? ctx->next_events() : nullptr);
+        flip_step_impl(ctx, sc, pr, 1, "gfsx_flip_step");
+    });
+}
+
+void gfsx_flip_step_batched(gfsx_ctx *ctx, const gfsx_scene *scenes, int nscenes,
+                            const gfsx_step_params *params, int *err) {
+    guarded(ctx, err, [&] {
+        require(scenes && params, "gfsx_flip_step_batched: null scenes or params");
+        require(nscenes >= 1 && nscenes <= 65536, "gfsx_flip_step_batched: nscenes must be 1..65536");
+        flip_step_impl(ctx, scenes, params, nscenes, "gfsx_flip_step_batched");
     });
 }
 
@@ -596,8 +698,15 @@ void gfsx_cell_sort_device(gfsx_ctx *ctx, const float *x, const float *y,
         bg.off[0] = bg.off[1] = bg.off[2] = 0.0f;
         bg.inv_dx = 1.0 / dx;
         bg.nbin = (uint32_t)((int64_t)ni * nj * nk);
-        gfsx::SortBuffers sb = ctx->sort_buffers(n, bg.nbin);
-        gfsx::cell_sort(nullptr, x, y, z, n, bg, sb, ctx->stream);
+        std::vector<gfsx::SceneDesc> t(1);
+        std::memset(t.data(), 0, sizeof(gfsx::SceneDesc));
+        t[0].xin[0] = x;
+        t[0].xin[1] = y;
+        t[0].xin[2] = z;
+        t[0].n = n;
+        t[0].bg = bg;
+        gfsx::SortBuffers sb = ctx->sort_buffers(t, n, bg.nbin + 1, (uint32_t)gfsx::sort_blocks(n));
+        gfsx::cell_sort(sb, n, ctx->stream, false);
         if (n > 0)
             GFSX_HIP(hipMemcpyAsync(perm, sb.perm, sizeof(uint32_t) * n,
                                     hipMemcpyDeviceToDevice, ctx->stream));

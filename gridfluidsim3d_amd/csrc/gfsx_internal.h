@@ -8,6 +8,7 @@
 #include <cstdint>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 namespace gfsx {
 
@@ -46,6 +47,40 @@ struct BinGrid {
     float off[3];   // FP32 shift applied to positions before binning
     double inv_dx;  // 1.0 / dx, rounded on the host like the reference
     uint32_t nbin;  // b[0]*b[1]*b[2]; also the index of the overflow bin
+};
+
+// One scene of a (batched) fused step or of a cell sort, in a device table
+// that the kernels index by particle, bin, binning block or brick: every
+// scene owns a contiguous range of each (the offsets p0, b0, blk0, k0, g0
+// grow with the scene index).  Bins of a scene: its bin grid bg (nbin
+// in-grid bins), then its overflow bin (local index nbin).
+struct SceneDesc {
+    // particles: binning input (AoS when aos is set), step input and output
+    const float *aos;
+    const float *xin[3], *vin[3];
+    float *xout[3], *vout[3];
+    const uint32_t *id_in;
+    uint32_t *id_out;
+    int64_t n;        // particles
+    int64_t p0;       // first global particle index (also its first sort slot)
+    BinGrid bg;
+    uint32_t b0;      // first global bin
+    uint32_t blk0;    // first binning block
+    // P2G bricks (4^3 cells) and G2P bricks (8^3 cells)
+    int nbx, nby, nbz;
+    uint32_t k0;
+    int gbx, gby, gbz;
+    uint32_t g0;
+    // grid and step constants
+    int ni, nj, nk;
+    double dx;
+    float limf[3];     // MacGrid::limf of the scene's grid (set_limits)
+    double lim[3];
+    float *grid[3], *grid_old[3], *wgt[3];
+    float grav_delta[3];
+    int grav_on[3];
+    float h_half, h_full, h_sixth;   // (float)(0.5*dt), (float)dt, (float)(dt/6.0f)
+    float r_pic, r_flip;             // (float)ratio, (float)(1-ratio)
 };
 
 // Node grid of one splatted component.  Node n sits at off + n*dx; its
@@ -110,8 +145,12 @@ void exclusive_scan(const uint32_t *in, uint32_t *out, int64_t m,
                     uint32_t *scratch, hipStream_t s);
 
 struct SortBuffers {
-    uint32_t *count;      // nbin+1
-    uint32_t *start;      // nbin+2
+    const SceneDesc *scenes;   // device table
+    int nscene;
+    uint32_t nbin_total;  // sum over scenes of (in-grid bins + 1 overflow bin)
+    uint32_t nblk_total;  // binning blocks, sum over scenes
+    uint32_t *count;      // nbin_total
+    uint32_t *start;      // nbin_total+1
     uint32_t *key;        // n
     uint32_t *slot;       // n
     uint32_t *perm;       // n: the result
@@ -122,26 +161,26 @@ struct SortBuffers {
     uint32_t *big_list;   // big_cap entries: bins too large for one thread
     uint32_t big_cap;
     uint32_t *big_count;  // 2
-    uint32_t *block_out;  // blocks(n)
-    uint32_t *block_out_scan;  // blocks(n)+1
+    uint32_t *block_out;  // nblk_total
+    uint32_t *block_out_scan;  // nblk_total+1
     uint32_t *scan_scratch;
     uint8_t *oct;         // n: half of its cell each particle lies in
     uint8_t *poct;        // n: the same, in perm order
 };
 int64_t sort_blocks(int64_t n);
 
-// Stable counting sort of particle indices by bin: perm lists bin 0's
-// particles in index order, then bin 1's, ..., then the overflow bin's.
-// start[b] is the first perm slot of bin b; start[nbin+1] == n.
-// With octant_order, particles inside a bin are ordered by (octant, index)
-// instead, the octant being the 3-bit index of the half of the cell the
-// particle lies in along x, y, z (so each half-cell is a contiguous run),
-// hrank[q] receives the rank of perm[q] inside its half-cell run and
-// soct[q] its octant.
-// aos != nullptr selects AoS input (x,y,z interleaved); otherwise SoA.
-void cell_sort(const float *aos, const float *x, const float *y,
-               const float *z, int64_t n, const BinGrid &g,
-               const SortBuffers &sb, hipStream_t s, bool octant_order = false);
+// Stable counting sort of particle indices by bin, for every scene of
+// sb.scenes at once: perm lists scene 0's bins in order (each bin's particles
+// in index order, the overflow bin last), then scene 1's, ...; start[b] is
+// the first perm slot of global bin b, start[nbin_total] == n.  Indices are
+// global (scene p0 + local index).
+// With octant_order, particles inside an in-grid bin are ordered by
+// (octant, index) instead, the octant being the 3-bit index of the half of
+// the cell the particle lies in along x, y, z (so each half-cell is a
+// contiguous run), hrank[q] receives the rank of perm[q] inside its
+// half-cell run and soct[q] its octant, and brick_flag (when set) the P2G
+// bricks with a particle within reach.
+void cell_sort(const SortBuffers &sb, int64_t n, hipStream_t s, bool octant_order);
 
 // Generic P2G gather for one component (ScalarField semantics).
 void p2g_gather(const float *aos, const float *val, int64_t n,
@@ -157,32 +196,38 @@ void g2p(const float *pos, int64_t n, const MacGrid &g, int stencil,
 void advect(const float *pos, int64_t n, const MacGrid &g, int order,
             double dt, int stencil, float *out, hipStream_t s);
 
-struct StepArgs {
-    int64_t n;
-    int ni, nj, nk;
-    double dx;
-    const float *xin[3], *vin[3];
-    float *xout[3], *vout[3];
-    float *xs[3], *vs[3];            // workspace: the input in sorted order
-    uint32_t *dense, *dense_count;   // workspace: P2G bricks over capacity
-    const uint32_t *id_in;
-    uint32_t *id_out;
-    float *grid[3], *grid_old[3], *wgt[3];
-    float grav_delta[3];
-    int grav_on[3];
-    float h_half, h_full, h_sixth;   // (float)(0.5*dt), (float)dt, (float)(dt/6.0f)
-    float r_pic, r_flip;             // (float)ratio, (float)(1-ratio)
-    int stencil;
-    int keep_order;
-    int p2g_variant;   // 0: brick-staged gather, 1: direct global gather
-    int g2p_variant;   // 0: brick-staged LDS boxes, 1: direct global sampling
-    int num_cu;        // compute units of the device (persistent launches)
+// Sizes of a scene table laid out by layout_scenes.
+struct SceneTotals {
+    int64_t n = 0;       // particles
+    int64_t nbin = 0;    // bins, overflow bins included
+    int64_t nblk = 0;    // binning blocks
+    int64_t np2g = 0;    // P2G bricks
+    int64_t ng2p = 0;    // G2P bricks
+    int64_t nface = 0;   // MAC faces
 };
 
-// The fused plain step.  Needs SortBuffers sized for n particles and
-// (ni+2)*(nj+2)*(nk+2) bins.  When ev is not null, ev[0..3] are recorded on
-// the stream before the sort, after the sort, after P2G and after G2P.
-void flip_step(const StepArgs &a, const SortBuffers &sb, hipStream_t s,
+// Fill the derived fields of every scene (bin grid, offsets p0/b0/blk0/k0/g0,
+// brick counts, grid limits) from n, ni/nj/nk and dx, in table order.
+void layout_scenes(SceneDesc *scenes, int nscene, SceneTotals &tot);
+
+// A fused step over one or more scenes.
+struct StepBatch {
+    std::vector<SceneDesc> scenes;   // host copy, laid out
+    const SceneDesc *scenes_dev;     // the same table on the device
+    SceneTotals tot;
+    float *xs[3], *vs[3];            // workspace: all particles in sort order
+    uint32_t *dense, *dense_count;   // workspace: P2G bricks over capacity
+    int stencil;
+    int keep_order;
+    int p2g_variant;   // 0: brick-staged gather, 1: direct global gather (one scene)
+    int g2p_variant;   // 0: brick-staged LDS boxes, 1: direct global sampling (one scene)
+};
+
+// The fused plain step of every scene of s, in one launch per kernel.
+// Needs SortBuffers laid out for s.tot (and sb.scenes == s.scenes_dev).
+// When ev is not null, ev[0..3] are recorded on the stream before the sort,
+// after the sort, after P2G and after G2P.
+void flip_step(const StepBatch &s, const SortBuffers &sb, hipStream_t st,
                hipEvent_t *ev = nullptr);
 
 }  // namespace gfsx

@@ -204,39 +204,54 @@ constexpr int64_t kBinTile = (int64_t)kBinItems * kBlock;
 
 int64_t sort_blocks(int64_t n) { return (n + kBinTile - 1) / kBinTile; }
 
-// Bin every particle.  In-grid particles take a slot in their bin with an
-// integer atomic; lanes of a wave that hold a run of equal bins (the input
-// is usually close to cell order) take one atomic per run and number
-// themselves within it in lane order.  The order inside a bin is then
-// fixed by k_order_bins / k_order_big.  Overflow particles get their rank
-// among the block's overflow particles (in index order) from wave ballots,
-// so that bin is filled in index order directly.  With an octant array,
-// each particle also records which half of its cell it occupies along
-// each axis.
-template <bool AOS>
-__global__ __launch_bounds__(kBlock) void k_bin(const float *aos,
-                                               const float *x, const float *y,
-                                               const float *z, int64_t n,
-                                               BinGrid g, uint32_t *count,
-                                               uint32_t *key, uint32_t *slot,
-                                               uint32_t *block_out,
+// Index of the scene that owns value v of a per-scene ascending offset
+// (p0, b0, blk0, k0 or g0): the last scene whose offset is <= v.
+template <class F>
+__device__ __forceinline__ int scene_of(const SceneDesc *__restrict__ sc, int nsc, uint64_t v,
+                                        F offset) {
+    int lo = 0, hi = nsc - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if ((uint64_t)offset(sc[mid]) <= v) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Bin every particle.  A block takes kBinTile consecutive particles of one
+// scene.  In-grid particles take a slot in their bin with an integer
+// atomic; lanes of a wave that hold a run of equal bins (the input is
+// usually close to cell order) take one atomic per run and number themselves
+// within it in lane order.  The order inside a bin is then fixed by
+// k_order_bins / k_order_big.  Overflow particles get their rank among the
+// block's overflow particles (in index order) from wave ballots, so a
+// scene's overflow bin is filled in index order directly.  With an octant
+// array, each particle also records which half of its cell it occupies
+// along each axis.
+__global__ __launch_bounds__(kBlock) void k_bin(const SceneDesc *__restrict__ scenes, int nsc,
+                                               uint32_t *count, uint32_t *key,
+                                               uint32_t *slot, uint32_t *block_out,
                                                uint8_t *oct) {
     __shared__ uint32_t wcnt[kBinItems][kBlock / 64];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    const int64_t base = (int64_t)blockIdx.x * kBinTile + threadIdx.x;
+    const uint32_t blk = blockIdx.x;
+    const SceneDesc &sd = scenes[scene_of(scenes, nsc, blk, [](const SceneDesc &d) { return d.blk0; })];
+    const BinGrid g = sd.bg;
+    const int64_t n = sd.n;
+    const int64_t base = (int64_t)(blk - sd.blk0) * kBinTile + threadIdx.x;
     float px[kBinItems], py[kBinItems], pz[kBinItems];
 #pragma unroll
     for (int m = 0; m < kBinItems; m++) {
         const int64_t i = base + (int64_t)m * kBlock;
         const int64_t ii = i < n ? i : 0;
-        if (AOS) {
-            px[m] = aos[3 * ii];
-            py[m] = aos[3 * ii + 1];
-            pz[m] = aos[3 * ii + 2];
+        if (sd.aos) {
+            px[m] = sd.aos[3 * ii];
+            py[m] = sd.aos[3 * ii + 1];
+            pz[m] = sd.aos[3 * ii + 2];
         } else {
-            px[m] = x[ii];
-            py[m] = y[ii];
-            pz[m] = z[ii];
+            px[m] = sd.xin[0][ii];
+            py[m] = sd.xin[1][ii];
+            pz[m] = sd.xin[2][ii];
         }
     }
     uint32_t b[kBinItems], sl[kBinItems];
@@ -268,7 +283,7 @@ __global__ __launch_bounds__(kBlock) void k_bin(const float *aos,
         uint32_t rbase = 0;
         if (rstart) {
             const int e0 = __ffsll((unsigned long long)(E & ~((1ull << lane) - 1ull))) - 1;
-            rbase = atomicAdd(&count[b[m]], (uint32_t)(e0 - lane + 1));
+            rbase = atomicAdd(&count[sd.b0 + b[m]], (uint32_t)(e0 - lane + 1));
         }
         rbase = __shfl(rbase, s0, 64);
         sl[m] = rbase + (uint32_t)(lane - s0);
@@ -293,38 +308,54 @@ __global__ __launch_bounds__(kBlock) void k_bin(const float *aos,
     for (int m = 0; m < kBinItems; m++) {
         const int64_t i = base + (int64_t)m * kBlock;
         if (i < n) {
-            key[i] = b[m];
-            slot[i] = sl[m];
-            if (oct) oct[i] = o[m];
+            key[sd.p0 + i] = sd.b0 + b[m];
+            slot[sd.p0 + i] = sl[m];
+            if (oct) oct[sd.p0 + i] = o[m];
         }
     }
-    if (threadIdx.x == 0) block_out[blockIdx.x] = tot;
+    if (threadIdx.x == 0) block_out[blk] = tot;
+}
+
+// Each scene's overflow count: the sum of its blocks' overflow counts.
+__global__ void k_overflow_counts(const SceneDesc *__restrict__ scenes, int nsc,
+                                  const uint32_t *__restrict__ block_out_scan,
+                                  uint32_t nblk_total, uint32_t *count) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsc) return;
+    const uint32_t e = s + 1 < nsc ? scenes[s + 1].blk0 : nblk_total;
+    count[scenes[s].b0 + scenes[s].bg.nbin] = block_out_scan[e] - block_out_scan[scenes[s].blk0];
 }
 
 __global__ void k_copy_one(uint32_t *dst, const uint32_t *src) { *dst = *src; }
 
 __global__ __launch_bounds__(kBlock) void k_scatter(
-    const uint32_t *key, const uint32_t *slot, int64_t n,
-    const uint32_t *start, const uint32_t *block_out_scan, uint32_t nbin,
-    uint32_t *perm, const uint8_t *oct, uint8_t *poct) {
-    const int64_t base = (int64_t)blockIdx.x * kBinTile + threadIdx.x;
+    const SceneDesc *__restrict__ scenes, int nsc, const uint32_t *key, const uint32_t *slot,
+    const uint32_t *start, const uint32_t *block_out_scan, uint32_t *perm,
+    const uint8_t *oct, uint8_t *poct) {
+    const uint32_t blk = blockIdx.x;
+    const SceneDesc &sd = scenes[scene_of(scenes, nsc, blk, [](const SceneDesc &d) { return d.blk0; })];
+    const int64_t n = sd.n;
+    const int64_t base = (int64_t)(blk - sd.blk0) * kBinTile + threadIdx.x;
+    const uint32_t ovf = sd.b0 + sd.bg.nbin;
     uint32_t b[kBinItems], sl[kBinItems], st[kBinItems];
 #pragma unroll
     for (int m = 0; m < kBinItems; m++) {
         const int64_t i = base + (int64_t)m * kBlock;
-        b[m] = i < n ? key[i] : nbin;
-        sl[m] = i < n ? slot[i] : 0u;
+        b[m] = i < n ? key[sd.p0 + i] : ovf;
+        sl[m] = i < n ? slot[sd.p0 + i] : 0u;
     }
-    const uint32_t over0 = start[nbin] + block_out_scan[blockIdx.x];
+    // the scene's overflow bin: this block's particles follow the overflow
+    // particles of the scene's earlier blocks
+    const uint32_t over0 = start[ovf] + block_out_scan[blk] - block_out_scan[sd.blk0];
 #pragma unroll
-    for (int m = 0; m < kBinItems; m++) st[m] = b[m] == nbin ? over0 : start[b[m]];
+    for (int m = 0; m < kBinItems; m++) st[m] = b[m] == ovf ? over0 : start[b[m]];
 #pragma unroll
     for (int m = 0; m < kBinItems; m++) {
         const int64_t i = base + (int64_t)m * kBlock;
         if (i < n) {
             const uint32_t dst = st[m] + sl[m];
-            perm[dst] = (uint32_t)i;
-            if (poct) poct[dst] = oct[i];
+            perm[dst] = (uint32_t)(sd.p0 + i);
+            if (poct) poct[dst] = oct[sd.p0 + i];
         }
     }
 }
@@ -405,32 +436,27 @@ __device__ __forceinline__ void write_sorted(const T (&v)[N], uint32_t s,
 }
 
 // P2G bricks (edge kMarkBrick cells) whose staged cells (the brick plus one
-// cell around it) hold a particle: flag[brick] = 1.
+// cell around it) hold a particle: flag[k0 + brick] = 1.
 constexpr int kMarkBrick = 4;
-struct BrickMarks {
-    uint8_t *flag;   // nullable
-    int b0, b1;      // bin grid dims x, y
-    int nb[3];       // bricks per axis
-};
 
-// Bin b (cell = bin - 1 per axis) is staged by the bricks bx with
-// 4bx - 1 <= cell <= 4bx + 4, i.e. 4bx <= bin <= 4bx + 5: from
+// Local bin lb of scene sd (cell = bin - 1 per axis) is staged by the bricks
+// bx with 4bx - 1 <= cell <= 4bx + 4, i.e. 4bx <= bin <= 4bx + 5: from
 // ceil((bin - 5)/4) = floor((bin - 2)/4) to floor(bin/4); every one of them
 // is marked (plain stores of 1).
-__device__ __forceinline__ void mark_bricks(const BrickMarks &m, uint32_t b) {
-    const int ix = (int)(b % (uint32_t)m.b0), iy = (int)((b / (uint32_t)m.b0) % (uint32_t)m.b1);
-    const int iz = (int)(b / ((uint32_t)m.b0 * (uint32_t)m.b1));
-    const int idx[3] = {ix, iy, iz};
+__device__ __forceinline__ void mark_bricks(const SceneDesc &sd, uint8_t *flag, uint32_t lb) {
+    const uint32_t b0 = (uint32_t)sd.bg.b[0], b1 = (uint32_t)sd.bg.b[1];
+    const int idx[3] = {(int)(lb % b0), (int)((lb / b0) % b1), (int)(lb / (b0 * b1))};
+    const int nb[3] = {sd.nbx, sd.nby, sd.nbz};
     int lo[3], hi[3];
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         lo[a] = idx[a] >= 2 ? (idx[a] - 2) / kMarkBrick : 0;
-        hi[a] = min(idx[a] / kMarkBrick, m.nb[a] - 1);
+        hi[a] = min(idx[a] / kMarkBrick, nb[a] - 1);
     }
     for (int z = lo[2]; z <= hi[2]; z++)
         for (int y = lo[1]; y <= hi[1]; y++)
             for (int x = lo[0]; x <= hi[0]; x++)
-                m.flag[x + m.nb[0] * (y + m.nb[1] * z)] = 1;
+                flag[sd.k0 + x + nb[0] * (y + nb[1] * z)] = 1;
 }
 
 constexpr uint32_t kSmallBin = 16;   // bins up to this size: a thread, in registers
@@ -445,17 +471,22 @@ constexpr uint32_t kMidBin = 256;    // bins up to this size: a wave, in registe
 // big_count[1]).
 template <bool NARROW>
 __global__ __launch_bounds__(kBlock) void k_order_bins(
-    const uint32_t *__restrict__ start, uint32_t nbin,
-    const uint32_t *__restrict__ perm_in, const uint8_t *__restrict__ poct, OrderOut out,
-    uint32_t *__restrict__ big_list, uint32_t big_cap, uint32_t *__restrict__ big_count,
-    BrickMarks marks) {
+    const SceneDesc *__restrict__ scenes, int nsc, const uint32_t *__restrict__ start,
+    uint32_t nbin, const uint32_t *__restrict__ perm_in, const uint8_t *__restrict__ poct,
+    OrderOut out, uint32_t *__restrict__ big_list, uint32_t big_cap,
+    uint32_t *__restrict__ big_count, uint8_t *__restrict__ brick_flag) {
     const uint32_t b = blockIdx.x * kBlock + threadIdx.x;
     uint32_t s = 0, len = 0;
     if (b < nbin) {
         s = start[b];
         len = start[b + 1] - s;
     }
-    if (len > 0 && marks.flag) mark_bricks(marks, b);
+    if (len > 0) {
+        const SceneDesc &sd = scenes[scene_of(scenes, nsc, b, [](const SceneDesc &d) { return d.b0; })];
+        const uint32_t lb = b - sd.b0;
+        if (lb == sd.bg.nbin) len = 0;   // overflow bin: already in index order (k_order_big copies it)
+        else if (brick_flag) mark_bricks(sd, brick_flag, lb);
+    }
     if (len > kSmallBin) {
         // the wave-sorted sizes go to the front of the list, the rest to
         // the back
@@ -596,18 +627,21 @@ constexpr int kOrderMidBlocks = 4096;
 
 // Bins of more than kMidBin entries: one wave per bin ranks every entry by
 // counting the smaller keys of its bin (tiles of the keys pass through LDS,
-// so any size works).  Also copies the overflow bin, which k_bin already
-// filled in index order.
+// so any size works).  Also copies the scenes' overflow bins, which k_bin
+// already filled in index order.
 __global__ __launch_bounds__(64) void k_order_big(
-    const uint32_t *__restrict__ start, uint32_t nbin,
+    const SceneDesc *__restrict__ scenes, int nsc, const uint32_t *__restrict__ start,
     const uint32_t *__restrict__ perm_in, const uint8_t *__restrict__ poct, OrderOut out,
     const uint32_t *__restrict__ big_list, uint32_t big_cap,
     const uint32_t *__restrict__ big_count) {
     __shared__ uint64_t tile[kBigTile];
     const uint32_t lane = threadIdx.x;
-    const uint32_t o0 = start[nbin], o1 = start[nbin + 1];
-    for (uint32_t q = o0 + blockIdx.x * 64u + lane; q < o1; q += gridDim.x * 64u)
-        out.perm[q] = perm_in[q];
+    for (int sc = 0; sc < nsc; sc++) {
+        const uint32_t ovf = scenes[sc].b0 + scenes[sc].bg.nbin;
+        const uint32_t o0 = start[ovf], o1 = start[ovf + 1];
+        for (uint32_t q = o0 + blockIdx.x * 64u + lane; q < o1; q += gridDim.x * 64u)
+            out.perm[q] = perm_in[q];
+    }
     const uint32_t nbig = big_count[1];
     for (uint32_t c = blockIdx.x; c < nbig; c += gridDim.x) {
         const uint32_t b = big_list[big_cap - 1u - c];
@@ -633,61 +667,47 @@ __global__ __launch_bounds__(64) void k_order_big(
     }
 }
 
-void cell_sort(const float *aos, const float *x, const float *y,
-               const float *z, int64_t n, const BinGrid &g,
-               const SortBuffers &sb, hipStream_t s, bool octant_order) {
-    fill_u32(sb.count, 0u, (int64_t)g.nbin + 1, s);
+void cell_sort(const SortBuffers &sb, int64_t n, hipStream_t s, bool octant_order) {
+    fill_u32(sb.count, 0u, (int64_t)sb.nbin_total, s);
     if (n <= 0) {
-        fill_u32(sb.start, 0u, (int64_t)g.nbin + 2, s);
+        fill_u32(sb.start, 0u, (int64_t)sb.nbin_total + 1, s);
         return;
     }
     uint8_t *oct = octant_order ? sb.oct : nullptr;
     uint8_t *poct = octant_order ? sb.poct : nullptr;
-    unsigned nb = (unsigned)sort_blocks(n);
-    if (aos)
-        hipLaunchKernelGGL(k_bin<true>, dim3(nb), dim3(kBlock), 0, s, aos, x,
-                           y, z, n, g, sb.count, sb.key, sb.slot, sb.block_out, oct);
-    else
-        hipLaunchKernelGGL(k_bin<false>, dim3(nb), dim3(kBlock), 0, s, aos, x,
-                           y, z, n, g, sb.count, sb.key, sb.slot, sb.block_out, oct);
+    const unsigned nb = sb.nblk_total;
+    hipLaunchKernelGGL(k_bin, dim3(nb), dim3(kBlock), 0, s, sb.scenes, sb.nscene, sb.count,
+                       sb.key, sb.slot, sb.block_out, oct);
     GFSX_LAUNCH_CHECK();
     exclusive_scan(sb.block_out, sb.block_out_scan, nb, sb.scan_scratch, s);
-    hipLaunchKernelGGL(k_copy_one, dim3(1), dim3(1), 0, s, sb.count + g.nbin,
-                       (const uint32_t *)(sb.block_out_scan + nb));
+    hipLaunchKernelGGL(k_overflow_counts, dim3(grid_for(sb.nscene, 64)), dim3(64), 0, s,
+                       sb.scenes, sb.nscene, (const uint32_t *)sb.block_out_scan, nb, sb.count);
     GFSX_LAUNCH_CHECK();
-    exclusive_scan(sb.count, sb.start, (int64_t)g.nbin + 1, sb.scan_scratch, s);
-    hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(kBlock), 0, s,
-                       (const uint32_t *)sb.key, (const uint32_t *)sb.slot, n,
-                       (const uint32_t *)sb.start,
-                       (const uint32_t *)sb.block_out_scan, g.nbin, sb.perm_tmp,
-                       (const uint8_t *)oct, poct);
+    exclusive_scan(sb.count, sb.start, (int64_t)sb.nbin_total, sb.scan_scratch, s);
+    hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(kBlock), 0, s, sb.scenes, sb.nscene,
+                       (const uint32_t *)sb.key, (const uint32_t *)sb.slot,
+                       (const uint32_t *)sb.start, (const uint32_t *)sb.block_out_scan,
+                       sb.perm_tmp, (const uint8_t *)oct, poct);
     GFSX_LAUNCH_CHECK();
     fill_u32(sb.big_count, 0u, 2, s);
     OrderOut out;
     out.perm = sb.perm;
     out.hrank = octant_order ? sb.hrank : nullptr;
     out.soct = octant_order ? sb.soct : nullptr;
-    BrickMarks marks;
-    marks.flag = nullptr;
-    if (octant_order && sb.brick_flag) {
-        marks.flag = sb.brick_flag;
-        marks.b0 = g.b[0];
-        marks.b1 = g.b[1];
-        for (int a = 0; a < 3; a++) marks.nb[a] = (g.b[a] - 2 + kMarkBrick - 1) / kMarkBrick;
-        fill_u8(sb.brick_flag, 0, (int64_t)marks.nb[0] * marks.nb[1] * marks.nb[2], s);
-    }
+    uint8_t *flag = octant_order ? sb.brick_flag : nullptr;
     auto *order = n < ((int64_t)1 << kNarrowBits) ? k_order_bins<true> : k_order_bins<false>;
-    hipLaunchKernelGGL(order, dim3(grid_for(g.nbin, kBlock)), dim3(kBlock), 0, s,
-                       (const uint32_t *)sb.start, g.nbin, (const uint32_t *)sb.perm_tmp,
-                       (const uint8_t *)poct, out, sb.big_list, sb.big_cap, sb.big_count, marks);
+    hipLaunchKernelGGL(order, dim3(grid_for(sb.nbin_total, kBlock)), dim3(kBlock), 0, s,
+                       sb.scenes, sb.nscene, (const uint32_t *)sb.start, sb.nbin_total,
+                       (const uint32_t *)sb.perm_tmp, (const uint8_t *)poct, out, sb.big_list,
+                       sb.big_cap, sb.big_count, flag);
     GFSX_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_order_mid, dim3(kOrderMidBlocks), dim3(kBlock), 0, s,
                        (const uint32_t *)sb.start, (const uint32_t *)sb.perm_tmp,
                        (const uint8_t *)poct, out, (const uint32_t *)sb.big_list,
                        (const uint32_t *)sb.big_count);
     GFSX_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_order_big, dim3(kOrderBigBlocks), dim3(64), 0, s,
-                       (const uint32_t *)sb.start, g.nbin, (const uint32_t *)sb.perm_tmp,
+    hipLaunchKernelGGL(k_order_big, dim3(kOrderBigBlocks), dim3(64), 0, s, sb.scenes,
+                       sb.nscene, (const uint32_t *)sb.start, (const uint32_t *)sb.perm_tmp,
                        (const uint8_t *)poct, out, (const uint32_t *)sb.big_list, sb.big_cap,
                        (const uint32_t *)sb.big_count);
     GFSX_LAUNCH_CHECK();
@@ -1101,6 +1121,48 @@ struct StepP2G {
     int grav_on[3];
     int ni, nj, nk;
 };
+
+// The particles of every scene in sort order (slot q of the sort).
+struct Sorted {
+    const float *x[3], *v[3];
+};
+
+// P2G view of a scene: faces of component c sit at off + n*dx with the
+// offset half a cell on the two other axes (fluidsimulation.cpp:2526-2570);
+// a face's candidate bins along axis a are [n + dlo, n + dhi] (pad 1, R = dx).
+__host__ __device__ inline StepP2G p2g_view(const SceneDesc &sd, const Sorted &so) {
+    StepP2G a;
+    a.x = so.x[0];
+    a.y = so.x[1];
+    a.z = so.x[2];
+    const float half = (float)(0.5 * sd.dx);
+    for (int c = 0; c < 3; c++) {
+        a.vel[c] = so.v[c];
+        a.grid[c] = sd.grid[c];
+        a.grid_old[c] = sd.grid_old[c];
+        a.wgt[c] = sd.wgt[c];
+        a.grav_delta[c] = sd.grav_delta[c];
+        a.grav_on[c] = sd.grav_on[c];
+        NodeGrid &ng = a.ng[c];
+        ng.n[0] = sd.ni + (c == 0);
+        ng.n[1] = sd.nj + (c == 1);
+        ng.n[2] = sd.nk + (c == 2);
+        ng.dx_f = (float)sd.dx;
+        for (int ax = 0; ax < 3; ax++) {
+            ng.off[ax] = (ax == c) ? 0.0f : half;
+            // node n sits at bin coordinate n + off/dx + pad; its candidate
+            // bins cover (n + base - R/dx, n + base + R/dx) with R = dx
+            const double base = (double)ng.off[ax] / sd.dx + 1.0;
+            ng.dlo[ax] = (int)floor(base - 1.0);
+            ng.dhi[ax] = (int)ceil(base + 1.0) - 1;
+        }
+        a.nface[c] = (int64_t)ng.n[0] * ng.n[1] * ng.n[2];
+    }
+    a.ni = sd.ni;
+    a.nj = sd.nj;
+    a.nk = sd.nk;
+    return a;
+}
 
 // One thread per face of u, v or w: gather, normalise
 // (scalarfield.cpp:90-106), snapshot (fluidsimulation.cpp:3306), then add
@@ -1623,10 +1685,10 @@ __device__ __forceinline__ void p2g_brick(
 // holding more than CAP particles is appended to dense[] for
 // k_flip_p2g_dense.
 template <int V, int CAP>
-__global__ __launch_bounds__(brick::NTC) void k_flip_p2g_brick(
-    StepP2G a, BinGrid bg, double dx, float inv_r2, int nbx, int nby, int nbz,
-    int per_xcd, const uint32_t *__restrict__ start, const uint32_t *__restrict__ hrank,
-    const uint8_t *__restrict__ soct, const uint8_t *__restrict__ brick_flag,
+__device__ __forceinline__ void p2g_brick_rows(
+    const StepP2G &a, const BinGrid &bg, double dx, float inv_r2, int nbx, int nby,
+    int brick_id, uint32_t gid, const uint32_t *__restrict__ start,
+    const uint32_t *__restrict__ hrank, const uint8_t *__restrict__ soct, bool flagged,
     uint32_t *__restrict__ dense, uint32_t *__restrict__ dense_count) {
     using namespace brick;
     constexpr int NCS = SX * SY * SZ;          // staged cells
@@ -1647,10 +1709,6 @@ __global__ __launch_bounds__(brick::NTC) void k_flip_p2g_brick(
     // once the layout is known
     uint32_t *s_cnt = reinterpret_cast<uint32_t *>(s_buf);
 
-    // consecutive bricks share halo particles: keep them on one XCD
-    // (work-groups are dealt round-robin over the 8 XCDs)
-    const int brick_id = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
-    if (brick_id >= nbx * nby * nbz) return;
     const int bxi = brick_id % nbx, byi = (brick_id / nbx) % nby, bzi = brick_id / (nbx * nby);
     const int i0 = bxi * BX, j0 = byi * BY, k0 = bzi * BZ;
     const int t = threadIdx.x;
@@ -1688,7 +1746,7 @@ __global__ __launch_bounds__(brick::NTC) void k_flip_p2g_brick(
             v = start[row + (uint32_t)min(i0 + k, bg.b[0])];
         }
     }
-    if (brick_flag[brick_id] == 0) {
+    if (!flagged) {
         // no particle within reach of any face of this brick: all values
         // are 0 and no face borders fluid
         if (ok) brick_zero_face(a, c, i0 + fi, j0 + fj, k0 + fk);
@@ -1701,7 +1759,7 @@ __global__ __launch_bounds__(brick::NTC) void k_flip_p2g_brick(
 #pragma unroll 4
     for (int r = 0; r < NROW; r++) total += s_bs[r][NB - 1] - s_bs[r][0];
     if (total > (uint32_t)CAP) {
-        if (t == 0) dense[atomicAdd(dense_count, 1u)] = (uint32_t)brick_id;
+        if (t == 0) dense[atomicAdd(dense_count, 1u)] = gid;
         return;
     }
     if (total == 0) {
@@ -1826,14 +1884,36 @@ __global__ __launch_bounds__(brick::NTC) void k_flip_p2g_brick(
     }
 }
 
+// One work-group per P2G brick of any scene (global brick numbering).
+template <int V, int CAP>
+__global__ __launch_bounds__(brick::NTC) void k_flip_p2g_brick(
+    const SceneDesc *__restrict__ scenes, int nsc, Sorted so, uint32_t nbrick, int per_xcd,
+    const uint32_t *__restrict__ start, const uint32_t *__restrict__ hrank,
+    const uint8_t *__restrict__ soct, const uint8_t *__restrict__ brick_flag,
+    uint32_t *__restrict__ dense, uint32_t *__restrict__ dense_count) {
+    // consecutive bricks share halo particles: keep them on one XCD
+    // (work-groups are dealt round-robin over the 8 XCDs)
+    const uint32_t gid = (blockIdx.x & 7u) * (uint32_t)per_xcd + (blockIdx.x >> 3);
+    if (gid >= nbrick) return;
+    const bool flagged = brick_flag[gid] != 0;
+    const SceneDesc &sd = scenes[scene_of(scenes, nsc, gid, [](const SceneDesc &d) { return d.k0; })];
+    // the scene's view in LDS: its arrays are indexed by component at run
+    // time, which private memory would turn into scratch
+    __shared__ StepP2G s_a;
+    if (threadIdx.x == 0) s_a = p2g_view(sd, so);
+    __syncthreads();
+    p2g_brick_rows<V, CAP>(s_a, sd.bg, sd.dx, (float)(1.0 / (sd.dx * sd.dx)), sd.nbx, sd.nby,
+                           (int)(gid - sd.k0), gid, start + sd.b0, hrank, soct, flagged, dense,
+                           dense_count);
+}
+
 // The bricks listed by k_flip_p2g_brick as over its capacity, in chunks.
 // Three waves per SIMD (a register budget of 168).
 template <int V, int CAP>
 __global__ __launch_bounds__(brick::NT, 3) void k_flip_p2g_dense(
-    StepP2G a, BinGrid bg, double dx, float inv_r2, int nbx, int nby,
-    const uint32_t *__restrict__ count, const uint32_t *__restrict__ start,
-    const uint32_t *__restrict__ hrank, uint32_t *__restrict__ dense,
-    uint32_t *__restrict__ dense_count) {
+    const SceneDesc *__restrict__ scenes, int nsc, Sorted so,
+    const uint32_t *__restrict__ start, const uint32_t *__restrict__ hrank,
+    uint32_t *__restrict__ dense, uint32_t *__restrict__ dense_count) {
     using namespace brick;
     static_assert(NH + 1 <= 6 * CAP, "half-cell counts live in the particle arrays");
     static_assert(CAP <= 65535, "LDS slots are 16-bit");
@@ -1842,18 +1922,23 @@ __global__ __launch_bounds__(brick::NT, 3) void k_flip_p2g_dense(
     __shared__ uint32_t s_rowq[NROW], s_rows0[NROW + 1];
     __shared__ uint32_t s_wsum[NT / 64];
     __shared__ uint8_t s_occ[SX * SY * SZ];
+    __shared__ StepP2G s_a;   // the scene's view (see k_flip_p2g_brick)
     const uint32_t nd = *dense_count;
     for (uint32_t d = blockIdx.x; d < nd; d += gridDim.x) {
         __syncthreads();   // the previous brick is done with LDS
-        p2g_brick<V, CAP, true>(a, bg, dx, inv_r2, nbx, nby, (int)dense[d], count, start,
-                                hrank, dense, dense_count, s_buf, s_hs, s_rowq, s_rows0, s_wsum,
-                                s_occ);
+        const uint32_t gid = dense[d];
+        const SceneDesc &sd = scenes[scene_of(scenes, nsc, gid, [](const SceneDesc &e) { return e.k0; })];
+        if (threadIdx.x == 0) s_a = p2g_view(sd, so);
+        __syncthreads();
+        p2g_brick<V, CAP, true>(s_a, sd.bg, sd.dx, (float)(1.0 / (sd.dx * sd.dx)), sd.nbx, sd.nby,
+                                (int)(gid - sd.k0), nullptr, start + sd.b0, hrank, dense,
+                                dense_count, s_buf, s_hs, s_rowq, s_rows0, s_wsum, s_occ);
     }
 }
 
 struct StepG2P {
-    const float *xin[3], *vin[3];
-    float *xout[3], *vout[3];
+    const float *xin[3], *vin[3];   // in sort order (all scenes)
+    float *xout[3], *vout[3];       // the scene's outputs
     const uint32_t *id_in;
     uint32_t *id_out;
     const float *old[3];
@@ -1861,16 +1946,57 @@ struct StepG2P {
     float r_pic, r_flip;
     int keep_order;
     int64_t n;
+    uint32_t p0;                    // the scene's first sort slot and particle index
 };
 
-// Write the new state of the particle in sorted slot q: at q (sorted
-// order) or at its original index perm[q] (kept order), with its id.
+__host__ __device__ inline StepG2P g2p_view(const SceneDesc &sd, const Sorted &so, int keep_order) {
+    StepG2P b;
+    for (int c = 0; c < 3; c++) {
+        b.xin[c] = so.x[c];
+        b.vin[c] = so.v[c];
+        b.xout[c] = sd.xout[c];
+        b.vout[c] = sd.vout[c];
+        b.old[c] = sd.grid_old[c];
+    }
+    b.id_in = sd.id_in;
+    b.id_out = sd.id_out;
+    b.rk.half = sd.h_half;
+    b.rk.three_q = 0.0f;
+    b.rk.full = sd.h_full;
+    b.rk.sixth = sd.h_sixth;
+    b.rk.ninth = 0.0f;
+    b.r_pic = sd.r_pic;
+    b.r_flip = sd.r_flip;
+    b.keep_order = keep_order;
+    b.n = sd.n;
+    b.p0 = (uint32_t)sd.p0;
+    return b;
+}
+
+__host__ __device__ inline MacGrid mac_view(const SceneDesc &sd) {
+    MacGrid g;
+    for (int c = 0; c < 3; c++) {
+        g.f[c] = sd.grid[c];
+        g.lim[c] = sd.lim[c];
+        g.limf[c] = sd.limf[c];
+    }
+    g.ni = sd.ni;
+    g.nj = sd.nj;
+    g.nk = sd.nk;
+    g.dx = sd.dx;
+    g.inv_dx = 1.0 / sd.dx;
+    return g;
+}
+
+// Write the new state of the particle in sort slot q: at q (sorted order)
+// or at its original index perm[q] (kept order), with its id, all relative
+// to the scene's first particle.
 __device__ __forceinline__ void store_particle(const StepG2P &a,
                                                const uint32_t *__restrict__ perm,
                                                uint32_t q, const float x1[3],
                                                const float vnew[3]) {
-    uint32_t o = q, p = q;
-    if (a.keep_order || a.id_out) p = perm[q];
+    uint32_t o = q - a.p0, p = o;
+    if (a.keep_order || a.id_out) p = perm[q] - a.p0;
     if (a.keep_order) o = p;
 #pragma unroll
     for (int c = 0; c < 3; c++) {
@@ -1881,23 +2007,25 @@ __device__ __forceinline__ void store_particle(const StepG2P &a,
 }
 
 // Positions and velocities copied into sorted order, so that the transfer
-// kernels read them by sort slot with no indirection.
-struct SortedCopy {
-    const float *src[6];
-    float *dst[6];
-};
-
-__global__ __launch_bounds__(kBlock) void k_sorted_copy(SortedCopy c,
-                                                       const uint32_t *__restrict__ perm,
-                                                       int64_t n) {
+// kernels read them by sort slot with no indirection.  Slot q belongs to the
+// scene whose particle range holds q (the sort keeps scenes in order).
+__global__ __launch_bounds__(kBlock) void k_sorted_copy(const SceneDesc *__restrict__ scenes,
+                                                       int nsc, const uint32_t *__restrict__ perm,
+                                                       int64_t n, float *xs0, float *xs1,
+                                                       float *xs2, float *vs0, float *vs1,
+                                                       float *vs2) {
     const int64_t q = blockIdx.x * (int64_t)kBlock + threadIdx.x;
     if (q >= n) return;
-    const uint32_t p = perm[q];
-    float v[6];
-#pragma unroll
-    for (int k = 0; k < 6; k++) v[k] = c.src[k][p];
-#pragma unroll
-    for (int k = 0; k < 6; k++) c.dst[k][q] = v[k];
+    const SceneDesc &sd = scenes[scene_of(scenes, nsc, (uint64_t)q, [](const SceneDesc &d) { return d.p0; })];
+    const uint32_t p = perm[q] - (uint32_t)sd.p0;
+    const float x = sd.xin[0][p], y = sd.xin[1][p], z = sd.xin[2][p];
+    const float vx = sd.vin[0][p], vy = sd.vin[1][p], vz = sd.vin[2][p];
+    xs0[q] = x;
+    xs1[q] = y;
+    xs2[q] = z;
+    vs0[q] = vx;
+    vs1[q] = vy;
+    vs2[q] = vz;
 }
 
 // One thread per particle, in cell order: PIC/FLIP blend
@@ -2082,10 +2210,13 @@ __device__ __forceinline__ void flip_blend(const StepG2P &a, const float v0[3],
     }
 }
 
+// The brick body of k_flip_g2p_brick for brick brick_id of one scene
+// (start: the scene's bin starts, local bin indices).
 template <int ST>
-__global__ __launch_bounds__(gbrick::NT) void k_flip_g2p_brick(
-    StepG2P a, MacGrid g, BinGrid bg, int nbx, int nby, int nbz, int per_xcd,
-    const uint32_t *__restrict__ start, const uint32_t *__restrict__ perm) {
+__device__ __forceinline__ void g2p_brick(const StepG2P &a, const MacGrid &g, const BinGrid &bg,
+                                          int nbx, int nby, int nbz, int brick_id,
+                                          const uint32_t *__restrict__ start,
+                                          const uint32_t *__restrict__ perm) {
     using namespace gbrick;
     using B = Box<ST>;
     __shared__ float s_cur[3][B::E * B::E * B::E];
@@ -2093,29 +2224,6 @@ __global__ __launch_bounds__(gbrick::NT) void k_flip_g2p_brick(
     __shared__ uint32_t s_rowq[MAXROW], s_rows0[MAXROW + 1];
     __shared__ uint32_t s_wsum;
     const int t = threadIdx.x;
-    const int nslot = 8 * per_xcd;
-
-    if ((int)blockIdx.x >= nslot) {
-        // particles beyond the padded bin grid: outside the domain, so every
-        // sample is 0 and the global path returns at once
-        const uint32_t q0 = start[bg.nbin], q1 = start[bg.nbin + 1];
-        for (uint32_t q = q0 + (uint32_t)((blockIdx.x - nslot) * NT + t); q < q1;
-             q += (uint32_t)(NEXTRA * NT)) {
-            const float x0[3] = {a.xin[0][q], a.xin[1][q], a.xin[2][q]};
-            const float v0[3] = {a.vin[0][q], a.vin[1][q], a.vin[2][q]};
-            float k1[3], vo[3], vnew[3], x1[3];
-            sample_mac<ST, true>(g, a.old[0], a.old[1], a.old[2], x0[0], x0[1], x0[2], k1, vo);
-            flip_blend(a, v0, k1, vo, vnew);
-            rk_step<ST>(g, 4, a.rk, x0, k1, x1);
-            store_particle(a, perm, q, x1, vnew);
-        }
-        return;
-    }
-
-    // consecutive bricks share box halos: keep them on one XCD (work-groups
-    // are dealt round-robin over the 8 XCDs)
-    const int brick_id = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
-    if (brick_id >= nbx * nby * nbz) return;
     const int bxi = brick_id % nbx, byi = (brick_id / nbx) % nby, bzi = brick_id / (nbx * nby);
     const int c0x = bxi * GB, c0y = byi * GB, c0z = bzi * GB;
     // bins of the brick (bin = cell + 1); the first and last bricks of an
@@ -2197,72 +2305,112 @@ __global__ __launch_bounds__(gbrick::NT) void k_flip_g2p_brick(
     }
 }
 
-void flip_step(const StepArgs &s, const SortBuffers &sb, hipStream_t st,
-               hipEvent_t *ev) {
-    if (ev) GFSX_HIP(hipEventRecord(ev[0], st));
-    BinGrid bg;
-    bg.b[0] = s.ni + 2;
-    bg.b[1] = s.nj + 2;
-    bg.b[2] = s.nk + 2;
-    bg.pad[0] = bg.pad[1] = bg.pad[2] = 1;
-    bg.off[0] = bg.off[1] = bg.off[2] = 0.0f;
-    bg.inv_dx = 1.0 / s.dx;
-    bg.nbin = (uint32_t)bg.b[0] * (uint32_t)bg.b[1] * (uint32_t)bg.b[2];
-    // cells ordered inside by (octant, index): each half-cell is a run
-    cell_sort(nullptr, s.xin[0], s.xin[1], s.xin[2], s.n, bg, sb, st, true);
-    if (s.n > 0) {
-        SortedCopy sc;
-        for (int c = 0; c < 3; c++) {
-            sc.src[c] = s.xin[c];
-            sc.src[3 + c] = s.vin[c];
-            sc.dst[c] = s.xs[c];
-            sc.dst[3 + c] = s.vs[c];
+// One work-group per G2P brick of any scene (global brick numbering); the
+// last NEXTRA work-groups take the particles beyond each scene's padded bin
+// grid: outside the domain, so every sample is 0 and the global path
+// returns at once.
+template <int ST>
+__global__ __launch_bounds__(gbrick::NT) void k_flip_g2p_brick(
+    const SceneDesc *__restrict__ scenes, int nsc, Sorted so, int keep_order,
+    uint32_t nbrick, int per_xcd, const uint32_t *__restrict__ start,
+    const uint32_t *__restrict__ perm) {
+    using namespace gbrick;
+    const int t = threadIdx.x;
+    const int nslot = 8 * per_xcd;
+    if ((int)blockIdx.x >= nslot) {
+        for (int sc = 0; sc < nsc; sc++) {
+            const SceneDesc &sd = scenes[sc];
+            const StepG2P a = g2p_view(sd, so, keep_order);
+            const MacGrid g = mac_view(sd);
+            const uint32_t ovf = sd.b0 + sd.bg.nbin;
+            const uint32_t q0 = start[ovf], q1 = start[ovf + 1];
+            for (uint32_t q = q0 + (uint32_t)((blockIdx.x - nslot) * NT + t); q < q1;
+                 q += (uint32_t)(NEXTRA * NT)) {
+                const float x0[3] = {a.xin[0][q], a.xin[1][q], a.xin[2][q]};
+                const float v0[3] = {a.vin[0][q], a.vin[1][q], a.vin[2][q]};
+                float k1[3], vo[3], vnew[3], x1[3];
+                sample_mac<ST, true>(g, a.old[0], a.old[1], a.old[2], x0[0], x0[1], x0[2], k1, vo);
+                flip_blend(a, v0, k1, vo, vnew);
+                rk_step<ST>(g, 4, a.rk, x0, k1, x1);
+                store_particle(a, perm, q, x1, vnew);
+            }
         }
-        hipLaunchKernelGGL(k_sorted_copy, dim3(grid_for(s.n, kBlock)), dim3(kBlock), 0, st,
-                           sc, (const uint32_t *)sb.perm, s.n);
+        return;
+    }
+    // consecutive bricks share box halos: keep them on one XCD (work-groups
+    // are dealt round-robin over the 8 XCDs)
+    const uint32_t gid = (blockIdx.x & 7u) * (uint32_t)per_xcd + (blockIdx.x >> 3);
+    if (gid >= nbrick) return;
+    const SceneDesc &sd = scenes[scene_of(scenes, nsc, gid, [](const SceneDesc &d) { return d.g0; })];
+    const StepG2P a = g2p_view(sd, so, keep_order);
+    const MacGrid g = mac_view(sd);
+    g2p_brick<ST>(a, g, sd.bg, sd.gbx, sd.gby, sd.gbz, (int)(gid - sd.g0), start + sd.b0, perm);
+}
+
+void layout_scenes(SceneDesc *sc, int nsc, SceneTotals &tot) {
+    tot = SceneTotals();
+    for (int i = 0; i < nsc; i++) {
+        SceneDesc &d = sc[i];
+        BinGrid &bg = d.bg;
+        bg.b[0] = d.ni + 2;
+        bg.b[1] = d.nj + 2;
+        bg.b[2] = d.nk + 2;
+        bg.pad[0] = bg.pad[1] = bg.pad[2] = 1;
+        bg.off[0] = bg.off[1] = bg.off[2] = 0.0f;
+        bg.inv_dx = 1.0 / d.dx;
+        bg.nbin = (uint32_t)bg.b[0] * (uint32_t)bg.b[1] * (uint32_t)bg.b[2];
+        d.p0 = tot.n;
+        d.b0 = (uint32_t)tot.nbin;
+        d.blk0 = (uint32_t)tot.nblk;
+        d.nbx = (d.ni + brick::BX - 1) / brick::BX;
+        d.nby = (d.nj + brick::BY - 1) / brick::BY;
+        d.nbz = (d.nk + brick::BZ - 1) / brick::BZ;
+        d.k0 = (uint32_t)tot.np2g;
+        d.gbx = (d.ni + gbrick::GB - 1) / gbrick::GB;
+        d.gby = (d.nj + gbrick::GB - 1) / gbrick::GB;
+        d.gbz = (d.nk + gbrick::GB - 1) / gbrick::GB;
+        d.g0 = (uint32_t)tot.ng2p;
+        MacGrid g;
+        g.ni = d.ni;
+        g.nj = d.nj;
+        g.nk = d.nk;
+        g.dx = d.dx;
+        set_limits(g);
+        for (int c = 0; c < 3; c++) {
+            d.lim[c] = g.lim[c];
+            d.limf[c] = g.limf[c];
+        }
+        tot.n += d.n;
+        tot.nbin += (int64_t)bg.nbin + 1;
+        tot.nblk += sort_blocks(d.n);
+        tot.np2g += (int64_t)d.nbx * d.nby * d.nbz;
+        tot.ng2p += (int64_t)d.gbx * d.gby * d.gbz;
+        tot.nface += (int64_t)(d.ni + 1) * d.nj * d.nk + (int64_t)d.ni * (d.nj + 1) * d.nk +
+                     (int64_t)d.ni * d.nj * (d.nk + 1);
+    }
+}
+
+void flip_step(const StepBatch &s, const SortBuffers &sb, hipStream_t st, hipEvent_t *ev) {
+    if (ev) GFSX_HIP(hipEventRecord(ev[0], st));
+    const int nsc = (int)s.scenes.size();
+    // cells ordered inside by (octant, index): each half-cell is a run
+    cell_sort(sb, s.tot.n, st, true);
+    Sorted so;
+    for (int c = 0; c < 3; c++) {
+        so.x[c] = s.xs[c];
+        so.v[c] = s.vs[c];
+    }
+    if (s.tot.n > 0) {
+        hipLaunchKernelGGL(k_sorted_copy, dim3(grid_for(s.tot.n, kBlock)), dim3(kBlock), 0, st,
+                           s.scenes_dev, nsc, (const uint32_t *)sb.perm, s.tot.n, s.xs[0],
+                           s.xs[1], s.xs[2], s.vs[0], s.vs[1], s.vs[2]);
         GFSX_LAUNCH_CHECK();
     }
     if (ev) GFSX_HIP(hipEventRecord(ev[1], st));
 
-    StepP2G a;
-    a.x = s.xs[0];
-    a.y = s.xs[1];
-    a.z = s.xs[2];
-    const float half = (float)(0.5 * s.dx);
-    for (int c = 0; c < 3; c++) {
-        a.vel[c] = s.vs[c];
-        a.grid[c] = s.grid[c];
-        a.grid_old[c] = s.grid_old[c];
-        a.wgt[c] = s.wgt[c];
-        a.grav_delta[c] = s.grav_delta[c];
-        a.grav_on[c] = s.grav_on[c];
-        NodeGrid &ng = a.ng[c];
-        ng.n[0] = s.ni + (c == 0);
-        ng.n[1] = s.nj + (c == 1);
-        ng.n[2] = s.nk + (c == 2);
-        ng.dx_f = (float)s.dx;
-        for (int ax = 0; ax < 3; ax++) {
-            ng.off[ax] = (ax == c) ? 0.0f : half;
-            // node n sits at bin coordinate n + off/dx + pad; its candidate
-            // bins cover (n + base - R/dx, n + base + R/dx) with R = dx
-            double base = (double)ng.off[ax] / s.dx + 1.0;
-            ng.dlo[ax] = (int)floor(base - 1.0);
-            ng.dhi[ax] = (int)ceil(base + 1.0) - 1;
-        }
-        a.nface[c] = (int64_t)ng.n[0] * ng.n[1] * ng.n[2];
-    }
-    a.ni = s.ni;
-    a.nj = s.nj;
-    a.nk = s.nk;
-    const float inv_r2 = (float)(1.0 / (s.dx * s.dx));
     if (s.p2g_variant != 1) {
-        const int nbx = (s.ni + brick::BX - 1) / brick::BX;
-        const int nby = (s.nj + brick::BY - 1) / brick::BY;
-        const int nbz = (s.nk + brick::BZ - 1) / brick::BZ;
-        const int64_t nbrick = (int64_t)nbx * nby * nbz;
+        const uint32_t nbrick = (uint32_t)s.tot.np2g;
         const int per_xcd = (int)((nbrick + 7) / 8);
-        const dim3 block(brick::NT);
-        const dim3 grid((unsigned)(8 * (int64_t)per_xcd));
         constexpr int cap = brick::CAP;
         auto *kern = s.p2g_variant == 2 ? k_flip_p2g_brick<2, cap>
                    : s.p2g_variant == 3 ? k_flip_p2g_brick<3, cap> : k_flip_p2g_brick<0, cap>;
@@ -2271,71 +2419,50 @@ void flip_step(const StepArgs &s, const SortBuffers &sb, hipStream_t st,
                          : s.p2g_variant == 3 ? k_flip_p2g_dense<3, dcap>
                                               : k_flip_p2g_dense<0, dcap>;
         fill_u32(s.dense_count, 0u, 1, st);
-        hipLaunchKernelGGL(kern, grid, dim3(brick::NTC), 0, st, a, bg, s.dx, inv_r2, nbx, nby,
-                           nbz, per_xcd, (const uint32_t *)sb.start,
+        hipLaunchKernelGGL(kern, dim3((unsigned)(8 * (int64_t)per_xcd)), dim3(brick::NTC), 0, st,
+                           s.scenes_dev, nsc, so, nbrick, per_xcd, (const uint32_t *)sb.start,
                            (const uint32_t *)sb.hrank, (const uint8_t *)sb.soct,
                            (const uint8_t *)sb.brick_flag, s.dense, s.dense_count);
         GFSX_LAUNCH_CHECK();
         // bricks holding more than cap particles, from the dense list
         const unsigned ndense = (unsigned)std::min<int64_t>(nbrick, brick::DENSE_BLOCKS);
-        hipLaunchKernelGGL(kern_dense, dim3(ndense), block, 0, st, a, bg, s.dx, inv_r2, nbx,
-                           nby, (const uint32_t *)sb.count, (const uint32_t *)sb.start,
-                           (const uint32_t *)sb.hrank, s.dense, s.dense_count);
+        hipLaunchKernelGGL(kern_dense, dim3(ndense), dim3(brick::NT), 0, st, s.scenes_dev, nsc,
+                           so, (const uint32_t *)sb.start, (const uint32_t *)sb.hrank, s.dense,
+                           s.dense_count);
     } else {
+        // the direct gather: one scene only
+        const SceneDesc &d = s.scenes[0];
+        const StepP2G a = p2g_view(d, so);
         const int64_t F = a.nface[0] + a.nface[1] + a.nface[2];
-        hipLaunchKernelGGL(k_flip_p2g, dim3(grid_for(F, kBlock)), dim3(kBlock), 0,
-                           st, a, bg, s.dx, inv_r2, (const uint32_t *)sb.count,
+        hipLaunchKernelGGL(k_flip_p2g, dim3(grid_for(F, kBlock)), dim3(kBlock), 0, st, a, d.bg,
+                           d.dx, (float)(1.0 / (d.dx * d.dx)), (const uint32_t *)sb.count,
                            (const uint32_t *)sb.start, (const uint32_t *)nullptr);
     }
     GFSX_LAUNCH_CHECK();
     if (ev) GFSX_HIP(hipEventRecord(ev[2], st));
 
-    if (s.n <= 0) {
+    if (s.tot.n <= 0) {
         if (ev) GFSX_HIP(hipEventRecord(ev[3], st));
         return;
     }
-    MacGrid g;
-    for (int c = 0; c < 3; c++) g.f[c] = s.grid[c];
-    g.ni = s.ni;
-    g.nj = s.nj;
-    g.nk = s.nk;
-    g.dx = s.dx;
-    g.inv_dx = 1.0 / s.dx;
-    set_limits(g);
-    StepG2P b;
-    for (int c = 0; c < 3; c++) {
-        b.xin[c] = s.xs[c];
-        b.vin[c] = s.vs[c];
-        b.xout[c] = s.xout[c];
-        b.vout[c] = s.vout[c];
-        b.old[c] = s.grid_old[c];
-    }
-    b.id_in = s.id_in;
-    b.id_out = s.id_out;
-    b.rk.half = s.h_half;
-    b.rk.three_q = 0.0f;
-    b.rk.full = s.h_full;
-    b.rk.sixth = s.h_sixth;
-    b.rk.ninth = 0.0f;
-    b.r_pic = s.r_pic;
-    b.r_flip = s.r_flip;
-    b.keep_order = s.keep_order;
-    b.n = s.n;
     if (s.g2p_variant == 0) {
-        const int nbx = (s.ni + gbrick::GB - 1) / gbrick::GB;
-        const int nby = (s.nj + gbrick::GB - 1) / gbrick::GB;
-        const int nbz = (s.nk + gbrick::GB - 1) / gbrick::GB;
-        const int per_xcd = (int)(((int64_t)nbx * nby * nbz + 7) / 8);
+        const uint32_t nbrick = (uint32_t)s.tot.ng2p;
+        const int per_xcd = (int)((nbrick + 7) / 8);
         const dim3 grid((unsigned)(8 * per_xcd + gbrick::NEXTRA)), block(gbrick::NT);
         auto *kern = s.stencil == 0 ? k_flip_g2p_brick<0> : k_flip_g2p_brick<1>;
-        hipLaunchKernelGGL(kern, grid, block, 0, st, b, g, bg, nbx, nby, nbz, per_xcd,
-                           (const uint32_t *)sb.start, (const uint32_t *)sb.perm);
-    } else if (s.stencil == 0) {
-        hipLaunchKernelGGL(k_flip_g2p<0>, dim3(grid_for(s.n, kBlock)),
-                           dim3(kBlock), 0, st, b, g, (const uint32_t *)sb.perm);
+        hipLaunchKernelGGL(kern, grid, block, 0, st, s.scenes_dev, nsc, so, s.keep_order, nbrick,
+                           per_xcd, (const uint32_t *)sb.start, (const uint32_t *)sb.perm);
     } else {
-        hipLaunchKernelGGL(k_flip_g2p<1>, dim3(grid_for(s.n, kBlock)),
-                           dim3(kBlock), 0, st, b, g, (const uint32_t *)sb.perm);
+        // the direct kernel: one scene only
+        const SceneDesc &d = s.scenes[0];
+        const StepG2P b = g2p_view(d, so, s.keep_order);
+        const MacGrid g = mac_view(d);
+        if (s.stencil == 0)
+            hipLaunchKernelGGL(k_flip_g2p<0>, dim3(grid_for(d.n, kBlock)), dim3(kBlock), 0, st, b,
+                               g, (const uint32_t *)sb.perm);
+        else
+            hipLaunchKernelGGL(k_flip_g2p<1>, dim3(grid_for(d.n, kBlock)), dim3(kBlock), 0, st, b,
+                               g, (const uint32_t *)sb.perm);
     }
     GFSX_LAUNCH_CHECK();
     if (ev) GFSX_HIP(hipEventRecord(ev[3], st));

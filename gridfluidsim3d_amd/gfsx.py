@@ -290,11 +290,10 @@ class Context:
                                 ni, nj, nk, _dptr(perm, "perm"), _dptr(start, "start")])
         return perm.long() & 0xFFFFFFFF, start.long() & 0xFFFFFFFF
 
-    def flip_step(self, state, dt, body_force=(0.0, -9.8, 0.0), pic_ratio=0.05,
-                  stencil=TRILINEAR, order=ORDER_SORTED):
-        """Advance a ``FlipState`` by one plain step (in its double buffers)."""
+    @staticmethod
+    def _scene_of(st):
+        """The C scene of a FlipState (its current buffers in, the others out)."""
         import torch
-        st = state
         sc = Scene()
         sc.ni, sc.nj, sc.nk = st.dims
         sc.dx = st.dx
@@ -318,16 +317,51 @@ class Context:
         if st.weights is not None:
             for nm, t, s in zip(("wu", "wv", "ww"), st.weights, shapes):
                 setattr(sc, nm, _dptr(t, nm, f32, int(np.prod(s))))
+        return sc
+
+    @staticmethod
+    def _params(dt, body_force, pic_ratio, stencil, order):
         pr = StepParams()
         pr.dt = float(dt)
         pr.body_force[0], pr.body_force[1], pr.body_force[2] = [float(b) for b in body_force]
         pr.pic_ratio = float(pic_ratio)
         pr.stencil = int(stencil)
         pr.order = int(order)
+        return pr
+
+    def flip_step(self, state, dt, body_force=(0.0, -9.8, 0.0), pic_ratio=0.05,
+                  stencil=TRILINEAR, order=ORDER_SORTED):
+        """Advance a ``FlipState`` by one plain step (in its double buffers)."""
+        st = state
+        sc = self._scene_of(st)
+        pr = self._params(dt, body_force, pic_ratio, stencil, order)
         self.use_torch_stream()
         f = pb.lib_func("gfsx_flip_step", [c_void_p, POINTER(Scene), POINTER(StepParams)])
         pb.execute_lib_func(f, [self(), ctypes.byref(sc), ctypes.byref(pr)])
         st.cur = 1 - st.cur
+
+    def flip_step_batched(self, states, dt, body_force=(0.0, -9.8, 0.0), pic_ratio=0.05,
+                          stencil=TRILINEAR, order=ORDER_SORTED):
+        """Advance several ``FlipState`` objects by one plain step each, in
+        one launch per kernel (``gfsx_flip_step_batched``).  ``dt``,
+        ``body_force`` and ``pic_ratio`` are one value for all states or one
+        per state."""
+        n = len(states)
+
+        def per(v, k):
+            if k == "body_force":
+                return v if (len(v) == n and hasattr(v[0], "__len__")) else [v] * n
+            return list(v) if hasattr(v, "__len__") else [v] * n
+        dts, bfs, prs = per(dt, "dt"), per(body_force, "body_force"), per(pic_ratio, "ratio")
+        scenes = (Scene * n)(*[self._scene_of(st) for st in states])
+        params = (StepParams * n)(*[self._params(dts[i], bfs[i], prs[i], stencil, order)
+                                    for i in range(n)])
+        self.use_torch_stream()
+        f = pb.lib_func("gfsx_flip_step_batched",
+                        [c_void_p, POINTER(Scene), c_int, POINTER(StepParams)])
+        pb.execute_lib_func(f, [self(), scenes, n, params])
+        for st in states:
+            st.cur = 1 - st.cur
 
 
 class FlipState:

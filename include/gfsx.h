@@ -164,6 +164,19 @@ typedef struct gfsx_step_params {
 void gfsx_flip_step(gfsx_ctx *ctx, const gfsx_scene *scene,
                     const gfsx_step_params *params, int *err);
 
+/* The same step for nscenes independent scenes (each with its own grid,
+ * particles and parameters; one stencil and one output order for all) in
+ * one launch per kernel: the sort, P2G and G2P kernels take every scene's
+ * cells, bricks and particles together.  Each scene's results are
+ * bit-identical to gfsx_flip_step on that scene alone.  Scenes with a NULL
+ * u_old/v_old/w_old share one workspace for their snapshots.  The call
+ * enqueues work only (no allocation) once the same scenes have run once, so
+ * it can be captured into a HIP graph on the context's stream.  Batching of
+ * GridFluidSim3D's step has no counterpart in the reference; it serves many
+ * small simulations on one device. */
+void gfsx_flip_step_batched(gfsx_ctx *ctx, const gfsx_scene *scenes, int nscenes,
+                            const gfsx_step_params *params, int *err);
+
 /* Tunables of the fused step, by name (an unknown name or value fails).
  * Every variant gives bit-identical results; they differ in speed only.
  *   "p2g_variant": 0 = brick-staged LDS gather, candidates streamed in

@@ -457,3 +457,83 @@ def test_options(ctx):
         ctx.set_option("no_such_option", 1)
     with pytest.raises(RuntimeError, match="p2g_variant"):
         ctx.set_option("p2g_variant", 7)
+
+
+# ------------------------------------------------------------ batched step
+
+def _batch_scenes(seed):
+    """Uneven scenes: different grids and counts, one without particles, one
+    with particles beyond its padded grid."""
+    from gridfluidsim3d_amd import scenes
+    rng = np.random.default_rng(seed)
+    out = []
+    for k, dims in enumerate([(9, 13, 7), (20, 11, 16), (5, 6, 7), (17, 17, 9)]):
+        sc = scenes.random_blobs(seed + k, dims, 0.1, amplitude=2.0)
+        pos, vel = sc.pos, sc.vel
+        if k == 3:
+            W = np.array(dims) * 0.1
+            extra = rng.uniform(-0.5, W + 0.5, size=(500, 3)).astype(np.float32)
+            pos = np.concatenate([pos, extra])
+            vel = np.concatenate([vel, rng.normal(size=extra.shape).astype(np.float32)])
+        out.append((pos, vel, dims, 0.1))
+    out.insert(2, (np.zeros((0, 3), np.float32), np.zeros((0, 3), np.float32), (6, 5, 4), 0.1))
+    return out
+
+
+def _state_result(st):
+    res = dict(pos=_np(st.positions()), vel=_np(st.velocities()),
+               u=_np(st.grid[0]), v=_np(st.grid[1]), w=_np(st.grid[2]),
+               u_old=_np(st.grid_old[0]), v_old=_np(st.grid_old[1]), w_old=_np(st.grid_old[2]),
+               wu=_np(st.weights[0]), wv=_np(st.weights[1]), ww=_np(st.weights[2]))
+    res["id"] = _np(st.particle_ids())
+    return res
+
+
+@pytest.mark.parametrize("stencil", [TRILINEAR, TRICUBIC])
+@pytest.mark.parametrize("keep", [False, True])
+def test_step_batched_matches_single(ctx, stencil, keep):
+    """Every scene of a batched step gives the bits of a step on that scene
+    alone; two steps in a row, with per-scene time steps."""
+    import gridfluidsim3d_amd as gf
+    sc = _batch_scenes(61)
+    dts = [1 / 240, 1 / 120, 1 / 200, 1 / 300, 1 / 240]
+    mk = lambda p, v, d, h: gf.FlipState(p, v, d, h, track_ids=True, keep_old=True,
+                                         keep_weights=True)
+    batch = [mk(*s) for s in sc]
+    alone = [mk(*s) for s in sc]
+    for _ in range(2):
+        ctx.flip_step_batched(batch, dts, (0.0, -9.8, 0.0), 0.05, stencil, int(keep))
+        for st, dt in zip(alone, dts):
+            ctx.flip_step(st, dt, (0.0, -9.8, 0.0), 0.05, stencil, int(keep))
+    _torch().cuda.synchronize()
+    for i, (a, b) in enumerate(zip(batch, alone)):
+        ra, rb = _state_result(a), _state_result(b)
+        for key in ra:
+            assert np.array_equal(ra[key], rb[key], equal_nan=True), (i, key)
+
+
+def test_step_batched_against_oracle(ctx, orc):
+    import gridfluidsim3d_amd as gf
+    sc = _batch_scenes(71)
+    states = [gf.FlipState(p, v, d, h, track_ids=True, keep_old=True, keep_weights=True)
+              for p, v, d, h in sc]
+    ctx.flip_step_batched(states, 1 / 240, (0.0, -9.8, 0.0), 0.05, TRILINEAR, 0)
+    _torch().cuda.synchronize()
+    for (p, v, d, h), st in zip(sc, states):
+        if len(p) == 0:
+            continue
+        exp = orc.flip_step(p, v, d, h, 1 / 240, (0.0, -9.8, 0.0), 0.05, TRILINEAR)
+        _check_step(_state_result(st), exp, False)
+
+
+def test_step_batched_rejects_direct_variants(ctx):
+    import gridfluidsim3d_amd as gf
+    sc = _batch_scenes(81)[:2]
+    states = [gf.FlipState(p, v, d, h) for p, v, d, h in sc]
+    old = ctx.get_option("g2p_variant")
+    ctx.set_option("g2p_variant", 1)
+    try:
+        with pytest.raises(RuntimeError, match="one scene"):
+            ctx.flip_step_batched(states, 1 / 240)
+    finally:
+        ctx.set_option("g2p_variant", old)

@@ -367,6 +367,12 @@ void flip_step_impl(gfsx_ctx *ctx, const gfsx_scene *scenes, const gfsx_step_par
     }
     s.dense = ctx->u32(S_DENSE, s.tot.np2g);
     s.dense_count = ctx->u32(S_DENSEN, 1);
+    gfsx::Sorted so;
+    for (int c = 0; c < 3; c++) {
+        so.x[c] = s.xs[c];
+        so.v[c] = s.vs[c];
+    }
+    for (auto &d : s.scenes) d.p2g = gfsx::p2g_view(d, so);
     s.stencil = params[0].stencil;
     s.keep_order = params[0].order == GFSX_ORDER_KEEP;
     s.p2g_variant = ctx->p2g_variant;

@@ -49,6 +49,56 @@ struct BinGrid {
     uint32_t nbin;  // b[0]*b[1]*b[2]; also the index of the overflow bin
 };
 
+
+// Node grid of one splatted component.  Node n sits at off + n*dx; its
+// candidate bins along axis a are [n + dlo[a], n + dhi[a]] (clipped).
+struct NodeGrid {
+    int n[3];
+    float off[3];
+    int dlo[3], dhi[3];
+    float dx_f;      // unused by the maths; kept for diagnostics
+};
+
+// A MAC grid view for G2P (macvelocityfield.cpp:37-45 layout).
+struct MacGrid {
+    const float *f[3];
+    int ni, nj, nk;
+    double dx, inv_dx;
+    double lim[3];   // dx*ni, dx*nj, dx*nk (Grid3d::isPositionInGrid)
+    float limf[3];   // for float x, x < limf[a] exactly when (double)x < lim[a]
+};
+
+// Fill lim/limf of g from dx and the cell counts.  limf is the smallest
+// float not below lim: every float under it is under lim, every float at
+// or above it is at or above lim.
+inline void set_limits(MacGrid &g) {
+    const int n[3] = {g.ni, g.nj, g.nk};
+    for (int a = 0; a < 3; a++) {
+        g.lim[a] = g.dx * n[a];
+        float f = (float)g.lim[a];
+        if ((double)f < g.lim[a]) f = std::nextafter(f, INFINITY);
+        g.limf[a] = f;
+    }
+}
+
+// What the P2G kernels read of a scene: the particles in sort order, the
+// scene's output grids, node grids and body force.
+struct StepP2G {
+    const float *x, *y, *z;
+    const float *vel[3];
+    float *grid[3], *grid_old[3], *wgt[3];
+    NodeGrid ng[3];
+    int64_t nface[3];
+    float grav_delta[3];
+    int grav_on[3];
+    int ni, nj, nk;
+};
+
+// The particles of every scene in sort order (slot q of the sort).
+struct Sorted {
+    const float *x[3], *v[3];
+};
+
 // One scene of a (batched) fused step or of a cell sort, in a device table
 // that the kernels index by particle, bin, binning block or brick: every
 // scene owns a contiguous range of each (the offsets p0, b0, blk0, k0, g0
@@ -81,37 +131,44 @@ struct SceneDesc {
     int grav_on[3];
     float h_half, h_full, h_sixth;   // (float)(0.5*dt), (float)dt, (float)(dt/6.0f)
     float r_pic, r_flip;             // (float)ratio, (float)(1-ratio)
+    StepP2G p2g;                     // the P2G view (p2g_view), filled before upload
 };
 
-// Node grid of one splatted component.  Node n sits at off + n*dx; its
-// candidate bins along axis a are [n + dlo[a], n + dhi[a]] (clipped).
-struct NodeGrid {
-    int n[3];
-    float off[3];
-    int dlo[3], dhi[3];
-    float dx_f;      // unused by the maths; kept for diagnostics
-};
-
-// A MAC grid view for G2P (macvelocityfield.cpp:37-45 layout).
-struct MacGrid {
-    const float *f[3];
-    int ni, nj, nk;
-    double dx, inv_dx;
-    double lim[3];   // dx*ni, dx*nj, dx*nk (Grid3d::isPositionInGrid)
-    float limf[3];   // for float x, x < limf[a] exactly when (double)x < lim[a]
-};
-
-// Fill lim/limf of g from dx and the cell counts.  limf is the smallest
-// float not below lim: every float under it is under lim, every float at
-// or above it is at or above lim.
-inline void set_limits(MacGrid &g) {
-    const int n[3] = {g.ni, g.nj, g.nk};
-    for (int a = 0; a < 3; a++) {
-        g.lim[a] = g.dx * n[a];
-        float f = (float)g.lim[a];
-        if ((double)f < g.lim[a]) f = std::nextafter(f, INFINITY);
-        g.limf[a] = f;
+// P2G view of a scene: faces of component c sit at off + n*dx with the
+// offset half a cell on the two other axes (fluidsimulation.cpp:2526-2570);
+// a face's candidate bins along axis a are [n + dlo, n + dhi] (pad 1, R = dx).
+__host__ __device__ inline StepP2G p2g_view(const SceneDesc &sd, const Sorted &so) {
+    StepP2G a;
+    a.x = so.x[0];
+    a.y = so.x[1];
+    a.z = so.x[2];
+    const float half = (float)(0.5 * sd.dx);
+    for (int c = 0; c < 3; c++) {
+        a.vel[c] = so.v[c];
+        a.grid[c] = sd.grid[c];
+        a.grid_old[c] = sd.grid_old[c];
+        a.wgt[c] = sd.wgt[c];
+        a.grav_delta[c] = sd.grav_delta[c];
+        a.grav_on[c] = sd.grav_on[c];
+        NodeGrid &ng = a.ng[c];
+        ng.n[0] = sd.ni + (c == 0);
+        ng.n[1] = sd.nj + (c == 1);
+        ng.n[2] = sd.nk + (c == 2);
+        ng.dx_f = (float)sd.dx;
+        for (int ax = 0; ax < 3; ax++) {
+            ng.off[ax] = (ax == c) ? 0.0f : half;
+            // node n sits at bin coordinate n + off/dx + pad; its candidate
+            // bins cover (n + base - R/dx, n + base + R/dx) with R = dx
+            const double base = (double)ng.off[ax] / sd.dx + 1.0;
+            ng.dlo[ax] = (int)floor(base - 1.0);
+            ng.dhi[ax] = (int)ceil(base + 1.0) - 1;
+        }
+        a.nface[c] = (int64_t)ng.n[0] * ng.n[1] * ng.n[2];
     }
+    a.ni = sd.ni;
+    a.nj = sd.nj;
+    a.nk = sd.nk;
+    return a;
 }
 
 __device__ __forceinline__ uint32_t bin_of(float x, float y, float z,

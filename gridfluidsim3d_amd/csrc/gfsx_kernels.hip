@@ -1111,59 +1111,6 @@ struct SoaLoad {
     }
 };
 
-struct StepP2G {
-    const float *x, *y, *z;
-    const float *vel[3];
-    float *grid[3], *grid_old[3], *wgt[3];
-    NodeGrid ng[3];
-    int64_t nface[3];
-    float grav_delta[3];
-    int grav_on[3];
-    int ni, nj, nk;
-};
-
-// The particles of every scene in sort order (slot q of the sort).
-struct Sorted {
-    const float *x[3], *v[3];
-};
-
-// P2G view of a scene: faces of component c sit at off + n*dx with the
-// offset half a cell on the two other axes (fluidsimulation.cpp:2526-2570);
-// a face's candidate bins along axis a are [n + dlo, n + dhi] (pad 1, R = dx).
-__host__ __device__ inline StepP2G p2g_view(const SceneDesc &sd, const Sorted &so) {
-    StepP2G a;
-    a.x = so.x[0];
-    a.y = so.x[1];
-    a.z = so.x[2];
-    const float half = (float)(0.5 * sd.dx);
-    for (int c = 0; c < 3; c++) {
-        a.vel[c] = so.v[c];
-        a.grid[c] = sd.grid[c];
-        a.grid_old[c] = sd.grid_old[c];
-        a.wgt[c] = sd.wgt[c];
-        a.grav_delta[c] = sd.grav_delta[c];
-        a.grav_on[c] = sd.grav_on[c];
-        NodeGrid &ng = a.ng[c];
-        ng.n[0] = sd.ni + (c == 0);
-        ng.n[1] = sd.nj + (c == 1);
-        ng.n[2] = sd.nk + (c == 2);
-        ng.dx_f = (float)sd.dx;
-        for (int ax = 0; ax < 3; ax++) {
-            ng.off[ax] = (ax == c) ? 0.0f : half;
-            // node n sits at bin coordinate n + off/dx + pad; its candidate
-            // bins cover (n + base - R/dx, n + base + R/dx) with R = dx
-            const double base = (double)ng.off[ax] / sd.dx + 1.0;
-            ng.dlo[ax] = (int)floor(base - 1.0);
-            ng.dhi[ax] = (int)ceil(base + 1.0) - 1;
-        }
-        a.nface[c] = (int64_t)ng.n[0] * ng.n[1] * ng.n[2];
-    }
-    a.ni = sd.ni;
-    a.nj = sd.nj;
-    a.nk = sd.nk;
-    return a;
-}
-
 // One thread per face of u, v or w: gather, normalise
 // (scalarfield.cpp:90-106), snapshot (fluidsimulation.cpp:3306), then add
 // the body force where the face borders a cell that holds a particle
@@ -1897,12 +1844,7 @@ __global__ __launch_bounds__(brick::NTC) void k_flip_p2g_brick(
     if (gid >= nbrick) return;
     const bool flagged = brick_flag[gid] != 0;
     const SceneDesc &sd = scenes[scene_of(scenes, nsc, gid, [](const SceneDesc &d) { return d.k0; })];
-    // the scene's view in LDS: its arrays are indexed by component at run
-    // time, which private memory would turn into scratch
-    __shared__ StepP2G s_a;
-    if (threadIdx.x == 0) s_a = p2g_view(sd, so);
-    __syncthreads();
-    p2g_brick_rows<V, CAP>(s_a, sd.bg, sd.dx, (float)(1.0 / (sd.dx * sd.dx)), sd.nbx, sd.nby,
+    p2g_brick_rows<V, CAP>(sd.p2g, sd.bg, sd.dx, (float)(1.0 / (sd.dx * sd.dx)), sd.nbx, sd.nby,
                            (int)(gid - sd.k0), gid, start + sd.b0, hrank, soct, flagged, dense,
                            dense_count);
 }
@@ -1922,15 +1864,12 @@ __global__ __launch_bounds__(brick::NT, 3) void k_flip_p2g_dense(
     __shared__ uint32_t s_rowq[NROW], s_rows0[NROW + 1];
     __shared__ uint32_t s_wsum[NT / 64];
     __shared__ uint8_t s_occ[SX * SY * SZ];
-    __shared__ StepP2G s_a;   // the scene's view (see k_flip_p2g_brick)
     const uint32_t nd = *dense_count;
     for (uint32_t d = blockIdx.x; d < nd; d += gridDim.x) {
         __syncthreads();   // the previous brick is done with LDS
         const uint32_t gid = dense[d];
         const SceneDesc &sd = scenes[scene_of(scenes, nsc, gid, [](const SceneDesc &e) { return e.k0; })];
-        if (threadIdx.x == 0) s_a = p2g_view(sd, so);
-        __syncthreads();
-        p2g_brick<V, CAP, true>(s_a, sd.bg, sd.dx, (float)(1.0 / (sd.dx * sd.dx)), sd.nbx, sd.nby,
+        p2g_brick<V, CAP, true>(sd.p2g, sd.bg, sd.dx, (float)(1.0 / (sd.dx * sd.dx)), sd.nbx, sd.nby,
                                 (int)(gid - sd.k0), nullptr, start + sd.b0, hrank, dense,
                                 dense_count, s_buf, s_hs, s_rowq, s_rows0, s_wsum, s_occ);
     }
@@ -2432,7 +2371,7 @@ void flip_step(const StepBatch &s, const SortBuffers &sb, hipStream_t st, hipEve
     } else {
         // the direct gather: one scene only
         const SceneDesc &d = s.scenes[0];
-        const StepP2G a = p2g_view(d, so);
+        const StepP2G &a = d.p2g;
         const int64_t F = a.nface[0] + a.nface[1] + a.nface[2];
         hipLaunchKernelGGL(k_flip_p2g, dim3(grid_for(F, kBlock)), dim3(kBlock), 0, st, a, d.bg,
                            d.dx, (float)(1.0 / (d.dx * d.dx)), (const uint32_t *)sb.count,

@@ -340,6 +340,37 @@ class Context:
         pb.execute_lib_func(f, [self(), ctypes.byref(sc), ctypes.byref(pr)])
         st.cur = 1 - st.cur
 
+    def capture_batched(self, states, dt, body_force=(0.0, -9.8, 0.0), pic_ratio=0.05,
+                        stencil=TRILINEAR, order=ORDER_SORTED):
+        """Capture the batched step of ``states`` into HIP graphs, one per
+        direction of the double buffers, and return a function that advances
+        all states by one step by replaying the right graph.  Runs one step
+        in each direction first (so that the work of the captured calls is
+        only kernels: workspace and scene tables exist by then)."""
+        import torch
+        for _ in range(2):
+            self.flip_step_batched(states, dt, body_force, pic_ratio, stencil, order)
+        torch.cuda.synchronize()
+        graphs = []
+        for _ in range(2):
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                self.flip_step_batched(states, dt, body_force, pic_ratio, stencil, order)
+            graphs.append(g)
+        torch.cuda.synchronize()
+        # the captures flipped every state twice: back where it started, and
+        # graphs[0] starts from the current buffers
+        for st in states:
+            if st.cur != states[0].cur:
+                raise RuntimeError("capture_batched: the states are in different buffers")
+        first = states[0].cur
+
+        def step():
+            graphs[0 if states[0].cur == first else 1].replay()
+            for st in states:
+                st.cur = 1 - st.cur
+        return step
+
     def flip_step_batched(self, states, dt, body_force=(0.0, -9.8, 0.0), pic_ratio=0.05,
                           stencil=TRILINEAR, order=ORDER_SORTED):
         """Advance several ``FlipState`` objects by one plain step each, in

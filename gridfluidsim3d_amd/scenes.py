@@ -194,6 +194,24 @@ def random_blobs(seed, dims, dx, ppc=8, amplitude=2.0, fill=(0.1, 0.6),
     return SceneSpec("blobs%dx%dx%d" % tuple(dims), dims, dx, pos, vel, dt)
 
 
+def batch64(seed=1004, nscenes=64, dx=1.0 / 64.0, dt=1.0 / 240.0):
+    """The batched workload: nscenes independent scenes, each grid axis drawn
+    log-uniform from 32 to 128 cells (independently), 10-60% of the cells
+    fluid as random boxes and spheres, 4-8 particles per cell, curl noise
+    at 1-3 m/s.  Returns a list of SceneSpec."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for s in range(nscenes):
+        dims = tuple(int(round(np.exp(rng.uniform(np.log(32.0), np.log(128.0))))) for _ in range(3))
+        ppc = int(rng.integers(4, 9))
+        amp = float(rng.uniform(1.0, 3.0))
+        sc = random_blobs(int(rng.integers(1 << 30)), dims, dx, ppc=ppc, amplitude=amp,
+                          fill=(0.1, 0.6), dt=dt)
+        sc.name = "batch%d_scene%d" % (nscenes, s)
+        out.append(sc)
+    return out
+
+
 CONFIGS = {
     "sphere64": sphere64,
     "dambreak128": dambreak128,

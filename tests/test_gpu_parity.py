@@ -537,3 +537,49 @@ def test_step_batched_rejects_direct_variants(ctx):
             ctx.flip_step_batched(states, 1 / 240)
     finally:
         ctx.set_option("g2p_variant", old)
+
+
+def test_step_batched_graph_replays_match_eager(ctx):
+    """The batched step captured into HIP graphs gives, replay after replay
+    (with other work and synchronisations between replays), the bits of the
+    eager batched step."""
+    import gridfluidsim3d_amd as gf
+    torch = _torch()
+    sc = _batch_scenes(91)
+    mk = lambda p, v, d, h: gf.FlipState(p, v, d, h, track_ids=True, keep_old=True,
+                                         keep_weights=True)
+    graph_states = [mk(*s) for s in sc]
+    eager_states = [mk(*s) for s in sc]
+    step = ctx.capture_batched(graph_states, 1 / 240)
+    # capture_batched ran two eager steps first: bring the others level
+    for _ in range(2):
+        ctx.flip_step_batched(eager_states, 1 / 240)
+    for rep in range(4):
+        step()
+        ctx.flip_step_batched(eager_states, 1 / 240)
+        junk = torch.ones(1 << 20, device="cuda").cumsum(0)   # unrelated eager work
+        torch.cuda.synchronize()
+        del junk
+        for i, (a, b) in enumerate(zip(graph_states, eager_states)):
+            ra, rb = _state_result(a), _state_result(b)
+            for key in ra:
+                assert np.array_equal(ra[key], rb[key], equal_nan=True), (rep, i, key)
+
+
+def test_autotune_picks_a_variant(ctx):
+    import gridfluidsim3d_amd as gf
+    from gridfluidsim3d_amd import autotune
+    sc = _batch_scenes(95)
+    states = [gf.FlipState(p, v, d, h) for p, v, d, h in sc]
+    before = [st.positions().clone() for st in states]
+    old = ctx.get_option("p2g_variant")
+    try:
+        res = autotune.tune(ctx, states, 1 / 240, steps=1)
+        best, times = res["p2g_variant"]
+        assert best in autotune.CANDIDATES["p2g_variant"] and best == ctx.get_option("p2g_variant")
+        assert set(times) == set(autotune.CANDIDATES["p2g_variant"])
+    finally:
+        ctx.set_option("p2g_variant", old)
+    # the caller's states did not move
+    for st, b in zip(states, before):
+        assert _torch().equal(st.positions(), b)

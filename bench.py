@@ -143,6 +143,83 @@ def _cpu_model():
     return None
 
 
+def batched_measure(steps, stencil, log_fn):
+    """The batched form (64 independent scenes of uneven size): particle-steps/s
+    of 64 sequential plain steps per step, of the batched step (after the
+    autotuner picked its variants), and of the batched step replayed from HIP
+    graphs, each timed over `steps` steps from the scenes' initial state."""
+    import torch
+
+    import gridfluidsim3d_amd as gf
+    from gridfluidsim3d_amd import autotune, scenes
+
+    t0 = time.time()
+    specs = scenes.batch64()
+    P = sum(sc.n for sc in specs)
+    F = sum(face_count(sc.dims) for sc in specs)
+    log_fn("batch64: %d scenes, %d particles, %d faces, built in %.1f s"
+           % (len(specs), P, F, time.time() - t0))
+    ctx = gf.Context(0)
+    dts = [sc.dt for sc in specs]
+
+    def fresh():
+        return [gf.FlipState(sc.pos, sc.vel, sc.dims, sc.dx) for sc in specs]
+
+    def timed(run, states):
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        ev0.record()
+        for _ in range(steps):
+            run(states)
+        ev1.record()
+        torch.cuda.synchronize()
+        return ev0.elapsed_time(ev1) / steps
+
+    def sequential(states):
+        for st, dt in zip(states, dts):
+            ctx.flip_step(st, dt, (0.0, -9.8, 0.0), 0.05, stencil, gf.ORDER_SORTED)
+
+    def batched(states):
+        ctx.flip_step_batched(states, dts, (0.0, -9.8, 0.0), 0.05, stencil, gf.ORDER_SORTED)
+
+    # warm both paths (workspace, tables) on scratch states
+    warm = fresh()
+    sequential(warm)
+    batched(warm)
+    del warm
+    ms_seq = timed(sequential, fresh())
+    tuned = autotune.tune(ctx, fresh(), dts, (0.0, -9.8, 0.0), 0.05, stencil, gf.ORDER_SORTED)
+    log_fn("autotune: %s" % tuned)
+    ms_bat = timed(batched, fresh())
+    states = fresh()
+    replay = ctx.capture_batched(states, dts, (0.0, -9.8, 0.0), 0.05, stencil, gf.ORDER_SORTED)
+    # the capture advanced the states two steps; time the replays from a
+    # fresh start by copying the initial state into the captured buffers
+    init = fresh()
+    for st, st0 in zip(states, init):
+        for a in range(3):
+            st.cur_x[a].copy_(st0.cur_x[a])
+            st.cur_v[a].copy_(st0.cur_v[a])
+    del init
+    ms_graph = timed(lambda _: replay(), states)
+    ctx.close()
+    alg = sum(48 * sc.n + 8 * face_count(sc.dims) for sc in specs)
+    out = dict(workload="batch64", scenes=len(specs), particles=P, faces=F, steps=steps,
+               ms_per_step_sequential=round(ms_seq, 4), ms_per_step_batched=round(ms_bat, 4),
+               ms_per_step_graph=round(ms_graph, 4),
+               value_sequential=P / (ms_seq / 1e3), value_batched=P / (ms_bat / 1e3),
+               value_graph=P / (ms_graph / 1e3), unit="particle-steps/s",
+               speedup_graph_vs_sequential=round(ms_seq / ms_graph, 3),
+               autotune={k: v[0] for k, v in tuned.items()},
+               step_roofline_graph=dict(achieved=round(alg / (ms_graph / 1e3) / 1e9, 1),
+                                        peak=HBM_PEAK_GBS, unit="GB/s",
+                                        frac=round(alg / (ms_graph / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+                                        algorithmic_bytes=alg))
+    log_fn("batched: %s" % out)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -153,6 +230,9 @@ def main():
     ap.add_argument("--stencil", default="trilinear", choices=["trilinear", "tricubic"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the batched-form measurement (64 scenes)")
+    ap.add_argument("--batched-steps", type=int, default=10)
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a plain copy of this many bytes (counter calibration)")
     args = ap.parse_args()
@@ -228,6 +308,13 @@ def main():
                          frac=round(step_gbs / HBM_PEAK_GBS, 4),
                          algorithmic_bytes=alg["step"])
 
+    batched_out = None
+    if not args.no_batched:
+        del st
+        torch.cuda.empty_cache()
+        batched_out = batched_measure(args.batched_steps, stencil, log)
+        batched_out["vs_plain"] = round(batched_out["value_graph"] / value, 4)
+
     cpu = None
     if not args.no_cpu:
         log("CPU baseline ...")
@@ -253,6 +340,7 @@ def main():
         "step_roofline": step_roofline,
         "stages_ms": {k: round(v, 4) for k, v in stage_avg.items()},
         "cpu_baseline": cpu,
+        "batched": batched_out,
     }
     print(json.dumps(line), flush=True)
     ctx.close()

@@ -220,6 +220,80 @@ def batched_measure(steps, stencil, log_fn):
     return out
 
 
+def fwdbwd_measure(reps, log_fn, steps=16, opt_iters=3):
+    """The differentiable form: a 16-step rollout of the 96^3 jittered slab
+    (1,990,656 particles, dx = 1/96) through the autograd step, the density
+    loss against a seeded smooth target, and the backward pass through the
+    adjoint kernels to the initial velocities.  Reports particle-steps/s of
+    forward+backward (P * steps / time) next to the same rollout forward
+    only, and a few gradient-descent iterations on the initial velocities."""
+    import torch
+
+    import gridfluidsim3d_amd as gf
+    from gridfluidsim3d_amd import autograd as ga
+    from gridfluidsim3d_amd import scenes
+
+    t0 = time.time()
+    sc = scenes.slab()
+    dims, dx, dt = sc.dims, sc.dx, sc.dt
+    P, F = sc.n, face_count(dims)
+    target = torch.tensor(scenes.density_target(dims, dx, 7, y_range=(20, 47)), device="cuda")
+    log_fn("slab96: %d particles, %d faces, built in %.1f s" % (P, F, time.time() - t0))
+    ctx = gf.Context(0)
+    x0 = torch.tensor(sc.pos.T.copy(), device="cuda")
+    v0 = torch.tensor(sc.vel.T.copy(), device="cuda")
+
+    def fwd(v_init=v0):
+        with torch.no_grad():
+            x, _v = ga.rollout(ctx, x0, v_init, dims, dx, steps, dt)
+            return ga.density_loss(ctx, x, target, dims, dx)
+
+    def fwd_bwd(v_init):
+        v = v_init.detach().clone().requires_grad_(True)
+        x, _v = ga.rollout(ctx, x0, v, dims, dx, steps, dt)
+        L = ga.density_loss(ctx, x, target, dims, dx)
+        L.backward()
+        return L.detach(), v.grad
+
+    def timed(fn):
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        ev0.record()
+        for _ in range(reps):
+            fn()
+        ev1.record()
+        torch.cuda.synchronize()
+        return ev0.elapsed_time(ev1) / reps
+
+    fwd()
+    fwd_bwd(v0)
+    ms_f = timed(fwd)
+    ms_fb = timed(lambda: fwd_bwd(v0))
+    # gradient descent on the initial velocities, step length fixed in
+    # velocity units (0.05 m/s on the largest gradient component)
+    losses = []
+    v = v0
+    for _ in range(opt_iters):
+        L, g = fwd_bwd(v)
+        losses.append(float(L))
+        v = v - 0.05 * g / g.abs().max()
+    losses.append(float(fwd(v)))
+    ctx.close()
+    alg = (120 * P + 24 * F) * steps
+    gbs = alg / (ms_fb / 1e3) / 1e9
+    out = dict(workload="slab96", grid=list(dims), dx=dx, particles=P, steps=steps, reps=reps,
+               ms_forward=round(ms_f, 3), ms_forward_backward=round(ms_fb, 3),
+               value_forward=P * steps / (ms_f / 1e3),
+               value_forward_backward=P * steps / (ms_fb / 1e3), unit="particle-steps/s",
+               roofline_forward_backward=dict(achieved=round(gbs, 1), peak=HBM_PEAK_GBS,
+                                              unit="GB/s", frac=round(gbs / HBM_PEAK_GBS, 4),
+                                              algorithmic_bytes=alg),
+               loss_descent=[round(l, 3) for l in losses])
+    log_fn("fwd+bwd: %s" % out)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -233,6 +307,9 @@ def main():
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the batched-form measurement (64 scenes)")
     ap.add_argument("--batched-steps", type=int, default=10)
+    ap.add_argument("--no-fwdbwd", action="store_true",
+                    help="skip the differentiable-form measurement (96^3 slab)")
+    ap.add_argument("--fwdbwd-reps", type=int, default=3)
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a plain copy of this many bytes (counter calibration)")
     args = ap.parse_args()
@@ -315,6 +392,14 @@ def main():
         batched_out = batched_measure(args.batched_steps, stencil, log)
         batched_out["vs_plain"] = round(batched_out["value_graph"] / value, 4)
 
+    fwdbwd_out = None
+    if not args.no_fwdbwd:
+        torch.cuda.empty_cache()
+        fwdbwd_out = fwdbwd_measure(args.fwdbwd_reps, log)
+        fwdbwd_out["vs_plain"] = round(fwdbwd_out["value_forward_backward"] / value, 4)
+        fwdbwd_out["vs_forward"] = round(fwdbwd_out["value_forward_backward"] /
+                                         fwdbwd_out["value_forward"], 4)
+
     cpu = None
     if not args.no_cpu:
         log("CPU baseline ...")
@@ -341,6 +426,7 @@ def main():
         "stages_ms": {k: round(v, 4) for k, v in stage_avg.items()},
         "cpu_baseline": cpu,
         "batched": batched_out,
+        "fwd_bwd": fwdbwd_out,
     }
     print(json.dumps(line), flush=True)
     ctx.close()

@@ -1,0 +1,332 @@
+// gfsx_adjoint.hip -- gfx950 adjoints of the FLIP transfer + advection step.
+//
+// The plain step (trilinear stencil) maps the particles (x, v) of one scene
+// to (x', v'):
+//
+//   S_f = sum_p K(|c_f - x_p|^2) v_p,   W_f = sum_p K(...)        (P2G)
+//   n_f = S_f / W_f where W_f > 0, else 0                     (normalise)
+//   g_f = n_f + m_f * body_force * dt   (m_f: face borders a cell holding
+//                                        a particle; constant in x)
+//   v'  = r_pic I(g, x) + r_flip (v + I(g, x) - I(n, x))       (PIC/FLIP)
+//   x'  = RK4 of x through I(g, .)                              (advect)
+//
+// with I the MAC trilinear sample.  Given dL/dx' and dL/dv' the adjoint
+// gives dL/dx and dL/dv in three passes over the particles:
+//
+//   1. k_adj_advect: recompute the RK4 stage positions from (x, g), walk the
+//      stages backwards, and scatter the sample adjoints onto the faces
+//      (dL/dg from the stages and the PIC/FLIP blend, dL/dn from the FLIP
+//      difference).  The scatter uses float atomics: the sums are exact up
+//      to the order of FP32 additions, which may change between runs.
+//   2. k_adj_p2g_step: a gather over each particle's faces through the
+//      quotient rule of the normalisation,
+//        dL/dv_p += K_pf lam_f / W_f,
+//        dL/dx_p += lam_f (v_p - n_f) / W_f * dK_pf/dx_p,
+//      with lam_f = dL/dg_f + dL/dn_f (the body force is additive).
+//   3. (scalar splat) k_adj_p2g_scalar: the adjoint of the unnormalised
+//      S = sum K val, W = sum K of gfsx_p2g for any radius.
+//
+// The forward maths being differentiated follow ScalarField::addPointValue
+// and applyWeightField (scalarfield.cpp:90-106,167-201),
+// MACVelocityField::_interpolateLinearU/V/W (macvelocityfield.cpp:452-543),
+// ParticleAdvector::_RK4 (particleadvector.cpp:1045-1054) and the PIC/FLIP
+// update of FluidSimulation (fluidsimulation.cpp:3115-3127).  The fluid mask
+// and the cell indices are piecewise constant in x: their derivative is zero
+// almost everywhere, which is the derivative the adjoint returns.
+
+#include "gfsx_internal.h"
+#include "gfsx_sample.h"
+
+#include <algorithm>
+
+namespace gfsx {
+
+namespace {
+
+inline unsigned blocks_for(int64_t n) {
+    return (unsigned)std::max<int64_t>((n + kBlock - 1) / kBlock, 1);
+}
+
+// dK/dt of the Wyvill kernel K(t) = (1-t)^2 (5/9 + 4(1-t)/9), t = r^2/R^2:
+// -a (10 + 12 a) / 9 with a = 1 - t, zero for t >= 1 (K is C1 at the cutoff).
+__device__ __forceinline__ float wyvill_dt(float t) {
+    const float a = fmaxf(1.0f - t, 0.0f);
+    return -a * fmaf(12.0f / 9.0f, a, 10.0f / 9.0f);
+}
+
+// One MAC component's trilinear sample at cell (i,j,k), fractions t: adds
+// lam * dI/dt to dt_out and scatters lam * (corner weight) into lf.
+// Out-of-range corners read 0 and receive nothing
+// (macvelocityfield.cpp:470-478).
+__device__ __forceinline__ void tri_adjoint(const float *__restrict__ f, float *lf, int w, int h,
+                                            int d, int i, int j, int k, float tx, float ty,
+                                            float tz, float lam, float dt_out[3]) {
+    if (lam == 0.0f) return;
+    const bool vi0 = (unsigned)i < (unsigned)w, vi1 = (unsigned)(i + 1) < (unsigned)w;
+    const bool vj0 = (unsigned)j < (unsigned)h, vj1 = (unsigned)(j + 1) < (unsigned)h;
+    const bool vk0 = (unsigned)k < (unsigned)d, vk1 = (unsigned)(k + 1) < (unsigned)d;
+    const uint32_t i0 = (uint32_t)min(max(i, 0), w - 1), i1 = (uint32_t)min(max(i + 1, 0), w - 1);
+    const uint32_t j0 = (uint32_t)min(max(j, 0), h - 1), j1 = (uint32_t)min(max(j + 1, 0), h - 1);
+    const uint32_t k0 = (uint32_t)min(max(k, 0), d - 1), k1 = (uint32_t)min(max(k + 1, 0), d - 1);
+    const uint32_t r00 = (uint32_t)w * (j0 + (uint32_t)h * k0);
+    const uint32_t r10 = (uint32_t)w * (j1 + (uint32_t)h * k0);
+    const uint32_t r01 = (uint32_t)w * (j0 + (uint32_t)h * k1);
+    const uint32_t r11 = (uint32_t)w * (j1 + (uint32_t)h * k1);
+    const bool o000 = vi0 && vj0 && vk0, o100 = vi1 && vj0 && vk0;
+    const bool o010 = vi0 && vj1 && vk0, o110 = vi1 && vj1 && vk0;
+    const bool o001 = vi0 && vj0 && vk1, o101 = vi1 && vj0 && vk1;
+    const bool o011 = vi0 && vj1 && vk1, o111 = vi1 && vj1 && vk1;
+    const float v000 = ld_or_zero(f, o000, r00 + i0), v100 = ld_or_zero(f, o100, r00 + i1);
+    const float v010 = ld_or_zero(f, o010, r10 + i0), v110 = ld_or_zero(f, o110, r10 + i1);
+    const float v001 = ld_or_zero(f, o001, r01 + i0), v101 = ld_or_zero(f, o101, r01 + i1);
+    const float v011 = ld_or_zero(f, o011, r11 + i0), v111 = ld_or_zero(f, o111, r11 + i1);
+    const float sx = 1.0f - tx, sy = 1.0f - ty, sz = 1.0f - tz;
+    dt_out[0] += lam * (sy * sz * (v100 - v000) + ty * sz * (v110 - v010) +
+                        sy * tz * (v101 - v001) + ty * tz * (v111 - v011));
+    dt_out[1] += lam * (sx * sz * (v010 - v000) + tx * sz * (v110 - v100) +
+                        sx * tz * (v011 - v001) + tx * tz * (v111 - v101));
+    dt_out[2] += lam * (sx * sy * (v001 - v000) + tx * sy * (v101 - v100) +
+                        sx * ty * (v011 - v010) + tx * ty * (v111 - v110));
+    if (o000) unsafeAtomicAdd(lf + r00 + i0, lam * (sx * sy * sz));
+    if (o100) unsafeAtomicAdd(lf + r00 + i1, lam * (tx * sy * sz));
+    if (o010) unsafeAtomicAdd(lf + r10 + i0, lam * (sx * ty * sz));
+    if (o110) unsafeAtomicAdd(lf + r10 + i1, lam * (tx * ty * sz));
+    if (o001) unsafeAtomicAdd(lf + r01 + i0, lam * (sx * sy * tz));
+    if (o101) unsafeAtomicAdd(lf + r01 + i1, lam * (tx * sy * tz));
+    if (o011) unsafeAtomicAdd(lf + r11 + i0, lam * (sx * ty * tz));
+    if (o111) unsafeAtomicAdd(lf + r11 + i1, lam * (tx * ty * tz));
+}
+
+// Adjoint of the MAC sample I(f, q) at q: lq += (dI/dq)^T lam, and lam
+// scattered into lf.  Zero outside the grid, where the sample is the
+// constant 0 (macvelocityfield.cpp:550-552).
+__device__ __forceinline__ void mac_adjoint(const StepAdjoint &a, const float *const f[3],
+                                            float *const lf[3], const float q[3],
+                                            const float lam[3], float lq[3]) {
+    if (!(q[0] >= 0.0f && q[1] >= 0.0f && q[2] >= 0.0f && q[0] < a.mac.limf[0] &&
+          q[1] < a.mac.limf[1] && q[2] < a.mac.limf[2]))
+        return;
+    const AxisCoords cx = axis_coords(q[0], a.mac.inv_dx);
+    const AxisCoords cy = axis_coords(q[1], a.mac.inv_dx);
+    const AxisCoords cz = axis_coords(q[2], a.mac.inv_dx);
+    const int ni = a.mac.ni, nj = a.mac.nj, nk = a.mac.nk;
+    float dt[3] = {0.0f, 0.0f, 0.0f};
+    tri_adjoint(f[0], lf[0], ni + 1, nj, nk, cx.ia, cy.ib, cz.ib, cx.ta, cy.tb, cz.tb, lam[0], dt);
+    tri_adjoint(f[1], lf[1], ni, nj + 1, nk, cx.ib, cy.ia, cz.ib, cx.tb, cy.ta, cz.tb, lam[1], dt);
+    tri_adjoint(f[2], lf[2], ni, nj, nk + 1, cx.ib, cy.ib, cz.ia, cx.tb, cy.tb, cz.ta, lam[2], dt);
+    // the cell fractions move by 1/dx per unit of x
+    const float inv_dx = (float)a.mac.inv_dx;
+    for (int c = 0; c < 3; c++) lq[c] += dt[c] * inv_dx;
+}
+
+__global__ __launch_bounds__(kBlock) void k_adj_advect(StepAdjoint a) {
+    const int64_t p = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    if (p >= a.n) return;
+    float x0[3], ax[3], av[3];
+    for (int c = 0; c < 3; c++) {
+        x0[c] = a.x[c][p];
+        ax[c] = a.ax[c][p];
+        av[c] = a.av[c][p];
+    }
+    // forward stages, as the fused step computes them (particleadvector.cpp:1045-1054)
+    MacGrid g = a.mac;
+    for (int c = 0; c < 3; c++) g.f[c] = a.g[c];
+    float k1[3], k2[3], k3[3], q2[3], q3[3], q4[3], unused[3];
+    sample_mac<0, false>(g, nullptr, nullptr, nullptr, x0[0], x0[1], x0[2], k1, unused);
+    for (int c = 0; c < 3; c++) q2[c] = x0[c] + a.h_half * k1[c];
+    sample_mac<0, false>(g, nullptr, nullptr, nullptr, q2[0], q2[1], q2[2], k2, unused);
+    for (int c = 0; c < 3; c++) q3[c] = x0[c] + a.h_half * k2[c];
+    sample_mac<0, false>(g, nullptr, nullptr, nullptr, q3[0], q3[1], q3[2], k3, unused);
+    for (int c = 0; c < 3; c++) q4[c] = x0[c] + a.h_full * k3[c];
+    // x' = x + sixth*(((k1 + 2k2) + 2k3) + k4), walked backwards
+    float lk[3], lq[3], lx[3];
+    for (int c = 0; c < 3; c++) {
+        lx[c] = ax[c];
+        lk[c] = a.h_sixth * ax[c];               // k4
+        lq[c] = 0.0f;
+    }
+    mac_adjoint(a, a.g, a.lg, q4, lk, lq);
+    for (int c = 0; c < 3; c++) {
+        lx[c] += lq[c];
+        lk[c] = 2.0f * a.h_sixth * ax[c] + a.h_full * lq[c];    // k3
+        lq[c] = 0.0f;
+    }
+    mac_adjoint(a, a.g, a.lg, q3, lk, lq);
+    for (int c = 0; c < 3; c++) {
+        lx[c] += lq[c];
+        lk[c] = 2.0f * a.h_sixth * ax[c] + a.h_half * lq[c];    // k2
+        lq[c] = 0.0f;
+    }
+    mac_adjoint(a, a.g, a.lg, q2, lk, lq);
+    // k1 and the PIC/FLIP samples share the point x: I(g, x) enters v' with
+    // r_pic + r_flip, I(n, x) with -r_flip
+    float lgx[3], lnx[3];
+    for (int c = 0; c < 3; c++) {
+        lx[c] += lq[c];
+        lgx[c] = a.h_sixth * ax[c] + a.h_half * lq[c] + (a.r_pic + a.r_flip) * av[c];
+        lnx[c] = -a.r_flip * av[c];
+    }
+    mac_adjoint(a, a.g, a.lg, x0, lgx, lx);
+    mac_adjoint(a, a.old, a.ln, x0, lnx, lx);
+    for (int c = 0; c < 3; c++) {
+        a.gx[c][p] = lx[c];
+        a.gv[c][p] = a.r_flip * av[c];
+    }
+}
+
+// Faces of component c within the kernel radius R = dx of x: the node
+// (i,j,k) of component c sits at (i,j,k)*dx + off_c in the frame where the
+// reference splats (fluidsimulation.cpp:2526-2570).  A particle at cell
+// coordinate s along an axis reaches nodes in (s - 1, s + 1); the loop takes
+// floor(s) - 1 .. floor(s) + 2 and lets the kernel's support decide.
+__global__ __launch_bounds__(kBlock) void k_adj_p2g_step(StepAdjoint a) {
+    const int64_t p = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    if (p >= a.n) return;
+    const float dxf_half = a.half_dx;
+    float lx[3] = {0.0f, 0.0f, 0.0f};
+    float lv[3] = {0.0f, 0.0f, 0.0f};
+    float xp[3], vp[3];
+    for (int c = 0; c < 3; c++) {
+        xp[c] = a.x[c][p];
+        vp[c] = a.v[c][p];
+    }
+    const double dx = a.mac.dx;
+    for (int c = 0; c < 3; c++) {
+        const int dims[3] = {a.mac.ni + (c == 0), a.mac.nj + (c == 1), a.mac.nk + (c == 2)};
+        float ps[3];
+        int lo[3], hi[3];
+        for (int ax = 0; ax < 3; ax++) {
+            // p -= offset (scalarfield.cpp:168)
+            ps[ax] = xp[ax] - (ax == c ? 0.0f : dxf_half);
+            const double s = floor((double)ps[ax] * a.mac.inv_dx);
+            // false for NaN; when true, s lies in [-2, dims] and converts safely
+            const bool any = s + 2.0 >= 0.0 && s - 1.0 <= (double)dims[ax] - 1.0;
+            lo[ax] = any ? (int)fmax(s - 1.0, 0.0) : 0;
+            hi[ax] = any ? (int)fmin(s + 2.0, (double)dims[ax] - 1.0) : -1;
+        }
+        const float *__restrict__ W = a.w[c];
+        const float *__restrict__ N = a.old[c];
+        const float *__restrict__ LG = a.lg[c];
+        const float *__restrict__ LN = a.ln[c];
+        for (int k = lo[2]; k <= hi[2]; k++) {
+            const float gz = (float)((float)k * dx);
+            for (int j = lo[1]; j <= hi[1]; j++) {
+                const float gy = (float)((float)j * dx);
+                for (int i = lo[0]; i <= hi[0]; i++) {
+                    const float gx = (float)((float)i * dx);
+                    const float vx = gx - ps[0], vy = gy - ps[1], vz = gz - ps[2];
+                    const float d2 = vx * vx + vy * vy + vz * vz;
+                    const float t = d2 * a.inv_r2;
+                    if (!(t < 1.0f)) continue;
+                    const uint32_t f = (uint32_t)i + (uint32_t)dims[0] * ((uint32_t)j + (uint32_t)dims[1] * (uint32_t)k);
+                    const float wf = W[f];
+                    if (!(wf > 0.0f)) continue;
+                    const float lam = LG[f] + LN[f];
+                    const float iw = 1.0f / wf;
+                    lv[c] += wyvill(t) * lam * iw;
+                    // d(d2)/dx_p = -2 (g - p)
+                    const float coef = lam * (vp[c] - N[f]) * iw * wyvill_dt(t) * a.inv_r2 * -2.0f;
+                    lx[0] += coef * vx;
+                    lx[1] += coef * vy;
+                    lx[2] += coef * vz;
+                }
+            }
+        }
+    }
+    for (int c = 0; c < 3; c++) {
+        a.gx[c][p] += lx[c];
+        a.gv[c][p] += lv[c];
+    }
+}
+
+struct ScalarAdjoint {
+    const float *pos, *val;
+    int64_t n;
+    float off[3];
+    double dx, inv_dx;
+    int dims[3];
+    int reach;      // ceil(R/dx) + 1
+    float inv_r2;
+    const float *gS, *gW;
+    float *gpos, *gval;
+};
+
+__global__ __launch_bounds__(kBlock) void k_adj_p2g_scalar(ScalarAdjoint a) {
+    const int64_t p = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    if (p >= a.n) return;
+    float ps[3];
+    int lo[3], hi[3];
+    for (int ax = 0; ax < 3; ax++) {
+        ps[ax] = a.pos[3 * p + ax] - a.off[ax];
+        const double s = floor((double)ps[ax] * a.inv_dx);
+        const bool any = s + a.reach + 1.0 >= 0.0 && s - a.reach <= (double)a.dims[ax] - 1.0;
+        lo[ax] = any ? (int)fmax(s - a.reach, 0.0) : 0;
+        hi[ax] = any ? (int)fmin(s + a.reach + 1.0, (double)a.dims[ax] - 1.0) : -1;
+    }
+    const float val = a.val ? a.val[p] : 1.0f;
+    float lx[3] = {0.0f, 0.0f, 0.0f};
+    float lval = 0.0f;
+    for (int k = lo[2]; k <= hi[2]; k++) {
+        const float gz = (float)((float)k * a.dx);
+        for (int j = lo[1]; j <= hi[1]; j++) {
+            const float gy = (float)((float)j * a.dx);
+            for (int i = lo[0]; i <= hi[0]; i++) {
+                const float gx = (float)((float)i * a.dx);
+                const float vx = gx - ps[0], vy = gy - ps[1], vz = gz - ps[2];
+                const float t = (vx * vx + vy * vy + vz * vz) * a.inv_r2;
+                if (!(t < 1.0f)) continue;
+                const uint32_t f = (uint32_t)i + (uint32_t)a.dims[0] * ((uint32_t)j + (uint32_t)a.dims[1] * (uint32_t)k);
+                const float gs = a.gS ? a.gS[f] : 0.0f;
+                const float gw = a.gW ? a.gW[f] : 0.0f;
+                lval += wyvill(t) * gs;
+                const float coef = (gs * val + gw) * wyvill_dt(t) * a.inv_r2 * -2.0f;
+                lx[0] += coef * vx;
+                lx[1] += coef * vy;
+                lx[2] += coef * vz;
+            }
+        }
+    }
+    for (int ax = 0; ax < 3; ax++) a.gpos[3 * p + ax] = lx[ax];
+    if (a.gval) a.gval[p] = lval;
+}
+
+}  // namespace
+
+void flip_step_adjoint(const StepAdjoint &a, hipStream_t s) {
+    for (int c = 0; c < 3; c++) {
+        const int64_t nf = (int64_t)(a.mac.ni + (c == 0)) * (a.mac.nj + (c == 1)) * (a.mac.nk + (c == 2));
+        fill_f32(a.lg[c], 0.0f, nf, s);
+        fill_f32(a.ln[c], 0.0f, nf, s);
+    }
+    if (a.n <= 0) return;
+    hipLaunchKernelGGL(k_adj_advect, dim3(blocks_for(a.n)), dim3(kBlock), 0, s, a);
+    GFSX_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_adj_p2g_step, dim3(blocks_for(a.n)), dim3(kBlock), 0, s, a);
+    GFSX_LAUNCH_CHECK();
+}
+
+void p2g_adjoint(const float *pos, const float *val, int64_t n, double radius,
+                 const float off[3], double dx, int ni, int nj, int nk, const float *gS,
+                 const float *gW, float *gpos, float *gval, hipStream_t s) {
+    if (n <= 0) return;
+    ScalarAdjoint a;
+    a.pos = pos;
+    a.val = val;
+    a.n = n;
+    for (int ax = 0; ax < 3; ax++) a.off[ax] = off[ax];
+    a.dx = dx;
+    a.inv_dx = 1.0 / dx;
+    a.dims[0] = ni;
+    a.dims[1] = nj;
+    a.dims[2] = nk;
+    a.reach = (int)std::ceil(radius / dx) + 1;
+    a.inv_r2 = (float)(1.0 / (radius * radius));
+    a.gS = gS;
+    a.gW = gW;
+    a.gpos = gpos;
+    a.gval = gval;
+    hipLaunchKernelGGL(k_adj_p2g_scalar, dim3(blocks_for(n)), dim3(kBlock), 0, s, a);
+    GFSX_LAUNCH_CHECK();
+}
+
+}  // namespace gfsx

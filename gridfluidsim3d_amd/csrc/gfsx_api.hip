@@ -37,6 +37,7 @@ enum Slot {
     S_BOUT, S_BOUTS, S_SCAN, S_OCT, S_POCT,
     S_OLD0, S_XS0, S_XS1, S_XS2, S_VS0, S_VS1, S_VS2, S_DENSE, S_DENSEN,
     S_D_POS, S_D_VAL, S_D_S, S_D_W, S_D_U, S_D_V, S_D_WW, S_D_OUT,
+    S_B_GRID, S_B_OLD, S_B_WGT, S_B_LG, S_B_LN, S_B_XO, S_B_VO,
     NSLOT
 };
 
@@ -627,6 +628,109 @@ void gfsx_flip_step_batched(gfsx_ctx *ctx, const gfsx_scene *scenes, int nscenes
         require(scenes && params, "gfsx_flip_step_batched: null scenes or params");
         require(nscenes >= 1 && nscenes <= 65536, "gfsx_flip_step_batched: nscenes must be 1..65536");
         flip_step_impl(ctx, scenes, params, nscenes, "gfsx_flip_step_batched");
+    });
+}
+
+// ---------------------------------------------------------------- adjoints
+
+void gfsx_p2g_backward_device(gfsx_ctx *ctx, const float *pos, const float *val,
+                              int64_t n, double radius, const float offset[3],
+                              double dx, int ni, int nj, int nk, const float *gS,
+                              const float *gW, float *gpos, float *gval, int *err) {
+    guarded(ctx, err, [&] {
+        check_grid(ni, nj, nk, dx);
+        check_particles(n, pos);
+        require(std::isfinite(radius) && radius > 0.0, "gfsx: radius must be positive and finite");
+        require(radius / dx <= 64.0, "gfsx: radius larger than 64 cells");
+        require(offset != nullptr, "gfsx: null offset");
+        require(gS || gW, "gfsx_p2g_backward_device: no field adjoint given");
+        require(n == 0 || gpos, "gfsx_p2g_backward_device: null position adjoint");
+        gfsx::p2g_adjoint(pos, val, n, radius, offset, dx, ni, nj, nk, gS, gW, gpos, gval,
+                          ctx->stream);
+    });
+}
+
+void gfsx_flip_step_backward(gfsx_ctx *ctx, const gfsx_scene *sc,
+                             const gfsx_step_params *pr, const gfsx_scene_grad *gr,
+                             int *err) {
+    guarded(ctx, err, [&] {
+        const char *who = "gfsx_flip_step_backward";
+        auto req = [&](bool c, const char *what) {
+            if (!c) throw Error(std::string(who) + ": " + what);
+        };
+        req(sc && pr && gr, "null scene, params or gradients");
+        req(pr->stencil == GFSX_TRILINEAR, "the adjoint covers the trilinear stencil only");
+        req(pr->order == GFSX_ORDER_KEEP,
+            "the adjoints refer to particles in input order (GFSX_ORDER_KEEP)");
+        check_grid(sc->ni, sc->nj, sc->nk, sc->dx);
+        check_particles(sc->n, sc->x_in);
+        const int64_t n = sc->n;
+        const float *ins[6] = {sc->x_in, sc->y_in, sc->z_in, sc->vx_in, sc->vy_in, sc->vz_in};
+        const float *aout[6] = {gr->gx_out, gr->gy_out, gr->gz_out,
+                                gr->gvx_out, gr->gvy_out, gr->gvz_out};
+        float *ain[6] = {gr->gx_in, gr->gy_in, gr->gz_in, gr->gvx_in, gr->gvy_in, gr->gvz_in};
+        if (n > 0) {
+            for (int a = 0; a < 6; a++) {
+                req(ins[a] != nullptr, "null particle input");
+                req(aout[a] != nullptr, "null output adjoint");
+                req(ain[a] != nullptr, "null input adjoint");
+                for (int b = 0; b < 6; b++)
+                    req((const float *)ain[a] != ins[b], "input adjoints must not alias the inputs");
+            }
+        }
+        // forward grids of this step, recomputed into the workspace
+        const int ni = sc->ni, nj = sc->nj, nk = sc->nk;
+        int64_t nf[3], tot = 0;
+        for (int c = 0; c < 3; c++) tot += (nf[c] = mac_count(c, ni, nj, nk));
+        float *grid = ctx->f32(S_B_GRID, tot), *old = ctx->f32(S_B_OLD, tot);
+        float *wgt = ctx->f32(S_B_WGT, tot);
+        float *lg = ctx->f32(S_B_LG, tot), *ln = ctx->f32(S_B_LN, tot);
+        float *xo = ctx->f32(S_B_XO, 3 * n), *vo = ctx->f32(S_B_VO, 3 * n);
+        const int64_t at[3] = {0, nf[0], nf[0] + nf[1]};
+        gfsx_scene f = *sc;
+        f.u = grid;
+        f.v = grid + at[1];
+        f.w = grid + at[2];
+        f.u_old = old;
+        f.v_old = old + at[1];
+        f.w_old = old + at[2];
+        f.wu = wgt;
+        f.wv = wgt + at[1];
+        f.ww = wgt + at[2];
+        f.x_out = xo;
+        f.y_out = xo + n;
+        f.z_out = xo + 2 * n;
+        f.vx_out = vo;
+        f.vy_out = vo + n;
+        f.vz_out = vo + 2 * n;
+        f.id_in = nullptr;
+        f.id_out = nullptr;
+        flip_step_impl(ctx, &f, pr, 1, who);
+        gfsx::StepAdjoint a;
+        std::memset(&a, 0, sizeof a);
+        a.n = n;
+        for (int c = 0; c < 3; c++) {
+            a.x[c] = ins[c];
+            a.v[c] = ins[3 + c];
+            a.ax[c] = aout[c];
+            a.av[c] = aout[3 + c];
+            a.gx[c] = ain[c];
+            a.gv[c] = ain[3 + c];
+            a.g[c] = grid + at[c];
+            a.old[c] = old + at[c];
+            a.w[c] = wgt + at[c];
+            a.lg[c] = lg + at[c];
+            a.ln[c] = ln + at[c];
+        }
+        a.mac = mac_grid(nullptr, nullptr, nullptr, ni, nj, nk, sc->dx);
+        a.r_pic = (float)pr->pic_ratio;
+        a.r_flip = (float)(1 - pr->pic_ratio);
+        a.h_half = (float)(0.5 * pr->dt);
+        a.h_full = (float)pr->dt;
+        a.h_sixth = (float)(pr->dt / 6.0f);
+        a.half_dx = (float)(0.5 * sc->dx);
+        a.inv_r2 = (float)(1.0 / (sc->dx * sc->dx));
+        gfsx::flip_step_adjoint(a, ctx->stream);
     });
 }
 

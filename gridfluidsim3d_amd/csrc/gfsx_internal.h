@@ -192,11 +192,36 @@ __device__ __forceinline__ float wyvill(float t) {
     return a * a * fmaf(4.0f / 9.0f, a, 5.0f / 9.0f);
 }
 
+// Adjoint of one plain step (trilinear stencil, particles in input order):
+// the step's inputs, the adjoints of its outputs, the grids of the forward
+// step recomputed from those inputs, and face workspace for the grid
+// adjoints.
+struct StepAdjoint {
+    int64_t n;
+    const float *x[3], *v[3];      // step inputs
+    const float *ax[3], *av[3];    // dL/dx', dL/dv'
+    float *gx[3], *gv[3];          // dL/dx, dL/dv (written)
+    const float *g[3];             // grid after the body force
+    const float *old[3];           // normalised P2G result
+    const float *w[3];             // P2G weight sums
+    float *lg[3], *ln[3];          // dL/dg, dL/dn (workspace, zeroed here)
+    MacGrid mac;                   // dims, dx, limits (f unused)
+    float r_pic, r_flip;
+    float h_half, h_full, h_sixth;
+    float half_dx;                 // (float)(0.5*dx): the face offsets
+    float inv_r2;                  // (float)(1/dx^2): P2G radius R = dx
+};
+
 // ---- launches (implemented in gfsx_kernels.hip) ------------------------
 
 void fill_u32(uint32_t *p, uint32_t value, int64_t count, hipStream_t s);
 void fill_f32(float *p, float value, int64_t count, hipStream_t s);
 void fill_u8(uint8_t *p, uint8_t value, int64_t count, hipStream_t s);
+// gfsx_adjoint.hip
+void flip_step_adjoint(const StepAdjoint &a, hipStream_t s);
+void p2g_adjoint(const float *pos, const float *val, int64_t n, double radius,
+                 const float off[3], double dx, int ni, int nj, int nk, const float *gS,
+                 const float *gW, float *gpos, float *gval, hipStream_t s);
 int64_t scan_scratch_words(int64_t m);
 void exclusive_scan(const uint32_t *in, uint32_t *out, int64_t m,
                     uint32_t *scratch, hipStream_t s);

@@ -59,6 +59,16 @@ class StepParams(ctypes.Structure):
     ]
 
 
+class SceneGrad(ctypes.Structure):
+    """ctypes image of ``gfsx_scene_grad`` (include/gfsx.h)."""
+    _fields_ = [
+        ("gx_out", c_void_p), ("gy_out", c_void_p), ("gz_out", c_void_p),
+        ("gvx_out", c_void_p), ("gvy_out", c_void_p), ("gvz_out", c_void_p),
+        ("gx_in", c_void_p), ("gy_in", c_void_p), ("gz_in", c_void_p),
+        ("gvx_in", c_void_p), ("gvy_in", c_void_p), ("gvz_in", c_void_p),
+    ]
+
+
 def mac_shapes(dims):
     """(k, j, i) array shapes of the u, v, w faces of an ni*nj*nk grid
     (macvelocityfield.cpp:37-45)."""
@@ -228,6 +238,31 @@ class Context:
                                 _dptr(W, "W", torch.float32, ni * nj * nk)])
         return S, W
 
+    def p2g_backward_device(self, pos, val, radius, offset, dx, node_dims, gS=None,
+                            gW=None, gpos=None, gval=None):
+        """Adjoint of ``p2g_device`` (``gfsx_p2g_backward_device``): returns
+        (dL/dpos shaped like pos, dL/dval or None) for field adjoints gS, gW
+        on the node grid (either may be None)."""
+        import torch
+        ni, nj, nk = node_dims
+        n = pos.shape[0]
+        off = _np_f32(offset, (3,))
+        gpos = torch.empty_like(pos) if gpos is None else gpos
+        if gval is None and val is not None and gS is not None:
+            gval = torch.empty_like(val)
+        self.use_torch_stream()
+        f = pb.lib_func("gfsx_p2g_backward_device",
+                        [c_void_p, c_void_p, c_void_p, c_int64, c_double, _fp, c_double,
+                         c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p])
+        pb.execute_lib_func(f, [self(), _dptr(pos, "pos", torch.float32, 3 * n),
+                                _dptr(val, "val", torch.float32, n), n, float(radius),
+                                _ptr(off), float(dx), ni, nj, nk,
+                                _dptr(gS, "gS", torch.float32, ni * nj * nk),
+                                _dptr(gW, "gW", torch.float32, ni * nj * nk),
+                                _dptr(gpos, "gpos", torch.float32, 3 * n),
+                                _dptr(gval, "gval", torch.float32, n)])
+        return gpos, gval
+
     def _mac_dev(self, u, v, w, dims):
         import torch
         shapes = mac_shapes(dims)
@@ -339,6 +374,80 @@ class Context:
         f = pb.lib_func("gfsx_flip_step", [c_void_p, POINTER(Scene), POINTER(StepParams)])
         pb.execute_lib_func(f, [self(), ctypes.byref(sc), ctypes.byref(pr)])
         st.cur = 1 - st.cur
+
+    @staticmethod
+    def _soa_scene(x, v, dims, dx, x_out=None, v_out=None, grids=None):
+        """A C scene over (3, n) SoA tensors: x, v in; x_out, v_out, grids
+        (u, v, w) out where given."""
+        import torch
+        sc = Scene()
+        sc.ni, sc.nj, sc.nk = dims
+        sc.dx = float(dx)
+        n = int(x.shape[1])
+        sc.n = n
+        f32 = torch.float32
+        for t, nm in ((x, "x"), (v, "v"), (x_out, "x_out"), (v_out, "v_out")):
+            if t is not None and (t.dim() != 2 or t.shape[0] != 3 or t.shape[1] != n):
+                raise ValueError("%s must have shape (3, %d)" % (nm, n))
+        names_in = ("x_in", "y_in", "z_in", "vx_in", "vy_in", "vz_in")
+        for nm, t in zip(names_in, [x[0], x[1], x[2], v[0], v[1], v[2]]):
+            setattr(sc, nm, _dptr(t, nm, f32, n))
+        if x_out is not None:
+            names_out = ("x_out", "y_out", "z_out", "vx_out", "vy_out", "vz_out")
+            for nm, t in zip(names_out, [x_out[0], x_out[1], x_out[2],
+                                         v_out[0], v_out[1], v_out[2]]):
+                setattr(sc, nm, _dptr(t, nm, f32, n))
+        if grids is not None:
+            for nm, t, sh in zip(("u", "v", "w"), grids, mac_shapes(dims)):
+                setattr(sc, nm, _dptr(t, nm, f32, int(np.prod(sh))))
+        return sc
+
+    def flip_step_soa(self, x, v, dims, dx, dt, body_force=(0.0, -9.8, 0.0), pic_ratio=0.05,
+                      x_out=None, v_out=None, grids=None):
+        """One plain step (trilinear, output in input order) of particles held
+        as (3, n) float32 tensors; returns (x_out, v_out, grids)."""
+        import torch
+        x_out = torch.empty_like(x) if x_out is None else x_out
+        v_out = torch.empty_like(v) if v_out is None else v_out
+        if grids is None:
+            grids = [torch.empty(sh, dtype=torch.float32, device=x.device)
+                     for sh in mac_shapes(dims)]
+        sc = self._soa_scene(x, v, dims, dx, x_out, v_out, grids)
+        pr = self._params(dt, body_force, pic_ratio, TRILINEAR, ORDER_KEEP)
+        self.use_torch_stream()
+        f = pb.lib_func("gfsx_flip_step", [c_void_p, POINTER(Scene), POINTER(StepParams)])
+        pb.execute_lib_func(f, [self(), ctypes.byref(sc), ctypes.byref(pr)])
+        return x_out, v_out, grids
+
+    def flip_step_backward(self, x, v, gx_out, gv_out, dims, dx, dt,
+                           body_force=(0.0, -9.8, 0.0), pic_ratio=0.05, gx=None, gv=None):
+        """Adjoint of ``flip_step_soa`` (``gfsx_flip_step_backward``): from the
+        step's inputs x, v and the adjoints of its outputs, all (3, n)
+        float32 tensors, returns (dL/dx, dL/dv)."""
+        import torch
+        gx = torch.empty_like(x) if gx is None else gx
+        gv = torch.empty_like(v) if gv is None else gv
+        n = int(x.shape[1])
+        for t, nm in ((gx_out, "gx_out"), (gv_out, "gv_out"), (gx, "gx"), (gv, "gv")):
+            if t.dim() != 2 or t.shape[0] != 3 or t.shape[1] != n:
+                raise ValueError("%s must have shape (3, %d)" % (nm, n))
+        sc = self._soa_scene(x, v, dims, dx)
+        pr = self._params(dt, body_force, pic_ratio, TRILINEAR, ORDER_KEEP)
+        g = SceneGrad()
+        f32 = torch.float32
+        for nm, t in zip(("gx_out", "gy_out", "gz_out"), gx_out):
+            setattr(g, nm, _dptr(t, nm, f32, n))
+        for nm, t in zip(("gvx_out", "gvy_out", "gvz_out"), gv_out):
+            setattr(g, nm, _dptr(t, nm, f32, n))
+        for nm, t in zip(("gx_in", "gy_in", "gz_in"), gx):
+            setattr(g, nm, _dptr(t, nm, f32, n))
+        for nm, t in zip(("gvx_in", "gvy_in", "gvz_in"), gv):
+            setattr(g, nm, _dptr(t, nm, f32, n))
+        self.use_torch_stream()
+        f = pb.lib_func("gfsx_flip_step_backward",
+                        [c_void_p, POINTER(Scene), POINTER(StepParams), POINTER(SceneGrad)])
+        pb.execute_lib_func(f, [self(), ctypes.byref(sc), ctypes.byref(pr), ctypes.byref(g)])
+        return gx, gv
 
     def capture_batched(self, states, dt, body_force=(0.0, -9.8, 0.0), pic_ratio=0.05,
                         stencil=TRILINEAR, order=ORDER_SORTED):

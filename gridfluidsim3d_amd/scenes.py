@@ -212,6 +212,50 @@ def batch64(seed=1004, nscenes=64, dx=1.0 / 64.0, dt=1.0 / 240.0):
     return out
 
 
+def slab(seed=1005, dims=(96, 96, 96), dx=1.0 / 96.0, ppc=8, y0=20, layers=27,
+         amplitude=1.0, dt=1.0 / 240.0):
+    """The differentiable workload: a jittered slab of ``layers`` cell layers
+    from cell row y0 (all of x and z), ppc per cell, curl noise at
+    ``amplitude`` m/s.  At 96^3 this is 1,990,656 particles."""
+    rng = np.random.default_rng(seed)
+    mask = np.zeros(dims[::-1], bool)
+    mask[:, y0:y0 + layers, :] = True
+    pos = seed_cells(mask, dx, rng, ppc=ppc)
+    vel = curl_noise(pos, dims[0] * dx, amplitude, rng)
+    return SceneSpec("slab%d" % dims[0], dims, dx, pos, vel, dt)
+
+
+def slab_subset32(seed=1006):
+    """The gradient-check subset: a 32^3 grid (dx 1/32) holding a 16x20x8-cell
+    jittered block, 8 per cell (20,480 particles), curl noise at 1 m/s."""
+    rng = np.random.default_rng(seed)
+    dims, dx = (32, 32, 32), 1.0 / 32.0
+    mask = np.zeros(dims[::-1], bool)
+    mask[12:20, 6:26, 8:24] = True
+    pos = seed_cells(mask, dx, rng)
+    vel = curl_noise(pos, dims[0] * dx, 1.0, rng)
+    return SceneSpec("slab_subset32", dims, dx, pos, vel, 1.0 / 240.0)
+
+
+def density_target(dims, dx, seed, level=6.5, y_range=None):
+    """A seeded smooth target for the density (splat weight) field on the
+    cell centres, shaped (nk, nj, ni) float32: ``level`` times a Gaussian
+    profile across y (centred on y_range, the slab's cell rows) times
+    1 + 0.3 sin(.) sin(.) waves along x and z."""
+    rng = np.random.default_rng(seed)
+    cx, cy, cz = cell_centres(dims, dx)
+    ext = np.array(dims, np.float64) * dx
+    lo, hi = y_range if y_range is not None else (0, dims[1])
+    yc = 0.5 * (lo + hi) * dx
+    wy = 0.5 * (hi - lo) * dx
+    kx, kz = rng.integers(1, 3, size=2)
+    p1, p2 = rng.uniform(0, 2 * math.pi, size=2)
+    prof = np.exp(-((cy - yc) / wy) ** 2)
+    wave = 1.0 + 0.3 * np.sin(2 * math.pi * kx * cx / ext[0] + p1) * \
+        np.sin(2 * math.pi * kz * cz / ext[2] + p2)
+    return (level * prof * wave).astype(np.float32)
+
+
 CONFIGS = {
     "sphere64": sphere64,
     "dambreak128": dambreak128,

@@ -200,6 +200,49 @@ int gfsx_get_option(gfsx_ctx *ctx, const char *name, int *err);
 void gfsx_timing_begin(gfsx_ctx *ctx, int *err);
 int gfsx_timing_end(gfsx_ctx *ctx, double stage_ms[4], int *err);
 
+/* ------------------------------------------------------------------------ */
+/* Adjoints (differentiable form).                                          */
+/* ------------------------------------------------------------------------ */
+/* Adjoint of gfsx_p2g_device: given dL/dS (gS) and/or dL/dW (gW) on the
+ * node grid, writes dL/dpos (gpos, n*3 floats) and, when gval is not NULL,
+ * dL/dval (gval, n floats) for S = sum K(|node - pos|^2) val and W = sum K.
+ * One thread per point gathers over the nodes in its kernel radius (no
+ * atomics; bit-identical run to run).  The reference has no adjoint; this
+ * differentiates the splat of CLScalarField::addPointValues / addPoints
+ * (clscalarfield.cpp:68-122,143-301, scalarfield.cpp:167-201). */
+void gfsx_p2g_backward_device(gfsx_ctx *ctx, const float *pos, const float *val,
+                              int64_t n, double radius, const float offset[3],
+                              double dx, int ni, int nj, int nk, const float *gS,
+                              const float *gW, float *gpos, float *gval, int *err);
+
+/* Adjoints of one step's outputs (d(loss)/d(x_out..), d(loss)/d(vx_out..))
+ * and, written by the call, of its inputs.  SoA device arrays of n floats;
+ * the written arrays must not alias the scene's inputs (they may alias the
+ * output adjoints). */
+typedef struct gfsx_scene_grad {
+    const float *gx_out, *gy_out, *gz_out;
+    const float *gvx_out, *gvy_out, *gvz_out;
+    float *gx_in, *gy_in, *gz_in;
+    float *gvx_in, *gvy_in, *gvz_in;
+} gfsx_scene_grad;
+
+/* Adjoint of gfsx_flip_step for the trilinear stencil with
+ * GFSX_ORDER_KEEP (output particle p is input particle p).  Reads the
+ * step's inputs from scene (x_in.., vx_in..; the scene's output and grid
+ * fields are not used), recomputes the forward grids (P2G, weights,
+ * snapshot, body force) in the context's workspace, then back-propagates
+ * through RK4 (stage positions recomputed from x_in), the PIC/FLIP blend,
+ * the trilinear samples and the P2G normalisation (quotient rule:
+ * dL/dS = dL/dn / W, dL/dW = -dL/dn * n / W).  The fluid mask and cell
+ * indices are piecewise constant, so they contribute nothing.  The grid
+ * adjoints are accumulated with float atomics, so the last bits may differ
+ * run to run.  The reference has no adjoint; this differentiates the step
+ * gfsx_flip_step replaces (fluidsimulation.cpp:2526-2595, 3104-3129,
+ * 3181-3209; particleadvector.cpp:1045-1054). */
+void gfsx_flip_step_backward(gfsx_ctx *ctx, const gfsx_scene *scene,
+                             const gfsx_step_params *params,
+                             const gfsx_scene_grad *grad, int *err);
+
 /* Stable counting sort of particle indices by cell (the binning the P2G
  * gather uses), exposed for testing: perm[n] receives the indices ordered by
  * cell, then by index; cell_start[ni*nj*nk+1] the cell offsets.  Particles

@@ -1,0 +1,131 @@
+"""Pins the gradient references before the GPU adjoints are checked against
+them (CPU only):
+
+* the dense FP64 PyTorch step (tests/dense_flip.py) reproduces the C
+  oracle's FP64 step and splat (``gfo64_flip_step``, ``gfo64_p2g``) to FP64
+  rounding, so its autograd gradient is the gradient of the oracle's maths;
+* that gradient agrees with FP64 central finite differences of the C
+  oracle itself, along random directions, for single steps and a rollout
+  with the density loss;
+* the Wyvill kernel's derivative used by the adjoint kernels
+  (-a (10 + 12 a) / 9, a = 1 - t) is the derivative of the reference's
+  expanded polynomial.
+"""
+import numpy as np
+import pytest
+import torch
+
+import dense_flip as dfl
+
+F64 = torch.float64
+
+
+def small_scene(seed, dims=(10, 9, 8), dx=0.1, box=((2, 7), (1, 6), (2, 6)), ppc=4, amp=1.5):
+    """Jittered particles in a box of cells plus a few loose ones, smooth
+    random velocities: (pos, vel) float32 (n, 3)."""
+    from gridfluidsim3d_amd import scenes
+    rng = np.random.default_rng(seed)
+    ni, nj, nk = dims
+    mask = np.zeros((nk, nj, ni), bool)
+    (i0, i1), (j0, j1), (k0, k1) = box
+    mask[k0:k1, j0:j1, i0:i1] = True
+    pos = scenes.seed_cells(mask, dx, rng, ppc=ppc)
+    loose = (rng.uniform(0.1, 0.9, size=(6, 3)) * np.array(dims) * dx).astype(np.float32)
+    pos = np.concatenate([pos, loose])
+    vel = scenes.curl_noise(pos, max(dims) * dx, amp, rng).astype(np.float32)
+    return pos, vel
+
+
+def oracle_rollout_loss(orc64, pos, vel, dims, dx, dt, steps, target, a=None, b=None):
+    """FP64 rollout with the C oracle; loss = sum (density - target)^2, or,
+    with a/b given, <a, x'> + <b, v'> after the steps."""
+    x, v = pos.astype(np.float64), vel.astype(np.float64)
+    for _ in range(steps):
+        r = orc64.flip_step(x, v, dims, dx, dt)
+        x, v = r["pos"], r["vel"]
+    if a is not None:
+        return float((a * x).sum() + (b * v).sum())
+    h = 0.5 * dx
+    _, W = orc64.p2g(x, None, dx, (h, h, h), dx, dims)
+    return float(((W - target) ** 2).sum())
+
+
+def test_wyvill_derivative_closed_form():
+    t = torch.linspace(0.0, 1.2, 241, dtype=F64, requires_grad=True)
+    k = dfl.wyvill(t)
+    (g,) = torch.autograd.grad(k.sum(), t)
+    a = torch.clamp(1.0 - t.detach(), min=0.0)
+    closed = -a * (10.0 + 12.0 * a) / 9.0
+    assert torch.allclose(g, closed, atol=1e-12)
+    # the factorised kernel the GPU evaluates is the same polynomial
+    fact = a * a * (5.0 / 9.0 + 4.0 / 9.0 * a)
+    assert torch.allclose(k.detach(), fact, atol=1e-13)
+
+
+def test_dense_step_matches_oracle64(orc64):
+    dims, dx, dt = (10, 9, 8), 0.1, 1.0 / 60.0
+    pos, vel = small_scene(11, dims, dx)
+    r = orc64.flip_step(pos, vel, dims, dx, dt)
+    x1, v1 = dfl.step(torch.tensor(pos, dtype=F64), torch.tensor(vel, dtype=F64), dims, dx, dt)
+    assert np.abs(x1.numpy() - r["pos"]).max() < 1e-12
+    assert np.abs(v1.numpy() - r["vel"]).max() < 1e-11
+
+
+def test_dense_density_matches_oracle64(orc64):
+    dims, dx = (10, 9, 8), 0.1
+    pos, _ = small_scene(12, dims, dx)
+    h = 0.5 * dx
+    _, W = orc64.p2g(pos, None, dx, (h, h, h), dx, dims)
+    d = dfl.density(torch.tensor(pos, dtype=F64), dims, dx)
+    assert np.abs(d.numpy() - W).max() < 1e-12
+
+
+@pytest.mark.parametrize("seed", [21, 22])
+def test_dense_step_gradient_matches_fd(orc64, seed):
+    """d<a,x'> + <b,v'> / d(x, v) along random directions: autograd of the
+    dense step vs central differences of the oracle, both FP64."""
+    dims, dx, dt = (10, 9, 8), 0.1, 1.0 / 60.0
+    pos, vel = small_scene(seed, dims, dx)
+    rng = np.random.default_rng(seed + 100)
+    a = rng.normal(size=pos.shape)
+    b = rng.normal(size=pos.shape)
+    x = torch.tensor(pos, dtype=F64, requires_grad=True)
+    v = torch.tensor(vel, dtype=F64, requires_grad=True)
+    x1, v1 = dfl.step(x, v, dims, dx, dt)
+    L = (torch.tensor(a) * x1).sum() + (torch.tensor(b) * v1).sum()
+    gx, gv = torch.autograd.grad(L, (x, v))
+    eps = 1e-7
+    for _ in range(3):
+        dxd = rng.normal(size=pos.shape) * 1e-2
+        dvd = rng.normal(size=pos.shape)
+        lp = oracle_rollout_loss(orc64, pos + eps * dxd, vel + eps * dvd, dims, dx, dt, 1,
+                                 None, a, b)
+        lm = oracle_rollout_loss(orc64, pos - eps * dxd, vel - eps * dvd, dims, dx, dt, 1,
+                                 None, a, b)
+        fd = (lp - lm) / (2 * eps)
+        an = float((gx.numpy() * dxd).sum() + (gv.numpy() * dvd).sum())
+        assert abs(fd - an) <= 1e-6 * max(abs(fd), 1.0), (fd, an)
+
+
+def test_dense_rollout_density_gradient_matches_fd(orc64):
+    """Gradient of the density loss after 3 steps w.r.t. the initial
+    velocities: dense autograd vs central differences of the oracle."""
+    dims, dx, dt = (10, 9, 8), 0.1, 1.0 / 60.0
+    pos, vel = small_scene(31, dims, dx)
+    rng = np.random.default_rng(131)
+    target = rng.uniform(0.0, 2.0, size=(dims[2], dims[1], dims[0]))
+    x = torch.tensor(pos, dtype=F64)
+    v0 = torch.tensor(vel, dtype=F64, requires_grad=True)
+    xx, vv = x, v0
+    for _ in range(3):
+        xx, vv = dfl.step(xx, vv, dims, dx, dt)
+    L = ((dfl.density(xx, dims, dx) - torch.tensor(target)) ** 2).sum()
+    (g,) = torch.autograd.grad(L, v0)
+    eps = 1e-7
+    for _ in range(3):
+        d = rng.normal(size=vel.shape)
+        lp = oracle_rollout_loss(orc64, pos, vel + eps * d, dims, dx, dt, 3, target)
+        lm = oracle_rollout_loss(orc64, pos, vel - eps * d, dims, dx, dt, 3, target)
+        fd = (lp - lm) / (2 * eps)
+        an = float((g.numpy() * d).sum())
+        assert abs(fd - an) <= 1e-6 * max(abs(fd), 1.0), (fd, an)

@@ -55,12 +55,13 @@ __device__ __forceinline__ float wyvill_dt(float t) {
 }
 
 // One MAC component's trilinear sample at cell (i,j,k), fractions t: adds
-// lam * dI/dt to dt_out and scatters lam * (corner weight) into lf.
-// Out-of-range corners read 0 and receive nothing
-// (macvelocityfield.cpp:470-478).
-__device__ __forceinline__ void tri_adjoint(const float *__restrict__ f, float *lf, int w, int h,
-                                            int d, int i, int j, int k, float tx, float ty,
-                                            float tz, float lam, float dt_out[3]) {
+// lam * dI/dt to dt_out and hands lam * (corner weight) for every in-range
+// corner to sink(corner i, j, k, flat index, value).  Out-of-range corners
+// read 0 and receive nothing (macvelocityfield.cpp:470-478).
+template <class Sink>
+__device__ __forceinline__ void tri_adjoint(const float *__restrict__ f, int w, int h, int d,
+                                            int i, int j, int k, float tx, float ty, float tz,
+                                            float lam, float dt_out[3], Sink &&sink) {
     if (lam == 0.0f) return;
     const bool vi0 = (unsigned)i < (unsigned)w, vi1 = (unsigned)(i + 1) < (unsigned)w;
     const bool vj0 = (unsigned)j < (unsigned)h, vj1 = (unsigned)(j + 1) < (unsigned)h;
@@ -87,22 +88,24 @@ __device__ __forceinline__ void tri_adjoint(const float *__restrict__ f, float *
                         sx * tz * (v011 - v001) + tx * tz * (v111 - v101));
     dt_out[2] += lam * (sx * sy * (v001 - v000) + tx * sy * (v101 - v100) +
                         sx * ty * (v011 - v010) + tx * ty * (v111 - v110));
-    if (o000) unsafeAtomicAdd(lf + r00 + i0, lam * (sx * sy * sz));
-    if (o100) unsafeAtomicAdd(lf + r00 + i1, lam * (tx * sy * sz));
-    if (o010) unsafeAtomicAdd(lf + r10 + i0, lam * (sx * ty * sz));
-    if (o110) unsafeAtomicAdd(lf + r10 + i1, lam * (tx * ty * sz));
-    if (o001) unsafeAtomicAdd(lf + r01 + i0, lam * (sx * sy * tz));
-    if (o101) unsafeAtomicAdd(lf + r01 + i1, lam * (tx * sy * tz));
-    if (o011) unsafeAtomicAdd(lf + r11 + i0, lam * (sx * ty * tz));
-    if (o111) unsafeAtomicAdd(lf + r11 + i1, lam * (tx * ty * tz));
+    if (o000) sink(i, j, k, r00 + i0, lam * (sx * sy * sz));
+    if (o100) sink(i + 1, j, k, r00 + i1, lam * (tx * sy * sz));
+    if (o010) sink(i, j + 1, k, r10 + i0, lam * (sx * ty * sz));
+    if (o110) sink(i + 1, j + 1, k, r10 + i1, lam * (tx * ty * sz));
+    if (o001) sink(i, j, k + 1, r01 + i0, lam * (sx * sy * tz));
+    if (o101) sink(i + 1, j, k + 1, r01 + i1, lam * (tx * sy * tz));
+    if (o011) sink(i, j + 1, k + 1, r11 + i0, lam * (sx * ty * tz));
+    if (o111) sink(i + 1, j + 1, k + 1, r11 + i1, lam * (tx * ty * tz));
 }
 
 // Adjoint of the MAC sample I(f, q) at q: lq += (dI/dq)^T lam, and lam
-// scattered into lf.  Zero outside the grid, where the sample is the
+// scattered through sink(array, i, j, k, flat, value) with array = base + c
+// (c the component).  Zero outside the grid, where the sample is the
 // constant 0 (macvelocityfield.cpp:550-552).
-__device__ __forceinline__ void mac_adjoint(const StepAdjoint &a, const float *const f[3],
-                                            float *const lf[3], const float q[3],
-                                            const float lam[3], float lq[3]) {
+template <class Sink>
+__device__ __forceinline__ void mac_adjoint(const StepAdjoint &a, const float *const f[3], int base,
+                                            const float q[3], const float lam[3], float lq[3],
+                                            Sink &&sink) {
     if (!(q[0] >= 0.0f && q[1] >= 0.0f && q[2] >= 0.0f && q[0] < a.mac.limf[0] &&
           q[1] < a.mac.limf[1] && q[2] < a.mac.limf[2]))
         return;
@@ -111,23 +114,24 @@ __device__ __forceinline__ void mac_adjoint(const StepAdjoint &a, const float *c
     const AxisCoords cz = axis_coords(q[2], a.mac.inv_dx);
     const int ni = a.mac.ni, nj = a.mac.nj, nk = a.mac.nk;
     float dt[3] = {0.0f, 0.0f, 0.0f};
-    tri_adjoint(f[0], lf[0], ni + 1, nj, nk, cx.ia, cy.ib, cz.ib, cx.ta, cy.tb, cz.tb, lam[0], dt);
-    tri_adjoint(f[1], lf[1], ni, nj + 1, nk, cx.ib, cy.ia, cz.ib, cx.tb, cy.ta, cz.tb, lam[1], dt);
-    tri_adjoint(f[2], lf[2], ni, nj, nk + 1, cx.ib, cy.ib, cz.ia, cx.tb, cy.tb, cz.ta, lam[2], dt);
+    tri_adjoint(f[0], ni + 1, nj, nk, cx.ia, cy.ib, cz.ib, cx.ta, cy.tb, cz.tb, lam[0], dt,
+                [&](int i, int j, int k, uint32_t o, float v) { sink(base, i, j, k, o, v); });
+    tri_adjoint(f[1], ni, nj + 1, nk, cx.ib, cy.ia, cz.ib, cx.tb, cy.ta, cz.tb, lam[1], dt,
+                [&](int i, int j, int k, uint32_t o, float v) { sink(base + 1, i, j, k, o, v); });
+    tri_adjoint(f[2], ni, nj, nk + 1, cx.ib, cy.ib, cz.ia, cx.tb, cy.tb, cz.ta, lam[2], dt,
+                [&](int i, int j, int k, uint32_t o, float v) { sink(base + 2, i, j, k, o, v); });
     // the cell fractions move by 1/dx per unit of x
     const float inv_dx = (float)a.mac.inv_dx;
     for (int c = 0; c < 3; c++) lq[c] += dt[c] * inv_dx;
 }
 
-__global__ __launch_bounds__(kBlock) void k_adj_advect(StepAdjoint a) {
-    const int64_t p = blockIdx.x * (int64_t)kBlock + threadIdx.x;
-    if (p >= a.n) return;
-    float x0[3], ax[3], av[3];
-    for (int c = 0; c < 3; c++) {
-        x0[c] = a.x[c][p];
-        ax[c] = a.ax[c][p];
-        av[c] = a.av[c][p];
-    }
+// The RK4 + PIC/FLIP adjoint of one particle: from x0 and the output
+// adjoints ax (of x') and av (of v'), returns lx (of x0) and scatters the
+// grid adjoints through sink (arrays 0-2: dL/dg, 3-5: dL/dn).
+template <class Sink>
+__device__ __forceinline__ void adj_particle(const StepAdjoint &a, const float x0[3],
+                                             const float ax[3], const float av[3], float lx[3],
+                                             Sink &&sink) {
     // forward stages, as the fused step computes them (particleadvector.cpp:1045-1054)
     MacGrid g = a.mac;
     for (int c = 0; c < 3; c++) g.f[c] = a.g[c];
@@ -139,25 +143,25 @@ __global__ __launch_bounds__(kBlock) void k_adj_advect(StepAdjoint a) {
     sample_mac<0, false>(g, nullptr, nullptr, nullptr, q3[0], q3[1], q3[2], k3, unused);
     for (int c = 0; c < 3; c++) q4[c] = x0[c] + a.h_full * k3[c];
     // x' = x + sixth*(((k1 + 2k2) + 2k3) + k4), walked backwards
-    float lk[3], lq[3], lx[3];
+    float lk[3], lq[3];
     for (int c = 0; c < 3; c++) {
         lx[c] = ax[c];
         lk[c] = a.h_sixth * ax[c];               // k4
         lq[c] = 0.0f;
     }
-    mac_adjoint(a, a.g, a.lg, q4, lk, lq);
+    mac_adjoint(a, a.g, 0, q4, lk, lq, sink);
     for (int c = 0; c < 3; c++) {
         lx[c] += lq[c];
         lk[c] = 2.0f * a.h_sixth * ax[c] + a.h_full * lq[c];    // k3
         lq[c] = 0.0f;
     }
-    mac_adjoint(a, a.g, a.lg, q3, lk, lq);
+    mac_adjoint(a, a.g, 0, q3, lk, lq, sink);
     for (int c = 0; c < 3; c++) {
         lx[c] += lq[c];
         lk[c] = 2.0f * a.h_sixth * ax[c] + a.h_half * lq[c];    // k2
         lq[c] = 0.0f;
     }
-    mac_adjoint(a, a.g, a.lg, q2, lk, lq);
+    mac_adjoint(a, a.g, 0, q2, lk, lq, sink);
     // k1 and the PIC/FLIP samples share the point x: I(g, x) enters v' with
     // r_pic + r_flip, I(n, x) with -r_flip
     float lgx[3], lnx[3];
@@ -166,11 +170,141 @@ __global__ __launch_bounds__(kBlock) void k_adj_advect(StepAdjoint a) {
         lgx[c] = a.h_sixth * ax[c] + a.h_half * lq[c] + (a.r_pic + a.r_flip) * av[c];
         lnx[c] = -a.r_flip * av[c];
     }
-    mac_adjoint(a, a.g, a.lg, x0, lgx, lx);
-    mac_adjoint(a, a.old, a.ln, x0, lnx, lx);
+    mac_adjoint(a, a.g, 0, x0, lgx, lx, sink);
+    mac_adjoint(a, a.old, 3, x0, lnx, lx, sink);
+}
+
+// One thread per particle in input order; grid adjoints straight to global
+// memory.
+__global__ __launch_bounds__(kBlock) void k_adj_advect(StepAdjoint a) {
+    const int64_t p = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    if (p >= a.n) return;
+    float x0[3], ax[3], av[3], lx[3];
+    for (int c = 0; c < 3; c++) {
+        x0[c] = a.x[c][p];
+        ax[c] = a.ax[c][p];
+        av[c] = a.av[c][p];
+    }
+    adj_particle(a, x0, ax, av, lx, [&](int arr, int, int, int, uint32_t o, float v) {
+        unsafeAtomicAdd((arr < 3 ? a.lg[arr] : a.ln[arr - 3]) + o, v);
+    });
     for (int c = 0; c < 3; c++) {
         a.gx[c][p] = lx[c];
         a.gv[c][p] = a.r_flip * av[c];
+    }
+}
+
+// Bricks of BX x BY x BY bins of the forward sort, one work-group each.  A
+// brick's particles (BY*BZ runs of sorted slots, one per bin row) add their
+// grid adjoints into an LDS tile of every face index within M cells of the
+// brick's trilinear footprint, which is then added to global memory once per
+// face.  Samples that leave the tile (a stage moved more than M cells) go to
+// global memory directly.  Work-groups past the bricks take the particles of
+// the overflow bin (outside the bin grid) with global adjoints.
+namespace abrick {
+constexpr int BX = 8, BY = 4, BZ = 4;
+constexpr int M = 1;
+constexpr int TX = BX + 2 + 2 * M, TY = BY + 2 + 2 * M, TZ = BZ + 2 + 2 * M;
+constexpr int TN = TX * TY * TZ;
+constexpr int NROW = BY * BZ;
+constexpr int NEXTRA = 16;
+}  // namespace abrick
+
+__global__ __launch_bounds__(kBlock) void k_adj_advect_brick(StepAdjoint a, AdjointSort so,
+                                                             int nbx, int nby, uint32_t nbrick) {
+    using namespace abrick;
+    __shared__ float tile[6 * TN];
+    __shared__ uint32_t s_r0[NROW];
+    __shared__ uint32_t s_rs[NROW + 1];
+    const int tid = threadIdx.x;
+    const uint32_t bid = blockIdx.x;
+    auto global_sink = [&](int arr, int, int, int, uint32_t o, float v) {
+        unsafeAtomicAdd((arr < 3 ? a.lg[arr] : a.ln[arr - 3]) + o, v);
+    };
+    auto finish = [&](uint32_t q, const float lx[3], const float av[3]) {
+        const uint32_t p = so.perm[q];
+        for (int c = 0; c < 3; c++) {
+            a.gx[c][p] = lx[c];
+            a.gv[c][p] = a.r_flip * av[c];
+        }
+    };
+    if (bid >= nbrick) {
+        // overflow bin: grid-stride over its slots
+        const uint32_t q0 = so.start[so.bg.nbin], q1 = so.start[so.bg.nbin + 1];
+        for (uint32_t q = q0 + (bid - nbrick) * kBlock + tid; q < q1; q += NEXTRA * kBlock) {
+            const uint32_t p = so.perm[q];
+            float x0[3], ax[3], av[3], lx[3];
+            for (int c = 0; c < 3; c++) {
+                x0[c] = so.xs[c][q];
+                ax[c] = a.ax[c][p];
+                av[c] = a.av[c][p];
+            }
+            adj_particle(a, x0, ax, av, lx, global_sink);
+            finish(q, lx, av);
+        }
+        return;
+    }
+    const int bx = (int)(bid % (uint32_t)nbx);
+    const int by = (int)((bid / (uint32_t)nbx) % (uint32_t)nby);
+    const int bz = (int)(bid / ((uint32_t)nbx * (uint32_t)nby));
+    if (tid < NROW) {
+        const int bj = by * BY + tid % BY, bk = bz * BZ + tid / BY;
+        const int bi0 = bx * BX, bi1 = min(bi0 + BX, so.bg.b[0]);
+        uint32_t lo = 0, len = 0;
+        if (bj < so.bg.b[1] && bk < so.bg.b[2]) {
+            const uint32_t row = (uint32_t)so.bg.b[0] * ((uint32_t)bj + (uint32_t)so.bg.b[1] * (uint32_t)bk);
+            lo = so.start[row + bi0];
+            len = so.start[row + bi1] - lo;
+        }
+        s_r0[tid] = lo;
+        s_rs[tid + 1] = len;
+    }
+    for (int e = tid; e < 6 * TN; e += kBlock) tile[e] = 0.0f;
+    __syncthreads();
+    if (tid == 0) {
+        s_rs[0] = 0;
+        for (int r = 0; r < NROW; r++) s_rs[r + 1] += s_rs[r];
+    }
+    __syncthreads();
+    const uint32_t total = s_rs[NROW];
+    if (total == 0) return;   // uniform across the work-group
+    // tile origin in face-index space: bin b holds cell b - 1; a particle in
+    // cell c touches faces c - 1 .. c + 1 per axis, stages move up to M cells
+    const int ox = bx * BX - 2 - M, oy = by * BY - 2 - M, oz = bz * BZ - 2 - M;
+    auto tile_sink = [&](int arr, int i, int j, int k, uint32_t o, float v) {
+        const int lx = i - ox, ly = j - oy, lz = k - oz;
+        if ((unsigned)lx < (unsigned)TX && (unsigned)ly < (unsigned)TY && (unsigned)lz < (unsigned)TZ)
+            atomicAdd(&tile[arr * TN + lx + TX * (ly + TY * lz)], v);
+        else
+            unsafeAtomicAdd((arr < 3 ? a.lg[arr] : a.ln[arr - 3]) + o, v);
+    };
+    for (uint32_t it = tid; it < total; it += kBlock) {
+        int r = 0;
+        while (r + 1 < NROW && s_rs[r + 1] <= it) r++;
+        const uint32_t q = s_r0[r] + (it - s_rs[r]);
+        const uint32_t p = so.perm[q];
+        float x0[3], ax[3], av[3], lx[3];
+        for (int c = 0; c < 3; c++) {
+            x0[c] = so.xs[c][q];
+            ax[c] = a.ax[c][p];
+            av[c] = a.av[c][p];
+        }
+        adj_particle(a, x0, ax, av, lx, tile_sink);
+        finish(q, lx, av);
+    }
+    __syncthreads();
+    for (int e = tid; e < 6 * TN; e += kBlock) {
+        const float v = tile[e];
+        if (v == 0.0f) continue;
+        const int arr = e / TN, l = e - arr * TN;
+        const int c = arr % 3;
+        const int i = ox + l % TX, j = oy + (l / TX) % TY, k = oz + l / (TX * TY);
+        // only in-range corners were ever added
+        const uint32_t w = (uint32_t)(a.mac.ni + (c == 0)), h = (uint32_t)(a.mac.nj + (c == 1));
+        const uint32_t o = (uint32_t)i + w * ((uint32_t)j + h * (uint32_t)k);
+        float *dst = c == 0 ? (arr < 3 ? a.lg[0] : a.ln[0])
+                   : c == 1 ? (arr < 3 ? a.lg[1] : a.ln[1]) : (arr < 3 ? a.lg[2] : a.ln[2]);
+        unsafeAtomicAdd(dst + o, v);
     }
 }
 
@@ -292,14 +426,23 @@ __global__ __launch_bounds__(kBlock) void k_adj_p2g_scalar(ScalarAdjoint a) {
 
 }  // namespace
 
-void flip_step_adjoint(const StepAdjoint &a, hipStream_t s) {
+void flip_step_adjoint(const StepAdjoint &a, const AdjointSort *so, hipStream_t s) {
     for (int c = 0; c < 3; c++) {
         const int64_t nf = (int64_t)(a.mac.ni + (c == 0)) * (a.mac.nj + (c == 1)) * (a.mac.nk + (c == 2));
         fill_f32(a.lg[c], 0.0f, nf, s);
         fill_f32(a.ln[c], 0.0f, nf, s);
     }
     if (a.n <= 0) return;
-    hipLaunchKernelGGL(k_adj_advect, dim3(blocks_for(a.n)), dim3(kBlock), 0, s, a);
+    if (so) {
+        const int nbx = (so->bg.b[0] + abrick::BX - 1) / abrick::BX;
+        const int nby = (so->bg.b[1] + abrick::BY - 1) / abrick::BY;
+        const int nbz = (so->bg.b[2] + abrick::BZ - 1) / abrick::BZ;
+        const uint32_t nbrick = (uint32_t)nbx * (uint32_t)nby * (uint32_t)nbz;
+        hipLaunchKernelGGL(k_adj_advect_brick, dim3(nbrick + abrick::NEXTRA), dim3(kBlock), 0, s,
+                           a, *so, nbx, nby, nbrick);
+    } else {
+        hipLaunchKernelGGL(k_adj_advect, dim3(blocks_for(a.n)), dim3(kBlock), 0, s, a);
+    }
     GFSX_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_adj_p2g_step, dim3(blocks_for(a.n)), dim3(kBlock), 0, s, a);
     GFSX_LAUNCH_CHECK();

@@ -57,6 +57,7 @@ struct gfsx_ctx {
     size_t bytes[NSLOT] = {};
     int p2g_variant = 0;
     int g2p_variant = 0;
+    int adjoint_variant = 0;
     bool timing = false;
     std::vector<std::array<hipEvent_t, 4>> events;
     size_t events_used = 0;
@@ -336,8 +337,10 @@ gfsx::SceneDesc scene_desc(const gfsx_scene *sc, const gfsx_step_params *pr, con
     return d;
 }
 
-void flip_step_impl(gfsx_ctx *ctx, const gfsx_scene *scenes, const gfsx_step_params *params,
-                    int nscenes, const char *who) {
+// Runs the step; returns the scenes as laid out (bin grids, offsets).
+std::vector<gfsx::SceneDesc> flip_step_impl(gfsx_ctx *ctx, const gfsx_scene *scenes,
+                                            const gfsx_step_params *params, int nscenes,
+                                            const char *who) {
     gfsx::StepBatch s;
     s.scenes.resize(nscenes);
     for (int i = 0; i < nscenes; i++) {
@@ -384,6 +387,7 @@ void flip_step_impl(gfsx_ctx *ctx, const gfsx_scene *scenes, const gfsx_step_par
     // P2G brick flags, one byte per 4x4x4-cell brick of every scene
     sb.brick_flag = (uint8_t *)ctx->get(S_BFLAG, (size_t)s.tot.np2g);
     gfsx::flip_step(s, sb, ctx->stream, ctx->timing ? ctx->next_events() : nullptr);
+    return s.scenes;
 }
 
 void copy_in(void *dst, const void *src, size_t bytes, hipStream_t s) {
@@ -705,7 +709,12 @@ void gfsx_flip_step_backward(gfsx_ctx *ctx, const gfsx_scene *sc,
         f.vz_out = vo + 2 * n;
         f.id_in = nullptr;
         f.id_out = nullptr;
-        flip_step_impl(ctx, &f, pr, 1, who);
+        const std::vector<gfsx::SceneDesc> laid = flip_step_impl(ctx, &f, pr, 1, who);
+        gfsx::AdjointSort so;
+        so.start = (const uint32_t *)ctx->buf[S_START];
+        so.perm = (const uint32_t *)ctx->buf[S_PERM];
+        for (int c = 0; c < 3; c++) so.xs[c] = (const float *)ctx->buf[S_XS0 + c];
+        so.bg = laid[0].bg;
         gfsx::StepAdjoint a;
         std::memset(&a, 0, sizeof a);
         a.n = n;
@@ -730,7 +739,7 @@ void gfsx_flip_step_backward(gfsx_ctx *ctx, const gfsx_scene *sc,
         a.h_sixth = (float)(pr->dt / 6.0f);
         a.half_dx = (float)(0.5 * sc->dx);
         a.inv_r2 = (float)(1.0 / (sc->dx * sc->dx));
-        gfsx::flip_step_adjoint(a, ctx->stream);
+        gfsx::flip_step_adjoint(a, ctx->adjoint_variant == 0 ? &so : nullptr, ctx->stream);
     });
 }
 
@@ -743,6 +752,9 @@ void gfsx_set_option(gfsx_ctx *ctx, const char *name, int value, int *err) {
         } else if (std::strcmp(name, "g2p_variant") == 0) {
             require(value >= 0 && value <= 1, "gfsx_set_option: g2p_variant must be 0..1");
             ctx->g2p_variant = value;
+        } else if (std::strcmp(name, "adjoint_variant") == 0) {
+            require(value >= 0 && value <= 1, "gfsx_set_option: adjoint_variant must be 0..1");
+            ctx->adjoint_variant = value;
         } else {
             throw Error(std::string("gfsx_set_option: unknown option ") + name);
         }
@@ -755,6 +767,7 @@ int gfsx_get_option(gfsx_ctx *ctx, const char *name, int *err) {
         require(name != nullptr, "gfsx_get_option: null name");
         if (std::strcmp(name, "p2g_variant") == 0) v = ctx->p2g_variant;
         else if (std::strcmp(name, "g2p_variant") == 0) v = ctx->g2p_variant;
+        else if (std::strcmp(name, "adjoint_variant") == 0) v = ctx->adjoint_variant;
         else throw Error(std::string("gfsx_get_option: unknown option ") + name);
     });
     return v;

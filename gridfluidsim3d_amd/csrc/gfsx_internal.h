@@ -212,13 +212,25 @@ struct StepAdjoint {
     float inv_r2;                  // (float)(1/dx^2): P2G radius R = dx
 };
 
+// The cell sort of the forward step that the adjoint recomputes: bin
+// starts, the permutation (sorted slot -> particle) and the positions in
+// sorted order, over the scene's bin grid.
+struct AdjointSort {
+    const uint32_t *start, *perm;
+    const float *xs[3];
+    BinGrid bg;
+};
+
 // ---- launches (implemented in gfsx_kernels.hip) ------------------------
 
 void fill_u32(uint32_t *p, uint32_t value, int64_t count, hipStream_t s);
 void fill_f32(float *p, float value, int64_t count, hipStream_t s);
 void fill_u8(uint8_t *p, uint8_t value, int64_t count, hipStream_t s);
 // gfsx_adjoint.hip
-void flip_step_adjoint(const StepAdjoint &a, hipStream_t s);
+// so == nullptr selects one thread per particle in input order with every
+// grid adjoint added straight to global memory; otherwise bricks of the
+// sorted particles accumulate in LDS first.
+void flip_step_adjoint(const StepAdjoint &a, const AdjointSort *so, hipStream_t s);
 void p2g_adjoint(const float *pos, const float *val, int64_t n, double radius,
                  const float off[3], double dx, int ni, int nj, int nk, const float *gS,
                  const float *gW, float *gpos, float *gval, hipStream_t s);

@@ -185,7 +185,13 @@ void gfsx_flip_step_batched(gfsx_ctx *ctx, const gfsx_scene *scenes, int nscenes
  *                  2 = brick-staged, candidates range by range,
  *                  3 = brick-staged, candidates as one flattened stream.
  *   "g2p_variant": 0 = grid boxes staged in LDS per brick (default),
- *                  1 = direct sampling from global memory. */
+ *                  1 = direct sampling from global memory.
+ *   "adjoint_variant" (gfsx_flip_step_backward; results equal up to the
+ *                  order of float additions):
+ *                  0 = bricks of sorted particles accumulate the grid
+ *                      adjoints in LDS, then add them once (default),
+ *                  1 = one thread per particle, every grid adjoint added
+ *                      to global memory. */
 void gfsx_set_option(gfsx_ctx *ctx, const char *name, int value, int *err);
 int gfsx_get_option(gfsx_ctx *ctx, const char *name, int *err);
 

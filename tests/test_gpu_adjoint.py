@@ -62,6 +62,57 @@ def test_step_backward_matches_dense(ctx, seed):
     assert ex < STEP_TOL and ev < STEP_TOL
 
 
+def test_step_backward_fast_and_outside_particles(ctx):
+    """Stages that move several cells (the brick kernel's global fallback)
+    and particles outside the grid, beyond the bin grid's pad layer (its
+    overflow bin), against the dense reference."""
+    dims, dx, dt = (10, 9, 8), 0.1, 1.0 / 60.0
+    pos, vel = small_scene(47, dims, dx, amp=25.0)
+    rng = np.random.default_rng(47)
+    far = (rng.uniform(-0.4, 1.4, size=(40, 3)) * np.array(dims) * dx).astype(np.float32)
+    pos = np.concatenate([pos, far])
+    vel = np.concatenate([vel, rng.normal(size=(40, 3)).astype(np.float32) * 5])
+    a = rng.normal(size=pos.shape)
+    b = rng.normal(size=pos.shape)
+    for variant in (0, 1):
+        ctx.set_option("adjoint_variant", variant)
+        try:
+            gx, gv = ctx.flip_step_backward(dev_soa(pos), dev_soa(vel), dev_soa(a), dev_soa(b),
+                                            dims, dx, dt)
+        finally:
+            ctx.set_option("adjoint_variant", 0)
+        x = torch.tensor(pos, dtype=F64, requires_grad=True)
+        v = torch.tensor(vel, dtype=F64, requires_grad=True)
+        x1, v1 = dfl.step(x, v, dims, dx, dt)
+        L = (torch.tensor(a) * x1).sum() + (torch.tensor(b) * v1).sum()
+        rx, rv = torch.autograd.grad(L, (x, v))
+        ex = nrel(gx.cpu().numpy().T, rx.numpy())
+        ev = nrel(gv.cpu().numpy().T, rv.numpy())
+        print("variant %d: dL/dx err %.2e, dL/dv err %.2e" % (variant, ex, ev))
+        assert ex < STEP_TOL and ev < STEP_TOL
+
+
+def test_adjoint_variants_agree(ctx):
+    """The LDS-brick adjoint and the global-atomic adjoint on a 40^3 scene
+    of random blobs: equal up to the order of FP32 additions."""
+    from gridfluidsim3d_amd import scenes
+    sc = scenes.random_blobs(48, (40, 36, 44), 1.0 / 40.0, ppc=6, amplitude=2.0)
+    rng = np.random.default_rng(48)
+    a = dev_soa(rng.normal(size=sc.pos.shape))
+    b = dev_soa(rng.normal(size=sc.pos.shape))
+    x, v = dev_soa(sc.pos), dev_soa(sc.vel)
+    out = []
+    for variant in (0, 1):
+        ctx.set_option("adjoint_variant", variant)
+        try:
+            out.append([t.cpu().numpy() for t in ctx.flip_step_backward(x, v, a, b, sc.dims,
+                                                                        sc.dx, sc.dt)])
+        finally:
+            ctx.set_option("adjoint_variant", 0)
+    assert nrel(out[0][0], out[1][0]) < 1e-5
+    assert nrel(out[0][1], out[1][1]) < 1e-5
+
+
 def test_step_backward_position_only_and_velocity_only(ctx):
     """Adjoints of x' alone and of v' alone, each against the dense
     reference (covers the RK4 path and the PIC/FLIP path separately)."""

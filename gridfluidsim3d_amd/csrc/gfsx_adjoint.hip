@@ -13,11 +13,12 @@
 // with I the MAC trilinear sample.  Given dL/dx' and dL/dv' the adjoint
 // gives dL/dx and dL/dv in three passes over the particles:
 //
-//   1. k_adj_advect: recompute the RK4 stage positions from (x, g), walk the
-//      stages backwards, and scatter the sample adjoints onto the faces
-//      (dL/dg from the stages and the PIC/FLIP blend, dL/dn from the FLIP
-//      difference).  The scatter uses float atomics: the sums are exact up
-//      to the order of FP32 additions, which may change between runs.
+//   1. k_adj_advect_brick (or k_adj_advect): recompute the RK4 stage
+//      positions from (x, g), walk the stages backwards, and scatter the
+//      sample adjoints onto the faces (dL/dg from the stages and the
+//      PIC/FLIP blend, dL/dn from the FLIP difference).  The brick kernel
+//      sums in 64-bit fixed point (deterministic); k_adj_advect adds floats
+//      atomically (the order of the FP32 additions may change between runs).
 //   2. k_adj_p2g_step: a gather over each particle's faces through the
 //      quotient rule of the normalisation,
 //        dL/dv_p += K_pf lam_f / W_f,
@@ -126,21 +127,22 @@ __device__ __forceinline__ void mac_adjoint(const StepAdjoint &a, const float *c
 }
 
 // The RK4 + PIC/FLIP adjoint of one particle: from x0 and the output
-// adjoints ax (of x') and av (of v'), returns lx (of x0) and scatters the
-// grid adjoints through sink (arrays 0-2: dL/dg, 3-5: dL/dn).
-template <class Sink>
+// adjoints ax (of x') and av (of v'), returns lx (of x0).  The policy
+// samples the grid after the body force (pol.sample(q, out)) and
+// back-propagates a sample (pol.adjoint(base, q, lam, lq): base 0 for the
+// grid after the body force, whose adjoint is dL/dg, 3 for the normalised
+// P2G result, whose adjoint is dL/dn).
+template <class Policy>
 __device__ __forceinline__ void adj_particle(const StepAdjoint &a, const float x0[3],
                                              const float ax[3], const float av[3], float lx[3],
-                                             Sink &&sink) {
+                                             Policy &&pol) {
     // forward stages, as the fused step computes them (particleadvector.cpp:1045-1054)
-    MacGrid g = a.mac;
-    for (int c = 0; c < 3; c++) g.f[c] = a.g[c];
-    float k1[3], k2[3], k3[3], q2[3], q3[3], q4[3], unused[3];
-    sample_mac<0, false>(g, nullptr, nullptr, nullptr, x0[0], x0[1], x0[2], k1, unused);
+    float k1[3], k2[3], k3[3], q2[3], q3[3], q4[3];
+    pol.sample(x0, k1);
     for (int c = 0; c < 3; c++) q2[c] = x0[c] + a.h_half * k1[c];
-    sample_mac<0, false>(g, nullptr, nullptr, nullptr, q2[0], q2[1], q2[2], k2, unused);
+    pol.sample(q2, k2);
     for (int c = 0; c < 3; c++) q3[c] = x0[c] + a.h_half * k2[c];
-    sample_mac<0, false>(g, nullptr, nullptr, nullptr, q3[0], q3[1], q3[2], k3, unused);
+    pol.sample(q3, k3);
     for (int c = 0; c < 3; c++) q4[c] = x0[c] + a.h_full * k3[c];
     // x' = x + sixth*(((k1 + 2k2) + 2k3) + k4), walked backwards
     float lk[3], lq[3];
@@ -149,19 +151,19 @@ __device__ __forceinline__ void adj_particle(const StepAdjoint &a, const float x
         lk[c] = a.h_sixth * ax[c];               // k4
         lq[c] = 0.0f;
     }
-    mac_adjoint(a, a.g, 0, q4, lk, lq, sink);
+    pol.adjoint(0, q4, lk, lq);
     for (int c = 0; c < 3; c++) {
         lx[c] += lq[c];
         lk[c] = 2.0f * a.h_sixth * ax[c] + a.h_full * lq[c];    // k3
         lq[c] = 0.0f;
     }
-    mac_adjoint(a, a.g, 0, q3, lk, lq, sink);
+    pol.adjoint(0, q3, lk, lq);
     for (int c = 0; c < 3; c++) {
         lx[c] += lq[c];
         lk[c] = 2.0f * a.h_sixth * ax[c] + a.h_half * lq[c];    // k2
         lq[c] = 0.0f;
     }
-    mac_adjoint(a, a.g, 0, q2, lk, lq, sink);
+    pol.adjoint(0, q2, lk, lq);
     // k1 and the PIC/FLIP samples share the point x: I(g, x) enters v' with
     // r_pic + r_flip, I(n, x) with -r_flip
     float lgx[3], lnx[3];
@@ -170,9 +172,26 @@ __device__ __forceinline__ void adj_particle(const StepAdjoint &a, const float x
         lgx[c] = a.h_sixth * ax[c] + a.h_half * lq[c] + (a.r_pic + a.r_flip) * av[c];
         lnx[c] = -a.r_flip * av[c];
     }
-    mac_adjoint(a, a.g, 0, x0, lgx, lx, sink);
-    mac_adjoint(a, a.old, 3, x0, lnx, lx, sink);
+    pol.adjoint(0, x0, lgx, lx);
+    pol.adjoint(3, x0, lnx, lx);
 }
+
+// Samples and adjoints straight from and to global memory.
+struct GlobalPolicy {
+    const StepAdjoint &a;
+    __device__ void sample(const float q[3], float out[3]) const {
+        MacGrid g = a.mac;
+        for (int c = 0; c < 3; c++) g.f[c] = a.g[c];
+        float unused[3];
+        sample_mac<0, false>(g, nullptr, nullptr, nullptr, q[0], q[1], q[2], out, unused);
+    }
+    __device__ void adjoint(int base, const float q[3], const float lam[3], float lq[3]) const {
+        mac_adjoint(a, base == 0 ? a.g : a.old, base, q, lam, lq,
+                    [&](int arr, int, int, int, uint32_t o, float v) {
+                        unsafeAtomicAdd((arr < 3 ? a.lg[arr] : a.ln[arr - 3]) + o, v);
+                    });
+    }
+};
 
 // One thread per particle in input order; grid adjoints straight to global
 // memory.
@@ -185,22 +204,32 @@ __global__ __launch_bounds__(kBlock) void k_adj_advect(StepAdjoint a) {
         ax[c] = a.ax[c][p];
         av[c] = a.av[c][p];
     }
-    adj_particle(a, x0, ax, av, lx, [&](int arr, int, int, int, uint32_t o, float v) {
-        unsafeAtomicAdd((arr < 3 ? a.lg[arr] : a.ln[arr - 3]) + o, v);
-    });
+    adj_particle(a, x0, ax, av, lx, GlobalPolicy{a});
     for (int c = 0; c < 3; c++) {
         a.gx[c][p] = lx[c];
         a.gv[c][p] = a.r_flip * av[c];
     }
 }
 
-// Bricks of BX x BY x BY bins of the forward sort, one work-group each.  A
-// brick's particles (BY*BZ runs of sorted slots, one per bin row) add their
-// grid adjoints into an LDS tile of every face index within M cells of the
-// brick's trilinear footprint, which is then added to global memory once per
-// face.  Samples that leave the tile (a stage moved more than M cells) go to
-// global memory directly.  Work-groups past the bricks take the particles of
-// the overflow bin (outside the bin grid) with global adjoints.
+// Bricks of BX x BY x BZ bins of the forward sort, one work-group each, in
+// two passes over the brick's particles (BY*BZ runs of sorted slots, one
+// per bin row).  A tile of every face index within M cells of the brick's
+// trilinear footprint holds the grid after the body force and the
+// normalised P2G result (0 for faces outside the arrays, which is what an
+// out-of-range sample reads).  Samples whose corners leave the tile (a stage
+// moved more than M cells) use global memory.  Work-groups past the bricks
+// take the particles of the overflow bin (outside the bin grid).
+//
+//   pass 1 (SCATTER false): every sample's adjoint is computed, dL/dx and
+//     dL/dv are written, and the largest |sample adjoint| is reduced into
+//     maxbits.
+//   pass 2 (SCATTER true): the samples are recomputed and lam * weight is
+//     added as a 64-bit fixed-point integer, scale 2^61 / (5 n max), into an
+//     LDS tile (ds_add_u64; LDS float atomics run at a small fraction of
+//     the integer rate on gfx950) and then once per face into global
+//     accumulators.  Integer sums do not depend on the order of the adds,
+//     so the grid adjoints are the same on every run; no face can overflow:
+//     it receives at most 5n terms of magnitude at most 2^61 / (5n).
 namespace abrick {
 constexpr int BX = 8, BY = 4, BZ = 4;
 constexpr int M = 1;
@@ -210,22 +239,70 @@ constexpr int NROW = BY * BZ;
 constexpr int NEXTRA = 16;
 }  // namespace abrick
 
+// fixed-point scale for adjoint terms of magnitude at most maxabs, n particles
+__device__ __forceinline__ double fixed_scale(uint32_t maxbits, int64_t n) {
+    const float m = __uint_as_float(maxbits);
+    if (!(m > 0.0f)) return 1.0;
+    return 2305843009213693952.0 / ((double)m * 5.0 * (double)(n > 0 ? n : 1));   // 2^61
+}
+
+__device__ __forceinline__ unsigned long long to_fixed(float v, double scale) {
+    return (unsigned long long)__double2ll_rn((double)v * scale);
+}
+
+template <bool SCATTER>
 __global__ __launch_bounds__(kBlock) void k_adj_advect_brick(StepAdjoint a, AdjointSort so,
                                                              int nbx, int nby, uint32_t nbrick) {
     using namespace abrick;
-    __shared__ float tile[6 * TN];
+    __shared__ float gt[6 * TN];                 // g (0-2) and n (3-5) on the tile
+    __shared__ unsigned long long at[SCATTER ? 6 * TN : 1];   // fixed-point dL/dg, dL/dn
     __shared__ uint32_t s_r0[NROW];
     __shared__ uint32_t s_rs[NROW + 1];
+    __shared__ float s_max[kBlock / 64];
     const int tid = threadIdx.x;
     const uint32_t bid = blockIdx.x;
-    auto global_sink = [&](int arr, int, int, int, uint32_t o, float v) {
-        unsafeAtomicAdd((arr < 3 ? a.lg[arr] : a.ln[arr - 3]) + o, v);
+    const double scale = SCATTER ? fixed_scale(*so.maxbits, a.n) : 0.0;
+    float mx = 0.0f;   // largest |sample adjoint| of this thread (pass 1)
+    auto fixed_sink = [&](int arr, int, int, int, uint32_t o, float v) {
+        atomicAdd(so.fx + (arr < 3 ? a.lg[arr] - a.lg[0] : a.ln[arr - 3] - a.lg[0]) + o,
+                  to_fixed(v, scale));
     };
+    // global path of one sample (outside the tile, or an overflow particle)
+    auto global_adjoint = [&](int base, const float q[3], const float lam[3], float lq[3]) {
+        if (SCATTER)
+            mac_adjoint(a, base == 0 ? a.g : a.old, base, q, lam, lq, fixed_sink);
+        else {
+            for (int c = 0; c < 3; c++) mx = fmaxf(mx, fabsf(lam[c]));
+            mac_adjoint(a, base == 0 ? a.g : a.old, base, q, lam, lq,
+                        [](int, int, int, int, uint32_t, float) {});
+        }
+    };
+    struct GlobalFixed {
+        const GlobalPolicy &gp;
+        decltype(global_adjoint) &ga;
+        __device__ void sample(const float q[3], float out[3]) const { gp.sample(q, out); }
+        __device__ void adjoint(int base, const float q[3], const float lam[3], float lq[3]) const {
+            ga(base, q, lam, lq);
+        }
+    };
+    const GlobalPolicy gpol{a};
+    const GlobalFixed gfix{gpol, global_adjoint};
     auto finish = [&](uint32_t q, const float lx[3], const float av[3]) {
+        if (SCATTER) return;
         const uint32_t p = so.perm[q];
         for (int c = 0; c < 3; c++) {
             a.gx[c][p] = lx[c];
             a.gv[c][p] = a.r_flip * av[c];
+        }
+    };
+    auto block_max = [&]() {
+        float m = mx;
+        for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+        if ((tid & 63) == 0) s_max[tid >> 6] = m;
+        __syncthreads();
+        if (tid == 0) {
+            for (int w = 1; w < kBlock / 64; w++) m = fmaxf(m, s_max[w]);
+            if (m > 0.0f) atomicMax(so.maxbits, __float_as_uint(m));
         }
     };
     if (bid >= nbrick) {
@@ -239,9 +316,10 @@ __global__ __launch_bounds__(kBlock) void k_adj_advect_brick(StepAdjoint a, Adjo
                 ax[c] = a.ax[c][p];
                 av[c] = a.av[c][p];
             }
-            adj_particle(a, x0, ax, av, lx, global_sink);
+            adj_particle(a, x0, ax, av, lx, gfix);
             finish(q, lx, av);
         }
+        if (!SCATTER) block_max();
         return;
     }
     const int bx = (int)(bid % (uint32_t)nbx);
@@ -259,7 +337,6 @@ __global__ __launch_bounds__(kBlock) void k_adj_advect_brick(StepAdjoint a, Adjo
         s_r0[tid] = lo;
         s_rs[tid + 1] = len;
     }
-    for (int e = tid; e < 6 * TN; e += kBlock) tile[e] = 0.0f;
     __syncthreads();
     if (tid == 0) {
         s_rs[0] = 0;
@@ -271,15 +348,125 @@ __global__ __launch_bounds__(kBlock) void k_adj_advect_brick(StepAdjoint a, Adjo
     // tile origin in face-index space: bin b holds cell b - 1; a particle in
     // cell c touches faces c - 1 .. c + 1 per axis, stages move up to M cells
     const int ox = bx * BX - 2 - M, oy = by * BY - 2 - M, oz = bz * BZ - 2 - M;
-    auto tile_sink = [&](int arr, int i, int j, int k, uint32_t o, float v) {
-        const int lx = i - ox, ly = j - oy, lz = k - oz;
-        if ((unsigned)lx < (unsigned)TX && (unsigned)ly < (unsigned)TY && (unsigned)lz < (unsigned)TZ)
-            atomicAdd(&tile[arr * TN + lx + TX * (ly + TY * lz)], v);
-        else
-            unsafeAtomicAdd((arr < 3 ? a.lg[arr] : a.ln[arr - 3]) + o, v);
+    const int ni = a.mac.ni, nj = a.mac.nj, nk = a.mac.nk;
+    for (int e = tid; e < 6 * TN; e += kBlock) {
+        const int arr = e / TN, l = e - arr * TN;
+        const int c = arr % 3;
+        const int i = ox + l % TX, j = oy + (l / TX) % TY, k = oz + l / (TX * TY);
+        const int w = ni + (c == 0), h = nj + (c == 1), d = nk + (c == 2);
+        float v = 0.0f;
+        if ((unsigned)i < (unsigned)w && (unsigned)j < (unsigned)h && (unsigned)k < (unsigned)d) {
+            const uint32_t o = (uint32_t)i + (uint32_t)w * ((uint32_t)j + (uint32_t)h * (uint32_t)k);
+            const float *src = c == 0 ? (arr < 3 ? a.g[0] : a.old[0])
+                             : c == 1 ? (arr < 3 ? a.g[1] : a.old[1]) : (arr < 3 ? a.g[2] : a.old[2]);
+            v = src[o];
+        }
+        gt[e] = v;
+        if (SCATTER) at[e] = 0ull;
+    }
+    __syncthreads();
+
+    // local flat index of lower corner (i,j,k), or -1 when a corner leaves the tile
+    auto local = [&](int i, int j, int k) -> int {
+        const int li = i - ox, lj = j - oy, lk = k - oz;
+        return ((unsigned)li < (unsigned)(TX - 1) && (unsigned)lj < (unsigned)(TY - 1) &&
+                (unsigned)lk < (unsigned)(TZ - 1)) ? li + TX * (lj + TY * lk) : -1;
     };
-    for (uint32_t it = tid; it < total; it += kBlock) {
-        int r = 0;
+    auto in_grid = [&](const float q[3]) {
+        return q[0] >= 0.0f && q[1] >= 0.0f && q[2] >= 0.0f && q[0] < a.mac.limf[0] &&
+               q[1] < a.mac.limf[1] && q[2] < a.mac.limf[2];
+    };
+    // trilinear value from the tile, in the order of the forward sampler
+    auto tile_value = [&](int arr, int l, float tx, float ty, float tz) {
+        const float *f = gt + arr * TN + l;
+        const float v000 = f[0], v100 = f[1], v010 = f[TX], v110 = f[TX + 1];
+        const float v001 = f[TX * TY], v101 = f[TX * TY + 1];
+        const float v011 = f[TX * TY + TX], v111 = f[TX * TY + TX + 1];
+        const float c00 = fmaf(tx, v100 - v000, v000);
+        const float c10 = fmaf(tx, v110 - v010, v010);
+        const float c01 = fmaf(tx, v101 - v001, v001);
+        const float c11 = fmaf(tx, v111 - v011, v011);
+        const float c0 = fmaf(ty, c10 - c00, c00);
+        const float c1 = fmaf(ty, c11 - c01, c01);
+        return fmaf(tz, c1 - c0, c0);
+    };
+    auto tile_adjoint = [&](int arr, int l, float tx, float ty, float tz, float lam, float dt[3]) {
+        if (lam == 0.0f) return;
+        const float *f = gt + arr * TN + l;
+        const float v000 = f[0], v100 = f[1], v010 = f[TX], v110 = f[TX + 1];
+        const float v001 = f[TX * TY], v101 = f[TX * TY + 1];
+        const float v011 = f[TX * TY + TX], v111 = f[TX * TY + TX + 1];
+        const float sx = 1.0f - tx, sy = 1.0f - ty, sz = 1.0f - tz;
+        dt[0] += lam * (sy * sz * (v100 - v000) + ty * sz * (v110 - v010) +
+                        sy * tz * (v101 - v001) + ty * tz * (v111 - v011));
+        dt[1] += lam * (sx * sz * (v010 - v000) + tx * sz * (v110 - v100) +
+                        sx * tz * (v011 - v001) + tx * tz * (v111 - v101));
+        dt[2] += lam * (sx * sy * (v001 - v000) + tx * sy * (v101 - v100) +
+                        sx * ty * (v011 - v010) + tx * ty * (v111 - v110));
+        if (!SCATTER) {
+            mx = fmaxf(mx, fabsf(lam));
+            return;
+        }
+        unsigned long long *t = at + arr * TN + l;
+        atomicAdd(t, to_fixed(lam * (sx * sy * sz), scale));
+        atomicAdd(t + 1, to_fixed(lam * (tx * sy * sz), scale));
+        atomicAdd(t + TX, to_fixed(lam * (sx * ty * sz), scale));
+        atomicAdd(t + TX + 1, to_fixed(lam * (tx * ty * sz), scale));
+        atomicAdd(t + TX * TY, to_fixed(lam * (sx * sy * tz), scale));
+        atomicAdd(t + TX * TY + 1, to_fixed(lam * (tx * sy * tz), scale));
+        atomicAdd(t + TX * TY + TX, to_fixed(lam * (sx * ty * tz), scale));
+        atomicAdd(t + TX * TY + TX + 1, to_fixed(lam * (tx * ty * tz), scale));
+    };
+    struct TilePolicy {
+        const StepAdjoint &a;
+        const GlobalFixed &gf;
+        decltype(local) &loc;
+        decltype(in_grid) &ing;
+        decltype(tile_value) &tv;
+        decltype(tile_adjoint) &tadj;
+        __device__ void sample(const float q[3], float out[3]) const {
+            if (!ing(q)) {
+                out[0] = out[1] = out[2] = 0.0f;
+                return;
+            }
+            const AxisCoords cx = axis_coords(q[0], a.mac.inv_dx);
+            const AxisCoords cy = axis_coords(q[1], a.mac.inv_dx);
+            const AxisCoords cz = axis_coords(q[2], a.mac.inv_dx);
+            const int l0 = loc(cx.ia, cy.ib, cz.ib), l1 = loc(cx.ib, cy.ia, cz.ib);
+            const int l2 = loc(cx.ib, cy.ib, cz.ia);
+            if (l0 < 0 || l1 < 0 || l2 < 0) {
+                gf.sample(q, out);
+                return;
+            }
+            out[0] = tv(0, l0, cx.ta, cy.tb, cz.tb);
+            out[1] = tv(1, l1, cx.tb, cy.ta, cz.tb);
+            out[2] = tv(2, l2, cx.tb, cy.tb, cz.ta);
+        }
+        __device__ void adjoint(int base, const float q[3], const float lam[3], float lq[3]) const {
+            if (!ing(q)) return;
+            const AxisCoords cx = axis_coords(q[0], a.mac.inv_dx);
+            const AxisCoords cy = axis_coords(q[1], a.mac.inv_dx);
+            const AxisCoords cz = axis_coords(q[2], a.mac.inv_dx);
+            const int l0 = loc(cx.ia, cy.ib, cz.ib), l1 = loc(cx.ib, cy.ia, cz.ib);
+            const int l2 = loc(cx.ib, cy.ib, cz.ia);
+            if (l0 < 0 || l1 < 0 || l2 < 0) {
+                gf.adjoint(base, q, lam, lq);
+                return;
+            }
+            float dt[3] = {0.0f, 0.0f, 0.0f};
+            tadj(base + 0, l0, cx.ta, cy.tb, cz.tb, lam[0], dt);
+            tadj(base + 1, l1, cx.tb, cy.ta, cz.tb, lam[1], dt);
+            tadj(base + 2, l2, cx.tb, cy.tb, cz.ta, lam[2], dt);
+            const float inv_dx = (float)a.mac.inv_dx;
+            for (int c = 0; c < 3; c++) lq[c] += dt[c] * inv_dx;
+        }
+    };
+    const TilePolicy tpol{a, gfix, local, in_grid, tile_value, tile_adjoint};
+    // blocked mapping: thread t takes items [t*per, (t+1)*per), so the lanes
+    // of a wave sit in different cells (fewer same-address LDS atomics)
+    const uint32_t per = (total + kBlock - 1) / kBlock;
+    int r = 0;
+    for (uint32_t it = tid * per, end = min(total, (uint32_t)(tid + 1) * per); it < end; it++) {
         while (r + 1 < NROW && s_rs[r + 1] <= it) r++;
         const uint32_t q = s_r0[r] + (it - s_rs[r]);
         const uint32_t p = so.perm[q];
@@ -289,23 +476,38 @@ __global__ __launch_bounds__(kBlock) void k_adj_advect_brick(StepAdjoint a, Adjo
             ax[c] = a.ax[c][p];
             av[c] = a.av[c][p];
         }
-        adj_particle(a, x0, ax, av, lx, tile_sink);
+        adj_particle(a, x0, ax, av, lx, tpol);
         finish(q, lx, av);
+    }
+    if (!SCATTER) {
+        block_max();
+        return;
     }
     __syncthreads();
     for (int e = tid; e < 6 * TN; e += kBlock) {
-        const float v = tile[e];
-        if (v == 0.0f) continue;
+        const unsigned long long v = at[e];
+        if (v == 0ull) continue;
         const int arr = e / TN, l = e - arr * TN;
         const int c = arr % 3;
         const int i = ox + l % TX, j = oy + (l / TX) % TY, k = oz + l / (TX * TY);
-        // only in-range corners were ever added
-        const uint32_t w = (uint32_t)(a.mac.ni + (c == 0)), h = (uint32_t)(a.mac.nj + (c == 1));
-        const uint32_t o = (uint32_t)i + w * ((uint32_t)j + h * (uint32_t)k);
-        float *dst = c == 0 ? (arr < 3 ? a.lg[0] : a.ln[0])
-                   : c == 1 ? (arr < 3 ? a.lg[1] : a.ln[1]) : (arr < 3 ? a.lg[2] : a.ln[2]);
-        unsafeAtomicAdd(dst + o, v);
+        const int w = ni + (c == 0), h = nj + (c == 1), d = nk + (c == 2);
+        // tile entries outside the arrays collect nothing that exists
+        if (!((unsigned)i < (unsigned)w && (unsigned)j < (unsigned)h && (unsigned)k < (unsigned)d))
+            continue;
+        const uint32_t o = (uint32_t)i + (uint32_t)w * ((uint32_t)j + (uint32_t)h * (uint32_t)k);
+        const float *base = c == 0 ? (arr < 3 ? a.lg[0] : a.ln[0])
+                          : c == 1 ? (arr < 3 ? a.lg[1] : a.ln[1]) : (arr < 3 ? a.lg[2] : a.ln[2]);
+        atomicAdd(so.fx + (base - a.lg[0]) + o, v);
     }
+}
+
+// dL/dg, dL/dn from the fixed-point sums (one array of `count` faces).
+__global__ __launch_bounds__(kBlock) void k_fixed_to_float(const unsigned long long *fx,
+                                                           float *out, int64_t count,
+                                                           const uint32_t *maxbits, int64_t n) {
+    const int64_t f = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    if (f >= count) return;
+    out[f] = (float)((double)(long long)fx[f] / fixed_scale(*maxbits, n));
 }
 
 // Faces of component c within the kernel radius R = dx of x: the node
@@ -427,23 +629,44 @@ __global__ __launch_bounds__(kBlock) void k_adj_p2g_scalar(ScalarAdjoint a) {
 }  // namespace
 
 void flip_step_adjoint(const StepAdjoint &a, const AdjointSort *so, hipStream_t s) {
-    for (int c = 0; c < 3; c++) {
-        const int64_t nf = (int64_t)(a.mac.ni + (c == 0)) * (a.mac.nj + (c == 1)) * (a.mac.nk + (c == 2));
-        fill_f32(a.lg[c], 0.0f, nf, s);
-        fill_f32(a.ln[c], 0.0f, nf, s);
+    int64_t nf[3], tot = 0;
+    for (int c = 0; c < 3; c++)
+        tot += (nf[c] = (int64_t)(a.mac.ni + (c == 0)) * (a.mac.nj + (c == 1)) * (a.mac.nk + (c == 2)));
+    if (so) {
+        // the grid adjoints are one block: u,v,w of dL/dg, then of dL/dn
+        if (a.lg[1] != a.lg[0] + nf[0] || a.lg[2] != a.lg[1] + nf[1] ||
+            a.ln[0] != a.lg[2] + nf[2] || a.ln[1] != a.ln[0] + nf[0] ||
+            a.ln[2] != a.ln[1] + nf[1])
+            throw Error("flip_step_adjoint: grid adjoint arrays must be contiguous");
+        fill_u32((uint32_t *)so->fx, 0u, 4 * tot, s);
+        fill_u32(so->maxbits, 0u, 1, s);
+        if (a.n > 0) {
+            const int nbx = (so->bg.b[0] + abrick::BX - 1) / abrick::BX;
+            const int nby = (so->bg.b[1] + abrick::BY - 1) / abrick::BY;
+            const int nbz = (so->bg.b[2] + abrick::BZ - 1) / abrick::BZ;
+            const uint32_t nbrick = (uint32_t)nbx * (uint32_t)nby * (uint32_t)nbz;
+            hipLaunchKernelGGL(k_adj_advect_brick<false>, dim3(nbrick + abrick::NEXTRA),
+                               dim3(kBlock), 0, s, a, *so, nbx, nby, nbrick);
+            GFSX_LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_adj_advect_brick<true>, dim3(nbrick + abrick::NEXTRA),
+                               dim3(kBlock), 0, s, a, *so, nbx, nby, nbrick);
+            GFSX_LAUNCH_CHECK();
+        }
+        hipLaunchKernelGGL(k_fixed_to_float, dim3(blocks_for(2 * tot)), dim3(kBlock), 0, s,
+                           (const unsigned long long *)so->fx, a.lg[0], 2 * tot,
+                           (const uint32_t *)so->maxbits, a.n);
+        GFSX_LAUNCH_CHECK();
+    } else {
+        for (int c = 0; c < 3; c++) {
+            fill_f32(a.lg[c], 0.0f, nf[c], s);
+            fill_f32(a.ln[c], 0.0f, nf[c], s);
+        }
+        if (a.n > 0) {
+            hipLaunchKernelGGL(k_adj_advect, dim3(blocks_for(a.n)), dim3(kBlock), 0, s, a);
+            GFSX_LAUNCH_CHECK();
+        }
     }
     if (a.n <= 0) return;
-    if (so) {
-        const int nbx = (so->bg.b[0] + abrick::BX - 1) / abrick::BX;
-        const int nby = (so->bg.b[1] + abrick::BY - 1) / abrick::BY;
-        const int nbz = (so->bg.b[2] + abrick::BZ - 1) / abrick::BZ;
-        const uint32_t nbrick = (uint32_t)nbx * (uint32_t)nby * (uint32_t)nbz;
-        hipLaunchKernelGGL(k_adj_advect_brick, dim3(nbrick + abrick::NEXTRA), dim3(kBlock), 0, s,
-                           a, *so, nbx, nby, nbrick);
-    } else {
-        hipLaunchKernelGGL(k_adj_advect, dim3(blocks_for(a.n)), dim3(kBlock), 0, s, a);
-    }
-    GFSX_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_adj_p2g_step, dim3(blocks_for(a.n)), dim3(kBlock), 0, s, a);
     GFSX_LAUNCH_CHECK();
 }

@@ -37,7 +37,7 @@ enum Slot {
     S_BOUT, S_BOUTS, S_SCAN, S_OCT, S_POCT,
     S_OLD0, S_XS0, S_XS1, S_XS2, S_VS0, S_VS1, S_VS2, S_DENSE, S_DENSEN,
     S_D_POS, S_D_VAL, S_D_S, S_D_W, S_D_U, S_D_V, S_D_WW, S_D_OUT,
-    S_B_GRID, S_B_OLD, S_B_WGT, S_B_LG, S_B_LN, S_B_XO, S_B_VO,
+    S_B_GRID, S_B_OLD, S_B_WGT, S_B_LG, S_B_XO, S_B_VO, S_B_FX, S_B_MAX,
     NSLOT
 };
 
@@ -688,7 +688,9 @@ void gfsx_flip_step_backward(gfsx_ctx *ctx, const gfsx_scene *sc,
         for (int c = 0; c < 3; c++) tot += (nf[c] = mac_count(c, ni, nj, nk));
         float *grid = ctx->f32(S_B_GRID, tot), *old = ctx->f32(S_B_OLD, tot);
         float *wgt = ctx->f32(S_B_WGT, tot);
-        float *lg = ctx->f32(S_B_LG, tot), *ln = ctx->f32(S_B_LN, tot);
+        // dL/dg then dL/dn, one block (the LDS-brick adjoint converts its
+        // fixed-point sums into it in one pass)
+        float *lg = ctx->f32(S_B_LG, 2 * tot), *ln = lg + tot;
         float *xo = ctx->f32(S_B_XO, 3 * n), *vo = ctx->f32(S_B_VO, 3 * n);
         const int64_t at[3] = {0, nf[0], nf[0] + nf[1]};
         gfsx_scene f = *sc;
@@ -715,6 +717,8 @@ void gfsx_flip_step_backward(gfsx_ctx *ctx, const gfsx_scene *sc,
         so.perm = (const uint32_t *)ctx->buf[S_PERM];
         for (int c = 0; c < 3; c++) so.xs[c] = (const float *)ctx->buf[S_XS0 + c];
         so.bg = laid[0].bg;
+        so.fx = (unsigned long long *)ctx->get(S_B_FX, sizeof(unsigned long long) * 2 * tot);
+        so.maxbits = ctx->u32(S_B_MAX, 1);
         gfsx::StepAdjoint a;
         std::memset(&a, 0, sizeof a);
         a.n = n;

@@ -215,10 +215,15 @@ struct StepAdjoint {
 // The cell sort of the forward step that the adjoint recomputes: bin
 // starts, the permutation (sorted slot -> particle) and the positions in
 // sorted order, over the scene's bin grid.
+// fx: fixed-point accumulators for the grid adjoints (int64, the face
+// arrays of dL/dg u,v,w then dL/dn u,v,w back to back); maxbits: one word
+// for the largest |sample adjoint| (float bits).
 struct AdjointSort {
     const uint32_t *start, *perm;
     const float *xs[3];
     BinGrid bg;
+    unsigned long long *fx;
+    uint32_t *maxbits;
 };
 
 // ---- launches (implemented in gfsx_kernels.hip) ------------------------

@@ -186,12 +186,13 @@ void gfsx_flip_step_batched(gfsx_ctx *ctx, const gfsx_scene *scenes, int nscenes
  *                  3 = brick-staged, candidates as one flattened stream.
  *   "g2p_variant": 0 = grid boxes staged in LDS per brick (default),
  *                  1 = direct sampling from global memory.
- *   "adjoint_variant" (gfsx_flip_step_backward; results equal up to the
- *                  order of float additions):
- *                  0 = bricks of sorted particles accumulate the grid
- *                      adjoints in LDS, then add them once (default),
+ *   "adjoint_variant" (gfsx_flip_step_backward; results equal up to FP32
+ *                  rounding):
+ *                  0 = bricks of sorted particles sum the grid adjoints in
+ *                      LDS in 64-bit fixed point, then add them once per
+ *                      face (default; deterministic),
  *                  1 = one thread per particle, every grid adjoint added
- *                      to global memory. */
+ *                      to global memory with float atomics. */
 void gfsx_set_option(gfsx_ctx *ctx, const char *name, int value, int *err);
 int gfsx_get_option(gfsx_ctx *ctx, const char *name, int *err);
 
@@ -240,9 +241,10 @@ typedef struct gfsx_scene_grad {
  * through RK4 (stage positions recomputed from x_in), the PIC/FLIP blend,
  * the trilinear samples and the P2G normalisation (quotient rule:
  * dL/dS = dL/dn / W, dL/dW = -dL/dn * n / W).  The fluid mask and cell
- * indices are piecewise constant, so they contribute nothing.  The grid
- * adjoints are accumulated with float atomics, so the last bits may differ
- * run to run.  The reference has no adjoint; this differentiates the step
+ * indices are piecewise constant, so they contribute nothing.  By default
+ * the grid adjoints are summed as 64-bit fixed-point integers, so results
+ * are bit-identical run to run (adjoint_variant 1 adds floats atomically and
+ * is not).  The reference has no adjoint; this differentiates the step
  * gfsx_flip_step replaces (fluidsimulation.cpp:2526-2595, 3104-3129,
  * 3181-3209; particleadvector.cpp:1045-1054). */
 void gfsx_flip_step_backward(gfsx_ctx *ctx, const gfsx_scene *scene,

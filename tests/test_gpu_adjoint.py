@@ -113,6 +113,20 @@ def test_adjoint_variants_agree(ctx):
     assert nrel(out[0][1], out[1][1]) < 1e-5
 
 
+def test_step_backward_is_deterministic(ctx):
+    """The default adjoint sums its grid adjoints in fixed point: two runs
+    give bit-identical gradients."""
+    from gridfluidsim3d_amd import scenes
+    sc = scenes.random_blobs(49, (40, 36, 44), 1.0 / 40.0, ppc=6, amplitude=2.0)
+    rng = np.random.default_rng(49)
+    a = dev_soa(rng.normal(size=sc.pos.shape))
+    b = dev_soa(rng.normal(size=sc.pos.shape))
+    x, v = dev_soa(sc.pos), dev_soa(sc.vel)
+    r1 = [t.cpu().numpy() for t in ctx.flip_step_backward(x, v, a, b, sc.dims, sc.dx, sc.dt)]
+    r2 = [t.cpu().numpy() for t in ctx.flip_step_backward(x, v, a, b, sc.dims, sc.dx, sc.dt)]
+    assert np.array_equal(r1[0], r2[0]) and np.array_equal(r1[1], r2[1])
+
+
 def test_step_backward_position_only_and_velocity_only(ctx):
     """Adjoints of x' alone and of v' alone, each against the dense
     reference (covers the RK4 path and the PIC/FLIP path separately)."""

@@ -310,6 +310,9 @@ def main():
     ap.add_argument("--no-fwdbwd", action="store_true",
                     help="skip the differentiable-form measurement (96^3 slab)")
     ap.add_argument("--fwdbwd-reps", type=int, default=3)
+    ap.add_argument("--only", choices=["batched", "fwdbwd"], default=None,
+                    help="run only that form's measurement and print its record "
+                         "(for profiling it alone)")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a plain copy of this many bytes (counter calibration)")
     args = ap.parse_args()
@@ -330,6 +333,12 @@ def main():
     from gridfluidsim3d_amd import scenes
 
     stencil = gf.TRILINEAR if args.stencil == "trilinear" else gf.TRICUBIC
+    if args.only == "batched":
+        print(json.dumps(batched_measure(args.batched_steps, stencil, log)), flush=True)
+        return
+    if args.only == "fwdbwd":
+        print(json.dumps(fwdbwd_measure(args.fwdbwd_reps, log)), flush=True)
+        return
     t0 = time.time()
     scene = scenes.CONFIGS[args.workload]()
     P = scene.n

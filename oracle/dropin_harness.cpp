@@ -15,6 +15,7 @@
 // Built by oracle/Makefile (target dropin) into oracle/_ref/.
 #include <cmath>
 #include <cstdint>
+#include <algorithm>
 #include <vector>
 
 #include "clscalarfield.h"
@@ -67,6 +68,85 @@ int dropin_p2g(const float *pos, const float *val, int64_t n, double radius,
     for (int64_t p = 0; p < n; p++) cpu.addPointValue(points[p], values[p]);
     copy_out(cpu.getPointerToScalarField(), S_cpu);
     copy_out(cpu.getPointerToWeightField(), W_cpu);
+    return 1;
+}
+
+// The isotropic mesher's call form (isotropicparticlemesher.cpp:332-391):
+// the threshold set, then addPoints(positions, field) on batches of `batch`
+// points into one ScalarField; against the mesher's CPU form, addPoint per
+// point (:325-329).  threshold < 0 leaves it unset.
+int dropin_add_points(const float *pos, int64_t n, double radius, const float off[3],
+                      double dx, int ni, int nj, int nk, float threshold, int batch,
+                      float *S_gpu, float *S_cpu) {
+    std::vector<vmath::vec3> points((size_t)n);
+    for (int64_t p = 0; p < n; p++) points[p] = vmath::vec3(pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]);
+    const vmath::vec3 offset(off[0], off[1], off[2]);
+
+    CLScalarField acc;
+    if (!acc.initialize()) return 0;
+    if (threshold >= 0.0f) acc.setMaxScalarFieldValueThreshold(threshold);
+    ScalarField gpu(ni, nj, nk, dx);
+    gpu.setPointRadius(radius);
+    gpu.setOffset(offset);
+    std::vector<vmath::vec3> part;
+    for (int64_t s = 0; s < n; s += batch) {
+        part.assign(points.begin() + s, points.begin() + std::min<int64_t>(n, s + batch));
+        acc.addPoints(part, gpu);
+    }
+    copy_out(gpu.getPointerToScalarField(), S_gpu);
+
+    ScalarField cpu(ni, nj, nk, dx);
+    cpu.setPointRadius(radius);
+    cpu.setOffset(offset);
+    for (int64_t p = 0; p < n; p++) cpu.addPoint(points[p]);
+    copy_out(cpu.getPointerToScalarField(), S_cpu);
+    return 1;
+}
+
+// ParticleAdvector::advectParticlesRK1..3 in the call form of
+// DiffuseParticleSimulation (diffuseparticlesimulation.cpp:570-582, RK2
+// into a separate output), against ParticleAdvector::_RK1.._RK3
+// (particleadvector.cpp:1056-1078) restated around the reference's
+// MACVelocityField.
+int dropin_advect(const float *pos, int64_t n, const float *u, const float *v, const float *w,
+                  int ni, int nj, int nk, double dx, double dt, int order, int stencil,
+                  float *out_gpu, float *out_cpu) {
+    MACVelocityField mac(ni, nj, nk, dx);
+    fill(mac.getArray3dU(), u);
+    fill(mac.getArray3dV(), v);
+    fill(mac.getArray3dW(), w);
+    std::vector<vmath::vec3> points((size_t)n);
+    for (int64_t p = 0; p < n; p++) points[p] = vmath::vec3(pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]);
+    ParticleAdvector adv;
+    if (!adv.initialize()) return 0;
+    adv.setStencil(stencil);
+    std::vector<vmath::vec3> moved;
+    if (order == 1) adv.advectParticlesRK1(points, &mac, dt, moved);
+    else if (order == 2) adv.advectParticlesRK2(points, &mac, dt, moved);
+    else if (order == 3) adv.advectParticlesRK3(points, &mac, dt, moved);
+    else return 0;
+    if ((int64_t)moved.size() != n) return 0;
+    for (int64_t p = 0; p < n; p++) {
+        out_gpu[3 * p] = moved[p].x;
+        out_gpu[3 * p + 1] = moved[p].y;
+        out_gpu[3 * p + 2] = moved[p].z;
+        const vmath::vec3 p0 = points[p];
+        const vmath::vec3 k1 = eval(mac, p0, stencil);
+        vmath::vec3 p1;
+        if (order == 1) {
+            p1 = p0 + (float)dt * k1;
+        } else if (order == 2) {
+            const vmath::vec3 k2 = eval(mac, p0 + (float)(0.5 * dt) * k1, stencil);
+            p1 = p0 + (float)dt * k2;
+        } else {
+            const vmath::vec3 k2 = eval(mac, p0 + (float)(0.5 * dt) * k1, stencil);
+            const vmath::vec3 k3 = eval(mac, p0 + (float)(0.75 * dt) * k2, stencil);
+            p1 = p0 + (float)(dt / 9.0f) * (2.0f * k1 + 3.0f * k2 + 4.0f * k3);
+        }
+        out_cpu[3 * p] = p1.x;
+        out_cpu[3 * p + 1] = p1.y;
+        out_cpu[3 * p + 2] = p1.z;
+    }
     return 1;
 }
 

@@ -30,6 +30,14 @@ def _lib():
     lib.dropin_g2p_rk4.argtypes = [_f32p, ctypes.c_int64, _f32p, _f32p, _f32p, ctypes.c_int,
                                    ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                    ctypes.c_double, ctypes.c_int, _f32p, _f32p, _f32p, _f32p]
+    lib.dropin_advect.restype = ctypes.c_int
+    lib.dropin_advect.argtypes = [_f32p, ctypes.c_int64, _f32p, _f32p, _f32p, ctypes.c_int,
+                                  ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                  ctypes.c_int, ctypes.c_int, _f32p, _f32p]
+    lib.dropin_add_points.restype = ctypes.c_int
+    lib.dropin_add_points.argtypes = [_f32p, ctypes.c_int64, ctypes.c_double, _f32p,
+                                      ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_float, ctypes.c_int, _f32p, _f32p]
     return lib
 
 
@@ -74,3 +82,46 @@ def test_particleadvector_interpolate_and_rk4(stencil):
     assert ok == 1
     assert norm_rel(g_gpu, g_cpu) <= NORM_REL_TOL
     assert norm_rel(r_gpu - pos, r_cpu - pos) <= NORM_REL_TOL
+
+
+@pytest.mark.parametrize("threshold,batch", [(-1.0, 7000), (0.5, 5000)])
+def test_clscalarfield_add_points_mesher_form(threshold, batch):
+    """CLScalarField::addPoints in the isotropic mesher's form (threshold
+    set, batches into one ScalarField, isotropicparticlemesher.cpp:332-391)
+    against the mesher's CPU form (ScalarField::addPoint per point).  The
+    threshold is the OpenCL path's chunk-culling heuristic; gfsx always
+    splats every point, which is what the CPU form computes."""
+    lib = _lib()
+    rng = np.random.default_rng(19)
+    dims = (24, 20, 16)
+    dx = 0.05
+    n = 18000
+    pos = rng.uniform(0.0, np.array(dims) * dx, size=(n, 3)).astype(np.float32)
+    off = np.array((0.5 * dx, 0.5 * dx, 0.5 * dx), np.float32)
+    shape = dims[::-1]
+    Sg, Sc = np.zeros(shape, np.float32), np.zeros(shape, np.float32)
+    ok = lib.dropin_add_points(_p(pos), n, 2.0 * dx, _p(off), dx, *dims, threshold, batch,
+                               _p(Sg), _p(Sc))
+    assert ok == 1
+    assert Sc.max() > 1.0
+    assert norm_rel(Sg, Sc) <= NORM_REL_TOL
+
+
+@pytest.mark.parametrize("order,stencil", [(1, 1), (2, 1), (2, 0), (3, 1)])
+def test_particleadvector_rk1_to_rk3(order, stencil):
+    """advectParticlesRK1..3 through the drop-in in DiffuseParticleSimulation's
+    call form (RK2 of the foam particles, tricubic) against the reference's
+    formulas on its own MACVelocityField."""
+    lib = _lib()
+    rng = np.random.default_rng(29 + order)
+    dims = (18, 15, 13)
+    dx = 0.05
+    shapes = [(13, 15, 19), (13, 16, 18), (14, 15, 18)]
+    u, v, w = (rng.normal(size=s).astype(np.float32) for s in shapes)
+    n = 12000
+    pos = rng.uniform(-0.02, np.array(dims) * dx + 0.02, size=(n, 3)).astype(np.float32)
+    gpu, cpu = np.zeros((n, 3), np.float32), np.zeros((n, 3), np.float32)
+    ok = lib.dropin_advect(_p(pos), n, _p(u), _p(v), _p(w), *dims, dx, 0.01, order, stencil,
+                           _p(gpu), _p(cpu))
+    assert ok == 1
+    assert norm_rel(gpu - pos, cpu - pos) <= NORM_REL_TOL

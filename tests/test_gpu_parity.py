@@ -583,3 +583,25 @@ def test_autotune_picks_a_variant(ctx):
     # the caller's states did not move
     for st, b in zip(states, before):
         assert _torch().equal(st.positions(), b)
+
+
+def test_step_on_loaded_save_state(ctx, orc, tmp_path):
+    """A scene read from a save state (the reference's binary layout,
+    gridfluidsim3d_amd/savestate.py) through the fused step, against the
+    oracle."""
+    import gridfluidsim3d_amd as gf
+    from gridfluidsim3d_amd import savestate, scenes
+    sc = scenes.random_blobs(61, (24, 20, 18), 0.05, ppc=8, amplitude=1.5)
+    f = tmp_path / "scene.state"
+    savestate.write_state(str(f), sc.dims, sc.dx, sc.pos, sc.vel, frame=3)
+    r = savestate.SaveStateReader()
+    assert r.load_state(str(f))
+    loaded = r.to_scene(1.0 / 120.0)
+    st = gf.FlipState(loaded.pos, loaded.vel, loaded.dims, loaded.dx, track_ids=True)
+    ctx.flip_step(st, loaded.dt, loaded.body_force, 0.05, gf.TRILINEAR, gf.ORDER_SORTED)
+    ref = orc.flip_step(loaded.pos, loaded.vel, loaded.dims, loaded.dx, loaded.dt,
+                        loaded.body_force, 0.05, gf.TRILINEAR)
+    ids = st.particle_ids().cpu().numpy()
+    assert norm_rel(st.positions().cpu().numpy()[np.argsort(ids)] - loaded.pos,
+                    ref["pos"] - loaded.pos) <= NORM_REL_TOL
+    assert norm_rel(st.velocities().cpu().numpy()[np.argsort(ids)], ref["vel"]) <= NORM_REL_TOL

@@ -385,6 +385,32 @@ static void apply_body_force_component(float *f, double *fd, int c,
             }
 }
 
+static void update_particles(float *pos, float *vel, int64_t n, float *const grid[3],
+                             float *const old[3], int ni, int nj, int nk, double dx,
+                             double dt, double pic_ratio, int stencil) {
+    macgrid gnew = {{grid[0], grid[1], grid[2]}, 0, ni, nj, nk, dx};
+    macgrid gold = {{old[0], old[1], old[2]}, 0, ni, nj, nk, dx};
+    float r_pic = (float)pic_ratio, r_flip = (float)(1 - pic_ratio);
+    for (int64_t p = 0; p < n; p++) {
+        v3f x0 = {pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]};
+        v3f v0 = {vel[3 * p], vel[3 * p + 1], vel[3 * p + 2]};
+        /* 5. PIC/FLIP (fluidsimulation.cpp:3115-3127) */
+        v3f vnew = validated(eval_f(&gnew, x0, stencil));
+        v3f vold = validated(eval_f(&gold, x0, stencil));
+        v3f vflip = vsub(vadd(v0, vnew), vold);
+        v3f vn = vadd(vscale(r_pic, vnew), vscale(r_flip, vflip));
+        /* 6. RK4 through the new field (fluidsimulation.cpp:3181-3207,
+           particleadvector.cpp:1045-1054) */
+        v3f x1 = rk_step(&gnew, x0, dt, 4, stencil);
+        vel[3 * p] = vn.x;
+        vel[3 * p + 1] = vn.y;
+        vel[3 * p + 2] = vn.z;
+        pos[3 * p] = x1.x;
+        pos[3 * p + 1] = x1.y;
+        pos[3 * p + 2] = x1.z;
+    }
+}
+
 void gfo_flip_step(float *pos, float *vel, int64_t n, int ni, int nj, int nk,
                    double dx, double dt, const float body_force[3],
                    double pic_ratio, int stencil,
@@ -432,27 +458,7 @@ void gfo_flip_step(float *pos, float *vel, int64_t n, int ni, int nj, int nk,
                                        (double)body_force[c] * dt);
     }
 
-    macgrid gnew = {{grid[0], grid[1], grid[2]}, 0, ni, nj, nk, dx};
-    macgrid gold = {{old[0], old[1], old[2]}, 0, ni, nj, nk, dx};
-    float r_pic = (float)pic_ratio, r_flip = (float)(1 - pic_ratio);
-    for (int64_t p = 0; p < n; p++) {
-        v3f x0 = {pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]};
-        v3f v0 = {vel[3 * p], vel[3 * p + 1], vel[3 * p + 2]};
-        /* 5. PIC/FLIP (fluidsimulation.cpp:3115-3127) */
-        v3f vnew = validated(eval_f(&gnew, x0, stencil));
-        v3f vold = validated(eval_f(&gold, x0, stencil));
-        v3f vflip = vsub(vadd(v0, vnew), vold);
-        v3f vn = vadd(vscale(r_pic, vnew), vscale(r_flip, vflip));
-        /* 6. RK4 through the new field (fluidsimulation.cpp:3181-3207,
-           particleadvector.cpp:1045-1054) */
-        v3f x1 = rk_step(&gnew, x0, dt, 4, stencil);
-        vel[3 * p] = vn.x;
-        vel[3 * p + 1] = vn.y;
-        vel[3 * p + 2] = vn.z;
-        pos[3 * p] = x1.x;
-        pos[3 * p + 1] = x1.y;
-        pos[3 * p + 2] = x1.z;
-    }
+    update_particles(pos, vel, n, grid, old, ni, nj, nk, dx, dt, pic_ratio, stencil);
 
     for (int c = 0; c < 3; c++) {
         if (outs[c]) memcpy(outs[c], grid[c], sizeof(float) * (size_t)nf[c]);
@@ -465,6 +471,234 @@ void gfo_flip_step(float *pos, float *vel, int64_t n, int ni, int nj, int nk,
     if (fluid_out) memcpy(fluid_out, fluid, (size_t)ncell);
     free(vals);
     free(fluid);
+}
+
+/* ------------------------------------------------------------------------- */
+/* The step with FluidSimulation's face fill and extrapolation               */
+/* ------------------------------------------------------------------------- */
+
+enum { MAT_AIR = 0, MAT_FLUID = 1, MAT_SOLID = 2 };
+
+typedef struct {
+    const uint8_t *mat;   /* ni*nj*nk material codes */
+    int ni, nj, nk;
+} matgrid;
+
+static inline void comp_dims(int c, int ni, int nj, int nk, int d[3]) {
+    d[0] = ni + (c == 0);
+    d[1] = nj + (c == 1);
+    d[2] = nk + (c == 2);
+}
+
+/* FluidMaterialGrid::isFaceBorderingMaterialU/V/W (fluidmaterialgrid.cpp:
+   119-143) over any per-cell code array, and MACVelocityField::
+   _isFaceBorderingGridValueU/V/W (macvelocityfield.h:159-173) over the layer
+   grid: the cell on the high side unless the face is the last one, the cell
+   on the low side unless the face is the first. */
+static int face_borders_u8(const uint8_t *cell, int c, int i, int j, int k, int ni, int nj,
+                           int nk, uint8_t value) {
+    int a = c == 0 ? i : (c == 1 ? j : k);
+    int na = c == 0 ? ni : (c == 1 ? nj : nk);
+    int li = i - (c == 0), lj = j - (c == 1), lk = k - (c == 2);
+    if (a == na) return cell[flat(li, lj, lk, ni, nj)] == value;
+    if (a > 0)
+        return cell[flat(i, j, k, ni, nj)] == value || cell[flat(li, lj, lk, ni, nj)] == value;
+    return cell[flat(i, j, k, ni, nj)] == value;
+}
+
+static int face_borders_i32(const int *cell, int c, int i, int j, int k, int ni, int nj,
+                            int nk, int value) {
+    int a = c == 0 ? i : (c == 1 ? j : k);
+    int na = c == 0 ? ni : (c == 1 ? nj : nk);
+    int li = i - (c == 0), lj = j - (c == 1), lk = k - (c == 2);
+    if (a == na) return cell[flat(li, lj, lk, ni, nj)] == value;
+    if (a > 0)
+        return cell[flat(i, j, k, ni, nj)] == value || cell[flat(li, lj, lk, ni, nj)] == value;
+    return cell[flat(i, j, k, ni, nj)] == value;
+}
+
+/* FluidSimulation::_advectVelocityFieldU/V/W (fluidsimulation.cpp:2597-2730):
+   the MAC component is cleared; faces bordering fluid take the normalised
+   P2G value where the weight exceeds 1e-9 (isValueSet, :2575-2585), else
+   the average of their 26 neighbours (the neighbour test is |value| > 0 for
+   U, :2630, but isValueSet for V and W, :2675,2720). */
+static void fill_component(float *out, const float *ugrid, const float *W, int c,
+                           const matgrid *m) {
+    int d[3];
+    comp_dims(c, m->ni, m->nj, m->nk, d);
+    int64_t nf = (int64_t)d[0] * d[1] * d[2];
+    memset(out, 0, sizeof(float) * (size_t)nf);
+    for (int k = 0; k < d[2]; k++)
+        for (int j = 0; j < d[1]; j++)
+            for (int i = 0; i < d[0]; i++) {
+                if (!face_borders_u8(m->mat, c, i, j, k, m->ni, m->nj, m->nk, MAT_FLUID))
+                    continue;
+                int64_t f = flat(i, j, k, d[0], d[1]);
+                if ((double)W[f] > 1e-9) {
+                    out[f] = ugrid[f];
+                    continue;
+                }
+                double avg = 0.0, weight = 0.0;
+                for (int nk2 = k - 1; nk2 <= k + 1; nk2++)
+                    for (int nj2 = j - 1; nj2 <= j + 1; nj2++)
+                        for (int ni2 = i - 1; ni2 <= i + 1; ni2++) {
+                            if (ni2 == i && nj2 == j && nk2 == k) continue;
+                            if (!in_range(ni2, nj2, nk2, d[0], d[1], d[2])) continue;
+                            int64_t g = flat(ni2, nj2, nk2, d[0], d[1]);
+                            int use = c == 0 ? fabs(ugrid[g]) > 0.0 : (double)W[g] > 1e-9;
+                            if (use) {
+                                avg += ugrid[g];
+                                weight += 1.0;
+                            }
+                        }
+                if (weight > 0.0) out[f] = (float)(avg / weight);
+            }
+}
+
+/* MACVelocityField::extrapolateVelocityField (macvelocityfield.cpp:577-798):
+   faces not bordering fluid are cleared; cells get BFS layers (fluid 0,
+   then 6-neighbour dilation through non-solid cells); for each layer L the
+   faces that border layer L, not layer L-1 and no solid cell take the
+   average of their 6 neighbour faces that border layer L-1 (0 if that sum
+   is 0). */
+static void extrapolate(float *g[3], const matgrid *m, int layers) {
+    int ni = m->ni, nj = m->nj, nk = m->nk;
+    int64_t ncell = (int64_t)ni * nj * nk;
+    int *layer = (int *)malloc(sizeof(int) * (size_t)(ncell > 0 ? ncell : 1));
+    for (int c = 0; c < 3; c++) {
+        int d[3];
+        comp_dims(c, ni, nj, nk, d);
+        for (int k = 0; k < d[2]; k++)
+            for (int j = 0; j < d[1]; j++)
+                for (int i = 0; i < d[0]; i++)
+                    if (!face_borders_u8(m->mat, c, i, j, k, ni, nj, nk, MAT_FLUID))
+                        g[c][flat(i, j, k, d[0], d[1])] = 0.0f;
+    }
+    for (int64_t q = 0; q < ncell; q++) layer[q] = m->mat[q] == MAT_FLUID ? 0 : -1;
+    for (int L = 1; L <= layers; L++)
+        for (int k = 0; k < nk; k++)
+            for (int j = 0; j < nj; j++)
+                for (int i = 0; i < ni; i++) {
+                    int64_t q = flat(i, j, k, ni, nj);
+                    if (layer[q] != L - 1 || m->mat[q] == MAT_SOLID) continue;
+                    int nb[6][3] = {{i - 1, j, k}, {i + 1, j, k}, {i, j - 1, k},
+                                    {i, j + 1, k}, {i, j, k - 1}, {i, j, k + 1}};
+                    for (int e = 0; e < 6; e++) {
+                        if (!in_range(nb[e][0], nb[e][1], nb[e][2], ni, nj, nk)) continue;
+                        int64_t r = flat(nb[e][0], nb[e][1], nb[e][2], ni, nj);
+                        if (layer[r] == -1 && m->mat[r] != MAT_SOLID) layer[r] = L;
+                    }
+                }
+    for (int L = 1; L <= layers; L++)
+        for (int c = 0; c < 3; c++) {
+            int d[3];
+            comp_dims(c, ni, nj, nk, d);
+            for (int k = 0; k < d[2]; k++)
+                for (int j = 0; j < d[1]; j++)
+                    for (int i = 0; i < d[0]; i++) {
+                        if (!(face_borders_i32(layer, c, i, j, k, ni, nj, nk, L) &&
+                              !face_borders_i32(layer, c, i, j, k, ni, nj, nk, L - 1) &&
+                              !face_borders_u8(m->mat, c, i, j, k, ni, nj, nk, MAT_SOLID)))
+                            continue;
+                        int nb[6][3] = {{i - 1, j, k}, {i + 1, j, k}, {i, j - 1, k},
+                                        {i, j + 1, k}, {i, j, k - 1}, {i, j, k + 1}};
+                        double sum = 0.0, wsum = 0.0;
+                        for (int e = 0; e < 6; e++) {
+                            int a = nb[e][0], b = nb[e][1], cc = nb[e][2];
+                            if (!in_range(a, b, cc, d[0], d[1], d[2])) continue;
+                            if (!face_borders_i32(layer, c, a, b, cc, ni, nj, nk, L - 1)) continue;
+                            sum += g[c][flat(a, b, cc, d[0], d[1])];
+                            wsum++;
+                        }
+                        double v = sum == 0.0 ? 0.0 : sum / wsum;
+                        g[c][flat(i, j, k, d[0], d[1])] = (float)v;
+                    }
+        }
+    free(layer);
+}
+
+void gfo_fill_component(float *out, const float *ugrid, const float *W, int c,
+                        const uint8_t *mat, int ni, int nj, int nk) {
+    matgrid m = {mat, ni, nj, nk};
+    fill_component(out, ugrid, W, c, &m);
+}
+
+void gfo_extrapolate(float *u, float *v, float *w, int ni, int nj, int nk,
+                     const uint8_t *mat, int layers) {
+    matgrid m = {mat, ni, nj, nk};
+    float *g[3] = {u, v, w};
+    extrapolate(g, &m, layers);
+}
+
+void gfo_flip_step_fs(float *pos, float *vel, int64_t n, int ni, int nj, int nk,
+                      double dx, double dt, const float body_force[3], double pic_ratio,
+                      int stencil, const uint8_t *solid, int layers,
+                      float *u, float *v, float *w, float *u_old, float *v_old,
+                      float *w_old, uint8_t *mat_out) {
+    int64_t ncell = (int64_t)ni * nj * nk;
+    int64_t nf[3] = {(int64_t)(ni + 1) * nj * nk, (int64_t)ni * (nj + 1) * nk,
+                     (int64_t)ni * nj * (nk + 1)};
+    int dims[3][3] = {{ni + 1, nj, nk}, {ni, nj + 1, nk}, {ni, nj, nk + 1}};
+    uint8_t *mat = (uint8_t *)calloc((size_t)(ncell > 0 ? ncell : 1), 1);
+    uint8_t *fluid = (uint8_t *)calloc((size_t)(ncell > 0 ? ncell : 1), 1);
+    float *grid[3], *old[3];
+    float *outs[3] = {u, v, w}, *olds[3] = {u_old, v_old, w_old};
+    float *vals = (float *)malloc(sizeof(float) * (size_t)(n > 0 ? n : 1));
+    float *S = NULL, *W = NULL;
+
+    /* solid cells, then fluid cells = non-solid cells holding a particle
+       (fluidsimulation.cpp:1990-2017; the reference removes particles from
+       solid cells beforehand, :1991) */
+    for (int64_t q = 0; q < ncell; q++) mat[q] = solid && solid[q] ? MAT_SOLID : MAT_AIR;
+    for (int64_t p = 0; p < n; p++) {
+        int64_t cell;
+        if (cell_of(pos[3 * p], pos[3 * p + 1], pos[3 * p + 2], dx, ni, nj, nk, &cell) &&
+            mat[cell] != MAT_SOLID)
+            mat[cell] = MAT_FLUID;
+    }
+    for (int64_t q = 0; q < ncell; q++) fluid[q] = mat[q] == MAT_FLUID;
+    matgrid m = {mat, ni, nj, nk};
+
+    for (int c = 0; c < 3; c++) {
+        grid[c] = (float *)calloc((size_t)nf[c], sizeof(float));
+        S = (float *)calloc((size_t)nf[c], sizeof(float));
+        W = (float *)calloc((size_t)nf[c], sizeof(float));
+        float off[3];
+        off[0] = c == 0 ? 0.0f : (float)(0.5 * dx);
+        off[1] = c == 1 ? 0.0f : (float)(0.5 * dx);
+        off[2] = c == 2 ? 0.0f : (float)(0.5 * dx);
+        for (int64_t p = 0; p < n; p++) vals[p] = vel[3 * p + c];
+        gfo_p2g(pos, vals, n, dx, off, dx, dims[c][0], dims[c][1], dims[c][2], S, W);
+        gfo_apply_weight(S, W, nf[c]);
+        fill_component(grid[c], S, W, c, &m);
+        free(S);
+        free(W);
+        /* snapshot (fluidsimulation.cpp:3306) */
+        old[c] = (float *)malloc(sizeof(float) * (size_t)nf[c]);
+        memcpy(old[c], grid[c], sizeof(float) * (size_t)nf[c]);
+    }
+    /* the snapshot is extrapolated (:3307), the field gets the body force
+       (:2766-2805) and is extrapolated after the (omitted) pressure solve
+       (:3334), with ceil(CFL + 2) layers (:3067-3070) */
+    extrapolate(old, &m, layers);
+    for (int c = 0; c < 3; c++)
+        if (fabs((double)body_force[c]) > 0.0)
+            apply_body_force_component(grid[c], NULL, c, fluid, ni, nj, nk,
+                                       (double)body_force[c] * dt);
+    extrapolate(grid, &m, layers);
+
+    update_particles(pos, vel, n, grid, old, ni, nj, nk, dx, dt, pic_ratio, stencil);
+
+    for (int c = 0; c < 3; c++) {
+        if (outs[c]) memcpy(outs[c], grid[c], sizeof(float) * (size_t)nf[c]);
+        if (olds[c]) memcpy(olds[c], old[c], sizeof(float) * (size_t)nf[c]);
+        free(grid[c]);
+        free(old[c]);
+    }
+    if (mat_out) memcpy(mat_out, mat, (size_t)ncell);
+    free(vals);
+    free(fluid);
+    free(mat);
 }
 
 int64_t gfo_cell_sort(const float *pos, int64_t n, double dx, int ni, int nj,

@@ -66,6 +66,30 @@ void gfo_flip_step(float *pos, float *vel, int64_t n, int ni, int nj, int nk,
                    float *u_old, float *v_old, float *w_old,
                    float *wu, float *wv, float *ww, uint8_t *fluid);
 
+/* The plain step with FluidSimulation's epilogue stages: the face fill of
+ * _advectVelocityFieldU/V/W (fluidsimulation.cpp:2597-2730: faces bordering
+ * fluid keep the P2G value where the weight exceeds 1e-9, else the average
+ * of their 26 neighbours; other faces 0), the snapshot, extrapolation of the
+ * snapshot, the body force, extrapolation of the field
+ * (MACVelocityField::extrapolateVelocityField, macvelocityfield.cpp:577-798,
+ * `layers` layers; fluidsimulation.cpp:3304-3334 without the pressure
+ * solve), then PIC/FLIP and RK4.  solid (ni*nj*nk bytes, nonzero = solid)
+ * may be NULL.  mat_out (optional) receives the cell codes: 0 air, 1 fluid,
+ * 2 solid. */
+void gfo_flip_step_fs(float *pos, float *vel, int64_t n, int ni, int nj, int nk,
+                      double dx, double dt, const float body_force[3], double pic_ratio,
+                      int stencil, const uint8_t *solid, int layers,
+                      float *u, float *v, float *w, float *u_old, float *v_old,
+                      float *w_old, uint8_t *mat_out);
+
+/* The two epilogue stages alone, on cell codes mat (0 air, 1 fluid,
+ * 2 solid): the face fill of one component c (out <- ugrid, W) and the
+ * extrapolation of u, v, w in place. */
+void gfo_fill_component(float *out, const float *ugrid, const float *W, int c,
+                        const uint8_t *mat, int ni, int nj, int nk);
+void gfo_extrapolate(float *u, float *v, float *w, int ni, int nj, int nk,
+                     const uint8_t *mat, int layers);
+
 /* Stable counting sort of particle indices by the cell that
  * Grid3d::positionToGridIndex (grid3d.h:58-63) assigns.  Particles outside
  * [0,ni)x[0,nj)x[0,nk) get no cell and are left out of perm.

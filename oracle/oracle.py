@@ -97,7 +97,27 @@ class CPUPath:
                        _f32p, _dbl, _int] + [_f32p] * 9 + [_u8p]
         st.restype = None
         self._step = st
+        fs = getattr(L, prefix + "_flip_step_fs")
+        fs.argtypes = [_f32p, _f32p, _i64, _int, _int, _int, _dbl, _dbl, _f32p, _dbl, _int,
+                       _u8p, _int] + [_f32p] * 6 + [_u8p]
+        fs.restype = None
+        self._step_fs = fs
+        if prefix == "ref":
+            ex = L.ref_extrapolate
+            ex.argtypes = [_f32p, _f32p, _f32p, _int, _int, _int, _dbl, _u8p, _int]
+            ex.restype = None
+            self._extrapolate = lambda u, v, w, ni, nj, nk, dx, m, layers: ex(
+                u, v, w, ni, nj, nk, dx, m, layers)
         if prefix == "gfo":
+            ex = L.gfo_extrapolate
+            ex.argtypes = [_f32p, _f32p, _f32p, _int, _int, _int, _u8p, _int]
+            ex.restype = None
+            self._extrapolate = lambda u, v, w, ni, nj, nk, dx, m, layers: ex(
+                u, v, w, ni, nj, nk, m, layers)
+            fc = L.gfo_fill_component
+            fc.argtypes = [_f32p, _f32p, _f32p, _int, _u8p, _int, _int, _int]
+            fc.restype = None
+            self._fill = fc
             aw = L.gfo_apply_weight
             aw.argtypes = [_f32p, _f32p, _i64]
             aw.restype = None
@@ -173,6 +193,48 @@ class CPUPath:
         return dict(pos=pos, vel=vel, u=g[0], v=g[1], w=g[2], u_old=o[0],
                     v_old=o[1], w_old=o[2], wu=wt[0], wv=wt[1], ww=wt[2],
                     fluid=fluid)
+
+    def flip_step_fs(self, pos, vel, dims, dx, dt, body_force=(0.0, -9.8, 0.0),
+                     pic_ratio=0.05, stencil=TRILINEAR, solid=None, layers=7):
+        """The step with FluidSimulation's face fill and extrapolation.
+        Returns pos/vel, the grids (u, v, w extrapolated after the body
+        force; u_old.. the extrapolated snapshot) and the cell codes
+        (0 air, 1 fluid, 2 solid)."""
+        pos = _f32(pos).reshape(-1, 3).copy()
+        vel = _f32(vel).reshape(-1, 3).copy()
+        shapes = mac_shapes(dims)
+        g = [np.zeros(s, np.float32) for s in shapes]
+        o = [np.zeros(s, np.float32) for s in shapes]
+        ni, nj, nk = dims
+        mat = np.zeros((nk, nj, ni), np.uint8)
+        sol = None if solid is None else np.ascontiguousarray(solid, np.uint8).reshape(nk, nj, ni)
+        bf = _f32(body_force)
+        self._step_fs(_ptr(pos, _f32p), _ptr(vel, _f32p), pos.shape[0], ni, nj, nk, float(dx),
+                      float(dt), _ptr(bf, _f32p), float(pic_ratio), int(stencil),
+                      _ptr(sol, _u8p), int(layers), *[_ptr(a, _f32p) for a in g + o],
+                      _ptr(mat, _u8p))
+        return dict(pos=pos, vel=vel, u=g[0], v=g[1], w=g[2], u_old=o[0], v_old=o[1],
+                    w_old=o[2], mat=mat)
+
+    def extrapolate(self, u, v, w, dims, dx, mat, layers):
+        """MACVelocityField::extrapolateVelocityField on copies of u, v, w."""
+        u, v, w = (np.array(a, np.float32) for a in (u, v, w))
+        ni, nj, nk = dims
+        m = np.ascontiguousarray(mat, np.uint8).reshape(nk, nj, ni)
+        self._extrapolate(_ptr(u, _f32p), _ptr(v, _f32p), _ptr(w, _f32p), ni, nj, nk,
+                          float(dx), _ptr(m, _u8p), int(layers))
+        return u, v, w
+
+    def fill_component(self, ugrid, W, c, dims, mat):
+        """The face fill of component c (restatement only)."""
+        ugrid = _f32(ugrid)
+        W = _f32(W)
+        out = np.zeros_like(ugrid)
+        ni, nj, nk = dims
+        m = np.ascontiguousarray(mat, np.uint8).reshape(nk, nj, ni)
+        self._fill(_ptr(out, _f32p), _ptr(ugrid, _f32p), _ptr(W, _f32p), int(c), _ptr(m, _u8p),
+                   ni, nj, nk)
+        return out
 
     def cell_sort(self, pos, dx, dims):
         pos = _f32(pos).reshape(-1, 3)

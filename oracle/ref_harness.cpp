@@ -133,6 +133,75 @@ void ref_advect(const float *pos, int64_t n, const float *u, const float *v,
     }
 }
 
+}  // extern "C"
+
+namespace {
+
+// FluidSimulation::_advectVelocityFieldU/V/W (fluidsimulation.cpp:2597-2730)
+// restated around the reference's classes (FluidSimulation itself does not
+// compile here): the component is cleared, faces bordering fluid take the
+// P2G value where the weight exceeds 1e-9, else the average of their 26
+// neighbours (|value| > 0 for U, isValueSet for V and W).
+void fill_face_values(MACVelocityField &mac, FluidMaterialGrid &mat, int c,
+                      Array3d<float> &ugrid, Array3d<float> &weight) {
+    const double eps = 1e-9;
+    Array3d<bool> isValueSet(ugrid.width, ugrid.height, ugrid.depth, false);
+    for (int k = 0; k < ugrid.depth; k++)
+        for (int j = 0; j < ugrid.height; j++)
+            for (int i = 0; i < ugrid.width; i++)
+                if (weight(i, j, k) > eps) isValueSet.set(i, j, k, true);
+    if (c == 0) mac.clearU();
+    else if (c == 1) mac.clearV();
+    else mac.clearW();
+    std::vector<GridIndex> todo;
+    for (int k = 0; k < ugrid.depth; k++)
+        for (int j = 0; j < ugrid.height; j++)
+            for (int i = 0; i < ugrid.width; i++) {
+                bool border = c == 0 ? mat.isFaceBorderingFluidU(i, j, k)
+                            : c == 1 ? mat.isFaceBorderingFluidV(i, j, k)
+                                     : mat.isFaceBorderingFluidW(i, j, k);
+                if (!border) continue;
+                if (!isValueSet(i, j, k)) {
+                    todo.push_back(GridIndex(i, j, k));
+                } else if (c == 0) {
+                    mac.setU(i, j, k, ugrid(i, j, k));
+                } else if (c == 1) {
+                    mac.setV(i, j, k, ugrid(i, j, k));
+                } else {
+                    mac.setW(i, j, k, ugrid(i, j, k));
+                }
+            }
+    GridIndex nb[26];
+    for (size_t t = 0; t < todo.size(); t++) {
+        GridIndex g = todo[t];
+        Grid3d::getNeighbourGridIndices26(g, nb);
+        double avg = 0.0, wsum = 0.0;
+        for (int e = 0; e < 26; e++) {
+            GridIndex q = nb[e];
+            if (!ugrid.isIndexInRange(q)) continue;
+            bool use = c == 0 ? fabs(ugrid(q)) > 0.0 : isValueSet(q);
+            if (use) {
+                avg += ugrid(q);
+                wsum += 1.0;
+            }
+        }
+        if (wsum > 0.0) {
+            if (c == 0) mac.setU(g, avg / wsum);
+            else if (c == 1) mac.setV(g, avg / wsum);
+            else mac.setW(g, avg / wsum);
+        }
+    }
+}
+
+void step_impl(float *pos, float *vel, int64_t n, int ni, int nj, int nk, double dx, double dt,
+               const float body_force[3], double pic_ratio, int stencil, bool fs,
+               const uint8_t *solid, int layers, float *u, float *v, float *w, float *u_old,
+               float *v_old, float *w_old, float *wu, float *wv, float *ww, uint8_t *mat_out);
+
+}  // namespace
+
+extern "C" {
+
 // The plain step, stage by stage, with the reference classes.
 void ref_flip_step(float *pos, float *vel, int64_t n, int ni, int nj, int nk,
                    double dx, double dt, const float body_force[3],
@@ -140,12 +209,62 @@ void ref_flip_step(float *pos, float *vel, int64_t n, int ni, int nj, int nk,
                    float *u, float *v, float *w,
                    float *u_old, float *v_old, float *w_old,
                    float *wu, float *wv, float *ww, uint8_t *fluid_out) {
-    // fluid cells (fluidsimulation.cpp:2010-2017)
+    step_impl(pos, vel, n, ni, nj, nk, dx, dt, body_force, pic_ratio, stencil, false, nullptr,
+              0, u, v, w, u_old, v_old, w_old, wu, wv, ww, fluid_out);
+}
+
+// The step with FluidSimulation's face fill and extrapolation (snapshot
+// extrapolated, body force, field extrapolated: fluidsimulation.cpp:3304-3334
+// without the pressure solve) using the reference's own
+// MACVelocityField::extrapolateVelocityField and FluidMaterialGrid.
+void ref_flip_step_fs(float *pos, float *vel, int64_t n, int ni, int nj, int nk, double dx,
+                      double dt, const float body_force[3], double pic_ratio, int stencil,
+                      const uint8_t *solid, int layers, float *u, float *v, float *w,
+                      float *u_old, float *v_old, float *w_old, uint8_t *mat_out) {
+    step_impl(pos, vel, n, ni, nj, nk, dx, dt, body_force, pic_ratio, stencil, true, solid,
+              layers, u, v, w, u_old, v_old, w_old, nullptr, nullptr, nullptr, mat_out);
+}
+
+// MACVelocityField::extrapolateVelocityField alone, on the given field and
+// cell codes (0 air, 1 fluid, 2 solid), in place.
+void ref_extrapolate(float *u, float *v, float *w, int ni, int nj, int nk, double dx,
+                     const uint8_t *mat_codes, int layers) {
     FluidMaterialGrid mat(ni, nj, nk);
+    for (int k = 0; k < nk; k++)
+        for (int j = 0; j < nj; j++)
+            for (int i = 0; i < ni; i++) {
+                uint8_t m = mat_codes[(int64_t)i + (int64_t)ni * ((int64_t)j + (int64_t)nj * k)];
+                if (m == 1) mat.setFluid(i, j, k);
+                else if (m == 2) mat.setSolid(i, j, k);
+            }
+    MACVelocityField mac(ni, nj, nk, dx);
+    load_mac(mac, u, v, w);
+    mac.extrapolateVelocityField(mat, layers);
+    read_array(mac.getArray3dU(), u, false);
+    read_array(mac.getArray3dV(), v, false);
+    read_array(mac.getArray3dW(), w, false);
+}
+
+}  // extern "C"
+
+namespace {
+
+void step_impl(float *pos, float *vel, int64_t n, int ni, int nj, int nk, double dx, double dt,
+               const float body_force[3], double pic_ratio, int stencil, bool fs,
+               const uint8_t *solid, int layers, float *u, float *v, float *w, float *u_old,
+               float *v_old, float *w_old, float *wu, float *wv, float *ww, uint8_t *fluid_out) {
+    // solid cells, then fluid cells (fluidsimulation.cpp:1990-2017)
+    FluidMaterialGrid mat(ni, nj, nk);
+    if (solid)
+        for (int k = 0; k < nk; k++)
+            for (int j = 0; j < nj; j++)
+                for (int i = 0; i < ni; i++)
+                    if (solid[(int64_t)i + (int64_t)ni * ((int64_t)j + (int64_t)nj * k)])
+                        mat.setSolid(i, j, k);
     for (int64_t p = 0; p < n; p++) {
         vmath::vec3 q(pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]);
         GridIndex g = Grid3d::positionToGridIndex(q, dx);
-        if (Grid3d::isGridIndexInRange(g, ni, nj, nk)) mat.setFluid(g);
+        if (Grid3d::isGridIndexInRange(g, ni, nj, nk) && !mat.isCellSolid(g)) mat.setFluid(g);
     }
 
     // P2G per component (fluidsimulation.cpp:2526-2586)
@@ -169,15 +288,20 @@ void ref_flip_step(float *pos, float *vel, int64_t n, int ni, int nj, int nk,
         }
         grid.applyWeightField();
         Array3d<float> *sf = grid.getPointerToScalarField();
-        for (int k = 0; k < field->depth; k++)
-            for (int j = 0; j < field->height; j++)
-                for (int i = 0; i < field->width; i++)
-                    field->set(i, j, k, sf->get(i, j, k));
+        if (fs) {
+            fill_face_values(mac, mat, c, *sf, *grid.getPointerToWeightField());
+        } else {
+            for (int k = 0; k < field->depth; k++)
+                for (int j = 0; j < field->height; j++)
+                    for (int i = 0; i < field->width; i++)
+                        field->set(i, j, k, sf->get(i, j, k));
+        }
         if (wouts[c]) read_array(grid.getPointerToWeightField(), wouts[c], false);
     }
 
-    // snapshot (fluidsimulation.cpp:3306)
+    // snapshot (fluidsimulation.cpp:3306), extrapolated in the epilogue form (:3307)
     MACVelocityField saved = mac;
+    if (fs) saved.extrapolateVelocityField(mat, layers);
 
     // constant body force (fluidsimulation.cpp:2766-2805)
     vmath::vec3 bf(body_force[0], body_force[1], body_force[2]);
@@ -196,6 +320,9 @@ void ref_flip_step(float *pos, float *vel, int64_t n, int ni, int nj, int nk,
             for (int j = 0; j < nj; j++)
                 for (int i = 0; i < ni; i++)
                     if (mat.isFaceBorderingFluidW(i, j, k)) mac.addW(i, j, k, bf.z * dt);
+
+    // the field is extrapolated after the (omitted) pressure solve (:3334)
+    if (fs) mac.extrapolateVelocityField(mat, layers);
 
     // PIC/FLIP (fluidsimulation.cpp:3104-3128) then RK4 (3181-3207)
     for (int64_t p = 0; p < n; p++) {
@@ -226,7 +353,7 @@ void ref_flip_step(float *pos, float *vel, int64_t n, int ni, int nj, int nk,
             for (int j = 0; j < nj; j++)
                 for (int i = 0; i < ni; i++)
                     fluid_out[(int64_t)i + (int64_t)ni * ((int64_t)j + (int64_t)nj * k)] =
-                        mat.isCellFluid(i, j, k) ? 1 : 0;
+                        mat.isCellFluid(i, j, k) ? 1 : (mat.isCellSolid(i, j, k) ? 2 : 0);
 }
 
-}  // extern "C"
+}  // namespace

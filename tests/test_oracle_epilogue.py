@@ -1,0 +1,133 @@
+"""Pins the oracle's FluidSimulation epilogue (gfo_flip_step_fs: the face
+fill of fluidsimulation.cpp:2597-2730 and MACVelocityField::
+extrapolateVelocityField, macvelocityfield.cpp:577-798):
+
+* the extrapolation bit for bit against the reference's own compiled
+  MACVelocityField / FluidMaterialGrid (oracle/_ref), on random fields and
+  cell codes with solid cells;
+* the whole step bit for bit against the reference classes driven stage by
+  stage (oracle/ref_harness.cpp::ref_flip_step_fs);
+* the face fill, which lives in FluidSimulation (not compilable here), by
+  closed-form cases, including the U-versus-V/W difference of its neighbour
+  test (:2630 vs :2675,2720);
+* extrapolation closed forms: a uniform fluid field spreads unchanged for
+  exactly `layers` cell layers, nothing beyond, nothing through solids.
+"""
+import numpy as np
+import pytest
+
+from gridfluidsim3d_amd import scenes
+
+TRILINEAR, TRICUBIC = 0, 1
+
+
+def mac_shapes(dims):
+    ni, nj, nk = dims
+    return [(nk, nj, ni + 1), (nk, nj + 1, ni), (nk + 1, nj, ni)]
+
+
+def random_codes(rng, dims, p_fluid=0.25, p_solid=0.08, border=True):
+    ni, nj, nk = dims
+    m = (rng.uniform(size=(nk, nj, ni)) < p_fluid).astype(np.uint8)
+    m[rng.uniform(size=m.shape) < p_solid] = 2
+    if border:
+        m[0, :, :] = m[-1, :, :] = 2
+        m[:, 0, :] = m[:, -1, :] = 2
+        m[:, :, 0] = m[:, :, -1] = 2
+    return m
+
+
+def border_solid(dims):
+    ni, nj, nk = dims
+    s = np.zeros((nk, nj, ni), np.uint8)
+    s[0, :, :] = s[-1, :, :] = 1
+    s[:, 0, :] = s[:, -1, :] = 1
+    s[:, :, 0] = s[:, :, -1] = 1
+    return s
+
+
+@pytest.mark.parametrize("layers", [1, 3, 7])
+@pytest.mark.parametrize("seed", [1, 2])
+def test_extrapolate_matches_reference(orc, ref, layers, seed):
+    if ref is None:
+        pytest.skip("reference build absent")
+    rng = np.random.default_rng(seed)
+    dims = (13, 11, 9)
+    codes = random_codes(rng, dims, border=seed == 1)
+    u, v, w = (rng.normal(size=s).astype(np.float32) for s in mac_shapes(dims))
+    a = orc.extrapolate(u, v, w, dims, 0.1, codes, layers)
+    b = ref.extrapolate(u, v, w, dims, 0.1, codes, layers)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("solid", [False, True])
+@pytest.mark.parametrize("stencil", [TRILINEAR, TRICUBIC])
+def test_flip_step_fs_matches_reference(orc, ref, solid, stencil):
+    if ref is None:
+        pytest.skip("reference build absent")
+    sc = scenes.random_blobs(71, (18, 15, 13), 0.1, ppc=4, amplitude=1.5, fill=(0.15, 0.3))
+    sol = border_solid(sc.dims) if solid else None
+    if solid:   # the reference keeps particles out of solid cells
+        c = np.floor(sc.pos / sc.dx).astype(int)
+        keep = sol[c[:, 2].clip(0, 12), c[:, 1].clip(0, 14), c[:, 0].clip(0, 17)] == 0
+        pos, vel = sc.pos[keep], sc.vel[keep]
+    else:
+        pos, vel = sc.pos, sc.vel
+    a = orc.flip_step_fs(pos, vel, sc.dims, sc.dx, 1 / 60, (0.0, -9.8, 0.0), 0.05, stencil,
+                         sol, 7)
+    b = ref.flip_step_fs(pos, vel, sc.dims, sc.dx, 1 / 60, (0.0, -9.8, 0.0), 0.05, stencil,
+                         sol, 7)
+    for key in ("pos", "vel", "u", "v", "w", "u_old", "v_old", "w_old", "mat"):
+        assert np.array_equal(a[key], b[key]), key
+    assert (a["mat"] == 1).any()
+
+
+def test_fill_closed_forms(orc):
+    """One fluid cell in a 4x3x3 grid; the faces around it."""
+    dims = (4, 3, 3)
+    ni, nj, nk = dims
+    mat = np.zeros((nk, nj, ni), np.uint8)
+    mat[1, 1, 1] = 1                     # cell (i,j,k) = (1,1,1)
+    # U: faces (1,1,1) and (2,1,1) border the fluid cell
+    ug = np.zeros((nk, nj, ni + 1), np.float32)
+    W = np.zeros_like(ug)
+    ug[1, 1, 1], W[1, 1, 1] = 3.0, 0.5               # set: kept
+    ug[1, 1, 3], W[1, 1, 3] = 7.0, 0.5               # set but not bordering fluid: cleared
+    # face (2,1,1) unset; neighbours: (3,1,1) nonzero with W tiny (counts for U),
+    # (2,0,1) zero but set (does not count for U), (1,1,1) = 3 (counts)
+    ug[1, 1, 3], W[1, 1, 3] = 7.0, 1e-10
+    ug[1, 0, 2], W[1, 0, 2] = 0.0, 0.9
+    out = orc.fill_component(ug, W, 0, dims, mat)
+    assert out[1, 1, 1] == 3.0
+    assert out[1, 1, 2] == np.float32((3.0 + 7.0) / 2)
+    assert out[1, 1, 3] == 0.0
+    assert np.count_nonzero(out) == 2
+    # V: the same pattern on the v faces (1,1,1) and (1,2,1); isValueSet rule
+    vg = np.zeros((nk, nj + 1, ni), np.float32)
+    Wv = np.zeros_like(vg)
+    vg[1, 1, 1], Wv[1, 1, 1] = 3.0, 0.5
+    vg[1, 3, 1], Wv[1, 3, 1] = 7.0, 1e-10            # nonzero, not set: ignored for V
+    vg[1, 2, 0], Wv[1, 2, 0] = 0.0, 0.9              # zero but set: counts for V
+    out = orc.fill_component(vg, Wv, 1, dims, mat)
+    assert out[1, 1, 1] == 3.0
+    assert out[1, 2, 1] == np.float32((3.0 + 0.0) / 2)
+    assert np.count_nonzero(out) == 2
+
+
+def test_extrapolate_closed_forms(orc):
+    """A uniform field on the faces of a fluid cell spreads, unchanged, to
+    faces within `layers` cell layers and no further; solids block it."""
+    dims = (9, 5, 5)
+    ni, nj, nk = dims
+    mat = np.zeros((nk, nj, ni), np.uint8)
+    mat[2, 2, 1] = 1
+    mat[:, :, 5] = 2                       # a solid wall at i = 5
+    u, v, w = (np.full(s, 2.5, np.float32) for s in mac_shapes(dims))
+    for layers in (1, 2, 3):
+        a, b, c = orc.extrapolate(u, v, w, dims, 0.1, mat, layers)
+        # v faces are at (i, j+-1/2, k): reachable cells i <= 1 + layers
+        for i in range(ni):
+            reach = abs(i - 1) <= layers and i < 5
+            assert (b[2, 2, i] == 2.5) == reach, (layers, i)
+        assert (a[:, :, 6:] == 0).all() and (b[:, :, 5:] == 0).all()

@@ -38,6 +38,7 @@ enum Slot {
     S_OLD0, S_XS0, S_XS1, S_XS2, S_VS0, S_VS1, S_VS2, S_DENSE, S_DENSEN,
     S_D_POS, S_D_VAL, S_D_S, S_D_W, S_D_U, S_D_V, S_D_WW, S_D_OUT,
     S_B_GRID, S_B_OLD, S_B_WGT, S_B_LG, S_B_XO, S_B_VO, S_B_FX, S_B_MAX,
+    S_E_N, S_E_W, S_E_LAYER,
     NSLOT
 };
 
@@ -285,6 +286,12 @@ gfsx::SceneDesc scene_desc(const gfsx_scene *sc, const gfsx_step_params *pr, con
     req(pr->stencil == GFSX_TRILINEAR || pr->stencil == GFSX_TRICUBIC, "unknown stencil");
     req(pr->order == GFSX_ORDER_SORTED || pr->order == GFSX_ORDER_KEEP, "unknown output order");
     req(std::isfinite(pr->dt), "dt must be finite");
+    req(pr->epilogue == GFSX_EPILOGUE_NONE || pr->epilogue == GFSX_EPILOGUE_FLUIDSIM,
+        "unknown epilogue");
+    req(pr->extrapolation_layers >= 0 && pr->extrapolation_layers <= 100,
+        "extrapolation_layers must be 0..100");
+    req(!sc->solid || pr->epilogue == GFSX_EPILOGUE_FLUIDSIM,
+        "solid cells are used by the FluidSimulation epilogue only");
     req(sc->u && sc->v && sc->w, "null grid output");
     if (sc->n > 0) {
         req(sc->x_in && sc->y_in && sc->z_in && sc->vx_in && sc->vy_in && sc->vz_in,
@@ -348,6 +355,9 @@ std::vector<gfsx::SceneDesc> flip_step_impl(gfsx_ctx *ctx, const gfsx_scene *sce
         if (params[i].stencil != params[0].stencil || params[i].order != params[0].order)
             throw gfsx::Error(std::string(who) + ": every scene needs the same stencil and order");
     }
+    const bool fs = params[0].epilogue == GFSX_EPILOGUE_FLUIDSIM;
+    if (fs && nscenes != 1)
+        throw gfsx::Error(std::string(who) + ": the FluidSimulation epilogue takes one scene");
     gfsx::layout_scenes(s.scenes.data(), nscenes, s.tot);
     require(s.tot.n < ((int64_t)1 << 32) - 1, "gfsx: too many particles in one step");
     require(s.tot.nbin < ((int64_t)1 << 32) - 2, "gfsx: too many cells in one step");
@@ -377,6 +387,38 @@ std::vector<gfsx::SceneDesc> flip_step_impl(gfsx_ctx *ctx, const gfsx_scene *sce
         so.v[c] = s.vs[c];
     }
     for (auto &d : s.scenes) d.p2g = gfsx::p2g_view(d, so);
+    gfsx::FsEpilogue epi;
+    if (fs) {
+        // P2G writes the bare normalised values and weights; the epilogue
+        // kernels fill, snapshot, add the body force and extrapolate
+        gfsx::SceneDesc &d = s.scenes[0];
+        int64_t nf[3], tot = 0;
+        for (int c = 0; c < 3; c++) tot += (nf[c] = mac_count(c, d.ni, d.nj, d.nk));
+        float *nbuf = ctx->f32(S_E_N, tot);
+        float *wbuf = ctx->f32(S_E_W, tot);
+        std::memset(&epi, 0, sizeof epi);
+        epi.ni = d.ni;
+        epi.nj = d.nj;
+        epi.nk = d.nk;
+        epi.solid = scenes[0].solid;
+        epi.layer = (int8_t *)ctx->get(S_E_LAYER, (size_t)d.ni * d.nj * d.nk);
+        epi.layers = params[0].extrapolation_layers > 0 ? params[0].extrapolation_layers : 7;
+        int64_t at = 0;
+        for (int c = 0; c < 3; c++) {
+            float *w = d.wgt[c] ? d.wgt[c] : wbuf + at;
+            epi.n[c] = nbuf + at;
+            epi.w[c] = w;
+            epi.grid[c] = d.grid[c];
+            epi.old[c] = d.grid_old[c];
+            epi.grav_delta[c] = d.grav_delta[c];
+            epi.grav_on[c] = d.grav_on[c];
+            d.p2g.grid[c] = nbuf + at;
+            d.p2g.grid_old[c] = nullptr;
+            d.p2g.wgt[c] = w;
+            d.p2g.grav_on[c] = 0;
+            at += nf[c];
+        }
+    }
     s.stencil = params[0].stencil;
     s.keep_order = params[0].order == GFSX_ORDER_KEEP;
     s.p2g_variant = ctx->p2g_variant;
@@ -386,6 +428,11 @@ std::vector<gfsx::SceneDesc> flip_step_impl(gfsx_ctx *ctx, const gfsx_scene *sce
     s.scenes_dev = sb.scenes;
     // P2G brick flags, one byte per 4x4x4-cell brick of every scene
     sb.brick_flag = (uint8_t *)ctx->get(S_BFLAG, (size_t)s.tot.np2g);
+    if (fs) {
+        epi.start = sb.start;
+        for (int a = 0; a < 3; a++) epi.b[a] = s.scenes[0].bg.b[a];
+        s.epi = &epi;
+    }
     gfsx::flip_step(s, sb, ctx->stream, ctx->timing ? ctx->next_events() : nullptr);
     return s.scenes;
 }
@@ -666,6 +713,8 @@ void gfsx_flip_step_backward(gfsx_ctx *ctx, const gfsx_scene *sc,
         req(pr->stencil == GFSX_TRILINEAR, "the adjoint covers the trilinear stencil only");
         req(pr->order == GFSX_ORDER_KEEP,
             "the adjoints refer to particles in input order (GFSX_ORDER_KEEP)");
+        req(pr->epilogue == GFSX_EPILOGUE_NONE,
+            "the adjoint covers the transfer without the FluidSimulation epilogue");
         check_grid(sc->ni, sc->nj, sc->nk, sc->dx);
         check_particles(sc->n, sc->x_in);
         const int64_t n = sc->n;

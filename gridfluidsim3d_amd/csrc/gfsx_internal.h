@@ -226,11 +226,30 @@ struct AdjointSort {
     uint32_t *maxbits;
 };
 
+// FluidSimulation's face fill and extrapolation around the transfer (one
+// scene): the sort's bin starts give the fluid cells, n and w are the P2G
+// gather's normalised values and weights, grid and old receive the field
+// (with the body force) and the snapshot, both extrapolated.
+struct FsEpilogue {
+    int ni, nj, nk;
+    const uint32_t *start;      // bin starts of the sort (bins (n+2)^3, pad 1)
+    int b[3];                   // bins per axis
+    const uint8_t *solid;       // optional, ni*nj*nk, nonzero = solid
+    int8_t *layer;              // workspace, ni*nj*nk
+    const float *n[3], *w[3];
+    float *grid[3], *old[3];
+    float grav_delta[3];
+    int grav_on[3];
+    int layers;
+};
+
 // ---- launches (implemented in gfsx_kernels.hip) ------------------------
 
 void fill_u32(uint32_t *p, uint32_t value, int64_t count, hipStream_t s);
 void fill_f32(float *p, float value, int64_t count, hipStream_t s);
 void fill_u8(uint8_t *p, uint8_t value, int64_t count, hipStream_t s);
+// gfsx_epilogue.hip
+void fs_epilogue(const FsEpilogue &e, hipStream_t s);
 // gfsx_adjoint.hip
 // so == nullptr selects one thread per particle in input order with every
 // grid adjoint added straight to global memory; otherwise bricks of the
@@ -320,6 +339,7 @@ struct StepBatch {
     int keep_order;
     int p2g_variant;   // 0: brick-staged gather, 1: direct global gather (one scene)
     int g2p_variant;   // 0: brick-staged LDS boxes, 1: direct global sampling (one scene)
+    const FsEpilogue *epi = nullptr;   // FluidSimulation epilogue after P2G (one scene)
 };
 
 // The fused plain step of every scene of s, in one launch per kernel.

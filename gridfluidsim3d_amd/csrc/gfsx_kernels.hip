@@ -2224,6 +2224,7 @@ void flip_step(const StepBatch &s, const SortBuffers &sb, hipStream_t st, hipEve
                            (const uint32_t *)sb.start, (const uint32_t *)nullptr);
     }
     GFSX_LAUNCH_CHECK();
+    if (s.epi) fs_epilogue(*s.epi, st);
     if (ev) GFSX_HIP(hipEventRecord(ev[2], st));
 
     if (s.tot.n <= 0) {

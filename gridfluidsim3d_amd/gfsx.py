@@ -26,6 +26,8 @@ TRILINEAR = 0
 TRICUBIC = 1
 ORDER_SORTED = 0
 ORDER_KEEP = 1
+EPILOGUE_NONE = 0
+EPILOGUE_FLUIDSIM = 1
 
 _fp = POINTER(c_float)
 _u32p = POINTER(ctypes.c_uint32)
@@ -45,6 +47,7 @@ class Scene(ctypes.Structure):
         ("u", c_void_p), ("v", c_void_p), ("w", c_void_p),
         ("u_old", c_void_p), ("v_old", c_void_p), ("w_old", c_void_p),
         ("wu", c_void_p), ("wv", c_void_p), ("ww", c_void_p),
+        ("solid", c_void_p),
     ]
 
 
@@ -56,6 +59,8 @@ class StepParams(ctypes.Structure):
         ("pic_ratio", c_double),
         ("stencil", c_int),
         ("order", c_int),
+        ("epilogue", c_int),
+        ("extrapolation_layers", c_int),
     ]
 
 
@@ -352,24 +357,34 @@ class Context:
         if st.weights is not None:
             for nm, t, s in zip(("wu", "wv", "ww"), st.weights, shapes):
                 setattr(sc, nm, _dptr(t, nm, f32, int(np.prod(s))))
+        if st.solid is not None:
+            sc.solid = _dptr(st.solid, "solid", torch.uint8, int(np.prod(st.dims)))
         return sc
 
     @staticmethod
-    def _params(dt, body_force, pic_ratio, stencil, order):
+    def _params(dt, body_force, pic_ratio, stencil, order, epilogue=EPILOGUE_NONE,
+                extrapolation_layers=0):
         pr = StepParams()
         pr.dt = float(dt)
         pr.body_force[0], pr.body_force[1], pr.body_force[2] = [float(b) for b in body_force]
         pr.pic_ratio = float(pic_ratio)
         pr.stencil = int(stencil)
         pr.order = int(order)
+        pr.epilogue = int(epilogue)
+        pr.extrapolation_layers = int(extrapolation_layers)
         return pr
 
     def flip_step(self, state, dt, body_force=(0.0, -9.8, 0.0), pic_ratio=0.05,
-                  stencil=TRILINEAR, order=ORDER_SORTED):
-        """Advance a ``FlipState`` by one plain step (in its double buffers)."""
+                  stencil=TRILINEAR, order=ORDER_SORTED, epilogue=EPILOGUE_NONE,
+                  extrapolation_layers=0):
+        """Advance a ``FlipState`` by one plain step (in its double buffers).
+        ``epilogue`` EPILOGUE_FLUIDSIM adds FluidSimulation's face fill and
+        velocity extrapolation (``extrapolation_layers``, 0 = 7) around the
+        transfer, with the state's solid cells."""
         st = state
         sc = self._scene_of(st)
-        pr = self._params(dt, body_force, pic_ratio, stencil, order)
+        pr = self._params(dt, body_force, pic_ratio, stencil, order, epilogue,
+                          extrapolation_layers)
         self.use_torch_stream()
         f = pb.lib_func("gfsx_flip_step", [c_void_p, POINTER(Scene), POINTER(StepParams)])
         pb.execute_lib_func(f, [self(), ctypes.byref(sc), ctypes.byref(pr)])
@@ -509,7 +524,7 @@ class FlipState:
     one scene, on one GPU."""
 
     def __init__(self, pos, vel, dims, dx, device="cuda", track_ids=False,
-                 keep_old=False, keep_weights=False):
+                 keep_old=False, keep_weights=False, solid=None):
         import torch
         pos = torch.as_tensor(pos, dtype=torch.float32)
         vel = torch.as_tensor(vel, dtype=torch.float32)
@@ -528,6 +543,9 @@ class FlipState:
                          if keep_old else None)
         self.weights = ([torch.zeros(s, dtype=torch.float32, device=dev) for s in shapes]
                         if keep_weights else None)
+        # solid cells (nk, nj, ni) uint8, used by the FluidSimulation epilogue
+        self.solid = None if solid is None else torch.as_tensor(
+            np.ascontiguousarray(solid, np.uint8)).reshape(-1).to(dev)
         self.ids = None
         if track_ids:
             self.ids = [torch.arange(self.n, dtype=torch.int32, device=dev),

@@ -151,6 +151,8 @@ typedef struct gfsx_scene {
     float *u, *v, *w;
     float *u_old, *v_old, *w_old; /* optional */
     float *wu, *wv, *ww;          /* optional */
+    const uint8_t *solid;         /* optional: ni*nj*nk cells, nonzero = solid
+                                     (GFSX_EPILOGUE_FLUIDSIM only) */
 } gfsx_scene;
 
 typedef struct gfsx_step_params {
@@ -159,7 +161,23 @@ typedef struct gfsx_step_params {
     double pic_ratio;    /* 0.05 in the reference (fluidsimulation.h:1161) */
     int stencil;         /* GFSX_TRILINEAR or GFSX_TRICUBIC */
     int order;           /* GFSX_ORDER_SORTED or GFSX_ORDER_KEEP */
+    int epilogue;        /* GFSX_EPILOGUE_NONE or GFSX_EPILOGUE_FLUIDSIM */
+    int extrapolation_layers; /* GFSX_EPILOGUE_FLUIDSIM: layers (0 = 7) */
 } gfsx_step_params;
+
+/* Epilogues of the P2G stage.  NONE: the transfer alone (normalised values,
+ * snapshot, body force on faces bordering fluid).  FLUIDSIM: the stages
+ * FluidSimulation runs around it, without the pressure solve: the face fill
+ * of _advectVelocityFieldU/V/W (fluidsimulation.cpp:2597-2730; faces
+ * bordering fluid keep the P2G value where the weight exceeds 1e-9, else the
+ * average of their 26 neighbours; other faces 0), the snapshot and its
+ * extrapolation, the body force, and the extrapolation of the field
+ * (MACVelocityField::extrapolateVelocityField, macvelocityfield.cpp:577-798,
+ * with the scene's solid cells; fluidsimulation.cpp:3304-3334).  One scene
+ * per call (not batched); u_old.. receive the extrapolated snapshot, wu..
+ * the P2G weights. */
+#define GFSX_EPILOGUE_NONE 0
+#define GFSX_EPILOGUE_FLUIDSIM 1
 
 void gfsx_flip_step(gfsx_ctx *ctx, const gfsx_scene *scene,
                     const gfsx_step_params *params, int *err);

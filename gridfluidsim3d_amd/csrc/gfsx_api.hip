@@ -403,6 +403,13 @@ std::vector<gfsx::SceneDesc> flip_step_impl(gfsx_ctx *ctx, const gfsx_scene *sce
         epi.solid = scenes[0].solid;
         epi.layer = (int8_t *)ctx->get(S_E_LAYER, (size_t)d.ni * d.nj * d.nk);
         epi.layers = params[0].extrapolation_layers > 0 ? params[0].extrapolation_layers : 7;
+        epi.dx = d.dx;
+        epi.np = d.n;
+        for (int c = 0; c < 3; c++) {
+            // the input of output index i: itself (keep) or sorted slot i
+            epi.x0[c] = params[0].order == GFSX_ORDER_KEEP ? d.xin[c] : s.xs[c];
+            epi.x1[c] = d.xout[c];
+        }
         int64_t at = 0;
         for (int c = 0; c < 3; c++) {
             float *w = d.wgt[c] ? d.wgt[c] : wbuf + at;

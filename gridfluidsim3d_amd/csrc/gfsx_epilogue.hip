@@ -179,7 +179,151 @@ __global__ __launch_bounds__(kBlock) void k_fs_extrapolate(FsEpilogue e, int64_t
     old[o] = so == 0.0 ? 0.0f : (float)(so / cnt);
 }
 
+// ---- collision fix (fluidsimulation.cpp:3145-3207, collision.cpp:303-449)
+// FP32 vec3 and FP64 scalar maths as the reference evaluates them; products
+// and sums are rounded one by one (no fused multiply-add), as on the CPU.
+
+__device__ __forceinline__ bool solid_at(const FsEpilogue &e, int i, int j, int k) {
+    return (unsigned)i < (unsigned)e.ni && (unsigned)j < (unsigned)e.nj &&
+           (unsigned)k < (unsigned)e.nk &&
+           e.solid[(uint32_t)i + (uint32_t)e.ni * ((uint32_t)j + (uint32_t)e.nj * (uint32_t)k)] != 0;
+}
+
+// Grid3d::positionToGridIndex then the solid test; out of the grid is not solid
+__device__ __forceinline__ bool point_in_solid(const FsEpilogue &e, float x, float y, float z) {
+    const double inv = 1.0 / e.dx;
+    const double fi = floor(__dmul_rn((double)x, inv));
+    const double fj = floor(__dmul_rn((double)y, inv));
+    const double fk = floor(__dmul_rn((double)z, inv));
+    if (!(fi >= 0.0 && fj >= 0.0 && fk >= 0.0 && fi < e.ni && fj < e.nj && fk < e.nk)) return false;
+    return solid_at(e, (int)fi, (int)fj, (int)fk);
+}
+
+// Collision::getLineSegmentVoxelIntersection: the first solid voxel on p0 -> p1
+__device__ bool segment_voxel(const FsEpilogue &e, const float a[3], const float b[3], int vox[3]) {
+    const float invdx = (float)(1.0 / e.dx);
+    float p0[3], p1[3];
+    for (int c = 0; c < 3; c++) {
+        p0[c] = __fmul_rn(a[c], invdx);
+        p1[c] = __fmul_rn(b[c], invdx);
+    }
+    int g0[3], g1[3], sgn[3];
+    for (int c = 0; c < 3; c++) {
+        g0[c] = (int)floor((double)p0[c]);
+        g1[c] = (int)floor((double)p1[c]);
+        sgn[c] = g1[c] > g0[c] ? 1 : g1[c] < g0[c] ? -1 : 0;
+    }
+    double v[3];
+    for (int c = 0; c < 3; c++)
+        v[c] = p1[c] == p0[c] ? 1.0 : (double)__fsub_rn(p1[c], p0[c]);
+    const double vxvy = __dmul_rn(v[0], v[1]), vxvz = __dmul_rn(v[0], v[2]);
+    const double vyvz = __dmul_rn(v[1], v[2]);
+    const double gp0 = g0[0] + (g1[0] > g0[0] ? 1 : 0);
+    const double gp1 = g0[1] + (g1[1] > g0[1] ? 1 : 0);
+    const double gp2 = g0[2] + (g1[2] > g0[2] ? 1 : 0);
+    double errx = __dmul_rn(__dsub_rn(gp0, (double)p0[0]), vyvz);
+    double erry = __dmul_rn(__dsub_rn(gp1, (double)p0[1]), vxvz);
+    double errz = __dmul_rn(__dsub_rn(gp2, (double)p0[2]), vxvy);
+    const double derrx = __dmul_rn((double)sgn[0], vyvz);
+    const double derry = __dmul_rn((double)sgn[1], vxvz);
+    const double derrz = __dmul_rn((double)sgn[2], vxvy);
+    int gx = g0[0], gy = g0[1], gz = g0[2];
+    // the walk reaches the end voxel in |dgx| + |dgy| + |dgz| steps
+    const long long steps = llabs((long long)g1[0] - g0[0]) + llabs((long long)g1[1] - g0[1]) +
+                            llabs((long long)g1[2] - g0[2]) + 1;
+    for (long long it = 0; it <= steps; it++) {
+        if (solid_at(e, gx, gy, gz)) {
+            vox[0] = gx;
+            vox[1] = gy;
+            vox[2] = gz;
+            return true;
+        }
+        if (gx == g1[0] && gy == g1[1] && gz == g1[2]) break;
+        const double xr = fabs(errx), yr = fabs(erry), zr = fabs(errz);
+        if (sgn[0] != 0 && (sgn[1] == 0 || xr < yr) && (sgn[2] == 0 || xr < zr)) {
+            gx += sgn[0];
+            errx = __dadd_rn(errx, derrx);
+        } else if (sgn[1] != 0 && (sgn[2] == 0 || yr < zr)) {
+            gy += sgn[1];
+            erry = __dadd_rn(erry, derry);
+        } else if (sgn[2] != 0) {
+            gz += sgn[2];
+            errz = __dadd_rn(errz, derrz);
+        }
+    }
+    return false;
+}
+
+// Collision::rayIntersectsAABB, with the reference's test of dir.x where
+// dir.z is clamped
+__device__ bool ray_aabb(const float p0[3], const float d[3], const float bmin[3],
+                         const float bmax[3], float hit[3]) {
+    const double eps = 1e-10;
+    float dir[3] = {d[0], d[1], d[2]};
+    if (fabs((double)dir[0]) < eps) dir[0] = dir[0] < 0 ? (float)-eps : (float)eps;
+    if (fabs((double)dir[1]) < eps) dir[1] = dir[1] < 0 ? (float)-eps : (float)eps;
+    if (fabs((double)dir[0]) < eps) dir[2] = dir[2] < 0 ? (float)-eps : (float)eps;
+    float inv[3];
+    for (int c = 0; c < 3; c++) inv[c] = (float)(1.0 / (double)dir[c]);
+    double tmin = 0.0, tmax = 0.0;
+    for (int c = 0; c < 3; c++) {
+        const double t1 = (double)__fmul_rn(__fsub_rn(bmin[c], p0[c]), inv[c]);
+        const double t2 = (double)__fmul_rn(__fsub_rn(bmax[c], p0[c]), inv[c]);
+        if (c == 0) {
+            tmin = fmin(t1, t2);
+            tmax = fmax(t1, t2);
+        } else {
+            tmin = fmax(tmin, fmin(t1, t2));
+            tmax = fmin(tmax, fmax(t1, t2));
+        }
+    }
+    if (tmax > fmax(tmin, 0.0)) {
+        const float tf = (float)tmin;
+        for (int c = 0; c < 3; c++) hit[c] = __fadd_rn(p0[c], __fmul_rn(dir[c], tf));
+        return true;
+    }
+    return false;
+}
+
+__global__ __launch_bounds__(kBlock) void k_fs_collide(FsEpilogue e) {
+    const int64_t p = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    if (p >= e.np) return;
+    float b[3] = {e.x1[0][p], e.x1[1][p], e.x1[2][p]};
+    if (!point_in_solid(e, b[0], b[1], b[2])) return;
+    const float a[3] = {e.x0[0][p], e.x0[1][p], e.x0[2][p]};
+    float r[3] = {a[0], a[1], a[2]};
+    int vox[3];
+    if (segment_voxel(e, a, b, vox)) {
+        float d[3];
+        for (int c = 0; c < 3; c++) d[c] = __fsub_rn(b[c], a[c]);
+        const float lsq = __fadd_rn(__fadd_rn(__fmul_rn(d[0], d[0]), __fmul_rn(d[1], d[1])),
+                                    __fmul_rn(d[2], d[2]));
+        const float len = (float)sqrt((double)lsq);
+        const float inv = (float)(1.0 / (double)len);
+        float rn[3], bmin[3], bmax[3], hit[3];
+        for (int c = 0; c < 3; c++) {
+            rn[c] = __fmul_rn(d[c], inv);
+            bmin[c] = (float)__dmul_rn((double)(float)vox[c], e.dx);
+            bmax[c] = __fadd_rn(bmin[c], (float)e.dx);
+        }
+        if (ray_aabb(a, rn, bmin, bmax, hit)) {
+            const float back = (float)(0.05 * e.dx);
+            float q[3];
+            for (int c = 0; c < 3; c++) q[c] = __fsub_rn(hit[c], __fmul_rn(rn[c], back));
+            if (!point_in_solid(e, q[0], q[1], q[2]))
+                for (int c = 0; c < 3; c++) r[c] = q[c];
+        }
+    }
+    for (int c = 0; c < 3; c++) e.x1[c][p] = r[c];
+}
+
 }  // namespace
+
+void fs_collide(const FsEpilogue &e, hipStream_t s) {
+    if (e.np <= 0 || !e.solid) return;
+    hipLaunchKernelGGL(k_fs_collide, dim3(blocks_for(e.np)), dim3(kBlock), 0, s, e);
+    GFSX_LAUNCH_CHECK();
+}
 
 void fs_epilogue(const FsEpilogue &e, hipStream_t s) {
     const int64_t ncell = (int64_t)e.ni * e.nj * e.nk;

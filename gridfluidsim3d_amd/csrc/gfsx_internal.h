@@ -241,6 +241,12 @@ struct FsEpilogue {
     float grav_delta[3];
     int grav_on[3];
     int layers;
+    double dx;
+    // collision fix after RK4 (with solid cells): the step's outputs x1 and
+    // the same particles' inputs x0 (by output index), np particles
+    int64_t np;
+    const float *x0[3];
+    float *x1[3];
 };
 
 // ---- launches (implemented in gfsx_kernels.hip) ------------------------
@@ -250,6 +256,7 @@ void fill_f32(float *p, float value, int64_t count, hipStream_t s);
 void fill_u8(uint8_t *p, uint8_t value, int64_t count, hipStream_t s);
 // gfsx_epilogue.hip
 void fs_epilogue(const FsEpilogue &e, hipStream_t s);
+void fs_collide(const FsEpilogue &e, hipStream_t s);
 // gfsx_adjoint.hip
 // so == nullptr selects one thread per particle in input order with every
 // grid adjoint added straight to global memory; otherwise bricks of the

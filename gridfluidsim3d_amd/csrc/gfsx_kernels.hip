@@ -2251,6 +2251,7 @@ void flip_step(const StepBatch &s, const SortBuffers &sb, hipStream_t st, hipEve
                                g, (const uint32_t *)sb.perm);
     }
     GFSX_LAUNCH_CHECK();
+    if (s.epi && s.epi->solid) fs_collide(*s.epi, st);
     if (ev) GFSX_HIP(hipEventRecord(ev[3], st));
 }
 

@@ -617,6 +617,108 @@ static void extrapolate(float *g[3], const matgrid *m, int layers) {
     free(layer);
 }
 
+/* Collision::getLineSegmentVoxelIntersection (collision.cpp:303-394): the
+   first solid voxel along the segment p0 -> p1, walking the voxels with the
+   reference's error accumulators; the points are scaled by (float)(1/dx) in
+   FP32 as vec3 *= float does. */
+static int segment_voxel(v3f p0, v3f p1, double dx, const uint8_t *mat, int ni, int nj,
+                         int nk, int vox[3]) {
+    float invdx = (float)(1.0 / dx);
+    p0.x *= invdx; p0.y *= invdx; p0.z *= invdx;
+    p1.x *= invdx; p1.y *= invdx; p1.z *= invdx;
+    int gx0 = (int)floor(p0.x), gy0 = (int)floor(p0.y), gz0 = (int)floor(p0.z);
+    int gx1 = (int)floor(p1.x), gy1 = (int)floor(p1.y), gz1 = (int)floor(p1.z);
+    int sx = gx1 > gx0 ? 1 : gx1 < gx0 ? -1 : 0;
+    int sy = gy1 > gy0 ? 1 : gy1 < gy0 ? -1 : 0;
+    int sz = gz1 > gz0 ? 1 : gz1 < gz0 ? -1 : 0;
+    int gx = gx0, gy = gy0, gz = gz0;
+    double gxp = gx0 + (gx1 > gx0 ? 1 : 0);
+    double gyp = gy0 + (gy1 > gy0 ? 1 : 0);
+    double gzp = gz0 + (gz1 > gz0 ? 1 : 0);
+    double vx = p1.x == p0.x ? 1 : (double)(p1.x - p0.x);
+    double vy = p1.y == p0.y ? 1 : (double)(p1.y - p0.y);
+    double vz = p1.z == p0.z ? 1 : (double)(p1.z - p0.z);
+    double vxvy = vx * vy, vxvz = vx * vz, vyvz = vy * vz;
+    double errx = (gxp - p0.x) * vyvz;
+    double erry = (gyp - p0.y) * vxvz;
+    double errz = (gzp - p0.z) * vxvy;
+    double derrx = sx * vyvz, derry = sy * vxvz, derrz = sz * vxvy;
+    /* the walk reaches the end voxel in |dgx| + |dgy| + |dgz| steps */
+    long maxsteps = labs((long)gx1 - gx0) + labs((long)gy1 - gy0) + labs((long)gz1 - gz0) + 1;
+    for (long it = 0; it <= maxsteps; it++) {
+        if (in_range(gx, gy, gz, ni, nj, nk) && mat[flat(gx, gy, gz, ni, nj)] == MAT_SOLID) {
+            vox[0] = gx; vox[1] = gy; vox[2] = gz;
+            return 1;
+        }
+        if (gx == gx1 && gy == gy1 && gz == gz1) break;
+        double xr = fabs(errx), yr = fabs(erry), zr = fabs(errz);
+        if (sx != 0 && (sy == 0 || xr < yr) && (sz == 0 || xr < zr)) {
+            gx += sx;
+            errx += derrx;
+        } else if (sy != 0 && (sz == 0 || yr < zr)) {
+            gy += sy;
+            erry += derry;
+        } else if (sz != 0) {
+            gz += sz;
+            errz += derrz;
+        }
+    }
+    return 0;
+}
+
+/* Collision::rayIntersectsAABB (collision.cpp:404-449), FP32 vec3 maths
+   and FP64 slab bounds, with the reference's test of dir.x where dir.z is
+   clamped (:414-416). */
+static int ray_aabb(v3f p0, v3f dir, v3f bmin, v3f bmax, v3f *hit) {
+    double eps = 1e-10;
+    if (fabs(dir.x) < eps) dir.x = dir.x < 0 ? (float)-eps : (float)eps;
+    if (fabs(dir.y) < eps) dir.y = dir.y < 0 ? (float)-eps : (float)eps;
+    if (fabs(dir.x) < eps) dir.z = dir.z < 0 ? (float)-eps : (float)eps;
+    v3f inv = {(float)(1.0 / dir.x), (float)(1.0 / dir.y), (float)(1.0 / dir.z)};
+    double t1 = (bmin.x - p0.x) * inv.x;
+    double t2 = (bmax.x - p0.x) * inv.x;
+    double tmin = fmin(t1, t2), tmax = fmax(t1, t2);
+    t1 = (bmin.y - p0.y) * inv.y;
+    t2 = (bmax.y - p0.y) * inv.y;
+    tmin = fmax(tmin, fmin(t1, t2));
+    tmax = fmin(tmax, fmax(t1, t2));
+    t1 = (bmin.z - p0.z) * inv.z;
+    t2 = (bmax.z - p0.z) * inv.z;
+    tmin = fmax(tmin, fmin(t1, t2));
+    tmax = fmin(tmax, fmax(t1, t2));
+    if (tmax > fmax(tmin, 0.0)) {
+        *hit = vadd(p0, vscale((float)tmin, dir));
+        return 1;
+    }
+    return 0;
+}
+
+static int cell_is_solid(v3f p, double dx, const uint8_t *mat, int ni, int nj, int nk) {
+    int64_t cell;
+    return cell_of(p.x, p.y, p.z, dx, ni, nj, nk, &cell) && mat[cell] == MAT_SOLID;
+}
+
+/* FluidSimulation::_resolveParticleSolidCellCollision (fluidsimulation.cpp:
+   3145-3179): back off 0.05 dx along the ray from where it enters the first
+   solid voxel; p0 when there is no hit or the backed-off point is solid. */
+static v3f resolve_collision(v3f p0, v3f p1, double dx, const uint8_t *mat, int ni, int nj,
+                             int nk) {
+    int vox[3];
+    if (!segment_voxel(p0, p1, dx, mat, ni, nj, nk, vox)) return p0;
+    v3f d = vsub(p1, p0);
+    float len = (float)sqrt((double)(d.x * d.x + d.y * d.y + d.z * d.z));
+    float inv = (float)(1.0 / len);
+    v3f raynorm = {d.x * inv, d.y * inv, d.z * inv};
+    v3f vpos = {(float)((float)vox[0] * dx), (float)((float)vox[1] * dx),
+                (float)((float)vox[2] * dx)};
+    v3f vmax = {vpos.x + (float)dx, vpos.y + (float)dx, vpos.z + (float)dx};
+    v3f cpoint;
+    if (!ray_aabb(p0, raynorm, vpos, vmax, &cpoint)) return p0;
+    v3f r = vsub(cpoint, vscale((float)(0.05 * dx), raynorm));
+    if (cell_is_solid(r, dx, mat, ni, nj, nk)) return p0;
+    return r;
+}
+
 void gfo_fill_component(float *out, const float *ugrid, const float *W, int c,
                         const uint8_t *mat, int ni, int nj, int nk) {
     matgrid m = {mat, ni, nj, nk};
@@ -687,7 +789,22 @@ void gfo_flip_step_fs(float *pos, float *vel, int64_t n, int ni, int nj, int nk,
                                        (double)body_force[c] * dt);
     extrapolate(grid, &m, layers);
 
+    float *x0 = (float *)malloc(sizeof(float) * (size_t)(n > 0 ? 3 * n : 1));
+    memcpy(x0, pos, sizeof(float) * (size_t)(3 * n));
     update_particles(pos, vel, n, grid, old, ni, nj, nk, dx, dt, pic_ratio, stencil);
+    /* a particle moved into a solid cell is pulled back to where its path
+       enters the solid (fluidsimulation.cpp:3181-3207) */
+    if (solid)
+        for (int64_t p = 0; p < n; p++) {
+            v3f a = {x0[3 * p], x0[3 * p + 1], x0[3 * p + 2]};
+            v3f b = {pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]};
+            if (!cell_is_solid(b, dx, mat, ni, nj, nk)) continue;
+            v3f r = resolve_collision(a, b, dx, mat, ni, nj, nk);
+            pos[3 * p] = r.x;
+            pos[3 * p + 1] = r.y;
+            pos[3 * p + 2] = r.z;
+        }
+    free(x0);
 
     for (int c = 0; c < 3; c++) {
         if (outs[c]) memcpy(outs[c], grid[c], sizeof(float) * (size_t)nf[c]);

@@ -19,6 +19,7 @@
 #include <cstdint>
 #include <cstring>
 
+#include "collision.h"
 #include "fluidmaterialgrid.h"
 #include "macvelocityfield.h"
 #include "scalarfield.h"
@@ -198,6 +199,25 @@ void step_impl(float *pos, float *vel, int64_t n, int ni, int nj, int nk, double
                const uint8_t *solid, int layers, float *u, float *v, float *w, float *u_old,
                float *v_old, float *w_old, float *wu, float *wv, float *ww, uint8_t *mat_out);
 
+// FluidSimulation::_resolveParticleSolidCellCollision (fluidsimulation.cpp:
+// 3145-3179) restated around the reference's own Collision functions; a
+// backed-off point outside the grid counts as not solid (the reference's
+// domain is walled by solid cells, so it never gets there).
+vmath::vec3 resolve_collision(FluidMaterialGrid &mat, double dx, vmath::vec3 p0, vmath::vec3 p1,
+                              int ni, int nj, int nk) {
+    GridIndex voxel;
+    if (!Collision::getLineSegmentVoxelIntersection(p0, p1, dx, mat, &voxel)) return p0;
+    vmath::vec3 raynorm = vmath::normalize(p1 - p0);
+    vmath::vec3 vpos = Grid3d::GridIndexToPosition(voxel, dx);
+    AABB bbox(vpos, dx, dx, dx);
+    vmath::vec3 cpoint;
+    if (!Collision::rayIntersectsAABB(p0, raynorm, bbox, &cpoint)) return p0;
+    vmath::vec3 r = cpoint - (float)(0.05 * dx) * raynorm;
+    GridIndex gr = Grid3d::positionToGridIndex(r, dx);
+    if (Grid3d::isGridIndexInRange(gr, ni, nj, nk) && mat.isCellSolid(gr)) return p0;
+    return r;
+}
+
 }  // namespace
 
 extern "C" {
@@ -324,7 +344,8 @@ void step_impl(float *pos, float *vel, int64_t n, int ni, int nj, int nk, double
     // the field is extrapolated after the (omitted) pressure solve (:3334)
     if (fs) mac.extrapolateVelocityField(mat, layers);
 
-    // PIC/FLIP (fluidsimulation.cpp:3104-3128) then RK4 (3181-3207)
+    // PIC/FLIP (fluidsimulation.cpp:3104-3128) then RK4 (3181-3207), with
+    // the solid-cell collision fix of _advanceRangeOfMarkerParticles
     for (int64_t p = 0; p < n; p++) {
         vmath::vec3 x0(pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]);
         vmath::vec3 v0(vel[3 * p], vel[3 * p + 1], vel[3 * p + 2]);
@@ -334,6 +355,11 @@ void step_impl(float *pos, float *vel, int64_t n, int ni, int nj, int nk, double
         vmath::vec3 vFLIP = v0 + vnew - vold;
         vmath::vec3 vn = (float)pic_ratio * vPIC + (float)(1 - pic_ratio) * vFLIP;
         vmath::vec3 x1 = rk(mac, x0, dt, 4, stencil);
+        if (fs && solid) {
+            GridIndex g = Grid3d::positionToGridIndex(x1, dx);
+            if (Grid3d::isGridIndexInRange(g, ni, nj, nk) && mat.isCellSolid(g))
+                x1 = resolve_collision(mat, dx, x0, x1, ni, nj, nk);
+        }
         vel[3 * p] = vn.x;
         vel[3 * p + 1] = vn.y;
         vel[3 * p + 2] = vn.z;

@@ -107,3 +107,24 @@ def test_step_fs_rejections(ctx):
         gf.gfsx.pb.execute_lib_func(f, [ctx(), scs, 2, prs])
     with pytest.raises(RuntimeError, match="epilogue"):
         ctx.flip_step(st2, sc.dt, epilogue=5)
+
+
+@pytest.mark.parametrize("order", [0, 1])
+def test_step_fs_collisions_match_oracle(ctx, orc, order):
+    """Particles driven into the solid border: the collision fix
+    (_resolveParticleSolidCellCollision, fluidsimulation.cpp:3145-3207)
+    against the oracle, which matches the reference's own Collision
+    functions bit for bit."""
+    from test_oracle_epilogue import wall_scene
+    dims, dx, pos, vel = wall_scene()
+    sol = border_solid(dims)
+    dt = 1.0 / 20.0
+    got = run_gpu(ctx, pos, vel, dims, dx, dt, TRILINEAR, order, sol, 7)
+    exp = orc.flip_step_fs(pos, vel, dims, dx, dt, (0.0, -9.8, 0.0), 0.05, TRILINEAR, sol, 7)
+    assert norm_rel(got["pos"] - pos, exp["pos"] - pos) <= NORM_REL_TOL
+    assert norm_rel(got["vel"], exp["vel"]) <= NORM_REL_TOL
+    cells = np.floor(got["pos"].astype(np.float64) / dx).astype(int)
+    assert (cells[:, 0] >= 1).all()
+    # every particle that the oracle pulled back was pulled back here too
+    free = orc.flip_step_fs(pos, vel, dims, dx, dt, (0.0, -9.8, 0.0), 0.05, TRILINEAR, None, 7)
+    assert (free["pos"][:, 0] < dx).sum() > len(pos) // 4

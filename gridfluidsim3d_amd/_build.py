@@ -16,7 +16,7 @@ LIB = os.path.join(LIBDIR, "libgfsx.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["gfsx_kernels.hip", "gfsx_adjoint.hip", "gfsx_epilogue.hip", "gfsx_api.hip"]
+SOURCES = ["gfsx_kernels.hip", "gfsx_adjoint.hip", "gfsx_epilogue.hip", "gfsx_remove.hip", "gfsx_api.hip"]
 
 HIP_FLAGS = [
     "--offload-arch=" + ARCH,
@@ -29,18 +29,35 @@ HIP_FLAGS = [
 
 
 def build(verbose=False):
-    """Compile every HIP source of the library into lib/libgfsx.so."""
+    """Compile every HIP source of the library (in parallel, one object per
+    source) and link them into lib/libgfsx.so."""
     os.makedirs(LIBDIR, exist_ok=True)
+    objdir = os.path.join(LIBDIR, "obj")
+    os.makedirs(objdir, exist_ok=True)
+    jobs = []
+    for src in SOURCES:
+        obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
+        cmd = ([HIPCC] + HIP_FLAGS + ["-c", "-I", os.path.join(ROOT, "include"), "-I", CSRC,
+                                      "-o", obj, os.path.join(CSRC, src)])
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        jobs.append((obj, cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE,
+                                                stderr=subprocess.PIPE, text=True)))
+    errors = []
+    for obj, cmd, proc in jobs:
+        out, err = proc.communicate()
+        if proc.returncode != 0:
+            errors.append(" ".join(cmd) + "\n" + out + err)
+    if errors:
+        raise RuntimeError("hipcc failed:\n" + "\n".join(errors))
     tmp = LIB + ".tmp"
-    cmd = ([HIPCC] + HIP_FLAGS + ["-shared", "-Wl,-soname," + os.path.basename(LIB),
-                                  "-I", os.path.join(ROOT, "include"),
-                                  "-I", CSRC, "-o", tmp] +
-           [os.path.join(CSRC, s) for s in SOURCES])
+    cmd = ([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC",
+            "-Wl,-soname," + os.path.basename(LIB), "-o", tmp] + [obj for obj, _, _ in jobs])
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
-        raise RuntimeError("hipcc failed:\n" + res.stdout + res.stderr)
+        raise RuntimeError("hipcc link failed:\n" + res.stdout + res.stderr)
     os.replace(tmp, LIB)
     return LIB
 

@@ -38,7 +38,7 @@ enum Slot {
     S_OLD0, S_XS0, S_XS1, S_XS2, S_VS0, S_VS1, S_VS2, S_DENSE, S_DENSEN,
     S_D_POS, S_D_VAL, S_D_S, S_D_W, S_D_U, S_D_V, S_D_WW, S_D_OUT,
     S_B_GRID, S_B_OLD, S_B_WGT, S_B_LG, S_B_XO, S_B_VO, S_B_FX, S_B_MAX,
-    S_E_N, S_E_W, S_E_LAYER,
+    S_E_N, S_E_W, S_E_LAYER, S_R_KEEP, S_R_DST, S_R_SCAN, S_R_N,
     NSLOT
 };
 
@@ -801,6 +801,89 @@ void gfsx_flip_step_backward(gfsx_ctx *ctx, const gfsx_scene *sc,
         a.inv_r2 = (float)(1.0 / (sc->dx * sc->dx));
         gfsx::flip_step_adjoint(a, ctx->adjoint_variant == 0 ? &so : nullptr, ctx->stream);
     });
+}
+
+// ---------------------------------------------------------------- removal
+
+int64_t gfsx_remove_particles(gfsx_ctx *ctx, const gfsx_scene *sc, int max_per_cell,
+                              uint64_t seed, int *err) {
+    int64_t kept = 0;
+    guarded(ctx, err, [&] {
+        const char *who = "gfsx_remove_particles";
+        auto req = [&](bool c, const char *what) {
+            if (!c) throw Error(std::string(who) + ": " + what);
+        };
+        req(sc != nullptr, "null scene");
+        check_grid(sc->ni, sc->nj, sc->nk, sc->dx);
+        check_particles(sc->n, sc->x_in);
+        req(max_per_cell >= 1, "max_per_cell must be at least 1");
+        const int64_t n = sc->n;
+        if (n > 0) {
+            req(sc->x_in && sc->y_in && sc->z_in && sc->vx_in && sc->vy_in && sc->vz_in,
+                "null particle input");
+            req(sc->x_out && sc->y_out && sc->z_out && sc->vx_out && sc->vy_out && sc->vz_out,
+                "null particle output");
+            const void *ins[6] = {sc->x_in, sc->y_in, sc->z_in, sc->vx_in, sc->vy_in, sc->vz_in};
+            const void *outs[6] = {sc->x_out, sc->y_out, sc->z_out, sc->vx_out, sc->vy_out,
+                                   sc->vz_out};
+            for (int a = 0; a < 6; a++)
+                for (int b = 0; b < 6; b++) req(ins[a] != outs[b], "outputs must not alias inputs");
+            req(!sc->id_out || sc->id_out != sc->id_in, "id_out must not alias id_in");
+        }
+        // the cell sort over the grid's cells (Grid3d::positionToGridIndex)
+        gfsx::BinGrid bg;
+        bg.b[0] = sc->ni;
+        bg.b[1] = sc->nj;
+        bg.b[2] = sc->nk;
+        bg.pad[0] = bg.pad[1] = bg.pad[2] = 0;
+        bg.off[0] = bg.off[1] = bg.off[2] = 0.0f;
+        bg.inv_dx = 1.0 / sc->dx;
+        bg.nbin = (uint32_t)((int64_t)sc->ni * sc->nj * sc->nk);
+        std::vector<gfsx::SceneDesc> t(1);
+        std::memset(t.data(), 0, sizeof(gfsx::SceneDesc));
+        t[0].xin[0] = sc->x_in;
+        t[0].xin[1] = sc->y_in;
+        t[0].xin[2] = sc->z_in;
+        t[0].n = n;
+        t[0].bg = bg;
+        gfsx::SortBuffers sb = ctx->sort_buffers(t, n, bg.nbin + 1, (uint32_t)gfsx::sort_blocks(n));
+        gfsx::cell_sort(sb, n, ctx->stream, false);
+        gfsx::RemoveArgs r;
+        std::memset(&r, 0, sizeof r);
+        r.n = n;
+        const float *xin[3] = {sc->x_in, sc->y_in, sc->z_in};
+        const float *vin[3] = {sc->vx_in, sc->vy_in, sc->vz_in};
+        float *xout[3] = {sc->x_out, sc->y_out, sc->z_out};
+        float *vout[3] = {sc->vx_out, sc->vy_out, sc->vz_out};
+        for (int c = 0; c < 3; c++) {
+            r.x[c] = xin[c];
+            r.v[c] = vin[c];
+            r.xo[c] = xout[c];
+            r.vo[c] = vout[c];
+        }
+        r.id_in = sc->id_in;
+        r.id_out = sc->id_out;
+        r.bg = bg;
+        r.start = sb.start;
+        r.perm = sb.perm;
+        r.solid = sc->solid;
+        r.max_per_cell = max_per_cell;
+        r.seed = seed;
+        r.keep = ctx->u32(S_R_KEEP, n + 1);
+        r.dst = ctx->u32(S_R_DST, n + 1);
+        uint32_t *scratch = ctx->u32(S_R_SCAN, gfsx::scan_scratch_words(n + 1));
+        gfsx::fill_u32(r.keep, 0u, n + 1, ctx->stream);
+        gfsx::remove_mark(r, ctx->stream);
+        // keep[n] = 0, so dst[n] is the number kept
+        gfsx::exclusive_scan(r.keep, r.dst, n + 1, scratch, ctx->stream);
+        gfsx::remove_compact(r, ctx->stream);
+        uint32_t total = 0;
+        GFSX_HIP(hipMemcpyAsync(&total, r.dst + n, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                ctx->stream));
+        GFSX_HIP(hipStreamSynchronize(ctx->stream));
+        kept = (int64_t)total;
+    });
+    return kept;
 }
 
 void gfsx_set_option(gfsx_ctx *ctx, const char *name, int value, int *err) {

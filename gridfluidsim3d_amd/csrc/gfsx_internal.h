@@ -249,6 +249,22 @@ struct FsEpilogue {
     float *x1[3];
 };
 
+// Particle removal (gfsx_remove.hip): positions/velocities in, kept ones
+// out (input order), the cell sort over the grid's cells (bg, no pad).
+struct RemoveArgs {
+    int64_t n;
+    const float *x[3], *v[3];
+    float *xo[3], *vo[3];
+    const uint32_t *id_in;
+    uint32_t *id_out;
+    BinGrid bg;
+    const uint32_t *start, *perm;
+    const uint8_t *solid;
+    int max_per_cell;
+    uint64_t seed;
+    uint32_t *keep, *dst;
+};
+
 // ---- launches (implemented in gfsx_kernels.hip) ------------------------
 
 void fill_u32(uint32_t *p, uint32_t value, int64_t count, hipStream_t s);
@@ -257,6 +273,9 @@ void fill_u8(uint8_t *p, uint8_t value, int64_t count, hipStream_t s);
 // gfsx_epilogue.hip
 void fs_epilogue(const FsEpilogue &e, hipStream_t s);
 void fs_collide(const FsEpilogue &e, hipStream_t s);
+// gfsx_remove.hip
+void remove_mark(const RemoveArgs &r, hipStream_t s);
+void remove_compact(const RemoveArgs &r, hipStream_t s);
 // gfsx_adjoint.hip
 // so == nullptr selects one thread per particle in input order with every
 // grid adjoint added straight to global memory; otherwise bricks of the

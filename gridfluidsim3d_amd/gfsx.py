@@ -390,6 +390,28 @@ class Context:
         pb.execute_lib_func(f, [self(), ctypes.byref(sc), ctypes.byref(pr)])
         st.cur = 1 - st.cur
 
+    def remove_particles(self, state, max_per_cell=100, seed=0):
+        """Drop the state's particles in solid cells and keep at most
+        ``max_per_cell`` per cell (``gfsx_remove_particles``), advancing its
+        buffers; returns the number kept (the state shrinks to it)."""
+        import torch
+        st = state
+        sc = self._scene_of(st)
+        f = pb.lib_func("gfsx_remove_particles",
+                        [c_void_p, POINTER(Scene), c_int, ctypes.c_uint64], c_int64)
+        self.use_torch_stream()
+        kept = int(pb.execute_lib_func(f, [self(), ctypes.byref(sc), int(max_per_cell),
+                                           int(seed) & 0xFFFFFFFFFFFFFFFF]))
+        st.cur = 1 - st.cur
+        st.n = kept
+        for b in range(2):
+            st.x[b] = [t[:kept] for t in st.x[b]]
+            st.v[b] = [t[:kept] for t in st.v[b]]
+            if st.ids is not None:
+                st.ids[b] = st.ids[b][:kept]
+        torch.cuda.current_stream().synchronize()
+        return kept
+
     @staticmethod
     def _soa_scene(x, v, dims, dx, x_out=None, v_out=None, grids=None):
         """A C scene over (3, n) SoA tensors: x, v in; x_out, v_out, grids

@@ -269,6 +269,21 @@ void gfsx_flip_step_backward(gfsx_ctx *ctx, const gfsx_scene *scene,
                              const gfsx_step_params *params,
                              const gfsx_scene_grad *grad, int *err);
 
+/* Particle removal after a step: FluidSimulation::
+ * _removeMarkerParticlesInSolidCells and _removeMarkerParticles
+ * (fluidsimulation.cpp:1985-1988, 3213-3243; the reference's cap is 100,
+ * fluidsimulation.h:1167).  Reads n particles from scene's *_in arrays,
+ * drops those in solid cells (scene->solid, optional) and keeps at most
+ * max_per_cell per cell, writing the kept ones, in input order, to the
+ * *_out arrays (and id_out: id_in[p], or p when id_in is NULL).  Where the
+ * reference keeps the first particles of a rand() shuffle, a cell here
+ * keeps the max_per_cell particles with the smallest SplitMix64 key of
+ * (seed, input index): the same result on every run.  Particles outside
+ * the grid are kept.  The scene's grid fields are not used.  Returns the
+ * number of particles kept (and waits for the result). */
+int64_t gfsx_remove_particles(gfsx_ctx *ctx, const gfsx_scene *scene, int max_per_cell,
+                              uint64_t seed, int *err);
+
 /* Stable counting sort of particle indices by cell (the binning the P2G
  * gather uses), exposed for testing: perm[n] receives the indices ordered by
  * cell, then by index; cell_start[ni*nj*nk+1] the cell offsets.  Particles

@@ -818,6 +818,60 @@ void gfo_flip_step_fs(float *pos, float *vel, int64_t n, int ni, int nj, int nk,
     free(mat);
 }
 
+/* ------------------------------------------------------------------------- */
+/* Particle removal                                                           */
+/* ------------------------------------------------------------------------- */
+
+/* SplitMix64 (Steele, Lea, Flood 2014; constants of Vigna's splitmix64.c):
+   a bijection of i for a fixed seed, so keys never tie. */
+uint64_t gfo_removal_key(uint64_t seed, uint64_t i) {
+    uint64_t z = seed + (i + 1) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+/* FluidSimulation::_removeMarkerParticlesInSolidCells and
+   _removeMarkerParticles (fluidsimulation.cpp:1985-1988, 3213-3243): drop
+   particles in solid cells, then keep at most max_per_cell per cell.  The
+   reference keeps the first ones of a rand() shuffle (:3213-3221); here a
+   cell keeps its max_per_cell particles of smallest gfo_removal_key(seed,
+   index).  Particles outside the grid are kept.  Returns the number kept;
+   keep[p] is set to 0/1. */
+int64_t gfo_remove_particles(const float *pos, int64_t n, double dx, int ni, int nj, int nk,
+                             const uint8_t *solid, int max_per_cell, uint64_t seed,
+                             uint8_t *keep) {
+    int64_t ncell = (int64_t)ni * nj * nk;
+    int64_t *cellof = (int64_t *)malloc(sizeof(int64_t) * (size_t)(n > 0 ? n : 1));
+    int64_t kept = 0;
+    for (int64_t p = 0; p < n; p++) {
+        int64_t c;
+        cellof[p] = cell_of(pos[3 * p], pos[3 * p + 1], pos[3 * p + 2], dx, ni, nj, nk, &c) ? c : -1;
+    }
+    /* brute force: a particle's rank among the non-solid particles of its cell */
+    int64_t *count = (int64_t *)calloc((size_t)(ncell > 0 ? ncell : 1), sizeof(int64_t));
+    for (int64_t p = 0; p < n; p++)
+        if (cellof[p] >= 0) count[cellof[p]]++;
+    for (int64_t p = 0; p < n; p++) {
+        int64_t c = cellof[p];
+        int k = 1;
+        if (c >= 0 && solid && solid[c]) {
+            k = 0;
+        } else if (c >= 0 && count[c] > max_per_cell) {
+            uint64_t kp = gfo_removal_key(seed, (uint64_t)p);
+            int64_t rank = 0;
+            for (int64_t q = 0; q < n; q++)
+                if (cellof[q] == c && gfo_removal_key(seed, (uint64_t)q) < kp) rank++;
+            k = rank < max_per_cell;
+        }
+        keep[p] = (uint8_t)k;
+        kept += k;
+    }
+    free(count);
+    free(cellof);
+    return kept;
+}
+
 int64_t gfo_cell_sort(const float *pos, int64_t n, double dx, int ni, int nj,
                       int nk, uint32_t *perm, uint32_t *cell_start) {
     /* Stable counting sort keyed by Grid3d::positionToGridIndex

@@ -90,6 +90,17 @@ void gfo_fill_component(float *out, const float *ugrid, const float *W, int c,
 void gfo_extrapolate(float *u, float *v, float *w, int ni, int nj, int nk,
                      const uint8_t *mat, int layers);
 
+/* Particle removal: drop particles in solid cells, keep at most
+ * max_per_cell per cell, chosen by the smallest gfo_removal_key(seed,
+ * index) (SplitMix64) where the reference shuffles with rand()
+ * (fluidsimulation.cpp:1985-1988, 3213-3243).  Particles outside the grid
+ * are kept.  keep[p] receives 0/1; returns the number kept.  O(n * cell)
+ * brute force: small inputs only. */
+uint64_t gfo_removal_key(uint64_t seed, uint64_t i);
+int64_t gfo_remove_particles(const float *pos, int64_t n, double dx, int ni, int nj, int nk,
+                             const uint8_t *solid, int max_per_cell, uint64_t seed,
+                             uint8_t *keep);
+
 /* Stable counting sort of particle indices by the cell that
  * Grid3d::positionToGridIndex (grid3d.h:58-63) assigns.  Particles outside
  * [0,ni)x[0,nj)x[0,nk) get no cell and are left out of perm.

@@ -114,6 +114,14 @@ class CPUPath:
             ex.restype = None
             self._extrapolate = lambda u, v, w, ni, nj, nk, dx, m, layers: ex(
                 u, v, w, ni, nj, nk, m, layers)
+            rk = L.gfo_removal_key
+            rk.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
+            rk.restype = ctypes.c_uint64
+            self.removal_key = rk
+            rp = L.gfo_remove_particles
+            rp.argtypes = [_f32p, _i64, _dbl, _int, _int, _int, _u8p, _int, ctypes.c_uint64, _u8p]
+            rp.restype = _i64
+            self._remove = rp
             fc = L.gfo_fill_component
             fc.argtypes = [_f32p, _f32p, _f32p, _int, _u8p, _int, _int, _int]
             fc.restype = None
@@ -224,6 +232,17 @@ class CPUPath:
         self._extrapolate(_ptr(u, _f32p), _ptr(v, _f32p), _ptr(w, _f32p), ni, nj, nk,
                           float(dx), _ptr(m, _u8p), int(layers))
         return u, v, w
+
+    def remove_particles(self, pos, dims, dx, solid=None, max_per_cell=100, seed=0):
+        """Keep mask (uint8) of the particle removal (restatement only)."""
+        pos = _f32(pos).reshape(-1, 3)
+        ni, nj, nk = dims
+        keep = np.zeros(pos.shape[0], np.uint8)
+        sol = None if solid is None else np.ascontiguousarray(solid, np.uint8).reshape(nk, nj, ni)
+        kept = self._remove(_ptr(pos, _f32p), pos.shape[0], float(dx), ni, nj, nk,
+                            _ptr(sol, _u8p), int(max_per_cell), int(seed), _ptr(keep, _u8p))
+        assert kept == int(keep.sum())
+        return keep
 
     def fill_component(self, ugrid, W, c, dims, mat):
         """The face fill of component c (restatement only)."""

@@ -38,7 +38,7 @@ enum Slot {
     S_OLD0, S_XS0, S_XS1, S_XS2, S_VS0, S_VS1, S_VS2, S_DENSE, S_DENSEN,
     S_D_POS, S_D_VAL, S_D_S, S_D_W, S_D_U, S_D_V, S_D_WW, S_D_OUT,
     S_B_GRID, S_B_OLD, S_B_WGT, S_B_LG, S_B_XO, S_B_VO, S_B_FX, S_B_MAX,
-    S_E_N, S_E_W, S_E_LAYER, S_R_KEEP, S_R_DST, S_R_SCAN, S_R_N,
+    S_E_N, S_E_W, S_E_LAYER, S_E_LIST, S_R_KEEP, S_R_DST, S_R_SCAN, S_R_N,
     NSLOT
 };
 
@@ -403,6 +403,7 @@ std::vector<gfsx::SceneDesc> flip_step_impl(gfsx_ctx *ctx, const gfsx_scene *sce
         epi.solid = scenes[0].solid;
         epi.layer = (int8_t *)ctx->get(S_E_LAYER, (size_t)d.ni * d.nj * d.nk);
         epi.layers = params[0].extrapolation_layers > 0 ? params[0].extrapolation_layers : 7;
+        epi.fclass = (uint16_t *)ctx->get(S_E_LIST, sizeof(uint16_t) * (size_t)tot);
         epi.dx = d.dx;
         epi.np = d.n;
         for (int c = 0; c < 3; c++) {

@@ -16,7 +16,10 @@
 //      neighbours (|n| > 0 for U, :2630; isValueSet for V, W, :2675,2720);
 //      other faces 0.  The snapshot takes that field; the field gets the
 //      body force on faces bordering fluid (:2766-2805, 3306).
-//   3. k_fs_extrapolate, one launch per layer L: on both the snapshot and
+//   3. k_fs_classify: each face's extrapolation layer (at most one) and its
+//      neighbours bordering the layer before; k_fs_extrapolate, one launch
+//      per layer L that changes the faces of layer L only: on both the
+//      snapshot and
 //      the field (:3307, 3334), faces that border layer L, not layer L-1 and
 //      no solid cell take the FP64 average of their 6 same-component
 //      neighbours that border layer L-1, or 0 when that sum is 0
@@ -58,26 +61,37 @@ __device__ __forceinline__ bool face_borders(const FsEpilogue &e, int c, int i, 
     return layer_at(e, i, j, k) == v;
 }
 
-// component and index of flattened face f (u faces, then v, then w)
-__device__ __forceinline__ void face_of(const FsEpilogue &e, int64_t f, int &c, int &i, int &j,
+// component and index of flattened face f (u faces, then v, then w);
+// every face count is below 2^31, so 32-bit arithmetic is exact
+__device__ __forceinline__ void face_of(const FsEpilogue &e, uint32_t f, int &c, int &i, int &j,
                                         int &k, int d[3]) {
-    const int64_t f0 = (int64_t)(e.ni + 1) * e.nj * e.nk;
-    const int64_t f1 = (int64_t)e.ni * (e.nj + 1) * e.nk;
+    const uint32_t f0 = (uint32_t)(e.ni + 1) * (uint32_t)e.nj * (uint32_t)e.nk;
+    const uint32_t f1 = (uint32_t)e.ni * (uint32_t)(e.nj + 1) * (uint32_t)e.nk;
     c = f < f0 ? 0 : (f < f0 + f1 ? 1 : 2);
-    const int64_t l = c == 0 ? f : (c == 1 ? f - f0 : f - f0 - f1);
+    const uint32_t l = c == 0 ? f : (c == 1 ? f - f0 : f - f0 - f1);
     d[0] = e.ni + (c == 0);
     d[1] = e.nj + (c == 1);
     d[2] = e.nk + (c == 2);
-    i = (int)(l % d[0]);
-    j = (int)((l / d[0]) % d[1]);
-    k = (int)(l / ((int64_t)d[0] * d[1]));
+    const uint32_t row = l / (uint32_t)d[0];
+    i = (int)(l - row * (uint32_t)d[0]);
+    j = (int)(row % (uint32_t)d[1]);
+    k = (int)(row / (uint32_t)d[1]);
+}
+
+__device__ __forceinline__ void cell_of_index(const FsEpilogue &e, uint32_t t, int &i, int &j,
+                                              int &k) {
+    const uint32_t row = t / (uint32_t)e.ni;
+    i = (int)(t - row * (uint32_t)e.ni);
+    j = (int)(row % (uint32_t)e.nj);
+    k = (int)(row / (uint32_t)e.nj);
 }
 
 __global__ __launch_bounds__(kBlock) void k_fs_layers_init(FsEpilogue e) {
     const int64_t t = blockIdx.x * (int64_t)kBlock + threadIdx.x;
     const int64_t ncell = (int64_t)e.ni * e.nj * e.nk;
     if (t >= ncell) return;
-    const int i = (int)(t % e.ni), j = (int)((t / e.ni) % e.nj), k = (int)(t / ((int64_t)e.ni * e.nj));
+    int i, j, k;
+    cell_of_index(e, (uint32_t)t, i, j, k);
     int8_t v;
     if (e.solid && e.solid[t]) {
         v = kSolid;
@@ -95,7 +109,8 @@ __global__ __launch_bounds__(kBlock) void k_fs_dilate(FsEpilogue e, int L) {
     const int64_t ncell = (int64_t)e.ni * e.nj * e.nk;
     if (t >= ncell) return;
     if (e.layer[t] != -1) return;
-    const int i = (int)(t % e.ni), j = (int)((t / e.ni) % e.nj), k = (int)(t / ((int64_t)e.ni * e.nj));
+    int i, j, k;
+    cell_of_index(e, (uint32_t)t, i, j, k);
     const int8_t src = (int8_t)(L - 1);
     // neighbours set in this launch hold L, never L - 1, whichever value is read
     bool hit = (i > 0 && layer_at(e, i - 1, j, k) == src) ||
@@ -111,7 +126,7 @@ __global__ __launch_bounds__(kBlock) void k_fs_fill(FsEpilogue e, int64_t nface)
     const int64_t f = blockIdx.x * (int64_t)kBlock + threadIdx.x;
     if (f >= nface) return;
     int c, i, j, k, d[3];
-    face_of(e, f, c, i, j, k, d);
+    face_of(e, (uint32_t)f, c, i, j, k, d);
     const uint32_t o = (uint32_t)i + (uint32_t)d[0] * ((uint32_t)j + (uint32_t)d[1] * (uint32_t)k);
     float *grid = c == 0 ? e.grid[0] : (c == 1 ? e.grid[1] : e.grid[2]);
     float *old = c == 0 ? e.old[0] : (c == 1 ? e.old[1] : e.old[2]);
@@ -150,31 +165,71 @@ __global__ __launch_bounds__(kBlock) void k_fs_fill(FsEpilogue e, int64_t nface)
     grid[o] = on ? val + gd : val;
 }
 
-__global__ __launch_bounds__(kBlock) void k_fs_extrapolate(FsEpilogue e, int64_t nface, int L) {
-    const int64_t f = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+// the layer L at which face (c,i,j,k) is extrapolated: L borders it, L-1
+// does not, no solid does, 1 <= L <= layers (macvelocityfield.cpp:694-706);
+// 0 when none.  A face borders at most two cells, so at most one L fits.
+__device__ __forceinline__ int face_layer(const FsEpilogue &e, int c, int i, int j, int k) {
+    const int a = c == 0 ? i : (c == 1 ? j : k);
+    const int na = c == 0 ? e.ni : (c == 1 ? e.nj : e.nk);
+    const int li = i - (c == 0), lj = j - (c == 1), lk = k - (c == 2);
+    const int hi = a < na ? layer_at(e, i, j, k) : -3;   // -3: no cell
+    const int lo = a > 0 ? layer_at(e, li, lj, lk) : -3;
+    if (hi == kSolid || lo == kSolid) return 0;
+    int L = 0;
+    if (hi >= 1 && lo != hi - 1) L = hi;
+    if (lo >= 1 && hi != lo - 1 && (L == 0 || lo < L)) L = lo;
+    return L <= e.layers ? L : 0;
+}
+
+// each face's extrapolation layer L (0: none) and, for a face that has one,
+// which of its 6 same-component neighbours (in range) border layer L-1:
+// one 16-bit word per face, L in the low byte, the mask in the high byte
+__global__ __launch_bounds__(kBlock) void k_fs_classify(FsEpilogue e, uint32_t nface) {
+    const uint32_t f = blockIdx.x * kBlock + threadIdx.x;
     if (f >= nface) return;
     int c, i, j, k, d[3];
     face_of(e, f, c, i, j, k, d);
-    if (!(face_borders(e, c, i, j, k, L) && !face_borders(e, c, i, j, k, L - 1) &&
-          !face_borders(e, c, i, j, k, kSolid)))
-        return;
+    const int L = face_layer(e, c, i, j, k);
+    uint32_t mask = 0;
+    if (L > 0) {
+        const int nb[6][3] = {{i - 1, j, k}, {i + 1, j, k}, {i, j - 1, k},
+                              {i, j + 1, k}, {i, j, k - 1}, {i, j, k + 1}};
+        for (int q = 0; q < 6; q++) {
+            const int a = nb[q][0], b = nb[q][1], cc = nb[q][2];
+            if ((unsigned)a >= (unsigned)d[0] || (unsigned)b >= (unsigned)d[1] ||
+                (unsigned)cc >= (unsigned)d[2])
+                continue;
+            if (face_borders(e, c, a, b, cc, L - 1)) mask |= 1u << q;
+        }
+    }
+    e.fclass[f] = (uint16_t)((uint32_t)L | (mask << 8));
+}
+
+// one layer: every face of layer L takes the FP64 average of its masked
+// neighbours (in the order -x, +x, -y, +y, -z, +z), 0 when that sum is 0,
+// on both grids
+__global__ __launch_bounds__(kBlock) void k_fs_extrapolate(FsEpilogue e, uint32_t nface, int L) {
+    const uint32_t f = blockIdx.x * kBlock + threadIdx.x;
+    if (f >= nface) return;
+    const uint32_t cl = e.fclass[f];
+    if ((cl & 0xFFu) != (uint32_t)L) return;
+    const uint32_t mask = cl >> 8;
+    const uint32_t f0 = (uint32_t)(e.ni + 1) * (uint32_t)e.nj * (uint32_t)e.nk;
+    const uint32_t f1 = (uint32_t)e.ni * (uint32_t)(e.nj + 1) * (uint32_t)e.nk;
+    const int c = f < f0 ? 0 : (f < f0 + f1 ? 1 : 2);
+    const uint32_t o = c == 0 ? f : (c == 1 ? f - f0 : f - f0 - f1);
+    const uint32_t d0 = (uint32_t)(e.ni + (c == 0)), d1 = (uint32_t)(e.nj + (c == 1));
+    const uint32_t step[3] = {1u, d0, d0 * d1};
     float *grid = c == 0 ? e.grid[0] : (c == 1 ? e.grid[1] : e.grid[2]);
     float *old = c == 0 ? e.old[0] : (c == 1 ? e.old[1] : e.old[2]);
     double sg = 0.0, so = 0.0, cnt = 0.0;
-    const int nb[6][3] = {{i - 1, j, k}, {i + 1, j, k}, {i, j - 1, k},
-                          {i, j + 1, k}, {i, j, k - 1}, {i, j, k + 1}};
     for (int q = 0; q < 6; q++) {
-        const int a = nb[q][0], b = nb[q][1], cc = nb[q][2];
-        if ((unsigned)a >= (unsigned)d[0] || (unsigned)b >= (unsigned)d[1] ||
-            (unsigned)cc >= (unsigned)d[2])
-            continue;
-        if (!face_borders(e, c, a, b, cc, L - 1)) continue;
-        const uint32_t g = (uint32_t)a + (uint32_t)d[0] * ((uint32_t)b + (uint32_t)d[1] * (uint32_t)cc);
+        if (!(mask & (1u << q))) continue;
+        const uint32_t g = (q & 1) ? o + step[q >> 1] : o - step[q >> 1];
         sg += (double)grid[g];
         so += (double)old[g];
         cnt += 1.0;
     }
-    const uint32_t o = (uint32_t)i + (uint32_t)d[0] * ((uint32_t)j + (uint32_t)d[1] * (uint32_t)k);
     grid[o] = sg == 0.0 ? 0.0f : (float)(sg / cnt);
     old[o] = so == 0.0 ? 0.0f : (float)(so / cnt);
 }
@@ -337,9 +392,14 @@ void fs_epilogue(const FsEpilogue &e, hipStream_t s) {
     }
     hipLaunchKernelGGL(k_fs_fill, dim3(blocks_for(nface)), dim3(kBlock), 0, s, e, nface);
     GFSX_LAUNCH_CHECK();
+    // each face's layer once; a layer's launch changes only faces of that
+    // layer and reads only faces bordering the layer before
+    hipLaunchKernelGGL(k_fs_classify, dim3(blocks_for(nface)), dim3(kBlock), 0, s, e,
+                       (uint32_t)nface);
+    GFSX_LAUNCH_CHECK();
     for (int L = 1; L <= e.layers; L++) {
         hipLaunchKernelGGL(k_fs_extrapolate, dim3(blocks_for(nface)), dim3(kBlock), 0, s, e,
-                           nface, L);
+                           (uint32_t)nface, L);
         GFSX_LAUNCH_CHECK();
     }
 }

@@ -247,6 +247,9 @@ struct FsEpilogue {
     int64_t np;
     const float *x0[3];
     float *x1[3];
+    // workspace: each face's extrapolation layer (low byte, 0: none) and
+    // neighbour mask (high byte), nface words
+    uint16_t *fclass;
 };
 
 // Particle removal (gfsx_remove.hip): positions/velocities in, kept ones

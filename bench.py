@@ -143,6 +143,76 @@ def _cpu_model():
     return None
 
 
+def parity_report(log_fn):
+    """Parity of the fused step on the CPU-runnable configuration (64^3
+    sphere, 268,416 particles, dt = 1/120) for both stencils: per field
+    (u, v, w after the body force; the u-face weights W; particle velocity;
+    particle displacement) the norm-wise relative error max|gpu-cpu|/max|cpu|
+    against the reference's own CPU code (oracle/_ref; the C restatement
+    where that was not built), the largest per-element relative error over
+    elements with |cpu| >= 1e-3 max|cpu|, and the norm-wise errors of the GPU
+    and of the CPU path against the FP64 evaluation of the same step."""
+    import numpy as np
+    import torch
+
+    import gridfluidsim3d_amd as gf
+    from gridfluidsim3d_amd import scenes
+    from oracle import oracle
+
+    sc = scenes.sphere64()
+    cpu = oracle.load_reference()
+    kind = "reference"
+    if cpu is None:
+        cpu = oracle.load_oracle()
+        kind = "port"
+    o64 = oracle.Oracle64()
+    ctx = gf.Context(0)
+
+    def nrel(a, b):
+        a = np.asarray(a, np.float64)
+        b = np.asarray(b, np.float64)
+        den = np.abs(b).max()
+        return float(np.abs(a - b).max() / den) if den > 0 else float(np.abs(a - b).max())
+
+    def erel(a, b):
+        a = np.asarray(a, np.float64).ravel()
+        b = np.asarray(b, np.float64).ravel()
+        m = np.abs(b) >= 1e-3 * np.abs(b).max()
+        return float((np.abs(a[m] - b[m]) / np.abs(b[m])).max()) if m.any() else 0.0
+
+    out = dict(workload="sphere64", particles=sc.n, cpu_kind=kind, bar=1e-5)
+    for stencil, name in ((gf.TRILINEAR, "trilinear"), (gf.TRICUBIC, "tricubic")):
+        st = gf.FlipState(sc.pos, sc.vel, sc.dims, sc.dx, keep_weights=True)
+        ctx.flip_step(st, sc.dt, sc.body_force, 0.05, stencil, gf.ORDER_KEEP)
+        torch.cuda.synchronize()
+        g = dict(u=st.grid[0].cpu().numpy(), v=st.grid[1].cpu().numpy(),
+                 w=st.grid[2].cpu().numpy(), W=st.weights[0].cpu().numpy(),
+                 vel=st.velocities().cpu().numpy(), pos=st.positions().cpu().numpy())
+        c = cpu.flip_step(sc.pos, sc.vel, sc.dims, sc.dx, sc.dt, sc.body_force, 0.05, stencil)
+        c = dict(u=c["u"], v=c["v"], w=c["w"], W=c["wu"], vel=c["vel"], pos=c["pos"])
+        d = o64.flip_step(sc.pos, sc.vel, sc.dims, sc.dx, sc.dt, sc.body_force, 0.05, stencil)
+        d = dict(u=d["u"], v=d["v"], w=d["w"], W=d["wu"], vel=d["vel"], pos=d["pos"])
+        fields = {}
+        for k in ("u", "v", "w", "W", "vel", "pos"):
+            fields[k] = dict(norm_rel=nrel(g[k], c[k]), elem_rel=erel(g[k], c[k]),
+                             gpu_vs_fp64=nrel(g[k], d[k]), cpu_vs_fp64=nrel(c[k], d[k]))
+        # the displacement x' - x, for information: its error is set by the
+        # FP32 rounding of the positions themselves (1 ulp of |x| ~ 4 m is
+        # 4.8e-7 m against displacements of a few cm), which the CPU path
+        # shows against FP64 as well
+        x0 = sc.pos.astype(np.float64)
+        fields["displacement_info"] = dict(
+            norm_rel=nrel(g["pos"] - x0, c["pos"] - x0),
+            gpu_vs_fp64=nrel(g["pos"] - x0, d["pos"] - x0),
+            cpu_vs_fp64=nrel(c["pos"] - x0, d["pos"] - x0))
+        out[name] = fields
+        worst = max(f["norm_rel"] for k, f in fields.items() if k != "displacement_info")
+        out[name + "_worst_norm_rel"] = worst
+        log_fn("parity %s: worst norm-wise %.2e (bar 1e-5)" % (name, worst))
+    ctx.close()
+    return out
+
+
 def batched_measure(steps, stencil, log_fn):
     """The batched form (64 independent scenes of uneven size): particle-steps/s
     of 64 sequential plain steps per step, of the batched step (after the
@@ -302,7 +372,11 @@ def main():
     ap.add_argument("--workload", default="pool256",
                     choices=["pool256", "dambreak128", "sphere64"])
     ap.add_argument("--stencil", default="trilinear", choices=["trilinear", "tricubic"])
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--epilogue", default="none", choices=["none", "fluidsim"],
+                    help="fluidsim: the step with FluidSimulation's face fill and "
+                         "velocity extrapolation (7 layers)")
+    ap.add_argument("--no-cpu", action="store_true",
+                    help="skip the CPU baseline and the parity report")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the batched-form measurement (64 scenes)")
@@ -350,8 +424,10 @@ def main():
     log(ctx.device_info)
     stream = torch.cuda.current_stream()
 
+    epilogue = gf.EPILOGUE_FLUIDSIM if args.epilogue == "fluidsim" else gf.EPILOGUE_NONE
+
     def step(st):
-        ctx.flip_step(st, scene.dt, scene.body_force, 0.05, stencil, gf.ORDER_SORTED)
+        ctx.flip_step(st, scene.dt, scene.body_force, 0.05, stencil, gf.ORDER_SORTED, epilogue)
 
     # warm-up on a scratch copy; the timed run starts from the scene's
     # initial state and advances it args.steps times
@@ -410,7 +486,9 @@ def main():
                                          fwdbwd_out["value_forward"], 4)
 
     cpu = None
+    parity = None
     if not args.no_cpu:
+        parity = parity_report(log)
         log("CPU baseline ...")
         cpu = cpu_baseline(scene, stencil, args.cpu_seconds)
         log("CPU baseline: %s" % cpu)
@@ -428,12 +506,14 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": dict(scene.describe(), stencil=args.stencil, form="plain",
+        "config": dict(scene.describe(), stencil=args.stencil,
+                       form="plain" if args.epilogue == "none" else "plain+fluidsim-epilogue",
                        pic_ratio=0.05, body_force=list(scene.body_force)),
         "roofline": roofline,
         "step_roofline": step_roofline,
         "stages_ms": {k: round(v, 4) for k, v in stage_avg.items()},
         "cpu_baseline": cpu,
+        "parity": parity,
         "batched": batched_out,
         "fwd_bwd": fwdbwd_out,
     }

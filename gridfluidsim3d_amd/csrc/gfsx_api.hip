@@ -347,7 +347,7 @@ gfsx::SceneDesc scene_desc(const gfsx_scene *sc, const gfsx_step_params *pr, con
 // Runs the step; returns the scenes as laid out (bin grids, offsets).
 std::vector<gfsx::SceneDesc> flip_step_impl(gfsx_ctx *ctx, const gfsx_scene *scenes,
                                             const gfsx_step_params *params, int nscenes,
-                                            const char *who) {
+                                            const char *who, bool grids_only = false) {
     gfsx::StepBatch s;
     s.scenes.resize(nscenes);
     for (int i = 0; i < nscenes; i++) {
@@ -431,6 +431,7 @@ std::vector<gfsx::SceneDesc> flip_step_impl(gfsx_ctx *ctx, const gfsx_scene *sce
     s.keep_order = params[0].order == GFSX_ORDER_KEEP;
     s.p2g_variant = ctx->p2g_variant;
     s.g2p_variant = ctx->g2p_variant;
+    s.grids_only = grids_only;
     gfsx::SortBuffers sb = ctx->sort_buffers(s.scenes, s.tot.n, (uint32_t)s.tot.nbin,
                                              (uint32_t)s.tot.nblk);
     s.scenes_dev = sb.scenes;
@@ -768,7 +769,8 @@ void gfsx_flip_step_backward(gfsx_ctx *ctx, const gfsx_scene *sc,
         f.vz_out = vo + 2 * n;
         f.id_in = nullptr;
         f.id_out = nullptr;
-        const std::vector<gfsx::SceneDesc> laid = flip_step_impl(ctx, &f, pr, 1, who);
+        // (the sort, its sorted copies and the P2G grids; no particle update)
+        const std::vector<gfsx::SceneDesc> laid = flip_step_impl(ctx, &f, pr, 1, who, true);
         gfsx::AdjointSort so;
         so.start = (const uint32_t *)ctx->buf[S_START];
         so.perm = (const uint32_t *)ctx->buf[S_PERM];

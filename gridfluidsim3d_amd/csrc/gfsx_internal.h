@@ -369,6 +369,7 @@ struct StepBatch {
     int p2g_variant;   // 0: brick-staged gather, 1: direct global gather (one scene)
     int g2p_variant;   // 0: brick-staged LDS boxes, 1: direct global sampling (one scene)
     const FsEpilogue *epi = nullptr;   // FluidSimulation epilogue after P2G (one scene)
+    bool grids_only = false;           // stop after P2G (the adjoint's recompute)
 };
 
 // The fused plain step of every scene of s, in one launch per kernel.

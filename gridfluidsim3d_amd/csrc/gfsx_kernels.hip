@@ -2227,7 +2227,7 @@ void flip_step(const StepBatch &s, const SortBuffers &sb, hipStream_t st, hipEve
     if (s.epi) fs_epilogue(*s.epi, st);
     if (ev) GFSX_HIP(hipEventRecord(ev[2], st));
 
-    if (s.tot.n <= 0) {
+    if (s.tot.n <= 0 || s.grids_only) {
         if (ev) GFSX_HIP(hipEventRecord(ev[3], st));
         return;
     }

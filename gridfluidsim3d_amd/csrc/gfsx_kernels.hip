@@ -1953,8 +1953,8 @@ __device__ __forceinline__ void brick_sample(const MacGrid &g, const float *o0,
     if (OLD) {
         if (in_box<ST, B::EO>(cx, cy, cz, lox, loy, loz)) {
             out_old[0] = box_comp<ST, B::EO>(old0, cx.ia - lox, cy.ib - loy, cz.ib - loz, cx.ta, cy.tb, cz.tb);
-            out_old[1] = box_comp<ST, B::EO>(old1, cx.ib - lox, cy.ia - loy, cz.ib - loz, cx.tb, cy.ta, cz.tb);
-            out_old[2] = box_comp<ST, B::EO>(old2, cx.ib - lox, cy.ib - loy, cz.ia - loz, cx.tb, cy.tb, cz.ta);
+                out_old[1] = box_comp<ST, B::EO>(old1, cx.ib - lox, cy.ia - loy, cz.ib - loz, cx.tb, cy.ta, cz.tb);
+                out_old[2] = box_comp<ST, B::EO>(old2, cx.ib - lox, cy.ib - loy, cz.ia - loz, cx.tb, cy.tb, cz.ta);
         } else {
             out_old[0] = sample_comp<ST>(o0, ni + 1, nj, nk, cx.ia, cy.ib, cz.ib, cx.ta, cy.tb, cz.tb);
             out_old[1] = sample_comp<ST>(o1, ni, nj + 1, nk, cx.ib, cy.ia, cz.ib, cx.tb, cy.ta, cz.tb);
@@ -2095,7 +2095,7 @@ __device__ __forceinline__ void g2p_brick(const StepG2P &a, const MacGrid &g, co
 // grid: outside the domain, so every sample is 0 and the global path
 // returns at once.
 template <int ST>
-__global__ __launch_bounds__(gbrick::NT) void k_flip_g2p_brick(
+__device__ __forceinline__ void g2p_brick_kernel(
     const SceneDesc *__restrict__ scenes, int nsc, Sorted so, int keep_order,
     uint32_t nbrick, int per_xcd, const uint32_t *__restrict__ start,
     const uint32_t *__restrict__ perm) {
@@ -2130,6 +2130,23 @@ __global__ __launch_bounds__(gbrick::NT) void k_flip_g2p_brick(
     const StepG2P a = g2p_view(sd, so, keep_order);
     const MacGrid g = mac_view(sd);
     g2p_brick<ST>(a, g, sd.bg, sd.gbx, sd.gby, sd.gbz, (int)(gid - sd.g0), start + sd.b0, perm);
+}
+
+// The trilinear form; the tricubic one is limited to two waves per SIMD
+// (its LDS boxes allow two work-groups per CU anyway), which keeps the
+// compiler from spending its whole register file on one wave.
+__global__ __launch_bounds__(gbrick::NT) void k_flip_g2p_brick_tl(
+    const SceneDesc *__restrict__ scenes, int nsc, Sorted so, int keep_order,
+    uint32_t nbrick, int per_xcd, const uint32_t *__restrict__ start,
+    const uint32_t *__restrict__ perm) {
+    g2p_brick_kernel<0>(scenes, nsc, so, keep_order, nbrick, per_xcd, start, perm);
+}
+
+__global__ __launch_bounds__(gbrick::NT, 2) void k_flip_g2p_brick_tc(
+    const SceneDesc *__restrict__ scenes, int nsc, Sorted so, int keep_order,
+    uint32_t nbrick, int per_xcd, const uint32_t *__restrict__ start,
+    const uint32_t *__restrict__ perm) {
+    g2p_brick_kernel<1>(scenes, nsc, so, keep_order, nbrick, per_xcd, start, perm);
 }
 
 void layout_scenes(SceneDesc *sc, int nsc, SceneTotals &tot) {
@@ -2235,7 +2252,7 @@ void flip_step(const StepBatch &s, const SortBuffers &sb, hipStream_t st, hipEve
         const uint32_t nbrick = (uint32_t)s.tot.ng2p;
         const int per_xcd = (int)((nbrick + 7) / 8);
         const dim3 grid((unsigned)(8 * per_xcd + gbrick::NEXTRA)), block(gbrick::NT);
-        auto *kern = s.stencil == 0 ? k_flip_g2p_brick<0> : k_flip_g2p_brick<1>;
+        auto *kern = s.stencil == 0 ? k_flip_g2p_brick_tl : k_flip_g2p_brick_tc;
         hipLaunchKernelGGL(kern, grid, block, 0, st, s.scenes_dev, nsc, so, s.keep_order, nbrick,
                            per_xcd, (const uint32_t *)sb.start, (const uint32_t *)sb.perm);
     } else {

@@ -51,12 +51,37 @@ def algorithmic_bytes(P, F, C):
                 step=48 * P + 8 * F)
 
 
+# MI355X's published FP32 vector rate; it counts packed FP32 instructions
+# (two FMAs per lane), which these kernels are built without, so their
+# arithmetic ceiling is half of it.
+FP32_PEAK_TFLOPS = 157.3
+
+
+def flops_per_particle(stencil):
+    """Useful arithmetic of one plain step per particle, counted from the
+    kernels' formulas with an FMA as two FLOPs (multiply + add); compares,
+    floors, clamps and index arithmetic are not counted, so it is a lower
+    bound on the work done.
+    P2G, per (node, particle) pair: offset (3), squared distance (5),
+    t = d2/R^2 (1), Wyvill a^2 (5/9 + 4a/9) with a = 1 - t (5), S and W sums
+    (3) = 17.  With R = dx a particle reaches the 2x2x2 nodes of each
+    component around it: 24 pairs = 408.  (The gather kernel evaluates the
+    2x3x3 cells of candidates per face, 54 pairs per particle; only the 24
+    are counted.)
+    G2P: a trilinear component sample is 7 lerps of 3 = 21; a tricubic one
+    21 Catmull-Rom evaluations of 18 = 378.  PIC/FLIP samples the field and
+    the snapshot (6 component samples) and RK4 three more positions (9);
+    the blend is 15 and the RK4 combinations 39."""
+    sample = 21 if stencil == "trilinear" else 378
+    return dict(p2g=24 * 17, g2p=15 * sample + 15 + 39)
+
+
 # The kernels each stage of gfsx_flip_step launches (HIP events bracket the
 # stage on the library's stream).
 STAGE_KERNELS = {
     "sort": "cell sort: k_bin, scans, k_scatter, k_order_bins/mid/big, k_sorted_copy",
     "p2g": "k_flip_p2g_brick + k_flip_p2g_dense",
-    "g2p": "k_flip_g2p_brick",
+    "g2p": "k_flip_g2p_brick_tl (trilinear) / k_flip_g2p_brick_tc (tricubic)",
 }
 
 
@@ -378,6 +403,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true",
                     help="skip the CPU baseline and the parity report")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-tricubic", action="store_true",
+                    help="skip the tricubic-stencil measurement (same scene, 5 steps)")
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the batched-form measurement (64 scenes)")
     ap.add_argument("--batched-steps", type=int, default=10)
@@ -470,6 +497,41 @@ def main():
                          frac=round(step_gbs / HBM_PEAK_GBS, 4),
                          algorithmic_bytes=alg["step"])
 
+    fpp = flops_per_particle(args.stencil)
+    compute = dict(flops_per_particle=fpp["p2g"] + fpp["g2p"],
+                   p2g_tflops=round(fpp["p2g"] * P / (stage_avg["p2g"] / 1e3) / 1e12, 2),
+                   g2p_tflops=round(fpp["g2p"] * P / (stage_avg["g2p"] / 1e3) / 1e12, 2),
+                   step_tflops=round((fpp["p2g"] + fpp["g2p"]) * P / (ms_per_step / 1e3) / 1e12, 2),
+                   peak_fp32_tflops=FP32_PEAK_TFLOPS,
+                   peak_fp32_unpacked_tflops=FP32_PEAK_TFLOPS / 2)
+    compute["step_frac_fp32"] = round(compute["step_tflops"] / FP32_PEAK_TFLOPS, 4)
+
+    tricubic = None
+    if not args.no_tricubic and args.stencil == "trilinear":
+        # the ParticleAdvector stencil on the same scene: 1 warm-up step on a
+        # copy, then 5 steps from the initial state
+        st = None
+        torch.cuda.empty_cache()
+        tst = gf.FlipState(scene.pos, scene.vel, scene.dims, scene.dx)
+        ctx.flip_step(tst, scene.dt, scene.body_force, 0.05, gf.TRICUBIC, gf.ORDER_SORTED)
+        del tst
+        tst = gf.FlipState(scene.pos, scene.vel, scene.dims, scene.dx)
+        torch.cuda.synchronize()
+        ctx.timing_begin()
+        for _ in range(5):
+            ctx.flip_step(tst, scene.dt, scene.body_force, 0.05, gf.TRICUBIC, gf.ORDER_SORTED)
+        nt, tms = ctx.timing_end()
+        del tst
+        tavg = {k: v / nt for k, v in tms.items()}
+        tf = flops_per_particle("tricubic")
+        g2p_tf = tf["g2p"] * P / (tavg["g2p"] / 1e3) / 1e12
+        tricubic = dict(steps=nt, ms_per_step=round(tavg["step"], 3),
+                        value=P / (tavg["step"] / 1e3), unit="particle-steps/s",
+                        stages_ms={k: round(v, 4) for k, v in tavg.items()},
+                        g2p_flops_per_particle=tf["g2p"], g2p_tflops=round(g2p_tf, 2),
+                        g2p_frac_fp32=round(g2p_tf / FP32_PEAK_TFLOPS, 4))
+        log("tricubic: %s" % tricubic)
+
     batched_out = None
     if not args.no_batched:
         del st
@@ -511,6 +573,8 @@ def main():
                        pic_ratio=0.05, body_force=list(scene.body_force)),
         "roofline": roofline,
         "step_roofline": step_roofline,
+        "compute": compute,
+        "tricubic": tricubic,
         "stages_ms": {k: round(v, 4) for k, v in stage_avg.items()},
         "cpu_baseline": cpu,
         "parity": parity,

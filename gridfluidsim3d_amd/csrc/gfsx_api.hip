@@ -381,6 +381,7 @@ std::vector<gfsx::SceneDesc> flip_step_impl(gfsx_ctx *ctx, const gfsx_scene *sce
     }
     s.dense = ctx->u32(S_DENSE, s.tot.np2g);
     s.dense_count = ctx->u32(S_DENSEN, 1);
+    s.num_cu = ctx->num_cu;
     gfsx::Sorted so;
     for (int c = 0; c < 3; c++) {
         so.x[c] = s.xs[c];

@@ -184,6 +184,18 @@ __device__ __forceinline__ uint32_t bin_of(float x, float y, float z,
     return (uint32_t)fi + (uint32_t)g.b[0] * ((uint32_t)fj + (uint32_t)g.b[1] * (uint32_t)fk);
 }
 
+// Element i of an array in device memory, read as a global-memory load.
+// Pointers taken from descriptor structs are generic (flat) to the
+// compiler: a flat load counts against the LDS wait counter too, and may
+// alias LDS, so the compiler keeps it in order with LDS accesses and waits
+// for it before each one.  Naming the global address space lets loads be
+// issued in batches ahead of the LDS work that uses them.
+template <class T>
+__device__ __forceinline__ T gld(const T *p, size_t i) {
+    typedef const __attribute__((address_space(1))) T GT;
+    return ((GT *)p)[i];
+}
+
 // Wyvill kernel for t = r^2/R^2 (scalarfield.cpp:569-571 rewritten as the
 // factorisation K = (1-t)^2 (1 - 4t/9) = a^2 (5/9 + 4a/9), a = 1-t):
 // exact zero for t >= 1 without a compare, and no cancellation near t = 1.
@@ -364,6 +376,7 @@ struct StepBatch {
     SceneTotals tot;
     float *xs[3], *vs[3];            // workspace: all particles in sort order
     uint32_t *dense, *dense_count;   // workspace: P2G bricks over capacity
+    int num_cu = 1;                  // compute units of the device
     int stencil;
     int keep_order;
     int p2g_variant;   // 0: brick-staged gather, 1: direct global gather (one scene)

@@ -1059,30 +1059,53 @@ __device__ __forceinline__ void face_epilogue(const StepP2G &a,
     a.grid[c][f] = val;
 }
 
+// What the epilogue of a face of component c writes and reads of the scene:
+// selected per lane from the three components' fields, which are uniform
+// (scalar loads).  Indexing the scene's arrays with a per-lane component
+// instead makes every field a dependent vector load from the scene table.
+struct FaceOut {
+    float *grid, *old, *wgt;
+    int n0, n1;          // nodes of component c along i and j
+    int nax;             // cells along axis c
+    int grav_on;
+    float grav_delta;
+};
+
+__device__ __forceinline__ FaceOut face_out(const StepP2G &a, int c) {
+    FaceOut o;
+    o.grid = c == 0 ? a.grid[0] : (c == 1 ? a.grid[1] : a.grid[2]);
+    o.old = c == 0 ? a.grid_old[0] : (c == 1 ? a.grid_old[1] : a.grid_old[2]);
+    o.wgt = c == 0 ? a.wgt[0] : (c == 1 ? a.wgt[1] : a.wgt[2]);
+    o.n0 = a.ni + (c == 0);
+    o.n1 = a.nj + (c == 1);
+    o.nax = c == 0 ? a.ni : (c == 1 ? a.nj : a.nk);
+    o.grav_on = c == 0 ? a.grav_on[0] : (c == 1 ? a.grav_on[1] : a.grav_on[2]);
+    o.grav_delta = c == 0 ? a.grav_delta[0] : (c == 1 ? a.grav_delta[1] : a.grav_delta[2]);
+    return o;
+}
+
 // face_epilogue for a brick face: whether the cells on either side hold a
 // particle comes from the brick's occupancy table (staged cells, x fastest)
 // instead of the global counts.
-__device__ __forceinline__ void brick_epilogue(const StepP2G &a, const uint8_t *occ,
+__device__ __forceinline__ void brick_epilogue(const FaceOut &o, const uint8_t *occ,
                                                int c, int i, int j, int k, int i0, int j0,
                                                int k0, float S, float W) {
     using namespace brick;
-    const NodeGrid &ng = a.ng[c];
-    const int64_t f = (int64_t)i + (int64_t)ng.n[0] * ((int64_t)j + (int64_t)ng.n[1] * k);
+    const int64_t f = (int64_t)i + (int64_t)o.n0 * ((int64_t)j + (int64_t)o.n1 * k);
     float val = W > 0.0f ? S / W : S;
-    if (a.grid_old[c]) a.grid_old[c][f] = val;
-    if (a.wgt[c]) a.wgt[c][f] = W;
-    if (a.grav_on[c]) {
+    if (o.old) o.old[f] = val;
+    if (o.wgt) o.wgt[f] = W;
+    if (o.grav_on) {
         const int ax = c == 0 ? i : (c == 1 ? j : k);
-        const int nax = c == 0 ? a.ni : (c == 1 ? a.nj : a.nk);
         // staged cell of (i,j,k); the cell below along c is one step back
         const int hi = (i - i0 + 1) + SX * ((j - j0 + 1) + SY * (k - k0 + 1));
         const int step = c == 0 ? 1 : (c == 1 ? SX : SX * SY);
         bool fluid = false;
-        if (ax < nax) fluid = fluid || occ[hi] != 0;
+        if (ax < o.nax) fluid = fluid || occ[hi] != 0;
         if (ax > 0) fluid = fluid || occ[hi - step] != 0;
-        if (fluid) val += a.grav_delta[c];
+        if (fluid) val += o.grav_delta;
     }
-    a.grid[c][f] = val;
+    o.grid[f] = val;
 }
 
 // Range r (0..15) of the 4x4 (y,z) rows of half-cells around a face; each
@@ -1252,13 +1275,13 @@ __device__ __forceinline__ void brick_accum(const StepP2G &a, const float *sx,
         brick_gather<C>(sx, sy, sz, sv, hs, li + 1, lj + 1, lk + 1, gx, gy, gz, offh, inv_r2, S, W);
 }
 
-__device__ __forceinline__ void brick_zero_face(const StepP2G &a, int c, int fi, int fj, int fk) {
+__device__ __forceinline__ void brick_zero_face(const StepP2G &a, const FaceOut &o, int c,
+                                                int fi, int fj, int fk) {
     if (fi >= a.ni + (c == 0) || fj >= a.nj + (c == 1) || fk >= a.nk + (c == 2)) return;
-    const NodeGrid &ng = a.ng[c];
-    const int64_t f = (int64_t)fi + (int64_t)ng.n[0] * ((int64_t)fj + (int64_t)ng.n[1] * fk);
-    if (a.grid_old[c]) a.grid_old[c][f] = 0.0f;
-    if (a.wgt[c]) a.wgt[c][f] = 0.0f;
-    a.grid[c][f] = 0.0f;
+    const int64_t f = (int64_t)fi + (int64_t)o.n0 * ((int64_t)fj + (int64_t)o.n1 * fk);
+    if (o.old) o.old[f] = 0.0f;
+    if (o.wgt) o.wgt[f] = 0.0f;
+    o.grid[f] = 0.0f;
 }
 
 // One work-group per brick of BX*BY*BZ cells; wave w computes the faces of
@@ -1325,6 +1348,7 @@ __device__ __forceinline__ void p2g_brick(
     // the faces on the domain's upper planes belong to the last brick:
     // lanes 0..15 of the wave take its 4x4 plane
     const int e0 = lane & 3, e1 = (lane >> 2) & 3;
+    const FaceOut fo = face_out(a, wave);
     const bool xtra = (wave == 0 && i0 + BX == a.ni) || (wave == 1 && j0 + BY == a.nj) ||
                       (wave == 2 && k0 + BZ == a.nk);
     int eli, elj, elk;
@@ -1337,8 +1361,8 @@ __device__ __forceinline__ void p2g_brick(
     if (N == 0) {
         // nothing within reach of any face of this brick: all values are 0
         // and no face borders fluid
-        if (own_ok) brick_zero_face(a, wave, i0 + li, j0 + lj, k0 + lk);
-        if (ext_ok) brick_zero_face(a, wave, i0 + eli, j0 + elj, k0 + elk);
+        if (own_ok) brick_zero_face(a, fo, wave, i0 + li, j0 + lj, k0 + lk);
+        if (ext_ok) brick_zero_face(a, fo, wave, i0 + eli, j0 + elj, k0 + elk);
         return;
     }
 
@@ -1368,13 +1392,13 @@ __device__ __forceinline__ void p2g_brick(
 #pragma unroll
         for (int m = 0; m < PER; m++) {
             const uint32_t q = rq[m];
-            rx[m] = a.x[q];
-            ry[m] = a.y[q];
-            rz[m] = a.z[q];
-            rvx[m] = a.vel[0][q];
-            rvy[m] = a.vel[1][q];
-            rvz[m] = a.vel[2][q];
-            rk[m] = hrank[q];
+            rx[m] = gld(a.x, q);
+            ry[m] = gld(a.y, q);
+            rz[m] = gld(a.z, q);
+            rvx[m] = gld(a.vel[0], q);
+            rvy[m] = gld(a.vel[1], q);
+            rvz[m] = gld(a.vel[2], q);
+            rk[m] = gld(hrank, q);
         }
 #pragma unroll
         for (int m = 0; m < PER; m++) {
@@ -1461,8 +1485,8 @@ __device__ __forceinline__ void p2g_brick(
         }
         if (!CHUNKED) break;
     }
-    if (own_ok) brick_epilogue(a, s_occ, wave, i0 + li, j0 + lj, k0 + lk, i0, j0, k0, S, W);
-    if (ext_ok) brick_epilogue(a, s_occ, wave, i0 + eli, j0 + elj, k0 + elk, i0, j0, k0, SE, WE);
+    if (own_ok) brick_epilogue(fo, s_occ, wave, i0 + li, j0 + lj, k0 + lk, i0, j0, k0, S, W);
+    if (ext_ok) brick_epilogue(fo, s_occ, wave, i0 + eli, j0 + elj, k0 + elk, i0, j0, k0, SE, WE);
 }
 
 // The common P2G kernel: one work-group of four waves per brick.  A brick
@@ -1477,10 +1501,33 @@ __device__ __forceinline__ void p2g_brick(
 // faces on the domain's upper planes when the brick has them.  A brick
 // holding more than CAP particles is appended to dense[] for
 // k_flip_p2g_dense.
+// Row bin starts of a P2G brick: row (cy, cz) holds world cells i0-1 ..
+// i0+4 at (j0-1+cy, k0-1+cz), bins i0 .. i0+5 (pad 1); thread t < 36*7
+// loads start t of row t/7.  A row outside the bin grid, or cells beyond
+// its end, are empty.
+__device__ __forceinline__ uint32_t brick_row_start(const StepP2G &a, const BinGrid &bg,
+                                                    int nbx, int nby, int brick_id,
+                                                    const uint32_t *__restrict__ start, int t) {
+    using namespace brick;
+    constexpr int NB = SX + 1;
+    const int bxi = brick_id % nbx, byi = (brick_id / nbx) % nby, bzi = brick_id / (nbx * nby);
+    const int i0 = bxi * BX, j0 = byi * BY, k0 = bzi * BZ;
+    uint32_t v = 0;
+    if (t < NROW * NB) {
+        const int r = t / NB, k = t % NB;
+        const int wy = j0 - 1 + r % SY, wz = k0 - 1 + r / SY;
+        if (wy <= a.nj && wz <= a.nk) {
+            const uint32_t row = (uint32_t)bg.b[0] * ((uint32_t)(wy + 1) + (uint32_t)bg.b[1] * (uint32_t)(wz + 1));
+            v = gld(start, row + (uint32_t)min(i0 + k, bg.b[0]));
+        }
+    }
+    return v;
+}
+
 template <int V, int CAP>
 __device__ __forceinline__ void p2g_brick_rows(
     const StepP2G &a, const BinGrid &bg, double dx, float inv_r2, int nbx, int nby,
-    int brick_id, uint32_t gid, const uint32_t *__restrict__ start,
+    int brick_id, uint32_t gid, uint32_t v,
     const uint32_t *__restrict__ hrank, const uint8_t *__restrict__ soct, bool flagged,
     uint32_t *__restrict__ dense, uint32_t *__restrict__ dense_count) {
     using namespace brick;
@@ -1525,24 +1572,14 @@ __device__ __forceinline__ void p2g_brick_rows(
         ok = ok && lane < 48;
     }
     ok = ok && i0 + fi < a.ni + (c == 0) && j0 + fj < a.nj + (c == 1) && k0 + fk < a.nk + (c == 2);
+    const FaceOut fo = face_out(a, c);
 
-    // row bin starts: row (cy, cz) holds world cells i0-1 .. i0+4 at
-    // (j0-1+cy, k0-1+cz), bins i0 .. i0+5 (pad 1); a row outside the bin
-    // grid, or cells beyond its end, are empty.  Loaded together with the
-    // brick's flag.
-    uint32_t v = 0;
-    if (t < NROW * NB) {
-        const int r = t / NB, k = t % NB;
-        const int wy = j0 - 1 + r % SY, wz = k0 - 1 + r / SY;
-        if (wy <= a.nj && wz <= a.nk) {
-            const uint32_t row = (uint32_t)bg.b[0] * ((uint32_t)(wy + 1) + (uint32_t)bg.b[1] * (uint32_t)(wz + 1));
-            v = start[row + (uint32_t)min(i0 + k, bg.b[0])];
-        }
-    }
+    // v: this thread's row bin start (brick_row_start), loaded by the
+    // caller ahead of time together with the brick's flag
     if (!flagged) {
         // no particle within reach of any face of this brick: all values
         // are 0 and no face borders fluid
-        if (ok) brick_zero_face(a, c, i0 + fi, j0 + fj, k0 + fk);
+        if (ok) brick_zero_face(a, fo, c, i0 + fi, j0 + fj, k0 + fk);
         return;
     }
     if (t < NROW * NB) s_bs[t / NB][t % NB] = v;
@@ -1556,7 +1593,7 @@ __device__ __forceinline__ void p2g_brick_rows(
         return;
     }
     if (total == 0) {
-        if (ok) brick_zero_face(a, c, i0 + fi, j0 + fj, k0 + fk);
+        if (ok) brick_zero_face(a, fo, c, i0 + fi, j0 + fj, k0 + fk);
         return;
     }
     if (t < NCS) {
@@ -1575,9 +1612,11 @@ __device__ __forceinline__ void p2g_brick_rows(
     };
 
     // count: each wave its rows; the first 64 particles of a row stay in
-    // registers for placing, the rest are read again then
-    float rx[RPW], ry[RPW], rz[RPW], rvx[RPW], rvy[RPW], rvz[RPW];
-    uint32_t rd[RPW];   // half-cell | rank << 16
+    // registers for placing, the rest are read again then.  Every row's
+    // loads are issued before any is used (octants and ranks first, which
+    // the counting needs first), so the wave waits for global memory once
+    // rather than once per row.
+    uint32_t rq[RPW], rlen[RPW];
 #pragma unroll
     for (int m = 0; m < RPW; m++) {
         const int r = wave + 4 * m;
@@ -1586,25 +1625,43 @@ __device__ __forceinline__ void p2g_brick_rows(
             base = s_bs[r][0];
             len = s_bs[r][NB - 1] - base;
         }
-        const bool valid = (uint32_t)lane < len;
-        const uint32_t q = valid ? base + (uint32_t)lane : 0u;
-        rx[m] = a.x[q];
-        ry[m] = a.y[q];
-        rz[m] = a.z[q];
-        rvx[m] = a.vel[0][q];
-        rvy[m] = a.vel[1][q];
-        rvz[m] = a.vel[2][q];
-        const uint32_t o = soct[q], rk = hrank[q];
+        rlen[m] = len;
+        rq[m] = (uint32_t)lane < len ? base + (uint32_t)lane : 0u;
+    }
+    uint32_t ro[RPW], rk[RPW];
+#pragma unroll
+    for (int m = 0; m < RPW; m++) {
+        ro[m] = gld(soct, rq[m]);
+        rk[m] = gld(hrank, rq[m]);
+    }
+    float rx[RPW], ry[RPW], rz[RPW], rvx[RPW], rvy[RPW], rvz[RPW];
+#pragma unroll
+    for (int m = 0; m < RPW; m++) {
+        rx[m] = gld(a.x, rq[m]);
+        ry[m] = gld(a.y, rq[m]);
+        rz[m] = gld(a.z, rq[m]);
+        rvx[m] = gld(a.vel[0], rq[m]);
+        rvy[m] = gld(a.vel[1], rq[m]);
+        rvz[m] = gld(a.vel[2], rq[m]);
+    }
+    uint32_t rd[RPW];   // half-cell | rank << 16
+#pragma unroll
+    for (int m = 0; m < RPW; m++) {
         rd[m] = 0xFFFFFFFFu;
-        if (valid) {
-            const uint32_t h = half_cell(r, q, o);
+        if ((uint32_t)lane < rlen[m]) {
+            const uint32_t h = half_cell(wave + 4 * m, rq[m], ro[m]);
             atomicAdd(&s_cnt[h], 1u);
-            rd[m] = h | (rk << 16);
+            rd[m] = h | (rk[m] << 16);
         }
-        for (uint32_t off = 64; off < len; off += 64) {
-            if (off + (uint32_t)lane < len) {
+    }
+#pragma unroll
+    for (int m = 0; m < RPW; m++) {
+        const int r = wave + 4 * m;
+        const uint32_t base = r < NROW ? s_bs[r][0] : 0u;
+        for (uint32_t off = 64; off < rlen[m]; off += 64) {
+            if (off + (uint32_t)lane < rlen[m]) {
                 const uint32_t q2 = base + off + (uint32_t)lane;
-                atomicAdd(&s_cnt[half_cell(r, q2, soct[q2])], 1u);
+                atomicAdd(&s_cnt[half_cell(r, q2, gld(soct, q2))], 1u);
             }
         }
     }
@@ -1655,13 +1712,17 @@ __device__ __forceinline__ void p2g_brick_rows(
             for (uint32_t off = 64; off < len; off += 64) {
                 if (off + (uint32_t)lane < len) {
                     const uint32_t q2 = base + off + (uint32_t)lane;
-                    const uint32_t d = (uint32_t)s_hs[half_cell(r, q2, soct[q2])] + hrank[q2];
-                    s_x[d] = a.x[q2];
-                    s_y[d] = a.y[q2];
-                    s_z[d] = a.z[q2];
-                    s_v0[d] = a.vel[0][q2];
-                    s_v1[d] = a.vel[1][q2];
-                    s_v2[d] = a.vel[2][q2];
+                    const float px = gld(a.x, q2), py = gld(a.y, q2), pz = gld(a.z, q2);
+                    const float pvx = gld(a.vel[0], q2), pvy = gld(a.vel[1], q2);
+                    const float pvz = gld(a.vel[2], q2);
+                    const uint32_t d = (uint32_t)s_hs[half_cell(r, q2, gld(soct, q2))] +
+                                       gld(hrank, q2);
+                    s_x[d] = px;
+                    s_y[d] = py;
+                    s_z[d] = pz;
+                    s_v0[d] = pvx;
+                    s_v1[d] = pvy;
+                    s_v2[d] = pvz;
                 }
             }
         }
@@ -1673,7 +1734,7 @@ __device__ __forceinline__ void p2g_brick_rows(
         if (c == 0) brick_accum<0, V>(a, s_x, s_y, s_z, s_v0, s_hs, i0, j0, k0, fi, fj, fk, dx, inv_r2, S, W);
         else if (c == 1) brick_accum<1, V>(a, s_x, s_y, s_z, s_v1, s_hs, i0, j0, k0, fi, fj, fk, dx, inv_r2, S, W);
         else brick_accum<2, V>(a, s_x, s_y, s_z, s_v2, s_hs, i0, j0, k0, fi, fj, fk, dx, inv_r2, S, W);
-        brick_epilogue(a, s_occ, c, i0 + fi, j0 + fj, k0 + fk, i0, j0, k0, S, W);
+        brick_epilogue(fo, s_occ, c, i0 + fi, j0 + fj, k0 + fk, i0, j0, k0, S, W);
     }
 }
 
@@ -1688,10 +1749,12 @@ __global__ __launch_bounds__(brick::NTC) void k_flip_p2g_brick(
     // (work-groups are dealt round-robin over the 8 XCDs)
     const uint32_t gid = (blockIdx.x & 7u) * (uint32_t)per_xcd + (blockIdx.x >> 3);
     if (gid >= nbrick) return;
-    const bool flagged = brick_flag[gid] != 0;
     const SceneDesc &sd = scenes[scene_of(scenes, nsc, gid, [](const SceneDesc &d) { return d.k0; })];
+    const bool flagged = gld(brick_flag, gid) != 0;
+    const uint32_t v = brick_row_start(sd.p2g, sd.bg, sd.nbx, sd.nby, (int)(gid - sd.k0),
+                                       start + sd.b0, threadIdx.x);
     p2g_brick_rows<V, CAP>(sd.p2g, sd.bg, sd.dx, (float)(1.0 / (sd.dx * sd.dx)), sd.nbx, sd.nby,
-                           (int)(gid - sd.k0), gid, start + sd.b0, hrank, soct, flagged, dense,
+                           (int)(gid - sd.k0), gid, v, hrank, soct, flagged, dense,
                            dense_count);
 }
 

@@ -380,7 +380,7 @@ std::vector<gfsx::SceneDesc> flip_step_impl(gfsx_ctx *ctx, const gfsx_scene *sce
         s.vs[c] = ctx->f32(S_VS0 + c, s.tot.n);
     }
     s.dense = ctx->u32(S_DENSE, s.tot.np2g);
-    s.dense_count = ctx->u32(S_DENSEN, 1);
+    s.dense_count = ctx->u32(S_DENSEN, 2);
     s.num_cu = ctx->num_cu;
     gfsx::Sorted so;
     for (int c = 0; c < 3; c++) {

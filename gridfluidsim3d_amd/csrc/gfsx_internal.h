@@ -375,7 +375,8 @@ struct StepBatch {
     const SceneDesc *scenes_dev;     // the same table on the device
     SceneTotals tot;
     float *xs[3], *vs[3];            // workspace: all particles in sort order
-    uint32_t *dense, *dense_count;   // workspace: P2G bricks over capacity
+    uint32_t *dense, *dense_count;   // workspace: P2G bricks over capacity (list; count
+                                     // and next unprocessed entry)
     int num_cu = 1;                  // compute units of the device
     int stencil;
     int keep_order;

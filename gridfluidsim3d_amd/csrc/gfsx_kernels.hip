@@ -1759,7 +1759,10 @@ __global__ __launch_bounds__(brick::NTC) void k_flip_p2g_brick(
 }
 
 // The bricks listed by k_flip_p2g_brick as over its capacity, in chunks.
-// Three waves per SIMD (a register budget of 168).
+// Three waves per SIMD (a register budget of 168).  A work-group takes the
+// next unprocessed brick of the list from a counter (dense_count[1]) each
+// time it finishes one, so large and small bricks balance over the
+// work-groups; the order does not change any result.
 template <int V, int CAP>
 __global__ __launch_bounds__(brick::NT, 3) void k_flip_p2g_dense(
     const SceneDesc *__restrict__ scenes, int nsc, Sorted so,
@@ -1773,9 +1776,14 @@ __global__ __launch_bounds__(brick::NT, 3) void k_flip_p2g_dense(
     __shared__ uint32_t s_rowq[NROW], s_rows0[NROW + 1];
     __shared__ uint32_t s_wsum[NT / 64];
     __shared__ uint8_t s_occ[SX * SY * SZ];
-    const uint32_t nd = *dense_count;
-    for (uint32_t d = blockIdx.x; d < nd; d += gridDim.x) {
-        __syncthreads();   // the previous brick is done with LDS
+    __shared__ uint32_t s_next;
+    const uint32_t nd = dense_count[0];
+    while (true) {
+        __syncthreads();   // the previous brick is done with LDS and s_next
+        if (threadIdx.x == 0) s_next = atomicAdd(&dense_count[1], 1u);
+        __syncthreads();
+        const uint32_t d = s_next;
+        if (d >= nd) break;
         const uint32_t gid = dense[d];
         const SceneDesc &sd = scenes[scene_of(scenes, nsc, gid, [](const SceneDesc &e) { return e.k0; })];
         p2g_brick<V, CAP, true>(sd.p2g, sd.bg, sd.dx, (float)(1.0 / (sd.dx * sd.dx)), sd.nbx, sd.nby,
@@ -2283,7 +2291,7 @@ void flip_step(const StepBatch &s, const SortBuffers &sb, hipStream_t st, hipEve
         auto *kern_dense = s.p2g_variant == 2 ? k_flip_p2g_dense<2, dcap>
                          : s.p2g_variant == 3 ? k_flip_p2g_dense<3, dcap>
                                               : k_flip_p2g_dense<0, dcap>;
-        fill_u32(s.dense_count, 0u, 1, st);
+        fill_u32(s.dense_count, 0u, 2, st);
         hipLaunchKernelGGL(kern, dim3((unsigned)(8 * (int64_t)per_xcd)), dim3(brick::NTC), 0, st,
                            s.scenes_dev, nsc, so, nbrick, per_xcd, (const uint32_t *)sb.start,
                            (const uint32_t *)sb.hrank, (const uint8_t *)sb.soct,

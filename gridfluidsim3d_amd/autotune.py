@@ -4,15 +4,16 @@ keeps the fastest.
 
 The variants are selected with ``Context.set_option``; every variant gives
 the same result for G2P and results within FP32 rounding of each other for
-P2G (the candidates of a face are summed in the same order, the arithmetic
-is contracted differently), so the choice is about speed only.  Timing runs
+P2G (the gathers sum the candidates of a face in the same order with the
+arithmetic contracted differently; the scatter sums in fixed point), so the
+choice is about speed only.  Timing runs
 on copies of the states, so the caller's states do not advance.
 """
 import copy
 
 # option -> candidate values (direct variants take one scene only)
 CANDIDATES = {
-    "p2g_variant": (0, 2, 3),
+    "p2g_variant": (4, 0, 2, 3),
 }
 
 

@@ -39,6 +39,7 @@ enum Slot {
     S_D_POS, S_D_VAL, S_D_S, S_D_W, S_D_U, S_D_V, S_D_WW, S_D_OUT,
     S_B_GRID, S_B_OLD, S_B_WGT, S_B_LG, S_B_XO, S_B_VO, S_B_FX, S_B_MAX,
     S_E_N, S_E_W, S_E_LAYER, S_E_LIST, S_R_KEEP, S_R_DST, S_R_SCAN, S_R_N,
+    S_P_TILE, S_P_TFLAG,
     NSLOT
 };
 
@@ -56,7 +57,7 @@ struct gfsx_ctx {
     int num_cu = 1;
     void *buf[NSLOT] = {};
     size_t bytes[NSLOT] = {};
-    int p2g_variant = 0;
+    int p2g_variant = 4;
     int g2p_variant = 0;
     int adjoint_variant = 0;
     bool timing = false;
@@ -381,6 +382,10 @@ std::vector<gfsx::SceneDesc> flip_step_impl(gfsx_ctx *ctx, const gfsx_scene *sce
     }
     s.dense = ctx->u32(S_DENSE, s.tot.np2g);
     s.dense_count = ctx->u32(S_DENSEN, 2);
+    if (ctx->p2g_variant == 4) {
+        s.tiles = (float2 *)ctx->f32(S_P_TILE, 2 * 3 * 216 * s.tot.np2g);
+        s.tile_flag = (uint8_t *)ctx->u32(S_P_TFLAG, (s.tot.np2g + 3) / 4);
+    }
     s.num_cu = ctx->num_cu;
     gfsx::Sorted so;
     for (int c = 0; c < 3; c++) {
@@ -894,7 +899,7 @@ void gfsx_set_option(gfsx_ctx *ctx, const char *name, int value, int *err) {
     guarded(ctx, err, [&] {
         require(name != nullptr, "gfsx_set_option: null name");
         if (std::strcmp(name, "p2g_variant") == 0) {
-            require(value >= 0 && value <= 3, "gfsx_set_option: p2g_variant must be 0..3");
+            require(value >= 0 && value <= 4, "gfsx_set_option: p2g_variant must be 0..4");
             ctx->p2g_variant = value;
         } else if (std::strcmp(name, "g2p_variant") == 0) {
             require(value >= 0 && value <= 1, "gfsx_set_option: g2p_variant must be 0..1");

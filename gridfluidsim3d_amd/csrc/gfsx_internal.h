@@ -121,6 +121,7 @@ struct SceneDesc {
     uint32_t k0;
     int gbx, gby, gbz;
     uint32_t g0;
+    int64_t f0;       // first global MAC face (faces of u, then v, then w, per scene)
     // grid and step constants
     int ni, nj, nk;
     double dx;
@@ -377,10 +378,13 @@ struct StepBatch {
     float *xs[3], *vs[3];            // workspace: all particles in sort order
     uint32_t *dense, *dense_count;   // workspace: P2G bricks over capacity (list; count
                                      // and next unprocessed entry)
+    float2 *tiles = nullptr;         // workspace of the scatter P2G: 3 * 216 (S, W) per brick
+    uint8_t *tile_flag = nullptr;    // per brick: its tile was written
     int num_cu = 1;                  // compute units of the device
     int stencil;
     int keep_order;
-    int p2g_variant;   // 0: brick-staged gather, 1: direct global gather (one scene)
+    int p2g_variant;   // 0, 2, 3: brick-staged gather, 1: direct global gather (one scene),
+                       // 4: scatter into per-brick tiles + combine
     int g2p_variant;   // 0: brick-staged LDS boxes, 1: direct global sampling (one scene)
     const FsEpilogue *epi = nullptr;   // FluidSimulation epilogue after P2G (one scene)
     bool grids_only = false;           // stop after P2G (the adjoint's recompute)

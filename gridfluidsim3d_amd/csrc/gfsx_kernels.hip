@@ -598,7 +598,10 @@ __device__ __forceinline__ void wave_order_bin(uint32_t s, uint32_t len,
         const uint32_t t = __shfl_up(carry, d, 64);
         if (lane >= d) carry = max(carry, t);
     }
-    const uint32_t from_below = lane > 0 ? __shfl_up(carry, 1, 64) : 0u;
+    // (the shuffle runs on every lane: evaluated under lane > 0, lane 0
+    // would be inactive and lane 1 would read 0 instead of its value)
+    const uint32_t carry_up = __shfl_up(carry, 1, 64);
+    const uint32_t from_below = lane > 0 ? carry_up : 0u;
 #pragma unroll
     for (int r = 0; r < K; r++) {
         const uint32_t e = (uint32_t)(lane * K + r);
@@ -1792,6 +1795,439 @@ __global__ __launch_bounds__(brick::NT, 3) void k_flip_p2g_dense(
     }
 }
 
+// ---------------------------------------------------------------------------
+// P2G by scatter (p2g_variant 4).
+//
+// k_p2g_scatter: one work-group per P2G brick takes the particles of its own
+// cells only (4^3 cells; the first and last brick of an axis also own the
+// pad cells beyond the domain).  Every particle adds K and K v to the 2x2x2
+// nodes of each component around it -- the candidates of the gather: the
+// cell and half-cell come from the same FP64 coordinates, and K is the same
+// FP32 expression -- in a tile of 6^3 nodes per component in LDS.  Node n
+// of axis a is tile entry n - 4b + (a != c): the own axis covers nodes
+// 4b..4b+4, the two others 4b-1..4b+4, which holds every node a particle of
+// the brick reaches.  The sums are integers: K and K v are rounded to fixed
+// point, 64-bit, so they do not depend on the order of the additions: K and
+// K (v + V), V = the brick's largest |v| (so that every term is >= 0 and
+// S = S' - V W), scaled by powers of two chosen per brick so that no sum can
+// exceed 2^62, then rounded to integers.  A term keeps the 24 significant
+// bits of its float product whatever its size, so the faces at the edge
+// of the particles' reach, where every K is small, keep their relative
+// precision.  Lane l adds into copy l & 1 of the tile, which spreads the
+// atomics of the particles of one cell over two addresses.
+// Non-finite products (a velocity that is not finite, or of magnitude
+// 2^100 or more) are not added; they set flag bits on the node instead and
+// the node's value becomes what the float sum would give: NaN, +inf or -inf
+// (for |v| >= 2^100 the float sum may stay finite; such a velocity is taken
+// as infinite).
+// At the end the copies are summed, converted back to float and the tile
+// written to the brick's slot of the tile buffer.
+//
+// k_p2g_combine: one work-group per brick computes the values of its faces
+// (the faces the gather kernels assign to it) by adding, in a fixed order,
+// the entries of the at most two bricks per axis whose tiles hold the face,
+// then applies the weight and the body force.
+namespace sbrick {
+constexpr int T = 6;               // tile edge, nodes, for every component
+constexpr int TN = T * T * T;      // nodes per component tile
+constexpr int NREP = 2;            // tile copies
+constexpr int NT = 256;
+constexpr int MAXROW = 36;         // (y,z) bin rows of a brick, pads included
+constexpr int PRE = 2;             // particles per thread kept in registers
+constexpr uint32_t BAD_NAN = 1u, BAD_PINF = 2u, BAD_NINF = 4u;
+}  // namespace sbrick
+
+// K of one tile corner from the per-axis squared distances.
+__device__ __forceinline__ float corner_k(float sx, float sy, float sz, float inv_r2) {
+    return wyvill(((sx + sy) + sz) * inv_r2);
+}
+
+// x >= 0 (a float below 2^63) rounded to an integer, as 64 bits: the high
+// word is floor(x / 2^32); the low word x - high * 2^32 is exact in float
+// (below 2^32, and for x >= 2^32 it keeps at most x's 24 significant bits).
+__device__ __forceinline__ unsigned long long fix64(float x) {
+    const float xr = rintf(x);
+    const float hi = floorf(xr * 0x1p-32f);
+    const float lo = fmaf(hi, -0x1p32f, xr);
+    return ((unsigned long long)(uint32_t)hi << 32) | (unsigned long long)(uint32_t)lo;
+}
+
+// Add K and K (v + V) for the 8 corners of one component of one particle.
+// (tx, ty, tz) is the tile position of corner (0,0,0); sq[a][0..1] the
+// squared distances to the two nodes along axis a; vs = (v + V) * scale of
+// the S sums.  FAST: every corner lies in the tile and v is finite (no
+// checks).  Otherwise corners outside the tile are skipped and, with vbad,
+// the product K v is recorded in the node's flag bits instead of the sum.
+template <bool FAST>
+__device__ __forceinline__ void scatter_comp(unsigned long long *tS, unsigned long long *tW,
+                                             uint32_t *tbad, int tx, int ty, int tz,
+                                             const float sq[3][2], float vs, float sW,
+                                             float inv_r2, bool vbad, uint32_t badbits) {
+    using namespace sbrick;
+#pragma unroll
+    for (int dz = 0; dz < 2; dz++)
+#pragma unroll
+        for (int dy = 0; dy < 2; dy++)
+#pragma unroll
+            for (int dx = 0; dx < 2; dx++) {
+                if (!FAST) {
+                    if ((unsigned)(tx + dx) >= (unsigned)T || (unsigned)(ty + dy) >= (unsigned)T ||
+                        (unsigned)(tz + dz) >= (unsigned)T)
+                        continue;
+                }
+                const int idx = (tx + dx) + T * ((ty + dy) + T * (tz + dz));
+                const float K = corner_k(sq[0][dx], sq[1][dy], sq[2][dz], inv_r2);
+                atomicAdd(&tW[idx], fix64(K * sW));
+                if (FAST || !vbad) {
+                    atomicAdd(&tS[idx], fix64(K * vs));
+                } else {
+                    // the float sum would add K * v: NaN when K is 0
+                    atomicOr(&tbad[idx], K == 0.0f ? BAD_NAN : badbits);
+                }
+            }
+}
+
+__global__ __launch_bounds__(sbrick::NT) void k_p2g_scatter(
+    const SceneDesc *__restrict__ scenes, int nsc, uint32_t nbrick, int per_xcd,
+    const uint32_t *__restrict__ start, const uint8_t *__restrict__ brick_flag,
+    float2 *__restrict__ tiles, uint8_t *__restrict__ tile_flag) {
+    using namespace sbrick;
+    __shared__ unsigned long long s_S[NREP][3][TN], s_W[NREP][3][TN];
+    __shared__ uint32_t s_bad[3][TN];
+    __shared__ uint32_t s_rowq[MAXROW], s_rows0[MAXROW + 1];
+    __shared__ float s_red[NT / 64][3];
+    __shared__ uint32_t s_cmax;
+    const uint32_t gid = (blockIdx.x & 7u) * (uint32_t)per_xcd + (blockIdx.x >> 3);
+    if (gid >= nbrick) return;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    if (gld(brick_flag, gid) == 0) {
+        // no particle within reach of this brick, so none in its cells
+        if (t == 0) tile_flag[gid] = 0;
+        return;
+    }
+    const SceneDesc &sd = scenes[scene_of(scenes, nsc, gid, [](const SceneDesc &d) { return d.k0; })];
+    const BinGrid &bg = sd.bg;
+    const int lb = (int)(gid - sd.k0);
+    const int bxi = lb % sd.nbx, byi = (lb / sd.nbx) % sd.nby, bzi = lb / (sd.nbx * sd.nby);
+    const int i0 = bxi * 4, j0 = byi * 4, k0 = bzi * 4;
+    // bins of the brick's own cells (bin = cell + 1), pads at the ends
+    const int bx_lo = bxi == 0 ? 0 : i0 + 1, bx_hi = bxi == sd.nbx - 1 ? bg.b[0] : i0 + 5;
+    const int by_lo = byi == 0 ? 0 : j0 + 1, by_hi = byi == sd.nby - 1 ? bg.b[1] : j0 + 5;
+    const int bz_lo = bzi == 0 ? 0 : k0 + 1, bz_hi = bzi == sd.nbz - 1 ? bg.b[2] : k0 + 5;
+    const int ny = by_hi - by_lo, nrow = ny * (bz_hi - bz_lo);
+    const uint32_t *st0 = start + sd.b0;
+    if (wave == 0) {
+        uint32_t q0 = 0, len = 0;
+        if (lane < nrow) {
+            const uint32_t by = (uint32_t)(by_lo + lane % ny), bz = (uint32_t)(bz_lo + lane / ny);
+            const uint32_t row = (uint32_t)bg.b[0] * (by + (uint32_t)bg.b[1] * bz);
+            q0 = gld(st0, row + (uint32_t)bx_lo);
+            len = gld(st0, row + (uint32_t)bx_hi) - q0;
+            s_rowq[lane] = q0;
+        }
+        const uint32_t inc = wave_inclusive_sum(len);
+        if (lane < nrow) s_rows0[lane + 1] = inc;
+        if (lane == 0) s_rows0[0] = 0u;
+    }
+    // particles of this thread's cell (one thread per cell), for the
+    // largest cell count
+    const int ncx = bx_hi - bx_lo;
+    uint32_t ccount = 0;
+    if (t < nrow * ncx) {
+        const int r = t / ncx, k = t - r * ncx;
+        const uint32_t by = (uint32_t)(by_lo + r % ny), bz = (uint32_t)(bz_lo + r / ny);
+        const uint32_t cell = (uint32_t)bg.b[0] * (by + (uint32_t)bg.b[1] * bz) + (uint32_t)(bx_lo + k);
+        ccount = gld(st0, cell + 1) - gld(st0, cell);
+    }
+    if (t == 0) s_cmax = 0u;
+    {
+        unsigned long long *z = &s_S[0][0][0];
+        for (int e = t; e < NREP * 3 * TN; e += NT) z[e] = 0ull;
+        z = &s_W[0][0][0];
+        for (int e = t; e < NREP * 3 * TN; e += NT) z[e] = 0ull;
+        for (int e = t; e < 3 * TN; e += NT) (&s_bad[0][0])[e] = 0u;
+    }
+    __syncthreads();
+    const uint32_t N = s_rows0[nrow];
+    if (N == 0) {
+        if (t == 0) tile_flag[gid] = 0;
+        return;
+    }
+    if (ccount > 0u) atomicMax(&s_cmax, ccount);
+    const StepP2G &a = sd.p2g;
+    auto slot_of = [&](uint32_t s) {
+        int lo = 0, hi = nrow - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (s_rows0[mid] <= s) lo = mid;
+            else hi = mid - 1;
+        }
+        return s_rowq[lo] + (s - s_rows0[lo]);
+    };
+
+    // pass 1: the largest finite |v| per component (velocities of the first
+    // PRE particles of each thread, and their positions, stay in registers)
+    float px[PRE], py[PRE], pz[PRE], pv[PRE][3];
+    float vm[3] = {0.0f, 0.0f, 0.0f};
+    constexpr float kBig = 1.2676506e30f;   // 2^100
+#pragma unroll
+    for (int m = 0; m < PRE; m++) {
+        const uint32_t s = (uint32_t)t + (uint32_t)(m * NT);
+        const uint32_t q = s < N ? slot_of(s) : s_rowq[0];
+        px[m] = gld(a.x, q);
+        py[m] = gld(a.y, q);
+        pz[m] = gld(a.z, q);
+#pragma unroll
+        for (int c = 0; c < 3; c++) pv[m][c] = gld(a.vel[c], q);
+    }
+#pragma unroll
+    for (int m = 0; m < PRE; m++) {
+        if ((uint32_t)t + (uint32_t)(m * NT) < N) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const float av = fabsf(pv[m][c]);
+                if (av < kBig) vm[c] = fmaxf(vm[c], av);
+            }
+        }
+    }
+    for (uint32_t s = (uint32_t)t + (uint32_t)(PRE * NT); s < N; s += NT) {
+        const uint32_t q = slot_of(s);
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const float av = fabsf(gld(a.vel[c], q));
+            if (av < kBig) vm[c] = fmaxf(vm[c], av);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        float v = vm[c];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+        if (lane == 0) s_red[wave][c] = v;
+    }
+    __syncthreads();
+    // scales.  A node takes particles of at most 2x3x3 cells (its
+    // neighbours within dx), so at most bound = min(N, 18 * largest cell
+    // count) terms, each K <= 1 and K (v + V) <= 2V: bound * scale <= 2^62
+    // for W, bound * 2V * scale <= 2^62 for S'.
+    const double bound = (double)min(N, 18u * s_cmax);
+    const float sW = ldexpf(1.0f, ilogb(4611686018427387904.0 / bound));
+    float sS[3], vbias[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        float v = 0.0f;
+#pragma unroll
+        for (int w = 0; w < NT / 64; w++) v = fmaxf(v, s_red[w][c]);
+        vbias[c] = v;
+        const int e = v > 0.0f ? ilogb(2305843009213693952.0 / (bound * (double)v)) : 0;
+        sS[c] = ldexpf(1.0f, e);
+    }
+
+    // pass 2: scatter
+    const double dxd = sd.dx;
+    const float inv_r2 = (float)(1.0 / (sd.dx * sd.dx));
+    const float half = a.ng[0].off[1];
+    const int rep = lane & (NREP - 1);
+    const int ob[3] = {i0, j0, k0};
+    auto scatter = [&](float x, float y, float z, const float v[3]) {
+        const float p[3] = {x, y, z};
+        int ia[3], ib[3];
+        float sq[2][3][2];   // [own?0:1][axis][node]
+#pragma unroll
+        for (int ax = 0; ax < 3; ax++) {
+            const AxisCoords cc = axis_coords(p[ax], bg.inv_dx);
+            ia[ax] = cc.ia;
+            ib[ax] = cc.ib;
+            const float ga0 = (float)((float)cc.ia * dxd), ga1 = (float)((float)(cc.ia + 1) * dxd);
+            const float gb0 = (float)((float)cc.ib * dxd), gb1 = (float)((float)(cc.ib + 1) * dxd);
+            const float ph = p[ax] - half;   // p -= offset (scalarfield.cpp:168)
+            const float d0 = ga0 - p[ax], d1 = ga1 - p[ax];
+            const float e0 = gb0 - ph, e1 = gb1 - ph;
+            sq[0][ax][0] = d0 * d0;
+            sq[0][ax][1] = d1 * d1;
+            sq[1][ax][0] = e0 * e0;
+            sq[1][ax][1] = e1 * e1;
+        }
+        // corner (0,0,0) of component c in tile coordinates
+        int tc[3][3];
+        bool fast = true;
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+#pragma unroll
+            for (int ax = 0; ax < 3; ax++) {
+                tc[c][ax] = (ax == c ? ia[ax] : ib[ax]) - ob[ax] + (ax == c ? 0 : 1);
+                fast = fast && (unsigned)tc[c][ax] <= 4u;
+            }
+        bool vbad[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            vbad[c] = !(fabsf(v[c]) < kBig);
+            fast = fast && !vbad[c];
+        }
+        // the common case (a particle of the brick's own cells with a finite
+        // velocity) without checks; pad cells and non-finite velocities
+        // take the checked form
+        if (fast) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                float q[3][2];
+#pragma unroll
+                for (int ax = 0; ax < 3; ax++) {
+                    q[ax][0] = sq[ax == c ? 0 : 1][ax][0];
+                    q[ax][1] = sq[ax == c ? 0 : 1][ax][1];
+                }
+                scatter_comp<true>(&s_S[rep][c][0], &s_W[rep][c][0], &s_bad[c][0], tc[c][0],
+                                   tc[c][1], tc[c][2], q, (v[c] + vbias[c]) * sS[c], sW, inv_r2,
+                                   false, 0u);
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                float q[3][2];
+#pragma unroll
+                for (int ax = 0; ax < 3; ax++) {
+                    q[ax][0] = sq[ax == c ? 0 : 1][ax][0];
+                    q[ax][1] = sq[ax == c ? 0 : 1][ax][1];
+                }
+                const uint32_t bits = v[c] != v[c] ? BAD_NAN : (v[c] > 0.0f ? BAD_PINF : BAD_NINF);
+                const float vs = vbad[c] ? 0.0f : (v[c] + vbias[c]) * sS[c];
+                scatter_comp<false>(&s_S[rep][c][0], &s_W[rep][c][0], &s_bad[c][0], tc[c][0],
+                                    tc[c][1], tc[c][2], q, vs, sW, inv_r2, vbad[c], bits);
+            }
+        }
+    };
+#pragma unroll
+    for (int m = 0; m < PRE; m++)
+        if ((uint32_t)t + (uint32_t)(m * NT) < N) scatter(px[m], py[m], pz[m], pv[m]);
+    for (uint32_t s = (uint32_t)t + (uint32_t)(PRE * NT); s < N; s += NT) {
+        const uint32_t q = slot_of(s);
+        const float v[3] = {gld(a.vel[0], q), gld(a.vel[1], q), gld(a.vel[2], q)};
+        scatter(gld(a.x, q), gld(a.y, q), gld(a.z, q), v);
+    }
+    __syncthreads();
+
+    // sum the copies, back to float, out to the tile buffer
+    float2 *out = tiles + (size_t)gid * 3 * TN;
+    for (int e = t; e < 3 * TN; e += NT) {
+        const int c = e / TN, n = e - c * TN;
+        unsigned long long S = 0, W = 0;
+#pragma unroll
+        for (int r = 0; r < NREP; r++) {
+            S += s_S[r][c][n];
+            W += s_W[r][c][n];
+        }
+        // S = S' - V W
+        const double Wd = (double)W / (double)sW;
+        float Sf = (float)((double)S / (double)sS[c] - (double)vbias[c] * Wd);
+        const float Wf = (float)Wd;
+        const uint32_t bad = s_bad[c][n];
+        if (bad) {
+            if ((bad & BAD_NAN) || ((bad & BAD_PINF) && (bad & BAD_NINF))) Sf = __builtin_nanf("");
+            else Sf = (bad & BAD_PINF) ? INFINITY : -INFINITY;
+        }
+        out[e] = make_float2(Sf, Wf);
+    }
+    if (t == 0) tile_flag[gid] = 1;
+}
+
+// face_epilogue with the component's fields selected per lane.
+__device__ __forceinline__ void face_epilogue_fo(const FaceOut &o, const BinGrid &bg,
+                                                 const uint32_t *__restrict__ count, int c,
+                                                 int i, int j, int k, float S, float W) {
+    const int64_t f = (int64_t)i + (int64_t)o.n0 * ((int64_t)j + (int64_t)o.n1 * k);
+    float val = W > 0.0f ? S / W : S;
+    if (o.old) o.old[f] = val;
+    if (o.wgt) o.wgt[f] = W;
+    if (o.grav_on) {
+        const int ax = c == 0 ? i : (c == 1 ? j : k);
+        bool fluid = false;
+        const uint32_t bhi = (uint32_t)(i + 1) + (uint32_t)bg.b[0] *
+                             ((uint32_t)(j + 1) + (uint32_t)bg.b[1] * (uint32_t)(k + 1));
+        const uint32_t step = c == 0 ? 1u : (c == 1 ? (uint32_t)bg.b[0]
+                                                    : (uint32_t)bg.b[0] * (uint32_t)bg.b[1]);
+        if (ax < o.nax) fluid = fluid || gld(count, bhi) > 0u;
+        if (ax > 0) fluid = fluid || gld(count, bhi - step) > 0u;
+        if (fluid) val += o.grav_delta;
+    }
+    o.grid[f] = val;
+}
+
+__global__ __launch_bounds__(sbrick::NT) void k_p2g_combine(
+    const SceneDesc *__restrict__ scenes, int nsc, int64_t nface, int64_t per_xcd,
+    const uint32_t *__restrict__ count, const float2 *__restrict__ tiles,
+    const uint8_t *__restrict__ tile_flag) {
+    using namespace sbrick;
+    // one thread per face, faces in array order (i fastest); each XCD takes
+    // a contiguous range, so the tiles its faces read stay in its L2
+    const int64_t gb = (int64_t)(blockIdx.x & 7u) * per_xcd + (int64_t)(blockIdx.x >> 3);
+    const int64_t gf = gb * NT + threadIdx.x;
+    if (gf >= nface) return;
+    const SceneDesc &sd = scenes[scene_of(scenes, nsc, (uint64_t)gf, [](const SceneDesc &d) { return (uint64_t)d.f0; })];
+    const StepP2G &a = sd.p2g;
+    int64_t lf = gf - sd.f0;
+    int c = 0;
+    if (lf >= a.nface[0]) {
+        lf -= a.nface[0];
+        c = 1;
+        if (lf >= a.nface[1]) {
+            lf -= a.nface[1];
+            c = 2;
+        }
+    }
+    const int w = a.ni + (c == 0), h = a.nj + (c == 1);
+    const int I = (int)(lf % w), J = (int)((lf / w) % h), K = (int)(lf / ((int64_t)w * h));
+    const int nb[3] = {sd.nbx, sd.nby, sd.nbz};
+    const int n[3] = {I, J, K};
+    // per axis the bricks (lower one first) whose tiles hold node n: the
+    // tile of brick b covers 4b - o .. 4b - o + 4 + o (o = 1 off the own axis)
+    int bl[3][2], tl[3][2], cnt[3];
+#pragma unroll
+    for (int ax = 0; ax < 3; ax++) {
+        const int o = ax == c ? 0 : 1;
+        const int m = n[ax] + o, b1 = m >> 2, t1 = m & 3;
+        const bool lower = (ax == c ? t1 == 0 : t1 <= 1) && b1 >= 1 && b1 - 1 < nb[ax];
+        const bool upper = b1 < nb[ax];
+        bl[ax][0] = lower ? b1 - 1 : b1;
+        tl[ax][0] = lower ? t1 + 4 : t1;
+        bl[ax][1] = b1;
+        tl[ax][1] = t1;
+        cnt[ax] = lower ? (upper ? 2 : 1) : (upper ? 1 : 0);
+    }
+    // the flags first (all together), then the entries of the bricks that
+    // wrote a tile (most faces of empty regions read none)
+    const float *tf = reinterpret_cast<const float *>(tiles);
+    uint32_t fl[8];
+    size_t qe[8];
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        const int ix = e & 1, iy = (e >> 1) & 1, iz = e >> 2;
+        fl[e] = 0;
+        qe[e] = 0;
+        if (iz < cnt[2] && iy < cnt[1] && ix < cnt[0]) {
+            const uint32_t bid = sd.k0 + (uint32_t)(bl[0][ix] + sd.nbx * (bl[1][iy] + sd.nby * bl[2][iz]));
+            qe[e] = ((size_t)bid * 3 + (size_t)c) * TN +
+                    (size_t)(tl[0][ix] + T * (tl[1][iy] + T * tl[2][iz]));
+            fl[e] = gld(tile_flag, bid);
+        }
+    }
+    float ps[8], pw[8];
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        ps[e] = pw[e] = 0.0f;
+        if (fl[e]) {
+            ps[e] = gld(tf, 2 * qe[e]);
+            pw[e] = gld(tf, 2 * qe[e] + 1);
+        }
+    }
+    float S = 0.0f, W = 0.0f;
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        S += fl[e] ? ps[e] : 0.0f;
+        W += fl[e] ? pw[e] : 0.0f;
+    }
+    face_epilogue_fo(face_out(a, c), sd.bg, count + sd.b0, c, I, J, K, S, W);
+}
+
 struct StepG2P {
     const float *xin[3], *vin[3];   // in sort order (all scenes)
     float *xout[3], *vout[3];       // the scene's outputs
@@ -2243,6 +2679,7 @@ void layout_scenes(SceneDesc *sc, int nsc, SceneTotals &tot) {
         d.gby = (d.nj + gbrick::GB - 1) / gbrick::GB;
         d.gbz = (d.nk + gbrick::GB - 1) / gbrick::GB;
         d.g0 = (uint32_t)tot.ng2p;
+        d.f0 = tot.nface;
         MacGrid g;
         g.ni = d.ni;
         g.nj = d.nj;
@@ -2281,7 +2718,20 @@ void flip_step(const StepBatch &s, const SortBuffers &sb, hipStream_t st, hipEve
     }
     if (ev) GFSX_HIP(hipEventRecord(ev[1], st));
 
-    if (s.p2g_variant != 1) {
+    if (s.p2g_variant == 4) {
+        const uint32_t nbrick = (uint32_t)s.tot.np2g;
+        const int per_xcd = (int)((nbrick + 7) / 8);
+        const dim3 grid((unsigned)(8 * (int64_t)per_xcd)), block(sbrick::NT);
+        hipLaunchKernelGGL(k_p2g_scatter, grid, block, 0, st, s.scenes_dev, nsc, nbrick, per_xcd,
+                           (const uint32_t *)sb.start, (const uint8_t *)sb.brick_flag, s.tiles,
+                           s.tile_flag);
+        GFSX_LAUNCH_CHECK();
+        const int64_t nfb = (s.tot.nface + sbrick::NT - 1) / sbrick::NT;
+        const int64_t fper = (nfb + 7) / 8;
+        hipLaunchKernelGGL(k_p2g_combine, dim3((unsigned)(8 * fper)), block, 0, st, s.scenes_dev,
+                           nsc, s.tot.nface, fper, (const uint32_t *)sb.count,
+                           (const float2 *)s.tiles, (const uint8_t *)s.tile_flag);
+    } else if (s.p2g_variant != 1) {
         const uint32_t nbrick = (uint32_t)s.tot.np2g;
         const int per_xcd = (int)((nbrick + 7) / 8);
         constexpr int cap = brick::CAP;

@@ -196,12 +196,16 @@ void gfsx_flip_step_batched(gfsx_ctx *ctx, const gfsx_scene *scenes, int nscenes
                             const gfsx_step_params *params, int *err);
 
 /* Tunables of the fused step, by name (an unknown name or value fails).
- * Every variant gives bit-identical results; they differ in speed only.
+ * Every variant is deterministic; they differ in speed, and the P2G
+ * variants in the rounding of the sums (within FP32 rounding of each other).
  *   "p2g_variant": 0 = brick-staged LDS gather, candidates streamed in
- *                      chunks (default),
+ *                      chunks,
  *                  1 = direct gather from global memory,
  *                  2 = brick-staged, candidates range by range,
- *                  3 = brick-staged, candidates as one flattened stream.
+ *                  3 = brick-staged, candidates as one flattened stream,
+ *                  4 = scatter of each brick's own particles into a tile of
+ *                      fixed-point sums in LDS, then a pass adding the
+ *                      tiles that hold each face (default).
  *   "g2p_variant": 0 = grid boxes staged in LDS per brick (default),
  *                  1 = direct sampling from global memory.
  *   "adjoint_variant" (gfsx_flip_step_backward; results equal up to FP32

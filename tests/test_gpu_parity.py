@@ -364,9 +364,10 @@ def test_bad_arguments_raise(ctx):
         ctx.p2g(np.zeros((1, 3), np.float32), None, -1.0, (0, 0, 0), 0.1, (2, 2, 2))
 
 
-@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("variant", [0, 1, 4])
 def test_step_p2g_variants(ctx, orc, variant):
-    """Brick-staged (0) and direct (1) P2G gathers both match the oracle."""
+    """Brick-staged (0) and direct (1) P2G gathers and the tile scatter (4)
+    all match the oracle."""
     from gridfluidsim3d_amd import scenes
     sc = scenes.random_blobs(21, (27, 18, 13), 0.1, amplitude=2.0)
     exp = orc.flip_step(sc.pos, sc.vel, sc.dims, 0.1, sc.dt, (0.0, -9.8, 0.0), 0.05, TRILINEAR)
@@ -378,6 +379,76 @@ def test_step_p2g_variants(ctx, orc, variant):
     finally:
         ctx.set_option("p2g_variant", old)
     _check_step(res, exp, False)
+
+
+def _scatter_edge_scene(kind):
+    rng = np.random.default_rng(41)
+    if kind == "clump":
+        # a clump of 600 particles in one cell inside ordinary fluid, some
+        # particles nearly at rest beside fast ones
+        dims, dx = (12, 10, 9), 0.1
+        pos = (rng.uniform(0.2, 0.9, size=(4000, 3)) * np.array(dims) * dx).astype(np.float32)
+        clump = np.array([0.55, 0.45, 0.35]) + rng.uniform(0.0, 0.1, size=(600, 3))
+        pos = np.concatenate([pos, clump]).astype(np.float32)
+        vel = rng.normal(size=pos.shape).astype(np.float32)
+        vel[:100] *= 1e-3
+        return pos, vel, dims, dx
+    # particles in the pad layers and beyond them, non-finite velocities
+    dims, dx = (9, 11, 7), 0.1
+    W = np.array(dims) * dx
+    pos = rng.uniform(-0.15, W + 0.15, size=(6000, 3)).astype(np.float32)
+    vel = (3.0 * rng.normal(size=pos.shape)).astype(np.float32)
+    vel[10, 0] = np.inf
+    vel[20, 1] = -np.inf
+    vel[30, 2] = np.nan
+    vel[40] = (np.inf, -np.inf, np.nan)
+    return pos, vel, dims, dx
+
+
+@pytest.mark.parametrize("variant", [0, 4])
+def test_step_clumped_cells_against_oracle(ctx, orc, variant):
+    """Cells of 65..256 particles (sorted in registers by one wave, four
+    keys per lane) among ordinary ones: both P2G forms match the oracle.
+    The gather needs each particle's rank inside its half-cell from that
+    sort, including runs that cross from lane 0 into lane 1."""
+    pos, vel, dims, dx = _scatter_edge_scene("clump")
+    exp = orc.flip_step(pos, vel, dims, dx, 0.01, (0.0, -9.8, 0.0), 0.05, TRILINEAR)
+    old = ctx.get_option("p2g_variant")
+    ctx.set_option("p2g_variant", variant)
+    try:
+        res = _run_step(ctx, pos, vel, dims, dx, 0.01, (0.0, -9.8, 0.0), 0.05, TRILINEAR, 0)
+    finally:
+        ctx.set_option("p2g_variant", old)
+    _check_step(res, exp, False)
+
+
+@pytest.mark.parametrize("kind", ["clump", "wild"])
+def test_step_p2g_scatter_matches_gather(ctx, kind):
+    """The tile scatter (4) against the brick gather (0) on a clumped cell,
+    the pad layers and non-finite velocities: the same faces are NaN and
+    +-inf (what the float sums give: K v with v infinite is +-inf, or NaN
+    where K is 0), and the finite values agree within the tolerance."""
+    pos, vel, dims, dx = _scatter_edge_scene(kind)
+    old = ctx.get_option("p2g_variant")
+    out = []
+    try:
+        for variant in (0, 4):
+            ctx.set_option("p2g_variant", variant)
+            out.append(_run_step(ctx, pos, vel, dims, dx, 0.01, (0.0, -9.8, 0.0), 0.05,
+                                 TRILINEAR, 1))
+    finally:
+        ctx.set_option("p2g_variant", old)
+    a, b = out
+    nonfinite = 0
+    for key in ("u", "v", "w", "u_old", "v_old", "w_old", "wu", "wv", "ww", "vel", "pos"):
+        fa, fb = np.isfinite(a[key]), np.isfinite(b[key])
+        assert np.array_equal(fa, fb), key
+        assert np.array_equal(np.isnan(a[key]), np.isnan(b[key])), key
+        assert np.array_equal(a[key][np.isinf(a[key])], b[key][np.isinf(b[key])]), key
+        assert norm_rel(b[key][fb], a[key][fa]) <= NORM_REL_TOL, key
+        nonfinite += int((~fa).sum())
+    if kind == "wild":
+        assert nonfinite > 0
 
 
 def _g2p_variant_scene(kind):
@@ -449,7 +520,7 @@ def test_step_is_deterministic(ctx):
 
 
 def test_options(ctx):
-    assert ctx.get_option("p2g_variant") == 0
+    assert ctx.get_option("p2g_variant") == 4
     assert ctx.get_option("g2p_variant") == 0
     with pytest.raises(RuntimeError, match="g2p_variant"):
         ctx.set_option("g2p_variant", 2)

@@ -338,9 +338,10 @@ int64_t sort_blocks(int64_t n);
 // (octant, index) instead, the octant being the 3-bit index of the half of
 // the cell the particle lies in along x, y, z (so each half-cell is a
 // contiguous run), hrank[q] receives the rank of perm[q] inside its
-// half-cell run and soct[q] its octant, and brick_flag (when set) the P2G
-// bricks with a particle within reach.
-void cell_sort(const SortBuffers &sb, int64_t n, hipStream_t s, bool octant_order);
+// half-cell run and soct[q] its octant.  With mark_bricks, brick_flag
+// receives the P2G bricks with a particle within reach.
+void cell_sort(const SortBuffers &sb, int64_t n, hipStream_t s, bool octant_order,
+               bool mark_bricks = false);
 
 // Generic P2G gather for one component (ScalarField semantics).
 void p2g_gather(const float *aos, const float *val, int64_t n,

@@ -671,7 +671,8 @@ __global__ __launch_bounds__(64) void k_order_big(
     }
 }
 
-void cell_sort(const SortBuffers &sb, int64_t n, hipStream_t s, bool octant_order) {
+void cell_sort(const SortBuffers &sb, int64_t n, hipStream_t s, bool octant_order,
+               bool mark_bricks) {
     fill_u32(sb.count, 0u, (int64_t)sb.nbin_total, s);
     if (n <= 0) {
         fill_u32(sb.start, 0u, (int64_t)sb.nbin_total + 1, s);
@@ -698,7 +699,7 @@ void cell_sort(const SortBuffers &sb, int64_t n, hipStream_t s, bool octant_orde
     out.perm = sb.perm;
     out.hrank = octant_order ? sb.hrank : nullptr;
     out.soct = octant_order ? sb.soct : nullptr;
-    uint8_t *flag = octant_order ? sb.brick_flag : nullptr;
+    uint8_t *flag = mark_bricks ? sb.brick_flag : nullptr;
     auto *order = n < ((int64_t)1 << kNarrowBits) ? k_order_bins<true> : k_order_bins<false>;
     hipLaunchKernelGGL(order, dim3(grid_for(sb.nbin_total, kBlock)), dim3(kBlock), 0, s,
                        sb.scenes, sb.nscene, (const uint32_t *)sb.start, sb.nbin_total,
@@ -1877,10 +1878,14 @@ __device__ __forceinline__ void scatter_comp(unsigned long long *tS, unsigned lo
                 }
                 const int idx = (tx + dx) + T * ((ty + dy) + T * (tz + dz));
                 const float K = corner_k(sq[0][dx], sq[1][dy], sq[2][dz], inv_r2);
-                atomicAdd(&tW[idx], fix64(K * sW));
                 if (FAST || !vbad) {
-                    atomicAdd(&tS[idx], fix64(K * vs));
+                    // about half the box corners lie beyond the radius
+                    if (K > 0.0f) {
+                        atomicAdd(&tW[idx], fix64(K * sW));
+                        atomicAdd(&tS[idx], fix64(K * vs));
+                    }
                 } else {
+                    atomicAdd(&tW[idx], fix64(K * sW));
                     // the float sum would add K * v: NaN when K is 0
                     atomicOr(&tbad[idx], K == 0.0f ? BAD_NAN : badbits);
                 }
@@ -2174,8 +2179,10 @@ __global__ __launch_bounds__(sbrick::NT) void k_p2g_combine(
             c = 2;
         }
     }
-    const int w = a.ni + (c == 0), h = a.nj + (c == 1);
-    const int I = (int)(lf % w), J = (int)((lf / w) % h), K = (int)(lf / ((int64_t)w * h));
+    // (a scene's face count of one component fits in 32 bits)
+    const uint32_t w = (uint32_t)(a.ni + (c == 0)), h = (uint32_t)(a.nj + (c == 1));
+    const uint32_t lf32 = (uint32_t)lf, row = lf32 / w;
+    const int I = (int)(lf32 - row * w), J = (int)(row % h), K = (int)(row / h);
     const int nb[3] = {sd.nbx, sd.nby, sd.nbz};
     const int n[3] = {I, J, K};
     // per axis the bricks (lower one first) whose tiles hold node n: the
@@ -2703,8 +2710,11 @@ void layout_scenes(SceneDesc *sc, int nsc, SceneTotals &tot) {
 void flip_step(const StepBatch &s, const SortBuffers &sb, hipStream_t st, hipEvent_t *ev) {
     if (ev) GFSX_HIP(hipEventRecord(ev[0], st));
     const int nsc = (int)s.scenes.size();
-    // cells ordered inside by (octant, index): each half-cell is a run
-    cell_sort(sb, s.tot.n, st, true);
+    // cells ordered inside by index; for the P2G gathers by (octant, index),
+    // so that each half-cell is a run.  The brick flags are set by the sort
+    // for this step's particles only.
+    fill_u8(sb.brick_flag, 0, s.tot.np2g, st);
+    cell_sort(sb, s.tot.n, st, s.p2g_variant != 4, true);
     Sorted so;
     for (int c = 0; c < 3; c++) {
         so.x[c] = s.xs[c];

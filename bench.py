@@ -39,15 +39,15 @@ def face_count(dims):
 def algorithmic_bytes(P, F, C):
     """Bytes each stage must move at minimum (FP32 / uint32):
     sort: read positions and velocities (24P), write them in cell order
-          (24P) with the permutation and the in-half-cell ranks (8P), one
-          pass over the cell counts and starts (8C);
+          (24P) with the permutation (4P), one pass over the cell counts and
+          starts (8C);
     p2g : read x, v once (24P) + write the field and its pre-force
           snapshot, one value per face each (8F);
     g2p : read x, v (24P) + write x', v' (24P) + read the field and the
           snapshot once (8F);
     step: the whole step: read x, v and write x', v' (48P), write the field
           and read it back (8F)."""
-    return dict(sort=56 * P + 8 * C, p2g=24 * P + 8 * F, g2p=48 * P + 8 * F,
+    return dict(sort=52 * P + 8 * C, p2g=24 * P + 8 * F, g2p=48 * P + 8 * F,
                 step=48 * P + 8 * F)
 
 
@@ -80,7 +80,7 @@ def flops_per_particle(stencil):
 # stage on the library's stream).
 STAGE_KERNELS = {
     "sort": "cell sort: k_bin, scans, k_scatter, k_order_bins/mid/big, k_sorted_copy",
-    "p2g": "k_flip_p2g_brick + k_flip_p2g_dense",
+    "p2g": "k_p2g_scatter + k_p2g_combine",
     "g2p": "k_flip_g2p_brick_tl (trilinear) / k_flip_g2p_brick_tc (tricubic)",
 }
 

@@ -1843,13 +1843,15 @@ __device__ __forceinline__ float corner_k(float sx, float sy, float sz, float in
     return wyvill(((sx + sy) + sz) * inv_r2);
 }
 
-// x >= 0 (a float below 2^63) rounded to an integer, as 64 bits: the high
+// x >= 0 (a float below 2^63) truncated to an integer, as 64 bits: the high
 // word is floor(x / 2^32); the low word x - high * 2^32 is exact in float
-// (below 2^32, and for x >= 2^32 it keeps at most x's 24 significant bits).
+// (below 2^32, and for x >= 2^32 it keeps at most x's 24 significant bits)
+// and its fraction is dropped by the conversion.  With the scales below
+// (2^62 / bound units for K = 1) a term loses less than one unit, far below
+// its own float rounding unless K is under about 2^-38 * bound.
 __device__ __forceinline__ unsigned long long fix64(float x) {
-    const float xr = rintf(x);
-    const float hi = floorf(xr * 0x1p-32f);
-    const float lo = fmaf(hi, -0x1p32f, xr);
+    const float hi = floorf(x * 0x1p-32f);
+    const float lo = fmaf(hi, -0x1p32f, x);
     return ((unsigned long long)(uint32_t)hi << 32) | (unsigned long long)(uint32_t)lo;
 }
 
@@ -2159,8 +2161,8 @@ __device__ __forceinline__ void face_epilogue_fo(const FaceOut &o, const BinGrid
 
 __global__ __launch_bounds__(sbrick::NT) void k_p2g_combine(
     const SceneDesc *__restrict__ scenes, int nsc, int64_t nface, int64_t per_xcd,
-    const uint32_t *__restrict__ count, const float2 *__restrict__ tiles,
-    const uint8_t *__restrict__ tile_flag) {
+    const uint32_t *__restrict__ count, const uint8_t *__restrict__ brick_flag,
+    const float2 *__restrict__ tiles, const uint8_t *__restrict__ tile_flag) {
     using namespace sbrick;
     // one thread per face, faces in array order (i fastest); each XCD takes
     // a contiguous range, so the tiles its faces read stay in its L2
@@ -2200,37 +2202,38 @@ __global__ __launch_bounds__(sbrick::NT) void k_p2g_combine(
         tl[ax][1] = t1;
         cnt[ax] = lower ? (upper ? 2 : 1) : (upper ? 1 : 0);
     }
-    // the flags first (all together), then the entries of the bricks that
-    // wrote a tile (most faces of empty regions read none)
-    const float *tf = reinterpret_cast<const float *>(tiles);
-    uint32_t fl[8];
-    size_t qe[8];
-#pragma unroll
-    for (int e = 0; e < 8; e++) {
-        const int ix = e & 1, iy = (e >> 1) & 1, iz = e >> 2;
-        fl[e] = 0;
-        qe[e] = 0;
-        if (iz < cnt[2] && iy < cnt[1] && ix < cnt[0]) {
-            const uint32_t bid = sd.k0 + (uint32_t)(bl[0][ix] + sd.nbx * (bl[1][iy] + sd.nby * bl[2][iz]));
-            qe[e] = ((size_t)bid * 3 + (size_t)c) * TN +
-                    (size_t)(tl[0][ix] + T * (tl[1][iy] + T * tl[2][iz]));
-            fl[e] = gld(tile_flag, bid);
-        }
-    }
-    float ps[8], pw[8];
-#pragma unroll
-    for (int e = 0; e < 8; e++) {
-        ps[e] = pw[e] = 0.0f;
-        if (fl[e]) {
-            ps[e] = gld(tf, 2 * qe[e]);
-            pw[e] = gld(tf, 2 * qe[e] + 1);
-        }
-    }
+    // The brick that owns the face (the gathers' assignment) has no particle
+    // within reach of any of its faces unless the sort flagged it: then every
+    // tile entry of the face is 0.  Otherwise the tile flags and entries are
+    // loaded together; entries of a brick without a tile hold stale data and
+    // are selected away.
+    const uint32_t own = sd.k0 + (uint32_t)(min(I >> 2, sd.nbx - 1) +
+                                            sd.nbx * (min(J >> 2, sd.nby - 1) +
+                                                      sd.nby * min(K >> 2, sd.nbz - 1)));
     float S = 0.0f, W = 0.0f;
+    if (gld(brick_flag, own) != 0) {
+        const float *tf = reinterpret_cast<const float *>(tiles);
+        uint32_t fl[8];
+        float ps[8], pw[8];
 #pragma unroll
-    for (int e = 0; e < 8; e++) {
-        S += fl[e] ? ps[e] : 0.0f;
-        W += fl[e] ? pw[e] : 0.0f;
+        for (int e = 0; e < 8; e++) {
+            const int ix = e & 1, iy = (e >> 1) & 1, iz = e >> 2;
+            fl[e] = 0;
+            ps[e] = pw[e] = 0.0f;
+            if (iz < cnt[2] && iy < cnt[1] && ix < cnt[0]) {
+                const uint32_t bid = sd.k0 + (uint32_t)(bl[0][ix] + sd.nbx * (bl[1][iy] + sd.nby * bl[2][iz]));
+                const size_t q = ((size_t)bid * 3 + (size_t)c) * TN +
+                                 (size_t)(tl[0][ix] + T * (tl[1][iy] + T * tl[2][iz]));
+                fl[e] = gld(tile_flag, bid);
+                ps[e] = gld(tf, 2 * q);
+                pw[e] = gld(tf, 2 * q + 1);
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            S += fl[e] ? ps[e] : 0.0f;
+            W += fl[e] ? pw[e] : 0.0f;
+        }
     }
     face_epilogue_fo(face_out(a, c), sd.bg, count + sd.b0, c, I, J, K, S, W);
 }
@@ -2740,7 +2743,8 @@ void flip_step(const StepBatch &s, const SortBuffers &sb, hipStream_t st, hipEve
         const int64_t fper = (nfb + 7) / 8;
         hipLaunchKernelGGL(k_p2g_combine, dim3((unsigned)(8 * fper)), block, 0, st, s.scenes_dev,
                            nsc, s.tot.nface, fper, (const uint32_t *)sb.count,
-                           (const float2 *)s.tiles, (const uint8_t *)s.tile_flag);
+                           (const uint8_t *)sb.brick_flag, (const float2 *)s.tiles,
+                           (const uint8_t *)s.tile_flag);
     } else if (s.p2g_variant != 1) {
         const uint32_t nbrick = (uint32_t)s.tot.np2g;
         const int per_xcd = (int)((nbrick + 7) / 8);

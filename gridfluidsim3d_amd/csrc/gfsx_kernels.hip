@@ -1863,7 +1863,7 @@ __device__ __forceinline__ unsigned long long fix64(float x) {
 // the product K v is recorded in the node's flag bits instead of the sum.
 template <bool FAST>
 __device__ __forceinline__ void scatter_comp(unsigned long long *tS, unsigned long long *tW,
-                                             uint32_t *tbad, int tx, int ty, int tz,
+                                             uint32_t *sbad, int cbase, int tx, int ty, int tz,
                                              const float sq[3][2], float vs, float sW,
                                              float inv_r2, bool vbad, uint32_t badbits) {
     using namespace sbrick;
@@ -1889,7 +1889,8 @@ __device__ __forceinline__ void scatter_comp(unsigned long long *tS, unsigned lo
                 } else {
                     atomicAdd(&tW[idx], fix64(K * sW));
                     // the float sum would add K * v: NaN when K is 0
-                    atomicOr(&tbad[idx], K == 0.0f ? BAD_NAN : badbits);
+                    const int nb = cbase + idx;   // flag bits: 4 per node, 8 nodes per word
+                    atomicOr(&sbad[nb >> 3], (K == 0.0f ? BAD_NAN : badbits) << ((nb & 7) * 4));
                 }
             }
 }
@@ -1900,7 +1901,7 @@ __global__ __launch_bounds__(sbrick::NT) void k_p2g_scatter(
     float2 *__restrict__ tiles, uint8_t *__restrict__ tile_flag) {
     using namespace sbrick;
     __shared__ unsigned long long s_S[NREP][3][TN], s_W[NREP][3][TN];
-    __shared__ uint32_t s_bad[3][TN];
+    __shared__ uint32_t s_bad[(3 * TN + 7) / 8];
     __shared__ uint32_t s_rowq[MAXROW], s_rows0[MAXROW + 1];
     __shared__ float s_red[NT / 64][3];
     __shared__ uint32_t s_cmax;
@@ -1952,7 +1953,7 @@ __global__ __launch_bounds__(sbrick::NT) void k_p2g_scatter(
         for (int e = t; e < NREP * 3 * TN; e += NT) z[e] = 0ull;
         z = &s_W[0][0][0];
         for (int e = t; e < NREP * 3 * TN; e += NT) z[e] = 0ull;
-        for (int e = t; e < 3 * TN; e += NT) (&s_bad[0][0])[e] = 0u;
+        for (int e = t; e < (3 * TN + 7) / 8; e += NT) s_bad[e] = 0u;
     }
     __syncthreads();
     const uint32_t N = s_rows0[nrow];
@@ -2083,7 +2084,7 @@ __global__ __launch_bounds__(sbrick::NT) void k_p2g_scatter(
                     q[ax][0] = sq[ax == c ? 0 : 1][ax][0];
                     q[ax][1] = sq[ax == c ? 0 : 1][ax][1];
                 }
-                scatter_comp<true>(&s_S[rep][c][0], &s_W[rep][c][0], &s_bad[c][0], tc[c][0],
+                scatter_comp<true>(&s_S[rep][c][0], &s_W[rep][c][0], s_bad, c * TN, tc[c][0],
                                    tc[c][1], tc[c][2], q, (v[c] + vbias[c]) * sS[c], sW, inv_r2,
                                    false, 0u);
             }
@@ -2098,7 +2099,7 @@ __global__ __launch_bounds__(sbrick::NT) void k_p2g_scatter(
                 }
                 const uint32_t bits = v[c] != v[c] ? BAD_NAN : (v[c] > 0.0f ? BAD_PINF : BAD_NINF);
                 const float vs = vbad[c] ? 0.0f : (v[c] + vbias[c]) * sS[c];
-                scatter_comp<false>(&s_S[rep][c][0], &s_W[rep][c][0], &s_bad[c][0], tc[c][0],
+                scatter_comp<false>(&s_S[rep][c][0], &s_W[rep][c][0], s_bad, c * TN, tc[c][0],
                                     tc[c][1], tc[c][2], q, vs, sW, inv_r2, vbad[c], bits);
             }
         }
@@ -2127,7 +2128,7 @@ __global__ __launch_bounds__(sbrick::NT) void k_p2g_scatter(
         const double Wd = (double)W / (double)sW;
         float Sf = (float)((double)S / (double)sS[c] - (double)vbias[c] * Wd);
         const float Wf = (float)Wd;
-        const uint32_t bad = s_bad[c][n];
+        const uint32_t bad = (s_bad[e >> 3] >> ((e & 7) * 4)) & 0xFu;
         if (bad) {
             if ((bad & BAD_NAN) || ((bad & BAD_PINF) && (bad & BAD_NINF))) Sf = __builtin_nanf("");
             else Sf = (bad & BAD_PINF) ? INFINITY : -INFINITY;

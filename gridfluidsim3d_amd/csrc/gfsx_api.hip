@@ -383,7 +383,7 @@ std::vector<gfsx::SceneDesc> flip_step_impl(gfsx_ctx *ctx, const gfsx_scene *sce
     s.dense = ctx->u32(S_DENSE, s.tot.np2g);
     s.dense_count = ctx->u32(S_DENSEN, 2);
     if (ctx->p2g_variant == 4) {
-        s.tiles = (float2 *)ctx->f32(S_P_TILE, 2 * 3 * 216 * s.tot.np2g);
+        s.tiles = (float2 *)ctx->f32(S_P_TILE, 2 * (int64_t)gfsx::kP2gTileNodes * s.tot.np2g);
         s.tile_flag = (uint8_t *)ctx->u32(S_P_TFLAG, (s.tot.np2g + 3) / 4);
     }
     s.num_cu = ctx->num_cu;

@@ -371,6 +371,10 @@ struct SceneTotals {
 // brick counts, grid limits) from n, ni/nj/nk and dx, in table order.
 void layout_scenes(SceneDesc *scenes, int nscene, SceneTotals &tot);
 
+// Nodes of one P2G brick's scatter tile, all three components: per
+// component 5 nodes along its own axis and 6 along the two others.
+constexpr int kP2gTileNodes = 3 * 5 * 6 * 6;
+
 // A fused step over one or more scenes.
 struct StepBatch {
     std::vector<SceneDesc> scenes;   // host copy, laid out
@@ -379,7 +383,7 @@ struct StepBatch {
     float *xs[3], *vs[3];            // workspace: all particles in sort order
     uint32_t *dense, *dense_count;   // workspace: P2G bricks over capacity (list; count
                                      // and next unprocessed entry)
-    float2 *tiles = nullptr;         // workspace of the scatter P2G: 3 * 216 (S, W) per brick
+    float2 *tiles = nullptr;         // workspace of the scatter P2G: kP2gTileNodes (S, W) per brick
     uint8_t *tile_flag = nullptr;    // per brick: its tile was written
     int num_cu = 1;                  // compute units of the device
     int stencil;

@@ -1804,10 +1804,11 @@ __global__ __launch_bounds__(brick::NT, 3) void k_flip_p2g_dense(
 // pad cells beyond the domain).  Every particle adds K and K v to the 2x2x2
 // nodes of each component around it -- the candidates of the gather: the
 // cell and half-cell come from the same FP64 coordinates, and K is the same
-// FP32 expression -- in a tile of 6^3 nodes per component in LDS.  Node n
-// of axis a is tile entry n - 4b + (a != c): the own axis covers nodes
-// 4b..4b+4, the two others 4b-1..4b+4, which holds every node a particle of
-// the brick reaches.  The sums are integers: K and K v are rounded to fixed
+// FP32 expression -- in a tile of 5x6x6 nodes per component in LDS.  Node
+// n of axis a is tile entry n - 4b + (a != c): the own axis covers nodes
+// 4b..4b+4, the two others 4b-1..4b+4, which holds every in-domain node a
+// particle of the brick reaches (a pad-cell particle's nodes beyond the
+// tile lie outside the domain).  The sums are integers: K and K v are rounded to fixed
 // point, 64-bit, so they do not depend on the order of the additions: K and
 // K (v + V), V = the brick's largest |v| (so that every term is >= 0 and
 // S = S' - V W), scaled by powers of two chosen per brick so that no sum can
@@ -1829,8 +1830,12 @@ __global__ __launch_bounds__(brick::NT, 3) void k_flip_p2g_dense(
 // the entries of the at most two bricks per axis whose tiles hold the face,
 // then applies the weight and the body force.
 namespace sbrick {
-constexpr int T = 6;               // tile edge, nodes, for every component
-constexpr int TN = T * T * T;      // nodes per component tile
+constexpr int T = 6;               // tile edge, nodes, off the component's own axis
+constexpr int TN = 5 * T * T;      // nodes per component tile (5 along the own axis)
+static_assert(3 * TN == kP2gTileNodes, "tile buffer size");
+// tile index of node (tx, ty, tz) of component c: the own axis is 5 wide
+__host__ __device__ constexpr int tile_sx(int c) { return c == 0 ? 5 : 6; }
+__host__ __device__ constexpr int tile_sy(int c) { return c == 1 ? 5 : 6; }
 constexpr int NREP = 2;            // tile copies
 constexpr int NT = 256;
 constexpr int MAXROW = 36;         // (y,z) bin rows of a brick, pads included
@@ -1861,7 +1866,7 @@ __device__ __forceinline__ unsigned long long fix64(float x) {
 // the S sums.  FAST: every corner lies in the tile and v is finite (no
 // checks).  Otherwise corners outside the tile are skipped and, with vbad,
 // the product K v is recorded in the node's flag bits instead of the sum.
-template <bool FAST>
+template <int C, bool FAST>
 __device__ __forceinline__ void scatter_comp(unsigned long long *tS, unsigned long long *tW,
                                              uint32_t *sbad, int cbase, int tx, int ty, int tz,
                                              const float sq[3][2], float vs, float sW,
@@ -1873,12 +1878,13 @@ __device__ __forceinline__ void scatter_comp(unsigned long long *tS, unsigned lo
         for (int dy = 0; dy < 2; dy++)
 #pragma unroll
             for (int dx = 0; dx < 2; dx++) {
+                constexpr int SX = tile_sx(C), SY = tile_sy(C), SZ = C == 2 ? 5 : 6;
                 if (!FAST) {
-                    if ((unsigned)(tx + dx) >= (unsigned)T || (unsigned)(ty + dy) >= (unsigned)T ||
-                        (unsigned)(tz + dz) >= (unsigned)T)
+                    if ((unsigned)(tx + dx) >= (unsigned)SX || (unsigned)(ty + dy) >= (unsigned)SY ||
+                        (unsigned)(tz + dz) >= (unsigned)SZ)
                         continue;
                 }
-                const int idx = (tx + dx) + T * ((ty + dy) + T * (tz + dz));
+                const int idx = (tx + dx) + SX * ((ty + dy) + SY * (tz + dz));
                 const float K = corner_k(sq[0][dx], sq[1][dy], sq[2][dz], inv_r2);
                 if (FAST || !vbad) {
                     // about half the box corners lie beyond the radius
@@ -2064,7 +2070,7 @@ __global__ __launch_bounds__(sbrick::NT) void k_p2g_scatter(
 #pragma unroll
             for (int ax = 0; ax < 3; ax++) {
                 tc[c][ax] = (ax == c ? ia[ax] : ib[ax]) - ob[ax] + (ax == c ? 0 : 1);
-                fast = fast && (unsigned)tc[c][ax] <= 4u;
+                fast = fast && (unsigned)tc[c][ax] <= (ax == c ? 3u : 4u);
             }
         bool vbad[3];
 #pragma unroll
@@ -2075,33 +2081,31 @@ __global__ __launch_bounds__(sbrick::NT) void k_p2g_scatter(
         // the common case (a particle of the brick's own cells with a finite
         // velocity) without checks; pad cells and non-finite velocities
         // take the checked form
+        auto comp = [&](auto cc, auto fastc) {
+            constexpr int c = decltype(cc)::value;
+            constexpr bool F = decltype(fastc)::value;
+            float q[3][2];
+#pragma unroll
+            for (int ax = 0; ax < 3; ax++) {
+                q[ax][0] = sq[ax == c ? 0 : 1][ax][0];
+                q[ax][1] = sq[ax == c ? 0 : 1][ax][1];
+            }
+            const uint32_t bits = v[c] != v[c] ? BAD_NAN : (v[c] > 0.0f ? BAD_PINF : BAD_NINF);
+            const float vs = vbad[c] ? 0.0f : (v[c] + vbias[c]) * sS[c];
+            scatter_comp<c, F>(&s_S[rep][c][0], &s_W[rep][c][0], s_bad, c * TN, tc[c][0],
+                               tc[c][1], tc[c][2], q, vs, sW, inv_r2, vbad[c], bits);
+        };
+        using I0 = std::integral_constant<int, 0>;
+        using I1 = std::integral_constant<int, 1>;
+        using I2 = std::integral_constant<int, 2>;
         if (fast) {
-#pragma unroll
-            for (int c = 0; c < 3; c++) {
-                float q[3][2];
-#pragma unroll
-                for (int ax = 0; ax < 3; ax++) {
-                    q[ax][0] = sq[ax == c ? 0 : 1][ax][0];
-                    q[ax][1] = sq[ax == c ? 0 : 1][ax][1];
-                }
-                scatter_comp<true>(&s_S[rep][c][0], &s_W[rep][c][0], s_bad, c * TN, tc[c][0],
-                                   tc[c][1], tc[c][2], q, (v[c] + vbias[c]) * sS[c], sW, inv_r2,
-                                   false, 0u);
-            }
+            comp(I0(), std::true_type());
+            comp(I1(), std::true_type());
+            comp(I2(), std::true_type());
         } else {
-#pragma unroll
-            for (int c = 0; c < 3; c++) {
-                float q[3][2];
-#pragma unroll
-                for (int ax = 0; ax < 3; ax++) {
-                    q[ax][0] = sq[ax == c ? 0 : 1][ax][0];
-                    q[ax][1] = sq[ax == c ? 0 : 1][ax][1];
-                }
-                const uint32_t bits = v[c] != v[c] ? BAD_NAN : (v[c] > 0.0f ? BAD_PINF : BAD_NINF);
-                const float vs = vbad[c] ? 0.0f : (v[c] + vbias[c]) * sS[c];
-                scatter_comp<false>(&s_S[rep][c][0], &s_W[rep][c][0], s_bad, c * TN, tc[c][0],
-                                    tc[c][1], tc[c][2], q, vs, sW, inv_r2, vbad[c], bits);
-            }
+            comp(I0(), std::false_type());
+            comp(I1(), std::false_type());
+            comp(I2(), std::false_type());
         }
     };
 #pragma unroll
@@ -2224,7 +2228,7 @@ __global__ __launch_bounds__(sbrick::NT) void k_p2g_combine(
             if (iz < cnt[2] && iy < cnt[1] && ix < cnt[0]) {
                 const uint32_t bid = sd.k0 + (uint32_t)(bl[0][ix] + sd.nbx * (bl[1][iy] + sd.nby * bl[2][iz]));
                 const size_t q = ((size_t)bid * 3 + (size_t)c) * TN +
-                                 (size_t)(tl[0][ix] + T * (tl[1][iy] + T * tl[2][iz]));
+                                 (size_t)(tl[0][ix] + tile_sx(c) * (tl[1][iy] + tile_sy(c) * tl[2][iz]));
                 fl[e] = gld(tile_flag, bid);
                 ps[e] = gld(tf, 2 * q);
                 pw[e] = gld(tf, 2 * q + 1);

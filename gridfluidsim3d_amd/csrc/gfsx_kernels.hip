@@ -2164,16 +2164,12 @@ __device__ __forceinline__ void face_epilogue_fo(const FaceOut &o, const BinGrid
     o.grid[f] = val;
 }
 
-__global__ __launch_bounds__(sbrick::NT) void k_p2g_combine(
-    const SceneDesc *__restrict__ scenes, int nsc, int64_t nface, int64_t per_xcd,
+// The value of global face gf (see k_p2g_combine).
+__device__ __forceinline__ void combine_face(
+    const SceneDesc *__restrict__ scenes, int nsc, int64_t gf,
     const uint32_t *__restrict__ count, const uint8_t *__restrict__ brick_flag,
     const float2 *__restrict__ tiles, const uint8_t *__restrict__ tile_flag) {
     using namespace sbrick;
-    // one thread per face, faces in array order (i fastest); each XCD takes
-    // a contiguous range, so the tiles its faces read stay in its L2
-    const int64_t gb = (int64_t)(blockIdx.x & 7u) * per_xcd + (int64_t)(blockIdx.x >> 3);
-    const int64_t gf = gb * NT + threadIdx.x;
-    if (gf >= nface) return;
     const SceneDesc &sd = scenes[scene_of(scenes, nsc, (uint64_t)gf, [](const SceneDesc &d) { return (uint64_t)d.f0; })];
     const StepP2G &a = sd.p2g;
     int64_t lf = gf - sd.f0;
@@ -2190,23 +2186,6 @@ __global__ __launch_bounds__(sbrick::NT) void k_p2g_combine(
     const uint32_t w = (uint32_t)(a.ni + (c == 0)), h = (uint32_t)(a.nj + (c == 1));
     const uint32_t lf32 = (uint32_t)lf, row = lf32 / w;
     const int I = (int)(lf32 - row * w), J = (int)(row % h), K = (int)(row / h);
-    const int nb[3] = {sd.nbx, sd.nby, sd.nbz};
-    const int n[3] = {I, J, K};
-    // per axis the bricks (lower one first) whose tiles hold node n: the
-    // tile of brick b covers 4b - o .. 4b - o + 4 + o (o = 1 off the own axis)
-    int bl[3][2], tl[3][2], cnt[3];
-#pragma unroll
-    for (int ax = 0; ax < 3; ax++) {
-        const int o = ax == c ? 0 : 1;
-        const int m = n[ax] + o, b1 = m >> 2, t1 = m & 3;
-        const bool lower = (ax == c ? t1 == 0 : t1 <= 1) && b1 >= 1 && b1 - 1 < nb[ax];
-        const bool upper = b1 < nb[ax];
-        bl[ax][0] = lower ? b1 - 1 : b1;
-        tl[ax][0] = lower ? t1 + 4 : t1;
-        bl[ax][1] = b1;
-        tl[ax][1] = t1;
-        cnt[ax] = lower ? (upper ? 2 : 1) : (upper ? 1 : 0);
-    }
     // The brick that owns the face (the gathers' assignment) has no particle
     // within reach of any of its faces unless the sort flagged it: then every
     // tile entry of the face is 0.  Otherwise the tile flags and entries are
@@ -2217,6 +2196,23 @@ __global__ __launch_bounds__(sbrick::NT) void k_p2g_combine(
                                                       sd.nby * min(K >> 2, sd.nbz - 1)));
     float S = 0.0f, W = 0.0f;
     if (gld(brick_flag, own) != 0) {
+        const int nb[3] = {sd.nbx, sd.nby, sd.nbz};
+        const int n[3] = {I, J, K};
+        // per axis the bricks (lower one first) whose tiles hold node n: the
+        // tile of brick b covers 4b - o .. 4b - o + 4 + o (o = 1 off the own axis)
+        int bl[3][2], tl[3][2], cnt[3];
+#pragma unroll
+        for (int ax = 0; ax < 3; ax++) {
+            const int o = ax == c ? 0 : 1;
+            const int m = n[ax] + o, b1 = m >> 2, t1 = m & 3;
+            const bool lower = (ax == c ? t1 == 0 : t1 <= 1) && b1 >= 1 && b1 - 1 < nb[ax];
+            const bool upper = b1 < nb[ax];
+            bl[ax][0] = lower ? b1 - 1 : b1;
+            tl[ax][0] = lower ? t1 + 4 : t1;
+            bl[ax][1] = b1;
+            tl[ax][1] = t1;
+            cnt[ax] = lower ? (upper ? 2 : 1) : (upper ? 1 : 0);
+        }
         const float *tf = reinterpret_cast<const float *>(tiles);
         uint32_t fl[8];
         float ps[8], pw[8];
@@ -2241,6 +2237,23 @@ __global__ __launch_bounds__(sbrick::NT) void k_p2g_combine(
         }
     }
     face_epilogue_fo(face_out(a, c), sd.bg, count + sd.b0, c, I, J, K, S, W);
+}
+
+// Two faces per thread, NT apart, faces in array order (i fastest), so
+// each thread has two independent chains of loads in flight; each XCD takes
+// a contiguous range of faces, so the tiles its faces read stay in its L2.
+constexpr int kCombineFaces = 2;
+
+__global__ __launch_bounds__(sbrick::NT) void k_p2g_combine(
+    const SceneDesc *__restrict__ scenes, int nsc, int64_t nface, int64_t per_xcd,
+    const uint32_t *__restrict__ count, const uint8_t *__restrict__ brick_flag,
+    const float2 *__restrict__ tiles, const uint8_t *__restrict__ tile_flag) {
+    const int64_t gb = (int64_t)(blockIdx.x & 7u) * per_xcd + (int64_t)(blockIdx.x >> 3);
+#pragma unroll
+    for (int r = 0; r < kCombineFaces; r++) {
+        const int64_t gf = (gb * kCombineFaces + r) * sbrick::NT + threadIdx.x;
+        if (gf < nface) combine_face(scenes, nsc, gf, count, brick_flag, tiles, tile_flag);
+    }
 }
 
 struct StepG2P {
@@ -2744,7 +2757,7 @@ void flip_step(const StepBatch &s, const SortBuffers &sb, hipStream_t st, hipEve
                            (const uint32_t *)sb.start, (const uint8_t *)sb.brick_flag, s.tiles,
                            s.tile_flag);
         GFSX_LAUNCH_CHECK();
-        const int64_t nfb = (s.tot.nface + sbrick::NT - 1) / sbrick::NT;
+        const int64_t nfb = (s.tot.nface + kCombineFaces * sbrick::NT - 1) / (kCombineFaces * sbrick::NT);
         const int64_t fper = (nfb + 7) / 8;
         hipLaunchKernelGGL(k_p2g_combine, dim3((unsigned)(8 * fper)), block, 0, st, s.scenes_dev,
                            nsc, s.tot.nface, fper, (const uint32_t *)sb.count,

@@ -1803,8 +1803,9 @@ __global__ __launch_bounds__(brick::NT, 3) void k_flip_p2g_dense(
 // cells only (4^3 cells; the first and last brick of an axis also own the
 // pad cells beyond the domain).  Every particle adds K and K v to the 2x2x2
 // nodes of each component around it -- the candidates of the gather: the
-// cell and half-cell come from the same FP64 coordinates, and K is the same
-// FP32 expression -- in a tile of 5x6x6 nodes per component in LDS.  Node
+// cell and half-cell come from the same FP64 coordinates, and K the same
+// FP32 offsets (the squared distance is scaled by 1/R^2 per axis, before the
+// sum) -- in a tile of 5x6x6 nodes per component in LDS.  Node
 // n of axis a is tile entry n - 4b + (a != c): the own axis covers nodes
 // 4b..4b+4, the two others 4b-1..4b+4, which holds every in-domain node a
 // particle of the brick reaches (a pad-cell particle's nodes beyond the
@@ -1843,9 +1844,9 @@ constexpr int PRE = 2;             // particles per thread kept in registers
 constexpr uint32_t BAD_NAN = 1u, BAD_PINF = 2u, BAD_NINF = 4u;
 }  // namespace sbrick
 
-// K of one tile corner from the per-axis squared distances.
-__device__ __forceinline__ float corner_k(float sx, float sy, float sz, float inv_r2) {
-    return wyvill(((sx + sy) + sz) * inv_r2);
+// K of one tile corner from the per-axis squared distances over R^2.
+__device__ __forceinline__ float corner_k(float sx, float sy, float sz) {
+    return wyvill((sx + sy) + sz);
 }
 
 // x >= 0 (a float below 2^63) truncated to an integer, as 64 bits: the high
@@ -1862,7 +1863,7 @@ __device__ __forceinline__ unsigned long long fix64(float x) {
 
 // Add K and K (v + V) for the 8 corners of one component of one particle.
 // (tx, ty, tz) is the tile position of corner (0,0,0); sq[a][0..1] the
-// squared distances to the two nodes along axis a; vs = (v + V) * scale of
+// squared distances over R^2 to the two nodes along axis a; vs = (v + V) * scale of
 // the S sums.  FAST: every corner lies in the tile and v is finite (no
 // checks).  Otherwise corners outside the tile are skipped and, with vbad,
 // the product K v is recorded in the node's flag bits instead of the sum.
@@ -1870,7 +1871,7 @@ template <int C, bool FAST>
 __device__ __forceinline__ void scatter_comp(unsigned long long *tS, unsigned long long *tW,
                                              uint32_t *sbad, int cbase, int tx, int ty, int tz,
                                              const float sq[3][2], float vs, float sW,
-                                             float inv_r2, bool vbad, uint32_t badbits) {
+                                             bool vbad, uint32_t badbits) {
     using namespace sbrick;
 #pragma unroll
     for (int dz = 0; dz < 2; dz++)
@@ -1885,7 +1886,7 @@ __device__ __forceinline__ void scatter_comp(unsigned long long *tS, unsigned lo
                         continue;
                 }
                 const int idx = (tx + dx) + SX * ((ty + dy) + SY * (tz + dz));
-                const float K = corner_k(sq[0][dx], sq[1][dy], sq[2][dz], inv_r2);
+                const float K = corner_k(sq[0][dx], sq[1][dy], sq[2][dz]);
                 if (FAST || !vbad) {
                     // about half the box corners lie beyond the radius
                     if (K > 0.0f) {
@@ -2057,10 +2058,10 @@ __global__ __launch_bounds__(sbrick::NT) void k_p2g_scatter(
             const float ph = p[ax] - half;   // p -= offset (scalarfield.cpp:168)
             const float d0 = ga0 - p[ax], d1 = ga1 - p[ax];
             const float e0 = gb0 - ph, e1 = gb1 - ph;
-            sq[0][ax][0] = d0 * d0;
-            sq[0][ax][1] = d1 * d1;
-            sq[1][ax][0] = e0 * e0;
-            sq[1][ax][1] = e1 * e1;
+            sq[0][ax][0] = d0 * d0 * inv_r2;
+            sq[0][ax][1] = d1 * d1 * inv_r2;
+            sq[1][ax][0] = e0 * e0 * inv_r2;
+            sq[1][ax][1] = e1 * e1 * inv_r2;
         }
         // corner (0,0,0) of component c in tile coordinates
         int tc[3][3];
@@ -2093,7 +2094,7 @@ __global__ __launch_bounds__(sbrick::NT) void k_p2g_scatter(
             const uint32_t bits = v[c] != v[c] ? BAD_NAN : (v[c] > 0.0f ? BAD_PINF : BAD_NINF);
             const float vs = vbad[c] ? 0.0f : (v[c] + vbias[c]) * sS[c];
             scatter_comp<c, F>(&s_S[rep][c][0], &s_W[rep][c][0], s_bad, c * TN, tc[c][0],
-                               tc[c][1], tc[c][2], q, vs, sW, inv_r2, vbad[c], bits);
+                               tc[c][1], tc[c][2], q, vs, sW, vbad[c], bits);
         };
         using I0 = std::integral_constant<int, 0>;
         using I1 = std::integral_constant<int, 1>;

@@ -1915,27 +1915,40 @@ __global__ __launch_bounds__(sbrick::NT) void k_p2g_scatter(
     const uint32_t gid = (blockIdx.x & 7u) * (uint32_t)per_xcd + (blockIdx.x >> 3);
     if (gid >= nbrick) return;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    if (gld(brick_flag, gid) == 0) {
+    const SceneDesc &sd = scenes[scene_of(scenes, nsc, gid, [](const SceneDesc &d) { return d.k0; })];
+    // every scene field used below, read into registers here: the loads go
+    // out in one batch together with the flag load, instead of one round
+    // trip to the scene table at each first use
+    const uint32_t sk0 = sd.k0, sb0 = sd.b0;
+    const int nbx = sd.nbx, nby = sd.nby, nbz = sd.nbz;
+    const int B0 = sd.bg.b[0], B1 = sd.bg.b[1], B2 = sd.bg.b[2];
+    const double inv_dx = sd.bg.inv_dx, dxd = sd.dx;
+    const float *xp = sd.p2g.x, *yp = sd.p2g.y, *zp = sd.p2g.z;
+    const float *vp0 = sd.p2g.vel[0], *vp1 = sd.p2g.vel[1], *vp2 = sd.p2g.vel[2];
+    const float half = sd.p2g.ng[0].off[1];
+    const uint8_t flag = gld(brick_flag, gid);
+    asm volatile("" ::"s"(sk0), "s"(sb0), "s"(nbx), "s"(nby), "s"(nbz), "s"(B0), "s"(B1), "s"(B2));
+    asm volatile("" ::"s"(inv_dx), "s"(dxd), "s"(xp), "s"(yp), "s"(zp), "s"(vp0), "s"(vp1),
+                 "s"(vp2), "s"(half));
+    if (flag == 0) {
         // no particle within reach of this brick, so none in its cells
         if (t == 0) tile_flag[gid] = 0;
         return;
     }
-    const SceneDesc &sd = scenes[scene_of(scenes, nsc, gid, [](const SceneDesc &d) { return d.k0; })];
-    const BinGrid &bg = sd.bg;
-    const int lb = (int)(gid - sd.k0);
-    const int bxi = lb % sd.nbx, byi = (lb / sd.nbx) % sd.nby, bzi = lb / (sd.nbx * sd.nby);
+    const int lb = (int)(gid - sk0);
+    const int bxi = lb % nbx, byi = (lb / nbx) % nby, bzi = lb / (nbx * nby);
     const int i0 = bxi * 4, j0 = byi * 4, k0 = bzi * 4;
     // bins of the brick's own cells (bin = cell + 1), pads at the ends
-    const int bx_lo = bxi == 0 ? 0 : i0 + 1, bx_hi = bxi == sd.nbx - 1 ? bg.b[0] : i0 + 5;
-    const int by_lo = byi == 0 ? 0 : j0 + 1, by_hi = byi == sd.nby - 1 ? bg.b[1] : j0 + 5;
-    const int bz_lo = bzi == 0 ? 0 : k0 + 1, bz_hi = bzi == sd.nbz - 1 ? bg.b[2] : k0 + 5;
+    const int bx_lo = bxi == 0 ? 0 : i0 + 1, bx_hi = bxi == nbx - 1 ? B0 : i0 + 5;
+    const int by_lo = byi == 0 ? 0 : j0 + 1, by_hi = byi == nby - 1 ? B1 : j0 + 5;
+    const int bz_lo = bzi == 0 ? 0 : k0 + 1, bz_hi = bzi == nbz - 1 ? B2 : k0 + 5;
     const int ny = by_hi - by_lo, nrow = ny * (bz_hi - bz_lo);
-    const uint32_t *st0 = start + sd.b0;
+    const uint32_t *st0 = start + sb0;
     if (wave == 0) {
         uint32_t q0 = 0, len = 0;
         if (lane < nrow) {
             const uint32_t by = (uint32_t)(by_lo + lane % ny), bz = (uint32_t)(bz_lo + lane / ny);
-            const uint32_t row = (uint32_t)bg.b[0] * (by + (uint32_t)bg.b[1] * bz);
+            const uint32_t row = (uint32_t)B0 * (by + (uint32_t)B1 * bz);
             q0 = gld(st0, row + (uint32_t)bx_lo);
             len = gld(st0, row + (uint32_t)bx_hi) - q0;
             s_rowq[lane] = q0;
@@ -1951,7 +1964,7 @@ __global__ __launch_bounds__(sbrick::NT) void k_p2g_scatter(
     if (t < nrow * ncx) {
         const int r = t / ncx, k = t - r * ncx;
         const uint32_t by = (uint32_t)(by_lo + r % ny), bz = (uint32_t)(bz_lo + r / ny);
-        const uint32_t cell = (uint32_t)bg.b[0] * (by + (uint32_t)bg.b[1] * bz) + (uint32_t)(bx_lo + k);
+        const uint32_t cell = (uint32_t)B0 * (by + (uint32_t)B1 * bz) + (uint32_t)(bx_lo + k);
         ccount = gld(st0, cell + 1) - gld(st0, cell);
     }
     if (t == 0) s_cmax = 0u;
@@ -1969,7 +1982,6 @@ __global__ __launch_bounds__(sbrick::NT) void k_p2g_scatter(
         return;
     }
     if (ccount > 0u) atomicMax(&s_cmax, ccount);
-    const StepP2G &a = sd.p2g;
     auto slot_of = [&](uint32_t s) {
         int lo = 0, hi = nrow - 1;
         while (lo < hi) {
@@ -1989,11 +2001,12 @@ __global__ __launch_bounds__(sbrick::NT) void k_p2g_scatter(
     for (int m = 0; m < PRE; m++) {
         const uint32_t s = (uint32_t)t + (uint32_t)(m * NT);
         const uint32_t q = s < N ? slot_of(s) : s_rowq[0];
-        px[m] = gld(a.x, q);
-        py[m] = gld(a.y, q);
-        pz[m] = gld(a.z, q);
-#pragma unroll
-        for (int c = 0; c < 3; c++) pv[m][c] = gld(a.vel[c], q);
+        px[m] = gld(xp, q);
+        py[m] = gld(yp, q);
+        pz[m] = gld(zp, q);
+        pv[m][0] = gld(vp0, q);
+        pv[m][1] = gld(vp1, q);
+        pv[m][2] = gld(vp2, q);
     }
 #pragma unroll
     for (int m = 0; m < PRE; m++) {
@@ -2009,7 +2022,7 @@ __global__ __launch_bounds__(sbrick::NT) void k_p2g_scatter(
         const uint32_t q = slot_of(s);
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            const float av = fabsf(gld(a.vel[c], q));
+            const float av = fabsf(gld(c == 0 ? vp0 : (c == 1 ? vp1 : vp2), q));
             if (av < kBig) vm[c] = fmaxf(vm[c], av);
         }
     }
@@ -2039,9 +2052,7 @@ __global__ __launch_bounds__(sbrick::NT) void k_p2g_scatter(
     }
 
     // pass 2: scatter
-    const double dxd = sd.dx;
-    const float inv_r2 = (float)(1.0 / (sd.dx * sd.dx));
-    const float half = a.ng[0].off[1];
+    const float inv_r2 = (float)(1.0 / (dxd * dxd));
     const int rep = lane & (NREP - 1);
     const int ob[3] = {i0, j0, k0};
     auto scatter = [&](float x, float y, float z, const float v[3]) {
@@ -2050,7 +2061,7 @@ __global__ __launch_bounds__(sbrick::NT) void k_p2g_scatter(
         float sq[2][3][2];   // [own?0:1][axis][node]
 #pragma unroll
         for (int ax = 0; ax < 3; ax++) {
-            const AxisCoords cc = axis_coords(p[ax], bg.inv_dx);
+            const AxisCoords cc = axis_coords(p[ax], inv_dx);
             ia[ax] = cc.ia;
             ib[ax] = cc.ib;
             const float ga0 = (float)((float)cc.ia * dxd), ga1 = (float)((float)(cc.ia + 1) * dxd);
@@ -2114,8 +2125,8 @@ __global__ __launch_bounds__(sbrick::NT) void k_p2g_scatter(
         if ((uint32_t)t + (uint32_t)(m * NT) < N) scatter(px[m], py[m], pz[m], pv[m]);
     for (uint32_t s = (uint32_t)t + (uint32_t)(PRE * NT); s < N; s += NT) {
         const uint32_t q = slot_of(s);
-        const float v[3] = {gld(a.vel[0], q), gld(a.vel[1], q), gld(a.vel[2], q)};
-        scatter(gld(a.x, q), gld(a.y, q), gld(a.z, q), v);
+        const float v[3] = {gld(vp0, q), gld(vp1, q), gld(vp2, q)};
+        scatter(gld(xp, q), gld(yp, q), gld(zp, q), v);
     }
     __syncthreads();
 

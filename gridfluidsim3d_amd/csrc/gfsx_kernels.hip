@@ -2597,17 +2597,40 @@ __device__ __forceinline__ void g2p_brick(const StepG2P &a, const MacGrid &g, co
     stage_box<B::EO>(s_old[2], a.old[2], g.ni, g.nj, g.nk + 1, lox, loy, loz);
     __syncthreads();
 
-    for (uint32_t s = (uint32_t)t; s < N; s += NT) {
-        // row of slot s: the last r with rows0[r] <= s
+    // slot s -> sort index: the row is the last r with rows0[r] <= s
+    auto slot_q = [&](uint32_t s) {
         int lo = 0, hi = nrow - 1;
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
             if (s_rows0[mid] <= s) lo = mid;
             else hi = mid - 1;
         }
-        const uint32_t q = s_rowq[lo] + (s - s_rows0[lo]);
-        const float x0[3] = {a.xin[0][q], a.xin[1][q], a.xin[2][q]};
-        const float v0[3] = {a.vin[0][q], a.vin[1][q], a.vin[2][q]};
+        return s_rowq[lo] + (s - s_rows0[lo]);
+    };
+    // the next particle's position and velocity are loaded while this one
+    // is computed
+    uint32_t qn = 0;
+    float xn[3] = {0.0f, 0.0f, 0.0f}, vn[3] = {0.0f, 0.0f, 0.0f};
+    if ((uint32_t)t < N) {
+        qn = slot_q((uint32_t)t);
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            xn[c] = gld(a.xin[c], qn);
+            vn[c] = gld(a.vin[c], qn);
+        }
+    }
+    for (uint32_t s = (uint32_t)t; s < N; s += NT) {
+        const uint32_t q = qn;
+        const float x0[3] = {xn[0], xn[1], xn[2]};
+        const float v0[3] = {vn[0], vn[1], vn[2]};
+        if (s + NT < N) {
+            qn = slot_q(s + NT);
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                xn[c] = gld(a.xin[c], qn);
+                vn[c] = gld(a.vin[c], qn);
+            }
+        }
         float k1[3], vo[3], vnew[3];
         brick_sample<ST, true>(g, a.old[0], a.old[1], a.old[2], s_cur[0], s_cur[1], s_cur[2],
                                s_old[0], s_old[1], s_old[2], lx, ly, lz, lox, loy, loz,

@@ -1813,7 +1813,7 @@ __global__ __launch_bounds__(brick::NT, 3) void k_flip_p2g_dense(
 // point, 64-bit, so they do not depend on the order of the additions: K and
 // K (v + V), V = the brick's largest |v| (so that every term is >= 0 and
 // S = S' - V W), scaled by powers of two chosen per brick so that no sum can
-// exceed 2^62, then rounded to integers.  A term keeps the 24 significant
+// exceed 2^62, then truncated to integers.  A term keeps the 24 significant
 // bits of its float product whatever its size, so the faces at the edge
 // of the particles' reach, where every K is small, keep their relative
 // precision.  Lane l adds into copy l & 1 of the tile, which spreads the
@@ -1826,10 +1826,10 @@ __global__ __launch_bounds__(brick::NT, 3) void k_flip_p2g_dense(
 // At the end the copies are summed, converted back to float and the tile
 // written to the brick's slot of the tile buffer.
 //
-// k_p2g_combine: one work-group per brick computes the values of its faces
-// (the faces the gather kernels assign to it) by adding, in a fixed order,
-// the entries of the at most two bricks per axis whose tiles hold the face,
-// then applies the weight and the body force.
+// k_p2g_combine: one thread per face (two faces per thread) computes its
+// value by adding, in a fixed order, the entries of the at most two bricks
+// per axis whose tiles hold the face, then applies the weight and the body
+// force; faces of bricks the sort did not flag are zero without any read.
 namespace sbrick {
 constexpr int T = 6;               // tile edge, nodes, off the component's own axis
 constexpr int TN = 5 * T * T;      // nodes per component tile (5 along the own axis)

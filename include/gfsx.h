@@ -196,8 +196,9 @@ void gfsx_flip_step_batched(gfsx_ctx *ctx, const gfsx_scene *scenes, int nscenes
                             const gfsx_step_params *params, int *err);
 
 /* Tunables of the fused step, by name (an unknown name or value fails).
- * Every variant is deterministic; they differ in speed, and the P2G
- * variants in the rounding of the sums (within FP32 rounding of each other).
+ * Every P2G and G2P variant is deterministic; they differ in speed, and the
+ * P2G variants in the rounding of the sums (within FP32 rounding of each
+ * other).
  *   "p2g_variant": 0 = brick-staged LDS gather, candidates streamed in
  *                      chunks,
  *                  1 = direct gather from global memory,
@@ -220,7 +221,7 @@ int gfsx_get_option(gfsx_ctx *ctx, const char *name, int *err);
 
 /* Per-stage timing of gfsx_flip_step with HIP events recorded on the
  * context's stream.  After gfsx_timing_begin every step records four events:
- * before the cell sort, after the sort, after the P2G gather and after the
+ * before the cell sort, after the sort, after P2G and after the
  * G2P/PIC-FLIP/RK4 kernel.  gfsx_timing_end waits for the stream, writes the
  * summed milliseconds of the three stages to stage_ms[0..2] and of whole
  * steps to stage_ms[3], stops recording and returns the number of steps

@@ -1,9 +1,9 @@
 #!/usr/bin/env python3
 """Throughput of the plain FLIP transfer + advection step on one GPU.
 
-One step = cell sort -> P2G gather (u, v, w with weights, normalisation,
-snapshot, body force on fluid-bordering faces) -> G2P with PIC/FLIP blend and
-RK4 advection, over every particle of the scene, inputs resident in HBM.
+One step = brick sort -> P2G (u, v, w with weights, normalisation, snapshot,
+body force on fluid-bordering faces) -> G2P with PIC/FLIP blend and RK4
+advection, over every particle of the scene, inputs resident in HBM.
 The default workload is the 256^3 pool-plus-spheres scene (~43M particles,
 dx = 1/32, dt = 1/240, curl noise at 3 m/s, seed 1003), trilinear stencil.
 
@@ -38,17 +38,24 @@ def face_count(dims):
 
 def algorithmic_bytes(P, F, C):
     """Bytes each stage must move at minimum (FP32 / uint32):
-    sort: read positions and velocities (24P), write them in cell order
-          (24P) with the permutation (4P), one pass over the cell counts and
-          starts (8C);
+    sort: read positions and velocities (24P), write them in brick order
+          (24P), one pass over the bins' counts and starts (16C, C bins:
+          64-bit counters);
     p2g : read x, v once (24P) + write the field and its pre-force
           snapshot, one value per face each (8F);
     g2p : read x, v (24P) + write x', v' (24P) + read the field and the
           snapshot once (8F);
     step: the whole step: read x, v and write x', v' (48P), write the field
           and read it back (8F)."""
-    return dict(sort=52 * P + 8 * C, p2g=24 * P + 8 * F, g2p=48 * P + 8 * F,
+    return dict(sort=48 * P + 16 * C, p2g=24 * P + 8 * F, g2p=48 * P + 8 * F,
                 step=48 * P + 8 * F)
+
+
+def brick_bins(dims):
+    """Bins of the brick sort: 8 per G2P brick (8^3 cells) plus the overflow
+    bin."""
+    g = [((d + 3) // 4 + 1) // 2 for d in dims]
+    return 8 * g[0] * g[1] * g[2] + 1
 
 
 # MI355X's published FP32 vector rate; it counts packed FP32 instructions
@@ -79,9 +86,10 @@ def flops_per_particle(stencil):
 # The kernels each stage of gfsx_flip_step launches (HIP events bracket the
 # stage on the library's stream).
 STAGE_KERNELS = {
-    "sort": "cell sort: k_bin, scans, k_scatter, k_order_bins/mid/big, k_sorted_copy",
-    "p2g": "k_p2g_scatter + k_p2g_combine",
-    "g2p": "k_flip_g2p_brick_tl (trilinear) / k_flip_g2p_brick_tc (tricubic)",
+    "sort": "brick sort: k_bsort_bin, k_tile_scan64, k_bsort_list, k_bsort_order_*, "
+            "k_bsort_scatter",
+    "p2g": "k_p2g_bscatter + k_p2g_bcombine",
+    "g2p": "k_flip_g2p_bb_tl (trilinear) / k_flip_g2p_bb_tc (tricubic)",
 }
 
 
@@ -125,9 +133,14 @@ def copy_calibration(nbytes, reps=5):
     print(json.dumps({"copy_bytes": src.numel() * 4, "reps": reps}), flush=True)
 
 
-def cpu_baseline(scene, stencil, target_s=15.0):
-    """Time the reference's CPU path (its own compiled sources, one thread)
-    on an x-slab of the scene sized for ~target_s seconds."""
+def cpu_baseline(scene, stencil, target_s=15.0, steps=None, gpu_check=False):
+    """Time the reference's CPU path (its own compiled sources, one thread).
+
+    steps=None: one step on an x-slab of the scene sized for ~target_s
+    seconds.  steps=k: k steps of the whole scene.  With gpu_check, the GPU
+    runs the same first step on the same particles and the record carries the
+    per-field norm-wise error of the GPU against the reference's output
+    (max|gpu - ref| / max|ref|), so the benchmark scene itself is checked."""
     import numpy as np
     from oracle import oracle
     lib = oracle.load_reference()
@@ -138,23 +151,78 @@ def cpu_baseline(scene, stencil, target_s=15.0):
     ni, nj, nk = scene.dims
 
     def run(cells):
-        keep = scene.pos[:, 0] < cells * scene.dx
+        keep = scene.pos[:, 0] < np.float32(cells * scene.dx)
         pos = np.ascontiguousarray(scene.pos[keep])
         vel = np.ascontiguousarray(scene.vel[keep])
         t0 = time.perf_counter()
-        lib.flip_step(pos, vel, (cells, nj, nk), scene.dx, scene.dt, scene.body_force,
-                      0.05, stencil)
-        return pos.shape[0], time.perf_counter() - t0
+        out = lib.flip_step(pos, vel, (cells, nj, nk), scene.dx, scene.dt, scene.body_force,
+                            0.05, stencil)
+        return pos, vel, out, time.perf_counter() - t0
 
-    cells = max(1, ni // 64)
-    p, t = run(cells)
-    if t > 0 and cells < ni:
-        cells = int(min(ni, max(cells, round(cells * target_s / t))))
-        p, t = run(cells)
-    return dict(value=p / t, unit="particle-steps/s", cores=1, kind=kind,
-                sample="one plain step on the x-slab [0,%d) x %d x %d of the same scene "
-                       "(%d particles), %.1f s" % (cells, nj, nk, p, t),
-                nproc=os.cpu_count(), cpu=_cpu_model())
+    if steps is None:
+        cells = max(1, ni // 64)
+        pos, vel, out, t = run(cells)
+        if t > 0 and cells < ni:
+            cells = int(min(ni, max(cells, round(cells * target_s / t))))
+            pos, vel, out, t = run(cells)
+        p = pos.shape[0]
+        rec = dict(value=p / t, unit="particle-steps/s", cores=1, kind=kind,
+                   sample="one plain step on the x-slab [0,%d) x %d x %d of the same scene "
+                          "(%d particles), %.1f s" % (cells, nj, nk, p, t),
+                   nproc=os.cpu_count(), cpu=_cpu_model())
+        dims = (cells, nj, nk)
+    else:
+        pos = np.ascontiguousarray(scene.pos)
+        vel = np.ascontiguousarray(scene.vel)
+        first = None
+        t = 0.0
+        x, v = pos, vel
+        for k in range(steps):
+            t0 = time.perf_counter()
+            o = lib.flip_step(x, v, scene.dims, scene.dx, scene.dt, scene.body_force, 0.05,
+                              stencil)
+            t += time.perf_counter() - t0
+            first = o if first is None else first
+            x, v = o["pos"], o["vel"]
+        out = first
+        p = pos.shape[0]
+        rec = dict(value=p * steps / t, unit="particle-steps/s", cores=1, kind=kind,
+                   sample="%d plain steps of the whole scene (%d particles), %.1f s"
+                          % (steps, p, t),
+                   nproc=os.cpu_count(), cpu=_cpu_model())
+        dims = scene.dims
+    if gpu_check:
+        rec["gpu_vs_cpu"] = gpu_step_errors(pos, vel, dims, scene.dx, scene.dt,
+                                            scene.body_force, stencil, out)
+    return rec
+
+
+def gpu_step_errors(pos, vel, dims, dx, dt, body_force, stencil, ref):
+    """Norm-wise error max|gpu - ref| / max|ref| per field of one GPU step
+    (GFSX_ORDER_KEEP) against a CPU step's output dict."""
+    import numpy as np
+    import torch
+
+    import gridfluidsim3d_amd as gf
+    ctx = gf.Context(0)
+    st = gf.FlipState(pos, vel, dims, dx, keep_old=True, keep_weights=True)
+    ctx.flip_step(st, dt, body_force, 0.05, stencil, gf.ORDER_KEEP)
+    torch.cuda.synchronize()
+    got = dict(u=st.grid[0], v=st.grid[1], w=st.grid[2], u_old=st.grid_old[0],
+               wu=st.weights[0], vel=st.velocities(), pos=st.positions())
+    out = {}
+    for k, t in got.items():
+        a = t.cpu().numpy().astype(np.float64)
+        b = np.asarray(ref[k], np.float64)
+        den = np.abs(b).max()
+        out[k] = float(np.abs(a - b).max() / den) if den > 0 else float(np.abs(a - b).max())
+    x0 = pos.astype(np.float64)
+    a = got["pos"].cpu().numpy().astype(np.float64) - x0
+    b = np.asarray(ref["pos"], np.float64) - x0
+    out["displacement"] = float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+    del st
+    ctx.close()
+    return out
 
 
 def _cpu_model():
@@ -389,11 +457,64 @@ def fwdbwd_measure(reps, log_fn, steps=16, opt_iters=3):
     return out
 
 
+def plain_measure(ctx, scene, stencil, steps, warmup, reps, epilogue=0):
+    """Plain steps of one scene: warm-up steps on a scratch copy, then `reps`
+    timed runs of `steps` steps, each from the scene's initial state (HIP
+    events on the library's stream around the step loop; per-stage events
+    inside the library).  Reports the median run."""
+    import torch
+
+    import gridfluidsim3d_amd as gf
+    stream = torch.cuda.current_stream()
+
+    def step(st):
+        ctx.flip_step(st, scene.dt, scene.body_force, 0.05, stencil, gf.ORDER_SORTED, epilogue)
+
+    st = gf.FlipState(scene.pos, scene.vel, scene.dims, scene.dx)
+    for _ in range(warmup):
+        step(st)
+    torch.cuda.synchronize()
+    del st
+    runs = []
+    for _ in range(reps):
+        st = gf.FlipState(scene.pos, scene.vel, scene.dims, scene.dx)
+        torch.cuda.synchronize()
+        ctx.timing_begin()
+        start = torch.cuda.Event(enable_timing=True)
+        end = torch.cuda.Event(enable_timing=True)
+        start.record(stream)
+        for _ in range(steps):
+            step(st)
+        end.record(stream)
+        torch.cuda.synchronize()
+        nsteps, stage_ms = ctx.timing_end()
+        assert nsteps == steps
+        runs.append((start.elapsed_time(end) / steps, {k: v / nsteps for k, v in stage_ms.items()}))
+        del st
+    runs_sorted = sorted(runs, key=lambda r: r[0])
+    ms, stages = runs_sorted[len(runs) // 2]
+    P, F = scene.n, face_count(scene.dims)
+    alg = algorithmic_bytes(P, F, brick_bins(scene.dims))
+    gbs = alg["step"] / (ms / 1e3) / 1e9
+    return dict(workload=scene.name, particles=P, faces=F, steps=steps,
+                ms_per_step=ms, value=P / (ms / 1e3), unit="particle-steps/s",
+                ms_per_step_runs=[round(r[0], 4) for r in runs],
+                stages_ms={k: round(v, 4) for k, v in stages.items()},
+                step_roofline=dict(achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                                   frac=round(gbs / HBM_PEAK_GBS, 4),
+                                   algorithmic_bytes=alg["step"]))
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--reps", type=int, default=3,
+                    help="timed runs of --steps steps, each from the scene's initial state; "
+                         "the median is reported")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the 64^3 and 128^3 workloads (GPU and CPU)")
     ap.add_argument("--workload", default="pool256",
                     choices=["pool256", "dambreak128", "sphere64"])
     ap.add_argument("--stencil", default="trilinear", choices=["trilinear", "tricubic"])
@@ -444,45 +565,19 @@ def main():
     scene = scenes.CONFIGS[args.workload]()
     P = scene.n
     F = face_count(scene.dims)
-    C = (scene.dims[0] + 2) * (scene.dims[1] + 2) * (scene.dims[2] + 2)
+    C = brick_bins(scene.dims)
     log("scene %s: %d particles, %d faces, built in %.1f s" % (scene.name, P, F, time.time() - t0))
 
     ctx = gf.Context(0)
     log(ctx.device_info)
-    stream = torch.cuda.current_stream()
-
     epilogue = gf.EPILOGUE_FLUIDSIM if args.epilogue == "fluidsim" else gf.EPILOGUE_NONE
 
-    def step(st):
-        ctx.flip_step(st, scene.dt, scene.body_force, 0.05, stencil, gf.ORDER_SORTED, epilogue)
-
-    # warm-up on a scratch copy; the timed run starts from the scene's
-    # initial state and advances it args.steps times
-    st = gf.FlipState(scene.pos, scene.vel, scene.dims, scene.dx)
-    for _ in range(args.warmup):
-        step(st)
-    torch.cuda.synchronize()
-    del st
-    st = gf.FlipState(scene.pos, scene.vel, scene.dims, scene.dx)
-    torch.cuda.synchronize()
-    log("warmup done (%d steps)" % args.warmup)
-
-    ctx.timing_begin()
-    start = torch.cuda.Event(enable_timing=True)
-    end = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    start.record(stream)
-    for _ in range(args.steps):
-        step(st)
-    end.record(stream)
-    torch.cuda.synchronize()
-    elapsed_ms = start.elapsed_time(end)
-    nsteps, stage_ms = ctx.timing_end()
-    assert nsteps == args.steps
-    ms_per_step = elapsed_ms / args.steps
-    value = P * args.steps / (elapsed_ms / 1e3)
-    stage_avg = {k: v / nsteps for k, v in stage_ms.items()}
-    log("ms/step %.3f  stages %s" % (ms_per_step, {k: round(v, 3) for k, v in stage_avg.items()}))
+    head = plain_measure(ctx, scene, stencil, args.steps, args.warmup, args.reps, epilogue)
+    value = head["value"]
+    ms_per_step = head["ms_per_step"]
+    stage_avg = head["stages_ms"]
+    log("ms/step %.3f (median of %d runs: %s)  stages %s" % (
+        ms_per_step, args.reps, head["ms_per_step_runs"], stage_avg))
 
     alg = algorithmic_bytes(P, F, C)
     dom = max(("sort", "p2g", "g2p"), key=lambda k: stage_avg[k])
@@ -492,10 +587,7 @@ def main():
                     frac=round(achieved / HBM_PEAK_GBS, 4),
                     traffic=measured_traffic(args.workload, dom),
                     algorithmic_bytes=alg[dom], avg_ms=round(stage_avg[dom], 4))
-    step_gbs = alg["step"] / (ms_per_step / 1e3) / 1e9
-    step_roofline = dict(achieved=round(step_gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                         frac=round(step_gbs / HBM_PEAK_GBS, 4),
-                         algorithmic_bytes=alg["step"])
+    step_roofline = head["step_roofline"]
 
     fpp = flops_per_particle(args.stencil)
     compute = dict(flops_per_particle=fpp["p2g"] + fpp["g2p"],
@@ -505,36 +597,40 @@ def main():
                    peak_fp32_tflops=FP32_PEAK_TFLOPS,
                    peak_fp32_unpacked_tflops=FP32_PEAK_TFLOPS / 2)
     compute["step_frac_fp32"] = round(compute["step_tflops"] / FP32_PEAK_TFLOPS, 4)
+    st = None
 
     tricubic = None
     if not args.no_tricubic and args.stencil == "trilinear":
         # the ParticleAdvector stencil on the same scene: 1 warm-up step on a
         # copy, then 5 steps from the initial state
-        st = None
         torch.cuda.empty_cache()
-        tst = gf.FlipState(scene.pos, scene.vel, scene.dims, scene.dx)
-        ctx.flip_step(tst, scene.dt, scene.body_force, 0.05, gf.TRICUBIC, gf.ORDER_SORTED)
-        del tst
-        tst = gf.FlipState(scene.pos, scene.vel, scene.dims, scene.dx)
-        torch.cuda.synchronize()
-        ctx.timing_begin()
-        for _ in range(5):
-            ctx.flip_step(tst, scene.dt, scene.body_force, 0.05, gf.TRICUBIC, gf.ORDER_SORTED)
-        nt, tms = ctx.timing_end()
-        del tst
-        tavg = {k: v / nt for k, v in tms.items()}
+        tc = plain_measure(ctx, scene, gf.TRICUBIC, 5, 1, 1)
         tf = flops_per_particle("tricubic")
-        g2p_tf = tf["g2p"] * P / (tavg["g2p"] / 1e3) / 1e12
-        tricubic = dict(steps=nt, ms_per_step=round(tavg["step"], 3),
-                        value=P / (tavg["step"] / 1e3), unit="particle-steps/s",
-                        stages_ms={k: round(v, 4) for k, v in tavg.items()},
+        g2p_tf = tf["g2p"] * P / (tc["stages_ms"]["g2p"] / 1e3) / 1e12
+        tricubic = dict(steps=5, ms_per_step=tc["ms_per_step"], value=tc["value"],
+                        unit="particle-steps/s", stages_ms=tc["stages_ms"],
                         g2p_flops_per_particle=tf["g2p"], g2p_tflops=round(g2p_tf, 2),
                         g2p_frac_fp32=round(g2p_tf / FP32_PEAK_TFLOPS, 4))
         log("tricubic: %s" % tricubic)
 
+    configs_out = None
+    if not args.no_configs and args.workload == "pool256":
+        # the other plain workloads: 64^3 sphere and 128^3 dam break, both
+        # stencils, 10 steps from the initial state after 2 warm-up steps
+        configs_out = {}
+        for name in ("sphere64", "dambreak128"):
+            sc = scenes.CONFIGS[name]()
+            for stn, sname in ((gf.TRILINEAR, "trilinear"), (gf.TRICUBIC, "tricubic")):
+                torch.cuda.empty_cache()
+                r = plain_measure(ctx, sc, stn, 10, 2, 1)
+                r.pop("ms_per_step_runs", None)
+                configs_out["%s_%s" % (name, sname)] = r
+                log("%s %s: %.3f ms/step, %.3e particle-steps/s, step roofline %.4f" % (
+                    name, sname, r["ms_per_step"], r["value"], r["step_roofline"]["frac"]))
+            del sc
+
     batched_out = None
     if not args.no_batched:
-        del st
         torch.cuda.empty_cache()
         batched_out = batched_measure(args.batched_steps, stencil, log)
         batched_out["vs_plain"] = round(batched_out["value_graph"] / value, 4)
@@ -549,11 +645,19 @@ def main():
 
     cpu = None
     parity = None
+    cpu_configs = None
     if not args.no_cpu:
         parity = parity_report(log)
         log("CPU baseline ...")
-        cpu = cpu_baseline(scene, stencil, args.cpu_seconds)
+        cpu = cpu_baseline(scene, stencil, args.cpu_seconds, gpu_check=True)
         log("CPU baseline: %s" % cpu)
+        if args.workload == "pool256" and not args.no_configs:
+            cpu_configs = {}
+            for name, k in (("sphere64", 10), ("dambreak128", 1)):
+                sc = scenes.CONFIGS[name]()
+                cpu_configs[name] = cpu_baseline(sc, stencil, steps=k, gpu_check=True)
+                log("CPU baseline %s: %s" % (name, cpu_configs[name]))
+                del sc
 
     line = {
         "metric": METRIC,
@@ -575,9 +679,12 @@ def main():
         "step_roofline": step_roofline,
         "compute": compute,
         "tricubic": tricubic,
-        "stages_ms": {k: round(v, 4) for k, v in stage_avg.items()},
+        "stages_ms": stage_avg,
+        "ms_per_step_runs": head["ms_per_step_runs"],
         "cpu_baseline": cpu,
+        "cpu_baseline_configs": cpu_configs,
         "parity": parity,
+        "configs": configs_out,
         "batched": batched_out,
         "fwd_bwd": fwdbwd_out,
     }

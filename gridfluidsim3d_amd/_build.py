@@ -16,7 +16,7 @@ LIB = os.path.join(LIBDIR, "libgfsx.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["gfsx_kernels.hip", "gfsx_adjoint.hip", "gfsx_epilogue.hip", "gfsx_remove.hip", "gfsx_api.hip"]
+SOURCES = ["gfsx_kernels.hip", "gfsx_brick.hip", "gfsx_adjoint.hip", "gfsx_epilogue.hip", "gfsx_remove.hip", "gfsx_api.hip"]
 
 HIP_FLAGS = [
     "--offload-arch=" + ARCH,

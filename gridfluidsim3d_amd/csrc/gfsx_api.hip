@@ -40,6 +40,9 @@ enum Slot {
     S_B_GRID, S_B_OLD, S_B_WGT, S_B_LG, S_B_XO, S_B_VO, S_B_FX, S_B_MAX,
     S_E_N, S_E_W, S_E_LAYER, S_E_LIST, S_R_KEEP, S_R_DST, S_R_SCAN, S_R_N,
     S_P_TILE, S_P_TFLAG,
+    S_K_COUNT, S_K_START, S_K_PSTART, S_K_RSTART, S_K_RUNREC, S_K_NRUNS, S_K_LIST, S_K_LIST2,
+    S_K_RUNBASE, S_K_MID, S_K_HUGE, S_K_LCOUNT, S_K_SCAN, S_K_FLUID, S_K_SRANGE, S_K_SSLOT,
+    S_K_SCOUNT,
     NSLOT
 };
 
@@ -187,6 +190,56 @@ struct gfsx_ctx {
 };
 
 namespace {
+
+// Device memory handed out front to back from one region (the static
+// scratch of gfsx_flip_step_noalloc).  `used` counts what a layout needs even
+// past the end, so the caller can refuse it, with the full size, before
+// anything is written.
+struct Carve {
+    char *base = nullptr;
+    size_t cap = 0;
+    size_t used = 0;
+    void *take(size_t bytes) {
+        used = (used + 255) & ~(size_t)255;
+        void *p = base + used;
+        used += std::max<size_t>(bytes, 1);
+        return p;
+    }
+};
+
+// Workspace of the brick sort for a step of n particles in nblk binning
+// blocks over nbin brick bins (overflow bins included); ws(slot, bytes)
+// hands out device memory.
+template <class WS>
+gfsx::BrickSort brick_sort(WS &&ws, int64_t n, int64_t nblk, uint32_t nbin, int64_t ng2p,
+                           bool want_perm) {
+    gfsx::BrickSort b;
+    const int64_t nwave = nblk * 16;   // a binning block is 4 x 256 particles
+    const size_t u64 = sizeof(unsigned long long), u32 = sizeof(uint32_t);
+    const size_t nn = (size_t)std::max<int64_t>(n, 1), nw = (size_t)std::max<int64_t>(nwave, 1);
+    b.nwave = nwave;
+    b.nbb_total = nbin;
+    b.count = (unsigned long long *)ws(S_K_COUNT, u64 * nbin);
+    b.start = (unsigned long long *)ws(S_K_START, u64 * ((size_t)nbin + 1));
+    b.pstart = (uint32_t *)ws(S_K_PSTART, u32 * ((size_t)nbin + 1));
+    b.rstart = (uint32_t *)ws(S_K_RSTART, u32 * ((size_t)nbin + 1));
+    b.runrec = (uint2 *)ws(S_K_RUNREC, sizeof(uint2) * nw * 64);
+    b.nruns = (uint8_t *)ws(S_K_NRUNS, nw);
+    b.list = (uint32_t *)ws(S_K_LIST, u32 * nn);
+    b.list2 = (uint32_t *)ws(S_K_LIST2, u32 * nn);
+    b.runbase = (uint32_t *)ws(S_K_RUNBASE, u32 * nw * 64);
+    b.mid = (uint32_t *)ws(S_K_MID, u32 * nbin);
+    b.huge = (uint32_t *)ws(S_K_HUGE, u32 * nbin);
+    b.lcount = (uint32_t *)ws(S_K_LCOUNT, u32 * 2);
+    b.scan_scratch = (unsigned long long *)ws(
+        S_K_SCAN, u64 * (size_t)std::max<int64_t>(gfsx::scan64_scratch_words(nbin), 1));
+    b.perm = want_perm ? (uint32_t *)ws(S_PERM, u32 * nn) : nullptr;
+    b.fluid = (unsigned long long *)ws(S_K_FLUID, u64 * nbin);
+    b.slow_ranges = (uint2 *)ws(S_K_SRANGE, sizeof(uint2) * (size_t)std::max<int64_t>(ng2p, 1));
+    b.slow_slots = (uint32_t *)ws(S_K_SSLOT, u32 * nn);
+    b.slow_count = (uint32_t *)ws(S_K_SCOUNT, u32 * 2);
+    return b;
+}
 
 template <class F>
 void guarded(gfsx_ctx *ctx, int *err, F &&f) {
@@ -346,9 +399,26 @@ gfsx::SceneDesc scene_desc(const gfsx_scene *sc, const gfsx_step_params *pr, con
 }
 
 // Runs the step; returns the scenes as laid out (bin grids, offsets).
+// Runs the step; returns the scenes as laid out (bin grids, offsets).  With
+// carve, every device buffer (and the scene table) comes from that region
+// and nothing is allocated: only the brick pipeline, one scene, no
+// epilogue and no timing events.
 std::vector<gfsx::SceneDesc> flip_step_impl(gfsx_ctx *ctx, const gfsx_scene *scenes,
                                             const gfsx_step_params *params, int nscenes,
-                                            const char *who, bool grids_only = false) {
+                                            const char *who, bool grids_only = false,
+                                            Carve *carve = nullptr) {
+    auto ws = [&](int slot, size_t bytes) -> void * {
+        return carve ? carve->take(bytes) : ctx->get(slot, bytes);
+    };
+    auto wsf = [&](int slot, int64_t count) -> float * {
+        return (float *)ws(slot, sizeof(float) * (size_t)std::max<int64_t>(count, 1));
+    };
+    gfsx::SceneDesc *table_dev = nullptr;
+    if (carve) {
+        require(nscenes == 1 && ctx->p2g_variant == 4 && params[0].epilogue == GFSX_EPILOGUE_NONE,
+                "the no-allocation step takes one scene, p2g_variant 4 and no epilogue");
+        table_dev = (gfsx::SceneDesc *)carve->take(sizeof(gfsx::SceneDesc));
+    }
     gfsx::StepBatch s;
     s.scenes.resize(nscenes);
     for (int i = 0; i < nscenes; i++) {
@@ -362,6 +432,8 @@ std::vector<gfsx::SceneDesc> flip_step_impl(gfsx_ctx *ctx, const gfsx_scene *sce
     gfsx::layout_scenes(s.scenes.data(), nscenes, s.tot);
     require(s.tot.n < ((int64_t)1 << 32) - 1, "gfsx: too many particles in one step");
     require(s.tot.nbin < ((int64_t)1 << 32) - 2, "gfsx: too many cells in one step");
+    require(s.tot.nbb < ((int64_t)1 << gfsx::kBrickBinBits), "gfsx: too many bricks in one step");
+    require(s.tot.nblk * 1024 < ((int64_t)1 << 32) - 1024, "gfsx: too many particles in one step");
     require(nscenes == 1 || (ctx->p2g_variant != 1 && ctx->g2p_variant != 1),
             "gfsx: the direct P2G and G2P variants take one scene");
     // snapshots the caller does not keep live in one workspace buffer
@@ -369,7 +441,7 @@ std::vector<gfsx::SceneDesc> flip_step_impl(gfsx_ctx *ctx, const gfsx_scene *sce
     for (auto &d : s.scenes)
         for (int c = 0; c < 3; c++)
             if (!d.grid_old[c]) old_faces += mac_count(c, d.ni, d.nj, d.nk);
-    float *old_ws = ctx->f32(S_OLD0, old_faces);
+    float *old_ws = wsf(S_OLD0, old_faces);
     for (auto &d : s.scenes)
         for (int c = 0; c < 3; c++)
             if (!d.grid_old[c]) {
@@ -377,14 +449,16 @@ std::vector<gfsx::SceneDesc> flip_step_impl(gfsx_ctx *ctx, const gfsx_scene *sce
                 old_ws += mac_count(c, d.ni, d.nj, d.nk);
             }
     for (int c = 0; c < 3; c++) {
-        s.xs[c] = ctx->f32(S_XS0 + c, s.tot.n);
-        s.vs[c] = ctx->f32(S_VS0 + c, s.tot.n);
+        s.xs[c] = wsf(S_XS0 + c, s.tot.n);
+        s.vs[c] = wsf(S_VS0 + c, s.tot.n);
     }
-    s.dense = ctx->u32(S_DENSE, s.tot.np2g);
-    s.dense_count = ctx->u32(S_DENSEN, 2);
     if (ctx->p2g_variant == 4) {
-        s.tiles = (float2 *)ctx->f32(S_P_TILE, 2 * (int64_t)gfsx::kP2gTileNodes * s.tot.np2g);
-        s.tile_flag = (uint8_t *)ctx->u32(S_P_TFLAG, (s.tot.np2g + 3) / 4);
+        // one tile per brick bin (the overflow bins' stay unused)
+        s.tiles = (float2 *)wsf(S_P_TILE, 2 * (int64_t)gfsx::kP2gTileNodes * s.tot.nbb);
+        s.tile_flag = (uint8_t *)ws(S_P_TFLAG, (size_t)s.tot.nbb);
+    } else {
+        s.dense = ctx->u32(S_DENSE, s.tot.np2g);
+        s.dense_count = ctx->u32(S_DENSEN, 2);
     }
     s.num_cu = ctx->num_cu;
     gfsx::Sorted so;
@@ -438,6 +512,35 @@ std::vector<gfsx::SceneDesc> flip_step_impl(gfsx_ctx *ctx, const gfsx_scene *sce
     s.p2g_variant = ctx->p2g_variant;
     s.g2p_variant = ctx->g2p_variant;
     s.grids_only = grids_only;
+    hipEvent_t *ev = ctx->timing && !carve ? ctx->next_events() : nullptr;
+    if (ctx->p2g_variant == 4) {
+        bool ids = false;
+        for (auto &d : s.scenes) ids = ids || d.id_out != nullptr;
+        gfsx::BrickSort bs = brick_sort(ws, s.tot.n, s.tot.nblk, (uint32_t)s.tot.nbb,
+                                        s.tot.ng2p, s.keep_order || ids);
+        if (carve) {
+            if (carve->used > carve->cap) {
+                std::ostringstream m;
+                m << who << ": the step needs " << carve->used << " bytes of scratch, the static "
+                  << "buffer holds " << carve->cap << " (fewer particles or a smaller grid)";
+                throw gfsx::Error(m.str());
+            }
+            gfsx::put_scenes(table_dev, s.scenes.data(), nscenes, ctx->stream);
+            s.scenes_dev = table_dev;
+        } else {
+            s.scenes_dev = ctx->table(s.scenes);
+        }
+        s.bs = &bs;
+        if (fs) {
+            epi.fluid = bs.fluid;
+            epi.kb0 = s.scenes[0].kb0;
+            epi.gbx = s.scenes[0].gbx;
+            epi.gby = s.scenes[0].gby;
+            s.epi = &epi;
+        }
+        gfsx::flip_step_brick(s, bs, ctx->stream, ev);
+        return s.scenes;
+    }
     gfsx::SortBuffers sb = ctx->sort_buffers(s.scenes, s.tot.n, (uint32_t)s.tot.nbin,
                                              (uint32_t)s.tot.nblk);
     s.scenes_dev = sb.scenes;
@@ -448,7 +551,7 @@ std::vector<gfsx::SceneDesc> flip_step_impl(gfsx_ctx *ctx, const gfsx_scene *sce
         for (int a = 0; a < 3; a++) epi.b[a] = s.scenes[0].bg.b[a];
         s.epi = &epi;
     }
-    gfsx::flip_step(s, sb, ctx->stream, ctx->timing ? ctx->next_events() : nullptr);
+    gfsx::flip_step(s, sb, ctx->stream, ev);
     return s.scenes;
 }
 
@@ -777,6 +880,19 @@ void gfsx_flip_step_backward(gfsx_ctx *ctx, const gfsx_scene *sc,
         f.id_out = nullptr;
         // (the sort, its sorted copies and the P2G grids; no particle update)
         const std::vector<gfsx::SceneDesc> laid = flip_step_impl(ctx, &f, pr, 1, who, true);
+        if (ctx->p2g_variant == 4) {
+            // the adjoint's bricks take the particles in cell order: the cell
+            // sort of the step's inputs and the sorted copies
+            gfsx::SortBuffers sb = ctx->sort_buffers(laid, n, laid[0].bg.nbin + 1,
+                                                     (uint32_t)gfsx::sort_blocks(n));
+            gfsx::cell_sort(sb, n, ctx->stream, false);
+            float *xs[3], *vs[3];
+            for (int c = 0; c < 3; c++) {
+                xs[c] = (float *)ctx->buf[S_XS0 + c];
+                vs[c] = (float *)ctx->buf[S_VS0 + c];
+            }
+            gfsx::sorted_copy(sb.scenes, 1, sb.perm, n, xs, vs, ctx->stream);
+        }
         gfsx::AdjointSort so;
         so.start = (const uint32_t *)ctx->buf[S_START];
         so.perm = (const uint32_t *)ctx->buf[S_PERM];

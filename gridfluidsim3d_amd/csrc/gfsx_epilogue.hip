@@ -95,6 +95,15 @@ __global__ __launch_bounds__(kBlock) void k_fs_layers_init(FsEpilogue e) {
     int8_t v;
     if (e.solid && e.solid[t]) {
         v = kSolid;
+    } else if (e.fluid) {
+        // the cell's bit in its P2G brick's mask (brick bins: the 8 P2G
+        // bricks of each G2P brick consecutive)
+        const int bx = i >> 2, by = j >> 2, bz = k >> 2;
+        const uint32_t g = (uint32_t)(bx >> 1) +
+                           (uint32_t)e.gbx * ((uint32_t)(by >> 1) + (uint32_t)e.gby * (uint32_t)(bz >> 1));
+        const uint32_t bin = e.kb0 + ((g << 3) | (uint32_t)((bx & 1) | ((by & 1) << 1) | ((bz & 1) << 2)));
+        const int bit = (i & 3) + 4 * (j & 3) + 16 * (k & 3);
+        v = ((e.fluid[bin] >> bit) & 1ull) ? 0 : -1;
     } else {
         // the cell's bin in the sort's padded bin grid
         const uint32_t b = (uint32_t)(i + 1) + (uint32_t)e.b[0] *

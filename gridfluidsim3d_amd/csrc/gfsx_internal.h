@@ -121,7 +121,12 @@ struct SceneDesc {
     uint32_t k0;
     int gbx, gby, gbz;
     uint32_t g0;
+    // brick sort bins: the 8 P2G bricks of every G2P brick (8*gbx*gby*gbz,
+    // some beyond the grid when nbx, nby or nbz is odd), then the overflow
+    // bin (local index nbb); kb0 is the first global one
+    uint32_t kb0, nbb;
     int64_t f0;       // first global MAC face (faces of u, then v, then w, per scene)
+    int64_t r0;       // first global face row (the x-rows of u, then v, then w, per scene)
     // grid and step constants
     int ni, nj, nk;
     double dx;
@@ -245,8 +250,11 @@ struct AdjointSort {
 // (with the body force) and the snapshot, both extrapolated.
 struct FsEpilogue {
     int ni, nj, nk;
-    const uint32_t *start;      // bin starts of the sort (bins (n+2)^3, pad 1)
+    const uint32_t *start;      // bin starts of the cell sort (bins (n+2)^3, pad 1), or
     int b[3];                   // bins per axis
+    const unsigned long long *fluid;  // the brick sort's per-brick fluid-cell masks
+    uint32_t kb0;               // the scene's first brick bin (with fluid)
+    int gbx, gby;               // G2P bricks per axis (with fluid)
     const uint8_t *solid;       // optional, ni*nj*nk, nonzero = solid
     int8_t *layer;              // workspace, ni*nj*nk
     const float *n[3], *w[3];
@@ -365,6 +373,8 @@ struct SceneTotals {
     int64_t np2g = 0;    // P2G bricks
     int64_t ng2p = 0;    // G2P bricks
     int64_t nface = 0;   // MAC faces
+    int64_t nbb = 0;     // brick sort bins, overflow bins included
+    int64_t nrow = 0;    // face rows along x, all components
 };
 
 // Fill the derived fields of every scene (bin grid, offsets p0/b0/blk0/k0/g0,
@@ -374,6 +384,37 @@ void layout_scenes(SceneDesc *scenes, int nscene, SceneTotals &tot);
 // Nodes of one P2G brick's scatter tile, all three components: per
 // component 5 nodes along its own axis and 6 along the two others.
 constexpr int kP2gTileNodes = 3 * 5 * 6 * 6;
+
+// Workspace of the brick sort (gfsx_brick.hip).  A particle's bin is its
+// P2G brick; a wave of 64 consecutive particles of the input forms one run
+// per distinct bin among its lanes, and a bin's particles are laid out run
+// by run in wave order, each run in lane order: the order of a stable
+// counting sort by bin.
+struct BrickSort {
+    unsigned long long *count;   // nbb_total: (runs << 32) | particles per bin
+    unsigned long long *start;   // nbb_total + 1: exclusive scan of count
+    uint32_t *pstart;            // nbb_total + 1: first sort slot of each bin
+    uint32_t *rstart;            // nbb_total + 1: first entry of each bin's run list
+    uint2 *runrec;               // 64 per wave: (bin | (len-1) << 26, provisional rank)
+    uint8_t *nruns;              // per wave: its number of runs
+    uint32_t *list;              // runs (wave*64 + run), grouped by bin
+    uint32_t *list2;             // the same ordered, for bins of many runs
+    uint32_t *runbase;           // 64 per wave: the first sort slot of each run
+    uint32_t *mid, *huge;        // bins of 17..256 runs, of more
+    uint32_t *lcount;            // 2: entries of mid and huge
+    unsigned long long *scan_scratch;
+    uint32_t *perm;              // nullable: sort slot -> global particle index
+    unsigned long long *fluid;   // per bin: the cells of the brick that hold a particle
+    // G2P work left to the global-memory kernel: brick ranges (at most one
+    // per G2P brick), single sort slots (at most n), and the two counts
+    uint2 *slow_ranges;
+    uint32_t *slow_slots;
+    uint32_t *slow_count;
+    int64_t nwave;
+    uint32_t nbb_total;
+};
+constexpr uint32_t kBrickBinBits = 26;   // global bins below 2^26 (runrec packs len above)
+int64_t scan64_scratch_words(int64_t m);
 
 // A fused step over one or more scenes.
 struct StepBatch {
@@ -393,7 +434,26 @@ struct StepBatch {
     int g2p_variant;   // 0: brick-staged LDS boxes, 1: direct global sampling (one scene)
     const FsEpilogue *epi = nullptr;   // FluidSimulation epilogue after P2G (one scene)
     bool grids_only = false;           // stop after P2G (the adjoint's recompute)
+    const BrickSort *bs = nullptr;     // p2g_variant 4: the brick sort's workspace
 };
+
+// The fused step with p2g_variant 4: brick sort, scatter P2G into per-brick
+// tiles, per-brick combine, G2P over brick ranges (gfsx_brick.hip).
+void flip_step_brick(const StepBatch &s, const BrickSort &bs, hipStream_t st, hipEvent_t *ev);
+
+// Write n scene table entries to device memory at dst with one small kernel
+// each (the entries travel as kernel arguments).
+void put_scenes(SceneDesc *dst, const SceneDesc *src, int n, hipStream_t st);
+
+// The direct G2P kernel (g2p_variant 1, one scene) over the particles in
+// s.xs/s.vs; perm maps sort slots to particle indices (needed for
+// GFSX_ORDER_KEEP and particle ids).
+void flip_g2p_direct(const StepBatch &s, const uint32_t *perm, hipStream_t st);
+
+// Positions and velocities of every scene copied into the order of perm
+// (the cell sort), for the adjoint's bricks.
+void sorted_copy(const SceneDesc *scenes_dev, int nsc, const uint32_t *perm, int64_t n,
+                 float *const xs[3], float *const vs[3], hipStream_t s);
 
 // The fused plain step of every scene of s, in one launch per kernel.
 // Needs SortBuffers laid out for s.tot (and sb.scenes == s.scenes_dev).

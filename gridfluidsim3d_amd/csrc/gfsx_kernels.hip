@@ -17,6 +17,7 @@
 
 #include "gfsx_internal.h"
 #include "gfsx_sample.h"
+#include "gfsx_device.h"
 
 #include <algorithm>
 #include <type_traits>
@@ -27,10 +28,6 @@ namespace gfsx {
 // Utilities
 // ===========================================================================
 
-static inline unsigned grid_for(int64_t n, int64_t per_block) {
-    int64_t b = (n + per_block - 1) / per_block;
-    return (unsigned)std::max<int64_t>(b, 1);
-}
 
 __global__ void k_fill_u32(uint32_t *p, uint32_t v, int64_t n) {
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -75,35 +72,7 @@ void fill_f32(float *p, float value, int64_t count, hipStream_t s) {
 // Device-wide exclusive scan of uint32 (tiles of 4096, three phases)
 // ===========================================================================
 
-__device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v) {
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        uint32_t t = __shfl_up(v, d, 64);
-        if (lane >= d) v += t;
-    }
-    return v;
-}
 
-// Exclusive prefix of v over the 256-thread block; total receives the sum.
-__device__ __forceinline__ uint32_t block_exclusive_sum(uint32_t v,
-                                                        uint32_t *wsum,
-                                                        uint32_t &total) {
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    uint32_t inc = wave_inclusive_sum(v);
-    if (lane == 63) wsum[wid] = inc;
-    __syncthreads();
-    uint32_t woff = 0, tot = 0;
-#pragma unroll
-    for (int w = 0; w < kBlock / 64; w++) {
-        uint32_t s = wsum[w];
-        woff += (w < wid) ? s : 0u;
-        tot += s;
-    }
-    __syncthreads();
-    total = tot;
-    return woff + inc - v;
-}
 
 __global__ __launch_bounds__(kBlock) void k_tile_reduce(const uint32_t *in,
                                                        int64_t m,
@@ -205,19 +174,6 @@ constexpr int64_t kBinTile = (int64_t)kBinItems * kBlock;
 
 int64_t sort_blocks(int64_t n) { return (n + kBinTile - 1) / kBinTile; }
 
-// Index of the scene that owns value v of a per-scene ascending offset
-// (p0, b0, blk0, k0 or g0): the last scene whose offset is <= v.
-template <class F>
-__device__ __forceinline__ int scene_of(const SceneDesc *__restrict__ sc, int nsc, uint64_t v,
-                                        F offset) {
-    int lo = 0, hi = nsc - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if ((uint64_t)offset(sc[mid]) <= v) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
-}
 
 // Bin every particle.  A block takes kBinTile consecutive particles of one
 // scene.  In-grid particles take a slot in their bin with an integer
@@ -361,27 +317,6 @@ __global__ __launch_bounds__(kBlock) void k_scatter(
     }
 }
 
-// Bitonic sorting network over N keys in registers, ascending.
-template <int N, class T>
-__device__ __forceinline__ void bitonic_sort(T (&v)[N]) {
-#pragma unroll
-    for (int k = 2; k <= N; k <<= 1) {
-#pragma unroll
-        for (int j = k >> 1; j > 0; j >>= 1) {
-#pragma unroll
-            for (int i = 0; i < N; i++) {
-                const int l = i ^ j;
-                if (l > i) {
-                    const bool up = (i & k) == 0;
-                    const T a = v[i], b = v[l];
-                    const T lo = a < b ? a : b, hi = a < b ? b : a;
-                    v[i] = up ? lo : hi;
-                    v[l] = up ? hi : lo;
-                }
-            }
-        }
-    }
-}
 
 // The sort key of slot q: the particle index, or (octant, index) when
 // octants are given.  Keys of one bin are distinct.
@@ -521,47 +456,6 @@ __global__ __launch_bounds__(kBlock) void k_order_bins(
     }
 }
 
-// Bitonic sort of the 64*K keys held by a wave, element e = lane*K + r in
-// v[r]: pairs inside a lane swap registers, pairs across lanes exchange
-// through a lane shuffle and keep the minimum or the maximum.
-template <int K>
-__device__ __forceinline__ void wave_bitonic(uint64_t (&v)[K]) {
-    constexpr int N = 64 * K;
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int k = 2; k <= N; k <<= 1) {
-#pragma unroll
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            if (j < K) {
-#pragma unroll
-                for (int r = 0; r < K; r++) {
-                    const int l = r ^ j;
-                    if (l > r) {
-                        const int e = lane * K + r;
-                        const bool up = (e & k) == 0;
-                        const uint64_t a = v[r], b = v[l];
-                        const bool sw = up ? (a > b) : (a < b);
-                        v[r] = sw ? b : a;
-                        v[l] = sw ? a : b;
-                    }
-                }
-            } else {
-                const int lx = j / K;
-#pragma unroll
-                for (int r = 0; r < K; r++) {
-                    const int e = lane * K + r;
-                    const uint32_t lo = __shfl_xor((uint32_t)v[r], lx, 64);
-                    const uint32_t hi = __shfl_xor((uint32_t)(v[r] >> 32), lx, 64);
-                    const uint64_t o = ((uint64_t)hi << 32) | lo;
-                    const bool up = (e & k) == 0, lower = (e & j) == 0;
-                    const bool keep_min = up == lower;
-                    const uint64_t mn = v[r] < o ? v[r] : o, mx = v[r] < o ? o : v[r];
-                    v[r] = keep_min ? mn : mx;
-                }
-            }
-        }
-    }
-}
 
 // Sort one bin of 16 < len <= 64*K entries with the calling wave and write
 // it out.
@@ -756,8 +650,11 @@ __device__ __forceinline__ void gather_node(int i, int j, int k,
                 pz -= ng.off[2];
                 const float vx = gx - px, vy = gy - py, vz = gz - pz;
                 const float d2 = vx * vx + vy * vy + vz * vz;
-                const float K = wyvill(d2 * inv_r2);
-                if (HAS_VAL) S = fmaf(K, val, S);
+                const float t = d2 * inv_r2;
+                const float K = wyvill(t);
+                // only inside the radius (scalarfield.cpp:191-196): a value
+                // that is not finite must not reach the nodes beyond it
+                if (HAS_VAL) S = fmaf(K, t < 1.0f ? val : 0.0f, S);
                 else S += K;
                 W += K;
             }
@@ -861,49 +758,7 @@ void g2p(const float *pos, int64_t n, const MacGrid &g, int stencil,
     GFSX_LAUNCH_CHECK();
 }
 
-struct RkConsts {
-    float half, three_q, full, sixth, ninth;
-};
 
-// ParticleAdvector::_RK1.._RK4 (particleadvector.cpp:1045-1078) in FP32
-// vec3 arithmetic; the step sizes are the reference's FP32 casts.
-template <int ST>
-__device__ __forceinline__ void rk_step(const MacGrid &g, int order,
-                                        const RkConsts &c, const float x0[3],
-                                        const float k1[3], float x1[3]) {
-    float q[3], k2[3], k3[3], k4[3], unused[3];
-    if (order == 1) {
-#pragma unroll
-        for (int a = 0; a < 3; a++) x1[a] = x0[a] + c.full * k1[a];
-        return;
-    }
-#pragma unroll
-    for (int a = 0; a < 3; a++) q[a] = x0[a] + c.half * k1[a];
-    sample_mac<ST, false>(g, nullptr, nullptr, nullptr, q[0], q[1], q[2], k2, unused);
-    if (order == 2) {
-#pragma unroll
-        for (int a = 0; a < 3; a++) x1[a] = x0[a] + c.full * k2[a];
-        return;
-    }
-    if (order == 3) {
-#pragma unroll
-        for (int a = 0; a < 3; a++) q[a] = x0[a] + c.three_q * k2[a];
-        sample_mac<ST, false>(g, nullptr, nullptr, nullptr, q[0], q[1], q[2], k3, unused);
-#pragma unroll
-        for (int a = 0; a < 3; a++)
-            x1[a] = x0[a] + c.ninth * ((2.0f * k1[a] + 3.0f * k2[a]) + 4.0f * k3[a]);
-        return;
-    }
-#pragma unroll
-    for (int a = 0; a < 3; a++) q[a] = x0[a] + c.half * k2[a];
-    sample_mac<ST, false>(g, nullptr, nullptr, nullptr, q[0], q[1], q[2], k3, unused);
-#pragma unroll
-    for (int a = 0; a < 3; a++) q[a] = x0[a] + c.full * k3[a];
-    sample_mac<ST, false>(g, nullptr, nullptr, nullptr, q[0], q[1], q[2], k4, unused);
-#pragma unroll
-    for (int a = 0; a < 3; a++)
-        x1[a] = x0[a] + c.sixth * (((k1[a] + 2.0f * k2[a]) + 2.0f * k3[a]) + k4[a]);
-}
 
 template <int ST>
 __global__ __launch_bounds__(kBlock) void k_advect(const float *pos, int64_t n,
@@ -1063,30 +918,6 @@ __device__ __forceinline__ void face_epilogue(const StepP2G &a,
     a.grid[c][f] = val;
 }
 
-// What the epilogue of a face of component c writes and reads of the scene:
-// selected per lane from the three components' fields, which are uniform
-// (scalar loads).  Indexing the scene's arrays with a per-lane component
-// instead makes every field a dependent vector load from the scene table.
-struct FaceOut {
-    float *grid, *old, *wgt;
-    int n0, n1;          // nodes of component c along i and j
-    int nax;             // cells along axis c
-    int grav_on;
-    float grav_delta;
-};
-
-__device__ __forceinline__ FaceOut face_out(const StepP2G &a, int c) {
-    FaceOut o;
-    o.grid = c == 0 ? a.grid[0] : (c == 1 ? a.grid[1] : a.grid[2]);
-    o.old = c == 0 ? a.grid_old[0] : (c == 1 ? a.grid_old[1] : a.grid_old[2]);
-    o.wgt = c == 0 ? a.wgt[0] : (c == 1 ? a.wgt[1] : a.wgt[2]);
-    o.n0 = a.ni + (c == 0);
-    o.n1 = a.nj + (c == 1);
-    o.nax = c == 0 ? a.ni : (c == 1 ? a.nj : a.nk);
-    o.grav_on = c == 0 ? a.grav_on[0] : (c == 1 ? a.grav_on[1] : a.grav_on[2]);
-    o.grav_delta = c == 0 ? a.grav_delta[0] : (c == 1 ? a.grav_delta[1] : a.grav_delta[2]);
-    return o;
-}
 
 // face_epilogue for a brick face: whether the cells on either side hold a
 // particle comes from the brick's occupancy table (staged cells, x fastest)
@@ -1157,8 +988,9 @@ __device__ __forceinline__ void brick_gather(const float *__restrict__ sx,
         if (C != 2) pz -= offh;
         const float vx = gx - px, vy = gy - py, vz = gz - pz;
         const float d2 = vx * vx + vy * vy + vz * vz;
-        const float K = wyvill(d2 * inv_r2);
-        S = fmaf(K, pv, S);
+        const float t = d2 * inv_r2;
+        const float K = wyvill(t);
+        S = fmaf(K, t < 1.0f ? pv : 0.0f, S);   // only inside the radius
         W += K;
         q++;
     }
@@ -1189,8 +1021,9 @@ __device__ __forceinline__ void brick_gather_nested(const float *__restrict__ sx
             if (C != 2) pz -= offh;
             const float vx = gx - px, vy = gy - py, vz = gz - pz;
             const float d2 = vx * vx + vy * vy + vz * vz;
-            const float K = wyvill(d2 * inv_r2);
-            S = fmaf(K, pv, S);
+            const float t = d2 * inv_r2;
+            const float K = wyvill(t);
+            S = fmaf(K, t < 1.0f ? pv : 0.0f, S);   // only inside the radius
             W += K;
         }
     }
@@ -1237,9 +1070,11 @@ __device__ __forceinline__ void brick_gather_chunked(const float *__restrict__ s
                 if (C != 2) pz[c] -= offh;
                 const float vx = gx - px[c], vy = gy - py[c], vz = gz - pz[c];
                 const float d2 = vx * vx + vy * vy + vz * vz;
-                float K = wyvill(d2 * inv_r2);
+                const float t = d2 * inv_r2;
+                float K = wyvill(t);
                 K = valid ? K : 0.0f;
-                const float v = valid ? pv[c] : 0.0f;
+                // values only inside the radius (scalarfield.cpp:191-196)
+                const float v = valid && t < 1.0f ? pv[c] : 0.0f;
                 S = fmaf(K, v, S);
                 W += K;
             }
@@ -1797,546 +1632,7 @@ __global__ __launch_bounds__(brick::NT, 3) void k_flip_p2g_dense(
 }
 
 // ---------------------------------------------------------------------------
-// P2G by scatter (p2g_variant 4).
-//
-// k_p2g_scatter: one work-group per P2G brick takes the particles of its own
-// cells only (4^3 cells; the first and last brick of an axis also own the
-// pad cells beyond the domain).  Every particle adds K and K v to the 2x2x2
-// nodes of each component around it -- the candidates of the gather: the
-// cell and half-cell come from the same FP64 coordinates, and K the same
-// FP32 offsets (the squared distance is scaled by 1/R^2 per axis, before the
-// sum) -- in a tile of 5x6x6 nodes per component in LDS.  Node
-// n of axis a is tile entry n - 4b + (a != c): the own axis covers nodes
-// 4b..4b+4, the two others 4b-1..4b+4, which holds every in-domain node a
-// particle of the brick reaches (a pad-cell particle's nodes beyond the
-// tile lie outside the domain).  The sums are integers: K and K v are rounded to fixed
-// point, 64-bit, so they do not depend on the order of the additions: K and
-// K (v + V), V = the brick's largest |v| (so that every term is >= 0 and
-// S = S' - V W), scaled by powers of two chosen per brick so that no sum can
-// exceed 2^62, then truncated to integers.  A term keeps the 24 significant
-// bits of its float product whatever its size, so the faces at the edge
-// of the particles' reach, where every K is small, keep their relative
-// precision.  Lane l adds into copy l & 1 of the tile, which spreads the
-// atomics of the particles of one cell over two addresses.
-// Non-finite products (a velocity that is not finite, or of magnitude
-// 2^100 or more) are not added; they set flag bits on the node instead and
-// the node's value becomes what the float sum would give: NaN, +inf or -inf
-// (for |v| >= 2^100 the float sum may stay finite; such a velocity is taken
-// as infinite).
-// At the end the copies are summed, converted back to float and the tile
-// written to the brick's slot of the tile buffer.
-//
-// k_p2g_combine: one thread per face (two faces per thread) computes its
-// value by adding, in a fixed order, the entries of the at most two bricks
-// per axis whose tiles hold the face, then applies the weight and the body
-// force; faces of bricks the sort did not flag are zero without any read.
-namespace sbrick {
-constexpr int T = 6;               // tile edge, nodes, off the component's own axis
-constexpr int TN = 5 * T * T;      // nodes per component tile (5 along the own axis)
-static_assert(3 * TN == kP2gTileNodes, "tile buffer size");
-// tile index of node (tx, ty, tz) of component c: the own axis is 5 wide
-__host__ __device__ constexpr int tile_sx(int c) { return c == 0 ? 5 : 6; }
-__host__ __device__ constexpr int tile_sy(int c) { return c == 1 ? 5 : 6; }
-constexpr int NREP = 2;            // tile copies
-constexpr int NT = 256;
-constexpr int MAXROW = 36;         // (y,z) bin rows of a brick, pads included
-constexpr int PRE = 2;             // particles per thread kept in registers
-constexpr uint32_t BAD_NAN = 1u, BAD_PINF = 2u, BAD_NINF = 4u;
-}  // namespace sbrick
 
-// K of one tile corner from the per-axis squared distances over R^2.
-__device__ __forceinline__ float corner_k(float sx, float sy, float sz) {
-    return wyvill((sx + sy) + sz);
-}
-
-// x >= 0 (a float below 2^63) truncated to an integer, as 64 bits: the high
-// word is floor(x / 2^32); the low word x - high * 2^32 is exact in float
-// (below 2^32, and for x >= 2^32 it keeps at most x's 24 significant bits)
-// and its fraction is dropped by the conversion.  With the scales below
-// (2^62 / bound units for K = 1) a term loses less than one unit, far below
-// its own float rounding unless K is under about 2^-38 * bound.
-__device__ __forceinline__ unsigned long long fix64(float x) {
-    const float hi = floorf(x * 0x1p-32f);
-    const float lo = fmaf(hi, -0x1p32f, x);
-    return ((unsigned long long)(uint32_t)hi << 32) | (unsigned long long)(uint32_t)lo;
-}
-
-// Add K and K (v + V) for the 8 corners of one component of one particle.
-// (tx, ty, tz) is the tile position of corner (0,0,0); sq[a][0..1] the
-// squared distances over R^2 to the two nodes along axis a; vs = (v + V) * scale of
-// the S sums.  FAST: every corner lies in the tile and v is finite (no
-// checks).  Otherwise corners outside the tile are skipped and, with vbad,
-// the product K v is recorded in the node's flag bits instead of the sum.
-template <int C, bool FAST>
-__device__ __forceinline__ void scatter_comp(unsigned long long *tS, unsigned long long *tW,
-                                             uint32_t *sbad, int cbase, int tx, int ty, int tz,
-                                             const float sq[3][2], float vs, float sW,
-                                             bool vbad, uint32_t badbits) {
-    using namespace sbrick;
-#pragma unroll
-    for (int dz = 0; dz < 2; dz++)
-#pragma unroll
-        for (int dy = 0; dy < 2; dy++)
-#pragma unroll
-            for (int dx = 0; dx < 2; dx++) {
-                constexpr int SX = tile_sx(C), SY = tile_sy(C), SZ = C == 2 ? 5 : 6;
-                if (!FAST) {
-                    if ((unsigned)(tx + dx) >= (unsigned)SX || (unsigned)(ty + dy) >= (unsigned)SY ||
-                        (unsigned)(tz + dz) >= (unsigned)SZ)
-                        continue;
-                }
-                const int idx = (tx + dx) + SX * ((ty + dy) + SY * (tz + dz));
-                const float K = corner_k(sq[0][dx], sq[1][dy], sq[2][dz]);
-                if (FAST || !vbad) {
-                    // about half the box corners lie beyond the radius
-                    if (K > 0.0f) {
-                        atomicAdd(&tW[idx], fix64(K * sW));
-                        atomicAdd(&tS[idx], fix64(K * vs));
-                    }
-                } else {
-                    atomicAdd(&tW[idx], fix64(K * sW));
-                    // the float sum would add K * v: NaN when K is 0
-                    const int nb = cbase + idx;   // flag bits: 4 per node, 8 nodes per word
-                    atomicOr(&sbad[nb >> 3], (K == 0.0f ? BAD_NAN : badbits) << ((nb & 7) * 4));
-                }
-            }
-}
-
-__global__ __launch_bounds__(sbrick::NT) void k_p2g_scatter(
-    const SceneDesc *__restrict__ scenes, int nsc, uint32_t nbrick, int per_xcd,
-    const uint32_t *__restrict__ start, const uint8_t *__restrict__ brick_flag,
-    float2 *__restrict__ tiles, uint8_t *__restrict__ tile_flag) {
-    using namespace sbrick;
-    __shared__ unsigned long long s_S[NREP][3][TN], s_W[NREP][3][TN];
-    __shared__ uint32_t s_bad[(3 * TN + 7) / 8];
-    __shared__ uint32_t s_rowq[MAXROW], s_rows0[MAXROW + 1];
-    __shared__ float s_red[NT / 64][3];
-    __shared__ uint32_t s_cmax;
-    const uint32_t gid = (blockIdx.x & 7u) * (uint32_t)per_xcd + (blockIdx.x >> 3);
-    if (gid >= nbrick) return;
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const SceneDesc &sd = scenes[scene_of(scenes, nsc, gid, [](const SceneDesc &d) { return d.k0; })];
-    // every scene field used below, read into registers here: the loads go
-    // out in one batch together with the flag load, instead of one round
-    // trip to the scene table at each first use
-    const uint32_t sk0 = sd.k0, sb0 = sd.b0;
-    const int nbx = sd.nbx, nby = sd.nby, nbz = sd.nbz;
-    const int B0 = sd.bg.b[0], B1 = sd.bg.b[1], B2 = sd.bg.b[2];
-    const double inv_dx = sd.bg.inv_dx, dxd = sd.dx;
-    const float *xp = sd.p2g.x, *yp = sd.p2g.y, *zp = sd.p2g.z;
-    const float *vp0 = sd.p2g.vel[0], *vp1 = sd.p2g.vel[1], *vp2 = sd.p2g.vel[2];
-    const float half = sd.p2g.ng[0].off[1];
-    const uint8_t flag = gld(brick_flag, gid);
-    asm volatile("" ::"s"(sk0), "s"(sb0), "s"(nbx), "s"(nby), "s"(nbz), "s"(B0), "s"(B1), "s"(B2));
-    asm volatile("" ::"s"(inv_dx), "s"(dxd), "s"(xp), "s"(yp), "s"(zp), "s"(vp0), "s"(vp1),
-                 "s"(vp2), "s"(half));
-    if (flag == 0) {
-        // no particle within reach of this brick, so none in its cells
-        if (t == 0) tile_flag[gid] = 0;
-        return;
-    }
-    const int lb = (int)(gid - sk0);
-    const int bxi = lb % nbx, byi = (lb / nbx) % nby, bzi = lb / (nbx * nby);
-    const int i0 = bxi * 4, j0 = byi * 4, k0 = bzi * 4;
-    // bins of the brick's own cells (bin = cell + 1), pads at the ends
-    const int bx_lo = bxi == 0 ? 0 : i0 + 1, bx_hi = bxi == nbx - 1 ? B0 : i0 + 5;
-    const int by_lo = byi == 0 ? 0 : j0 + 1, by_hi = byi == nby - 1 ? B1 : j0 + 5;
-    const int bz_lo = bzi == 0 ? 0 : k0 + 1, bz_hi = bzi == nbz - 1 ? B2 : k0 + 5;
-    const int ny = by_hi - by_lo, nrow = ny * (bz_hi - bz_lo);
-    const uint32_t *st0 = start + sb0;
-    if (wave == 0) {
-        uint32_t q0 = 0, len = 0;
-        if (lane < nrow) {
-            const uint32_t by = (uint32_t)(by_lo + lane % ny), bz = (uint32_t)(bz_lo + lane / ny);
-            const uint32_t row = (uint32_t)B0 * (by + (uint32_t)B1 * bz);
-            q0 = gld(st0, row + (uint32_t)bx_lo);
-            len = gld(st0, row + (uint32_t)bx_hi) - q0;
-            s_rowq[lane] = q0;
-        }
-        const uint32_t inc = wave_inclusive_sum(len);
-        if (lane < nrow) s_rows0[lane + 1] = inc;
-        if (lane == 0) s_rows0[0] = 0u;
-    }
-    // particles of this thread's cell (one thread per cell), for the
-    // largest cell count
-    const int ncx = bx_hi - bx_lo;
-    uint32_t ccount = 0;
-    if (t < nrow * ncx) {
-        const int r = t / ncx, k = t - r * ncx;
-        const uint32_t by = (uint32_t)(by_lo + r % ny), bz = (uint32_t)(bz_lo + r / ny);
-        const uint32_t cell = (uint32_t)B0 * (by + (uint32_t)B1 * bz) + (uint32_t)(bx_lo + k);
-        ccount = gld(st0, cell + 1) - gld(st0, cell);
-    }
-    if (t == 0) s_cmax = 0u;
-    {
-        unsigned long long *z = &s_S[0][0][0];
-        for (int e = t; e < NREP * 3 * TN; e += NT) z[e] = 0ull;
-        z = &s_W[0][0][0];
-        for (int e = t; e < NREP * 3 * TN; e += NT) z[e] = 0ull;
-        for (int e = t; e < (3 * TN + 7) / 8; e += NT) s_bad[e] = 0u;
-    }
-    __syncthreads();
-    const uint32_t N = s_rows0[nrow];
-    if (N == 0) {
-        if (t == 0) tile_flag[gid] = 0;
-        return;
-    }
-    if (ccount > 0u) atomicMax(&s_cmax, ccount);
-    auto slot_of = [&](uint32_t s) {
-        int lo = 0, hi = nrow - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (s_rows0[mid] <= s) lo = mid;
-            else hi = mid - 1;
-        }
-        return s_rowq[lo] + (s - s_rows0[lo]);
-    };
-
-    // pass 1: the largest finite |v| per component (velocities of the first
-    // PRE particles of each thread, and their positions, stay in registers)
-    float px[PRE], py[PRE], pz[PRE], pv[PRE][3];
-    float vm[3] = {0.0f, 0.0f, 0.0f};
-    constexpr float kBig = 1.2676506e30f;   // 2^100
-#pragma unroll
-    for (int m = 0; m < PRE; m++) {
-        const uint32_t s = (uint32_t)t + (uint32_t)(m * NT);
-        const uint32_t q = s < N ? slot_of(s) : s_rowq[0];
-        px[m] = gld(xp, q);
-        py[m] = gld(yp, q);
-        pz[m] = gld(zp, q);
-        pv[m][0] = gld(vp0, q);
-        pv[m][1] = gld(vp1, q);
-        pv[m][2] = gld(vp2, q);
-    }
-#pragma unroll
-    for (int m = 0; m < PRE; m++) {
-        if ((uint32_t)t + (uint32_t)(m * NT) < N) {
-#pragma unroll
-            for (int c = 0; c < 3; c++) {
-                const float av = fabsf(pv[m][c]);
-                if (av < kBig) vm[c] = fmaxf(vm[c], av);
-            }
-        }
-    }
-    for (uint32_t s = (uint32_t)t + (uint32_t)(PRE * NT); s < N; s += NT) {
-        const uint32_t q = slot_of(s);
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-            const float av = fabsf(gld(c == 0 ? vp0 : (c == 1 ? vp1 : vp2), q));
-            if (av < kBig) vm[c] = fmaxf(vm[c], av);
-        }
-    }
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-        float v = vm[c];
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
-        if (lane == 0) s_red[wave][c] = v;
-    }
-    __syncthreads();
-    // scales.  A node takes particles of at most 2x3x3 cells (its
-    // neighbours within dx), so at most bound = min(N, 18 * largest cell
-    // count) terms, each K <= 1 and K (v + V) <= 2V: bound * scale <= 2^62
-    // for W, bound * 2V * scale <= 2^62 for S'.
-    const double bound = (double)min(N, 18u * s_cmax);
-    const float sW = ldexpf(1.0f, ilogb(4611686018427387904.0 / bound));
-    float sS[3], vbias[3];
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-        float v = 0.0f;
-#pragma unroll
-        for (int w = 0; w < NT / 64; w++) v = fmaxf(v, s_red[w][c]);
-        vbias[c] = v;
-        const int e = v > 0.0f ? ilogb(2305843009213693952.0 / (bound * (double)v)) : 0;
-        sS[c] = ldexpf(1.0f, e);
-    }
-
-    // pass 2: scatter
-    const float inv_r2 = (float)(1.0 / (dxd * dxd));
-    const int rep = lane & (NREP - 1);
-    const int ob[3] = {i0, j0, k0};
-    auto scatter = [&](float x, float y, float z, const float v[3]) {
-        const float p[3] = {x, y, z};
-        int ia[3], ib[3];
-        float sq[2][3][2];   // [own?0:1][axis][node]
-#pragma unroll
-        for (int ax = 0; ax < 3; ax++) {
-            const AxisCoords cc = axis_coords(p[ax], inv_dx);
-            ia[ax] = cc.ia;
-            ib[ax] = cc.ib;
-            const float ga0 = (float)((float)cc.ia * dxd), ga1 = (float)((float)(cc.ia + 1) * dxd);
-            const float gb0 = (float)((float)cc.ib * dxd), gb1 = (float)((float)(cc.ib + 1) * dxd);
-            const float ph = p[ax] - half;   // p -= offset (scalarfield.cpp:168)
-            const float d0 = ga0 - p[ax], d1 = ga1 - p[ax];
-            const float e0 = gb0 - ph, e1 = gb1 - ph;
-            sq[0][ax][0] = d0 * d0 * inv_r2;
-            sq[0][ax][1] = d1 * d1 * inv_r2;
-            sq[1][ax][0] = e0 * e0 * inv_r2;
-            sq[1][ax][1] = e1 * e1 * inv_r2;
-        }
-        // corner (0,0,0) of component c in tile coordinates
-        int tc[3][3];
-        bool fast = true;
-#pragma unroll
-        for (int c = 0; c < 3; c++)
-#pragma unroll
-            for (int ax = 0; ax < 3; ax++) {
-                tc[c][ax] = (ax == c ? ia[ax] : ib[ax]) - ob[ax] + (ax == c ? 0 : 1);
-                fast = fast && (unsigned)tc[c][ax] <= (ax == c ? 3u : 4u);
-            }
-        bool vbad[3];
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-            vbad[c] = !(fabsf(v[c]) < kBig);
-            fast = fast && !vbad[c];
-        }
-        // the common case (a particle of the brick's own cells with a finite
-        // velocity) without checks; pad cells and non-finite velocities
-        // take the checked form
-        auto comp = [&](auto cc, auto fastc) {
-            constexpr int c = decltype(cc)::value;
-            constexpr bool F = decltype(fastc)::value;
-            float q[3][2];
-#pragma unroll
-            for (int ax = 0; ax < 3; ax++) {
-                q[ax][0] = sq[ax == c ? 0 : 1][ax][0];
-                q[ax][1] = sq[ax == c ? 0 : 1][ax][1];
-            }
-            const uint32_t bits = v[c] != v[c] ? BAD_NAN : (v[c] > 0.0f ? BAD_PINF : BAD_NINF);
-            const float vs = vbad[c] ? 0.0f : (v[c] + vbias[c]) * sS[c];
-            scatter_comp<c, F>(&s_S[rep][c][0], &s_W[rep][c][0], s_bad, c * TN, tc[c][0],
-                               tc[c][1], tc[c][2], q, vs, sW, vbad[c], bits);
-        };
-        using I0 = std::integral_constant<int, 0>;
-        using I1 = std::integral_constant<int, 1>;
-        using I2 = std::integral_constant<int, 2>;
-        if (fast) {
-            comp(I0(), std::true_type());
-            comp(I1(), std::true_type());
-            comp(I2(), std::true_type());
-        } else {
-            comp(I0(), std::false_type());
-            comp(I1(), std::false_type());
-            comp(I2(), std::false_type());
-        }
-    };
-#pragma unroll
-    for (int m = 0; m < PRE; m++)
-        if ((uint32_t)t + (uint32_t)(m * NT) < N) scatter(px[m], py[m], pz[m], pv[m]);
-    for (uint32_t s = (uint32_t)t + (uint32_t)(PRE * NT); s < N; s += NT) {
-        const uint32_t q = slot_of(s);
-        const float v[3] = {gld(vp0, q), gld(vp1, q), gld(vp2, q)};
-        scatter(gld(xp, q), gld(yp, q), gld(zp, q), v);
-    }
-    __syncthreads();
-
-    // sum the copies, back to float, out to the tile buffer
-    float2 *out = tiles + (size_t)gid * 3 * TN;
-    for (int e = t; e < 3 * TN; e += NT) {
-        const int c = e / TN, n = e - c * TN;
-        unsigned long long S = 0, W = 0;
-#pragma unroll
-        for (int r = 0; r < NREP; r++) {
-            S += s_S[r][c][n];
-            W += s_W[r][c][n];
-        }
-        // S = S' - V W
-        const double Wd = (double)W / (double)sW;
-        float Sf = (float)((double)S / (double)sS[c] - (double)vbias[c] * Wd);
-        const float Wf = (float)Wd;
-        const uint32_t bad = (s_bad[e >> 3] >> ((e & 7) * 4)) & 0xFu;
-        if (bad) {
-            if ((bad & BAD_NAN) || ((bad & BAD_PINF) && (bad & BAD_NINF))) Sf = __builtin_nanf("");
-            else Sf = (bad & BAD_PINF) ? INFINITY : -INFINITY;
-        }
-        out[e] = make_float2(Sf, Wf);
-    }
-    if (t == 0) tile_flag[gid] = 1;
-}
-
-// face_epilogue with the component's fields selected per lane.
-__device__ __forceinline__ void face_epilogue_fo(const FaceOut &o, const BinGrid &bg,
-                                                 const uint32_t *__restrict__ count, int c,
-                                                 int i, int j, int k, float S, float W) {
-    const int64_t f = (int64_t)i + (int64_t)o.n0 * ((int64_t)j + (int64_t)o.n1 * k);
-    float val = W > 0.0f ? S / W : S;
-    if (o.old) o.old[f] = val;
-    if (o.wgt) o.wgt[f] = W;
-    if (o.grav_on) {
-        const int ax = c == 0 ? i : (c == 1 ? j : k);
-        bool fluid = false;
-        const uint32_t bhi = (uint32_t)(i + 1) + (uint32_t)bg.b[0] *
-                             ((uint32_t)(j + 1) + (uint32_t)bg.b[1] * (uint32_t)(k + 1));
-        const uint32_t step = c == 0 ? 1u : (c == 1 ? (uint32_t)bg.b[0]
-                                                    : (uint32_t)bg.b[0] * (uint32_t)bg.b[1]);
-        if (ax < o.nax) fluid = fluid || gld(count, bhi) > 0u;
-        if (ax > 0) fluid = fluid || gld(count, bhi - step) > 0u;
-        if (fluid) val += o.grav_delta;
-    }
-    o.grid[f] = val;
-}
-
-// The value of global face gf (see k_p2g_combine).
-__device__ __forceinline__ void combine_face(
-    const SceneDesc *__restrict__ scenes, int nsc, int64_t gf,
-    const uint32_t *__restrict__ count, const uint8_t *__restrict__ brick_flag,
-    const float2 *__restrict__ tiles, const uint8_t *__restrict__ tile_flag) {
-    using namespace sbrick;
-    const SceneDesc &sd = scenes[scene_of(scenes, nsc, (uint64_t)gf, [](const SceneDesc &d) { return (uint64_t)d.f0; })];
-    const StepP2G &a = sd.p2g;
-    int64_t lf = gf - sd.f0;
-    int c = 0;
-    if (lf >= a.nface[0]) {
-        lf -= a.nface[0];
-        c = 1;
-        if (lf >= a.nface[1]) {
-            lf -= a.nface[1];
-            c = 2;
-        }
-    }
-    // (a scene's face count of one component fits in 32 bits)
-    const uint32_t w = (uint32_t)(a.ni + (c == 0)), h = (uint32_t)(a.nj + (c == 1));
-    const uint32_t lf32 = (uint32_t)lf, row = lf32 / w;
-    const int I = (int)(lf32 - row * w), J = (int)(row % h), K = (int)(row / h);
-    // The brick that owns the face (the gathers' assignment) has no particle
-    // within reach of any of its faces unless the sort flagged it: then every
-    // tile entry of the face is 0.  Otherwise the tile flags and entries are
-    // loaded together; entries of a brick without a tile hold stale data and
-    // are selected away.
-    const uint32_t own = sd.k0 + (uint32_t)(min(I >> 2, sd.nbx - 1) +
-                                            sd.nbx * (min(J >> 2, sd.nby - 1) +
-                                                      sd.nby * min(K >> 2, sd.nbz - 1)));
-    float S = 0.0f, W = 0.0f;
-    if (gld(brick_flag, own) != 0) {
-        const int nb[3] = {sd.nbx, sd.nby, sd.nbz};
-        const int n[3] = {I, J, K};
-        // per axis the bricks (lower one first) whose tiles hold node n: the
-        // tile of brick b covers 4b - o .. 4b - o + 4 + o (o = 1 off the own axis)
-        int bl[3][2], tl[3][2], cnt[3];
-#pragma unroll
-        for (int ax = 0; ax < 3; ax++) {
-            const int o = ax == c ? 0 : 1;
-            const int m = n[ax] + o, b1 = m >> 2, t1 = m & 3;
-            const bool lower = (ax == c ? t1 == 0 : t1 <= 1) && b1 >= 1 && b1 - 1 < nb[ax];
-            const bool upper = b1 < nb[ax];
-            bl[ax][0] = lower ? b1 - 1 : b1;
-            tl[ax][0] = lower ? t1 + 4 : t1;
-            bl[ax][1] = b1;
-            tl[ax][1] = t1;
-            cnt[ax] = lower ? (upper ? 2 : 1) : (upper ? 1 : 0);
-        }
-        const float *tf = reinterpret_cast<const float *>(tiles);
-        uint32_t fl[8];
-        float ps[8], pw[8];
-#pragma unroll
-        for (int e = 0; e < 8; e++) {
-            const int ix = e & 1, iy = (e >> 1) & 1, iz = e >> 2;
-            fl[e] = 0;
-            ps[e] = pw[e] = 0.0f;
-            if (iz < cnt[2] && iy < cnt[1] && ix < cnt[0]) {
-                const uint32_t bid = sd.k0 + (uint32_t)(bl[0][ix] + sd.nbx * (bl[1][iy] + sd.nby * bl[2][iz]));
-                const size_t q = ((size_t)bid * 3 + (size_t)c) * TN +
-                                 (size_t)(tl[0][ix] + tile_sx(c) * (tl[1][iy] + tile_sy(c) * tl[2][iz]));
-                fl[e] = gld(tile_flag, bid);
-                ps[e] = gld(tf, 2 * q);
-                pw[e] = gld(tf, 2 * q + 1);
-            }
-        }
-#pragma unroll
-        for (int e = 0; e < 8; e++) {
-            S += fl[e] ? ps[e] : 0.0f;
-            W += fl[e] ? pw[e] : 0.0f;
-        }
-    }
-    face_epilogue_fo(face_out(a, c), sd.bg, count + sd.b0, c, I, J, K, S, W);
-}
-
-// Two faces per thread, NT apart, faces in array order (i fastest), so
-// each thread has two independent chains of loads in flight; each XCD takes
-// a contiguous range of faces, so the tiles its faces read stay in its L2.
-constexpr int kCombineFaces = 2;
-
-__global__ __launch_bounds__(sbrick::NT) void k_p2g_combine(
-    const SceneDesc *__restrict__ scenes, int nsc, int64_t nface, int64_t per_xcd,
-    const uint32_t *__restrict__ count, const uint8_t *__restrict__ brick_flag,
-    const float2 *__restrict__ tiles, const uint8_t *__restrict__ tile_flag) {
-    const int64_t gb = (int64_t)(blockIdx.x & 7u) * per_xcd + (int64_t)(blockIdx.x >> 3);
-#pragma unroll
-    for (int r = 0; r < kCombineFaces; r++) {
-        const int64_t gf = (gb * kCombineFaces + r) * sbrick::NT + threadIdx.x;
-        if (gf < nface) combine_face(scenes, nsc, gf, count, brick_flag, tiles, tile_flag);
-    }
-}
-
-struct StepG2P {
-    const float *xin[3], *vin[3];   // in sort order (all scenes)
-    float *xout[3], *vout[3];       // the scene's outputs
-    const uint32_t *id_in;
-    uint32_t *id_out;
-    const float *old[3];
-    RkConsts rk;
-    float r_pic, r_flip;
-    int keep_order;
-    int64_t n;
-    uint32_t p0;                    // the scene's first sort slot and particle index
-};
-
-__host__ __device__ inline StepG2P g2p_view(const SceneDesc &sd, const Sorted &so, int keep_order) {
-    StepG2P b;
-    for (int c = 0; c < 3; c++) {
-        b.xin[c] = so.x[c];
-        b.vin[c] = so.v[c];
-        b.xout[c] = sd.xout[c];
-        b.vout[c] = sd.vout[c];
-        b.old[c] = sd.grid_old[c];
-    }
-    b.id_in = sd.id_in;
-    b.id_out = sd.id_out;
-    b.rk.half = sd.h_half;
-    b.rk.three_q = 0.0f;
-    b.rk.full = sd.h_full;
-    b.rk.sixth = sd.h_sixth;
-    b.rk.ninth = 0.0f;
-    b.r_pic = sd.r_pic;
-    b.r_flip = sd.r_flip;
-    b.keep_order = keep_order;
-    b.n = sd.n;
-    b.p0 = (uint32_t)sd.p0;
-    return b;
-}
-
-__host__ __device__ inline MacGrid mac_view(const SceneDesc &sd) {
-    MacGrid g;
-    for (int c = 0; c < 3; c++) {
-        g.f[c] = sd.grid[c];
-        g.lim[c] = sd.lim[c];
-        g.limf[c] = sd.limf[c];
-    }
-    g.ni = sd.ni;
-    g.nj = sd.nj;
-    g.nk = sd.nk;
-    g.dx = sd.dx;
-    g.inv_dx = 1.0 / sd.dx;
-    return g;
-}
-
-// Write the new state of the particle in sort slot q: at q (sorted order)
-// or at its original index perm[q] (kept order), with its id, all relative
-// to the scene's first particle.
-__device__ __forceinline__ void store_particle(const StepG2P &a,
-                                               const uint32_t *__restrict__ perm,
-                                               uint32_t q, const float x1[3],
-                                               const float vnew[3]) {
-    uint32_t o = q - a.p0, p = o;
-    if (a.keep_order || a.id_out) p = perm[q] - a.p0;
-    if (a.keep_order) o = p;
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-        a.xout[c][o] = x1[c];
-        a.vout[c][o] = vnew[c];
-    }
-    if (a.id_out) a.id_out[o] = a.id_in ? a.id_in[p] : p;
-}
 
 // Positions and velocities copied into sorted order, so that the transfer
 // kernels read them by sort slot with no indirection.  Slot q belongs to the
@@ -2398,149 +1694,6 @@ __global__ __launch_bounds__(kBlock) void k_flip_g2p(
 // the box reads global memory instead, so every position is handled and
 // the result is bit-identical to k_flip_g2p.
 // ---------------------------------------------------------------------------
-namespace gbrick {
-constexpr int GB = 8;        // cells per brick edge
-constexpr int GH = 1;        // halo cells for RK stage samples
-constexpr int NT = 256;      // threads per work-group
-constexpr int MAXROW = (GB + 2) * (GB + 2);   // (y,z) bin rows, pads included
-constexpr int NEXTRA = 64;   // work-groups for particles outside the bin grid
-template <int ST> struct Box {
-    static constexpr int S = ST;                        // stencil reach below the cell: 0 or 1
-    static constexpr int E = GB + 2 + 2 * GH + 2 * S;   // new-field box edge
-    static constexpr int LO = 1 + GH + S;               // new-field box origin: C0 - LO
-    static constexpr int EO = GB + 2 + 2 * S;           // snapshot box edge
-    static constexpr int LOO = 1 + S;                   // snapshot box origin: C0 - LOO
-};
-static_assert(MAXROW <= 2 * 64, "two waves scan the rows");
-}  // namespace gbrick
-
-// Box sample of one component: (i,j,k) is the stencil's base cell relative
-// to the box origin; the stencil (i-S..i+1+S per axis) lies inside the box.
-template <int ST, int E>
-__device__ __forceinline__ float box_comp(const float *__restrict__ box, int i,
-                                          int j, int k, float tx, float ty,
-                                          float tz) {
-    if (ST == 0) {
-        const float *b = box + (i + E * (j + E * k));
-        const float v000 = b[0], v100 = b[1], v010 = b[E], v110 = b[E + 1];
-        const float v001 = b[E * E], v101 = b[E * E + 1];
-        const float v011 = b[E * E + E], v111 = b[E * E + E + 1];
-        const float c00 = fmaf(tx, v100 - v000, v000);
-        const float c10 = fmaf(tx, v110 - v010, v010);
-        const float c01 = fmaf(tx, v101 - v001, v001);
-        const float c11 = fmaf(tx, v111 - v011, v011);
-        const float c0 = fmaf(ty, c10 - c00, c00);
-        const float c1 = fmaf(ty, c11 - c01, c01);
-        return fmaf(tz, c1 - c0, c0);
-    }
-    const float *b = box + ((i - 1) + E * ((j - 1) + E * (k - 1)));
-    float zs[4];
-#pragma unroll
-    for (int mk = 0; mk < 4; mk++) {
-        float ys[4];
-#pragma unroll
-        for (int mj = 0; mj < 4; mj++) {
-            const float *r = b + E * (mj + E * mk);
-            ys[mj] = cubic(r[0], r[1], r[2], r[3], tx);
-        }
-        zs[mk] = cubic(ys[0], ys[1], ys[2], ys[3], ty);
-    }
-    return cubic(zs[0], zs[1], zs[2], zs[3], tz);
-}
-
-// True when the stencils of all three components at cell coordinates
-// (cx,cy,cz) lie in the box of edge E whose origin is at cell lo[].  Per
-// axis the components use base cells ia (own axis) and ib <= ia (others),
-// so the lowest cell read is ib - S and the highest ia + 1 + S.
-template <int ST, int E>
-__device__ __forceinline__ bool in_box(const AxisCoords &cx, const AxisCoords &cy,
-                                       const AxisCoords &cz, int lx, int ly, int lz) {
-    constexpr int S = ST;
-    return (unsigned)(cx.ib - S - lx) < (unsigned)E && (unsigned)(cx.ia + 1 + S - lx) < (unsigned)E &&
-           (unsigned)(cy.ib - S - ly) < (unsigned)E && (unsigned)(cy.ia + 1 + S - ly) < (unsigned)E &&
-           (unsigned)(cz.ib - S - lz) < (unsigned)E && (unsigned)(cz.ia + 1 + S - lz) < (unsigned)E;
-}
-
-// sample_mac with the LDS boxes as the first choice.  cur[c] / old[c] are
-// the staged boxes (edges E, EO; origins l*, lo*); g and o* are the global
-// arrays used when a stencil leaves its box.
-template <int ST, bool OLD>
-__device__ __forceinline__ void brick_sample(const MacGrid &g, const float *o0,
-                                             const float *o1, const float *o2,
-                                             const float *__restrict__ cur0,
-                                             const float *__restrict__ cur1,
-                                             const float *__restrict__ cur2,
-                                             const float *__restrict__ old0,
-                                             const float *__restrict__ old1,
-                                             const float *__restrict__ old2,
-                                             int lx, int ly, int lz,
-                                             int lox, int loy, int loz,
-                                             float x, float y, float z,
-                                             float out[3], float out_old[3]) {
-    using B = gbrick::Box<ST>;
-    if (!(x >= 0.0f && y >= 0.0f && z >= 0.0f && x < g.limf[0] && y < g.limf[1] &&
-          z < g.limf[2])) {
-        out[0] = out[1] = out[2] = 0.0f;
-        if (OLD) out_old[0] = out_old[1] = out_old[2] = 0.0f;
-        return;
-    }
-    const AxisCoords cx = axis_coords(x, g.inv_dx);
-    const AxisCoords cy = axis_coords(y, g.inv_dx);
-    const AxisCoords cz = axis_coords(z, g.inv_dx);
-    const int ni = g.ni, nj = g.nj, nk = g.nk;
-    if (in_box<ST, B::E>(cx, cy, cz, lx, ly, lz)) {
-        out[0] = box_comp<ST, B::E>(cur0, cx.ia - lx, cy.ib - ly, cz.ib - lz, cx.ta, cy.tb, cz.tb);
-        out[1] = box_comp<ST, B::E>(cur1, cx.ib - lx, cy.ia - ly, cz.ib - lz, cx.tb, cy.ta, cz.tb);
-        out[2] = box_comp<ST, B::E>(cur2, cx.ib - lx, cy.ib - ly, cz.ia - lz, cx.tb, cy.tb, cz.ta);
-    } else {
-        out[0] = sample_comp<ST>(g.f[0], ni + 1, nj, nk, cx.ia, cy.ib, cz.ib, cx.ta, cy.tb, cz.tb);
-        out[1] = sample_comp<ST>(g.f[1], ni, nj + 1, nk, cx.ib, cy.ia, cz.ib, cx.tb, cy.ta, cz.tb);
-        out[2] = sample_comp<ST>(g.f[2], ni, nj, nk + 1, cx.ib, cy.ib, cz.ia, cx.tb, cy.tb, cz.ta);
-    }
-    if (OLD) {
-        if (in_box<ST, B::EO>(cx, cy, cz, lox, loy, loz)) {
-            out_old[0] = box_comp<ST, B::EO>(old0, cx.ia - lox, cy.ib - loy, cz.ib - loz, cx.ta, cy.tb, cz.tb);
-                out_old[1] = box_comp<ST, B::EO>(old1, cx.ib - lox, cy.ia - loy, cz.ib - loz, cx.tb, cy.ta, cz.tb);
-                out_old[2] = box_comp<ST, B::EO>(old2, cx.ib - lox, cy.ib - loy, cz.ia - loz, cx.tb, cy.tb, cz.ta);
-        } else {
-            out_old[0] = sample_comp<ST>(o0, ni + 1, nj, nk, cx.ia, cy.ib, cz.ib, cx.ta, cy.tb, cz.tb);
-            out_old[1] = sample_comp<ST>(o1, ni, nj + 1, nk, cx.ib, cy.ia, cz.ib, cx.tb, cy.ta, cz.tb);
-            out_old[2] = sample_comp<ST>(o2, ni, nj, nk + 1, cx.ib, cy.ib, cz.ia, cx.tb, cy.tb, cz.ta);
-        }
-    }
-}
-
-// Copy the box of edge E with origin (ox,oy,oz) of a component array with
-// dims (w,h,d) into LDS; cells outside the array are 0.
-template <int E>
-__device__ __forceinline__ void stage_box(float *__restrict__ box,
-                                          const float *__restrict__ f, int w,
-                                          int h, int d, int ox, int oy, int oz) {
-    for (int e = threadIdx.x; e < E * E * E; e += gbrick::NT) {
-        const int ii = e % E, jj = (e / E) % E, kk = e / (E * E);
-        const int gi = ox + ii, gj = oy + jj, gk = oz + kk;
-        float v = 0.0f;
-        if ((unsigned)gi < (unsigned)w && (unsigned)gj < (unsigned)h && (unsigned)gk < (unsigned)d)
-            v = f[(int64_t)gi + (int64_t)w * ((int64_t)gj + (int64_t)h * gk)];
-        box[e] = v;
-    }
-}
-
-// PIC/FLIP blend and RK4 for one particle, given its new-field and
-// snapshot velocities at the start position.
-__device__ __forceinline__ void flip_blend(const StepG2P &a, const float v0[3],
-                                           const float k1[3], const float vo_in[3],
-                                           float vnew[3]) {
-    float vn[3] = {k1[0], k1[1], k1[2]};
-    float vo[3] = {vo_in[0], vo_in[1], vo_in[2]};
-    validate3(vn);
-    validate3(vo);
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-        const float vflip = (v0[c] + vn[c]) - vo[c];
-        vnew[c] = a.r_pic * vn[c] + a.r_flip * vflip;
-    }
-}
 
 // The brick body of k_flip_g2p_brick for brick brick_id of one scene
 // (start: the scene's bin starts, local bin indices).
@@ -2742,7 +1895,10 @@ void layout_scenes(SceneDesc *sc, int nsc, SceneTotals &tot) {
         d.gby = (d.nj + gbrick::GB - 1) / gbrick::GB;
         d.gbz = (d.nk + gbrick::GB - 1) / gbrick::GB;
         d.g0 = (uint32_t)tot.ng2p;
+        d.nbb = 8u * (uint32_t)(d.gbx * d.gby * d.gbz);
+        d.kb0 = (uint32_t)tot.nbb;
         d.f0 = tot.nface;
+        d.r0 = tot.nrow;
         MacGrid g;
         g.ni = d.ni;
         g.nj = d.nj;
@@ -2758,47 +1914,57 @@ void layout_scenes(SceneDesc *sc, int nsc, SceneTotals &tot) {
         tot.nblk += sort_blocks(d.n);
         tot.np2g += (int64_t)d.nbx * d.nby * d.nbz;
         tot.ng2p += (int64_t)d.gbx * d.gby * d.gbz;
+        tot.nbb += (int64_t)d.nbb + 1;
+        tot.nrow += (int64_t)d.nj * d.nk + (int64_t)(d.nj + 1) * d.nk + (int64_t)d.nj * (d.nk + 1);
         tot.nface += (int64_t)(d.ni + 1) * d.nj * d.nk + (int64_t)d.ni * (d.nj + 1) * d.nk +
                      (int64_t)d.ni * d.nj * (d.nk + 1);
     }
 }
 
-void flip_step(const StepBatch &s, const SortBuffers &sb, hipStream_t st, hipEvent_t *ev) {
-    if (ev) GFSX_HIP(hipEventRecord(ev[0], st));
-    const int nsc = (int)s.scenes.size();
-    // cells ordered inside by index; for the P2G gathers by (octant, index),
-    // so that each half-cell is a run.  The brick flags are set by the sort
-    // for this step's particles only.
-    fill_u8(sb.brick_flag, 0, s.tot.np2g, st);
-    cell_sort(sb, s.tot.n, st, s.p2g_variant != 4, true);
+void sorted_copy(const SceneDesc *scenes_dev, int nsc, const uint32_t *perm, int64_t n,
+                 float *const xs[3], float *const vs[3], hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_sorted_copy, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, s, scenes_dev,
+                       nsc, perm, n, xs[0], xs[1], xs[2], vs[0], vs[1], vs[2]);
+    GFSX_LAUNCH_CHECK();
+}
+
+void flip_g2p_direct(const StepBatch &s, const uint32_t *perm, hipStream_t st) {
+    // one scene only
+    const SceneDesc &d = s.scenes[0];
     Sorted so;
     for (int c = 0; c < 3; c++) {
         so.x[c] = s.xs[c];
         so.v[c] = s.vs[c];
     }
-    if (s.tot.n > 0) {
-        hipLaunchKernelGGL(k_sorted_copy, dim3(grid_for(s.tot.n, kBlock)), dim3(kBlock), 0, st,
-                           s.scenes_dev, nsc, (const uint32_t *)sb.perm, s.tot.n, s.xs[0],
-                           s.xs[1], s.xs[2], s.vs[0], s.vs[1], s.vs[2]);
-        GFSX_LAUNCH_CHECK();
+    const StepG2P b = g2p_view(d, so, s.keep_order);
+    const MacGrid g = mac_view(d);
+    if (s.stencil == 0)
+        hipLaunchKernelGGL(k_flip_g2p<0>, dim3(grid_for(d.n, kBlock)), dim3(kBlock), 0, st, b, g,
+                           perm);
+    else
+        hipLaunchKernelGGL(k_flip_g2p<1>, dim3(grid_for(d.n, kBlock)), dim3(kBlock), 0, st, b, g,
+                           perm);
+    GFSX_LAUNCH_CHECK();
+}
+
+void flip_step(const StepBatch &s, const SortBuffers &sb, hipStream_t st, hipEvent_t *ev) {
+    if (ev) GFSX_HIP(hipEventRecord(ev[0], st));
+    const int nsc = (int)s.scenes.size();
+    // cells ordered inside by (octant, index), so that each half-cell is a
+    // run for the P2G gathers.  The brick flags are set by the sort for this
+    // step's particles only.
+    fill_u8(sb.brick_flag, 0, s.tot.np2g, st);
+    cell_sort(sb, s.tot.n, st, true, true);
+    Sorted so;
+    for (int c = 0; c < 3; c++) {
+        so.x[c] = s.xs[c];
+        so.v[c] = s.vs[c];
     }
+    sorted_copy(s.scenes_dev, nsc, (const uint32_t *)sb.perm, s.tot.n, s.xs, s.vs, st);
     if (ev) GFSX_HIP(hipEventRecord(ev[1], st));
 
-    if (s.p2g_variant == 4) {
-        const uint32_t nbrick = (uint32_t)s.tot.np2g;
-        const int per_xcd = (int)((nbrick + 7) / 8);
-        const dim3 grid((unsigned)(8 * (int64_t)per_xcd)), block(sbrick::NT);
-        hipLaunchKernelGGL(k_p2g_scatter, grid, block, 0, st, s.scenes_dev, nsc, nbrick, per_xcd,
-                           (const uint32_t *)sb.start, (const uint8_t *)sb.brick_flag, s.tiles,
-                           s.tile_flag);
-        GFSX_LAUNCH_CHECK();
-        const int64_t nfb = (s.tot.nface + kCombineFaces * sbrick::NT - 1) / (kCombineFaces * sbrick::NT);
-        const int64_t fper = (nfb + 7) / 8;
-        hipLaunchKernelGGL(k_p2g_combine, dim3((unsigned)(8 * fper)), block, 0, st, s.scenes_dev,
-                           nsc, s.tot.nface, fper, (const uint32_t *)sb.count,
-                           (const uint8_t *)sb.brick_flag, (const float2 *)s.tiles,
-                           (const uint8_t *)s.tile_flag);
-    } else if (s.p2g_variant != 1) {
+    if (s.p2g_variant != 1) {
         const uint32_t nbrick = (uint32_t)s.tot.np2g;
         const int per_xcd = (int)((nbrick + 7) / 8);
         constexpr int cap = brick::CAP;
@@ -2843,19 +2009,10 @@ void flip_step(const StepBatch &s, const SortBuffers &sb, hipStream_t st, hipEve
         auto *kern = s.stencil == 0 ? k_flip_g2p_brick_tl : k_flip_g2p_brick_tc;
         hipLaunchKernelGGL(kern, grid, block, 0, st, s.scenes_dev, nsc, so, s.keep_order, nbrick,
                            per_xcd, (const uint32_t *)sb.start, (const uint32_t *)sb.perm);
+        GFSX_LAUNCH_CHECK();
     } else {
-        // the direct kernel: one scene only
-        const SceneDesc &d = s.scenes[0];
-        const StepG2P b = g2p_view(d, so, s.keep_order);
-        const MacGrid g = mac_view(d);
-        if (s.stencil == 0)
-            hipLaunchKernelGGL(k_flip_g2p<0>, dim3(grid_for(d.n, kBlock)), dim3(kBlock), 0, st, b,
-                               g, (const uint32_t *)sb.perm);
-        else
-            hipLaunchKernelGGL(k_flip_g2p<1>, dim3(grid_for(d.n, kBlock)), dim3(kBlock), 0, st, b,
-                               g, (const uint32_t *)sb.perm);
+        flip_g2p_direct(s, (const uint32_t *)sb.perm, st);
     }
-    GFSX_LAUNCH_CHECK();
     if (s.epi && s.epi->solid) fs_collide(*s.epi, st);
     if (ev) GFSX_HIP(hipEventRecord(ev[3], st));
 }

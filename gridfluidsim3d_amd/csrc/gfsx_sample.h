@@ -73,19 +73,48 @@ __device__ __forceinline__ float trilinear(const float *__restrict__ f, int w,
     return fmaf(tz, c1 - c0, c0);
 }
 
-// Catmull-Rom cubic (Keys a = -1/2) between p1 and p2, the formula of
-// Interpolation::cubicInterpolate (interpolation.cpp:44-46).
-__device__ __forceinline__ float cubic(float p0, float p1, float p2, float p3,
-                                       float t) {
-    return p1 + 0.5f * t *
-                    (p2 - p0 +
-                     t * (2.0f * p0 - 5.0f * p1 + 4.0f * p2 - p3 +
-                          t * (3.0f * (p1 - p2) + p3 - p0)));
+// The Catmull-Rom cubic (Keys a = -1/2) between p1 and p2 of
+// Interpolation::cubicInterpolate (interpolation.cpp:44-46) as weights of
+// p0..p3, its coefficients expanded: w0 = (-t + 2t^2 - t^3)/2,
+// w1 = 1 - 5t^2/2 + 3t^3/2, w2 = (t + 4t^2 - 3t^3)/2, w3 = (t^3 - t^2)/2.
+// A tricubic sample then costs 84 multiply-adds on top of three weight
+// sets, instead of 21 cubic evaluations.
+__device__ __forceinline__ void cubic_weights(float t, float w[4]) {
+    const float t2 = t * t;
+    w[0] = 0.5f * t * fmaf(2.0f - t, t, -1.0f);
+    w[1] = fmaf(t2, fmaf(1.5f, t, -2.5f), 1.0f);
+    w[2] = 0.5f * t * fmaf(t, fmaf(-3.0f, t, 4.0f), 1.0f);
+    w[3] = 0.5f * t2 * (t - 1.0f);
+}
+
+// Sum over a 4x4x4 stencil of at(i,j,k) times the weights, x innermost, in
+// one fixed order (every tricubic sampler uses it, so they agree bit for
+// bit).
+template <class F>
+__device__ __forceinline__ float tricubic_sum(F &&at, float tx, float ty, float tz) {
+    float wx[4], wy[4], wz[4];
+    cubic_weights(tx, wx);
+    cubic_weights(ty, wy);
+    cubic_weights(tz, wz);
+    float acc = 0.0f;
+#pragma unroll
+    for (int mk = 0; mk < 4; mk++) {
+        float pl = 0.0f;
+#pragma unroll
+        for (int mj = 0; mj < 4; mj++) {
+            float row = wx[0] * at(0, mj, mk);
+#pragma unroll
+            for (int mi = 1; mi < 4; mi++) row = fmaf(wx[mi], at(mi, mj, mk), row);
+            pl = fmaf(wy[mj], row, pl);
+        }
+        acc = fmaf(wz[mk], pl, acc);
+    }
+    return acc;
 }
 
 // Tricubic sample over the 4x4x4 stencil at (i-1..i+2, j-1..j+2,
-// k-1..k+2), x first, then y, then z (interpolation.cpp:26-42); samples
-// outside the array read 0 (macvelocityfield.cpp:372-378).
+// k-1..k+2) (interpolation.cpp:26-42); samples outside the array read 0
+// (macvelocityfield.cpp:372-378).
 __device__ __forceinline__ float tricubic(const float *__restrict__ f, int w,
                                           int h, int d, int i, int j, int k,
                                           float tx, float ty, float tz) {
@@ -101,22 +130,12 @@ __device__ __forceinline__ float tricubic(const float *__restrict__ f, int w,
         jr[m] = (uint32_t)min(max(b, 0), h - 1);
         kr[m] = (uint32_t)min(max(c, 0), d - 1);
     }
-    float zs[4];
-#pragma unroll
-    for (int mk = 0; mk < 4; mk++) {
-        float ys[4];
-#pragma unroll
-        for (int mj = 0; mj < 4; mj++) {
+    return tricubic_sum(
+        [&](int mi, int mj, int mk) {
             const uint32_t row = (uint32_t)w * (jr[mj] + (uint32_t)h * kr[mk]);
-            const bool okr = vj[mj] && vk[mk];
-            float p[4];
-#pragma unroll
-            for (int mi = 0; mi < 4; mi++) p[mi] = ld_or_zero(f, okr && vi[mi], row + ic[mi]);
-            ys[mj] = cubic(p[0], p[1], p[2], p[3], tx);
-        }
-        zs[mk] = cubic(ys[0], ys[1], ys[2], ys[3], ty);
-    }
-    return cubic(zs[0], zs[1], zs[2], zs[3], tz);
+            return ld_or_zero(f, vj[mj] && vk[mk] && vi[mi], row + ic[mi]);
+        },
+        tx, ty, tz);
 }
 
 template <int ST>

@@ -5,8 +5,10 @@ reference run at that size:
 * two runs of the same step give identical bits (grids, snapshot, weights,
   positions, velocities, particle labels);
 * the particle labels are a permutation and, with GFSX_ORDER_SORTED, the
-  input positions taken in output order are grouped by cell in bin order
-  (x fastest, one pad layer, the out-of-grid particles last);
+  output order is a stable sort of the input by P2G brick (4^3 cells, the
+  8 bricks of every 8^3-cell G2P brick consecutive, the pad cells in the
+  edge bricks, the particles beyond them last): keys non-decreasing, and
+  input indices increasing within each brick;
 * P2G is exactly linear in the velocities: with the body force off,
   doubling every velocity doubles the pre-force snapshot bit for bit and
   leaves the weights unchanged (the scatter's fixed-point terms are the same
@@ -47,15 +49,19 @@ def test_fullsize_step_deterministic_and_sorted(ctx, pool):
     assert torch.equal(a.particle_ids(), b.particle_ids())
     n = pool.pos.shape[0]
     assert torch.equal(torch.sort(ids).values, torch.arange(n, device=ids.device))
-    # cell (bin) of every input position, in output order
+    # brick of every input position, in output order
     p = torch.as_tensor(pool.pos, device=ids.device)[ids].double()
-    ni, nj, nk = pool.dims
-    cell = torch.floor(p * (1.0 / pool.dx)).long() + 1
-    b0, b1, b2 = ni + 2, nj + 2, nk + 2
-    inside = ((cell >= 0).all(1) & (cell[:, 0] < b0) & (cell[:, 1] < b1) & (cell[:, 2] < b2))
-    key = torch.where(inside, cell[:, 0] + b0 * (cell[:, 1] + b1 * cell[:, 2]),
-                      torch.full_like(cell[:, 0], b0 * b1 * b2))
+    dims = torch.tensor(pool.dims, device=ids.device)
+    cell = torch.floor(p * (1.0 / pool.dx)).long()
+    inside = ((cell >= -1) & (cell <= dims)).all(1)
+    b = torch.minimum(torch.clamp(cell, min=0), dims - 1) >> 2
+    gb = ((dims + 3) // 4 + 1) // 2
+    g = (b[:, 0] >> 1) + gb[0] * ((b[:, 1] >> 1) + gb[1] * (b[:, 2] >> 1))
+    key = (g << 3) | (b[:, 0] & 1) | ((b[:, 1] & 1) << 1) | ((b[:, 2] & 1) << 2)
+    key = torch.where(inside, key, torch.full_like(key, int(8 * gb.prod())))
     assert bool((key[1:] >= key[:-1]).all())
+    same = key[1:] == key[:-1]
+    assert bool((ids[1:][same] > ids[:-1][same]).all())
     # every particle advanced: no NaN in the outputs of this scene
     for c in range(3):
         assert bool(torch.isfinite(a.cur_x[c]).all()) and bool(torch.isfinite(a.cur_v[c]).all())

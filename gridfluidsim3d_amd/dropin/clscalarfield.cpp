@@ -8,6 +8,10 @@
 //    overloads use its weight field iff it is enabled (:269-301).
 // The maximum-value threshold of addPoints (a chunk-culling heuristic of the
 // OpenCL path, :992-1013) is not applied: points are always splatted.
+// disableOpenCL() selects the reference's CPU path (:75-77, :499-509), which
+// this drop-in does not carry: the flag is kept and reported truthfully, and
+// a splat while it is cleared fails with std::runtime_error instead of
+// silently running on the device.
 #include "clscalarfield.h"
 
 #include <cstdio>
@@ -40,6 +44,10 @@ bool CLScalarField::initialize() {
 void CLScalarField::_splat(std::vector<vmath::vec3> &points, const float *values,
                            double radius, vmath::vec3 offset, double dx,
                            Array3d<float> *field, Array3d<float> *weight) {
+    if (!_isOpenCLEnabled)
+        throw std::runtime_error("CLScalarField: the accelerated path is disabled "
+                                 "(disableOpenCL) and this drop-in has no CPU path; "
+                                 "call enableOpenCL() or use ScalarField::addPointValue");
     if (!_ctx && !initialize())
         throw std::runtime_error("CLScalarField: no HIP device (" +
                                  std::string(gfsx_get_error_message()) + ")");
@@ -123,12 +131,17 @@ std::string CLScalarField::getDeviceInfo() {
     return gfsx_get_device_info(_ctx, &err);
 }
 void CLScalarField::printDeviceInfo() { std::cout << getDeviceInfo() << std::endl; }
-std::string CLScalarField::getKernelInfo() { return _ctx ? "gfsx brick-staged P2G gather (gfx950)" : ""; }
+// The splat every addPoints / addPointValues call runs (gfsx_p2g): a
+// counting sort of the points by node cell, then one gather per node over
+// the cells within the radius.
+std::string CLScalarField::getKernelInfo() {
+    return _ctx ? "gfsx_p2g: cell-sorted gather P2G (k_p2g_gather, gfx950)" : "";
+}
 void CLScalarField::printKernelInfo() { std::cout << getKernelInfo() << std::endl; }
 bool CLScalarField::isUsingGPU() { return _ctx != nullptr; }
 bool CLScalarField::isUsingCPU() { return false; }
-void CLScalarField::disableOpenCL() {}
-void CLScalarField::enableOpenCL() {}
-bool CLScalarField::isOpenCLEnabled() { return false; }
+void CLScalarField::disableOpenCL() { _isOpenCLEnabled = false; }
+void CLScalarField::enableOpenCL() { _isOpenCLEnabled = true; }
+bool CLScalarField::isOpenCLEnabled() { return _isOpenCLEnabled; }
 int CLScalarField::getKernelWorkLoadSize() { return _kernelWorkLoadSize; }
 void CLScalarField::setKernelWorkLoadSize(int n) { _kernelWorkLoadSize = n; }

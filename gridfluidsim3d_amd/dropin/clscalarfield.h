@@ -71,6 +71,7 @@ private:
                 Array3d<float> *field, Array3d<float> *weight);
 
     gfsx_ctx *_ctx = nullptr;
+    bool _isOpenCLEnabled = true;   // the accelerated (here: HIP) path, as in the reference
     bool _isMaxScalarFieldValueThresholdSet = false;
     float _maxScalarFieldValueThreshold = 1.0f;
     int _kernelWorkLoadSize = 1000;

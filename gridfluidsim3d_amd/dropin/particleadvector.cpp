@@ -6,11 +6,14 @@
 //  * tricubicInterpolate resizes output to the particle count, overwrites it
 //    and zeroes NaN/Inf results (:401-455, :1139-1149);
 //  * advectParticlesRK1..4 replace output with one Runge-Kutta step per
-//    particle, with the FP32 step-size casts of the CPU path (:1045-1122).
+//    particle, with the FP32 step-size casts of the CPU path (:1045-1122);
+//  * disableOpenCL() selects the reference's CPU path (:189-199, :213-215),
+//    which this drop-in does not carry: the flag is kept and reported, and a
+//    call while it is cleared fails with std::runtime_error.
+// The stencil is chosen with setStencil (tricubic by default, the
+// reference's).
 #include "particleadvector.h"
 
-#include <cstdlib>
-#include <cstring>
 #include <iostream>
 #include <stdexcept>
 
@@ -21,14 +24,9 @@ void check(int err, const char *what) {
     if (err != GFSX_SUCCESS)
         throw std::runtime_error(std::string(what) + ": " + gfsx_get_error_message());
 }
-
-int default_stencil() {
-    const char *s = std::getenv("GFSX_STENCIL");
-    return (s && std::strcmp(s, "trilinear") == 0) ? GFSX_TRILINEAR : GFSX_TRICUBIC;
-}
 }  // namespace
 
-ParticleAdvector::ParticleAdvector() : _stencil(default_stencil()) {}
+ParticleAdvector::ParticleAdvector() {}
 
 ParticleAdvector::~ParticleAdvector() {
     int err;
@@ -42,7 +40,18 @@ bool ParticleAdvector::initialize() {
     return err == GFSX_SUCCESS && _ctx != nullptr;
 }
 
+void ParticleAdvector::setStencil(int stencil) {
+    if (stencil != GFSX_TRILINEAR && stencil != GFSX_TRICUBIC)
+        throw std::invalid_argument("ParticleAdvector: stencil must be GFSX_TRILINEAR or "
+                                    "GFSX_TRICUBIC");
+    _stencil = stencil;
+}
+
 void ParticleAdvector::_ensure() {
+    if (!_isOpenCLEnabled)
+        throw std::runtime_error("ParticleAdvector: the accelerated path is disabled "
+                                 "(disableOpenCL) and this drop-in has no CPU path; "
+                                 "call enableOpenCL()");
     if (!_ctx && !initialize())
         throw std::runtime_error("ParticleAdvector: no HIP device (" +
                                  std::string(gfsx_get_error_message()) + ")");
@@ -121,8 +130,8 @@ std::string ParticleAdvector::getKernelInfo() {
 void ParticleAdvector::printKernelInfo() { std::cout << getKernelInfo() << std::endl; }
 bool ParticleAdvector::isUsingGPU() { return _ctx != nullptr; }
 bool ParticleAdvector::isUsingCPU() { return false; }
-void ParticleAdvector::disableOpenCL() {}
-void ParticleAdvector::enableOpenCL() {}
-bool ParticleAdvector::isOpenCLEnabled() { return false; }
+void ParticleAdvector::disableOpenCL() { _isOpenCLEnabled = false; }
+void ParticleAdvector::enableOpenCL() { _isOpenCLEnabled = true; }
+bool ParticleAdvector::isOpenCLEnabled() { return _isOpenCLEnabled; }
 int ParticleAdvector::getKernelWorkLoadSize() { return _kernelWorkLoadSize; }
 void ParticleAdvector::setKernelWorkLoadSize(int n) { _kernelWorkLoadSize = n; }

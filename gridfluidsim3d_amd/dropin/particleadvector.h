@@ -59,10 +59,11 @@ public:
                              MACVelocityField *vfield);
 
     // The G2P stencil used by every method above: GFSX_TRICUBIC (the
-    // reference's) unless the environment variable GFSX_STENCIL says
-    // "trilinear".
+    // reference's, the default) or GFSX_TRILINEAR (the stencil of
+    // MACVelocityField::evaluateVelocityAtPositionLinear).  No counterpart
+    // in the reference; anything else throws std::invalid_argument.
     int stencil() const { return _stencil; }
-    void setStencil(int stencil) { _stencil = stencil; }
+    void setStencil(int stencil);
 
 private:
     void _advect(int order, std::vector<vmath::vec3> &particles,
@@ -71,7 +72,8 @@ private:
     void _ensure();
 
     gfsx_ctx *_ctx = nullptr;
-    int _stencil;
+    int _stencil = 1;               // GFSX_TRICUBIC
+    bool _isOpenCLEnabled = true;   // the accelerated (here: HIP) path, as in the reference
     int _kernelWorkLoadSize = 1000;
 };
 

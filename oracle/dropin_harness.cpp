@@ -14,6 +14,7 @@
 //     particleadvector.cpp needs <CL/cl.hpp> and is not built here).
 // Built by oracle/Makefile (target dropin) into oracle/_ref/.
 #include <cmath>
+#include <stdexcept>
 #include <cstdint>
 #include <algorithm>
 #include <vector>
@@ -42,6 +43,54 @@ vmath::vec3 eval(MACVelocityField &m, vmath::vec3 p, int stencil) {
 }  // namespace
 
 extern "C" {
+
+// The accelerated-path toggles of both classes (clscalarfield.cpp:499-509,
+// particleadvector.cpp:189-199): enabled by default, reported as set, and a
+// call while disabled fails with std::runtime_error before any device work
+// (this drop-in has no CPU path).  No device is needed.  Returns a bit mask
+// of the checks that held (all ten: 0x3FF).
+int dropin_toggles(void) {
+    int ok = 0, bit = 0;
+    auto mark = [&](bool c) { ok |= (c ? 1 : 0) << bit++; };
+    CLScalarField acc;
+    mark(acc.isOpenCLEnabled());
+    acc.disableOpenCL();
+    mark(!acc.isOpenCLEnabled());
+    std::vector<vmath::vec3> points(4, vmath::vec3(0.1f, 0.1f, 0.1f));
+    std::vector<float> values(4, 1.0f);
+    Array3d<float> field(4, 4, 4, 0.0f);
+    try {
+        acc.addPointValues(points, values, 0.1, vmath::vec3(), 0.1, &field);
+        mark(false);
+    } catch (const std::runtime_error &) {
+        mark(true);
+    }
+    mark(!acc.isUsingGPU());   // nothing ran, nothing initialised
+    acc.enableOpenCL();
+    mark(acc.isOpenCLEnabled());
+
+    ParticleAdvector adv;
+    mark(adv.isOpenCLEnabled() && adv.stencil() == GFSX_TRICUBIC);
+    adv.disableOpenCL();
+    mark(!adv.isOpenCLEnabled());
+    MACVelocityField mac(4, 4, 4, 0.1);
+    std::vector<vmath::vec3> out;
+    try {
+        adv.tricubicInterpolate(points, &mac, out);
+        mark(false);
+    } catch (const std::runtime_error &) {
+        mark(true);
+    }
+    try {
+        adv.setStencil(7);
+        mark(false);
+    } catch (const std::invalid_argument &) {
+        mark(adv.stencil() == GFSX_TRICUBIC);
+    }
+    adv.enableOpenCL();
+    mark(adv.isOpenCLEnabled() && !adv.isUsingGPU());
+    return ok;
+}
 
 int dropin_p2g(const float *pos, const float *val, int64_t n, double radius,
                const float off[3], double dx, int ni, int nj, int nk,

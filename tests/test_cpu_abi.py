@@ -104,3 +104,18 @@ def test_package_has_no_cpu_fallback():
             if f.endswith((".py", ".hip", ".h", ".cpp")):
                 text = open(os.path.join(dirpath, f)).read()
                 assert "oracle" not in text.replace("oracles", ""), f
+
+
+def test_dropin_toggles_report_truthfully():
+    """disableOpenCL / enableOpenCL / isOpenCLEnabled of the drop-in classes
+    keep and report the flag, and a call while it is cleared fails with an
+    error instead of running on the device (clscalarfield.cpp:499-509,
+    particleadvector.cpp:189-199); setStencil refuses unknown stencils."""
+    import gridfluidsim3d_amd as g
+    from oracle import oracle
+    g.load_library()
+    if not os.path.exists(oracle.DROPIN_SO):
+        pytest.skip("drop-in checker not built (needs the reference tree at build time)")
+    lib = ctypes.CDLL(oracle.DROPIN_SO)
+    lib.dropin_toggles.restype = ctypes.c_int
+    assert lib.dropin_toggles() == 0x3FF

@@ -38,7 +38,7 @@ enum Slot {
     S_OLD0, S_XS0, S_XS1, S_XS2, S_VS0, S_VS1, S_VS2, S_DENSE, S_DENSEN,
     S_D_POS, S_D_VAL, S_D_S, S_D_W, S_D_U, S_D_V, S_D_WW, S_D_OUT,
     S_B_GRID, S_B_OLD, S_B_WGT, S_B_LG, S_B_XO, S_B_VO, S_B_FX, S_B_MAX,
-    S_E_N, S_E_W, S_E_LAYER, S_E_LIST, S_R_KEEP, S_R_DST, S_R_SCAN, S_R_N,
+    S_E_N, S_E_W, S_E_LAYER, S_E_LIST, S_R_KEEP, S_R_DST, S_R_SCAN, S_R_N, S_R_FULL, S_R_NFULL,
     S_P_TILE, S_P_TFLAG,
     S_K_COUNT, S_K_START, S_K_PSTART, S_K_RSTART, S_K_RUNREC, S_K_NRUNS, S_K_LIST, S_K_LIST2,
     S_K_RUNBASE, S_K_MID, S_K_HUGE, S_K_LCOUNT, S_K_SCAN, S_K_FLUID, S_K_SRANGE, S_K_SSLOT,
@@ -996,6 +996,9 @@ int64_t gfsx_remove_particles(gfsx_ctx *ctx, const gfsx_scene *sc, int max_per_c
         r.seed = seed;
         r.keep = ctx->u32(S_R_KEEP, n + 1);
         r.dst = ctx->u32(S_R_DST, n + 1);
+        // every over-full cell holds more than max_per_cell particles
+        r.full = ctx->u32(S_R_FULL, std::min<int64_t>((int64_t)bg.nbin, n / (max_per_cell + 1) + 1));
+        r.nfull = ctx->u32(S_R_NFULL, 1);
         uint32_t *scratch = ctx->u32(S_R_SCAN, gfsx::scan_scratch_words(n + 1));
         gfsx::fill_u32(r.keep, 0u, n + 1, ctx->stream);
         gfsx::remove_mark(r, ctx->stream);

@@ -287,6 +287,7 @@ struct RemoveArgs {
     int max_per_cell;
     uint64_t seed;
     uint32_t *keep, *dst;
+    uint32_t *full, *nfull;   // over-full cells: list (capacity nbin) and count
 };
 
 // ---- launches (implemented in gfsx_kernels.hip) ------------------------

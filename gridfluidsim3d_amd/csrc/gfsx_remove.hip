@@ -7,9 +7,13 @@
 // whole array; here a cell keeps the max_per_cell particles with the
 // smallest SplitMix64 key of (seed, input index), a seeded order that is the
 // same on every run and every machine.  After the cell sort (integer work),
-// k_remove_mark ranks the particles of over-full cells by key inside their
-// cell, a scan of the keep flags gives each kept particle its output index
-// and k_remove_compact copies the kept particles in input order.
+// k_remove_mark keeps the particles of cells that are not over-full and
+// lists the over-full cells; k_remove_select finds, per listed cell, the
+// max_per_cell-th smallest key by a radix select over the cell's keys (eight
+// byte passes, each a 256-bucket histogram in LDS: linear in the cell's
+// count) and keeps the particles whose key does not exceed it; a scan of the
+// keep flags gives each kept particle its output index and k_remove_compact
+// copies the kept particles in input order.
 
 #include "gfsx_internal.h"
 
@@ -43,15 +47,65 @@ __global__ __launch_bounds__(kBlock) void k_remove_mark(RemoveArgs r) {
         } else {
             const uint32_t q0 = r.start[b], q1 = r.start[b + 1];
             if (q1 - q0 > (uint32_t)r.max_per_cell) {
-                const uint64_t kp = removal_key(r.seed, (uint64_t)p);
-                uint32_t rank = 0;
-                for (uint32_t q = q0; q < q1; q++)
-                    rank += removal_key(r.seed, (uint64_t)r.perm[q]) < kp;
-                k = rank < (uint32_t)r.max_per_cell;
+                // decided by k_remove_select; the cell's first particle in
+                // sort order lists it
+                k = 0;
+                if (r.perm[q0] == (uint32_t)p) r.full[atomicAdd(r.nfull, 1u)] = b;
             }
         }
     }
     r.keep[p] = k;
+}
+
+// One work-group per over-full cell (grid-stride over the list): the
+// max_per_cell-th smallest key of the cell, byte by byte from the top, then
+// keep[p] = 1 for the particles whose key is not above it.  Keys are
+// distinct (a bijection of the index), so exactly max_per_cell are kept.
+static_assert(kBlock == 256, "one histogram bucket per thread");
+
+__global__ __launch_bounds__(kBlock) void k_remove_select(RemoveArgs r) {
+    __shared__ uint32_t hist[256];
+    __shared__ uint64_t s_prefix;
+    __shared__ uint32_t s_rank;
+    const uint32_t nfull = *r.nfull;
+    const int t = threadIdx.x;
+    for (uint32_t c = blockIdx.x; c < nfull; c += gridDim.x) {
+        const uint32_t b = r.full[c];
+        const uint32_t q0 = r.start[b], q1 = r.start[b + 1];
+        if (t == 0) {
+            s_prefix = 0ull;
+            s_rank = (uint32_t)r.max_per_cell;   // 1-based rank still to find
+        }
+        for (int pass = 7; pass >= 0; pass--) {
+            const int sh = 8 * pass;
+            hist[t] = 0u;   // kBlock == 256 buckets
+            __syncthreads();
+            const uint64_t prefix = s_prefix;
+            for (uint32_t q = q0 + t; q < q1; q += kBlock) {
+                const uint64_t key = removal_key(r.seed, (uint64_t)r.perm[q]);
+                // the bytes above this pass match the prefix found so far
+                const bool match = pass == 7 || (key >> (sh + 8)) == (prefix >> (sh + 8));
+                if (match) atomicAdd(&hist[(key >> sh) & 255u], 1u);
+            }
+            __syncthreads();
+            if (t == 0) {
+                uint32_t rank = s_rank, d = 0;
+                while (hist[d] < rank) {
+                    rank -= hist[d];
+                    d++;
+                }
+                s_rank = rank;
+                s_prefix = prefix | ((uint64_t)d << sh);
+            }
+            __syncthreads();
+        }
+        const uint64_t kth = s_prefix;
+        for (uint32_t q = q0 + t; q < q1; q += kBlock) {
+            const uint32_t p = r.perm[q];
+            if (removal_key(r.seed, (uint64_t)p) <= kth) r.keep[p] = 1u;
+        }
+        __syncthreads();
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void k_remove_compact(RemoveArgs r) {
@@ -70,7 +124,10 @@ __global__ __launch_bounds__(kBlock) void k_remove_compact(RemoveArgs r) {
 
 void remove_mark(const RemoveArgs &r, hipStream_t s) {
     if (r.n <= 0) return;
+    fill_u32(r.nfull, 0u, 1, s);
     hipLaunchKernelGGL(k_remove_mark, dim3(blocks_for(r.n)), dim3(kBlock), 0, s, r);
+    GFSX_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_remove_select, dim3(1024), dim3(kBlock), 0, s, r);
     GFSX_LAUNCH_CHECK();
 }
 

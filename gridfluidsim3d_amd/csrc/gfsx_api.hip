@@ -14,10 +14,15 @@
 #include <cmath>
 #include <vector>
 #include <cstring>
+#include <mutex>
 #include <sstream>
 #include <string>
 
 using gfsx::Error;
+
+// Size of the static scratch of gfsx_flip_step_noalloc (GFSX_NOALLOC_SCRATCH_BYTES).
+constexpr size_t kNoallocScratchBytes = (size_t)GFSX_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
 
 namespace {
 
@@ -190,6 +195,16 @@ struct gfsx_ctx {
 };
 
 namespace {
+
+// The static scratch of gfsx_flip_step_noalloc: every device buffer of that
+// entry point's step (sort, tiles, staged particles, snapshot, scene table)
+// is carved from it.  Each device holds its own copy from the load of the
+// library to the end of the process.  In host code `scratch` only names the
+// symbol: its device address comes from hipGetSymbolAddress, once per device
+// (with that device current) in gfsx_create, and is kept in g_scratch_dev.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_dev[kMaxDevices] = {};
 
 // Device memory handed out front to back from one region (the static
 // scratch of gfsx_flip_step_noalloc).  `used` counts what a layout needs even
@@ -398,7 +413,6 @@ gfsx::SceneDesc scene_desc(const gfsx_scene *sc, const gfsx_step_params *pr, con
     return d;
 }
 
-// Runs the step; returns the scenes as laid out (bin grids, offsets).
 // Runs the step; returns the scenes as laid out (bin grids, offsets).  With
 // carve, every device buffer (and the scene table) comes from that region
 // and nothing is allocated: only the brick pipeline, one scene, no
@@ -599,6 +613,17 @@ gfsx_ctx *gfsx_create(int device, int *err) {
            << (prop.totalGlobalMem >> 20) << " MiB";
         c->info = ss.str();
         c->num_cu = std::max(prop.multiProcessorCount, 1);
+        // the device address of this device's copy of the static scratch,
+        // looked up once per device with that device current
+        if (device < kMaxDevices) {
+            std::lock_guard<std::mutex> lock(g_scratch_mu);
+            if (!g_scratch_dev[device]) {
+                void *p = nullptr;
+                GFSX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+                require(p != nullptr, "gfsx_create: no device address for the static scratch");
+                g_scratch_dev[device] = p;
+            }
+        }
         ok(err);
         return c;
     } catch (const std::exception &e) {
@@ -788,6 +813,24 @@ void gfsx_flip_step(gfsx_ctx *ctx, const gfsx_scene *sc,
     guarded(ctx, err, [&] {
         require(sc && pr, "gfsx_flip_step: null scene or params");
         flip_step_impl(ctx, sc, pr, 1, "gfsx_flip_step");
+    });
+}
+
+void gfsx_flip_step_noalloc(gfsx_ctx *ctx, const gfsx_scene *sc,
+                            const gfsx_step_params *pr, int *err) {
+    guarded(ctx, err, [&] {
+        const char *who = "gfsx_flip_step_noalloc";
+        require(sc && pr, "gfsx_flip_step_noalloc: null scene or params");
+        void *base = nullptr;
+        if (ctx->device < kMaxDevices) {
+            std::lock_guard<std::mutex> lock(g_scratch_mu);
+            base = g_scratch_dev[ctx->device];
+        }
+        require(base != nullptr, "gfsx_flip_step_noalloc: no static scratch on this device");
+        Carve carve;
+        carve.base = static_cast<char *>(base);
+        carve.cap = kNoallocScratchBytes;
+        flip_step_impl(ctx, sc, pr, 1, who, false, &carve);
     });
 }
 

@@ -28,6 +28,8 @@ ORDER_SORTED = 0
 ORDER_KEEP = 1
 EPILOGUE_NONE = 0
 EPILOGUE_FLUIDSIM = 1
+# bytes of the static device scratch of gfsx_flip_step_noalloc (GFSX_NOALLOC_SCRATCH_BYTES)
+NOALLOC_SCRATCH_BYTES = 64 << 20
 
 _fp = POINTER(c_float)
 _u32p = POINTER(ctypes.c_uint32)
@@ -387,6 +389,21 @@ class Context:
                           extrapolation_layers)
         self.use_torch_stream()
         f = pb.lib_func("gfsx_flip_step", [c_void_p, POINTER(Scene), POINTER(StepParams)])
+        pb.execute_lib_func(f, [self(), ctypes.byref(sc), ctypes.byref(pr)])
+        st.cur = 1 - st.cur
+
+    def flip_step_noalloc(self, state, dt, body_force=(0.0, -9.8, 0.0), pic_ratio=0.05,
+                          stencil=TRILINEAR, order=ORDER_SORTED):
+        """``flip_step`` through ``gfsx_flip_step_noalloc``: the same step with
+        every workspace buffer carved from the library's static device
+        scratch (``NOALLOC_SCRATCH_BYTES``); no device allocation.  One scene,
+        no epilogue; a step too large for the scratch raises."""
+        st = state
+        sc = self._scene_of(st)
+        pr = self._params(dt, body_force, pic_ratio, stencil, order)
+        self.use_torch_stream()
+        f = pb.lib_func("gfsx_flip_step_noalloc",
+                        [c_void_p, POINTER(Scene), POINTER(StepParams)])
         pb.execute_lib_func(f, [self(), ctypes.byref(sc), ctypes.byref(pr)])
         st.cur = 1 - st.cur
 

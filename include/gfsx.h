@@ -195,6 +195,26 @@ void gfsx_flip_step(gfsx_ctx *ctx, const gfsx_scene *scene,
 void gfsx_flip_step_batched(gfsx_ctx *ctx, const gfsx_scene *scenes, int nscenes,
                             const gfsx_step_params *params, int *err);
 
+/* Bytes of the static device scratch of gfsx_flip_step_noalloc (64 MiB,
+ * held on every device the library runs on until the process ends). */
+#define GFSX_NOALLOC_SCRATCH_BYTES 67108864
+
+/* gfsx_flip_step with no allocation: the same kernels and the same bits,
+ * every workspace buffer (sort, P2G tiles, staged particles, the snapshot
+ * when u_old.. are NULL, the scene table) carved from a static scratch of
+ * GFSX_NOALLOC_SCRATCH_BYTES per device; no hipMalloc, hipFree or
+ * allocating library call, no host-to-device copy, no timing events.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Limits: one scene, p2g_variant 4, no
+ * epilogue; a step whose workspace exceeds the scratch (about 56 bytes per
+ * particle plus 80 per cell) fails with a message giving the size it needs,
+ * before any work is enqueued.  Every call on a device shares that device's
+ * scratch and leaves its contents behind: calls that overlap on one device
+ * (other streams, contexts or threads) race. */
+void gfsx_flip_step_noalloc(gfsx_ctx *ctx, const gfsx_scene *scene,
+                            const gfsx_step_params *params, int *err);
+
 /* Tunables of the fused step, by name (an unknown name or value fails).
  * Every P2G and G2P variant is deterministic; they differ in speed, and the
  * P2G variants in the rounding of the sums (within FP32 rounding of each

@@ -6,8 +6,13 @@ The variants are selected with ``Context.set_option``; every variant gives
 the same result for G2P and results within FP32 rounding of each other for
 P2G (the gathers sum the candidates of a face in the same order with the
 arithmetic contracted differently; the scatter sums in fixed point), so the
-choice is about speed only.  Timing runs
-on copies of the states, so the caller's states do not advance.
+choice is about speed only.  Timing runs on copies of the states, so the
+caller's states do not advance.
+
+Run-to-run bit identity holds for a fixed choice of variants.  A timed
+choice can differ between runs (timing noise), and with it the rounding of
+the P2G sums; a caller that needs repeatable bits keeps the choice ``tune``
+returns and applies it again with ``apply`` instead of timing.
 """
 import copy
 
@@ -57,3 +62,16 @@ def tune(ctx, states, dt, body_force=(0.0, -9.8, 0.0), pic_ratio=0.05, stencil=0
         ctx.set_option(opt, best)
         result[opt] = (best, times)
     return result
+
+
+def choice(result):
+    """{option: value} of a ``tune`` result (the form ``apply`` takes)."""
+    return {opt: best for opt, (best, _times) in result.items()}
+
+
+def apply(ctx, chosen):
+    """Set a recorded choice ({option: value}) on ``ctx`` without timing."""
+    for opt, value in chosen.items():
+        if opt not in CANDIDATES:
+            raise ValueError("unknown tunable %r" % opt)
+        ctx.set_option(opt, int(value))

@@ -46,7 +46,7 @@ enum Slot {
     S_E_N, S_E_W, S_E_LAYER, S_E_LIST, S_R_KEEP, S_R_DST, S_R_SCAN, S_R_N, S_R_FULL, S_R_NFULL,
     S_P_TILE, S_P_TFLAG,
     S_K_COUNT, S_K_START, S_K_PSTART, S_K_RSTART, S_K_RUNREC, S_K_NRUNS, S_K_LIST, S_K_LIST2,
-    S_K_RUNBASE, S_K_MID, S_K_HUGE, S_K_LCOUNT, S_K_SCAN, S_K_FLUID, S_K_SRANGE, S_K_SSLOT,
+    S_K_RUNBASE, S_K_MID, S_K_HUGE, S_K_LCOUNT, S_K_SCAN, S_K_FLUID, S_K_SSLOT,
     S_K_SCOUNT,
     NSLOT
 };
@@ -226,8 +226,7 @@ struct Carve {
 // blocks over nbin brick bins (overflow bins included); ws(slot, bytes)
 // hands out device memory.
 template <class WS>
-gfsx::BrickSort brick_sort(WS &&ws, int64_t n, int64_t nblk, uint32_t nbin, int64_t ng2p,
-                           bool want_perm) {
+gfsx::BrickSort brick_sort(WS &&ws, int64_t n, int64_t nblk, uint32_t nbin, bool want_perm) {
     gfsx::BrickSort b;
     const int64_t nwave = nblk * 16;   // a binning block is 4 x 256 particles
     const size_t u64 = sizeof(unsigned long long), u32 = sizeof(uint32_t);
@@ -250,9 +249,8 @@ gfsx::BrickSort brick_sort(WS &&ws, int64_t n, int64_t nblk, uint32_t nbin, int6
         S_K_SCAN, u64 * (size_t)std::max<int64_t>(gfsx::scan64_scratch_words(nbin), 1));
     b.perm = want_perm ? (uint32_t *)ws(S_PERM, u32 * nn) : nullptr;
     b.fluid = (unsigned long long *)ws(S_K_FLUID, u64 * nbin);
-    b.slow_ranges = (uint2 *)ws(S_K_SRANGE, sizeof(uint2) * (size_t)std::max<int64_t>(ng2p, 1));
     b.slow_slots = (uint32_t *)ws(S_K_SSLOT, u32 * nn);
-    b.slow_count = (uint32_t *)ws(S_K_SCOUNT, u32 * 2);
+    b.slow_count = (uint32_t *)ws(S_K_SCOUNT, u32);
     return b;
 }
 
@@ -531,7 +529,7 @@ std::vector<gfsx::SceneDesc> flip_step_impl(gfsx_ctx *ctx, const gfsx_scene *sce
         bool ids = false;
         for (auto &d : s.scenes) ids = ids || d.id_out != nullptr;
         gfsx::BrickSort bs = brick_sort(ws, s.tot.n, s.tot.nblk, (uint32_t)s.tot.nbb,
-                                        s.tot.ng2p, s.keep_order || ids);
+                                        s.keep_order || ids);
         if (carve) {
             if (carve->used > carve->cap) {
                 std::ostringstream m;

@@ -644,11 +644,12 @@ __device__ __forceinline__ void bscatter_comp(unsigned long long *acc, uint32_t 
                 }
                 const int off = dx + SX * (dy + SY * dz);
                 const float tt = qx[dx] + yz[dy][dz];
-                const float a = __saturatef(1.0f - tt);
-                const float kw = a * a * fmaf(c1, a, c0);
                 // about half the corners lie beyond the radius (K = 0): those
-                // lanes issue no LDS traffic
+                // lanes issue no LDS traffic.  Inside it a = 1 - t lies in
+                // (0, 1] (t is a sum of squares), so no clamp is needed.
                 if (tt < 1.0f) {
+                    const float a = 1.0f - tt;
+                    const float kw = a * a * fmaf(c1, a, c0);
                     atomicAdd(Wa + off, fix64(kw));
                     if (FAST || !bad) {
                         atomicAdd(Sa + off, fix64(kw * vr));
@@ -941,22 +942,60 @@ __global__ __launch_bounds__(kBlock) void k_p2g_bcombine(
     const AxisTiles ay = axis_tiles(J, sd.nby, c == 1), az = axis_tiles(K, sd.nbz, c == 2);
     const TileGrid tg = tile_grid(sd, c);
     const int sx = tsx(c), sy = tsy(c), sz = tsz(c);
-    const float *tf = reinterpret_cast<const float *>(tiles);
+    // per (z, y) brick of the row (uniform over the wave): the tile row the
+    // face row reads, and the bin of the row's brick bx = 0 (brick_local:
+    // bx adds (bx >> 1) << 3 | (bx & 1))
+    size_t trow[2][2];
+    uint32_t brow[2][2];
+#pragma unroll
+    for (int iz = 0; iz < 2; iz++)
+#pragma unroll
+        for (int iy = 0; iy < 2; iy++) {
+            const int by = ay.b + ay.db[iy], bz = az.b + az.db[iz];
+            trow[iz][iy] = tile_at(tg, 0, by * sy + ay.tp[iy], bz * sz + az.tp[iz]);
+            brow[iz][iy] = sd.kb0 + brick_local(sd, 0, by, bz);
+        }
     const FaceOut fo = face_out(sd.p2g, c);
+    const unsigned long long *te = reinterpret_cast<const unsigned long long *>(tiles);
     for (int I = lane; I < nxc; I += 64) {
         const AxisTiles ax = axis_tiles(I, sd.nbx, c == 0);
-        float S = 0.0f, W = 0.0f;
-        for (int iz = 0; iz < az.cnt; iz++)
-            for (int iy = 0; iy < ay.cnt; iy++)
-                for (int ix = 0; ix < ax.cnt; ix++) {
-                    const int bx = ax.b + ax.db[ix], by = ay.b + ay.db[iy], bz = az.b + az.db[iz];
-                    if (gld(tile_flag, sd.kb0 + brick_local(sd, bx, by, bz))) {
-                        const size_t q = tile_at(tg, bx * sx + ax.tp[ix], by * sy + ay.tp[iy],
-                                                 bz * sz + az.tp[iz]);
-                        S += gld(tf, 2 * q);
-                        W += gld(tf, 2 * q + 1);
+        const bool two = ax.cnt == 2;
+        const int bx0 = ax.b + ax.db[0], bx1 = two ? ax.b + ax.db[1] : bx0;
+        const uint32_t fb0 = ((uint32_t)(bx0 >> 1) << 3) | (uint32_t)(bx0 & 1);
+        const uint32_t fb1 = ((uint32_t)(bx1 >> 1) << 3) | (uint32_t)(bx1 & 1);
+        const size_t e0 = (size_t)(bx0 * sx + ax.tp[0]), e1 = (size_t)(bx1 * sx + ax.tp[1]);
+        // every flag and entry load goes out before any is used
+        uint32_t f0[2][2], f1[2][2];
+        unsigned long long v0[2][2], v1[2][2];   // (S, W) float pairs
+#pragma unroll
+        for (int iz = 0; iz < 2; iz++)
+#pragma unroll
+            for (int iy = 0; iy < 2; iy++) {
+                f0[iz][iy] = f1[iz][iy] = 0u;
+                if (iz < az.cnt && iy < ay.cnt) {
+                    f0[iz][iy] = gld(tile_flag, brow[iz][iy] + fb0);
+                    v0[iz][iy] = gld(te, trow[iz][iy] + e0);
+                    if (two) {
+                        f1[iz][iy] = gld(tile_flag, brow[iz][iy] + fb1);
+                        v1[iz][iy] = gld(te, trow[iz][iy] + e1);
                     }
                 }
+            }
+        // the entries added lower brick first: z, then y, then x
+        float S = 0.0f, W = 0.0f;
+#pragma unroll
+        for (int iz = 0; iz < 2; iz++)
+#pragma unroll
+            for (int iy = 0; iy < 2; iy++) {
+                if (f0[iz][iy]) {
+                    S += __uint_as_float((uint32_t)v0[iz][iy]);
+                    W += __uint_as_float((uint32_t)(v0[iz][iy] >> 32));
+                }
+                if (f1[iz][iy]) {
+                    S += __uint_as_float((uint32_t)v1[iz][iy]);
+                    W += __uint_as_float((uint32_t)(v1[iz][iy] >> 32));
+                }
+            }
         const int64_t f = (int64_t)I + (int64_t)nxc * ((int64_t)J + (int64_t)nyc * K);
         float val = W > 0.0f ? S / W : S;
         if (fo.old) fo.old[f] = val;
@@ -1022,13 +1061,41 @@ __device__ __forceinline__ bool g2p_particle_in(const StepG2P &a, const MacGrid 
     return true;
 }
 
-// Work the brick kernel leaves to k_flip_g2p_slow: whole brick ranges
-// (bricks whose field may carry a stage out of the boxes) and single
-// particles (start cell outside the brick).
+// The same for any start position, every sample checked against its box
+// (brick_sample: a stencil that leaves the box reads global memory; the
+// same bits as the boxes give).
+template <int ST>
+__device__ __forceinline__ void g2p_particle_checked(const StepG2P &a, const MacGrid &g,
+                                                     const float *const cur[3],
+                                                     const float *const old[3], int lx, int ly,
+                                                     int lz, int lox, int loy, int loz,
+                                                     const float x0[3], const float v0[3],
+                                                     float x1[3], float vnew[3]) {
+    float k1[3], vo[3];
+    brick_sample<ST, true>(g, a.old[0], a.old[1], a.old[2], cur[0], cur[1], cur[2], old[0],
+                           old[1], old[2], lx, ly, lz, lox, loy, loz, x0[0], x0[1], x0[2], k1, vo);
+    flip_blend(a, v0, k1, vo, vnew);
+    float xq[3], k2[3], k3[3], k4[3], unused[3];
+    auto stage = [&](const float h, const float k[3], float out[3]) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) xq[c] = x0[c] + h * k[c];
+        brick_sample<ST, false>(g, nullptr, nullptr, nullptr, cur[0], cur[1], cur[2], nullptr,
+                                nullptr, nullptr, lx, ly, lz, 0, 0, 0, xq[0], xq[1], xq[2], out,
+                                unused);
+    };
+    stage(a.rk.half, k1, k2);
+    stage(a.rk.half, k2, k3);
+    stage(a.rk.full, k3, k4);
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+        x1[c] = x0[c] + a.rk.sixth * (((k1[c] + 2.0f * k2[c]) + 2.0f * k3[c]) + k4[c]);
+}
+
+// Work the brick kernel leaves to k_flip_g2p_slow: single particles whose
+// start cell lies outside their brick (in the grid).
 struct G2PSlow {
-    uint2 *ranges;        // [q0, q1) of the sort
     uint32_t *slots;      // sort slots
-    uint32_t *count;      // 2: ranges, slots
+    uint32_t *count;      // number of slots
 };
 
 // The largest |value| of three per-thread maxima over the work-group (NaN
@@ -1043,6 +1110,7 @@ __device__ __forceinline__ float nan_max(float a, float b) {
 // (trilinear weights are convex; Catmull-Rom weights sum to at most 1.25 in
 // magnitude per axis, so at most 2 overall), a start cell inside the brick
 // keeps every stencil inside the staged boxes and no sample tests them.
+// Otherwise every sample is checked against its box.
 template <int ST>
 __device__ __forceinline__ void g2p_bbrick(const StepG2P &a, const MacGrid &g, int gbx, int gby,
                                            int lg, uint32_t q0, uint32_t q1,
@@ -1062,6 +1130,9 @@ __device__ __forceinline__ void g2p_bbrick(const StepG2P &a, const MacGrid &g, i
     m[0] = stage_box<B::E>(s_cur[0], g.f[0], g.ni + 1, g.nj, g.nk, lx, ly, lz);
     m[1] = stage_box<B::E>(s_cur[1], g.f[1], g.ni, g.nj + 1, g.nk, lx, ly, lz);
     m[2] = stage_box<B::E>(s_cur[2], g.f[2], g.ni, g.nj, g.nk + 1, lx, ly, lz);
+    stage_box<B::EO>(s_old[0], a.old[0], g.ni + 1, g.nj, g.nk, lox, loy, loz);
+    stage_box<B::EO>(s_old[1], a.old[1], g.ni, g.nj + 1, g.nk, lox, loy, loz);
+    stage_box<B::EO>(s_old[2], a.old[2], g.ni, g.nj, g.nk + 1, lox, loy, loz);
 #pragma unroll
     for (int c = 0; c < 3; c++) {
         float v = m[c];
@@ -1079,17 +1150,19 @@ __device__ __forceinline__ void g2p_bbrick(const StepG2P &a, const MacGrid &g, i
         for (int w = 1; w < NT / 64; w++) v = nan_max(v, s_max[w][c]);
         fast = fast && reach * v <= 0.99f;
     }
-    if (!fast) {
-        // the whole brick through the checked global path
-        if (t == 0) slow.ranges[atomicAdd(&slow.count[0], 1u)] = make_uint2(q0, q1);
-        return;
-    }
-    stage_box<B::EO>(s_old[0], a.old[0], g.ni + 1, g.nj, g.nk, lox, loy, loz);
-    stage_box<B::EO>(s_old[1], a.old[1], g.ni, g.nj + 1, g.nk, lox, loy, loz);
-    stage_box<B::EO>(s_old[2], a.old[2], g.ni, g.nj, g.nk + 1, lox, loy, loz);
-    __syncthreads();
     const float *const cur[3] = {s_cur[0], s_cur[1], s_cur[2]};
     const float *const old[3] = {s_old[0], s_old[1], s_old[2]};
+    if (!fast) {
+        // a stage may leave the boxes: every sample checked
+        for (uint32_t q = q0 + (uint32_t)t; q < q1; q += NT) {
+            const float x0[3] = {gld(a.xin[0], q), gld(a.xin[1], q), gld(a.xin[2], q)};
+            const float v0[3] = {gld(a.vin[0], q), gld(a.vin[1], q), gld(a.vin[2], q)};
+            float x1[3], vnew[3];
+            g2p_particle_checked<ST>(a, g, cur, old, lx, ly, lz, lox, loy, loz, x0, v0, x1, vnew);
+            store_particle(a, perm, q, x1, vnew);
+        }
+        return;
+    }
     // the next particle is loaded while this one is computed
     float xn[3] = {0.0f, 0.0f, 0.0f}, vn[3] = {0.0f, 0.0f, 0.0f};
     if (q0 + (uint32_t)t < q1) {
@@ -1119,7 +1192,7 @@ __device__ __forceinline__ void g2p_bbrick(const StepG2P &a, const MacGrid &g, i
         if (mk) {
             const int lead = __ffsll((unsigned long long)mk) - 1;
             uint32_t base = 0;
-            if (lane == lead) base = atomicAdd(&slow.count[1], (uint32_t)__popcll(mk));
+            if (lane == lead) base = atomicAdd(slow.count, (uint32_t)__popcll(mk));
             base = __shfl(base, lead, 64);
             if (!done) slow.slots[base + (uint32_t)__popcll(mk & ((1ull << lane) - 1ull))] = q;
         }
@@ -1159,9 +1232,9 @@ __global__ __launch_bounds__(gbrick::NT, 2) void k_flip_g2p_bb_tc(
 }
 
 // The particles the brick kernel left, sampled from global memory
-// (sample_mac: the same bits as the boxes give): the slow bricks' ranges,
-// the single slots, and each scene's overflow bin (particles beyond the
-// padded grid, where every sample is 0).
+// (sample_mac: the same bits as the boxes give): the single slots, and each
+// scene's overflow bin (particles beyond the padded grid, where every
+// sample is 0).
 template <int ST>
 __device__ __forceinline__ void g2p_direct_one(const SceneDesc *__restrict__ scenes, int nsc,
                                                Sorted so, int keep_order,
@@ -1184,13 +1257,7 @@ __global__ __launch_bounds__(kBlock) void k_flip_g2p_slow(
     const uint32_t *__restrict__ pstart, const uint32_t *__restrict__ perm, G2PSlow slow) {
     const uint32_t stride = gridDim.x * kBlock;
     const uint32_t t0 = blockIdx.x * kBlock + threadIdx.x;
-    const uint32_t nr = slow.count[0], ns = slow.count[1];
-    // ranges: one work-group per range at a time
-    for (uint32_t r = blockIdx.x; r < nr; r += gridDim.x) {
-        const uint2 rg = slow.ranges[r];
-        for (uint32_t q = rg.x + threadIdx.x; q < rg.y; q += kBlock)
-            g2p_direct_one<ST>(scenes, nsc, so, keep_order, perm, q);
-    }
+    const uint32_t ns = *slow.count;
     for (uint32_t e = t0; e < ns; e += stride)
         g2p_direct_one<ST>(scenes, nsc, so, keep_order, perm, slow.slots[e]);
     for (int sc = 0; sc < nsc; sc++) {
@@ -1292,10 +1359,9 @@ void flip_step_brick(const StepBatch &s, const BrickSort &bs, hipStream_t st, hi
         const uint32_t nbrick = (uint32_t)s.tot.ng2p;
         const int gper = (int)((nbrick + 7) / 8);
         G2PSlow slow;
-        slow.ranges = bs.slow_ranges;
         slow.slots = bs.slow_slots;
         slow.count = bs.slow_count;
-        fill_u32(bs.slow_count, 0u, 2, st);
+        fill_u32(bs.slow_count, 0u, 1, st);
         auto *kern = s.stencil == 0 ? k_flip_g2p_bb_tl : k_flip_g2p_bb_tc;
         hipLaunchKernelGGL(kern, dim3((unsigned)(8 * gper)), dim3(gbrick::NT), 0, st, s.scenes_dev,
                            nsc, so, s.keep_order, nbrick, gper, (const uint32_t *)bs.pstart,

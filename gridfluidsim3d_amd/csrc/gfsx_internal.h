@@ -406,9 +406,8 @@ struct BrickSort {
     unsigned long long *scan_scratch;
     uint32_t *perm;              // nullable: sort slot -> global particle index
     unsigned long long *fluid;   // per bin: the cells of the brick that hold a particle
-    // G2P work left to the global-memory kernel: brick ranges (at most one
-    // per G2P brick), single sort slots (at most n), and the two counts
-    uint2 *slow_ranges;
+    // G2P work left to the global-memory kernel: single sort slots (at most
+    // n) and their count
     uint32_t *slow_slots;
     uint32_t *slow_count;
     int64_t nwave;

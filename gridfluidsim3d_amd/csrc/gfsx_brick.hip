@@ -1126,13 +1126,13 @@ __device__ __forceinline__ void g2p_bbrick(const StepG2P &a, const MacGrid &g, i
     const int c0x = gx * GB, c0y = gy * GB, c0z = gz * GB;
     const int lx = c0x - B::LO, ly = c0y - B::LO, lz = c0z - B::LO;
     const int lox = c0x - B::LOO, loy = c0y - B::LOO, loz = c0z - B::LOO;
-    float m[3];
-    m[0] = stage_box<B::E>(s_cur[0], g.f[0], g.ni + 1, g.nj, g.nk, lx, ly, lz);
-    m[1] = stage_box<B::E>(s_cur[1], g.f[1], g.ni, g.nj + 1, g.nk, lx, ly, lz);
-    m[2] = stage_box<B::E>(s_cur[2], g.f[2], g.ni, g.nj, g.nk + 1, lx, ly, lz);
-    stage_box<B::EO>(s_old[0], a.old[0], g.ni + 1, g.nj, g.nk, lox, loy, loz);
-    stage_box<B::EO>(s_old[1], a.old[1], g.ni, g.nj + 1, g.nk, lox, loy, loz);
-    stage_box<B::EO>(s_old[2], a.old[2], g.ni, g.nj, g.nk + 1, lox, loy, loz);
+    float m[3], mo[3];
+    {
+        float *const bc[3] = {s_cur[0], s_cur[1], s_cur[2]};
+        float *const bo[3] = {s_old[0], s_old[1], s_old[2]};
+        stage_mac_boxes<B::E>(bc, g.f, g.ni, g.nj, g.nk, lx, ly, lz, m);
+        stage_mac_boxes<B::EO>(bo, a.old, g.ni, g.nj, g.nk, lox, loy, loz, mo);
+    }
 #pragma unroll
     for (int c = 0; c < 3; c++) {
         float v = m[c];

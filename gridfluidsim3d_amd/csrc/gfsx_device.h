@@ -413,25 +413,74 @@ __device__ __forceinline__ void brick_sample_in(const MacGrid &g,
     }
 }
 
+// One thread's share of a box of edge E with origin (ox,oy,oz) in a component
+// array with dims (w,h,d): the loads are issued here, stored by
+// BoxShare::store.  Cells outside the array are 0.
+template <int E>
+struct BoxShare {
+    static constexpr int N = E * E * E;
+    static constexpr int PER = (N + gbrick::NT - 1) / gbrick::NT;
+    float v[PER];
+    __device__ __forceinline__ void load(const float *__restrict__ f, int w, int h, int d, int ox,
+                                         int oy, int oz) {
+#pragma unroll
+        for (int m = 0; m < PER; m++) {
+            const int e = (int)threadIdx.x + m * gbrick::NT;
+            const int ii = e % E, jj = (e / E) % E, kk = e / (E * E);
+            const int gi = ox + ii, gj = oy + jj, gk = oz + kk;
+            const bool in = e < N && (unsigned)gi < (unsigned)w && (unsigned)gj < (unsigned)h &&
+                            (unsigned)gk < (unsigned)d;
+            // an element outside the array reads element 0 and is replaced by 0
+            const size_t idx =
+                in ? (size_t)gi + (size_t)w * ((size_t)gj + (size_t)h * (size_t)gk) : 0;
+            v[m] = gld(f, idx);
+            v[m] = in ? v[m] : 0.0f;
+        }
+    }
+    // Stores into the LDS box; returns the largest |value| stored (NaN if a
+    // NaN was).
+    __device__ __forceinline__ float store(float *__restrict__ box) const {
+        float mx = 0.0f;
+#pragma unroll
+        for (int m = 0; m < PER; m++) {
+            const int e = (int)threadIdx.x + m * gbrick::NT;
+            if (e < N) {
+                box[e] = v[m];
+                // NaN stays: fmaxf would drop it
+                mx = (fabsf(v[m]) > mx || v[m] != v[m]) ? fabsf(v[m]) : mx;
+            }
+        }
+        return mx;
+    }
+};
+
 // Copy the box of edge E with origin (ox,oy,oz) of a component array with
 // dims (w,h,d) into LDS; cells outside the array are 0.  Returns the largest
-// |value| this thread staged (NaN if it staged a NaN).
+// |value| this thread staged (NaN if it staged a NaN).  Every global load a
+// thread makes goes out before the first LDS store.
 template <int E>
 __device__ __forceinline__ float stage_box(float *__restrict__ box,
                                            const float *__restrict__ f, int w,
                                            int h, int d, int ox, int oy, int oz) {
-    float m = 0.0f;
-    for (int e = threadIdx.x; e < E * E * E; e += gbrick::NT) {
-        const int ii = e % E, jj = (e / E) % E, kk = e / (E * E);
-        const int gi = ox + ii, gj = oy + jj, gk = oz + kk;
-        float v = 0.0f;
-        if ((unsigned)gi < (unsigned)w && (unsigned)gj < (unsigned)h && (unsigned)gk < (unsigned)d)
-            v = f[(int64_t)gi + (int64_t)w * ((int64_t)gj + (int64_t)h * gk)];
-        box[e] = v;
-        // NaN stays: fmaxf would drop it
-        m = (fabsf(v) > m || v != v) ? fabsf(v) : m;
-    }
-    return m;
+    BoxShare<E> b;
+    b.load(f, w, h, d, ox, oy, oz);
+    return b.store(box);
+}
+
+// The three component boxes of a MAC grid (dims (ni+1,nj,nk), (ni,nj+1,nk),
+// (ni,nj,nk+1)) with one origin, every load of the three issued first: one
+// memory round trip for all three.  m[c] receives stage_box's maximum.
+template <int E>
+__device__ __forceinline__ void stage_mac_boxes(float *const box[3], const float *const f[3],
+                                                int ni, int nj, int nk, int ox, int oy, int oz,
+                                                float m[3]) {
+    BoxShare<E> b0, b1, b2;
+    b0.load(f[0], ni + 1, nj, nk, ox, oy, oz);
+    b1.load(f[1], ni, nj + 1, nk, ox, oy, oz);
+    b2.load(f[2], ni, nj, nk + 1, ox, oy, oz);
+    m[0] = b0.store(box[0]);
+    m[1] = b1.store(box[1]);
+    m[2] = b2.store(box[2]);
 }
 
 // PIC/FLIP blend and RK4 for one particle, given its new-field and

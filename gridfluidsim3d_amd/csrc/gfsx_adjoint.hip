@@ -349,20 +349,35 @@ __global__ __launch_bounds__(kBlock) void k_adj_advect_brick(StepAdjoint a, Adjo
     // cell c touches faces c - 1 .. c + 1 per axis, stages move up to M cells
     const int ox = bx * BX - 2 - M, oy = by * BY - 2 - M, oz = bz * BZ - 2 - M;
     const int ni = a.mac.ni, nj = a.mac.nj, nk = a.mac.nk;
-    for (int e = tid; e < 6 * TN; e += kBlock) {
-        const int arr = e / TN, l = e - arr * TN;
-        const int c = arr % 3;
-        const int i = ox + l % TX, j = oy + (l / TX) % TY, k = oz + l / (TX * TY);
-        const int w = ni + (c == 0), h = nj + (c == 1), d = nk + (c == 2);
-        float v = 0.0f;
-        if ((unsigned)i < (unsigned)w && (unsigned)j < (unsigned)h && (unsigned)k < (unsigned)d) {
-            const uint32_t o = (uint32_t)i + (uint32_t)w * ((uint32_t)j + (uint32_t)h * (uint32_t)k);
+    {
+        // every global load of this thread's share goes out before the
+        // first LDS store (one memory round trip per work-group)
+        constexpr int PER = (6 * TN + kBlock - 1) / kBlock;
+        float v[PER];
+#pragma unroll
+        for (int m = 0; m < PER; m++) {
+            const int e = tid + m * kBlock;
+            const int arr = e / TN, l = e - arr * TN;
+            const int c = arr % 3;
+            const int i = ox + l % TX, j = oy + (l / TX) % TY, k = oz + l / (TX * TY);
+            const int w = ni + (c == 0), h = nj + (c == 1), d = nk + (c == 2);
+            const bool in = e < 6 * TN && (unsigned)i < (unsigned)w && (unsigned)j < (unsigned)h &&
+                            (unsigned)k < (unsigned)d;
+            const uint32_t o =
+                in ? (uint32_t)i + (uint32_t)w * ((uint32_t)j + (uint32_t)h * (uint32_t)k) : 0u;
             const float *src = c == 0 ? (arr < 3 ? a.g[0] : a.old[0])
                              : c == 1 ? (arr < 3 ? a.g[1] : a.old[1]) : (arr < 3 ? a.g[2] : a.old[2]);
-            v = src[o];
+            v[m] = gld(src, o);
+            v[m] = in ? v[m] : 0.0f;
         }
-        gt[e] = v;
-        if (SCATTER) at[e] = 0ull;
+#pragma unroll
+        for (int m = 0; m < PER; m++) {
+            const int e = tid + m * kBlock;
+            if (e < 6 * TN) {
+                gt[e] = v[m];
+                if (SCATTER) at[e] = 0ull;
+            }
+        }
     }
     __syncthreads();
 

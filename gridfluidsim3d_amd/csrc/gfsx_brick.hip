@@ -964,22 +964,24 @@ __global__ __launch_bounds__(kBlock) void k_p2g_bcombine(
         const uint32_t fb0 = ((uint32_t)(bx0 >> 1) << 3) | (uint32_t)(bx0 & 1);
         const uint32_t fb1 = ((uint32_t)(bx1 >> 1) << 3) | (uint32_t)(bx1 & 1);
         const size_t e0 = (size_t)(bx0 * sx + ax.tp[0]), e1 = (size_t)(bx1 * sx + ax.tp[1]);
-        // every flag and entry load goes out before any is used
+        // every flag load goes out first, then the entries of the bricks
+        // that hold particles (a brick without particles wrote no tile)
         uint32_t f0[2][2], f1[2][2];
+#pragma unroll
+        for (int iz = 0; iz < 2; iz++)
+#pragma unroll
+            for (int iy = 0; iy < 2; iy++) {
+                const bool ok = iz < az.cnt && iy < ay.cnt;
+                f0[iz][iy] = ok ? (uint32_t)gld(tile_flag, brow[iz][iy] + fb0) : 0u;
+                f1[iz][iy] = ok && two ? (uint32_t)gld(tile_flag, brow[iz][iy] + fb1) : 0u;
+            }
         unsigned long long v0[2][2], v1[2][2];   // (S, W) float pairs
 #pragma unroll
         for (int iz = 0; iz < 2; iz++)
 #pragma unroll
             for (int iy = 0; iy < 2; iy++) {
-                f0[iz][iy] = f1[iz][iy] = 0u;
-                if (iz < az.cnt && iy < ay.cnt) {
-                    f0[iz][iy] = gld(tile_flag, brow[iz][iy] + fb0);
-                    v0[iz][iy] = gld(te, trow[iz][iy] + e0);
-                    if (two) {
-                        f1[iz][iy] = gld(tile_flag, brow[iz][iy] + fb1);
-                        v1[iz][iy] = gld(te, trow[iz][iy] + e1);
-                    }
-                }
+                v0[iz][iy] = f0[iz][iy] ? gld(te, trow[iz][iy] + e0) : 0ull;
+                v1[iz][iy] = f1[iz][iy] ? gld(te, trow[iz][iy] + e1) : 0ull;
             }
         // the entries added lower brick first: z, then y, then x
         float S = 0.0f, W = 0.0f;

@@ -19,6 +19,7 @@ import copy
 # option -> candidate values (direct variants take one scene only)
 CANDIDATES = {
     "p2g_variant": (4, 0, 2, 3),
+    "g2p_brick": (8, 4),
 }
 
 

@@ -67,6 +67,7 @@ struct gfsx_ctx {
     size_t bytes[NSLOT] = {};
     int p2g_variant = 4;
     int g2p_variant = 0;
+    int g2p_brick = 8;
     int adjoint_variant = 0;
     bool timing = false;
     std::vector<std::array<hipEvent_t, 4>> events;
@@ -523,6 +524,7 @@ std::vector<gfsx::SceneDesc> flip_step_impl(gfsx_ctx *ctx, const gfsx_scene *sce
     s.keep_order = params[0].order == GFSX_ORDER_KEEP;
     s.p2g_variant = ctx->p2g_variant;
     s.g2p_variant = ctx->g2p_variant;
+    s.g2p_brick = ctx->g2p_brick;
     s.grids_only = grids_only;
     hipEvent_t *ev = ctx->timing && !carve ? ctx->next_events() : nullptr;
     if (ctx->p2g_variant == 4) {
@@ -1064,6 +1066,9 @@ void gfsx_set_option(gfsx_ctx *ctx, const char *name, int value, int *err) {
         } else if (std::strcmp(name, "g2p_variant") == 0) {
             require(value >= 0 && value <= 1, "gfsx_set_option: g2p_variant must be 0..1");
             ctx->g2p_variant = value;
+        } else if (std::strcmp(name, "g2p_brick") == 0) {
+            require(value == 8 || value == 4, "gfsx_set_option: g2p_brick must be 8 or 4");
+            ctx->g2p_brick = value;
         } else if (std::strcmp(name, "adjoint_variant") == 0) {
             require(value >= 0 && value <= 1, "gfsx_set_option: adjoint_variant must be 0..1");
             ctx->adjoint_variant = value;
@@ -1079,6 +1084,7 @@ int gfsx_get_option(gfsx_ctx *ctx, const char *name, int *err) {
         require(name != nullptr, "gfsx_get_option: null name");
         if (std::strcmp(name, "p2g_variant") == 0) v = ctx->p2g_variant;
         else if (std::strcmp(name, "g2p_variant") == 0) v = ctx->g2p_variant;
+        else if (std::strcmp(name, "g2p_brick") == 0) v = ctx->g2p_brick;
         else if (std::strcmp(name, "adjoint_variant") == 0) v = ctx->adjoint_variant;
         else throw Error(std::string("gfsx_get_option: unknown option ") + name);
     });

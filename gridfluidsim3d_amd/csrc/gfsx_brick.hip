@@ -558,10 +558,12 @@ __global__ __launch_bounds__(kBlock) void k_bsort_scatter(const SceneDesc *__res
 // bits of its float product whatever the other particles' speeds.  The
 // scales are powers of two per brick: W terms K * 2^eW with N * 2^eW <= 2^62
 // (N particles, K <= 1), S terms K |v| * 2^eS with N * max|v| * 2^eS <= 2^62,
-// so no sum overflows.  A velocity that is not finite, or of magnitude
-// 2^90 or more, is not added: it marks the nodes within its radius NaN,
-// +inf or -inf, what the reference's float sum gives there (the reference
-// adds only inside the radius, scalarfield.cpp:191-196).
+// so no sum overflows.  The 32 fractional bits matter: a face reached only
+// at the edge of the radius has a tiny W, and S / W keeps its precision only
+// while the resolution stays far below W.  A velocity that is not finite,
+// or of magnitude 2^90 or more, is not added: it marks the nodes within its
+// radius NaN, +inf or -inf, what the reference's float sum gives there (the
+// reference adds only inside the radius, scalarfield.cpp:191-196).
 namespace bscat {
 constexpr int TN = 5 * 6 * 6;   // nodes per component tile
 constexpr int NREP = 2;         // tile copies (by lane parity)
@@ -611,8 +613,8 @@ __device__ __forceinline__ unsigned long long fix64(float y) {
 // Add one component of one particle.  acc: the component's three sums
 // (S of v >= 0, S of v < 0, W) of this lane's tile copy; (tx,ty,tz) the tile
 // position of corner (0,0,0); q*[0..1] the squared distances over R^2 to the
-// two nodes per axis.  kw = K * 2^eW / 2^32 = a^2 (c0 + c1 a), a = 1 - t
-// clamped to [0,1]; ratio = 2^(eS - eW).  FAST: every corner in the tile and
+// two nodes per axis.  kw = K * 2^eW / 2^32 = a^2 (c0 + c1 a), a = 1 - t;
+// ratio = 2^(eS - eW).  FAST: every corner in the tile and
 // v finite.
 template <int C, bool FAST>
 __device__ __forceinline__ void bscatter_comp(unsigned long long *acc, uint32_t *sbad, int tx,
@@ -1021,11 +1023,11 @@ __global__ __launch_bounds__(kBlock) void k_p2g_bcombine(
 
 // PIC/FLIP blend (fluidsimulation.cpp:3115-3127) and RK4
 // (particleadvector.cpp:1045-1054) of one particle through the staged
-// boxes, for a brick whose field keeps every RK stage within 0.99 cells of
-// the start.  Returns false, computing nothing, when the start position
-// lies in the grid but outside the brick's cells: then a stage may leave
-// the boxes.
-template <int ST>
+// boxes of a brick of GBE^3 cells, for a brick whose field keeps every RK
+// stage within 0.99 cells of the start.  Returns false, computing nothing,
+// when the start position lies in the grid but outside the brick's cells:
+// then a stage may leave the boxes.
+template <int ST, int GBE>
 __device__ __forceinline__ bool g2p_particle_in(const StepG2P &a, const MacGrid &g,
                                                 const float *const cur[3],
                                                 const float *const old[3], int lx, int ly,
@@ -1038,21 +1040,21 @@ __device__ __forceinline__ bool g2p_particle_in(const StepG2P &a, const MacGrid 
         const int ix = (int)floor((double)x0[0] * g.inv_dx) - c0x;
         const int iy = (int)floor((double)x0[1] * g.inv_dx) - c0y;
         const int iz = (int)floor((double)x0[2] * g.inv_dx) - c0z;
-        if (!((unsigned)ix < (unsigned)gbrick::GB && (unsigned)iy < (unsigned)gbrick::GB &&
-              (unsigned)iz < (unsigned)gbrick::GB))
+        if (!((unsigned)ix < (unsigned)GBE && (unsigned)iy < (unsigned)GBE &&
+              (unsigned)iz < (unsigned)GBE))
             return false;
     }
     // outside the grid every sample is 0 and the stages stay at x0
     float k1[3], vo[3];
-    brick_sample_in<ST, true>(g, cur[0], cur[1], cur[2], old[0], old[1], old[2], lx, ly, lz, lox,
-                              loy, loz, x0[0], x0[1], x0[2], k1, vo);
+    brick_sample_in<ST, true, GBE>(g, cur[0], cur[1], cur[2], old[0], old[1], old[2], lx, ly, lz,
+                                   lox, loy, loz, x0[0], x0[1], x0[2], k1, vo);
     flip_blend(a, v0, k1, vo, vnew);
     float xq[3], k2[3], k3[3], k4[3], unused[3];
     auto stage = [&](const float h, const float k[3], float out[3]) {
 #pragma unroll
         for (int c = 0; c < 3; c++) xq[c] = x0[c] + h * k[c];
-        brick_sample_in<ST, false>(g, cur[0], cur[1], cur[2], nullptr, nullptr, nullptr, lx, ly,
-                                   lz, 0, 0, 0, xq[0], xq[1], xq[2], out, unused);
+        brick_sample_in<ST, false, GBE>(g, cur[0], cur[1], cur[2], nullptr, nullptr, nullptr, lx,
+                                        ly, lz, 0, 0, 0, xq[0], xq[1], xq[2], out, unused);
     };
     stage(a.rk.half, k1, k2);
     stage(a.rk.half, k2, k3);
@@ -1066,7 +1068,7 @@ __device__ __forceinline__ bool g2p_particle_in(const StepG2P &a, const MacGrid 
 // The same for any start position, every sample checked against its box
 // (brick_sample: a stencil that leaves the box reads global memory; the
 // same bits as the boxes give).
-template <int ST>
+template <int ST, int GBE>
 __device__ __forceinline__ void g2p_particle_checked(const StepG2P &a, const MacGrid &g,
                                                      const float *const cur[3],
                                                      const float *const old[3], int lx, int ly,
@@ -1074,16 +1076,17 @@ __device__ __forceinline__ void g2p_particle_checked(const StepG2P &a, const Mac
                                                      const float x0[3], const float v0[3],
                                                      float x1[3], float vnew[3]) {
     float k1[3], vo[3];
-    brick_sample<ST, true>(g, a.old[0], a.old[1], a.old[2], cur[0], cur[1], cur[2], old[0],
-                           old[1], old[2], lx, ly, lz, lox, loy, loz, x0[0], x0[1], x0[2], k1, vo);
+    brick_sample<ST, true, GBE>(g, a.old[0], a.old[1], a.old[2], cur[0], cur[1], cur[2], old[0],
+                                old[1], old[2], lx, ly, lz, lox, loy, loz, x0[0], x0[1], x0[2],
+                                k1, vo);
     flip_blend(a, v0, k1, vo, vnew);
     float xq[3], k2[3], k3[3], k4[3], unused[3];
     auto stage = [&](const float h, const float k[3], float out[3]) {
 #pragma unroll
         for (int c = 0; c < 3; c++) xq[c] = x0[c] + h * k[c];
-        brick_sample<ST, false>(g, nullptr, nullptr, nullptr, cur[0], cur[1], cur[2], nullptr,
-                                nullptr, nullptr, lx, ly, lz, 0, 0, 0, xq[0], xq[1], xq[2], out,
-                                unused);
+        brick_sample<ST, false, GBE>(g, nullptr, nullptr, nullptr, cur[0], cur[1], cur[2],
+                                     nullptr, nullptr, nullptr, lx, ly, lz, 0, 0, 0, xq[0], xq[1],
+                                     xq[2], out, unused);
     };
     stage(a.rk.half, k1, k2);
     stage(a.rk.half, k2, k3);
@@ -1106,26 +1109,25 @@ __device__ __forceinline__ float nan_max(float a, float b) {
     return (a != a || b != b) ? __builtin_nanf("") : fmaxf(a, b);
 }
 
-// One work-group per G2P brick: the particles of its 8 P2G bricks, slots
-// [q0, q1) of the brick sort.  When the largest |value| of the staged new
-// field keeps every RK stage within 0.99 cells of the start position
-// (trilinear weights are convex; Catmull-Rom weights sum to at most 1.25 in
-// magnitude per axis, so at most 2 overall), a start cell inside the brick
-// keeps every stencil inside the staged boxes and no sample tests them.
-// Otherwise every sample is checked against its box.
-template <int ST>
-__device__ __forceinline__ void g2p_bbrick(const StepG2P &a, const MacGrid &g, int gbx, int gby,
-                                           int lg, uint32_t q0, uint32_t q1,
+// One work-group per G2P brick of GBE^3 cells from cell (c0x, c0y, c0z):
+// its particles are slots [q0, q1) of the brick sort (8 consecutive bins
+// for GBE = 8, one bin for GBE = 4).  When the largest |value| of the
+// staged new field keeps every RK stage within 0.99 cells of the start
+// position (trilinear weights are convex; Catmull-Rom weights sum to at
+// most 1.25 in magnitude per axis, so at most 2 overall), a start cell
+// inside the brick keeps every stencil inside the staged boxes and no
+// sample tests them.  Otherwise every sample is checked against its box.
+template <int ST, int GBE>
+__device__ __forceinline__ void g2p_bbrick(const StepG2P &a, const MacGrid &g, int c0x, int c0y,
+                                           int c0z, uint32_t q0, uint32_t q1,
                                            const uint32_t *__restrict__ perm, G2PSlow slow) {
     using namespace gbrick;
-    using B = Box<ST>;
+    using B = Box<ST, GBE>;
     __shared__ float s_cur[3][B::E * B::E * B::E];
     __shared__ float s_old[3][B::EO * B::EO * B::EO];
     __shared__ float s_max[NT / 64][3];
     if (q1 == q0) return;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const int gx = lg % gbx, gy = (lg / gbx) % gby, gz = lg / (gbx * gby);
-    const int c0x = gx * GB, c0y = gy * GB, c0z = gz * GB;
     const int lx = c0x - B::LO, ly = c0y - B::LO, lz = c0z - B::LO;
     const int lox = c0x - B::LOO, loy = c0y - B::LOO, loz = c0z - B::LOO;
     float m[3], mo[3];
@@ -1160,7 +1162,8 @@ __device__ __forceinline__ void g2p_bbrick(const StepG2P &a, const MacGrid &g, i
             const float x0[3] = {gld(a.xin[0], q), gld(a.xin[1], q), gld(a.xin[2], q)};
             const float v0[3] = {gld(a.vin[0], q), gld(a.vin[1], q), gld(a.vin[2], q)};
             float x1[3], vnew[3];
-            g2p_particle_checked<ST>(a, g, cur, old, lx, ly, lz, lox, loy, loz, x0, v0, x1, vnew);
+            g2p_particle_checked<ST, GBE>(a, g, cur, old, lx, ly, lz, lox, loy, loz, x0, v0, x1,
+                                          vnew);
             store_particle(a, perm, q, x1, vnew);
         }
         return;
@@ -1185,8 +1188,8 @@ __device__ __forceinline__ void g2p_bbrick(const StepG2P &a, const MacGrid &g, i
             }
         }
         float x1[3], vnew[3];
-        const bool done = g2p_particle_in<ST>(a, g, cur, old, lx, ly, lz, lox, loy, loz, c0x, c0y,
-                                              c0z, x0, v0, x1, vnew);
+        const bool done = g2p_particle_in<ST, GBE>(a, g, cur, old, lx, ly, lz, lox, loy, loz, c0x,
+                                                   c0y, c0z, x0, v0, x1, vnew);
         if (done) store_particle(a, perm, q, x1, vnew);
         // the rest to the slow list, one atomic per wave (by the lowest lane
         // that has one: lanes past the end have left the loop)
@@ -1201,7 +1204,7 @@ __device__ __forceinline__ void g2p_bbrick(const StepG2P &a, const MacGrid &g, i
     }
 }
 
-// One work-group per G2P brick of any scene.
+// One work-group per G2P brick of 8^3 cells of any scene.
 template <int ST>
 __device__ __forceinline__ void g2p_bb_kernel(const SceneDesc *__restrict__ scenes, int nsc,
                                               Sorted so, int keep_order, uint32_t nbrick,
@@ -1214,8 +1217,27 @@ __device__ __forceinline__ void g2p_bb_kernel(const SceneDesc *__restrict__ scen
     const StepG2P a = g2p_view(sd, so, keep_order);
     const MacGrid g = mac_view(sd);
     const int lg = (int)(gid - sd.g0);
+    const int gx = lg % sd.gbx, gy = (lg / sd.gbx) % sd.gby, gz = lg / (sd.gbx * sd.gby);
     const uint32_t b0 = sd.kb0 + 8u * (uint32_t)lg;
-    g2p_bbrick<ST>(a, g, sd.gbx, sd.gby, lg, pstart[b0], pstart[b0 + 8], perm, slow);
+    g2p_bbrick<ST, 8>(a, g, gx * 8, gy * 8, gz * 8, pstart[b0], pstart[b0 + 8], perm, slow);
+}
+
+// One work-group per P2G brick (4^3 cells, one bin of the sort) of any scene.
+template <int ST>
+__device__ __forceinline__ void g2p_b4_kernel(const SceneDesc *__restrict__ scenes, int nsc,
+                                              Sorted so, int keep_order, uint32_t nbin,
+                                              int per_xcd, const uint32_t *__restrict__ pstart,
+                                              const uint32_t *__restrict__ perm, G2PSlow slow) {
+    const uint32_t gid = (blockIdx.x & 7u) * (uint32_t)per_xcd + (blockIdx.x >> 3);
+    if (gid >= nbin) return;
+    const SceneDesc &sd = scenes[scene_of(scenes, nsc, gid, [](const SceneDesc &d) { return d.kb0; })];
+    const uint32_t lb = gid - sd.kb0;
+    if (lb >= sd.nbb) return;   // the overflow bin: k_flip_g2p_slow
+    int bx, by, bz;
+    brick_coords(sd, lb, bx, by, bz);
+    const StepG2P a = g2p_view(sd, so, keep_order);
+    const MacGrid g = mac_view(sd);
+    g2p_bbrick<ST, 4>(a, g, bx * 4, by * 4, bz * 4, pstart[gid], pstart[gid + 1], perm, slow);
 }
 
 // Trilinear: four work-groups per CU fit in LDS; four waves per SIMD.
@@ -1231,6 +1253,20 @@ __global__ __launch_bounds__(gbrick::NT, 2) void k_flip_g2p_bb_tc(
     int per_xcd, const uint32_t *__restrict__ pstart, const uint32_t *__restrict__ perm,
     G2PSlow slow) {
     g2p_bb_kernel<1>(scenes, nsc, so, keep_order, nbrick, per_xcd, pstart, perm, slow);
+}
+
+__global__ __launch_bounds__(gbrick::NT, 4) void k_flip_g2p_b4_tl(
+    const SceneDesc *__restrict__ scenes, int nsc, Sorted so, int keep_order, uint32_t nbin,
+    int per_xcd, const uint32_t *__restrict__ pstart, const uint32_t *__restrict__ perm,
+    G2PSlow slow) {
+    g2p_b4_kernel<0>(scenes, nsc, so, keep_order, nbin, per_xcd, pstart, perm, slow);
+}
+
+__global__ __launch_bounds__(gbrick::NT, 3) void k_flip_g2p_b4_tc(
+    const SceneDesc *__restrict__ scenes, int nsc, Sorted so, int keep_order, uint32_t nbin,
+    int per_xcd, const uint32_t *__restrict__ pstart, const uint32_t *__restrict__ perm,
+    G2PSlow slow) {
+    g2p_b4_kernel<1>(scenes, nsc, so, keep_order, nbin, per_xcd, pstart, perm, slow);
 }
 
 // The particles the brick kernel left, sampled from global memory
@@ -1364,10 +1400,18 @@ void flip_step_brick(const StepBatch &s, const BrickSort &bs, hipStream_t st, hi
         slow.slots = bs.slow_slots;
         slow.count = bs.slow_count;
         fill_u32(bs.slow_count, 0u, 1, st);
-        auto *kern = s.stencil == 0 ? k_flip_g2p_bb_tl : k_flip_g2p_bb_tc;
-        hipLaunchKernelGGL(kern, dim3((unsigned)(8 * gper)), dim3(gbrick::NT), 0, st, s.scenes_dev,
-                           nsc, so, s.keep_order, nbrick, gper, (const uint32_t *)bs.pstart,
-                           (const uint32_t *)bs.perm, slow);
+        if (s.g2p_brick == 4) {
+            const int bper = (int)((nbin + 7) / 8);
+            auto *kern = s.stencil == 0 ? k_flip_g2p_b4_tl : k_flip_g2p_b4_tc;
+            hipLaunchKernelGGL(kern, dim3((unsigned)(8 * bper)), dim3(gbrick::NT), 0, st,
+                               s.scenes_dev, nsc, so, s.keep_order, nbin, bper,
+                               (const uint32_t *)bs.pstart, (const uint32_t *)bs.perm, slow);
+        } else {
+            auto *kern = s.stencil == 0 ? k_flip_g2p_bb_tl : k_flip_g2p_bb_tc;
+            hipLaunchKernelGGL(kern, dim3((unsigned)(8 * gper)), dim3(gbrick::NT), 0, st,
+                               s.scenes_dev, nsc, so, s.keep_order, nbrick, gper,
+                               (const uint32_t *)bs.pstart, (const uint32_t *)bs.perm, slow);
+        }
         GFSX_LAUNCH_CHECK();
         auto *slowk = s.stencil == 0 ? k_flip_g2p_slow<0> : k_flip_g2p_slow<1>;
         hipLaunchKernelGGL(slowk, dim3(kSlowBlocks), dim3(kBlock), 0, st, s.scenes_dev, nsc, so,

@@ -269,12 +269,13 @@ constexpr int GH = 1;        // halo cells for RK stage samples
 constexpr int NT = 256;      // threads per work-group
 constexpr int MAXROW = (GB + 2) * (GB + 2);   // (y,z) bin rows, pads included
 constexpr int NEXTRA = 64;   // work-groups for particles outside the bin grid
-template <int ST> struct Box {
-    static constexpr int S = ST;                        // stencil reach below the cell: 0 or 1
-    static constexpr int E = GB + 2 + 2 * GH + 2 * S;   // new-field box edge
-    static constexpr int LO = 1 + GH + S;               // new-field box origin: C0 - LO
-    static constexpr int EO = GB + 2 + 2 * S;           // snapshot box edge
-    static constexpr int LOO = 1 + S;                   // snapshot box origin: C0 - LOO
+// Boxes of a brick of GBE^3 cells (GBE = GB by default).
+template <int ST, int GBE = GB> struct Box {
+    static constexpr int S = ST;                         // stencil reach below the cell: 0 or 1
+    static constexpr int E = GBE + 2 + 2 * GH + 2 * S;   // new-field box edge
+    static constexpr int LO = 1 + GH + S;                // new-field box origin: C0 - LO
+    static constexpr int EO = GBE + 2 + 2 * S;           // snapshot box edge
+    static constexpr int LOO = 1 + S;                    // snapshot box origin: C0 - LOO
 };
 static_assert(MAXROW <= 2 * 64, "two waves scan the rows");
 }  // namespace gbrick
@@ -326,7 +327,7 @@ __device__ __forceinline__ bool in_box(const AxisCoords &cx, const AxisCoords &c
 // sample_mac with the LDS boxes as the first choice.  cur[c] / old[c] are
 // the staged boxes (edges E, EO; origins l*, lo*); g and o* are the global
 // arrays used when a stencil leaves its box.
-template <int ST, bool OLD>
+template <int ST, bool OLD, int GBE = gbrick::GB>
 __device__ __forceinline__ void brick_sample(const MacGrid &g, const float *o0,
                                              const float *o1, const float *o2,
                                              const float *__restrict__ cur0,
@@ -339,7 +340,7 @@ __device__ __forceinline__ void brick_sample(const MacGrid &g, const float *o0,
                                              int lox, int loy, int loz,
                                              float x, float y, float z,
                                              float out[3], float out_old[3]) {
-    using B = gbrick::Box<ST>;
+    using B = gbrick::Box<ST, GBE>;
     if (!(x >= 0.0f && y >= 0.0f && z >= 0.0f && x < g.limf[0] && y < g.limf[1] &&
           z < g.limf[2])) {
         out[0] = out[1] = out[2] = 0.0f;
@@ -376,7 +377,7 @@ __device__ __forceinline__ void brick_sample(const MacGrid &g, const float *o0,
 // boxes: one base index per box from the half-shifted cells, the unshifted
 // axis of each component one step further.  The same arithmetic as the
 // in-box branch of brick_sample, so the results are the same bits.
-template <int ST, bool OLD>
+template <int ST, bool OLD, int GBE = gbrick::GB>
 __device__ __forceinline__ void brick_sample_in(const MacGrid &g,
                                                 const float *__restrict__ cur0,
                                                 const float *__restrict__ cur1,
@@ -388,7 +389,7 @@ __device__ __forceinline__ void brick_sample_in(const MacGrid &g,
                                                 int lox, int loy, int loz,
                                                 float x, float y, float z,
                                                 float out[3], float out_old[3]) {
-    using B = gbrick::Box<ST>;
+    using B = gbrick::Box<ST, GBE>;
     if (!(x >= 0.0f && y >= 0.0f && z >= 0.0f && x < g.limf[0] && y < g.limf[1] &&
           z < g.limf[2])) {
         out[0] = out[1] = out[2] = 0.0f;

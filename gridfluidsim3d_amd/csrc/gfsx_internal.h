@@ -432,6 +432,7 @@ struct StepBatch {
     int p2g_variant;   // 0, 2, 3: brick-staged gather, 1: direct global gather (one scene),
                        // 4: scatter into per-brick tiles + combine
     int g2p_variant;   // 0: brick-staged LDS boxes, 1: direct global sampling (one scene)
+    int g2p_brick;     // cells per edge of a G2P brick with g2p_variant 0 and p2g_variant 4: 8 or 4
     const FsEpilogue *epi = nullptr;   // FluidSimulation epilogue after P2G (one scene)
     bool grids_only = false;           // stop after P2G (the adjoint's recompute)
     const BrickSort *bs = nullptr;     // p2g_variant 4: the brick sort's workspace

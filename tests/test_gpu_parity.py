@@ -476,21 +476,24 @@ def _g2p_variant_scene(kind):
 @pytest.mark.parametrize("stencil", [TRILINEAR, TRICUBIC])
 @pytest.mark.parametrize("keep", [False, True])
 def test_step_g2p_variants_bit_identical(ctx, kind, stencil, keep):
-    """The LDS-staged G2P (0) and the direct one (1) give identical bits,
-    including samples that fall back to global memory and particles outside
-    the bin grid."""
+    """The LDS-staged G2P (0) over bricks of 8^3 and of 4^3 cells and the
+    direct one (1) give identical bits, including samples that fall back to
+    global memory and particles outside the bin grid."""
     pos, vel, dims, dx, dt = _g2p_variant_scene(kind)
-    old = ctx.get_option("g2p_variant")
+    old = ctx.get_option("g2p_variant"), ctx.get_option("g2p_brick")
     out = []
     try:
-        for variant in (0, 1):
+        for variant, brick in ((0, 8), (0, 4), (1, 8)):
             ctx.set_option("g2p_variant", variant)
+            ctx.set_option("g2p_brick", brick)
             out.append(_run_step(ctx, pos, vel, dims, dx, dt, (0.0, -9.8, 0.0), 0.05,
                                  stencil, int(keep)))
     finally:
-        ctx.set_option("g2p_variant", old)
-    for key in out[0]:
-        assert np.array_equal(out[0][key], out[1][key], equal_nan=True), key
+        ctx.set_option("g2p_variant", old[0])
+        ctx.set_option("g2p_brick", old[1])
+    for other in out[1:]:
+        for key in out[0]:
+            assert np.array_equal(out[0][key], other[key], equal_nan=True), key
 
 
 def test_step_dense_cells_use_fallback(ctx, orc):
@@ -522,8 +525,11 @@ def test_step_is_deterministic(ctx):
 def test_options(ctx):
     assert ctx.get_option("p2g_variant") == 4
     assert ctx.get_option("g2p_variant") == 0
+    assert ctx.get_option("g2p_brick") == 8
     with pytest.raises(RuntimeError, match="g2p_variant"):
         ctx.set_option("g2p_variant", 2)
+    with pytest.raises(RuntimeError, match="g2p_brick"):
+        ctx.set_option("g2p_brick", 6)
     with pytest.raises(RuntimeError, match="unknown option"):
         ctx.set_option("no_such_option", 1)
     with pytest.raises(RuntimeError, match="p2g_variant"):
@@ -595,6 +601,33 @@ def test_step_batched_against_oracle(ctx, orc):
             continue
         exp = orc.flip_step(p, v, d, h, 1 / 240, (0.0, -9.8, 0.0), 0.05, TRILINEAR)
         _check_step(_state_result(st), exp, False)
+
+
+@pytest.mark.parametrize("stencil", [TRILINEAR, TRICUBIC])
+def test_step_batched_g2p_bricks_agree(ctx, stencil):
+    """The batched step over G2P bricks of 4^3 cells gives the bits of the
+    one over 8^3-cell bricks, two steps in a row."""
+    import gridfluidsim3d_amd as gf
+    sc = _batch_scenes(71)
+    dts = [1 / 240, 1 / 120, 1 / 200, 1 / 300, 1 / 240]
+    mk = lambda p, v, d, h: gf.FlipState(p, v, d, h, track_ids=True, keep_old=True,
+                                         keep_weights=True)
+    runs = []
+    old = ctx.get_option("g2p_brick")
+    try:
+        for brick in (8, 4):
+            ctx.set_option("g2p_brick", brick)
+            states = [mk(*s) for s in sc]
+            for _ in range(2):
+                ctx.flip_step_batched(states, dts, (0.0, -9.8, 0.0), 0.05, stencil,
+                                      gf.ORDER_SORTED)
+            _torch().cuda.synchronize()
+            runs.append([_state_result(st) for st in states])
+    finally:
+        ctx.set_option("g2p_brick", old)
+    for i, (a, b) in enumerate(zip(*runs)):
+        for key in a:
+            assert np.array_equal(a[key], b[key], equal_nan=True), (i, key)
 
 
 def test_step_batched_rejects_direct_variants(ctx):

@@ -30,6 +30,8 @@
 #include "gfsx_sample.h"
 #include "gfsx_device.h"
 
+#include <type_traits>
+
 namespace gfsx {
 
 namespace bsort {
@@ -826,33 +828,37 @@ __global__ __launch_bounds__(bscat::NT) void k_p2g_bscatter(
     }
     __syncthreads();
 
-    // the copies summed, back to float, out to the tile grids
-    float *out = reinterpret_cast<float *>(tiles);
-    for (int e = t; e < 3 * TN; e += NT) {
-        const int c = e / TN, nd = e - c * TN;
-        unsigned long long Sp = 0, Sn = 0, W = 0;
+    // the copies summed, back to float, out to the tile grids: one
+    // component at a time, so the tile shape is a constant
+    auto flush = [&](auto cc) {
+        constexpr int C = decltype(cc)::value;
+        constexpr int SX = tsx(C), SY = tsy(C), SZ = tsz(C);
+        const TileGrid tg = tile_grid(sd, C);
+        const int es = eS[C];
+        for (int nd = t; nd < TN; nd += NT) {
+            unsigned long long Sp = 0, Sn = 0, W = 0;
 #pragma unroll
-        for (int r = 0; r < NREP; r++) {
-            Sp += s_acc[r][c][0][nd];
-            Sn += s_acc[r][c][1][nd];
-            W += s_acc[r][c][2][nd];
+            for (int r = 0; r < NREP; r++) {
+                Sp += s_acc[r][C][0][nd];
+                Sn += s_acc[r][C][1][nd];
+                W += s_acc[r][C][2][nd];
+            }
+            // the sums hold K |v| 2^eS and K 2^eW
+            float Sf = (float)ldexp((double)(long long)(Sp - Sn), -es);
+            const float Wf = (float)ldexp((double)W, -eW);
+            const int e = C * TN + nd;
+            const uint32_t badb = (s_bad[e >> 3] >> ((e & 7) * 4)) & 0xFu;
+            if (badb) {
+                if ((badb & BAD_NAN) || ((badb & BAD_PINF) && (badb & BAD_NINF))) Sf = __builtin_nanf("");
+                else Sf = (badb & BAD_PINF) ? INFINITY : -INFINITY;
+            }
+            const int tz = nd / (SX * SY), ty = (nd / SX) % SY, tx = nd % SX;
+            tiles[tile_at(tg, bxi * SX + tx, byi * SY + ty, bzi * SZ + tz)] = make_float2(Sf, Wf);
         }
-        const int es = c == 0 ? eS[0] : (c == 1 ? eS[1] : eS[2]);
-        // the sums hold K |v| 2^eS and K 2^eW
-        float Sf = (float)ldexp((double)(long long)(Sp - Sn), -es);
-        const float Wf = (float)ldexp((double)W, -eW);
-        const uint32_t badb = (s_bad[e >> 3] >> ((e & 7) * 4)) & 0xFu;
-        if (badb) {
-            if ((badb & BAD_NAN) || ((badb & BAD_PINF) && (badb & BAD_NINF))) Sf = __builtin_nanf("");
-            else Sf = (badb & BAD_PINF) ? INFINITY : -INFINITY;
-        }
-        const int sx = tsx(c), sy = tsy(c);
-        const int tz = nd / (sx * sy), ty = (nd / sx) % sy, tx = nd % sx;
-        const TileGrid tg = tile_grid(sd, c);
-        const size_t q = tile_at(tg, bxi * sx + tx, byi * sy + ty, bzi * tsz(c) + tz);
-        out[2 * q] = Sf;
-        out[2 * q + 1] = Wf;
-    }
+    };
+    flush(std::integral_constant<int, 0>());
+    flush(std::integral_constant<int, 1>());
+    flush(std::integral_constant<int, 2>());
     if (t == 0) {
         tile_flag[gid] = 1;
         fluid[gid] = s_fluid;
@@ -864,7 +870,8 @@ __global__ __launch_bounds__(bscat::NT) void k_p2g_bscatter(
 // ---------------------------------------------------------------------------
 //
 // One wave per face row along x (component c, node row j, k), 64 faces at a
-// time in array order.  Node 4b+r of an axis lies in the tile of brick b at
+// time in array order; rows shorter than 64 faces are packed 64 / nx to a
+// wave.  Node 4b+r of an axis lies in the tile of brick b at
 // position r (own axis: nodes 4b .. 4b+4) or r+1 (across: 4b-1 .. 4b+4);
 // also in brick b-1's tile (r = 0: position 4 along the own axis, 5
 // across) or, across, in brick b+1's (r = 3: position 0).  The entries are
@@ -917,36 +924,32 @@ __device__ __forceinline__ bool cell_fluid(const SceneDesc &sd,
 }
 
 __global__ __launch_bounds__(kBlock) void k_p2g_bcombine(
-    const SceneDesc *__restrict__ scenes, int nsc, int64_t nrow,
+    const SceneDesc *__restrict__ scenes, int nsc, int64_t nwave,
     const float2 *__restrict__ tiles, const uint8_t *__restrict__ tile_flag,
     const unsigned long long *__restrict__ fluid) {
     using namespace bscat;
     const int lane = threadIdx.x & 63;
-    const int64_t row = (int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
-    if (row >= nrow) return;
-    const SceneDesc &sd = scenes[scene_of(scenes, nsc, (uint64_t)row, [](const SceneDesc &d) { return (uint64_t)d.r0; })];
-    int64_t lr = row - sd.r0;
+    const int64_t wv = (int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+    if (wv >= nwave) return;
+    const SceneDesc &sd = scenes[scene_of(scenes, nsc, (uint64_t)wv, [](const SceneDesc &d) { return (uint64_t)d.r0; })];
+    int64_t lw = wv - sd.r0;
+    const int c = lw >= sd.cw[1] ? 2 : (lw >= sd.cw[0] ? 1 : 0);
+    if (c > 0) lw -= sd.cw[c - 1];
     const int ni = sd.ni, nj = sd.nj, nk = sd.nk;
-    int c = 0;
-    int64_t rows_c = (int64_t)nj * nk;
-    if (lr >= rows_c) {
-        lr -= rows_c;
-        c = 1;
-        rows_c = (int64_t)(nj + 1) * nk;
-        if (lr >= rows_c) {
-            lr -= rows_c;
-            c = 2;
-        }
-    }
-    const int nyc = nj + (c == 1);
+    const int nxc = ni + (c == 0), nyc = nj + (c == 1), nzc = nk + (c == 2);
+    // rows shorter than a wave: 64 / nxc rows per wave, nxc lanes each
+    const int rpw = nxc < 64 ? 64 / nxc : 1;
+    const int sub = rpw > 1 ? lane / nxc : 0;
+    const int64_t lr = lw * rpw + sub;
+    if (sub >= rpw || lr >= (int64_t)nyc * nzc) return;
+    const int I0 = rpw > 1 ? lane - sub * nxc : lane, step = rpw > 1 ? nxc : 64;
     const int J = (int)(lr % nyc), K = (int)(lr / nyc);
-    const int nxc = ni + (c == 0);
     const AxisTiles ay = axis_tiles(J, sd.nby, c == 1), az = axis_tiles(K, sd.nbz, c == 2);
     const TileGrid tg = tile_grid(sd, c);
     const int sx = tsx(c), sy = tsy(c), sz = tsz(c);
-    // per (z, y) brick of the row (uniform over the wave): the tile row the
-    // face row reads, and the bin of the row's brick bx = 0 (brick_local:
-    // bx adds (bx >> 1) << 3 | (bx & 1))
+    // per (z, y) brick of the row: the tile row the face row reads, and the
+    // bin of the row's brick bx = 0 (brick_local: bx adds (bx >> 1) << 3 |
+    // (bx & 1))
     size_t trow[2][2];
     uint32_t brow[2][2];
 #pragma unroll
@@ -959,7 +962,7 @@ __global__ __launch_bounds__(kBlock) void k_p2g_bcombine(
         }
     const FaceOut fo = face_out(sd.p2g, c);
     const unsigned long long *te = reinterpret_cast<const unsigned long long *>(tiles);
-    for (int I = lane; I < nxc; I += 64) {
+    for (int I = I0; I < nxc; I += step) {
         const AxisTiles ax = axis_tiles(I, sd.nbx, c == 0);
         const bool two = ax.cnt == 2;
         const int bx0 = ax.b + ax.db[0], bx1 = two ? ax.b + ax.db[1] : bx0;

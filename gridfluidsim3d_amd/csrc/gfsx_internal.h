@@ -126,7 +126,8 @@ struct SceneDesc {
     // bin (local index nbb); kb0 is the first global one
     uint32_t kb0, nbb;
     int64_t f0;       // first global MAC face (faces of u, then v, then w, per scene)
-    int64_t r0;       // first global face row (the x-rows of u, then v, then w, per scene)
+    int64_t r0;       // first combine wave (over the x-rows of u, then v, then w)
+    int64_t cw[2];    // combine waves of the scene before its v rows and before its w rows
     // grid and step constants
     int ni, nj, nk;
     double dx;
@@ -375,7 +376,7 @@ struct SceneTotals {
     int64_t ng2p = 0;    // G2P bricks
     int64_t nface = 0;   // MAC faces
     int64_t nbb = 0;     // brick sort bins, overflow bins included
-    int64_t nrow = 0;    // face rows along x, all components
+    int64_t nrow = 0;    // combine waves (face rows along x, short rows packed), all components
 };
 
 // Fill the derived fields of every scene (bin grid, offsets p0/b0/blk0/k0/g0,

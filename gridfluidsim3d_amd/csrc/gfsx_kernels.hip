@@ -1915,7 +1915,17 @@ void layout_scenes(SceneDesc *sc, int nsc, SceneTotals &tot) {
         tot.np2g += (int64_t)d.nbx * d.nby * d.nbz;
         tot.ng2p += (int64_t)d.gbx * d.gby * d.gbz;
         tot.nbb += (int64_t)d.nbb + 1;
-        tot.nrow += (int64_t)d.nj * d.nk + (int64_t)(d.nj + 1) * d.nk + (int64_t)d.nj * (d.nk + 1);
+        // combine waves: one face row along x per wave, or 64 / nx rows when
+        // a row has fewer than 64 faces
+        int64_t waves[3];
+        for (int c = 0; c < 3; c++) {
+            const int64_t nx = d.ni + (c == 0), rows = (int64_t)(d.nj + (c == 1)) * (d.nk + (c == 2));
+            const int64_t rpw = nx < 64 ? 64 / nx : 1;
+            waves[c] = (rows + rpw - 1) / rpw;
+        }
+        d.cw[0] = waves[0];
+        d.cw[1] = waves[0] + waves[1];
+        tot.nrow += waves[0] + waves[1] + waves[2];
         tot.nface += (int64_t)(d.ni + 1) * d.nj * d.nk + (int64_t)d.ni * (d.nj + 1) * d.nk +
                      (int64_t)d.ni * d.nj * (d.nk + 1);
     }

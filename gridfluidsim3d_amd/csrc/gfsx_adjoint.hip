@@ -211,9 +211,10 @@ __global__ __launch_bounds__(kBlock) void k_adj_advect(StepAdjoint a) {
     }
 }
 
-// Bricks of BX x BY x BZ bins of the forward sort, one work-group each, in
-// two passes over the brick's particles (BY*BZ runs of sorted slots, one
-// per bin row).  A tile of every face index within M cells of the brick's
+// The P2G bricks (4x4x4 cells) of the forward step's brick sort, one
+// work-group each, in two passes over the brick's particles (one
+// contiguous range of sorted slots).  A tile of every face index within M
+// cells of the brick's
 // trilinear footprint holds the grid after the body force and the
 // normalised P2G result (0 for faces outside the arrays, which is what an
 // out-of-range sample reads).  Samples whose corners leave the tile (a stage
@@ -231,11 +232,10 @@ __global__ __launch_bounds__(kBlock) void k_adj_advect(StepAdjoint a) {
 //     so the grid adjoints are the same on every run; no face can overflow:
 //     it receives at most 5n terms of magnitude at most 2^61 / (5n).
 namespace abrick {
-constexpr int BX = 8, BY = 4, BZ = 4;
+constexpr int BX = 4, BY = 4, BZ = 4;   // the P2G bricks of the forward's brick sort
 constexpr int M = 1;
 constexpr int TX = BX + 2 + 2 * M, TY = BY + 2 + 2 * M, TZ = BZ + 2 + 2 * M;
 constexpr int TN = TX * TY * TZ;
-constexpr int NROW = BY * BZ;
 constexpr int NEXTRA = 16;
 }  // namespace abrick
 
@@ -252,12 +252,10 @@ __device__ __forceinline__ unsigned long long to_fixed(float v, double scale) {
 
 template <bool SCATTER>
 __global__ __launch_bounds__(kBlock) void k_adj_advect_brick(StepAdjoint a, AdjointSort so,
-                                                             int nbx, int nby, uint32_t nbrick) {
+                                                             uint32_t nbrick) {
     using namespace abrick;
     __shared__ float gt[6 * TN];                 // g (0-2) and n (3-5) on the tile
     __shared__ unsigned long long at[SCATTER ? 6 * TN : 1];   // fixed-point dL/dg, dL/dn
-    __shared__ uint32_t s_r0[NROW];
-    __shared__ uint32_t s_rs[NROW + 1];
     __shared__ float s_max[kBlock / 64];
     const int tid = threadIdx.x;
     const uint32_t bid = blockIdx.x;
@@ -307,7 +305,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_advect_brick(StepAdjoint a, Adjo
     };
     if (bid >= nbrick) {
         // overflow bin: grid-stride over its slots
-        const uint32_t q0 = so.start[so.bg.nbin], q1 = so.start[so.bg.nbin + 1];
+        const uint32_t q0 = so.pstart[so.nbb], q1 = so.pstart[so.nbb + 1];
         for (uint32_t q = q0 + (bid - nbrick) * kBlock + tid; q < q1; q += NEXTRA * kBlock) {
             const uint32_t p = so.perm[q];
             float x0[3], ax[3], av[3], lx[3];
@@ -322,32 +320,16 @@ __global__ __launch_bounds__(kBlock) void k_adj_advect_brick(StepAdjoint a, Adjo
         if (!SCATTER) block_max();
         return;
     }
-    const int bx = (int)(bid % (uint32_t)nbx);
-    const int by = (int)((bid / (uint32_t)nbx) % (uint32_t)nby);
-    const int bz = (int)(bid / ((uint32_t)nbx * (uint32_t)nby));
-    if (tid < NROW) {
-        const int bj = by * BY + tid % BY, bk = bz * BZ + tid / BY;
-        const int bi0 = bx * BX, bi1 = min(bi0 + BX, so.bg.b[0]);
-        uint32_t lo = 0, len = 0;
-        if (bj < so.bg.b[1] && bk < so.bg.b[2]) {
-            const uint32_t row = (uint32_t)so.bg.b[0] * ((uint32_t)bj + (uint32_t)so.bg.b[1] * (uint32_t)bk);
-            lo = so.start[row + bi0];
-            len = so.start[row + bi1] - lo;
-        }
-        s_r0[tid] = lo;
-        s_rs[tid + 1] = len;
-    }
-    __syncthreads();
-    if (tid == 0) {
-        s_rs[0] = 0;
-        for (int r = 0; r < NROW; r++) s_rs[r + 1] += s_rs[r];
-    }
-    __syncthreads();
-    const uint32_t total = s_rs[NROW];
+    // brick bid: bin bid of the sort, one contiguous range of slots
+    const uint32_t q0 = so.pstart[bid], total = so.pstart[bid + 1] - q0;
     if (total == 0) return;   // uniform across the work-group
-    // tile origin in face-index space: bin b holds cell b - 1; a particle in
-    // cell c touches faces c - 1 .. c + 1 per axis, stages move up to M cells
-    const int ox = bx * BX - 2 - M, oy = by * BY - 2 - M, oz = bz * BZ - 2 - M;
+    const uint32_t g = bid >> 3, sub = bid & 7u;
+    const int bx = (int)(2 * (g % (uint32_t)so.gbx) + (sub & 1u));
+    const int by = (int)(2 * ((g / (uint32_t)so.gbx) % (uint32_t)so.gby) + ((sub >> 1) & 1u));
+    const int bz = (int)(2 * (g / ((uint32_t)so.gbx * (uint32_t)so.gby)) + (sub >> 2));
+    // tile origin in face-index space: a particle in cell c touches faces
+    // c - 1 .. c + 1 per axis, stages move up to M cells
+    const int ox = bx * BX - 1 - M, oy = by * BY - 1 - M, oz = bz * BZ - 1 - M;
     const int ni = a.mac.ni, nj = a.mac.nj, nk = a.mac.nk;
     {
         // every global load of this thread's share goes out before the
@@ -480,10 +462,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_advect_brick(StepAdjoint a, Adjo
     // blocked mapping: thread t takes items [t*per, (t+1)*per), so the lanes
     // of a wave sit in different cells (fewer same-address LDS atomics)
     const uint32_t per = (total + kBlock - 1) / kBlock;
-    int r = 0;
     for (uint32_t it = tid * per, end = min(total, (uint32_t)(tid + 1) * per); it < end; it++) {
-        while (r + 1 < NROW && s_rs[r + 1] <= it) r++;
-        const uint32_t q = s_r0[r] + (it - s_rs[r]);
+        const uint32_t q = q0 + it;
         const uint32_t p = so.perm[q];
         float x0[3], ax[3], av[3], lx[3];
         for (int c = 0; c < 3; c++) {
@@ -528,8 +508,9 @@ __global__ __launch_bounds__(kBlock) void k_fixed_to_float(const unsigned long l
 // Faces of component c within the kernel radius R = dx of x: the node
 // (i,j,k) of component c sits at (i,j,k)*dx + off_c in the frame where the
 // reference splats (fluidsimulation.cpp:2526-2570).  A particle at cell
-// coordinate s along an axis reaches nodes in (s - 1, s + 1); the loop takes
-// floor(s) - 1 .. floor(s) + 2 and lets the kernel's support decide.
+// coordinate s along an axis reaches nodes in (s - 1, s + 1): floor(s) and
+// floor(s) + 1, 2x2x2 candidates per component; the kernel's support
+// decides which of them take weight.
 __global__ __launch_bounds__(kBlock) void k_adj_p2g_step(StepAdjoint a) {
     const int64_t p = blockIdx.x * (int64_t)kBlock + threadIdx.x;
     if (p >= a.n) return;
@@ -549,11 +530,14 @@ __global__ __launch_bounds__(kBlock) void k_adj_p2g_step(StepAdjoint a) {
         for (int ax = 0; ax < 3; ax++) {
             // p -= offset (scalarfield.cpp:168)
             ps[ax] = xp[ax] - (ax == c ? 0.0f : dxf_half);
+            // the nodes within R = dx of ps along this axis are floor(ps/dx)
+            // and the one after it (a node exactly dx away has t = 1 and
+            // takes no weight)
             const double s = floor((double)ps[ax] * a.mac.inv_dx);
-            // false for NaN; when true, s lies in [-2, dims] and converts safely
-            const bool any = s + 2.0 >= 0.0 && s - 1.0 <= (double)dims[ax] - 1.0;
-            lo[ax] = any ? (int)fmax(s - 1.0, 0.0) : 0;
-            hi[ax] = any ? (int)fmin(s + 2.0, (double)dims[ax] - 1.0) : -1;
+            // false for NaN; when true, s lies in [-1, dims - 1] and converts safely
+            const bool any = s + 1.0 >= 0.0 && s <= (double)dims[ax] - 1.0;
+            lo[ax] = any ? (int)fmax(s, 0.0) : 0;
+            hi[ax] = any ? (int)fmin(s + 1.0, (double)dims[ax] - 1.0) : -1;
         }
         const float *__restrict__ W = a.w[c];
         const float *__restrict__ N = a.old[c];
@@ -656,15 +640,13 @@ void flip_step_adjoint(const StepAdjoint &a, const AdjointSort *so, hipStream_t 
         fill_u32((uint32_t *)so->fx, 0u, 4 * tot, s);
         fill_u32(so->maxbits, 0u, 1, s);
         if (a.n > 0) {
-            const int nbx = (so->bg.b[0] + abrick::BX - 1) / abrick::BX;
-            const int nby = (so->bg.b[1] + abrick::BY - 1) / abrick::BY;
-            const int nbz = (so->bg.b[2] + abrick::BZ - 1) / abrick::BZ;
-            const uint32_t nbrick = (uint32_t)nbx * (uint32_t)nby * (uint32_t)nbz;
+            // one work-group per bin of the brick sort, then the overflow bin's
+            const uint32_t nbrick = so->nbb;
             hipLaunchKernelGGL(k_adj_advect_brick<false>, dim3(nbrick + abrick::NEXTRA),
-                               dim3(kBlock), 0, s, a, *so, nbx, nby, nbrick);
+                               dim3(kBlock), 0, s, a, *so, nbrick);
             GFSX_LAUNCH_CHECK();
             hipLaunchKernelGGL(k_adj_advect_brick<true>, dim3(nbrick + abrick::NEXTRA),
-                               dim3(kBlock), 0, s, a, *so, nbx, nby, nbrick);
+                               dim3(kBlock), 0, s, a, *so, nbrick);
             GFSX_LAUNCH_CHECK();
         }
         hipLaunchKernelGGL(k_fixed_to_float, dim3(blocks_for(2 * tot)), dim3(kBlock), 0, s,

@@ -106,8 +106,11 @@ struct gfsx_ctx {
     // alternates between a few tables (its buffers swap every step), so
     // after the first steps nothing is uploaded, and a step captured into a
     // graph holds no copy.
+    // Each device table holds the n descriptors followed by the n combine
+    // wave offsets (r0) as a dense array, which the combine searches.
     struct Table {
         std::vector<gfsx::SceneDesc> host;
+        std::vector<int64_t> r0;
         gfsx::SceneDesc *dev = nullptr;
         size_t cap = 0;
         uint64_t used = 0;
@@ -138,14 +141,18 @@ struct gfsx_ctx {
             if (slot->dev) GFSX_HIP(hipFree(slot->dev));
             slot->dev = nullptr;
             slot->cap = 0;
-            GFSX_HIP(hipMalloc(&slot->dev, std::max<size_t>(bytes, 1)));
+            GFSX_HIP(hipMalloc(&slot->dev, std::max<size_t>(bytes + sizeof(int64_t) * t.size(), 1)));
             slot->cap = t.size();
         } else if (slot->used) {
             // queued work may still read the table being replaced
             GFSX_HIP(hipStreamSynchronize(stream));
         }
         slot->host = t;
+        slot->r0.resize(t.size());
+        for (size_t i = 0; i < t.size(); i++) slot->r0[i] = t[i].r0;
         GFSX_HIP(hipMemcpyAsync(slot->dev, slot->host.data(), bytes, hipMemcpyHostToDevice, stream));
+        GFSX_HIP(hipMemcpyAsync(slot->dev + t.size(), slot->r0.data(), sizeof(int64_t) * t.size(),
+                                hipMemcpyHostToDevice, stream));
         slot->used = ++table_clock;
         return slot->dev;
     }
@@ -543,6 +550,7 @@ std::vector<gfsx::SceneDesc> flip_step_impl(gfsx_ctx *ctx, const gfsx_scene *sce
             s.scenes_dev = table_dev;
         } else {
             s.scenes_dev = ctx->table(s.scenes);
+            s.r0s = reinterpret_cast<const int64_t *>(s.scenes_dev + nscenes);
         }
         s.bs = &bs;
         if (fs) {
@@ -921,26 +929,25 @@ void gfsx_flip_step_backward(gfsx_ctx *ctx, const gfsx_scene *sc,
         f.vz_out = vo + 2 * n;
         f.id_in = nullptr;
         f.id_out = nullptr;
-        // (the sort, its sorted copies and the P2G grids; no particle update)
-        const std::vector<gfsx::SceneDesc> laid = flip_step_impl(ctx, &f, pr, 1, who, true);
-        if (ctx->p2g_variant == 4) {
-            // the adjoint's bricks take the particles in cell order: the cell
-            // sort of the step's inputs and the sorted copies
-            gfsx::SortBuffers sb = ctx->sort_buffers(laid, n, laid[0].bg.nbin + 1,
-                                                     (uint32_t)gfsx::sort_blocks(n));
-            gfsx::cell_sort(sb, n, ctx->stream, false);
-            float *xs[3], *vs[3];
-            for (int c = 0; c < 3; c++) {
-                xs[c] = (float *)ctx->buf[S_XS0 + c];
-                vs[c] = (float *)ctx->buf[S_VS0 + c];
-            }
-            gfsx::sorted_copy(sb.scenes, 1, sb.perm, n, xs, vs, ctx->stream);
+        // (the brick sort, its sorted copies and the P2G grids; no particle
+        // update), always by the scatter P2G, whose sort the adjoint reads
+        const int p2g_variant = ctx->p2g_variant;
+        ctx->p2g_variant = 4;
+        std::vector<gfsx::SceneDesc> laid;
+        try {
+            laid = flip_step_impl(ctx, &f, pr, 1, who, true);
+        } catch (...) {
+            ctx->p2g_variant = p2g_variant;
+            throw;
         }
+        ctx->p2g_variant = p2g_variant;
         gfsx::AdjointSort so;
-        so.start = (const uint32_t *)ctx->buf[S_START];
+        so.pstart = (const uint32_t *)ctx->buf[S_K_PSTART];
         so.perm = (const uint32_t *)ctx->buf[S_PERM];
         for (int c = 0; c < 3; c++) so.xs[c] = (const float *)ctx->buf[S_XS0 + c];
-        so.bg = laid[0].bg;
+        so.nbb = laid[0].nbb;
+        so.gbx = laid[0].gbx;
+        so.gby = laid[0].gby;
         so.fx = (unsigned long long *)ctx->get(S_B_FX, sizeof(unsigned long long) * 2 * tot);
         so.maxbits = ctx->u32(S_B_MAX, 1);
         gfsx::StepAdjoint a;

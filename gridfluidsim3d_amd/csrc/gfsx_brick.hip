@@ -869,9 +869,8 @@ __global__ __launch_bounds__(bscat::NT) void k_p2g_bscatter(
 // Combine: every face from the tiles that hold it
 // ---------------------------------------------------------------------------
 //
-// One wave per face row along x (component c, node row j, k), 64 faces at a
-// time in array order; rows shorter than 64 faces are packed 64 / nx to a
-// wave.  Node 4b+r of an axis lies in the tile of brick b at
+// One wave per combine_rows(nx) consecutive face rows along x (component c,
+// node rows j, k), each row 64 faces at a time in array order.  Node 4b+r of an axis lies in the tile of brick b at
 // position r (own axis: nodes 4b .. 4b+4) or r+1 (across: 4b-1 .. 4b+4);
 // also in brick b-1's tile (r = 0: position 4 along the own axis, 5
 // across) or, across, in brick b+1's (r = 3: position 0).  The entries are
@@ -924,99 +923,120 @@ __device__ __forceinline__ bool cell_fluid(const SceneDesc &sd,
 }
 
 __global__ __launch_bounds__(kBlock) void k_p2g_bcombine(
-    const SceneDesc *__restrict__ scenes, int nsc, int64_t nwave,
+    const SceneDesc *__restrict__ scenes, const int64_t *__restrict__ r0s, int nsc, int64_t nwave,
     const float2 *__restrict__ tiles, const uint8_t *__restrict__ tile_flag,
     const unsigned long long *__restrict__ fluid) {
     using namespace bscat;
     const int lane = threadIdx.x & 63;
-    const int64_t wv = (int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+    // the wave index, known to the compiler as the same for all lanes, so
+    // that the scene and row set-up below is scalar work
+    const int64_t wv = (int64_t)blockIdx.x * (kBlock / 64) +
+                       __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     if (wv >= nwave) return;
-    const SceneDesc &sd = scenes[scene_of(scenes, nsc, (uint64_t)wv, [](const SceneDesc &d) { return (uint64_t)d.r0; })];
+    // the scene: a search of the dense wave offsets (a few cache lines for
+    // every wave of the launch), not of the descriptors
+    int si = 0;
+    if (nsc > 1) {
+        int lo = 0, hi = nsc - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (r0s[mid] <= wv) lo = mid;
+            else hi = mid - 1;
+        }
+        si = lo;
+    }
+    const SceneDesc &sd = scenes[si];
     int64_t lw = wv - sd.r0;
     const int c = lw >= sd.cw[1] ? 2 : (lw >= sd.cw[0] ? 1 : 0);
     if (c > 0) lw -= sd.cw[c - 1];
     const int ni = sd.ni, nj = sd.nj, nk = sd.nk;
     const int nxc = ni + (c == 0), nyc = nj + (c == 1), nzc = nk + (c == 2);
-    // rows shorter than a wave: 64 / nxc rows per wave, nxc lanes each
-    const int rpw = nxc < 64 ? 64 / nxc : 1;
-    const int sub = rpw > 1 ? lane / nxc : 0;
-    const int64_t lr = lw * rpw + sub;
-    if (sub >= rpw || lr >= (int64_t)nyc * nzc) return;
-    const int I0 = rpw > 1 ? lane - sub * nxc : lane, step = rpw > 1 ? nxc : 64;
-    const int J = (int)(lr % nyc), K = (int)(lr / nyc);
-    const AxisTiles ay = axis_tiles(J, sd.nby, c == 1), az = axis_tiles(K, sd.nbz, c == 2);
+    // this wave's rows, in turn: [r, r + rpw) of the component's nyc * nzc;
+    // every index below is the same for all lanes
+    const int rpw = combine_rows(nxc);
+    const uint32_t nrows = (uint32_t)nyc * (uint32_t)nzc;
+    const uint32_t r0 = (uint32_t)lw * (uint32_t)rpw;
+    const uint32_t r1 = min(r0 + (uint32_t)rpw, nrows);
+    int J = (int)(r0 % (uint32_t)nyc), K = (int)(r0 / (uint32_t)nyc);
     const TileGrid tg = tile_grid(sd, c);
     const int sx = tsx(c), sy = tsy(c), sz = tsz(c);
-    // per (z, y) brick of the row: the tile row the face row reads, and the
-    // bin of the row's brick bx = 0 (brick_local: bx adds (bx >> 1) << 3 |
-    // (bx & 1))
-    size_t trow[2][2];
-    uint32_t brow[2][2];
-#pragma unroll
-    for (int iz = 0; iz < 2; iz++)
-#pragma unroll
-        for (int iy = 0; iy < 2; iy++) {
-            const int by = ay.b + ay.db[iy], bz = az.b + az.db[iz];
-            trow[iz][iy] = tile_at(tg, 0, by * sy + ay.tp[iy], bz * sz + az.tp[iz]);
-            brow[iz][iy] = sd.kb0 + brick_local(sd, 0, by, bz);
-        }
     const FaceOut fo = face_out(sd.p2g, c);
     const unsigned long long *te = reinterpret_cast<const unsigned long long *>(tiles);
-    for (int I = I0; I < nxc; I += step) {
-        const AxisTiles ax = axis_tiles(I, sd.nbx, c == 0);
-        const bool two = ax.cnt == 2;
-        const int bx0 = ax.b + ax.db[0], bx1 = two ? ax.b + ax.db[1] : bx0;
-        const uint32_t fb0 = ((uint32_t)(bx0 >> 1) << 3) | (uint32_t)(bx0 & 1);
-        const uint32_t fb1 = ((uint32_t)(bx1 >> 1) << 3) | (uint32_t)(bx1 & 1);
-        const size_t e0 = (size_t)(bx0 * sx + ax.tp[0]), e1 = (size_t)(bx1 * sx + ax.tp[1]);
-        // every flag load goes out first, then the entries of the bricks
-        // that hold particles (a brick without particles wrote no tile)
-        uint32_t f0[2][2], f1[2][2];
+    for (uint32_t r = r0; r < r1; r++) {
+        const AxisTiles ay = axis_tiles(J, sd.nby, c == 1), az = axis_tiles(K, sd.nbz, c == 2);
+        // per (z, y) brick of the row: the tile row the face row reads, and
+        // the bin of the row's brick bx = 0 (brick_local: bx adds
+        // (bx >> 1) << 3 | (bx & 1))
+        size_t trow[2][2];
+        uint32_t brow[2][2];
 #pragma unroll
         for (int iz = 0; iz < 2; iz++)
 #pragma unroll
             for (int iy = 0; iy < 2; iy++) {
-                const bool ok = iz < az.cnt && iy < ay.cnt;
-                f0[iz][iy] = ok ? (uint32_t)gld(tile_flag, brow[iz][iy] + fb0) : 0u;
-                f1[iz][iy] = ok && two ? (uint32_t)gld(tile_flag, brow[iz][iy] + fb1) : 0u;
+                const int by = ay.b + ay.db[iy], bz = az.b + az.db[iz];
+                trow[iz][iy] = tile_at(tg, 0, by * sy + ay.tp[iy], bz * sz + az.tp[iz]);
+                brow[iz][iy] = sd.kb0 + brick_local(sd, 0, by, bz);
             }
-        unsigned long long v0[2][2], v1[2][2];   // (S, W) float pairs
+        for (int I = lane; I < nxc; I += 64) {
+            const AxisTiles ax = axis_tiles(I, sd.nbx, c == 0);
+            const bool two = ax.cnt == 2;
+            const int bx0 = ax.b + ax.db[0], bx1 = two ? ax.b + ax.db[1] : bx0;
+            const uint32_t fb0 = ((uint32_t)(bx0 >> 1) << 3) | (uint32_t)(bx0 & 1);
+            const uint32_t fb1 = ((uint32_t)(bx1 >> 1) << 3) | (uint32_t)(bx1 & 1);
+            const size_t e0 = (size_t)(bx0 * sx + ax.tp[0]), e1 = (size_t)(bx1 * sx + ax.tp[1]);
+            // every flag load goes out first, then the entries of the bricks
+            // that hold particles (a brick without particles wrote no tile)
+            uint32_t f0[2][2], f1[2][2];
 #pragma unroll
-        for (int iz = 0; iz < 2; iz++)
+            for (int iz = 0; iz < 2; iz++)
 #pragma unroll
-            for (int iy = 0; iy < 2; iy++) {
-                v0[iz][iy] = f0[iz][iy] ? gld(te, trow[iz][iy] + e0) : 0ull;
-                v1[iz][iy] = f1[iz][iy] ? gld(te, trow[iz][iy] + e1) : 0ull;
-            }
-        // the entries added lower brick first: z, then y, then x
-        float S = 0.0f, W = 0.0f;
-#pragma unroll
-        for (int iz = 0; iz < 2; iz++)
-#pragma unroll
-            for (int iy = 0; iy < 2; iy++) {
-                if (f0[iz][iy]) {
-                    S += __uint_as_float((uint32_t)v0[iz][iy]);
-                    W += __uint_as_float((uint32_t)(v0[iz][iy] >> 32));
+                for (int iy = 0; iy < 2; iy++) {
+                    const bool ok = iz < az.cnt && iy < ay.cnt;
+                    f0[iz][iy] = ok ? (uint32_t)gld(tile_flag, brow[iz][iy] + fb0) : 0u;
+                    f1[iz][iy] = ok && two ? (uint32_t)gld(tile_flag, brow[iz][iy] + fb1) : 0u;
                 }
-                if (f1[iz][iy]) {
-                    S += __uint_as_float((uint32_t)v1[iz][iy]);
-                    W += __uint_as_float((uint32_t)(v1[iz][iy] >> 32));
+            unsigned long long v0[2][2], v1[2][2];   // (S, W) float pairs
+#pragma unroll
+            for (int iz = 0; iz < 2; iz++)
+#pragma unroll
+                for (int iy = 0; iy < 2; iy++) {
+                    v0[iz][iy] = f0[iz][iy] ? gld(te, trow[iz][iy] + e0) : 0ull;
+                    v1[iz][iy] = f1[iz][iy] ? gld(te, trow[iz][iy] + e1) : 0ull;
                 }
+            // the entries added lower brick first: z, then y, then x
+            float S = 0.0f, W = 0.0f;
+#pragma unroll
+            for (int iz = 0; iz < 2; iz++)
+#pragma unroll
+                for (int iy = 0; iy < 2; iy++) {
+                    if (f0[iz][iy]) {
+                        S += __uint_as_float((uint32_t)v0[iz][iy]);
+                        W += __uint_as_float((uint32_t)(v0[iz][iy] >> 32));
+                    }
+                    if (f1[iz][iy]) {
+                        S += __uint_as_float((uint32_t)v1[iz][iy]);
+                        W += __uint_as_float((uint32_t)(v1[iz][iy] >> 32));
+                    }
+                }
+            const int64_t f = (int64_t)I + (int64_t)nxc * ((int64_t)J + (int64_t)nyc * K);
+            float val = W > 0.0f ? S / W : S;
+            if (fo.old) fo.old[f] = val;
+            if (fo.wgt) fo.wgt[f] = W;
+            if (fo.grav_on) {
+                // the cells on either side along axis c
+                const int n = c == 0 ? I : (c == 1 ? J : K);
+                bool fl = false;
+                if (n < fo.nax) fl = cell_fluid(sd, fluid, I, J, K);
+                if (n > 0)
+                    fl = fl || cell_fluid(sd, fluid, I - (c == 0), J - (c == 1), K - (c == 2));
+                if (fl) val += fo.grav_delta;
             }
-        const int64_t f = (int64_t)I + (int64_t)nxc * ((int64_t)J + (int64_t)nyc * K);
-        float val = W > 0.0f ? S / W : S;
-        if (fo.old) fo.old[f] = val;
-        if (fo.wgt) fo.wgt[f] = W;
-        if (fo.grav_on) {
-            // the cells on either side along axis c
-            const int n = c == 0 ? I : (c == 1 ? J : K);
-            bool fl = false;
-            if (n < fo.nax) fl = cell_fluid(sd, fluid, I, J, K);
-            if (n > 0)
-                fl = fl || cell_fluid(sd, fluid, I - (c == 0), J - (c == 1), K - (c == 2));
-            if (fl) val += fo.grav_delta;
+            fo.grid[f] = val;
         }
-        fo.grid[f] = val;
+        if (++J == nyc) {
+            J = 0;
+            K++;
+        }
     }
 }
 
@@ -1301,10 +1321,11 @@ __global__ __launch_bounds__(kBlock) void k_flip_g2p_slow(
     const uint32_t ns = *slow.count;
     for (uint32_t e = t0; e < ns; e += stride)
         g2p_direct_one<ST>(scenes, nsc, so, keep_order, perm, slow.slots[e]);
-    for (int sc = 0; sc < nsc; sc++) {
+    // each scene's overflow bin by one work-group at a time
+    for (int sc = (int)blockIdx.x; sc < nsc; sc += (int)gridDim.x) {
         const uint32_t ovf = scenes[sc].kb0 + scenes[sc].nbb;
         const uint32_t q1 = pstart[ovf + 1];
-        for (uint32_t q = pstart[ovf] + t0; q < q1; q += stride)
+        for (uint32_t q = pstart[ovf] + threadIdx.x; q < q1; q += kBlock)
             g2p_direct_one<ST>(scenes, nsc, so, keep_order, perm, q);
     }
 }
@@ -1380,8 +1401,9 @@ void flip_step_brick(const StepBatch &s, const BrickSort &bs, hipStream_t st, hi
                        s.scenes_dev, nsc, nbin, per_xcd, (const uint32_t *)bs.pstart, s.tiles,
                        s.tile_flag, bs.fluid);
     GFSX_LAUNCH_CHECK();
+    if (nsc > 1 && !s.r0s) throw Error("gfsx: the combine of several scenes needs their wave offsets");
     hipLaunchKernelGGL(k_p2g_bcombine, dim3(grid_for(s.tot.nrow, kBlock / 64)), dim3(kBlock), 0, st,
-                       s.scenes_dev, nsc, s.tot.nrow, (const float2 *)s.tiles,
+                       s.scenes_dev, s.r0s, nsc, s.tot.nrow, (const float2 *)s.tiles,
                        (const uint8_t *)s.tile_flag, (const unsigned long long *)bs.fluid);
     GFSX_LAUNCH_CHECK();
     if (s.epi) fs_epilogue(*s.epi, st);

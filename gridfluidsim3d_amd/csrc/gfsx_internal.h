@@ -231,16 +231,20 @@ struct StepAdjoint {
     float inv_r2;                  // (float)(1/dx^2): P2G radius R = dx
 };
 
-// The cell sort of the forward step that the adjoint recomputes: bin
-// starts, the permutation (sorted slot -> particle) and the positions in
-// sorted order, over the scene's bin grid.
-// fx: fixed-point accumulators for the grid adjoints (int64, the face
-// arrays of dL/dg u,v,w then dL/dn u,v,w back to back); maxbits: one word
-// for the largest |sample adjoint| (float bits).
+// The brick sort of the forward step that the adjoint recomputes (one
+// scene): bin lb < nbb is the P2G brick of 4^3 cells numbered as the sort
+// numbers them (the 8 bins of a G2P brick consecutive, gbx x gby G2P bricks
+// per z layer), bin nbb the particles beyond the padded grid; its particles
+// are slots [pstart[lb], pstart[lb + 1]) of the sorted positions xs, and
+// perm maps a slot to its input index.  fx: fixed-point accumulators for
+// the grid adjoints (int64, the face arrays of dL/dg u,v,w then dL/dn u,v,w
+// back to back); maxbits: one word for the largest |sample adjoint| (float
+// bits).
 struct AdjointSort {
-    const uint32_t *start, *perm;
+    const uint32_t *pstart, *perm;
     const float *xs[3];
-    BinGrid bg;
+    uint32_t nbb;
+    int gbx, gby;
     unsigned long long *fx;
     uint32_t *maxbits;
 };
@@ -367,6 +371,10 @@ void g2p(const float *pos, int64_t n, const MacGrid &g, int stencil,
 void advect(const float *pos, int64_t n, const MacGrid &g, int order,
             double dt, int stencil, float *out, hipStream_t s);
 
+// Face rows along x that one wave of the combine takes in turn: enough for
+// about 256 faces, so that short rows share a wave's set-up.
+__host__ __device__ inline int combine_rows(int nx) { return nx < 256 ? 256 / nx : 1; }
+
 // Sizes of a scene table laid out by layout_scenes.
 struct SceneTotals {
     int64_t n = 0;       // particles
@@ -421,6 +429,7 @@ int64_t scan64_scratch_words(int64_t m);
 struct StepBatch {
     std::vector<SceneDesc> scenes;   // host copy, laid out
     const SceneDesc *scenes_dev;     // the same table on the device
+    const int64_t *r0s = nullptr;    // the scenes' first combine waves, dense (several scenes)
     SceneTotals tot;
     float *xs[3], *vs[3];            // workspace: all particles in sort order
     uint32_t *dense, *dense_count;   // workspace: P2G bricks over capacity (list; count

@@ -1915,12 +1915,12 @@ void layout_scenes(SceneDesc *sc, int nsc, SceneTotals &tot) {
         tot.np2g += (int64_t)d.nbx * d.nby * d.nbz;
         tot.ng2p += (int64_t)d.gbx * d.gby * d.gbz;
         tot.nbb += (int64_t)d.nbb + 1;
-        // combine waves: one face row along x per wave, or 64 / nx rows when
-        // a row has fewer than 64 faces
+        // combine waves: each takes combine_rows(nx) consecutive face rows
+        // along x
         int64_t waves[3];
         for (int c = 0; c < 3; c++) {
-            const int64_t nx = d.ni + (c == 0), rows = (int64_t)(d.nj + (c == 1)) * (d.nk + (c == 2));
-            const int64_t rpw = nx < 64 ? 64 / nx : 1;
+            const int64_t rows = (int64_t)(d.nj + (c == 1)) * (d.nk + (c == 2));
+            const int64_t rpw = combine_rows(d.ni + (c == 0));
             waves[c] = (rows + rpw - 1) / rpw;
         }
         d.cw[0] = waves[0];

@@ -453,6 +453,7 @@ std::vector<gfsx::SceneDesc> flip_step_impl(gfsx_ctx *ctx, const gfsx_scene *sce
     require(s.tot.n < ((int64_t)1 << 32) - 1, "gfsx: too many particles in one step");
     require(s.tot.nbin < ((int64_t)1 << 32) - 2, "gfsx: too many cells in one step");
     require(s.tot.nbb < ((int64_t)1 << gfsx::kBrickBinBits), "gfsx: too many bricks in one step");
+    require(s.tot.nrm < ((int64_t)1 << 32), "gfsx: too many bricks in one step");
     require(s.tot.nblk * 1024 < ((int64_t)1 << 32) - 1024, "gfsx: too many particles in one step");
     require(nscenes == 1 || (ctx->p2g_variant != 1 && ctx->g2p_variant != 1),
             "gfsx: the direct P2G and G2P variants take one scene");
@@ -475,7 +476,8 @@ std::vector<gfsx::SceneDesc> flip_step_impl(gfsx_ctx *ctx, const gfsx_scene *sce
     if (ctx->p2g_variant == 4) {
         // one tile per brick bin (the overflow bins' stay unused)
         s.tiles = (float2 *)wsf(S_P_TILE, 2 * (int64_t)gfsx::kP2gTileNodes * s.tot.nbb);
-        s.tile_flag = (uint8_t *)ws(S_P_TFLAG, (size_t)s.tot.nbb);
+        s.tile_rows = (unsigned long long *)ws(
+            S_P_TFLAG, sizeof(unsigned long long) * (size_t)std::max<int64_t>(s.tot.nrm, 1));
     } else {
         s.dense = ctx->u32(S_DENSE, s.tot.np2g);
         s.dense_count = ctx->u32(S_DENSEN, 2);

@@ -668,7 +668,7 @@ __device__ __forceinline__ void bscatter_comp(unsigned long long *acc, uint32_t 
 __global__ __launch_bounds__(bscat::NT) void k_p2g_bscatter(
     const SceneDesc *__restrict__ scenes, int nsc, uint32_t nbin, int per_xcd,
     const uint32_t *__restrict__ pstart, float2 *__restrict__ tiles,
-    uint8_t *__restrict__ tile_flag, unsigned long long *__restrict__ fluid) {
+    unsigned long long *__restrict__ tile_rows, unsigned long long *__restrict__ fluid) {
     using namespace bscat;
     __shared__ unsigned long long s_acc[NREP][3][3][TN];
     __shared__ uint32_t s_bad[(3 * TN + 7) / 8];
@@ -683,10 +683,7 @@ __global__ __launch_bounds__(bscat::NT) void k_p2g_bscatter(
     if (lb >= sd.nbb) return;   // the overflow bin reaches no face
     const uint32_t q0 = gld(pstart, gid), N = gld(pstart, gid + 1) - q0;
     if (N == 0) {
-        if (t == 0) {
-            tile_flag[gid] = 0;
-            fluid[gid] = 0ull;
-        }
+        if (t == 0) fluid[gid] = 0ull;
         return;
     }
     int bxi, byi, bzi;
@@ -860,7 +857,10 @@ __global__ __launch_bounds__(bscat::NT) void k_p2g_bscatter(
     flush(std::integral_constant<int, 1>());
     flush(std::integral_constant<int, 2>());
     if (t == 0) {
-        tile_flag[gid] = 1;
+        // the tile is written: its brick's bit in the occupied-brick mask
+        atomicOr(&tile_rows[sd.rm0 + (uint32_t)sd.rmw * ((uint32_t)byi + (uint32_t)sd.nby * (uint32_t)bzi) +
+                            (uint32_t)(bxi >> 6)],
+                 1ull << (bxi & 63));
         fluid[gid] = s_fluid;
     }
 }
@@ -869,15 +869,17 @@ __global__ __launch_bounds__(bscat::NT) void k_p2g_bscatter(
 // Combine: every face from the tiles that hold it
 // ---------------------------------------------------------------------------
 //
-// One wave per combine_rows(nx) consecutive face rows along x (component c,
-// node rows j, k), each row 64 faces at a time in array order.  Node 4b+r of an axis lies in the tile of brick b at
-// position r (own axis: nodes 4b .. 4b+4) or r+1 (across: 4b-1 .. 4b+4);
-// also in brick b-1's tile (r = 0: position 4 along the own axis, 5
-// across) or, across, in brick b+1's (r = 3: position 0).  The entries are
-// added lower brick first, z then y then x, so each face's sum has a fixed
-// order; a brick without particles (tile flag 0) adds nothing.  The cells
-// on either side of a face along its axis are fluid when their bit is set
-// in the mask their brick's scatter wrote (fluidmaterialgrid.cpp:119-143).
+// One wave per combine_rows(nx) consecutive face rows along x (component
+// c, node rows j, k), each row 64 faces at a time in array order.  Node
+// 4b+r of an axis lies in the tile of brick b at position r (own axis:
+// nodes 4b .. 4b+4) or r+1 (across: 4b-1 .. 4b+4); also in brick b-1's tile
+// (r = 0: position 4 along the own axis, 5 across) or, across, in brick
+// b+1's (r = 3: position 0).  The entries are added lower brick first, z
+// then y then x, so each face's sum has a fixed order; a brick without
+// particles (its bit clear in the occupied-brick mask of its row of bricks)
+// adds nothing.  The cells on either side of a face along its axis are
+// fluid when their bit is set in the mask their brick's scatter wrote
+// (fluidmaterialgrid.cpp:119-143).
 
 // Bricks and tile positions holding node n of an axis with nb bricks:
 // cnt entries (brick offset db, position tp), lower brick first.
@@ -924,7 +926,7 @@ __device__ __forceinline__ bool cell_fluid(const SceneDesc &sd,
 
 __global__ __launch_bounds__(kBlock) void k_p2g_bcombine(
     const SceneDesc *__restrict__ scenes, const int64_t *__restrict__ r0s, int nsc, int64_t nwave,
-    const float2 *__restrict__ tiles, const uint8_t *__restrict__ tile_flag,
+    const float2 *__restrict__ tiles, const unsigned long long *__restrict__ tile_rows,
     const unsigned long long *__restrict__ fluid) {
     using namespace bscat;
     const int lane = threadIdx.x & 63;
@@ -964,36 +966,42 @@ __global__ __launch_bounds__(kBlock) void k_p2g_bcombine(
     const unsigned long long *te = reinterpret_cast<const unsigned long long *>(tiles);
     for (uint32_t r = r0; r < r1; r++) {
         const AxisTiles ay = axis_tiles(J, sd.nby, c == 1), az = axis_tiles(K, sd.nbz, c == 2);
-        // per (z, y) brick of the row: the tile row the face row reads, and
-        // the bin of the row's brick bx = 0 (brick_local: bx adds
-        // (bx >> 1) << 3 | (bx & 1))
+        // per (z, y) brick row of the face row: the tile row it reads, and
+        // that row's occupied-brick mask (one word, loaded once for the
+        // wave, when the grid has at most 64 bricks along x)
         size_t trow[2][2];
-        uint32_t brow[2][2];
+        uint32_t mrow[2][2];
+        unsigned long long mw[2][2];
 #pragma unroll
         for (int iz = 0; iz < 2; iz++)
 #pragma unroll
             for (int iy = 0; iy < 2; iy++) {
                 const int by = ay.b + ay.db[iy], bz = az.b + az.db[iz];
+                const bool ok = iz < az.cnt && iy < ay.cnt;
                 trow[iz][iy] = tile_at(tg, 0, by * sy + ay.tp[iy], bz * sz + az.tp[iz]);
-                brow[iz][iy] = sd.kb0 + brick_local(sd, 0, by, bz);
+                mrow[iz][iy] = sd.rm0 + (uint32_t)sd.rmw * ((uint32_t)by + (uint32_t)sd.nby * (uint32_t)bz);
+                mw[iz][iy] = ok && sd.rmw == 1 ? tile_rows[mrow[iz][iy]] : 0ull;
             }
         for (int I = lane; I < nxc; I += 64) {
             const AxisTiles ax = axis_tiles(I, sd.nbx, c == 0);
             const bool two = ax.cnt == 2;
             const int bx0 = ax.b + ax.db[0], bx1 = two ? ax.b + ax.db[1] : bx0;
-            const uint32_t fb0 = ((uint32_t)(bx0 >> 1) << 3) | (uint32_t)(bx0 & 1);
-            const uint32_t fb1 = ((uint32_t)(bx1 >> 1) << 3) | (uint32_t)(bx1 & 1);
             const size_t e0 = (size_t)(bx0 * sx + ax.tp[0]), e1 = (size_t)(bx1 * sx + ax.tp[1]);
-            // every flag load goes out first, then the entries of the bricks
-            // that hold particles (a brick without particles wrote no tile)
+            // the entries of the bricks that hold particles (a brick without
+            // particles wrote no tile), every load issued before any is used
             uint32_t f0[2][2], f1[2][2];
 #pragma unroll
             for (int iz = 0; iz < 2; iz++)
 #pragma unroll
                 for (int iy = 0; iy < 2; iy++) {
                     const bool ok = iz < az.cnt && iy < ay.cnt;
-                    f0[iz][iy] = ok ? (uint32_t)gld(tile_flag, brow[iz][iy] + fb0) : 0u;
-                    f1[iz][iy] = ok && two ? (uint32_t)gld(tile_flag, brow[iz][iy] + fb1) : 0u;
+                    unsigned long long m0 = mw[iz][iy], m1 = mw[iz][iy];
+                    if (sd.rmw > 1 && ok) {
+                        m0 = gld(tile_rows, (size_t)mrow[iz][iy] + (size_t)(bx0 >> 6));
+                        m1 = gld(tile_rows, (size_t)mrow[iz][iy] + (size_t)(bx1 >> 6));
+                    }
+                    f0[iz][iy] = ok ? (uint32_t)(m0 >> (bx0 & 63)) & 1u : 0u;
+                    f1[iz][iy] = ok && two ? (uint32_t)(m1 >> (bx1 & 63)) & 1u : 0u;
                 }
             unsigned long long v0[2][2], v1[2][2];   // (S, W) float pairs
 #pragma unroll
@@ -1397,14 +1405,15 @@ void flip_step_brick(const StepBatch &s, const BrickSort &bs, hipStream_t st, hi
     if (ev) GFSX_HIP(hipEventRecord(ev[1], st));
 
     const int per_xcd = (int)((nbin + 7) / 8);
+    fill_u32((uint32_t *)s.tile_rows, 0u, 2 * std::max<int64_t>(s.tot.nrm, 1), st);
     hipLaunchKernelGGL(k_p2g_bscatter, dim3((unsigned)(8 * (int64_t)per_xcd)), dim3(bscat::NT), 0, st,
                        s.scenes_dev, nsc, nbin, per_xcd, (const uint32_t *)bs.pstart, s.tiles,
-                       s.tile_flag, bs.fluid);
+                       s.tile_rows, bs.fluid);
     GFSX_LAUNCH_CHECK();
     if (nsc > 1 && !s.r0s) throw Error("gfsx: the combine of several scenes needs their wave offsets");
     hipLaunchKernelGGL(k_p2g_bcombine, dim3(grid_for(s.tot.nrow, kBlock / 64)), dim3(kBlock), 0, st,
                        s.scenes_dev, s.r0s, nsc, s.tot.nrow, (const float2 *)s.tiles,
-                       (const uint8_t *)s.tile_flag, (const unsigned long long *)bs.fluid);
+                       (const unsigned long long *)s.tile_rows, (const unsigned long long *)bs.fluid);
     GFSX_LAUNCH_CHECK();
     if (s.epi) fs_epilogue(*s.epi, st);
     if (ev) GFSX_HIP(hipEventRecord(ev[2], st));

@@ -125,6 +125,11 @@ struct SceneDesc {
     // some beyond the grid when nbx, nby or nbz is odd), then the overflow
     // bin (local index nbb); kb0 is the first global one
     uint32_t kb0, nbb;
+    // occupied-brick masks of the scatter P2G: per (by, bz) row of bricks
+    // rmw 64-bit words, bit bx set when brick (bx, by, bz) holds particles;
+    // rm0 is the scene's first word
+    uint32_t rm0;
+    int rmw;
     int64_t f0;       // first global MAC face (faces of u, then v, then w, per scene)
     int64_t r0;       // first combine wave (over the x-rows of u, then v, then w)
     int64_t cw[2];    // combine waves of the scene before its v rows and before its w rows
@@ -385,6 +390,7 @@ struct SceneTotals {
     int64_t nface = 0;   // MAC faces
     int64_t nbb = 0;     // brick sort bins, overflow bins included
     int64_t nrow = 0;    // combine waves (face rows along x, short rows packed), all components
+    int64_t nrm = 0;     // occupied-brick mask words
 };
 
 // Fill the derived fields of every scene (bin grid, offsets p0/b0/blk0/k0/g0,
@@ -435,7 +441,7 @@ struct StepBatch {
     uint32_t *dense, *dense_count;   // workspace: P2G bricks over capacity (list; count
                                      // and next unprocessed entry)
     float2 *tiles = nullptr;         // workspace of the scatter P2G: kP2gTileNodes (S, W) per brick
-    uint8_t *tile_flag = nullptr;    // per brick: its tile was written
+    unsigned long long *tile_rows = nullptr;   // occupied-brick masks (rm0, rmw)
     int num_cu = 1;                  // compute units of the device
     int stencil;
     int keep_order;

@@ -1899,6 +1899,9 @@ void layout_scenes(SceneDesc *sc, int nsc, SceneTotals &tot) {
         d.kb0 = (uint32_t)tot.nbb;
         d.f0 = tot.nface;
         d.r0 = tot.nrow;
+        d.rm0 = (uint32_t)tot.nrm;
+        d.rmw = (d.nbx + 63) / 64;
+        tot.nrm += (int64_t)d.rmw * d.nby * d.nbz;
         MacGrid g;
         g.ni = d.ni;
         g.nj = d.nj;

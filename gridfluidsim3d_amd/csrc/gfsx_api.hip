@@ -68,6 +68,7 @@ struct gfsx_ctx {
     int p2g_variant = 4;
     int g2p_variant = 0;
     int g2p_brick = 8;
+    int particle_gather = 2;
     int adjoint_variant = 0;
     bool timing = false;
     std::vector<std::array<hipEvent_t, 4>> events;
@@ -469,9 +470,16 @@ std::vector<gfsx::SceneDesc> flip_step_impl(gfsx_ctx *ctx, const gfsx_scene *sce
                 d.grid_old[c] = old_ws;
                 old_ws += mac_count(c, d.ni, d.nj, d.nk);
             }
+    // input already grouped by brick (the previous step's sorted output):
+    // the sort writes no copies; P2G reads each sort slot's particle through
+    // the permutation, at nearly consecutive addresses, and so does G2P
+    // (particle_gather 1), or P2G writes the sorted copies for G2P (2)
+    s.gather = ctx->particle_gather != 0 && ctx->p2g_variant == 4 && ctx->g2p_variant == 0 &&
+               params[0].order == GFSX_ORDER_SORTED;
+    const bool no_copies = s.gather && ctx->particle_gather == 1;
     for (int c = 0; c < 3; c++) {
-        s.xs[c] = wsf(S_XS0 + c, s.tot.n);
-        s.vs[c] = wsf(S_VS0 + c, s.tot.n);
+        s.xs[c] = no_copies ? nullptr : wsf(S_XS0 + c, s.tot.n);
+        s.vs[c] = no_copies ? nullptr : wsf(S_VS0 + c, s.tot.n);
     }
     if (ctx->p2g_variant == 4) {
         // one tile per brick bin (the overflow bins' stay unused)
@@ -509,8 +517,9 @@ std::vector<gfsx::SceneDesc> flip_step_impl(gfsx_ctx *ctx, const gfsx_scene *sce
         epi.dx = d.dx;
         epi.np = d.n;
         for (int c = 0; c < 3; c++) {
-            // the input of output index i: itself (keep) or sorted slot i
-            epi.x0[c] = params[0].order == GFSX_ORDER_KEEP ? d.xin[c] : s.xs[c];
+            // the input of output index i: itself (keep), sorted slot i, or
+            // the input at the permutation's entry i (no sorted copies)
+            epi.x0[c] = params[0].order == GFSX_ORDER_KEEP || !s.xs[0] ? d.xin[c] : s.xs[c];
             epi.x1[c] = d.xout[c];
         }
         int64_t at = 0;
@@ -540,7 +549,7 @@ std::vector<gfsx::SceneDesc> flip_step_impl(gfsx_ctx *ctx, const gfsx_scene *sce
         bool ids = false;
         for (auto &d : s.scenes) ids = ids || d.id_out != nullptr;
         gfsx::BrickSort bs = brick_sort(ws, s.tot.n, s.tot.nblk, (uint32_t)s.tot.nbb,
-                                        s.keep_order || ids);
+                                        s.keep_order || ids || s.gather);
         if (carve) {
             if (carve->used > carve->cap) {
                 std::ostringstream m;
@@ -556,6 +565,7 @@ std::vector<gfsx::SceneDesc> flip_step_impl(gfsx_ctx *ctx, const gfsx_scene *sce
         }
         s.bs = &bs;
         if (fs) {
+            epi.x0_perm = s.xs[0] ? nullptr : bs.perm;
             epi.fluid = bs.fluid;
             epi.kb0 = s.scenes[0].kb0;
             epi.gbx = s.scenes[0].gbx;
@@ -1078,6 +1088,9 @@ void gfsx_set_option(gfsx_ctx *ctx, const char *name, int value, int *err) {
         } else if (std::strcmp(name, "g2p_brick") == 0) {
             require(value == 8 || value == 4, "gfsx_set_option: g2p_brick must be 8 or 4");
             ctx->g2p_brick = value;
+        } else if (std::strcmp(name, "particle_gather") == 0) {
+            require(value >= 0 && value <= 2, "gfsx_set_option: particle_gather must be 0..2");
+            ctx->particle_gather = value;
         } else if (std::strcmp(name, "adjoint_variant") == 0) {
             require(value >= 0 && value <= 1, "gfsx_set_option: adjoint_variant must be 0..1");
             ctx->adjoint_variant = value;
@@ -1094,6 +1107,7 @@ int gfsx_get_option(gfsx_ctx *ctx, const char *name, int *err) {
         if (std::strcmp(name, "p2g_variant") == 0) v = ctx->p2g_variant;
         else if (std::strcmp(name, "g2p_variant") == 0) v = ctx->g2p_variant;
         else if (std::strcmp(name, "g2p_brick") == 0) v = ctx->g2p_brick;
+        else if (std::strcmp(name, "particle_gather") == 0) v = ctx->particle_gather;
         else if (std::strcmp(name, "adjoint_variant") == 0) v = ctx->adjoint_variant;
         else throw Error(std::string("gfsx_get_option: unknown option ") + name);
     });

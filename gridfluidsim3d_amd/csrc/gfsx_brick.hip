@@ -495,9 +495,9 @@ __global__ __launch_bounds__(kBlock) void k_bsort_order_huge(
     }
 }
 
-// Positions and velocities (and, when perm is set, the particle index)
-// written to their sort slots: the run's first slot plus the rank in the
-// run, recomputed exactly as k_bsort_bin found them.
+// Positions and velocities (when out.x[0] is set) and the particle index
+// (when perm is set) written to their sort slots: the run's first slot plus
+// the rank in the run, recomputed exactly as k_bsort_bin found them.
 struct SortedOut {
     float *x[3], *v[3];
 };
@@ -511,6 +511,7 @@ __global__ __launch_bounds__(kBlock) void k_bsort_scatter(const SceneDesc *__res
     const SceneDesc &sd = scenes[scene_of(scenes, nsc, blk, [](const SceneDesc &d) { return d.blk0; })];
     const int64_t n = sd.n;
     const int64_t base = (int64_t)(blk - sd.blk0) * TILE + threadIdx.x;
+    const bool copies = out.x[0] != nullptr;
     float p[ITEMS][6];
 #pragma unroll
     for (int m = 0; m < ITEMS; m++) {
@@ -519,7 +520,7 @@ __global__ __launch_bounds__(kBlock) void k_bsort_scatter(const SceneDesc *__res
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             p[m][c] = gld(sd.xin[c], ii);
-            p[m][3 + c] = gld(sd.vin[c], ii);
+            p[m][3 + c] = copies ? gld(sd.vin[c], ii) : 0.0f;
         }
     }
 #pragma unroll
@@ -532,10 +533,12 @@ __global__ __launch_bounds__(kBlock) void k_bsort_scatter(const SceneDesc *__res
         wave_runs(key, valid, j, rank, [](bool, uint32_t, uint32_t, int) {});
         if (valid) {
             const uint32_t dst = gld(runbase, (size_t)wave * 64 + j) + (uint32_t)rank;
+            if (copies) {
 #pragma unroll
-            for (int c = 0; c < 3; c++) {
-                out.x[c][dst] = p[m][c];
-                out.v[c][dst] = p[m][3 + c];
+                for (int c = 0; c < 3; c++) {
+                    out.x[c][dst] = p[m][c];
+                    out.v[c][dst] = p[m][3 + c];
+                }
             }
             if (perm) perm[dst] = (uint32_t)(sd.p0 + i);
         }
@@ -604,12 +607,13 @@ __device__ __forceinline__ size_t tile_at(const TileGrid &t, int ex, int ey, int
     return t.base + (size_t)ex + (size_t)t.tx * ((size_t)ey + (size_t)t.ty * (size_t)ez);
 }
 
-// floor(y * 2^32) of y in [0, 2^31) as 64 bits: the integer and fractional
-// parts of y are exact floats, and so is the fraction times 2^32.
+// floor(y * 2^32) of y in [0, 2^31) as 64 bits: the integer part is the
+// truncating conversion of y (y >= 0), the fraction y - floor(y) is exact
+// (v_fract), and so is the fraction times 2^32.
 __device__ __forceinline__ unsigned long long fix64(float y) {
-    const float hi = floorf(y);
-    const float lo = (y - hi) * 4294967296.0f;
-    return ((unsigned long long)(uint32_t)hi << 32) | (unsigned long long)(uint32_t)lo;
+    const uint32_t hi = (uint32_t)y;
+    const float lo = __builtin_amdgcn_fractf(y) * 4294967296.0f;
+    return ((unsigned long long)hi << 32) | (unsigned long long)(uint32_t)lo;
 }
 
 // Add one component of one particle.  acc: the component's three sums
@@ -667,8 +671,9 @@ __device__ __forceinline__ void bscatter_comp(unsigned long long *acc, uint32_t 
 
 __global__ __launch_bounds__(bscat::NT) void k_p2g_bscatter(
     const SceneDesc *__restrict__ scenes, int nsc, uint32_t nbin, int per_xcd,
-    const uint32_t *__restrict__ pstart, float2 *__restrict__ tiles,
-    unsigned long long *__restrict__ tile_rows, unsigned long long *__restrict__ fluid) {
+    const uint32_t *__restrict__ pstart, const uint32_t *__restrict__ gperm, SortedOut copy,
+    float2 *__restrict__ tiles, unsigned long long *__restrict__ tile_rows,
+    unsigned long long *__restrict__ fluid) {
     using namespace bscat;
     __shared__ unsigned long long s_acc[NREP][3][3][TN];
     __shared__ uint32_t s_bad[(3 * TN + 7) / 8];
@@ -680,7 +685,22 @@ __global__ __launch_bounds__(bscat::NT) void k_p2g_bscatter(
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const SceneDesc &sd = scenes[scene_of(scenes, nsc, gid, [](const SceneDesc &d) { return d.kb0; })];
     const uint32_t lb = gid - sd.kb0;
-    if (lb >= sd.nbb) return;   // the overflow bin reaches no face
+    if (lb >= sd.nbb) {
+        // the overflow bin reaches no face; its particles' sorted copies
+        // (when asked for) are written here
+        if (copy.x[0]) {
+            const uint32_t p0 = (uint32_t)sd.p0;
+            for (uint32_t q = gld(pstart, gid) + t, q1 = gld(pstart, gid + 1); q < q1; q += NT) {
+                const uint32_t i = gld(gperm, q) - p0;
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    copy.x[c][q] = gld(sd.xin[c], i);
+                    copy.v[c][q] = gld(sd.vin[c], i);
+                }
+            }
+        }
+        return;
+    }
     const uint32_t q0 = gld(pstart, gid), N = gld(pstart, gid + 1) - q0;
     if (N == 0) {
         if (t == 0) fluid[gid] = 0ull;
@@ -691,8 +711,24 @@ __global__ __launch_bounds__(bscat::NT) void k_p2g_bscatter(
     const int o[3] = {4 * bxi, 4 * byi, 4 * bzi};
     const int ni = sd.ni, nj = sd.nj, nk = sd.nk;
     const double inv_dx = sd.bg.inv_dx;
-    const float *xp = sd.p2g.x, *yp = sd.p2g.y, *zp = sd.p2g.z;
-    const float *vp[3] = {sd.p2g.vel[0], sd.p2g.vel[1], sd.p2g.vel[2]};
+    // slot q's particle: the sorted copies at q, or (gperm set) the scene's
+    // inputs at gperm[q] - p0
+    const float *xp = gperm ? sd.xin[0] : sd.p2g.x, *yp = gperm ? sd.xin[1] : sd.p2g.y,
+                *zp = gperm ? sd.xin[2] : sd.p2g.z;
+    const float *vp[3] = {gperm ? sd.vin[0] : sd.p2g.vel[0], gperm ? sd.vin[1] : sd.p2g.vel[1],
+                          gperm ? sd.vin[2] : sd.p2g.vel[2]};
+    const uint32_t p0 = (uint32_t)sd.p0;
+    auto at = [&](uint32_t q) { return gperm ? gld(gperm, q) - p0 : q; };
+    // with copy.x[0] set, every particle read through gperm is also written
+    // to its sort slot (for G2P to read in order)
+    const bool copies = copy.x[0] != nullptr;
+    auto put = [&](uint32_t q, float x, float y, float z, const float v[3]) {
+        copy.x[0][q] = x;
+        copy.x[1][q] = y;
+        copy.x[2][q] = z;
+#pragma unroll
+        for (int c = 0; c < 3; c++) copy.v[c][q] = v[c];
+    };
     {
         unsigned long long *z = &s_acc[0][0][0][0];
         for (int e = t; e < NREP * 3 * 3 * TN; e += NT) z[e] = 0ull;
@@ -707,7 +743,7 @@ __global__ __launch_bounds__(bscat::NT) void k_p2g_bscatter(
 #pragma unroll
     for (int m = 0; m < PRE; m++) {
         const uint32_t s = (uint32_t)(t + m * NT);
-        const uint32_t q = q0 + (s < N ? s : 0u);
+        const uint32_t q = at(q0 + (s < N ? s : 0u));
         px[m] = gld(xp, q);
         py[m] = gld(yp, q);
         pz[m] = gld(zp, q);
@@ -722,12 +758,14 @@ __global__ __launch_bounds__(bscat::NT) void k_p2g_bscatter(
                 const float av = fabsf(pv[m][c]);
                 if (av < kBig) vm[c] = fmaxf(vm[c], av);
             }
+            if (copies) put(q0 + (uint32_t)(t + m * NT), px[m], py[m], pz[m], pv[m]);
         }
     }
     for (uint32_t s = (uint32_t)(t + PRE * NT); s < N; s += NT) {
+        const uint32_t q = at(q0 + s);
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            const float av = fabsf(gld(vp[c], q0 + s));
+            const float av = fabsf(gld(vp[c], q));
             if (av < kBig) vm[c] = fmaxf(vm[c], av);
         }
     }
@@ -810,9 +848,11 @@ __global__ __launch_bounds__(bscat::NT) void k_p2g_bscatter(
     for (int m = 0; m < PRE; m++)
         if ((uint32_t)(t + m * NT) < N) scatter(px[m], py[m], pz[m], pv[m]);
     for (uint32_t s = (uint32_t)(t + PRE * NT); s < N; s += NT) {
-        const uint32_t q = q0 + s;
+        const uint32_t q = at(q0 + s);
         const float v[3] = {gld(vp[0], q), gld(vp[1], q), gld(vp[2], q)};
-        scatter(gld(xp, q), gld(yp, q), gld(zp, q), v);
+        const float x = gld(xp, q), y = gld(yp, q), z = gld(zp, q);
+        if (copies) put(q0 + s, x, y, z, v);
+        scatter(x, y, z, v);
     }
     {
         uint32_t lo = (uint32_t)fl, hi = (uint32_t)(fl >> 32);
@@ -1190,8 +1230,8 @@ __device__ __forceinline__ void g2p_bbrick(const StepG2P &a, const MacGrid &g, i
     if (!fast) {
         // a stage may leave the boxes: every sample checked
         for (uint32_t q = q0 + (uint32_t)t; q < q1; q += NT) {
-            const float x0[3] = {gld(a.xin[0], q), gld(a.xin[1], q), gld(a.xin[2], q)};
-            const float v0[3] = {gld(a.vin[0], q), gld(a.vin[1], q), gld(a.vin[2], q)};
+            float x0[3], v0[3];
+            load_particle(a, q, x0, v0);
             float x1[3], vnew[3];
             g2p_particle_checked<ST, GBE>(a, g, cur, old, lx, ly, lz, lox, loy, loz, x0, v0, x1,
                                           vnew);
@@ -1201,23 +1241,11 @@ __device__ __forceinline__ void g2p_bbrick(const StepG2P &a, const MacGrid &g, i
     }
     // the next particle is loaded while this one is computed
     float xn[3] = {0.0f, 0.0f, 0.0f}, vn[3] = {0.0f, 0.0f, 0.0f};
-    if (q0 + (uint32_t)t < q1) {
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-            xn[c] = gld(a.xin[c], q0 + t);
-            vn[c] = gld(a.vin[c], q0 + t);
-        }
-    }
+    if (q0 + (uint32_t)t < q1) load_particle(a, q0 + t, xn, vn);
     for (uint32_t q = q0 + (uint32_t)t; q < q1; q += NT) {
         const float x0[3] = {xn[0], xn[1], xn[2]};
         const float v0[3] = {vn[0], vn[1], vn[2]};
-        if (q + NT < q1) {
-#pragma unroll
-            for (int c = 0; c < 3; c++) {
-                xn[c] = gld(a.xin[c], q + NT);
-                vn[c] = gld(a.vin[c], q + NT);
-            }
-        }
+        if (q + NT < q1) load_particle(a, q + NT, xn, vn);
         float x1[3], vnew[3];
         const bool done = g2p_particle_in<ST, GBE>(a, g, cur, old, lx, ly, lz, lox, loy, loz, c0x,
                                                    c0y, c0z, x0, v0, x1, vnew);
@@ -1279,7 +1307,9 @@ __global__ __launch_bounds__(gbrick::NT, 4) void k_flip_g2p_bb_tl(
     g2p_bb_kernel<0>(scenes, nsc, so, keep_order, nbrick, per_xcd, pstart, perm, slow);
 }
 
-__global__ __launch_bounds__(gbrick::NT, 2) void k_flip_g2p_bb_tc(
+// Tricubic: three waves per SIMD (168 VGPRs) run about 10% faster than two
+// (256) at the cost of a few bytes of spill.
+__global__ __launch_bounds__(gbrick::NT, 3) void k_flip_g2p_bb_tc(
     const SceneDesc *__restrict__ scenes, int nsc, Sorted so, int keep_order, uint32_t nbrick,
     int per_xcd, const uint32_t *__restrict__ pstart, const uint32_t *__restrict__ perm,
     G2PSlow slow) {
@@ -1311,8 +1341,8 @@ __device__ __forceinline__ void g2p_direct_one(const SceneDesc *__restrict__ sce
     const SceneDesc &sd = scenes[scene_of(scenes, nsc, (uint64_t)q, [](const SceneDesc &d) { return d.p0; })];
     const StepG2P a = g2p_view(sd, so, keep_order);
     const MacGrid g = mac_view(sd);
-    const float x0[3] = {a.xin[0][q], a.xin[1][q], a.xin[2][q]};
-    const float v0[3] = {a.vin[0][q], a.vin[1][q], a.vin[2][q]};
+    float x0[3], v0[3];
+    load_particle(a, q, x0, v0);
     float k1[3], vo[3], vnew[3], x1[3];
     sample_mac<ST, true>(g, a.old[0], a.old[1], a.old[2], x0[0], x0[1], x0[2], k1, vo);
     flip_blend(a, v0, k1, vo, vnew);
@@ -1374,9 +1404,10 @@ void flip_step_brick(const StepBatch &s, const BrickSort &bs, hipStream_t st, hi
     GFSX_LAUNCH_CHECK();
     SortedOut so_out;
     for (int c = 0; c < 3; c++) {
-        so_out.x[c] = s.xs[c];
-        so_out.v[c] = s.vs[c];
+        so_out.x[c] = s.gather ? nullptr : s.xs[c];
+        so_out.v[c] = s.gather ? nullptr : s.vs[c];
     }
+    if (s.gather && !bs.perm) throw Error("gfsx: the particle gather needs the sort's permutation");
     if (n > 0) {
         hipLaunchKernelGGL(k_bsort_list, dim3(grid_for(bs.nwave, kBlock)), dim3(kBlock), 0, st,
                            bs.nwave, (const uint8_t *)bs.nruns, (const uint2 *)bs.runrec,
@@ -1405,9 +1436,16 @@ void flip_step_brick(const StepBatch &s, const BrickSort &bs, hipStream_t st, hi
     if (ev) GFSX_HIP(hipEventRecord(ev[1], st));
 
     const int per_xcd = (int)((nbin + 7) / 8);
+    // gather with sorted copies: the P2G kernel writes them
+    SortedOut p2g_copy;
+    for (int c = 0; c < 3; c++) {
+        p2g_copy.x[c] = s.gather && s.xs[0] ? s.xs[c] : nullptr;
+        p2g_copy.v[c] = s.gather && s.xs[0] ? s.vs[c] : nullptr;
+    }
     fill_u32((uint32_t *)s.tile_rows, 0u, 2 * std::max<int64_t>(s.tot.nrm, 1), st);
     hipLaunchKernelGGL(k_p2g_bscatter, dim3((unsigned)(8 * (int64_t)per_xcd)), dim3(bscat::NT), 0, st,
-                       s.scenes_dev, nsc, nbin, per_xcd, (const uint32_t *)bs.pstart, s.tiles,
+                       s.scenes_dev, nsc, nbin, per_xcd, (const uint32_t *)bs.pstart,
+                       s.gather ? (const uint32_t *)bs.perm : nullptr, p2g_copy, s.tiles,
                        s.tile_rows, bs.fluid);
     GFSX_LAUNCH_CHECK();
     if (nsc > 1 && !s.r0s) throw Error("gfsx: the combine of several scenes needs their wave offsets");
@@ -1426,6 +1464,10 @@ void flip_step_brick(const StepBatch &s, const BrickSort &bs, hipStream_t st, hi
     for (int c = 0; c < 3; c++) {
         so.x[c] = s.xs[c];
         so.v[c] = s.vs[c];
+    }
+    if (s.gather && !s.xs[0]) {
+        if (s.g2p_variant != 0) throw Error("gfsx: the particle gather needs the brick G2P");
+        so.perm = bs.perm;
     }
     if (s.g2p_variant == 0) {
         const uint32_t nbrick = (uint32_t)s.tot.ng2p;

@@ -194,7 +194,8 @@ __device__ __forceinline__ FaceOut face_out(const StepP2G &a, int c) {
 }
 
 struct StepG2P {
-    const float *xin[3], *vin[3];   // in sort order (all scenes)
+    const float *xin[3], *vin[3];   // in sort order (all scenes), or the scene's inputs
+    const uint32_t *gperm;          // when set: slot q holds input particle gperm[q] - p0
     float *xout[3], *vout[3];       // the scene's outputs
     const uint32_t *id_in;
     uint32_t *id_out;
@@ -208,9 +209,10 @@ struct StepG2P {
 
 __host__ __device__ inline StepG2P g2p_view(const SceneDesc &sd, const Sorted &so, int keep_order) {
     StepG2P b;
+    b.gperm = so.perm;
     for (int c = 0; c < 3; c++) {
-        b.xin[c] = so.x[c];
-        b.vin[c] = so.v[c];
+        b.xin[c] = so.perm ? sd.xin[c] : so.x[c];
+        b.vin[c] = so.perm ? sd.vin[c] : so.v[c];
         b.xout[c] = sd.xout[c];
         b.vout[c] = sd.vout[c];
         b.old[c] = sd.grid_old[c];
@@ -248,6 +250,22 @@ __host__ __device__ inline MacGrid mac_view(const SceneDesc &sd) {
 // Write the new state of the particle in sort slot q: at q (sorted order)
 // or at its original index perm[q] (kept order), with its id, all relative
 // to the scene's first particle.
+// Index into a.xin / a.vin of the particle in sort slot q.
+__device__ __forceinline__ uint32_t particle_at(const StepG2P &a, uint32_t q) {
+    return a.gperm ? gld(a.gperm, q) - a.p0 : q;
+}
+
+// Position and velocity of the particle in sort slot q.
+__device__ __forceinline__ void load_particle(const StepG2P &a, uint32_t q, float x[3],
+                                              float v[3]) {
+    const uint32_t i = particle_at(a, q);
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        x[c] = gld(a.xin[c], i);
+        v[c] = gld(a.vin[c], i);
+    }
+}
+
 __device__ __forceinline__ void store_particle(const StepG2P &a,
                                                const uint32_t *__restrict__ perm,
                                                uint32_t q, const float x1[3],

@@ -354,7 +354,8 @@ __global__ __launch_bounds__(kBlock) void k_fs_collide(FsEpilogue e) {
     if (p >= e.np) return;
     float b[3] = {e.x1[0][p], e.x1[1][p], e.x1[2][p]};
     if (!point_in_solid(e, b[0], b[1], b[2])) return;
-    const float a[3] = {e.x0[0][p], e.x0[1][p], e.x0[2][p]};
+    const int64_t p0 = e.x0_perm ? (int64_t)e.x0_perm[p] : p;
+    const float a[3] = {e.x0[0][p0], e.x0[1][p0], e.x0[2][p0]};
     float r[3] = {a[0], a[1], a[2]};
     int vox[3];
     if (segment_voxel(e, a, b, vox)) {

@@ -94,9 +94,12 @@ struct StepP2G {
     int ni, nj, nk;
 };
 
-// The particles of every scene in sort order (slot q of the sort).
+// The particles of every scene in sort order (slot q of the sort): sorted
+// copies x, v indexed by slot, or, when perm is set, each scene's own input
+// arrays read at perm[q] - p0 (the particle the slot holds; x, v unused).
 struct Sorted {
     const float *x[3], *v[3];
+    const uint32_t *perm = nullptr;
 };
 
 // One scene of a (batched) fused step or of a cell sort, in a device table
@@ -274,9 +277,11 @@ struct FsEpilogue {
     int layers;
     double dx;
     // collision fix after RK4 (with solid cells): the step's outputs x1 and
-    // the same particles' inputs x0 (by output index), np particles
+    // the same particles' inputs x0 (by output index, or at x0_perm[index]
+    // when x0_perm is set), np particles
     int64_t np;
     const float *x0[3];
+    const uint32_t *x0_perm;
     float *x1[3];
     // workspace: each face's extrapolation layer (low byte, 0: none) and
     // neighbour mask (high byte), nface words
@@ -437,7 +442,12 @@ struct StepBatch {
     const SceneDesc *scenes_dev;     // the same table on the device
     const int64_t *r0s = nullptr;    // the scenes' first combine waves, dense (several scenes)
     SceneTotals tot;
-    float *xs[3], *vs[3];            // workspace: all particles in sort order
+    float *xs[3], *vs[3];            // workspace: all particles in sort order (may be
+                                     // null when gather is set)
+    bool gather = false;             // p2g_variant 4: the sort writes only the permutation;
+                                     // P2G reads each slot's particle from the inputs
+                                     // through it, and writes xs/vs when they are set, else
+                                     // G2P reads through it as well
     uint32_t *dense, *dense_count;   // workspace: P2G bricks over capacity (list; count
                                      // and next unprocessed entry)
     float2 *tiles = nullptr;         // workspace of the scatter P2G: kP2gTileNodes (S, W) per brick

@@ -229,6 +229,19 @@ void gfsx_flip_step_noalloc(gfsx_ctx *ctx, const gfsx_scene *scene,
  *                      tiles that hold each face (default).
  *   "g2p_variant": 0 = grid boxes staged in LDS per brick (default),
  *                  1 = direct sampling from global memory.
+ *   "g2p_brick":   cells per edge of the G2P brick of g2p_variant 0 with
+ *                  p2g_variant 4: 8 (default) or 4.
+ *   "particle_gather" (p2g_variant 4, g2p_variant 0, GFSX_ORDER_SORTED):
+ *                  0 = the sort writes sorted copies of the particles,
+ *                      P2G and G2P read them,
+ *                  1 = the sort writes only the permutation; P2G and G2P
+ *                      read each sort slot's particle from the *_in arrays
+ *                      through it (when the *_in arrays are the previous
+ *                      step's sorted output the reads are nearly
+ *                      consecutive),
+ *                  2 = as 1, but P2G writes the sorted copies for G2P
+ *                      (default).
+ *                  Results are the same bits in every mode.
  *   "adjoint_variant" (gfsx_flip_step_backward; results equal up to FP32
  *                  rounding):
  *                  0 = bricks of sorted particles sum the grid adjoints in

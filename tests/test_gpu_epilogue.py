@@ -128,3 +128,24 @@ def test_step_fs_collisions_match_oracle(ctx, orc, order):
     # every particle that the oracle pulled back was pulled back here too
     free = orc.flip_step_fs(pos, vel, dims, dx, dt, (0.0, -9.8, 0.0), 0.05, TRILINEAR, None, 7)
     assert (free["pos"][:, 0] < dx).sum() > len(pos) // 4
+
+
+def test_step_fs_collisions_particle_gather_bit_identical(ctx):
+    """The collision fix reads each particle's start position through the
+    sort's permutation when the step keeps no sorted copies (particle_gather
+    1): the same bits as with the copies (2: written by P2G, 0: by the
+    sort)."""
+    from test_oracle_epilogue import wall_scene
+    dims, dx, pos, vel = wall_scene()
+    sol = border_solid(dims)
+    old = ctx.get_option("particle_gather")
+    out = []
+    try:
+        for gather in (1, 2, 0):
+            ctx.set_option("particle_gather", gather)
+            out.append(run_gpu(ctx, pos, vel, dims, dx, 1.0 / 20.0, TRILINEAR, 0, sol, 7))
+    finally:
+        ctx.set_option("particle_gather", old)
+    for other in out[1:]:
+        for key in out[0]:
+            assert np.array_equal(out[0][key], other[key]), key

@@ -496,6 +496,54 @@ def test_step_g2p_variants_bit_identical(ctx, kind, stencil, keep):
             assert np.array_equal(out[0][key], other[key], equal_nan=True), key
 
 
+def _gather_runs(ctx, fn):
+    old = ctx.get_option("particle_gather")
+    out = []
+    try:
+        for gather in (1, 2, 0):
+            ctx.set_option("particle_gather", gather)
+            out.append(fn())
+    finally:
+        ctx.set_option("particle_gather", old)
+    return out
+
+
+@pytest.mark.parametrize("kind", ["blobs", "tiny", "wild"])
+@pytest.mark.parametrize("stencil", [TRILINEAR, TRICUBIC])
+@pytest.mark.parametrize("g2p_brick", [8, 4])
+def test_step_particle_gather_bit_identical(ctx, kind, stencil, g2p_brick):
+    """P2G and G2P reading each sort slot's particle through the sort's
+    permutation (particle_gather 1), P2G doing so and writing the sorted
+    copies for G2P (2), and the sort writing them (0) give the same bits,
+    three sorted steps in a row with particle ids: the first from an
+    unsorted input, the next from the previous step's sorted output."""
+    import gridfluidsim3d_amd as gf
+    pos, vel, dims, dx, dt = _g2p_variant_scene(kind)
+
+    def run():
+        st = gf.FlipState(pos, vel, dims, dx, track_ids=True, keep_old=True,
+                          keep_weights=True)
+        res = []
+        for _ in range(3):
+            ctx.flip_step(st, dt, (0.0, -9.8, 0.0), 0.05, stencil, gf.ORDER_SORTED)
+            _torch().cuda.synchronize()
+            res.append(dict(pos=_np(st.positions()), vel=_np(st.velocities()),
+                            id=_np(st.particle_ids()), u=_np(st.grid[0]),
+                            v=_np(st.grid[1]), w=_np(st.grid[2]), wu=_np(st.weights[0])))
+        return res
+
+    old = ctx.get_option("g2p_brick")
+    try:
+        ctx.set_option("g2p_brick", g2p_brick)
+        runs = _gather_runs(ctx, run)
+    finally:
+        ctx.set_option("g2p_brick", old)
+    for other in runs[1:]:
+        for step, (ra, rb) in enumerate(zip(runs[0], other)):
+            for key in ra:
+                assert np.array_equal(ra[key], rb[key], equal_nan=True), (step, key)
+
+
 def test_step_dense_cells_use_fallback(ctx, orc):
     """Bricks holding more particles than fit in LDS take the global-memory
     gather; results still match the oracle."""
@@ -526,6 +574,9 @@ def test_options(ctx):
     assert ctx.get_option("p2g_variant") == 4
     assert ctx.get_option("g2p_variant") == 0
     assert ctx.get_option("g2p_brick") == 8
+    assert ctx.get_option("particle_gather") == 2
+    with pytest.raises(RuntimeError, match="particle_gather"):
+        ctx.set_option("particle_gather", 3)
     with pytest.raises(RuntimeError, match="g2p_variant"):
         ctx.set_option("g2p_variant", 2)
     with pytest.raises(RuntimeError, match="g2p_brick"):
@@ -628,6 +679,30 @@ def test_step_batched_g2p_bricks_agree(ctx, stencil):
     for i, (a, b) in enumerate(zip(*runs)):
         for key in a:
             assert np.array_equal(a[key], b[key], equal_nan=True), (i, key)
+
+
+@pytest.mark.parametrize("stencil", [TRILINEAR, TRICUBIC])
+def test_step_batched_particle_gather_agrees(ctx, stencil):
+    """The batched step reading particles through the sort's permutation
+    gives the bits of the one over sorted copies, three steps in a row."""
+    import gridfluidsim3d_amd as gf
+    sc = _batch_scenes(73)
+    dts = [1 / 240, 1 / 120, 1 / 200, 1 / 300, 1 / 240]
+
+    def run():
+        states = [gf.FlipState(p, v, d, h, track_ids=True, keep_old=True, keep_weights=True)
+                  for p, v, d, h in sc]
+        for _ in range(3):
+            ctx.flip_step_batched(states, dts, (0.0, -9.8, 0.0), 0.05, stencil,
+                                  gf.ORDER_SORTED)
+        _torch().cuda.synchronize()
+        return [_state_result(st) for st in states]
+
+    runs = _gather_runs(ctx, run)
+    for other in runs[1:]:
+        for i, (ra, rb) in enumerate(zip(runs[0], other)):
+            for key in ra:
+                assert np.array_equal(ra[key], rb[key], equal_nan=True), (i, key)
 
 
 def test_step_batched_rejects_direct_variants(ctx):

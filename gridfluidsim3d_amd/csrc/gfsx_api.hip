@@ -69,6 +69,7 @@ struct gfsx_ctx {
     int g2p_variant = 0;
     int g2p_brick = 8;
     int particle_gather = 2;
+    int fp32_cells = 1;
     int adjoint_variant = 0;
     bool timing = false;
     std::vector<std::array<hipEvent_t, 4>> events;
@@ -544,6 +545,13 @@ std::vector<gfsx::SceneDesc> flip_step_impl(gfsx_ctx *ctx, const gfsx_scene *sce
     s.g2p_variant = ctx->g2p_variant;
     s.g2p_brick = ctx->g2p_brick;
     s.grids_only = grids_only;
+    // 1/dx a power of two (within FP32's normal range) in every scene
+    s.p2 = ctx->fp32_cells != 0;
+    for (const auto &d : s.scenes) {
+        int e = 0;
+        const double m = std::frexp(d.bg.inv_dx, &e);
+        s.p2 = s.p2 && m == 0.5 && e > -100 && e < 100;
+    }
     hipEvent_t *ev = ctx->timing && !carve ? ctx->next_events() : nullptr;
     if (ctx->p2g_variant == 4) {
         bool ids = false;
@@ -1088,6 +1096,9 @@ void gfsx_set_option(gfsx_ctx *ctx, const char *name, int value, int *err) {
         } else if (std::strcmp(name, "g2p_brick") == 0) {
             require(value == 8 || value == 4, "gfsx_set_option: g2p_brick must be 8 or 4");
             ctx->g2p_brick = value;
+        } else if (std::strcmp(name, "fp32_cells") == 0) {
+            require(value == 0 || value == 1, "gfsx_set_option: fp32_cells must be 0 or 1");
+            ctx->fp32_cells = value;
         } else if (std::strcmp(name, "particle_gather") == 0) {
             require(value >= 0 && value <= 2, "gfsx_set_option: particle_gather must be 0..2");
             ctx->particle_gather = value;
@@ -1108,6 +1119,7 @@ int gfsx_get_option(gfsx_ctx *ctx, const char *name, int *err) {
         else if (std::strcmp(name, "g2p_variant") == 0) v = ctx->g2p_variant;
         else if (std::strcmp(name, "g2p_brick") == 0) v = ctx->g2p_brick;
         else if (std::strcmp(name, "particle_gather") == 0) v = ctx->particle_gather;
+        else if (std::strcmp(name, "fp32_cells") == 0) v = ctx->fp32_cells;
         else if (std::strcmp(name, "adjoint_variant") == 0) v = ctx->adjoint_variant;
         else throw Error(std::string("gfsx_get_option: unknown option ") + name);
     });

@@ -1097,8 +1097,8 @@ __global__ __launch_bounds__(kBlock) void k_p2g_bcombine(
 // boxes of a brick of GBE^3 cells, for a brick whose field keeps every RK
 // stage within 0.99 cells of the start.  Returns false, computing nothing,
 // when the start position lies in the grid but outside the brick's cells:
-// then a stage may leave the boxes.
-template <int ST, int GBE>
+// then a stage may leave the boxes.  P2: inv_dx is a power of two.
+template <int ST, int GBE, bool P2>
 __device__ __forceinline__ bool g2p_particle_in(const StepG2P &a, const MacGrid &g,
                                                 const float *const cur[3],
                                                 const float *const old[3], int lx, int ly,
@@ -1108,24 +1108,26 @@ __device__ __forceinline__ bool g2p_particle_in(const StepG2P &a, const MacGrid 
     const bool ing = x0[0] >= 0.0f && x0[1] >= 0.0f && x0[2] >= 0.0f && x0[0] < g.limf[0] &&
                      x0[1] < g.limf[1] && x0[2] < g.limf[2];
     if (ing) {
-        const int ix = (int)floor((double)x0[0] * g.inv_dx) - c0x;
-        const int iy = (int)floor((double)x0[1] * g.inv_dx) - c0y;
-        const int iz = (int)floor((double)x0[2] * g.inv_dx) - c0z;
+        // x0 >= 0 here: with P2 the FP32 product is exact
+        const float fi = (float)g.inv_dx;
+        const int ix = (P2 ? (int)floorf(x0[0] * fi) : (int)floor((double)x0[0] * g.inv_dx)) - c0x;
+        const int iy = (P2 ? (int)floorf(x0[1] * fi) : (int)floor((double)x0[1] * g.inv_dx)) - c0y;
+        const int iz = (P2 ? (int)floorf(x0[2] * fi) : (int)floor((double)x0[2] * g.inv_dx)) - c0z;
         if (!((unsigned)ix < (unsigned)GBE && (unsigned)iy < (unsigned)GBE &&
               (unsigned)iz < (unsigned)GBE))
             return false;
     }
     // outside the grid every sample is 0 and the stages stay at x0
     float k1[3], vo[3];
-    brick_sample_in<ST, true, GBE>(g, cur[0], cur[1], cur[2], old[0], old[1], old[2], lx, ly, lz,
-                                   lox, loy, loz, x0[0], x0[1], x0[2], k1, vo);
+    brick_sample_in<ST, true, GBE, P2>(g, cur[0], cur[1], cur[2], old[0], old[1], old[2], lx, ly,
+                                       lz, lox, loy, loz, x0[0], x0[1], x0[2], k1, vo);
     flip_blend(a, v0, k1, vo, vnew);
     float xq[3], k2[3], k3[3], k4[3], unused[3];
     auto stage = [&](const float h, const float k[3], float out[3]) {
 #pragma unroll
         for (int c = 0; c < 3; c++) xq[c] = x0[c] + h * k[c];
-        brick_sample_in<ST, false, GBE>(g, cur[0], cur[1], cur[2], nullptr, nullptr, nullptr, lx,
-                                        ly, lz, 0, 0, 0, xq[0], xq[1], xq[2], out, unused);
+        brick_sample_in<ST, false, GBE, P2>(g, cur[0], cur[1], cur[2], nullptr, nullptr, nullptr,
+                                            lx, ly, lz, 0, 0, 0, xq[0], xq[1], xq[2], out, unused);
     };
     stage(a.rk.half, k1, k2);
     stage(a.rk.half, k2, k3);
@@ -1139,7 +1141,7 @@ __device__ __forceinline__ bool g2p_particle_in(const StepG2P &a, const MacGrid 
 // The same for any start position, every sample checked against its box
 // (brick_sample: a stencil that leaves the box reads global memory; the
 // same bits as the boxes give).
-template <int ST, int GBE>
+template <int ST, int GBE, bool P2>
 __device__ __forceinline__ void g2p_particle_checked(const StepG2P &a, const MacGrid &g,
                                                      const float *const cur[3],
                                                      const float *const old[3], int lx, int ly,
@@ -1147,17 +1149,17 @@ __device__ __forceinline__ void g2p_particle_checked(const StepG2P &a, const Mac
                                                      const float x0[3], const float v0[3],
                                                      float x1[3], float vnew[3]) {
     float k1[3], vo[3];
-    brick_sample<ST, true, GBE>(g, a.old[0], a.old[1], a.old[2], cur[0], cur[1], cur[2], old[0],
-                                old[1], old[2], lx, ly, lz, lox, loy, loz, x0[0], x0[1], x0[2],
-                                k1, vo);
+    brick_sample<ST, true, GBE, P2>(g, a.old[0], a.old[1], a.old[2], cur[0], cur[1], cur[2],
+                                    old[0], old[1], old[2], lx, ly, lz, lox, loy, loz, x0[0],
+                                    x0[1], x0[2], k1, vo);
     flip_blend(a, v0, k1, vo, vnew);
     float xq[3], k2[3], k3[3], k4[3], unused[3];
     auto stage = [&](const float h, const float k[3], float out[3]) {
 #pragma unroll
         for (int c = 0; c < 3; c++) xq[c] = x0[c] + h * k[c];
-        brick_sample<ST, false, GBE>(g, nullptr, nullptr, nullptr, cur[0], cur[1], cur[2],
-                                     nullptr, nullptr, nullptr, lx, ly, lz, 0, 0, 0, xq[0], xq[1],
-                                     xq[2], out, unused);
+        brick_sample<ST, false, GBE, P2>(g, nullptr, nullptr, nullptr, cur[0], cur[1], cur[2],
+                                         nullptr, nullptr, nullptr, lx, ly, lz, 0, 0, 0, xq[0],
+                                         xq[1], xq[2], out, unused);
     };
     stage(a.rk.half, k1, k2);
     stage(a.rk.half, k2, k3);
@@ -1188,7 +1190,7 @@ __device__ __forceinline__ float nan_max(float a, float b) {
 // most 1.25 in magnitude per axis, so at most 2 overall), a start cell
 // inside the brick keeps every stencil inside the staged boxes and no
 // sample tests them.  Otherwise every sample is checked against its box.
-template <int ST, int GBE>
+template <int ST, int GBE, bool P2>
 __device__ __forceinline__ void g2p_bbrick(const StepG2P &a, const MacGrid &g, int c0x, int c0y,
                                            int c0z, uint32_t q0, uint32_t q1,
                                            const uint32_t *__restrict__ perm, G2PSlow slow) {
@@ -1233,8 +1235,8 @@ __device__ __forceinline__ void g2p_bbrick(const StepG2P &a, const MacGrid &g, i
             float x0[3], v0[3];
             load_particle(a, q, x0, v0);
             float x1[3], vnew[3];
-            g2p_particle_checked<ST, GBE>(a, g, cur, old, lx, ly, lz, lox, loy, loz, x0, v0, x1,
-                                          vnew);
+            g2p_particle_checked<ST, GBE, P2>(a, g, cur, old, lx, ly, lz, lox, loy, loz, x0, v0,
+                                              x1, vnew);
             store_particle(a, perm, q, x1, vnew);
         }
         return;
@@ -1247,8 +1249,8 @@ __device__ __forceinline__ void g2p_bbrick(const StepG2P &a, const MacGrid &g, i
         const float v0[3] = {vn[0], vn[1], vn[2]};
         if (q + NT < q1) load_particle(a, q + NT, xn, vn);
         float x1[3], vnew[3];
-        const bool done = g2p_particle_in<ST, GBE>(a, g, cur, old, lx, ly, lz, lox, loy, loz, c0x,
-                                                   c0y, c0z, x0, v0, x1, vnew);
+        const bool done = g2p_particle_in<ST, GBE, P2>(a, g, cur, old, lx, ly, lz, lox, loy, loz,
+                                                       c0x, c0y, c0z, x0, v0, x1, vnew);
         if (done) store_particle(a, perm, q, x1, vnew);
         // the rest to the slow list, one atomic per wave (by the lowest lane
         // that has one: lanes past the end have left the loop)
@@ -1264,7 +1266,7 @@ __device__ __forceinline__ void g2p_bbrick(const StepG2P &a, const MacGrid &g, i
 }
 
 // One work-group per G2P brick of 8^3 cells of any scene.
-template <int ST>
+template <int ST, bool P2>
 __device__ __forceinline__ void g2p_bb_kernel(const SceneDesc *__restrict__ scenes, int nsc,
                                               Sorted so, int keep_order, uint32_t nbrick,
                                               int per_xcd, const uint32_t *__restrict__ pstart,
@@ -1278,11 +1280,11 @@ __device__ __forceinline__ void g2p_bb_kernel(const SceneDesc *__restrict__ scen
     const int lg = (int)(gid - sd.g0);
     const int gx = lg % sd.gbx, gy = (lg / sd.gbx) % sd.gby, gz = lg / (sd.gbx * sd.gby);
     const uint32_t b0 = sd.kb0 + 8u * (uint32_t)lg;
-    g2p_bbrick<ST, 8>(a, g, gx * 8, gy * 8, gz * 8, pstart[b0], pstart[b0 + 8], perm, slow);
+    g2p_bbrick<ST, 8, P2>(a, g, gx * 8, gy * 8, gz * 8, pstart[b0], pstart[b0 + 8], perm, slow);
 }
 
 // One work-group per P2G brick (4^3 cells, one bin of the sort) of any scene.
-template <int ST>
+template <int ST, bool P2>
 __device__ __forceinline__ void g2p_b4_kernel(const SceneDesc *__restrict__ scenes, int nsc,
                                               Sorted so, int keep_order, uint32_t nbin,
                                               int per_xcd, const uint32_t *__restrict__ pstart,
@@ -1296,38 +1298,42 @@ __device__ __forceinline__ void g2p_b4_kernel(const SceneDesc *__restrict__ scen
     brick_coords(sd, lb, bx, by, bz);
     const StepG2P a = g2p_view(sd, so, keep_order);
     const MacGrid g = mac_view(sd);
-    g2p_bbrick<ST, 4>(a, g, bx * 4, by * 4, bz * 4, pstart[gid], pstart[gid + 1], perm, slow);
+    g2p_bbrick<ST, 4, P2>(a, g, bx * 4, by * 4, bz * 4, pstart[gid], pstart[gid + 1], perm, slow);
 }
 
 // Trilinear: four work-groups per CU fit in LDS; four waves per SIMD.
+template <bool P2>
 __global__ __launch_bounds__(gbrick::NT, 4) void k_flip_g2p_bb_tl(
     const SceneDesc *__restrict__ scenes, int nsc, Sorted so, int keep_order, uint32_t nbrick,
     int per_xcd, const uint32_t *__restrict__ pstart, const uint32_t *__restrict__ perm,
     G2PSlow slow) {
-    g2p_bb_kernel<0>(scenes, nsc, so, keep_order, nbrick, per_xcd, pstart, perm, slow);
+    g2p_bb_kernel<0, P2>(scenes, nsc, so, keep_order, nbrick, per_xcd, pstart, perm, slow);
 }
 
 // Tricubic: three waves per SIMD (168 VGPRs) run about 10% faster than two
 // (256) at the cost of a few bytes of spill.
+template <bool P2>
 __global__ __launch_bounds__(gbrick::NT, 3) void k_flip_g2p_bb_tc(
     const SceneDesc *__restrict__ scenes, int nsc, Sorted so, int keep_order, uint32_t nbrick,
     int per_xcd, const uint32_t *__restrict__ pstart, const uint32_t *__restrict__ perm,
     G2PSlow slow) {
-    g2p_bb_kernel<1>(scenes, nsc, so, keep_order, nbrick, per_xcd, pstart, perm, slow);
+    g2p_bb_kernel<1, P2>(scenes, nsc, so, keep_order, nbrick, per_xcd, pstart, perm, slow);
 }
 
+template <bool P2>
 __global__ __launch_bounds__(gbrick::NT, 4) void k_flip_g2p_b4_tl(
     const SceneDesc *__restrict__ scenes, int nsc, Sorted so, int keep_order, uint32_t nbin,
     int per_xcd, const uint32_t *__restrict__ pstart, const uint32_t *__restrict__ perm,
     G2PSlow slow) {
-    g2p_b4_kernel<0>(scenes, nsc, so, keep_order, nbin, per_xcd, pstart, perm, slow);
+    g2p_b4_kernel<0, P2>(scenes, nsc, so, keep_order, nbin, per_xcd, pstart, perm, slow);
 }
 
+template <bool P2>
 __global__ __launch_bounds__(gbrick::NT, 3) void k_flip_g2p_b4_tc(
     const SceneDesc *__restrict__ scenes, int nsc, Sorted so, int keep_order, uint32_t nbin,
     int per_xcd, const uint32_t *__restrict__ pstart, const uint32_t *__restrict__ perm,
     G2PSlow slow) {
-    g2p_b4_kernel<1>(scenes, nsc, so, keep_order, nbin, per_xcd, pstart, perm, slow);
+    g2p_b4_kernel<1, P2>(scenes, nsc, so, keep_order, nbin, per_xcd, pstart, perm, slow);
 }
 
 // The particles the brick kernel left, sampled from global memory
@@ -1478,12 +1484,14 @@ void flip_step_brick(const StepBatch &s, const BrickSort &bs, hipStream_t st, hi
         fill_u32(bs.slow_count, 0u, 1, st);
         if (s.g2p_brick == 4) {
             const int bper = (int)((nbin + 7) / 8);
-            auto *kern = s.stencil == 0 ? k_flip_g2p_b4_tl : k_flip_g2p_b4_tc;
+            auto *kern = s.stencil == 0 ? (s.p2 ? k_flip_g2p_b4_tl<true> : k_flip_g2p_b4_tl<false>)
+                                        : (s.p2 ? k_flip_g2p_b4_tc<true> : k_flip_g2p_b4_tc<false>);
             hipLaunchKernelGGL(kern, dim3((unsigned)(8 * bper)), dim3(gbrick::NT), 0, st,
                                s.scenes_dev, nsc, so, s.keep_order, nbin, bper,
                                (const uint32_t *)bs.pstart, (const uint32_t *)bs.perm, slow);
         } else {
-            auto *kern = s.stencil == 0 ? k_flip_g2p_bb_tl : k_flip_g2p_bb_tc;
+            auto *kern = s.stencil == 0 ? (s.p2 ? k_flip_g2p_bb_tl<true> : k_flip_g2p_bb_tl<false>)
+                                        : (s.p2 ? k_flip_g2p_bb_tc<true> : k_flip_g2p_bb_tc<false>);
             hipLaunchKernelGGL(kern, dim3((unsigned)(8 * gper)), dim3(gbrick::NT), 0, st,
                                s.scenes_dev, nsc, so, s.keep_order, nbrick, gper,
                                (const uint32_t *)bs.pstart, (const uint32_t *)bs.perm, slow);

@@ -344,8 +344,9 @@ __device__ __forceinline__ bool in_box(const AxisCoords &cx, const AxisCoords &c
 
 // sample_mac with the LDS boxes as the first choice.  cur[c] / old[c] are
 // the staged boxes (edges E, EO; origins l*, lo*); g and o* are the global
-// arrays used when a stencil leaves its box.
-template <int ST, bool OLD, int GBE = gbrick::GB>
+// arrays used when a stencil leaves its box.  P2: g.inv_dx is a power of two
+// (cell coordinates in FP32, the same bits).
+template <int ST, bool OLD, int GBE = gbrick::GB, bool P2 = false>
 __device__ __forceinline__ void brick_sample(const MacGrid &g, const float *o0,
                                              const float *o1, const float *o2,
                                              const float *__restrict__ cur0,
@@ -365,9 +366,9 @@ __device__ __forceinline__ void brick_sample(const MacGrid &g, const float *o0,
         if (OLD) out_old[0] = out_old[1] = out_old[2] = 0.0f;
         return;
     }
-    const AxisCoords cx = axis_coords(x, g.inv_dx);
-    const AxisCoords cy = axis_coords(y, g.inv_dx);
-    const AxisCoords cz = axis_coords(z, g.inv_dx);
+    const AxisCoords cx = axis_coords_t<P2>(x, g.inv_dx);
+    const AxisCoords cy = axis_coords_t<P2>(y, g.inv_dx);
+    const AxisCoords cz = axis_coords_t<P2>(z, g.inv_dx);
     const int ni = g.ni, nj = g.nj, nk = g.nk;
     if (in_box<ST, B::E>(cx, cy, cz, lx, ly, lz)) {
         out[0] = box_comp<ST, B::E>(cur0, cx.ia - lx, cy.ib - ly, cz.ib - lz, cx.ta, cy.tb, cz.tb);
@@ -395,7 +396,7 @@ __device__ __forceinline__ void brick_sample(const MacGrid &g, const float *o0,
 // boxes: one base index per box from the half-shifted cells, the unshifted
 // axis of each component one step further.  The same arithmetic as the
 // in-box branch of brick_sample, so the results are the same bits.
-template <int ST, bool OLD, int GBE = gbrick::GB>
+template <int ST, bool OLD, int GBE = gbrick::GB, bool P2 = false>
 __device__ __forceinline__ void brick_sample_in(const MacGrid &g,
                                                 const float *__restrict__ cur0,
                                                 const float *__restrict__ cur1,
@@ -414,9 +415,9 @@ __device__ __forceinline__ void brick_sample_in(const MacGrid &g,
         if (OLD) out_old[0] = out_old[1] = out_old[2] = 0.0f;
         return;
     }
-    const AxisCoords cx = axis_coords(x, g.inv_dx);
-    const AxisCoords cy = axis_coords(y, g.inv_dx);
-    const AxisCoords cz = axis_coords(z, g.inv_dx);
+    const AxisCoords cx = axis_coords_t<P2>(x, g.inv_dx);
+    const AxisCoords cy = axis_coords_t<P2>(y, g.inv_dx);
+    const AxisCoords cz = axis_coords_t<P2>(z, g.inv_dx);
     const int dx = cx.ia - cx.ib, dy = cy.ia - cy.ib, dz = cz.ia - cz.ib;
     constexpr int E = B::E;
     const int b = (cx.ib - lx) + E * ((cy.ib - ly) + E * (cz.ib - lz));

@@ -461,6 +461,8 @@ struct StepBatch {
     int g2p_brick;     // cells per edge of a G2P brick with g2p_variant 0 and p2g_variant 4: 8 or 4
     const FsEpilogue *epi = nullptr;   // FluidSimulation epilogue after P2G (one scene)
     bool grids_only = false;           // stop after P2G (the adjoint's recompute)
+    bool p2 = false;                   // every scene's 1/dx is a power of two: the brick
+                                       // G2P takes cell coordinates in FP32 (same bits)
     const BrickSort *bs = nullptr;     // p2g_variant 4: the brick sort's workspace
 };
 

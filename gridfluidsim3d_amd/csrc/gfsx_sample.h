@@ -35,6 +35,32 @@ __device__ __forceinline__ AxisCoords axis_coords(float x, double inv_dx) {
     return c;
 }
 
+// axis_coords for x >= 0 when inv_dx is a power of two, in FP32 and with
+// the same bits.  x * inv_dx is then exact in FP32 (a change of exponent),
+// so are floor(ga) and r = ga - floor(ga) (r keeps a subset of ga's bits,
+// or is ga itself below 1), and ta = r as in FP64.  tb: r - 1/2 (r >= 1/2)
+// is exact; r + 1/2 (r < 1/2) is one rounding of the exact sum in FP32, as
+// (float)(r + 0.5) is one rounding of the exact FP64 sum.
+__device__ __forceinline__ AxisCoords axis_coords_p2(float x, float inv_dx) {
+    AxisCoords c;
+    const float ga = x * inv_dx;
+    const float fa = floorf(ga);
+    const float r = ga - fa;
+    const bool low = r < 0.5f;
+    c.ia = (int)fa;
+    c.ib = c.ia - (int)low;
+    c.ta = r;
+    c.tb = low ? r + 0.5f : r - 0.5f;
+    return c;
+}
+
+// axis_coords, or axis_coords_p2 when P2 (the caller's guarantee: inv_dx is
+// a power of two and x >= 0).
+template <bool P2>
+__device__ __forceinline__ AxisCoords axis_coords_t(float x, double inv_dx) {
+    return P2 ? axis_coords_p2(x, (float)inv_dx) : axis_coords(x, inv_dx);
+}
+
 __device__ __forceinline__ float ld_or_zero(const float *__restrict__ f,
                                             bool ok, uint32_t idx) {
     float v = f[idx];

@@ -242,6 +242,11 @@ void gfsx_flip_step_noalloc(gfsx_ctx *ctx, const gfsx_scene *scene,
  *                  2 = as 1, but P2G writes the sorted copies for G2P
  *                      (default).
  *                  Results are the same bits in every mode.
+ *   "fp32_cells":  1 = when 1/dx is a power of two (in every scene of the
+ *                      call), the brick G2P computes cell indices and
+ *                      fractions in FP32, where x/dx is exact: the same
+ *                      bits as FP64 (default),
+ *                  0 = always FP64.
  *   "adjoint_variant" (gfsx_flip_step_backward; results equal up to FP32
  *                  rounding):
  *                  0 = bricks of sorted particles sum the grid adjoints in

@@ -544,6 +544,71 @@ def test_step_particle_gather_bit_identical(ctx, kind, stencil, g2p_brick):
                 assert np.array_equal(ra[key], rb[key], equal_nan=True), (step, key)
 
 
+def _pow2_scene(kind):
+    """Scenes with 1/dx a power of two, where the brick G2P takes cell
+    coordinates in FP32: pad-layer particles (x < 0), positions on cell
+    faces, tiny and subnormal coordinates of either sign, NaN and far-out
+    particles."""
+    from gridfluidsim3d_amd import scenes
+    rng = np.random.default_rng(37)
+    if kind == "blobs":
+        sc = scenes.random_blobs(23, (27, 18, 13), 0.0625, amplitude=2.0)
+        return sc.pos, sc.vel, sc.dims, 0.0625, sc.dt
+    dx = 0.125 if kind == "edges" else 2.0
+    dims = (19, 17, 21)
+    W = np.array(dims) * dx
+    pos = rng.uniform(-1.5 * dx, W + 1.5 * dx, size=(20000, 3)).astype(np.float32)
+    k = np.arange(4000)
+    pos[:4000, 0] = (rng.integers(-1, dims[0] + 1, 4000) * dx).astype(np.float32)
+    pos[4000:5000, 1] = rng.choice(np.array([0.0, -0.0, 1e-30, -1e-30, 1e-42, -1e-42,
+                                             np.float32(1e-7) * dx, -np.float32(1e-7) * dx],
+                                            np.float32), 1000)
+    pos[5000:5100] = rng.choice(np.array([1e30, -1e30, 3e38], np.float32), (100, 3))
+    pos[7] = np.nan
+    pos[k[::97], 2] = np.nextafter(np.float32(W[2]), np.float32(0))
+    vel = (4.0 / 0.1 * dx * rng.normal(size=pos.shape)).astype(np.float32)
+    return pos, vel, dims, dx, 0.01
+
+
+@pytest.mark.parametrize("kind", ["blobs", "edges", "coarse"])
+@pytest.mark.parametrize("stencil", [TRILINEAR, TRICUBIC])
+@pytest.mark.parametrize("order", [0, 1])
+def test_step_fp32_cells_bit_identical(ctx, kind, stencil, order):
+    """With 1/dx a power of two the brick G2P takes cell coordinates in FP32
+    (fp32_cells 1): the same bits as the FP64 coordinates (0), three steps
+    in a row, G2P bricks of 8^3 and 4^3 cells."""
+    import gridfluidsim3d_amd as gf
+    pos, vel, dims, dx, dt = _pow2_scene(kind)
+
+    def run():
+        st = gf.FlipState(pos, vel, dims, dx, track_ids=True, keep_old=True,
+                          keep_weights=True)
+        res = []
+        for _ in range(3):
+            ctx.flip_step(st, dt, (0.0, -9.8, 0.0), 0.05, stencil, order)
+            _torch().cuda.synchronize()
+            res.append(dict(pos=_np(st.positions()), vel=_np(st.velocities()),
+                            id=_np(st.particle_ids()), u=_np(st.grid[0]),
+                            v=_np(st.grid[1]), w=_np(st.grid[2]), wu=_np(st.weights[0]),
+                            u_old=_np(st.grid_old[0])))
+        return res
+
+    old = ctx.get_option("fp32_cells"), ctx.get_option("g2p_brick")
+    runs = []
+    try:
+        for cells, brick in ((1, 8), (0, 8), (1, 4), (0, 4)):
+            ctx.set_option("fp32_cells", cells)
+            ctx.set_option("g2p_brick", brick)
+            runs.append(run())
+    finally:
+        ctx.set_option("fp32_cells", old[0])
+        ctx.set_option("g2p_brick", old[1])
+    for other in runs[1:]:
+        for step, (ra, rb) in enumerate(zip(runs[0], other)):
+            for key in ra:
+                assert np.array_equal(ra[key], rb[key], equal_nan=True), (step, key)
+
+
 def test_step_dense_cells_use_fallback(ctx, orc):
     """Bricks holding more particles than fit in LDS take the global-memory
     gather; results still match the oracle."""
@@ -575,6 +640,9 @@ def test_options(ctx):
     assert ctx.get_option("g2p_variant") == 0
     assert ctx.get_option("g2p_brick") == 8
     assert ctx.get_option("particle_gather") == 2
+    assert ctx.get_option("fp32_cells") == 1
+    with pytest.raises(RuntimeError, match="fp32_cells"):
+        ctx.set_option("fp32_cells", 2)
     with pytest.raises(RuntimeError, match="particle_gather"):
         ctx.set_option("particle_gather", 3)
     with pytest.raises(RuntimeError, match="g2p_variant"):

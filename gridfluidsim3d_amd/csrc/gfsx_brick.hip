@@ -124,11 +124,13 @@ __device__ __forceinline__ int wave_runs(uint32_t key, bool valid, int &j, int &
 // Count every bin's particles and runs.  The leader of each run adds
 // (1 << 32) | length to its bin, and records the bin, the length and the
 // number of runs the bin had before (a provisional rank, ordered later) in
-// runrec[wave * 64 + run].
+// runrec[wave * 64 + run].  Every particle's run and rank in the run go to
+// jr[wave * 64 + lane].
 __global__ __launch_bounds__(kBlock) void k_bsort_bin(const SceneDesc *__restrict__ scenes, int nsc,
                                                      unsigned long long *__restrict__ count,
                                                      uint2 *__restrict__ runrec,
-                                                     uint8_t *__restrict__ nruns) {
+                                                     uint8_t *__restrict__ nruns,
+                                                     uint16_t *__restrict__ jr) {
     using namespace bsort;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const uint32_t blk = blockIdx.x;
@@ -170,6 +172,7 @@ __global__ __launch_bounds__(kBlock) void k_bsort_bin(const SceneDesc *__restric
             }
         });
         if (lane == 0) nruns[wave] = (uint8_t)nr;
+        if (valid) jr[(size_t)wave * 64 + lane] = (uint16_t)((j << 6) | rank);
     }
 #pragma unroll
     for (int m = 0; m < ITEMS; m++) {
@@ -501,6 +504,33 @@ __global__ __launch_bounds__(kBlock) void k_bsort_order_huge(
 struct SortedOut {
     float *x[3], *v[3];
 };
+
+// The permutation alone (no copies): each particle's slot from the run and
+// rank k_bsort_bin recorded.
+__global__ __launch_bounds__(kBlock) void k_bsort_perm(const SceneDesc *__restrict__ scenes, int nsc,
+                                                      const uint16_t *__restrict__ jr,
+                                                      const uint32_t *__restrict__ runbase,
+                                                      uint32_t *__restrict__ perm) {
+    using namespace bsort;
+    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t blk = blockIdx.x;
+    const SceneDesc &sd = scenes[scene_of(scenes, nsc, blk, [](const SceneDesc &d) { return d.blk0; })];
+    const int64_t n = sd.n;
+    const int64_t base = (int64_t)(blk - sd.blk0) * TILE + threadIdx.x;
+    uint32_t e[ITEMS];
+#pragma unroll
+    for (int m = 0; m < ITEMS; m++) {
+        const uint32_t wave = blk * WAVES + (uint32_t)(m * (kBlock / 64) + wid);
+        e[m] = base + (int64_t)m * kBlock < n ? (uint32_t)gld(jr, (size_t)wave * 64 + lane) : 0xFFFFu;
+    }
+#pragma unroll
+    for (int m = 0; m < ITEMS; m++) {
+        if (e[m] == 0xFFFFu) continue;
+        const uint32_t wave = blk * WAVES + (uint32_t)(m * (kBlock / 64) + wid);
+        const uint32_t dst = gld(runbase, (size_t)wave * 64 + (e[m] >> 6)) + (e[m] & 63u);
+        perm[dst] = (uint32_t)(sd.p0 + base + (int64_t)m * kBlock);
+    }
+}
 
 __global__ __launch_bounds__(kBlock) void k_bsort_scatter(const SceneDesc *__restrict__ scenes, int nsc,
                                                          const uint32_t *__restrict__ runbase,
@@ -1401,7 +1431,7 @@ void flip_step_brick(const StepBatch &s, const BrickSort &bs, hipStream_t st, hi
     fill_u32((uint32_t *)bs.count, 0u, 2 * (int64_t)nbin, st);
     if (n > 0) {
         hipLaunchKernelGGL(k_bsort_bin, dim3(nblk), dim3(kBlock), 0, st, s.scenes_dev, nsc, bs.count,
-                           bs.runrec, bs.nruns);
+                           bs.runrec, bs.nruns, bs.jr);
         GFSX_LAUNCH_CHECK();
     }
     exclusive_scan64(bs.count, bs.start, nbin, bs.scan_scratch, st);
@@ -1435,8 +1465,12 @@ void flip_step_brick(const StepBatch &s, const BrickSort &bs, hipStream_t st, hi
                            (const uint32_t *)bs.list, bs.list2, (const uint2 *)bs.runrec,
                            bs.runbase, (const uint32_t *)bs.huge, (const uint32_t *)bs.lcount);
         GFSX_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_bsort_scatter, dim3(nblk), dim3(kBlock), 0, st, s.scenes_dev, nsc,
-                           (const uint32_t *)bs.runbase, so_out, bs.perm);
+        if (s.gather)
+            hipLaunchKernelGGL(k_bsort_perm, dim3(nblk), dim3(kBlock), 0, st, s.scenes_dev, nsc,
+                               (const uint16_t *)bs.jr, (const uint32_t *)bs.runbase, bs.perm);
+        else
+            hipLaunchKernelGGL(k_bsort_scatter, dim3(nblk), dim3(kBlock), 0, st, s.scenes_dev, nsc,
+                               (const uint32_t *)bs.runbase, so_out, bs.perm);
         GFSX_LAUNCH_CHECK();
     }
     if (ev) GFSX_HIP(hipEventRecord(ev[1], st));

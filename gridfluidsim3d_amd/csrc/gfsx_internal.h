@@ -418,6 +418,8 @@ struct BrickSort {
     uint32_t *rstart;            // nbb_total + 1: first entry of each bin's run list
     uint2 *runrec;               // 64 per wave: (bin | (len-1) << 26, provisional rank)
     uint8_t *nruns;              // per wave: its number of runs
+    uint16_t *jr;                // 64 per wave: each particle's run (bits 6..11) and rank
+                                 // in the run (bits 0..5)
     uint32_t *list;              // runs (wave*64 + run), grouped by bin
     uint32_t *list2;             // the same ordered, for bins of many runs
     uint32_t *runbase;           // 64 per wave: the first sort slot of each run

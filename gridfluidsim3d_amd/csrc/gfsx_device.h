@@ -304,10 +304,18 @@ static_assert(MAXROW <= 2 * 64, "two waves scan the rows");
 template <int ST, int E>
 __device__ __forceinline__ float box_comp_at(const float *__restrict__ b, float tx, float ty,
                                              float tz) {
+    typedef const __attribute__((address_space(3))) float LdsFloat;
     if (ST == 0) {
-        const float v000 = b[0], v100 = b[1], v010 = b[E], v110 = b[E + 1];
-        const float v001 = b[E * E], v101 = b[E * E + 1];
-        const float v011 = b[E * E + E], v111 = b[E * E + E + 1];
+        // b as an opaque LDS base: the 8 taps are immediate offsets of at
+        // most E^2 + E + 1 < 256 dwords from it (ds_read2's range), where
+        // the compiler would otherwise fold the box's own LDS offset into
+        // each read and add per read.  b points into an LDS box.
+        static_assert(E * E + E + 1 < 256, "box too large for one base");
+        LdsFloat *o = (LdsFloat *)b;
+        __asm__("" : "+v"(o));
+        const float v000 = o[0], v100 = o[1], v010 = o[E], v110 = o[E + 1];
+        const float v001 = o[E * E], v101 = o[E * E + 1];
+        const float v011 = o[E * E + E], v111 = o[E * E + E + 1];
         const float c00 = fmaf(tx, v100 - v000, v000);
         const float c10 = fmaf(tx, v110 - v010, v010);
         const float c01 = fmaf(tx, v101 - v001, v001);
@@ -316,9 +324,22 @@ __device__ __forceinline__ float box_comp_at(const float *__restrict__ b, float 
         const float c1 = fmaf(ty, c11 - c01, c01);
         return fmaf(tz, c1 - c0, c0);
     }
-    const float *o = b - (1 + E + E * E);
-    return tricubic_sum([&](int mi, int mj, int mk) { return o[mi + E * (mj + E * mk)]; }, tx,
-                        ty, tz);
+    // Two LDS bases, the stencil's lowest entry and the one two planes up,
+    // kept opaque to the compiler: every tap is then a non-negative
+    // immediate offset of at most 2E^2 + 3E + 3 < 256 dwords from one of
+    // them (ds_read2's offset range) instead of a negative or out-of-range
+    // offset that costs an address add per read.  b points into an LDS box
+    // (every caller's boxes are __shared__ arrays).
+    static_assert(E * E + 3 * E + 3 < 256, "box too large for two plane bases");
+    LdsFloat *o0 = (LdsFloat *)(b - (1 + E + E * E));
+    LdsFloat *o2 = o0 + 2 * E * E;
+    __asm__("" : "+v"(o0));
+    __asm__("" : "+v"(o2));
+    return tricubic_sum(
+        [&](int mi, int mj, int mk) {
+            return mk < 2 ? o0[mi + E * (mj + E * mk)] : o2[mi + E * (mj + E * (mk - 2))];
+        },
+        tx, ty, tz);
 }
 
 // The same with the base cell given relative to the box origin.

@@ -43,6 +43,8 @@ constexpr uint32_t MID = 256;                     // ... by one wave
 constexpr int HUGE_WORDS = 8192;                  // bitmap words per pass (k_bsort_order_huge)
 constexpr uint32_t BIN_MASK = (1u << kBrickBinBits) - 1u;
 constexpr int MID_BLOCKS = 2048, HUGE_BLOCKS = 256;
+constexpr int HT_BITS = 8, HT = 1 << HT_BITS;     // bin table entries per work-group
+constexpr int HT_PROBES = 16;                     // entries a run tries
 }  // namespace bsort
 
 constexpr int kSlowBlocks = 1024;   // work-groups of k_flip_g2p_slow
@@ -121,19 +123,31 @@ __device__ __forceinline__ int wave_runs(uint32_t key, bool valid, int &j, int &
 // Brick sort
 // ---------------------------------------------------------------------------
 
-// Count every bin's particles and runs.  The leader of each run adds
-// (1 << 32) | length to its bin, and records the bin, the length and the
-// number of runs the bin had before (a provisional rank, ordered later) in
-// runrec[wave * 64 + run].  Every particle's run and rank in the run go to
-// jr[wave * 64 + lane].
+// Count every bin's particles and runs.  Each run's leader records the
+// bin, the length and a provisional rank of the run among the bin's runs
+// (ordered later) in runrec[wave * 64 + run], and every particle's run and
+// rank in the run go to jr[wave * 64 + lane].  The runs of a work-group
+// are gathered per bin in an LDS table first, so that the work-group adds
+// (runs << 32) | particles to each of its bins with one atomic (the input
+// is close to brick order: a few bins per work-group); a run whose bin
+// finds no table entry adds its own (1 << 32) | length.  Either way the
+// provisional ranks of a bin's runs are distinct and below its run count.
 __global__ __launch_bounds__(kBlock) void k_bsort_bin(const SceneDesc *__restrict__ scenes, int nsc,
                                                      unsigned long long *__restrict__ count,
                                                      uint2 *__restrict__ runrec,
                                                      uint8_t *__restrict__ nruns,
                                                      uint16_t *__restrict__ jr) {
     using namespace bsort;
+    constexpr uint32_t EMPTY = 0xFFFFFFFFu;
+    __shared__ uint32_t s_key[HT];
+    __shared__ unsigned long long s_cnt[HT];
+    __shared__ uint32_t s_base[HT];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const uint32_t blk = blockIdx.x;
+    for (int e = threadIdx.x; e < HT; e += kBlock) {
+        s_key[e] = EMPTY;
+        s_cnt[e] = 0ull;
+    }
     const SceneDesc &sd = scenes[scene_of(scenes, nsc, blk, [](const SceneDesc &d) { return d.blk0; })];
     const int64_t n = sd.n;
     const uint32_t kb0 = sd.kb0;
@@ -147,10 +161,8 @@ __global__ __launch_bounds__(kBlock) void k_bsort_bin(const SceneDesc *__restric
         py[m] = gld(sd.xin[1], ii);
         pz[m] = gld(sd.xin[2], ii);
     }
-    // every run's atomic goes out before any result is used: a lane leads at
-    // most one run per item, and keeps that run's bin, length, index and old
-    // count until the end
-    unsigned long long old[ITEMS];
+    // a lane leads at most one run per item, and keeps that run's bin,
+    // length and index until the end
     uint32_t lbin[ITEMS], llen[ITEMS];
     int lrun[ITEMS];
 #pragma unroll
@@ -162,24 +174,54 @@ __global__ __launch_bounds__(kBlock) void k_bsort_bin(const SceneDesc *__restric
         lrun[m] = -1;
         lbin[m] = 0;
         llen[m] = 0;
-        old[m] = 0;
         const int nr = wave_runs(key, valid, j, rank, [&](bool lead, uint32_t k, uint32_t len, int r) {
             if (lead) {
                 lbin[m] = kb0 + k;
                 llen[m] = len;
                 lrun[m] = r;
-                old[m] = atomicAdd(&count[kb0 + k], (1ull << 32) | len);
             }
         });
         if (lane == 0) nruns[wave] = (uint8_t)nr;
         if (valid) jr[(size_t)wave * 64 + lane] = (uint16_t)((j << 6) | rank);
     }
+    __syncthreads();   // the table is clear
+    // every leader finds its bin's entry (linear probing from a hash of the
+    // bin) and adds its run there; the runs that find none add to count
+    int lslot[ITEMS];
+    unsigned long long lold[ITEMS];
+#pragma unroll
+    for (int m = 0; m < ITEMS; m++) {
+        lslot[m] = -1;
+        lold[m] = 0ull;
+        if (lrun[m] < 0) continue;
+        const uint32_t h = (lbin[m] * 2654435761u) >> (32 - HT_BITS);
+        for (int p = 0; p < HT_PROBES; p++) {
+            const uint32_t e = (h + (uint32_t)p) & (uint32_t)(HT - 1);
+            const uint32_t prev = atomicCAS(&s_key[e], EMPTY, lbin[m]);
+            if (prev == EMPTY || prev == lbin[m]) {
+                lslot[m] = (int)e;
+                break;
+            }
+        }
+        const unsigned long long add = (1ull << 32) | llen[m];
+        lold[m] = lslot[m] >= 0 ? atomicAdd(&s_cnt[lslot[m]], add) : atomicAdd(&count[lbin[m]], add);
+    }
+    __syncthreads();
+    // one atomic per bin of the work-group; its old run count is the base of
+    // the provisional ranks the work-group's runs of that bin took in LDS
+    for (int e = threadIdx.x; e < HT; e += kBlock) {
+        const uint32_t g = s_key[e];
+        if (g != EMPTY) s_base[e] = (uint32_t)(atomicAdd(&count[g], s_cnt[e]) >> 32);
+    }
+    __syncthreads();
 #pragma unroll
     for (int m = 0; m < ITEMS; m++) {
         const uint32_t wave = blk * WAVES + (uint32_t)(m * (kBlock / 64) + wid);
-        if (lrun[m] >= 0)
+        if (lrun[m] >= 0) {
+            const uint32_t prank = (uint32_t)(lold[m] >> 32) + (lslot[m] >= 0 ? s_base[lslot[m]] : 0u);
             runrec[(size_t)wave * 64 + lrun[m]] =
-                make_uint2(lbin[m] | ((llen[m] - 1u) << kBrickBinBits), (uint32_t)(old[m] >> 32));
+                make_uint2(lbin[m] | ((llen[m] - 1u) << kBrickBinBits), prank);
+        }
     }
 }
 

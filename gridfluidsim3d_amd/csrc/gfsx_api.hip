@@ -47,7 +47,7 @@ enum Slot {
     S_P_TILE, S_P_TFLAG,
     S_K_COUNT, S_K_START, S_K_PSTART, S_K_RSTART, S_K_RUNREC, S_K_NRUNS, S_K_LIST, S_K_LIST2,
     S_K_RUNBASE, S_K_MID, S_K_HUGE, S_K_LCOUNT, S_K_SCAN, S_K_FLUID, S_K_SSLOT,
-    S_K_SCOUNT, S_K_JR,
+    S_K_SCOUNT, S_K_JR, S_K_RLEN, S_K_NCHUNK,
     NSLOT
 };
 
@@ -249,7 +249,9 @@ gfsx::BrickSort brick_sort(WS &&ws, int64_t n, int64_t nblk, uint32_t nbin, bool
     b.rstart = (uint32_t *)ws(S_K_RSTART, u32 * ((size_t)nbin + 1));
     b.runrec = (uint2 *)ws(S_K_RUNREC, sizeof(uint2) * nw * 64);
     b.nruns = (uint8_t *)ws(S_K_NRUNS, nw);
-    b.jr = (uint16_t *)ws(S_K_JR, sizeof(uint16_t) * nw * 64);
+    b.jr = (uint32_t *)ws(S_K_JR, u32 * nw * 64);
+    b.rlen = (uint32_t *)ws(S_K_RLEN, u32 * nw * 64);
+    b.nchunk = (uint32_t *)ws(S_K_NCHUNK, u32 * (size_t)std::max<int64_t>(nblk, 1));
     b.list = (uint32_t *)ws(S_K_LIST, u32 * nn);
     b.list2 = (uint32_t *)ws(S_K_LIST2, u32 * nn);
     b.runbase = (uint32_t *)ws(S_K_RUNBASE, u32 * nw * 64);

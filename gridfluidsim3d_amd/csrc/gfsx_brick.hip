@@ -41,10 +41,10 @@ constexpr int WAVES = (int)(TILE / 64);           // waves per block tile
 constexpr uint32_t SMALL = 16;                    // runs per bin ordered by one thread
 constexpr uint32_t MID = 256;                     // ... by one wave
 constexpr int HUGE_WORDS = 8192;                  // bitmap words per pass (k_bsort_order_huge)
-constexpr uint32_t BIN_MASK = (1u << kBrickBinBits) - 1u;
 constexpr int MID_BLOCKS = 2048, HUGE_BLOCKS = 256;
 constexpr int HT_BITS = 8, HT = 1 << HT_BITS;     // bin table entries per work-group
 constexpr int HT_PROBES = 16;                     // entries a run tries
+constexpr uint32_t ALL_RUNS = 0xFFFFFFFFu;        // nchunk: every run a chunk of its own
 }  // namespace bsort
 
 constexpr int kSlowBlocks = 1024;   // work-groups of k_flip_g2p_slow
@@ -92,18 +92,20 @@ __device__ __forceinline__ uint32_t brick_key(const SceneDesc &sd, float x, floa
     return ok ? brick_local(sd, bx, by, bz) : sd.nbb;
 }
 
-// The runs of equal keys among the valid lanes of a wave, in the order of
-// each run's lowest lane: a lane receives its run's index j and its rank in
-// the run (lane order); at_leader(is_leader, key, length, j) is called by
-// every lane once per run.  Returns the number of runs.
+// The runs of equal keys among the valid lanes of a wave (all its lanes
+// with that key), in the order of each run's lowest lane: a lane receives
+// its run's index j, its rank in the run (lane order) and the run's lowest
+// lane; at_leader(is_leader, key, length, j) is called by every lane once
+// per run.  Returns the number of runs.
 template <class F>
 __device__ __forceinline__ int wave_runs(uint32_t key, bool valid, int &j, int &rank,
-                                         F &&at_leader) {
+                                         int &leader, F &&at_leader) {
     const int lane = threadIdx.x & 63;
     uint64_t rem = __ballot(valid);
     int r = 0;
     j = 0;
     rank = 0;
+    leader = 0;
     while (rem) {
         const int lead = __ffsll((unsigned long long)rem) - 1;
         const uint32_t k = (uint32_t)__builtin_amdgcn_readlane((int)key, lead);
@@ -111,6 +113,7 @@ __device__ __forceinline__ int wave_runs(uint32_t key, bool valid, int &j, int &
         if ((m >> lane) & 1ull) {
             j = r;
             rank = __popcll(m & ((1ull << lane) - 1ull));
+            leader = lead;
         }
         at_leader(lane == lead, k, (uint32_t)__popcll(m), r);
         rem &= ~m;
@@ -123,35 +126,49 @@ __device__ __forceinline__ int wave_runs(uint32_t key, bool valid, int &j, int &
 // Brick sort
 // ---------------------------------------------------------------------------
 
-// Count every bin's particles and runs.  Each run's leader records the
-// bin, the length and a provisional rank of the run among the bin's runs
-// (ordered later) in runrec[wave * 64 + run], and every particle's run and
-// rank in the run go to jr[wave * 64 + lane].  The runs of a work-group
-// are gathered per bin in an LDS table first, so that the work-group adds
-// (runs << 32) | particles to each of its bins with one atomic (the input
-// is close to brick order: a few bins per work-group); a run whose bin
-// finds no table entry adds its own (1 << 32) | length.  Either way the
-// provisional ranks of a bin's runs are distinct and below its run count.
+// Count every bin's particles and chunks.  A chunk is what one work-group
+// (TILE particles, 16 waves) holds of one bin: the input is close to brick
+// order, so a work-group's particles fall in a few bins.  The runs of each
+// wave (its particles of one bin) are gathered per bin in an LDS table: a
+// 16-bit mask of the waves with a run of that bin and each run's length,
+// so a run's offset in its chunk is the length of the bin's runs in the
+// waves before it (wave order, then lane order: the order of a stable
+// sort).  The work-group adds (1 << 32) | particles to each of its bins
+// with one atomic; the old value's high word is the chunk's provisional
+// rank among the bin's chunks (ordered later by chunk id).  Chunk records
+// (bin, provisional rank) and lengths go to runrec / rlen at
+// blk * TILE + c, c the chunk's index among the work-group's bins in table
+// order; nchunk[blk] = their number.  A work-group whose runs do not all
+// find a table entry takes each run as a chunk of its own, c = wave * 64 +
+// run, and sets nchunk[blk] = ALL_RUNS.  Each particle's chunk and offset
+// in it go to jr (c << 10 | offset) at its binning position.
 __global__ __launch_bounds__(kBlock) void k_bsort_bin(const SceneDesc *__restrict__ scenes, int nsc,
                                                      unsigned long long *__restrict__ count,
                                                      uint2 *__restrict__ runrec,
+                                                     uint32_t *__restrict__ rlen,
                                                      uint8_t *__restrict__ nruns,
-                                                     uint16_t *__restrict__ jr) {
+                                                     uint32_t *__restrict__ nchunk,
+                                                     uint32_t *__restrict__ jr) {
     using namespace bsort;
+    static_assert(HT == kBlock && WAVES == 16, "one table entry per thread, 16-bit wave masks");
     constexpr uint32_t EMPTY = 0xFFFFFFFFu;
-    __shared__ uint32_t s_key[HT];
-    __shared__ unsigned long long s_cnt[HT];
-    __shared__ uint32_t s_base[HT];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    __shared__ uint32_t s_key[HT];       // bin, or EMPTY
+    __shared__ uint32_t s_cnt[HT];       // particles
+    __shared__ uint32_t s_mask[HT];      // waves with a run of the bin
+    __shared__ uint16_t s_len[HT][WAVES];   // those runs' lengths
+    __shared__ uint32_t s_cc[HT];        // chunk index of a used entry
+    __shared__ uint32_t s_wsum[kBlock / 64];
+    __shared__ int s_fail;
+    const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
     const uint32_t blk = blockIdx.x;
-    for (int e = threadIdx.x; e < HT; e += kBlock) {
-        s_key[e] = EMPTY;
-        s_cnt[e] = 0ull;
-    }
+    s_key[t] = EMPTY;
+    s_cnt[t] = 0u;
+    s_mask[t] = 0u;
+    if (t == 0) s_fail = 0;
     const SceneDesc &sd = scenes[scene_of(scenes, nsc, blk, [](const SceneDesc &d) { return d.blk0; })];
     const int64_t n = sd.n;
     const uint32_t kb0 = sd.kb0;
-    const int64_t base = (int64_t)(blk - sd.blk0) * TILE + threadIdx.x;
+    const int64_t base = (int64_t)(blk - sd.blk0) * TILE + t;
     float px[ITEMS], py[ITEMS], pz[ITEMS];
 #pragma unroll
     for (int m = 0; m < ITEMS; m++) {
@@ -161,38 +178,36 @@ __global__ __launch_bounds__(kBlock) void k_bsort_bin(const SceneDesc *__restric
         py[m] = gld(sd.xin[1], ii);
         pz[m] = gld(sd.xin[2], ii);
     }
-    // a lane leads at most one run per item, and keeps that run's bin,
-    // length and index until the end
+    // a lane leads at most one run per item; every lane keeps its run's
+    // index, its rank in it and its leader
     uint32_t lbin[ITEMS], llen[ITEMS];
-    int lrun[ITEMS];
+    int lrun[ITEMS], lj[ITEMS], lrank[ITEMS], llead[ITEMS];
 #pragma unroll
     for (int m = 0; m < ITEMS; m++) {
         const bool valid = base + (int64_t)m * kBlock < n;
         const uint32_t key = valid ? brick_key(sd, px[m], py[m], pz[m]) : 0u;
         const uint32_t wave = blk * WAVES + (uint32_t)(m * (kBlock / 64) + wid);
-        int j, rank;
         lrun[m] = -1;
         lbin[m] = 0;
         llen[m] = 0;
-        const int nr = wave_runs(key, valid, j, rank, [&](bool lead, uint32_t k, uint32_t len, int r) {
-            if (lead) {
-                lbin[m] = kb0 + k;
-                llen[m] = len;
-                lrun[m] = r;
-            }
-        });
+        const int nr = wave_runs(key, valid, lj[m], lrank[m], llead[m],
+                                 [&](bool lead, uint32_t k, uint32_t len, int r) {
+                                     if (lead) {
+                                         lbin[m] = kb0 + k;
+                                         llen[m] = len;
+                                         lrun[m] = r;
+                                     }
+                                 });
         if (lane == 0) nruns[wave] = (uint8_t)nr;
-        if (valid) jr[(size_t)wave * 64 + lane] = (uint16_t)((j << 6) | rank);
+        if (!valid) lj[m] = -1;
     }
     __syncthreads();   // the table is clear
     // every leader finds its bin's entry (linear probing from a hash of the
-    // bin) and adds its run there; the runs that find none add to count
+    // bin) and enters its run there
     int lslot[ITEMS];
-    unsigned long long lold[ITEMS];
 #pragma unroll
     for (int m = 0; m < ITEMS; m++) {
         lslot[m] = -1;
-        lold[m] = 0ull;
         if (lrun[m] < 0) continue;
         const uint32_t h = (lbin[m] * 2654435761u) >> (32 - HT_BITS);
         for (int p = 0; p < HT_PROBES; p++) {
@@ -203,25 +218,64 @@ __global__ __launch_bounds__(kBlock) void k_bsort_bin(const SceneDesc *__restric
                 break;
             }
         }
-        const unsigned long long add = (1ull << 32) | llen[m];
-        lold[m] = lslot[m] >= 0 ? atomicAdd(&s_cnt[lslot[m]], add) : atomicAdd(&count[lbin[m]], add);
+        const int wv = m * (kBlock / 64) + wid;
+        if (lslot[m] >= 0) {
+            atomicAdd(&s_cnt[lslot[m]], llen[m]);
+            atomicOr(&s_mask[lslot[m]], 1u << wv);
+            s_len[lslot[m]][wv] = (uint16_t)llen[m];
+        } else {
+            s_fail = 1;
+        }
     }
     __syncthreads();
-    // one atomic per bin of the work-group; its old run count is the base of
-    // the provisional ranks the work-group's runs of that bin took in LDS
-    for (int e = threadIdx.x; e < HT; e += kBlock) {
-        const uint32_t g = s_key[e];
-        if (g != EMPTY) s_base[e] = (uint32_t)(atomicAdd(&count[g], s_cnt[e]) >> 32);
+    const bool all_runs = s_fail != 0;
+    const size_t c0 = (size_t)blk * TILE;
+    if (!all_runs) {
+        // the used entries in table order are the chunks
+        const bool used = s_key[t] != EMPTY;
+        uint32_t tot;
+        const uint32_t cc = block_exclusive_sum(used ? 1u : 0u, s_wsum, tot);
+        s_cc[t] = cc;
+        if (used) {
+            const uint32_t g = s_key[t];
+            const unsigned long long old = atomicAdd(&count[g], (1ull << 32) | s_cnt[t]);
+            runrec[c0 + cc] = make_uint2(g, (uint32_t)(old >> 32));
+            rlen[c0 + cc] = s_cnt[t];
+        }
+        if (t == 0) nchunk[blk] = tot;
+    } else {
+#pragma unroll
+        for (int m = 0; m < ITEMS; m++) {
+            if (lrun[m] < 0) continue;
+            const size_t id = c0 + (size_t)(m * (kBlock / 64) + wid) * 64 + (size_t)lrun[m];
+            const unsigned long long old = atomicAdd(&count[lbin[m]], (1ull << 32) | llen[m]);
+            runrec[id] = make_uint2(lbin[m], (uint32_t)(old >> 32));
+            rlen[id] = llen[m];
+        }
+        if (t == 0) nchunk[blk] = ALL_RUNS;
     }
-    __syncthreads();
+    __syncthreads();   // s_cc
 #pragma unroll
     for (int m = 0; m < ITEMS; m++) {
-        const uint32_t wave = blk * WAVES + (uint32_t)(m * (kBlock / 64) + wid);
+        const int wv = m * (kBlock / 64) + wid;
+        // the leader: its run's chunk and the run's offset in it
+        uint32_t rcc = 0, roff = 0;
         if (lrun[m] >= 0) {
-            const uint32_t prank = (uint32_t)(lold[m] >> 32) + (lslot[m] >= 0 ? s_base[lslot[m]] : 0u);
-            runrec[(size_t)wave * 64 + lrun[m]] =
-                make_uint2(lbin[m] | ((llen[m] - 1u) << kBrickBinBits), prank);
+            if (!all_runs) {
+                const int e = lslot[m];
+                rcc = s_cc[e];
+                uint32_t mk = s_mask[e] & ((1u << wv) - 1u);
+                while (mk) {
+                    roff += s_len[e][__ffs(mk) - 1];
+                    mk &= mk - 1u;
+                }
+            } else {
+                rcc = (uint32_t)(wv * 64 + lrun[m]);
+            }
         }
+        const uint32_t cc = (uint32_t)__shfl((int)rcc, llead[m], 64);
+        const uint32_t off = (uint32_t)__shfl((int)roff, llead[m], 64);
+        if (lj[m] >= 0) jr[c0 + (size_t)wv * 64 + lane] = (cc << 10) | (off + (uint32_t)lrank[m]);
     }
 }
 
@@ -341,35 +395,44 @@ __global__ void k_bsort_split(const unsigned long long *__restrict__ start, uint
     rstart[b] = (uint32_t)(v >> 32);
 }
 
-// Every run into its bin's run list, at its provisional rank.
-__global__ __launch_bounds__(kBlock) void k_bsort_list(int64_t nwave, const uint8_t *__restrict__ nruns,
+// Every chunk into its bin's chunk list, at its provisional rank (one
+// thread per binning work-group).  nlist bounds the list.
+__global__ __launch_bounds__(kBlock) void k_bsort_list(int64_t nblk, const uint32_t *__restrict__ nchunk,
+                                                      const uint8_t *__restrict__ nruns,
                                                       const uint2 *__restrict__ runrec,
                                                       const uint32_t *__restrict__ rstart,
-                                                      uint32_t *__restrict__ list) {
-    const int64_t w = blockIdx.x * (int64_t)kBlock + threadIdx.x;
-    if (w >= nwave) return;
-    const int nr = nruns[w];
-    for (int r = 0; r < nr; r++) {
-        const uint2 rec = runrec[w * 64 + r];
-        list[rstart[rec.x & bsort::BIN_MASK] + rec.y] = (uint32_t)(w * 64 + r);
+                                                      uint32_t *__restrict__ list, uint32_t nlist) {
+    using namespace bsort;
+    const int64_t b = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    if (b >= nblk) return;
+    const uint32_t c0 = (uint32_t)b * (uint32_t)TILE;
+    auto put = [&](uint32_t id) {
+        const uint2 rec = runrec[id];
+        const uint32_t at = rstart[rec.x] + rec.y;
+        if (at < nlist) list[at] = id;
+    };
+    const uint32_t nc = nchunk[b];
+    if (nc != ALL_RUNS) {
+        for (uint32_t c = 0; c < nc; c++) put(c0 + c);
+    } else {
+        for (int w = 0; w < WAVES; w++) {
+            const int nr = nruns[(size_t)b * WAVES + w];
+            for (int r = 0; r < nr; r++) put(c0 + (uint32_t)(w * 64 + r));
+        }
     }
-}
-
-__device__ __forceinline__ uint32_t run_len(const uint2 *__restrict__ runrec, uint32_t run) {
-    return (runrec[run].x >> kBrickBinBits) + 1u;
 }
 
 // Sorted run ids of one bin (ascending = wave order) -> each run's first
 // sort slot: the bin's first slot plus the lengths of the runs before it.
 template <int N>
 __device__ __forceinline__ void write_bases(const uint32_t (&v)[N], uint32_t len, uint32_t at,
-                                            const uint2 *__restrict__ runrec,
+                                            const uint32_t *__restrict__ rlen,
                                             uint32_t *__restrict__ runbase) {
 #pragma unroll
     for (int m = 0; m < N; m++) {
         if ((uint32_t)m < len) {
             runbase[v[m]] = at;
-            at += run_len(runrec, v[m]);
+            at += rlen[v[m]];
         }
     }
 }
@@ -378,7 +441,7 @@ __device__ __forceinline__ void write_bases(const uint32_t (&v)[N], uint32_t len
 // larger bins are listed for the wave and work-group passes.
 __global__ __launch_bounds__(kBlock) void k_bsort_order_small(
     uint32_t nbin, const uint32_t *__restrict__ rstart, const uint32_t *__restrict__ pstart,
-    const uint32_t *__restrict__ list, const uint2 *__restrict__ runrec,
+    const uint32_t *__restrict__ list, const uint32_t *__restrict__ rlen,
     uint32_t *__restrict__ runbase, uint32_t *__restrict__ mid, uint32_t *__restrict__ huge,
     uint32_t *__restrict__ lcount) {
     using namespace bsort;
@@ -417,13 +480,13 @@ __global__ __launch_bounds__(kBlock) void k_bsort_order_small(
 #pragma unroll
         for (int m = 0; m < 16; m++) v[m] = (uint32_t)m < len ? list[s + m] : 0xFFFFFFFFu;
         bitonic_sort<16>(v);
-        write_bases<16>(v, len, at, runrec, runbase);
+        write_bases<16>(v, len, at, rlen, runbase);
     } else if (need8) {
         uint32_t v[8];
 #pragma unroll
         for (int m = 0; m < 8; m++) v[m] = (uint32_t)m < len ? list[s + m] : 0xFFFFFFFFu;
         bitonic_sort<8>(v);
-        write_bases<8>(v, len, at, runrec, runbase);
+        write_bases<8>(v, len, at, rlen, runbase);
     }
 }
 
@@ -432,7 +495,7 @@ __global__ __launch_bounds__(kBlock) void k_bsort_order_small(
 template <int K>
 __device__ __forceinline__ void wave_order_runs(uint32_t s, uint32_t len, uint32_t at,
                                                 const uint32_t *__restrict__ list,
-                                                const uint2 *__restrict__ runrec,
+                                                const uint32_t *__restrict__ rlen,
                                                 uint32_t *__restrict__ runbase) {
     const int lane = threadIdx.x & 63;
     uint64_t v[K];
@@ -446,7 +509,7 @@ __device__ __forceinline__ void wave_order_runs(uint32_t s, uint32_t len, uint32
 #pragma unroll
     for (int r = 0; r < K; r++) {
         const uint32_t e = (uint32_t)(lane * K + r);
-        l[r] = e < len ? run_len(runrec, (uint32_t)v[r]) : 0u;
+        l[r] = e < len ? rlen[(uint32_t)v[r]] : 0u;
         tot += l[r];
     }
     uint32_t pos = at + wave_inclusive_sum(tot) - tot;
@@ -460,7 +523,7 @@ __device__ __forceinline__ void wave_order_runs(uint32_t s, uint32_t len, uint32
 
 __global__ __launch_bounds__(kBlock) void k_bsort_order_mid(
     const uint32_t *__restrict__ rstart, const uint32_t *__restrict__ pstart,
-    const uint32_t *__restrict__ list, const uint2 *__restrict__ runrec,
+    const uint32_t *__restrict__ list, const uint32_t *__restrict__ rlen,
     uint32_t *__restrict__ runbase, const uint32_t *__restrict__ mid,
     const uint32_t *__restrict__ lcount) {
     const uint32_t nmid = lcount[0];
@@ -468,21 +531,23 @@ __global__ __launch_bounds__(kBlock) void k_bsort_order_mid(
     for (uint32_t c = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); c < nmid; c += waves) {
         const uint32_t b = mid[c];
         const uint32_t s = rstart[b], len = rstart[b + 1] - s, at = pstart[b];
-        if (len <= 64u) wave_order_runs<1>(s, len, at, list, runrec, runbase);
-        else wave_order_runs<4>(s, len, at, list, runrec, runbase);
+        if (len <= 64u) wave_order_runs<1>(s, len, at, list, rlen, runbase);
+        else wave_order_runs<4>(s, len, at, list, rlen, runbase);
     }
 }
 
-// Bins of more than MID runs: one work-group each.  A bin holds at most one
-// run per wave, so ordering its runs means ordering their wave indices: the
-// work-group sets one bit per wave in an LDS bitmap (a window of
-// HUGE_WORDS*32 waves at a time), ranks each run by the population count
-// below its bit, writes the runs in that order to list2, then takes the
-// bases from a prefix sum of the lengths in that order.
+// Bins of more than MID chunks: one work-group each.  Chunk ids are
+// blk * TILE + c with c below 256 (one chunk per bin and work-group) or
+// wave * 64 + run (one per bin and wave), so a bin holds at most one chunk
+// per id >> 6, and ordering its chunks means ordering those: the
+// work-group sets one bit per id >> 6 in an LDS bitmap (a window of
+// HUGE_WORDS*32 at a time), ranks each chunk by the population count below
+// its bit, writes the chunks in that order to list2, then takes the bases
+// from a prefix sum of the lengths in that order.
 __global__ __launch_bounds__(kBlock) void k_bsort_order_huge(
     int64_t nwave, const uint32_t *__restrict__ rstart, const uint32_t *__restrict__ pstart,
     const uint32_t *__restrict__ list, uint32_t *__restrict__ list2,
-    const uint2 *__restrict__ runrec, uint32_t *__restrict__ runbase,
+    const uint32_t *__restrict__ rlen, uint32_t *__restrict__ runbase,
     const uint32_t *__restrict__ huge, const uint32_t *__restrict__ lcount) {
     using namespace bsort;
     __shared__ uint32_t bits[HUGE_WORDS], pre[HUGE_WORDS];
@@ -530,7 +595,7 @@ __global__ __launch_bounds__(kBlock) void k_bsort_order_huge(
         for (uint32_t e0 = 0; e0 < len; e0 += kBlock) {
             const uint32_t e = e0 + t;
             const uint32_t id = e < len ? list2[s + e] : 0u;
-            const uint32_t l = e < len ? run_len(runrec, id) : 0u;
+            const uint32_t l = e < len ? rlen[id] : 0u;
             uint32_t tot;
             const uint32_t ex = block_exclusive_sum(l, wsum, tot);
             if (e < len) runbase[id] = at + ex;
@@ -541,79 +606,52 @@ __global__ __launch_bounds__(kBlock) void k_bsort_order_huge(
 }
 
 // Positions and velocities (when out.x[0] is set) and the particle index
-// (when perm is set) written to their sort slots: the run's first slot plus
-// the rank in the run, recomputed exactly as k_bsort_bin found them.
+// (when perm is set) written to their sort slots: the chunk's first slot
+// plus the particle's offset in the chunk, both as k_bsort_bin recorded
+// them (jr).  nslot bounds the slots.
 struct SortedOut {
     float *x[3], *v[3];
 };
 
-// The permutation alone (no copies): each particle's slot from the run and
-// rank k_bsort_bin recorded.
-__global__ __launch_bounds__(kBlock) void k_bsort_perm(const SceneDesc *__restrict__ scenes, int nsc,
-                                                      const uint16_t *__restrict__ jr,
-                                                      const uint32_t *__restrict__ runbase,
-                                                      uint32_t *__restrict__ perm) {
-    using namespace bsort;
-    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const uint32_t blk = blockIdx.x;
-    const SceneDesc &sd = scenes[scene_of(scenes, nsc, blk, [](const SceneDesc &d) { return d.blk0; })];
-    const int64_t n = sd.n;
-    const int64_t base = (int64_t)(blk - sd.blk0) * TILE + threadIdx.x;
-    uint32_t e[ITEMS];
-#pragma unroll
-    for (int m = 0; m < ITEMS; m++) {
-        const uint32_t wave = blk * WAVES + (uint32_t)(m * (kBlock / 64) + wid);
-        e[m] = base + (int64_t)m * kBlock < n ? (uint32_t)gld(jr, (size_t)wave * 64 + lane) : 0xFFFFu;
-    }
-#pragma unroll
-    for (int m = 0; m < ITEMS; m++) {
-        if (e[m] == 0xFFFFu) continue;
-        const uint32_t wave = blk * WAVES + (uint32_t)(m * (kBlock / 64) + wid);
-        const uint32_t dst = gld(runbase, (size_t)wave * 64 + (e[m] >> 6)) + (e[m] & 63u);
-        perm[dst] = (uint32_t)(sd.p0 + base + (int64_t)m * kBlock);
-    }
-}
-
 __global__ __launch_bounds__(kBlock) void k_bsort_scatter(const SceneDesc *__restrict__ scenes, int nsc,
+                                                         const uint32_t *__restrict__ jr,
                                                          const uint32_t *__restrict__ runbase,
-                                                         SortedOut out, uint32_t *__restrict__ perm) {
+                                                         SortedOut out, uint32_t *__restrict__ perm,
+                                                         uint32_t nslot) {
     using namespace bsort;
-    const int wid = threadIdx.x >> 6;
     const uint32_t blk = blockIdx.x;
     const SceneDesc &sd = scenes[scene_of(scenes, nsc, blk, [](const SceneDesc &d) { return d.blk0; })];
     const int64_t n = sd.n;
     const int64_t base = (int64_t)(blk - sd.blk0) * TILE + threadIdx.x;
+    const size_t c0 = (size_t)blk * TILE;
     const bool copies = out.x[0] != nullptr;
+    uint32_t e[ITEMS];
     float p[ITEMS][6];
 #pragma unroll
     for (int m = 0; m < ITEMS; m++) {
         const int64_t i = base + (int64_t)m * kBlock;
-        const int64_t ii = i < n ? i : 0;
+        const bool valid = i < n;
+        const int64_t ii = valid ? i : 0;
+        e[m] = valid ? gld(jr, c0 + (size_t)(m * kBlock) + threadIdx.x) : 0xFFFFFFFFu;
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            p[m][c] = gld(sd.xin[c], ii);
+            p[m][c] = copies ? gld(sd.xin[c], ii) : 0.0f;
             p[m][3 + c] = copies ? gld(sd.vin[c], ii) : 0.0f;
         }
     }
 #pragma unroll
     for (int m = 0; m < ITEMS; m++) {
-        const int64_t i = base + (int64_t)m * kBlock;
-        const bool valid = i < n;
-        const uint32_t key = valid ? brick_key(sd, p[m][0], p[m][1], p[m][2]) : 0u;
-        const uint32_t wave = blk * WAVES + (uint32_t)(m * (kBlock / 64) + wid);
-        int j, rank;
-        wave_runs(key, valid, j, rank, [](bool, uint32_t, uint32_t, int) {});
-        if (valid) {
-            const uint32_t dst = gld(runbase, (size_t)wave * 64 + j) + (uint32_t)rank;
-            if (copies) {
+        if (e[m] == 0xFFFFFFFFu) continue;
+        const uint32_t dst = gld(runbase, c0 + (e[m] >> 10)) + (e[m] & 1023u);
+        if (dst >= nslot) continue;
+        if (copies) {
 #pragma unroll
-                for (int c = 0; c < 3; c++) {
-                    out.x[c][dst] = p[m][c];
-                    out.v[c][dst] = p[m][3 + c];
-                }
+            for (int c = 0; c < 3; c++) {
+                out.x[c][dst] = p[m][c];
+                out.v[c][dst] = p[m][3 + c];
             }
-            if (perm) perm[dst] = (uint32_t)(sd.p0 + i);
         }
+        if (perm) perm[dst] = (uint32_t)(sd.p0 + base + (int64_t)m * kBlock);
     }
 }
 
@@ -1473,7 +1511,7 @@ void flip_step_brick(const StepBatch &s, const BrickSort &bs, hipStream_t st, hi
     fill_u32((uint32_t *)bs.count, 0u, 2 * (int64_t)nbin, st);
     if (n > 0) {
         hipLaunchKernelGGL(k_bsort_bin, dim3(nblk), dim3(kBlock), 0, st, s.scenes_dev, nsc, bs.count,
-                           bs.runrec, bs.nruns, bs.jr);
+                           bs.runrec, bs.rlen, bs.nruns, bs.nchunk, bs.jr);
         GFSX_LAUNCH_CHECK();
     }
     exclusive_scan64(bs.count, bs.start, nbin, bs.scan_scratch, st);
@@ -1487,32 +1525,30 @@ void flip_step_brick(const StepBatch &s, const BrickSort &bs, hipStream_t st, hi
     }
     if (s.gather && !bs.perm) throw Error("gfsx: the particle gather needs the sort's permutation");
     if (n > 0) {
-        hipLaunchKernelGGL(k_bsort_list, dim3(grid_for(bs.nwave, kBlock)), dim3(kBlock), 0, st,
-                           bs.nwave, (const uint8_t *)bs.nruns, (const uint2 *)bs.runrec,
-                           (const uint32_t *)bs.rstart, bs.list);
+        hipLaunchKernelGGL(k_bsort_list, dim3(grid_for((int64_t)nblk, kBlock)), dim3(kBlock), 0, st,
+                           (int64_t)nblk, (const uint32_t *)bs.nchunk, (const uint8_t *)bs.nruns,
+                           (const uint2 *)bs.runrec, (const uint32_t *)bs.rstart, bs.list,
+                           (uint32_t)n);
         GFSX_LAUNCH_CHECK();
         fill_u32(bs.lcount, 0u, 2, st);
         hipLaunchKernelGGL(k_bsort_order_small, dim3(grid_for(nbin, kBlock)), dim3(kBlock), 0, st,
                            nbin, (const uint32_t *)bs.rstart, (const uint32_t *)bs.pstart,
-                           (const uint32_t *)bs.list, (const uint2 *)bs.runrec, bs.runbase, bs.mid,
+                           (const uint32_t *)bs.list, (const uint32_t *)bs.rlen, bs.runbase, bs.mid,
                            bs.huge, bs.lcount);
         GFSX_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_bsort_order_mid, dim3(MID_BLOCKS), dim3(kBlock), 0, st,
                            (const uint32_t *)bs.rstart, (const uint32_t *)bs.pstart,
-                           (const uint32_t *)bs.list, (const uint2 *)bs.runrec, bs.runbase,
+                           (const uint32_t *)bs.list, (const uint32_t *)bs.rlen, bs.runbase,
                            (const uint32_t *)bs.mid, (const uint32_t *)bs.lcount);
         GFSX_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_bsort_order_huge, dim3(HUGE_BLOCKS), dim3(kBlock), 0, st, bs.nwave,
                            (const uint32_t *)bs.rstart, (const uint32_t *)bs.pstart,
-                           (const uint32_t *)bs.list, bs.list2, (const uint2 *)bs.runrec,
+                           (const uint32_t *)bs.list, bs.list2, (const uint32_t *)bs.rlen,
                            bs.runbase, (const uint32_t *)bs.huge, (const uint32_t *)bs.lcount);
         GFSX_LAUNCH_CHECK();
-        if (s.gather)
-            hipLaunchKernelGGL(k_bsort_perm, dim3(nblk), dim3(kBlock), 0, st, s.scenes_dev, nsc,
-                               (const uint16_t *)bs.jr, (const uint32_t *)bs.runbase, bs.perm);
-        else
-            hipLaunchKernelGGL(k_bsort_scatter, dim3(nblk), dim3(kBlock), 0, st, s.scenes_dev, nsc,
-                               (const uint32_t *)bs.runbase, so_out, bs.perm);
+        hipLaunchKernelGGL(k_bsort_scatter, dim3(nblk), dim3(kBlock), 0, st, s.scenes_dev, nsc,
+                           (const uint32_t *)bs.jr, (const uint32_t *)bs.runbase, so_out, bs.perm,
+                           (uint32_t)n);
         GFSX_LAUNCH_CHECK();
     }
     if (ev) GFSX_HIP(hipEventRecord(ev[1], st));

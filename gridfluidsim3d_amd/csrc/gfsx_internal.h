@@ -416,13 +416,15 @@ struct BrickSort {
     unsigned long long *start;   // nbb_total + 1: exclusive scan of count
     uint32_t *pstart;            // nbb_total + 1: first sort slot of each bin
     uint32_t *rstart;            // nbb_total + 1: first entry of each bin's run list
-    uint2 *runrec;               // 64 per wave: (bin | (len-1) << 26, provisional rank)
+    uint2 *runrec;               // TILE per binning work-group: chunk (bin, provisional rank)
+    uint32_t *rlen;              // TILE per binning work-group: chunk length
     uint8_t *nruns;              // per wave: its number of runs
-    uint16_t *jr;                // 64 per wave: each particle's run (bits 6..11) and rank
-                                 // in the run (bits 0..5)
-    uint32_t *list;              // runs (wave*64 + run), grouped by bin
+    uint32_t *nchunk;            // per binning work-group: chunks, or every run a chunk
+    uint32_t *jr;                // per binning position: chunk (bits 10..) and offset in
+                                 // it (bits 0..9)
+    uint32_t *list;              // chunks (blk*TILE + c), grouped by bin
     uint32_t *list2;             // the same ordered, for bins of many runs
-    uint32_t *runbase;           // 64 per wave: the first sort slot of each run
+    uint32_t *runbase;           // TILE per binning work-group: each chunk's first slot
     uint32_t *mid, *huge;        // bins of 17..256 runs, of more
     uint32_t *lcount;            // 2: entries of mid and huge
     unsigned long long *scan_scratch;
@@ -435,7 +437,7 @@ struct BrickSort {
     int64_t nwave;
     uint32_t nbb_total;
 };
-constexpr uint32_t kBrickBinBits = 26;   // global bins below 2^26 (runrec packs len above)
+constexpr uint32_t kBrickBinBits = 26;   // global bins below 2^26
 int64_t scan64_scratch_words(int64_t m);
 
 // A fused step over one or more scenes.

@@ -38,7 +38,7 @@ extern "C" {
 #define GFSX_TRICUBIC 1
 
 /* Output order of gfsx_flip_step. */
-#define GFSX_ORDER_SORTED 0 /* particles come out grouped by cell */
+#define GFSX_ORDER_SORTED 0 /* particles come out grouped by brick of 4x4x4 cells */
 #define GFSX_ORDER_KEEP 1   /* particle i stays at index i */
 
 typedef struct gfsx_ctx gfsx_ctx;
@@ -133,7 +133,8 @@ void gfsx_advect_device(gfsx_ctx *ctx, int order, const float *pos, int64_t n,
  * 3306, 2766-2805, 3104-3129, 3181-3209).  Particles are structure-of-arrays
  * device buffers; the *_in arrays are read, the *_out arrays written (they
  * must not alias the inputs).  With GFSX_ORDER_SORTED the outputs come out
- * grouped by cell (the reference reorders its particles every step as well,
+ * grouped by brick of 4x4x4 cells, in the order of a stable sort (the
+ * reference reorders its particles every step as well,
  * fluidsimulation.cpp:3211-3223); id_in/id_out (optional) carry a particle
  * label through that permutation.  u,v,w receive the grid after the body
  * force; the snapshot (u_old..) and the weights (wu..) are written when not
@@ -207,7 +208,7 @@ void gfsx_flip_step_batched(gfsx_ctx *ctx, const gfsx_scene *scenes, int nscenes
  * counterpart in the reference: it is for callers that cannot allocate
  * device memory (code under HIP graph capture, tight serving loops,
  * harnesses that forbid hipMalloc).  Limits: one scene, p2g_variant 4, no
- * epilogue; a step whose workspace exceeds the scratch (about 56 bytes per
+ * epilogue; a step whose workspace exceeds the scratch (about 60 bytes per
  * particle plus 80 per cell) fails with a message giving the size it needs,
  * before any work is enqueued.  Every call on a device shares that device's
  * scratch and leaves its contents behind: calls that overlap on one device

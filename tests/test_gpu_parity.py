@@ -609,6 +609,54 @@ def test_step_fp32_cells_bit_identical(ctx, kind, stencil, order):
                 assert np.array_equal(ra[key], rb[key], equal_nan=True), (step, key)
 
 
+def _brick_keys(pos, dims, dx):
+    """The brick sort's key of every position (numpy restatement): P2G brick
+    of 4^3 cells, the 8 bricks of an 8^3-cell G2P brick consecutive, pad
+    cells in the edge bricks, the rest in the overflow bin."""
+    d = np.array(dims)
+    cell = np.floor(pos.astype(np.float64) * (1.0 / dx))
+    inside = ((cell >= -1) & (cell <= d)).all(1)
+    c = np.where(np.isfinite(cell), cell, 0).astype(np.int64)
+    b = np.minimum(np.maximum(c, 0), d - 1) >> 2
+    gb = ((d + 3) // 4 + 1) // 2
+    g = (b[:, 0] >> 1) + gb[0] * ((b[:, 1] >> 1) + gb[1] * (b[:, 2] >> 1))
+    key = (g << 3) | (b[:, 0] & 1) | ((b[:, 1] & 1) << 1) | ((b[:, 2] & 1) << 2)
+    return np.where(inside, key, 8 * int(np.prod(gb)))
+
+
+@pytest.mark.parametrize("gather", [2, 0])
+def test_step_sort_of_unsorted_input(ctx, orc, gather):
+    """The brick sort on input in no useful order: a cell-ordered scene
+    followed by uniformly random particles over many bricks (work-groups of
+    the sort whose particles fall in more bins than their LDS table holds
+    take every run as a chunk of its own), pad-layer and far-out particles.
+    The output order is a stable sort by brick, and the step matches the
+    oracle."""
+    from gridfluidsim3d_amd import scenes
+    dims, dx = (64, 48, 40), 1.0 / 64.0
+    sc = scenes.random_blobs(41, dims, dx, ppc=4, amplitude=1.0, fill=(0.1, 0.2))
+    rng = np.random.default_rng(43)
+    W = np.array(dims) * dx
+    extra = rng.uniform(-1.5 * dx, W + 1.5 * dx, size=(120000, 3)).astype(np.float32)
+    extra[:200] *= np.float32(3.0)
+    pos = np.concatenate([sc.pos, extra])
+    vel = np.concatenate([sc.vel, rng.normal(size=extra.shape).astype(np.float32)])
+    old = ctx.get_option("particle_gather")
+    try:
+        ctx.set_option("particle_gather", gather)
+        res = _run_step(ctx, pos, vel, dims, dx, 1 / 240, (0.0, -9.8, 0.0), 0.05, TRILINEAR, 0)
+    finally:
+        ctx.set_option("particle_gather", old)
+    ids = res["id"].astype(np.int64)
+    assert np.array_equal(np.sort(ids), np.arange(len(pos)))
+    key = _brick_keys(pos[ids], dims, dx)
+    assert (key[1:] >= key[:-1]).all()
+    same = key[1:] == key[:-1]
+    assert (ids[1:][same] > ids[:-1][same]).all()
+    exp = orc.flip_step(pos, vel, dims, dx, 1 / 240, (0.0, -9.8, 0.0), 0.05, TRILINEAR)
+    _check_step(res, exp, False)
+
+
 def test_step_dense_cells_use_fallback(ctx, orc):
     """Bricks holding more particles than fit in LDS take the global-memory
     gather; results still match the oracle."""

@@ -1300,12 +1300,37 @@ __device__ __forceinline__ float nan_max(float a, float b) {
 // most 1.25 in magnitude per axis, so at most 2 overall), a start cell
 // inside the brick keeps every stencil inside the staged boxes and no
 // sample tests them.  Otherwise every sample is checked against its box.
-template <int ST, int GBE, bool P2>
+template <int ST, int GBE, bool P2, bool SIMPLE>
 __device__ __forceinline__ void g2p_bbrick(const StepG2P &a, const MacGrid &g, int c0x, int c0y,
                                            int c0z, uint32_t q0, uint32_t q1,
                                            const uint32_t *__restrict__ perm, G2PSlow slow) {
     using namespace gbrick;
     using B = Box<ST, GBE>;
+    // SIMPLE: sorted copies in, sorted order out, no particle ids: slot q
+    // reads its particle at q and writes it at q - p0
+    auto ld = [&](uint32_t q, float x[3], float v[3]) {
+        if (SIMPLE) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                x[c] = gld(a.xin[c], q);
+                v[c] = gld(a.vin[c], q);
+            }
+        } else {
+            load_particle(a, q, x, v);
+        }
+    };
+    auto put = [&](uint32_t q, const float x1[3], const float vn[3]) {
+        if (SIMPLE) {
+            const uint32_t o = q - a.p0;
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                gst(a.xout[c], o, x1[c]);
+                gst(a.vout[c], o, vn[c]);
+            }
+        } else {
+            store_particle(a, perm, q, x1, vn);
+        }
+    };
     __shared__ float s_cur[3][B::E * B::E * B::E];
     __shared__ float s_old[3][B::EO * B::EO * B::EO];
     __shared__ float s_max[NT / 64][3];
@@ -1343,25 +1368,25 @@ __device__ __forceinline__ void g2p_bbrick(const StepG2P &a, const MacGrid &g, i
         // a stage may leave the boxes: every sample checked
         for (uint32_t q = q0 + (uint32_t)t; q < q1; q += NT) {
             float x0[3], v0[3];
-            load_particle(a, q, x0, v0);
+            ld(q, x0, v0);
             float x1[3], vnew[3];
             g2p_particle_checked<ST, GBE, P2>(a, g, cur, old, lx, ly, lz, lox, loy, loz, x0, v0,
                                               x1, vnew);
-            store_particle(a, perm, q, x1, vnew);
+            put(q, x1, vnew);
         }
         return;
     }
     // the next particle is loaded while this one is computed
     float xn[3] = {0.0f, 0.0f, 0.0f}, vn[3] = {0.0f, 0.0f, 0.0f};
-    if (q0 + (uint32_t)t < q1) load_particle(a, q0 + t, xn, vn);
+    if (q0 + (uint32_t)t < q1) ld(q0 + t, xn, vn);
     for (uint32_t q = q0 + (uint32_t)t; q < q1; q += NT) {
         const float x0[3] = {xn[0], xn[1], xn[2]};
         const float v0[3] = {vn[0], vn[1], vn[2]};
-        if (q + NT < q1) load_particle(a, q + NT, xn, vn);
+        if (q + NT < q1) ld(q + NT, xn, vn);
         float x1[3], vnew[3];
         const bool done = g2p_particle_in<ST, GBE, P2>(a, g, cur, old, lx, ly, lz, lox, loy, loz,
                                                        c0x, c0y, c0z, x0, v0, x1, vnew);
-        if (done) store_particle(a, perm, q, x1, vnew);
+        if (done) put(q, x1, vnew);
         // the rest to the slow list, one atomic per wave (by the lowest lane
         // that has one: lanes past the end have left the loop)
         const uint64_t mk = __ballot(!done);
@@ -1376,7 +1401,7 @@ __device__ __forceinline__ void g2p_bbrick(const StepG2P &a, const MacGrid &g, i
 }
 
 // One work-group per G2P brick of 8^3 cells of any scene.
-template <int ST, bool P2>
+template <int ST, bool P2, bool SIMPLE>
 __device__ __forceinline__ void g2p_bb_kernel(const SceneDesc *__restrict__ scenes, int nsc,
                                               Sorted so, int keep_order, uint32_t nbrick,
                                               int per_xcd, const uint32_t *__restrict__ pstart,
@@ -1390,11 +1415,12 @@ __device__ __forceinline__ void g2p_bb_kernel(const SceneDesc *__restrict__ scen
     const int lg = (int)(gid - sd.g0);
     const int gx = lg % sd.gbx, gy = (lg / sd.gbx) % sd.gby, gz = lg / (sd.gbx * sd.gby);
     const uint32_t b0 = sd.kb0 + 8u * (uint32_t)lg;
-    g2p_bbrick<ST, 8, P2>(a, g, gx * 8, gy * 8, gz * 8, pstart[b0], pstart[b0 + 8], perm, slow);
+    g2p_bbrick<ST, 8, P2, SIMPLE>(a, g, gx * 8, gy * 8, gz * 8, pstart[b0], pstart[b0 + 8], perm,
+                                  slow);
 }
 
 // One work-group per P2G brick (4^3 cells, one bin of the sort) of any scene.
-template <int ST, bool P2>
+template <int ST, bool P2, bool SIMPLE>
 __device__ __forceinline__ void g2p_b4_kernel(const SceneDesc *__restrict__ scenes, int nsc,
                                               Sorted so, int keep_order, uint32_t nbin,
                                               int per_xcd, const uint32_t *__restrict__ pstart,
@@ -1408,42 +1434,43 @@ __device__ __forceinline__ void g2p_b4_kernel(const SceneDesc *__restrict__ scen
     brick_coords(sd, lb, bx, by, bz);
     const StepG2P a = g2p_view(sd, so, keep_order);
     const MacGrid g = mac_view(sd);
-    g2p_bbrick<ST, 4, P2>(a, g, bx * 4, by * 4, bz * 4, pstart[gid], pstart[gid + 1], perm, slow);
+    g2p_bbrick<ST, 4, P2, SIMPLE>(a, g, bx * 4, by * 4, bz * 4, pstart[gid], pstart[gid + 1], perm,
+                                  slow);
 }
 
 // Trilinear: four work-groups per CU fit in LDS; four waves per SIMD.
-template <bool P2>
+template <bool P2, bool SIMPLE>
 __global__ __launch_bounds__(gbrick::NT, 4) void k_flip_g2p_bb_tl(
     const SceneDesc *__restrict__ scenes, int nsc, Sorted so, int keep_order, uint32_t nbrick,
     int per_xcd, const uint32_t *__restrict__ pstart, const uint32_t *__restrict__ perm,
     G2PSlow slow) {
-    g2p_bb_kernel<0, P2>(scenes, nsc, so, keep_order, nbrick, per_xcd, pstart, perm, slow);
+    g2p_bb_kernel<0, P2, SIMPLE>(scenes, nsc, so, keep_order, nbrick, per_xcd, pstart, perm, slow);
 }
 
 // Tricubic: three waves per SIMD (168 VGPRs) run about 10% faster than two
 // (256) at the cost of a few bytes of spill.
-template <bool P2>
+template <bool P2, bool SIMPLE>
 __global__ __launch_bounds__(gbrick::NT, 3) void k_flip_g2p_bb_tc(
     const SceneDesc *__restrict__ scenes, int nsc, Sorted so, int keep_order, uint32_t nbrick,
     int per_xcd, const uint32_t *__restrict__ pstart, const uint32_t *__restrict__ perm,
     G2PSlow slow) {
-    g2p_bb_kernel<1, P2>(scenes, nsc, so, keep_order, nbrick, per_xcd, pstart, perm, slow);
+    g2p_bb_kernel<1, P2, SIMPLE>(scenes, nsc, so, keep_order, nbrick, per_xcd, pstart, perm, slow);
 }
 
-template <bool P2>
+template <bool P2, bool SIMPLE>
 __global__ __launch_bounds__(gbrick::NT, 4) void k_flip_g2p_b4_tl(
     const SceneDesc *__restrict__ scenes, int nsc, Sorted so, int keep_order, uint32_t nbin,
     int per_xcd, const uint32_t *__restrict__ pstart, const uint32_t *__restrict__ perm,
     G2PSlow slow) {
-    g2p_b4_kernel<0, P2>(scenes, nsc, so, keep_order, nbin, per_xcd, pstart, perm, slow);
+    g2p_b4_kernel<0, P2, SIMPLE>(scenes, nsc, so, keep_order, nbin, per_xcd, pstart, perm, slow);
 }
 
-template <bool P2>
+template <bool P2, bool SIMPLE>
 __global__ __launch_bounds__(gbrick::NT, 3) void k_flip_g2p_b4_tc(
     const SceneDesc *__restrict__ scenes, int nsc, Sorted so, int keep_order, uint32_t nbin,
     int per_xcd, const uint32_t *__restrict__ pstart, const uint32_t *__restrict__ perm,
     G2PSlow slow) {
-    g2p_b4_kernel<1, P2>(scenes, nsc, so, keep_order, nbin, per_xcd, pstart, perm, slow);
+    g2p_b4_kernel<1, P2, SIMPLE>(scenes, nsc, so, keep_order, nbin, per_xcd, pstart, perm, slow);
 }
 
 // The particles the brick kernel left, sampled from global memory
@@ -1587,6 +1614,9 @@ void flip_step_brick(const StepBatch &s, const BrickSort &bs, hipStream_t st, hi
         if (s.g2p_variant != 0) throw Error("gfsx: the particle gather needs the brick G2P");
         so.perm = bs.perm;
     }
+    // the brick G2P without the permutation read, particle ids or kept order
+    bool simple = !so.perm && !s.keep_order;
+    for (const auto &d : s.scenes) simple = simple && !d.id_out;
     if (s.g2p_variant == 0) {
         const uint32_t nbrick = (uint32_t)s.tot.ng2p;
         const int gper = (int)((nbrick + 7) / 8);
@@ -1596,14 +1626,20 @@ void flip_step_brick(const StepBatch &s, const BrickSort &bs, hipStream_t st, hi
         fill_u32(bs.slow_count, 0u, 1, st);
         if (s.g2p_brick == 4) {
             const int bper = (int)((nbin + 7) / 8);
-            auto *kern = s.stencil == 0 ? (s.p2 ? k_flip_g2p_b4_tl<true> : k_flip_g2p_b4_tl<false>)
-                                        : (s.p2 ? k_flip_g2p_b4_tc<true> : k_flip_g2p_b4_tc<false>);
+            auto *kern = s.stencil == 0
+                             ? (s.p2 ? (simple ? k_flip_g2p_b4_tl<true, true> : k_flip_g2p_b4_tl<true, false>)
+                                     : (simple ? k_flip_g2p_b4_tl<false, true> : k_flip_g2p_b4_tl<false, false>))
+                             : (s.p2 ? (simple ? k_flip_g2p_b4_tc<true, true> : k_flip_g2p_b4_tc<true, false>)
+                                     : (simple ? k_flip_g2p_b4_tc<false, true> : k_flip_g2p_b4_tc<false, false>));
             hipLaunchKernelGGL(kern, dim3((unsigned)(8 * bper)), dim3(gbrick::NT), 0, st,
                                s.scenes_dev, nsc, so, s.keep_order, nbin, bper,
                                (const uint32_t *)bs.pstart, (const uint32_t *)bs.perm, slow);
         } else {
-            auto *kern = s.stencil == 0 ? (s.p2 ? k_flip_g2p_bb_tl<true> : k_flip_g2p_bb_tl<false>)
-                                        : (s.p2 ? k_flip_g2p_bb_tc<true> : k_flip_g2p_bb_tc<false>);
+            auto *kern = s.stencil == 0
+                             ? (s.p2 ? (simple ? k_flip_g2p_bb_tl<true, true> : k_flip_g2p_bb_tl<true, false>)
+                                     : (simple ? k_flip_g2p_bb_tl<false, true> : k_flip_g2p_bb_tl<false, false>))
+                             : (s.p2 ? (simple ? k_flip_g2p_bb_tc<true, true> : k_flip_g2p_bb_tc<true, false>)
+                                     : (simple ? k_flip_g2p_bb_tc<false, true> : k_flip_g2p_bb_tc<false, false>));
             hipLaunchKernelGGL(kern, dim3((unsigned)(8 * gper)), dim3(gbrick::NT), 0, st,
                                s.scenes_dev, nsc, so, s.keep_order, nbrick, gper,
                                (const uint32_t *)bs.pstart, (const uint32_t *)bs.perm, slow);

@@ -211,6 +211,14 @@ __device__ __forceinline__ T gld(const T *p, size_t i) {
     return ((GT *)p)[i];
 }
 
+// Element i of an array in device memory written as a global-memory store
+// (a flat store counts against the LDS wait counter as well).
+template <class T>
+__device__ __forceinline__ void gst(T *p, size_t i, T v) {
+    typedef __attribute__((address_space(1))) T GT;
+    ((GT *)p)[i] = v;
+}
+
 // Wyvill kernel for t = r^2/R^2 (scalarfield.cpp:569-571 rewritten as the
 // factorisation K = (1-t)^2 (1 - 4t/9) = a^2 (5/9 + 4a/9), a = 1-t):
 // exact zero for t >= 1 without a compare, and no cancellation near t = 1.

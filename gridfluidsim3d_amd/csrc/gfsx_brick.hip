@@ -1178,8 +1178,8 @@ __global__ __launch_bounds__(kBlock) void k_p2g_bcombine(
                 }
             const int64_t f = (int64_t)I + (int64_t)nxc * ((int64_t)J + (int64_t)nyc * K);
             float val = W > 0.0f ? S / W : S;
-            if (fo.old) fo.old[f] = val;
-            if (fo.wgt) fo.wgt[f] = W;
+            if (fo.old) gst(fo.old, (size_t)f, val);
+            if (fo.wgt) gst(fo.wgt, (size_t)f, W);
             if (fo.grav_on) {
                 // the cells on either side along axis c
                 const int n = c == 0 ? I : (c == 1 ? J : K);
@@ -1189,7 +1189,7 @@ __global__ __launch_bounds__(kBlock) void k_p2g_bcombine(
                     fl = fl || cell_fluid(sd, fluid, I - (c == 0), J - (c == 1), K - (c == 2));
                 if (fl) val += fo.grav_delta;
             }
-            fo.grid[f] = val;
+            gst(fo.grid, (size_t)f, val);
         }
         if (++J == nyc) {
             J = 0;

@@ -275,10 +275,10 @@ __device__ __forceinline__ void store_particle(const StepG2P &a,
     if (a.keep_order) o = p;
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-        a.xout[c][o] = x1[c];
-        a.vout[c][o] = vnew[c];
+        gst(a.xout[c], o, x1[c]);
+        gst(a.vout[c], o, vnew[c]);
     }
-    if (a.id_out) a.id_out[o] = a.id_in ? a.id_in[p] : p;
+    if (a.id_out) gst(a.id_out, o, a.id_in ? gld(a.id_in, p) : p);
 }
 
 namespace gbrick {

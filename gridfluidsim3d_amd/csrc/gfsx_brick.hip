@@ -231,6 +231,20 @@ __global__ __launch_bounds__(kBlock) void k_bsort_bin(const SceneDesc *__restric
     const bool all_runs = s_fail != 0;
     const size_t c0 = (size_t)blk * TILE;
     if (!all_runs) {
+        // each entry's run lengths become the runs' offsets in the chunk:
+        // the lengths of the bin's runs in earlier waves
+        if (s_key[t] != EMPTY) {
+            const uint32_t mk = s_mask[t];
+            uint32_t at = 0;
+#pragma unroll
+            for (int w = 0; w < WAVES; w++) {
+                if ((mk >> w) & 1u) {
+                    const uint32_t l = s_len[t][w];
+                    s_len[t][w] = (uint16_t)at;
+                    at += l;
+                }
+            }
+        }
         // the used entries in table order are the chunks
         const bool used = s_key[t] != EMPTY;
         uint32_t tot;
@@ -262,13 +276,8 @@ __global__ __launch_bounds__(kBlock) void k_bsort_bin(const SceneDesc *__restric
         uint32_t rcc = 0, roff = 0;
         if (lrun[m] >= 0) {
             if (!all_runs) {
-                const int e = lslot[m];
-                rcc = s_cc[e];
-                uint32_t mk = s_mask[e] & ((1u << wv) - 1u);
-                while (mk) {
-                    roff += s_len[e][__ffs(mk) - 1];
-                    mk &= mk - 1u;
-                }
+                rcc = s_cc[lslot[m]];
+                roff = s_len[lslot[m]][wv];
             } else {
                 rcc = (uint32_t)(wv * 64 + lrun[m]);
             }

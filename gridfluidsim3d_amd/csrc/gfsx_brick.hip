@@ -395,9 +395,17 @@ static void exclusive_scan64(const unsigned long long *in, unsigned long long *o
 }
 
 // Particle and run starts of every bin from the scanned counts.
+// Also clears the order kernels' two list counters and G2P's slow-list
+// count (zeros: three words, no separate fill launches).
 __global__ void k_bsort_split(const unsigned long long *__restrict__ start, uint32_t m,
-                              uint32_t *__restrict__ pstart, uint32_t *__restrict__ rstart) {
+                              uint32_t *__restrict__ pstart, uint32_t *__restrict__ rstart,
+                              uint32_t *__restrict__ lcount, uint32_t *__restrict__ slow_count) {
     const uint32_t b = blockIdx.x * kBlock + threadIdx.x;
+    if (b == 0) {
+        lcount[0] = 0u;
+        lcount[1] = 0u;
+        *slow_count = 0u;
+    }
     if (b >= m) return;
     const unsigned long long v = start[b];
     pstart[b] = (uint32_t)v;
@@ -1552,7 +1560,8 @@ void flip_step_brick(const StepBatch &s, const BrickSort &bs, hipStream_t st, hi
     }
     exclusive_scan64(bs.count, bs.start, nbin, bs.scan_scratch, st);
     hipLaunchKernelGGL(k_bsort_split, dim3(grid_for((int64_t)nbin + 1, kBlock)), dim3(kBlock), 0, st,
-                       (const unsigned long long *)bs.start, nbin + 1, bs.pstart, bs.rstart);
+                       (const unsigned long long *)bs.start, nbin + 1, bs.pstart, bs.rstart, bs.lcount,
+                       bs.slow_count);
     GFSX_LAUNCH_CHECK();
     SortedOut so_out;
     for (int c = 0; c < 3; c++) {
@@ -1566,7 +1575,6 @@ void flip_step_brick(const StepBatch &s, const BrickSort &bs, hipStream_t st, hi
                            (const uint2 *)bs.runrec, (const uint32_t *)bs.rstart, bs.list,
                            (uint32_t)n);
         GFSX_LAUNCH_CHECK();
-        fill_u32(bs.lcount, 0u, 2, st);
         hipLaunchKernelGGL(k_bsort_order_small, dim3(grid_for(nbin, kBlock)), dim3(kBlock), 0, st,
                            nbin, (const uint32_t *)bs.rstart, (const uint32_t *)bs.pstart,
                            (const uint32_t *)bs.list, (const uint32_t *)bs.rlen, bs.runbase, bs.mid,
@@ -1632,7 +1640,6 @@ void flip_step_brick(const StepBatch &s, const BrickSort &bs, hipStream_t st, hi
         G2PSlow slow;
         slow.slots = bs.slow_slots;
         slow.count = bs.slow_count;
-        fill_u32(bs.slow_count, 0u, 1, st);
         if (s.g2p_brick == 4) {
             const int bper = (int)((nbin + 7) / 8);
             auto *kern = s.stencil == 0

@@ -116,7 +116,13 @@ struct gfsx_ctx {
         gfsx::SceneDesc *dev = nullptr;
         size_t cap = 0;
         uint64_t used = 0;
+        hipStream_t stream = nullptr;   // the stream of the last work that read it
     };
+    // Tables of at most this many scenes are written by kernels (the
+    // entries travel as kernel arguments), ordered after every earlier
+    // reader on the stream: replacing one needs no host wait, so a loop
+    // whose buffers change every step (an autograd rollout) keeps queueing.
+    static constexpr size_t kPutScenesMax = 16;
     std::array<Table, 4> tables;
     uint64_t table_clock = 0;
 
@@ -126,6 +132,7 @@ struct gfsx_ctx {
             if (e.dev && e.host.size() == t.size() &&
                 std::memcmp(e.host.data(), t.data(), bytes) == 0) {
                 e.used = ++table_clock;
+                e.stream = stream;
                 return e.dev;
             }
         }
@@ -140,22 +147,32 @@ struct gfsx_ctx {
         if (slot->cap < t.size()) {
             // the old table may still be read by queued work
             GFSX_HIP(hipStreamSynchronize(stream));
+            if (slot->used && slot->stream != stream) GFSX_HIP(hipStreamSynchronize(slot->stream));
             if (slot->dev) GFSX_HIP(hipFree(slot->dev));
             slot->dev = nullptr;
             slot->cap = 0;
             GFSX_HIP(hipMalloc(&slot->dev, std::max<size_t>(bytes + sizeof(int64_t) * t.size(), 1)));
             slot->cap = t.size();
         } else if (slot->used) {
-            // queued work may still read the table being replaced
-            GFSX_HIP(hipStreamSynchronize(stream));
+            // queued work may still read the table being replaced: work on
+            // this stream reads it before kernels written after it run;
+            // anything else is waited for
+            if (t.size() > kPutScenesMax || slot->stream != stream)
+                GFSX_HIP(hipStreamSynchronize(slot->stream));
         }
         slot->host = t;
         slot->r0.resize(t.size());
         for (size_t i = 0; i < t.size(); i++) slot->r0[i] = t[i].r0;
-        GFSX_HIP(hipMemcpyAsync(slot->dev, slot->host.data(), bytes, hipMemcpyHostToDevice, stream));
-        GFSX_HIP(hipMemcpyAsync(slot->dev + t.size(), slot->r0.data(), sizeof(int64_t) * t.size(),
-                                hipMemcpyHostToDevice, stream));
+        if (t.size() <= kPutScenesMax) {
+            gfsx::put_scenes(slot->dev, slot->host.data(), (int)t.size(), stream,
+                             reinterpret_cast<int64_t *>(slot->dev + t.size()));
+        } else {
+            GFSX_HIP(hipMemcpyAsync(slot->dev, slot->host.data(), bytes, hipMemcpyHostToDevice, stream));
+            GFSX_HIP(hipMemcpyAsync(slot->dev + t.size(), slot->r0.data(), sizeof(int64_t) * t.size(),
+                                    hipMemcpyHostToDevice, stream));
+        }
         slot->used = ++table_clock;
+        slot->stream = stream;
         return slot->dev;
     }
 

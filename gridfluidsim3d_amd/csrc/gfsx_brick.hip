@@ -1532,15 +1532,20 @@ __global__ __launch_bounds__(kBlock) void k_flip_g2p_slow(
 // The step
 // ---------------------------------------------------------------------------
 
-// A scene table entry written by a kernel from its argument (no copy from
-// host memory: nothing to stage, nothing to allocate).
-__global__ void k_put_scene(SceneDesc *dst, SceneDesc d) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *dst = d;
+// A scene table entry (and, when r0 is set, its combine wave offset)
+// written by a kernel from its argument (no copy from host memory: nothing
+// to stage, nothing to allocate).
+__global__ void k_put_scene(SceneDesc *dst, SceneDesc d, int64_t *r0) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        *dst = d;
+        if (r0) *r0 = d.r0;
+    }
 }
 
-void put_scenes(SceneDesc *dst, const SceneDesc *src, int n, hipStream_t st) {
+void put_scenes(SceneDesc *dst, const SceneDesc *src, int n, hipStream_t st, int64_t *r0) {
     for (int i = 0; i < n; i++) {
-        hipLaunchKernelGGL(k_put_scene, dim3(1), dim3(64), 0, st, dst + i, src[i]);
+        hipLaunchKernelGGL(k_put_scene, dim3(1), dim3(64), 0, st, dst + i, src[i],
+                           r0 ? r0 + i : nullptr);
         GFSX_LAUNCH_CHECK();
     }
 }

@@ -482,9 +482,11 @@ struct StepBatch {
 // tiles, per-brick combine, G2P over brick ranges (gfsx_brick.hip).
 void flip_step_brick(const StepBatch &s, const BrickSort &bs, hipStream_t st, hipEvent_t *ev);
 
-// Write n scene table entries to device memory at dst with one small kernel
-// each (the entries travel as kernel arguments).
-void put_scenes(SceneDesc *dst, const SceneDesc *src, int n, hipStream_t st);
+// Write n scene table entries to device memory at dst (and their combine
+// wave offsets to r0[0..n) when r0 is set) with one small kernel each (the
+// entries travel as kernel arguments).
+void put_scenes(SceneDesc *dst, const SceneDesc *src, int n, hipStream_t st,
+                int64_t *r0 = nullptr);
 
 // The direct G2P kernel (g2p_variant 1, one scene) over the particles in
 // s.xs/s.vs; perm maps sort slots to particle indices (needed for

@@ -388,8 +388,10 @@ def fwdbwd_measure(reps, log_fn, steps=16, opt_iters=3):
     (1,990,656 particles, dx = 1/96) through the autograd step, the density
     loss against a seeded smooth target, and the backward pass through the
     adjoint kernels to the initial velocities.  Reports particle-steps/s of
-    forward+backward (P * steps / time) next to the same rollout forward
-    only, and a few gradient-descent iterations on the initial velocities."""
+    forward+backward (P * steps / time; each step keeps its sort and grids
+    for the backward pass) next to the same rollout forward only and to the
+    backward that recomputes them, and a few gradient-descent iterations on
+    the initial velocities."""
     import torch
 
     import gridfluidsim3d_amd as gf
@@ -411,9 +413,9 @@ def fwdbwd_measure(reps, log_fn, steps=16, opt_iters=3):
             x, _v = ga.rollout(ctx, x0, v_init, dims, dx, steps, dt)
             return ga.density_loss(ctx, x, target, dims, dx)
 
-    def fwd_bwd(v_init):
+    def fwd_bwd(v_init, tape=True):
         v = v_init.detach().clone().requires_grad_(True)
-        x, _v = ga.rollout(ctx, x0, v, dims, dx, steps, dt)
+        x, _v = ga.rollout(ctx, x0, v, dims, dx, steps, dt, tape=tape)
         L = ga.density_loss(ctx, x, target, dims, dx)
         L.backward()
         return L.detach(), v.grad
@@ -431,8 +433,10 @@ def fwdbwd_measure(reps, log_fn, steps=16, opt_iters=3):
 
     fwd()
     fwd_bwd(v0)
+    fwd_bwd(v0, tape=False)
     ms_f = timed(fwd)
     ms_fb = timed(lambda: fwd_bwd(v0))
+    ms_fb_re = timed(lambda: fwd_bwd(v0, tape=False))
     # gradient descent on the initial velocities, step length fixed in
     # velocity units (0.05 m/s on the largest gradient component)
     losses = []
@@ -447,6 +451,7 @@ def fwdbwd_measure(reps, log_fn, steps=16, opt_iters=3):
     gbs = alg / (ms_fb / 1e3) / 1e9
     out = dict(workload="slab96", grid=list(dims), dx=dx, particles=P, steps=steps, reps=reps,
                ms_forward=round(ms_f, 3), ms_forward_backward=round(ms_fb, 3),
+               ms_forward_backward_recompute=round(ms_fb_re, 3),
                value_forward=P * steps / (ms_f / 1e3),
                value_forward_backward=P * steps / (ms_fb / 1e3), unit="particle-steps/s",
                roofline_forward_backward=dict(achieved=round(gbs, 1), peak=HBM_PEAK_GBS,

@@ -4,11 +4,11 @@ advection and the fused plain FLIP step, behind the C ABI in include/gfsx.h.
 
 The HIP library (lib/libgfsx.so) is required; there is no CPU fallback.
 """
-from .gfsx import (EPILOGUE_FLUIDSIM, EPILOGUE_NONE, Context, FlipState, ORDER_KEEP,
-                   ORDER_SORTED, TRICUBIC, TRILINEAR, mac_shapes)
+from .gfsx import (EPILOGUE_FLUIDSIM, EPILOGUE_NONE, Context, FlipState, FlipTape,
+                   ORDER_KEEP, ORDER_SORTED, TRICUBIC, TRILINEAR, mac_shapes)
 from .pybindings import LibraryLoadError, library_path, load_library
 
-__all__ = ["Context", "FlipState", "EPILOGUE_FLUIDSIM", "EPILOGUE_NONE", "ORDER_KEEP",
+__all__ = ["Context", "FlipState", "FlipTape", "EPILOGUE_FLUIDSIM", "EPILOGUE_NONE", "ORDER_KEEP",
            "ORDER_SORTED", "TRICUBIC",
            "TRILINEAR", "mac_shapes", "LibraryLoadError", "library_path",
            "load_library"]

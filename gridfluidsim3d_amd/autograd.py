@@ -5,9 +5,11 @@ in input order) as a ``torch.autograd.Function``: forward is
 ``gfsx_flip_step``, backward is ``gfsx_flip_step_backward``, both called
 through the library's ctypes layer with device pointers (the pattern of
 pyfluid's ``init_lib_func`` / ``execute_lib_func``, pybindings.py:10-36).
-The tensors autograd saves for each step are that step's inputs (x_t, v_t):
-they are the per-step checkpoints of a rollout, and the backward pass
-recomputes the step's grids and RK4 stages from them.
+The tensors autograd saves for each step are that step's inputs (x_t, v_t)
+and, by default, its tape (``gfsx_flip_step_record``: the brick sort of x_t
+and the step's grids), so that the backward pass recomputes only the RK4
+stages; with ``tape=False`` it recomputes the sort and the grids as well
+(24 bytes per particle per step saved instead of about 40 plus 36 per face).
 
 ``SplatWeightFunction`` is the weight field of the point splat
 (``gfsx_p2g_device`` with no values, CLScalarField::addPoints) with its
@@ -30,10 +32,15 @@ class FlipStepFunction(torch.autograd.Function):
     """(x, v) -> (x', v') of one plain step; see the module docstring."""
 
     @staticmethod
-    def forward(fctx, x, v, gctx, dims, dx, dt, body_force, pic_ratio):
+    def forward(fctx, x, v, gctx, dims, dx, dt, body_force, pic_ratio, tape=True):
         x = x.contiguous()
         v = v.contiguous()
-        x_out, v_out, _ = gctx.flip_step_soa(x, v, dims, dx, dt, body_force, pic_ratio)
+        fctx.tape = None
+        if tape and any(fctx.needs_input_grad[:2]):
+            x_out, v_out, fctx.tape = gctx.flip_step_record(x, v, dims, dx, dt, body_force,
+                                                            pic_ratio)
+        else:
+            x_out, v_out, _ = gctx.flip_step_soa(x, v, dims, dx, dt, body_force, pic_ratio)
         fctx.save_for_backward(x, v)
         fctx.gctx = gctx
         fctx.args = (dims, dx, dt, body_force, pic_ratio)
@@ -44,8 +51,10 @@ class FlipStepFunction(torch.autograd.Function):
         x, v = fctx.saved_tensors
         gx_out = _zeros_if_none(gx_out, x)
         gv_out = _zeros_if_none(gv_out, v)
-        gx, gv = fctx.gctx.flip_step_backward(x, v, gx_out, gv_out, *fctx.args)
-        return gx, gv, None, None, None, None, None, None
+        gx, gv = fctx.gctx.flip_step_backward(x, v, gx_out, gv_out, *fctx.args,
+                                              tape=fctx.tape)
+        fctx.tape = None
+        return gx, gv, None, None, None, None, None, None, None
 
 
 class SplatWeightFunction(torch.autograd.Function):
@@ -73,19 +82,24 @@ class SplatWeightFunction(torch.autograd.Function):
         return gpos.t().contiguous(), None, None, None, None, None
 
 
-def flip_step(gctx, x, v, dims, dx, dt, body_force=(0.0, -9.8, 0.0), pic_ratio=0.05):
-    """One differentiable plain step."""
+def flip_step(gctx, x, v, dims, dx, dt, body_force=(0.0, -9.8, 0.0), pic_ratio=0.05,
+              tape=True):
+    """One differentiable plain step (``tape``: keep the step's sort and
+    grids for the backward pass instead of recomputing them)."""
     return FlipStepFunction.apply(x, v, gctx, tuple(dims), float(dx), float(dt),
-                                  tuple(float(b) for b in body_force), float(pic_ratio))
+                                  tuple(float(b) for b in body_force), float(pic_ratio),
+                                  bool(tape))
 
 
-def rollout(gctx, x0, v0, dims, dx, steps, dt, body_force=(0.0, -9.8, 0.0), pic_ratio=0.05):
+def rollout(gctx, x0, v0, dims, dx, steps, dt, body_force=(0.0, -9.8, 0.0), pic_ratio=0.05,
+            tape=True):
     """``steps`` differentiable plain steps from (x0, v0); returns (x, v).
     Autograd keeps one (x_t, v_t) checkpoint per step (24 bytes per
-    particle per step)."""
+    particle), plus with ``tape`` the step's sort and grids (16 bytes per
+    particle, 36 per face)."""
     x, v = x0, v0
     for _ in range(steps):
-        x, v = flip_step(gctx, x, v, dims, dx, dt, body_force, pic_ratio)
+        x, v = flip_step(gctx, x, v, dims, dx, dt, body_force, pic_ratio, tape)
     return x, v
 
 

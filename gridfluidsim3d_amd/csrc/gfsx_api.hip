@@ -625,6 +625,137 @@ void copy_out(void *dst, const void *src, size_t bytes, hipStream_t s) {
     if (bytes) GFSX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
 }
 
+// The step the adjoint differentiates, by the scatter P2G (whose sort the
+// adjoint reads) whatever P2G variant the context has selected.
+std::vector<gfsx::SceneDesc> scatter_step(gfsx_ctx *ctx, const gfsx_scene *sc,
+                                          const gfsx_step_params *pr, const char *who,
+                                          bool grids_only) {
+    const int p2g_variant = ctx->p2g_variant;
+    ctx->p2g_variant = 4;
+    try {
+        std::vector<gfsx::SceneDesc> laid = flip_step_impl(ctx, sc, pr, 1, who, grids_only);
+        ctx->p2g_variant = p2g_variant;
+        return laid;
+    } catch (...) {
+        ctx->p2g_variant = p2g_variant;
+        throw;
+    }
+}
+
+// entries of the brick sort's bin starts for one scene: 8 bins per G2P brick
+// of 8^3 cells, the overflow bin, the end
+int64_t tape_starts(int ni, int nj, int nk) {
+    return 8 * (int64_t)((ni + 7) / 8) * ((nj + 7) / 8) * ((nk + 7) / 8) + 2;
+}
+
+void check_step_kind(const char *who, const gfsx_step_params *pr) {
+    auto req = [&](bool c, const char *what) {
+        if (!c) throw Error(std::string(who) + ": " + what);
+    };
+    req(pr->stencil == GFSX_TRILINEAR, "the adjoint covers the trilinear stencil only");
+    req(pr->order == GFSX_ORDER_KEEP,
+        "the adjoints refer to particles in input order (GFSX_ORDER_KEEP)");
+    req(pr->epilogue == GFSX_EPILOGUE_NONE,
+        "the adjoint covers the transfer without the FluidSimulation epilogue");
+}
+
+// the particle arrays of a backward call
+struct AdjointIo {
+    const float *ins[6], *aout[6];
+    float *ain[6];
+};
+
+AdjointIo check_adjoint(const char *who, const gfsx_scene *sc, const gfsx_step_params *pr,
+                        const gfsx_scene_grad *gr) {
+    auto req = [&](bool c, const char *what) {
+        if (!c) throw Error(std::string(who) + ": " + what);
+    };
+    req(sc && pr && gr, "null scene, params or gradients");
+    check_step_kind(who, pr);
+    check_grid(sc->ni, sc->nj, sc->nk, sc->dx);
+    check_particles(sc->n, sc->x_in);
+    AdjointIo io = {{sc->x_in, sc->y_in, sc->z_in, sc->vx_in, sc->vy_in, sc->vz_in},
+                    {gr->gx_out, gr->gy_out, gr->gz_out, gr->gvx_out, gr->gvy_out, gr->gvz_out},
+                    {gr->gx_in, gr->gy_in, gr->gz_in, gr->gvx_in, gr->gvy_in, gr->gvz_in}};
+    if (sc->n > 0) {
+        for (int a = 0; a < 6; a++) {
+            req(io.ins[a] != nullptr, "null particle input");
+            req(io.aout[a] != nullptr, "null output adjoint");
+            req(io.ain[a] != nullptr, "null input adjoint");
+            for (int b = 0; b < 6; b++)
+                req((const float *)io.ain[a] != io.ins[b],
+                    "input adjoints must not alias the inputs");
+        }
+    }
+    return io;
+}
+
+// a recorded step: every grid of the scene, and a tape sized for it
+void check_taped(const char *who, const gfsx_scene *sc, const gfsx_step_params *pr,
+                 const gfsx_step_tape *tape) {
+    auto req = [&](bool c, const char *what) {
+        if (!c) throw Error(std::string(who) + ": " + what);
+    };
+    req(sc && pr && tape, "null scene, params or tape");
+    check_step_kind(who, pr);
+    check_grid(sc->ni, sc->nj, sc->nk, sc->dx);
+    check_particles(sc->n, sc->x_in);
+    const float *grids[9] = {sc->u, sc->v, sc->w, sc->u_old, sc->v_old, sc->w_old,
+                             sc->wu, sc->wv, sc->ww};
+    for (const float *g : grids)
+        req(g != nullptr, "every grid of the scene (u, v, w, *_old, wu, wv, ww) is required");
+    req(tape->n == sc->n, "the tape holds a different particle count");
+    req(tape->starts == tape_starts(sc->ni, sc->nj, sc->nk),
+        "tape.starts must be gfsx_step_tape_starts(ni, nj, nk)");
+    req(tape->start != nullptr, "null tape.start");
+    req(sc->n == 0 || (tape->perm && tape->xs && tape->ys && tape->zs),
+        "null tape array (perm, xs, ys, zs)");
+}
+
+// The adjoint passes of one step whose grids (u, v, w: after the body force;
+// *_old: the normalised P2G result; wu, wv, ww: the P2G weights) are those
+// of `grids` and whose brick sort is `so`.
+void run_adjoint(gfsx_ctx *ctx, const gfsx_scene *sc, const gfsx_step_params *pr,
+                 const AdjointIo &io, const gfsx_scene &grids, gfsx::AdjointSort so) {
+    const int ni = sc->ni, nj = sc->nj, nk = sc->nk;
+    int64_t nf[3], tot = 0;
+    for (int c = 0; c < 3; c++) tot += (nf[c] = mac_count(c, ni, nj, nk));
+    const int64_t at[3] = {0, nf[0], nf[0] + nf[1]};
+    // dL/dg then dL/dn, one block (the LDS-brick adjoint converts its
+    // fixed-point sums into it in one pass)
+    float *lg = ctx->f32(S_B_LG, 2 * tot), *ln = lg + tot;
+    so.fx = (unsigned long long *)ctx->get(S_B_FX, sizeof(unsigned long long) * 2 * tot);
+    so.maxbits = ctx->u32(S_B_MAX, 1);
+    const float *g[3] = {grids.u, grids.v, grids.w};
+    const float *old[3] = {grids.u_old, grids.v_old, grids.w_old};
+    const float *wgt[3] = {grids.wu, grids.wv, grids.ww};
+    gfsx::StepAdjoint a;
+    std::memset(&a, 0, sizeof a);
+    a.n = sc->n;
+    for (int c = 0; c < 3; c++) {
+        a.x[c] = io.ins[c];
+        a.v[c] = io.ins[3 + c];
+        a.ax[c] = io.aout[c];
+        a.av[c] = io.aout[3 + c];
+        a.gx[c] = io.ain[c];
+        a.gv[c] = io.ain[3 + c];
+        a.g[c] = g[c];
+        a.old[c] = old[c];
+        a.w[c] = wgt[c];
+        a.lg[c] = lg + at[c];
+        a.ln[c] = ln + at[c];
+    }
+    a.mac = mac_grid(nullptr, nullptr, nullptr, ni, nj, nk, sc->dx);
+    a.r_pic = (float)pr->pic_ratio;
+    a.r_flip = (float)(1 - pr->pic_ratio);
+    a.h_half = (float)(0.5 * pr->dt);
+    a.h_full = (float)pr->dt;
+    a.h_sixth = (float)(pr->dt / 6.0f);
+    a.half_dx = (float)(0.5 * sc->dx);
+    a.inv_r2 = (float)(1.0 / (sc->dx * sc->dx));
+    gfsx::flip_step_adjoint(a, ctx->adjoint_variant == 0 ? &so : nullptr, ctx->stream);
+}
+
 }  // namespace
 
 extern "C" {
@@ -915,40 +1046,13 @@ void gfsx_flip_step_backward(gfsx_ctx *ctx, const gfsx_scene *sc,
                              int *err) {
     guarded(ctx, err, [&] {
         const char *who = "gfsx_flip_step_backward";
-        auto req = [&](bool c, const char *what) {
-            if (!c) throw Error(std::string(who) + ": " + what);
-        };
-        req(sc && pr && gr, "null scene, params or gradients");
-        req(pr->stencil == GFSX_TRILINEAR, "the adjoint covers the trilinear stencil only");
-        req(pr->order == GFSX_ORDER_KEEP,
-            "the adjoints refer to particles in input order (GFSX_ORDER_KEEP)");
-        req(pr->epilogue == GFSX_EPILOGUE_NONE,
-            "the adjoint covers the transfer without the FluidSimulation epilogue");
-        check_grid(sc->ni, sc->nj, sc->nk, sc->dx);
-        check_particles(sc->n, sc->x_in);
-        const int64_t n = sc->n;
-        const float *ins[6] = {sc->x_in, sc->y_in, sc->z_in, sc->vx_in, sc->vy_in, sc->vz_in};
-        const float *aout[6] = {gr->gx_out, gr->gy_out, gr->gz_out,
-                                gr->gvx_out, gr->gvy_out, gr->gvz_out};
-        float *ain[6] = {gr->gx_in, gr->gy_in, gr->gz_in, gr->gvx_in, gr->gvy_in, gr->gvz_in};
-        if (n > 0) {
-            for (int a = 0; a < 6; a++) {
-                req(ins[a] != nullptr, "null particle input");
-                req(aout[a] != nullptr, "null output adjoint");
-                req(ain[a] != nullptr, "null input adjoint");
-                for (int b = 0; b < 6; b++)
-                    req((const float *)ain[a] != ins[b], "input adjoints must not alias the inputs");
-            }
-        }
+        const AdjointIo io = check_adjoint(who, sc, pr, gr);
         // forward grids of this step, recomputed into the workspace
-        const int ni = sc->ni, nj = sc->nj, nk = sc->nk;
+        const int64_t n = sc->n;
         int64_t nf[3], tot = 0;
-        for (int c = 0; c < 3; c++) tot += (nf[c] = mac_count(c, ni, nj, nk));
+        for (int c = 0; c < 3; c++) tot += (nf[c] = mac_count(c, sc->ni, sc->nj, sc->nk));
         float *grid = ctx->f32(S_B_GRID, tot), *old = ctx->f32(S_B_OLD, tot);
         float *wgt = ctx->f32(S_B_WGT, tot);
-        // dL/dg then dL/dn, one block (the LDS-brick adjoint converts its
-        // fixed-point sums into it in one pass)
-        float *lg = ctx->f32(S_B_LG, 2 * tot), *ln = lg + tot;
         float *xo = ctx->f32(S_B_XO, 3 * n), *vo = ctx->f32(S_B_VO, 3 * n);
         const int64_t at[3] = {0, nf[0], nf[0] + nf[1]};
         gfsx_scene f = *sc;
@@ -971,16 +1075,7 @@ void gfsx_flip_step_backward(gfsx_ctx *ctx, const gfsx_scene *sc,
         f.id_out = nullptr;
         // (the brick sort, its sorted copies and the P2G grids; no particle
         // update), always by the scatter P2G, whose sort the adjoint reads
-        const int p2g_variant = ctx->p2g_variant;
-        ctx->p2g_variant = 4;
-        std::vector<gfsx::SceneDesc> laid;
-        try {
-            laid = flip_step_impl(ctx, &f, pr, 1, who, true);
-        } catch (...) {
-            ctx->p2g_variant = p2g_variant;
-            throw;
-        }
-        ctx->p2g_variant = p2g_variant;
+        const std::vector<gfsx::SceneDesc> laid = scatter_step(ctx, &f, pr, who, true);
         gfsx::AdjointSort so;
         so.pstart = (const uint32_t *)ctx->buf[S_K_PSTART];
         so.perm = (const uint32_t *)ctx->buf[S_PERM];
@@ -988,33 +1083,65 @@ void gfsx_flip_step_backward(gfsx_ctx *ctx, const gfsx_scene *sc,
         so.nbb = laid[0].nbb;
         so.gbx = laid[0].gbx;
         so.gby = laid[0].gby;
-        so.fx = (unsigned long long *)ctx->get(S_B_FX, sizeof(unsigned long long) * 2 * tot);
-        so.maxbits = ctx->u32(S_B_MAX, 1);
-        gfsx::StepAdjoint a;
-        std::memset(&a, 0, sizeof a);
-        a.n = n;
-        for (int c = 0; c < 3; c++) {
-            a.x[c] = ins[c];
-            a.v[c] = ins[3 + c];
-            a.ax[c] = aout[c];
-            a.av[c] = aout[3 + c];
-            a.gx[c] = ain[c];
-            a.gv[c] = ain[3 + c];
-            a.g[c] = grid + at[c];
-            a.old[c] = old + at[c];
-            a.w[c] = wgt + at[c];
-            a.lg[c] = lg + at[c];
-            a.ln[c] = ln + at[c];
+        run_adjoint(ctx, sc, pr, io, f, so);
+    });
+}
+
+int64_t gfsx_step_tape_starts(int ni, int nj, int nk, int *err) {
+    try {
+        check_grid(ni, nj, nk, 1.0);
+        ok(err);
+        return tape_starts(ni, nj, nk);
+    } catch (const std::exception &e) {
+        fail(err, e.what());
+        return 0;
+    }
+}
+
+void gfsx_flip_step_record(gfsx_ctx *ctx, const gfsx_scene *sc, const gfsx_step_params *pr,
+                           gfsx_step_tape *tape, int *err) {
+    guarded(ctx, err, [&] {
+        const char *who = "gfsx_flip_step_record";
+        check_taped(who, sc, pr, tape);
+        const int64_t n = sc->n;
+        const std::vector<gfsx::SceneDesc> laid = scatter_step(ctx, sc, pr, who, false);
+        if (n > 0) {
+            const size_t b = sizeof(float) * (size_t)n;
+            GFSX_HIP(hipMemcpyAsync(tape->perm, ctx->buf[S_PERM], sizeof(uint32_t) * (size_t)n, hipMemcpyDeviceToDevice,
+                                    ctx->stream));
+            float *xs[3] = {tape->xs, tape->ys, tape->zs};
+            for (int c = 0; c < 3; c++)
+                GFSX_HIP(hipMemcpyAsync(xs[c], ctx->buf[S_XS0 + c], b, hipMemcpyDeviceToDevice,
+                                        ctx->stream));
+            GFSX_HIP(hipMemcpyAsync(tape->start, ctx->buf[S_K_PSTART],
+                                    sizeof(uint32_t) * (size_t)tape->starts,
+                                    hipMemcpyDeviceToDevice, ctx->stream));
         }
-        a.mac = mac_grid(nullptr, nullptr, nullptr, ni, nj, nk, sc->dx);
-        a.r_pic = (float)pr->pic_ratio;
-        a.r_flip = (float)(1 - pr->pic_ratio);
-        a.h_half = (float)(0.5 * pr->dt);
-        a.h_full = (float)pr->dt;
-        a.h_sixth = (float)(pr->dt / 6.0f);
-        a.half_dx = (float)(0.5 * sc->dx);
-        a.inv_r2 = (float)(1.0 / (sc->dx * sc->dx));
-        gfsx::flip_step_adjoint(a, ctx->adjoint_variant == 0 ? &so : nullptr, ctx->stream);
+        tape->gbx = laid[0].gbx;
+        tape->gby = laid[0].gby;
+    });
+}
+
+void gfsx_flip_step_backward_tape(gfsx_ctx *ctx, const gfsx_scene *sc,
+                                  const gfsx_step_params *pr, const gfsx_scene_grad *gr,
+                                  const gfsx_step_tape *tape, int *err) {
+    guarded(ctx, err, [&] {
+        const char *who = "gfsx_flip_step_backward_tape";
+        const AdjointIo io = check_adjoint(who, sc, pr, gr);
+        check_taped(who, sc, pr, tape);
+        const int gbx = (sc->ni + 7) / 8, gby = (sc->nj + 7) / 8;
+        if (tape->gbx != gbx || tape->gby != gby)
+            throw Error(std::string(who) + ": the tape was not recorded for this grid");
+        gfsx::AdjointSort so;
+        so.pstart = tape->start;
+        so.perm = tape->perm;
+        so.xs[0] = tape->xs;
+        so.xs[1] = tape->ys;
+        so.xs[2] = tape->zs;
+        so.nbb = (uint32_t)(tape->starts - 2);
+        so.gbx = gbx;
+        so.gby = gby;
+        run_adjoint(ctx, sc, pr, io, *sc, so);
     });
 }
 

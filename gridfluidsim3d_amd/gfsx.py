@@ -76,6 +76,53 @@ class SceneGrad(ctypes.Structure):
     ]
 
 
+class StepTape(ctypes.Structure):
+    """ctypes image of ``gfsx_step_tape`` (include/gfsx.h)."""
+    _fields_ = [
+        ("perm", c_void_p), ("start", c_void_p),
+        ("xs", c_void_p), ("ys", c_void_p), ("zs", c_void_p),
+        ("n", c_int64), ("starts", c_int64),
+        ("gbx", c_int), ("gby", c_int),
+    ]
+
+
+class FlipTape:
+    """What ``Context.flip_step_record`` keeps of one step for its backward:
+    the brick sort of the input positions (``perm``, ``start``, ``xs``) and
+    the step's grids (``grids``: u, v, w after the body force; the normalised
+    P2G result; the P2G weights), all device tensors."""
+
+    def __init__(self, n, dims, starts, device):
+        import torch
+        i32, f32 = torch.int32, torch.float32
+        self.n = n
+        self.dims = tuple(dims)
+        self.perm = torch.empty(n, dtype=i32, device=device)
+        self.start = torch.empty(starts, dtype=i32, device=device)
+        self.xs = torch.empty((3, n), dtype=f32, device=device)
+        self.grids = [[torch.empty(sh, dtype=f32, device=device) for sh in mac_shapes(dims)]
+                      for _ in range(3)]
+        self.gbx = self.gby = 0
+
+    def c_tape(self):
+        import torch
+        t = StepTape()
+        t.perm = _dptr(self.perm, "perm", torch.int32, self.n)
+        t.start = _dptr(self.start, "start", torch.int32)
+        t.xs, t.ys, t.zs = [_dptr(r, "xs", torch.float32, self.n) for r in self.xs]
+        t.n = self.n
+        t.starts = int(self.start.numel())
+        t.gbx, t.gby = self.gbx, self.gby
+        return t
+
+    def set_grids(self, sc):
+        """Point a ``Scene``'s grid fields at the tape's grids."""
+        names = (("u", "v", "w"), ("u_old", "v_old", "w_old"), ("wu", "wv", "ww"))
+        for nms, ts in zip(names, self.grids):
+            for nm, t in zip(nms, ts):
+                setattr(sc, nm, _dptr(t, nm))
+
+
 def mac_shapes(dims):
     """(k, j, i) array shapes of the u, v, w faces of an ni*nj*nk grid
     (macvelocityfield.cpp:37-45)."""
@@ -473,11 +520,38 @@ class Context:
         pb.execute_lib_func(f, [self(), ctypes.byref(sc), ctypes.byref(pr)])
         return x_out, v_out, grids
 
+    def flip_step_record(self, x, v, dims, dx, dt, body_force=(0.0, -9.8, 0.0),
+                         pic_ratio=0.05):
+        """``flip_step_soa`` that also keeps what its backward needs
+        (``gfsx_flip_step_record``); returns (x_out, v_out, tape), tape a
+        ``FlipTape`` to pass to ``flip_step_backward``."""
+        import torch
+        n = int(x.shape[1])
+        x_out = torch.empty_like(x)
+        v_out = torch.empty_like(v)
+        f = pb.lib_func("gfsx_step_tape_starts", [c_int, c_int, c_int], c_int64)
+        starts = int(pb.execute_lib_func(f, [int(d) for d in dims]))
+        tape = FlipTape(n, dims, starts, x.device)
+        sc = self._soa_scene(x, v, dims, dx, x_out, v_out)
+        tape.set_grids(sc)
+        t = tape.c_tape()
+        pr = self._params(dt, body_force, pic_ratio, TRILINEAR, ORDER_KEEP)
+        self.use_torch_stream()
+        f = pb.lib_func("gfsx_flip_step_record",
+                        [c_void_p, POINTER(Scene), POINTER(StepParams), POINTER(StepTape)])
+        pb.execute_lib_func(f, [self(), ctypes.byref(sc), ctypes.byref(pr), ctypes.byref(t)])
+        tape.gbx, tape.gby = t.gbx, t.gby
+        return x_out, v_out, tape
+
     def flip_step_backward(self, x, v, gx_out, gv_out, dims, dx, dt,
-                           body_force=(0.0, -9.8, 0.0), pic_ratio=0.05, gx=None, gv=None):
+                           body_force=(0.0, -9.8, 0.0), pic_ratio=0.05, gx=None, gv=None,
+                           tape=None):
         """Adjoint of ``flip_step_soa`` (``gfsx_flip_step_backward``): from the
         step's inputs x, v and the adjoints of its outputs, all (3, n)
-        float32 tensors, returns (dL/dx, dL/dv)."""
+        float32 tensors, returns (dL/dx, dL/dv).  With the ``tape`` of
+        ``flip_step_record`` on the same inputs it runs
+        ``gfsx_flip_step_backward_tape`` (same bits, no recompute of the
+        forward's sort and grids)."""
         import torch
         gx = torch.empty_like(x) if gx is None else gx
         gv = torch.empty_like(v) if gv is None else gv
@@ -498,6 +572,17 @@ class Context:
         for nm, t in zip(("gvx_in", "gvy_in", "gvz_in"), gv):
             setattr(g, nm, _dptr(t, nm, f32, n))
         self.use_torch_stream()
+        if tape is not None:
+            if tape.n != n or tape.dims != tuple(dims):
+                raise ValueError("the tape was recorded for another step")
+            tape.set_grids(sc)
+            t = tape.c_tape()
+            f = pb.lib_func("gfsx_flip_step_backward_tape",
+                            [c_void_p, POINTER(Scene), POINTER(StepParams), POINTER(SceneGrad),
+                             POINTER(StepTape)])
+            pb.execute_lib_func(f, [self(), ctypes.byref(sc), ctypes.byref(pr), ctypes.byref(g),
+                                    ctypes.byref(t)])
+            return gx, gv
         f = pb.lib_func("gfsx_flip_step_backward",
                         [c_void_p, POINTER(Scene), POINTER(StepParams), POINTER(SceneGrad)])
         pb.execute_lib_func(f, [self(), ctypes.byref(sc), ctypes.byref(pr), ctypes.byref(g)])

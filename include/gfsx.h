@@ -313,6 +313,42 @@ void gfsx_flip_step_backward(gfsx_ctx *ctx, const gfsx_scene *scene,
                              const gfsx_step_params *params,
                              const gfsx_scene_grad *grad, int *err);
 
+/* What a forward step saves for its backward pass, so that the backward
+ * need not recompute the brick sort and P2G: the sort of the step's input
+ * positions.  Device arrays the caller allocates: perm and xs, ys, zs of n
+ * entries, start of gfsx_step_tape_starts(ni, nj, nk) entries.  The caller
+ * sets n and starts; gfsx_flip_step_record fills the arrays and gbx, gby. */
+typedef struct gfsx_step_tape {
+    uint32_t *perm;       /* sort slot -> input index */
+    uint32_t *start;      /* first sort slot of each brick bin, then the end */
+    float *xs, *ys, *zs;  /* input positions in sort order */
+    int64_t n;
+    int64_t starts;       /* entries of start */
+    int32_t gbx, gby;     /* bin grid of 8^3-cell groups along x and y */
+} gfsx_step_tape;
+
+/* Entries of gfsx_step_tape.start for an ni x nj x nk grid. */
+int64_t gfsx_step_tape_starts(int ni, int nj, int nk, int *err);
+
+/* gfsx_flip_step (trilinear, GFSX_ORDER_KEEP, no epilogue: the step
+ * gfsx_flip_step_backward differentiates) that also fills tape.  Every grid
+ * field of scene (u, v, w, u_old, v_old, w_old, wu, wv, ww) is required: the
+ * taped backward reads them.  Outputs and grids are bit-identical to
+ * gfsx_flip_step with the default P2G variant.  No counterpart in the
+ * reference, which has no adjoint; for gradient callers that keep each
+ * step's state (autograd rollouts). */
+void gfsx_flip_step_record(gfsx_ctx *ctx, const gfsx_scene *scene,
+                           const gfsx_step_params *params, gfsx_step_tape *tape, int *err);
+
+/* gfsx_flip_step_backward from a step recorded by gfsx_flip_step_record:
+ * scene holds that step's inputs and grids (unchanged since), tape its sort.
+ * Results are bit-identical to gfsx_flip_step_backward on the same inputs;
+ * the forward's sort and P2G are not recomputed. */
+void gfsx_flip_step_backward_tape(gfsx_ctx *ctx, const gfsx_scene *scene,
+                                  const gfsx_step_params *params,
+                                  const gfsx_scene_grad *grad, const gfsx_step_tape *tape,
+                                  int *err);
+
 /* Particle removal after a step: FluidSimulation::
  * _removeMarkerParticlesInSolidCells and _removeMarkerParticles
  * (fluidsimulation.cpp:1985-1988, 3213-3243; the reference's cap is 100,

@@ -30,10 +30,12 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_struct_layouts_match_header(tmp_path):
-    from gridfluidsim3d_amd.gfsx import Scene, StepParams
+    from gridfluidsim3d_amd.gfsx import Scene, SceneGrad, StepParams, StepTape
+    structs = ((Scene, "gfsx_scene"), (StepParams, "gfsx_step_params"),
+               (SceneGrad, "gfsx_scene_grad"), (StepTape, "gfsx_step_tape"))
     src = tmp_path / "layout.c"
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "gfsx.h"', "int main(void){"]
-    for cls, cname in ((Scene, "gfsx_scene"), (StepParams, "gfsx_step_params")):
+    for cls, cname in structs:
         lines.append('printf("%s %%zu\\n", sizeof(%s));' % (cname, cname))
         for f, _ in cls._fields_:
             lines.append('printf("%s.%s %%zu\\n", offsetof(%s, %s));' % (cname, f, cname, f))
@@ -44,7 +46,7 @@ def test_struct_layouts_match_header(tmp_path):
                    check=True)
     out = dict(l.split() for l in subprocess.run([str(exe)], capture_output=True, text=True,
                                                  check=True).stdout.split("\n") if l)
-    for cls, cname in ((Scene, "gfsx_scene"), (StepParams, "gfsx_step_params")):
+    for cls, cname in structs:
         assert int(out[cname]) == ctypes.sizeof(cls)
         for f, _ in cls._fields_:
             assert int(out["%s.%s" % (cname, f)]) == getattr(cls, f).offset, (cname, f)

@@ -127,6 +127,93 @@ def test_step_backward_is_deterministic(ctx):
     assert np.array_equal(r1[0], r2[0]) and np.array_equal(r1[1], r2[1])
 
 
+def _tape_scene(seed, n_far):
+    """random_blobs scene plus n_far particles scattered around and outside
+    the grid (the sort's overflow bin)."""
+    from gridfluidsim3d_amd import scenes
+    sc = scenes.random_blobs(seed, (40, 36, 44), 1.0 / 40.0, ppc=6, amplitude=2.0)
+    rng = np.random.default_rng(seed)
+    ext = np.array(sc.dims) * sc.dx
+    far = (rng.uniform(-0.4, 1.4, size=(n_far, 3)) * ext).astype(np.float32)
+    pos = np.concatenate([sc.pos, far])
+    vel = np.concatenate([sc.vel, rng.normal(size=(n_far, 3)).astype(np.float32)])
+    return sc, pos, vel, rng
+
+
+@pytest.mark.parametrize("seed,n_far", [(51, 0), (52, 300)])
+def test_step_record_and_tape_backward_bit_identical(ctx, seed, n_far):
+    """gfsx_flip_step_record gives gfsx_flip_step's outputs and grids bit for
+    bit, and gfsx_flip_step_backward_tape gives gfsx_flip_step_backward's
+    gradients bit for bit (the tape replaces the recompute of the same sort
+    and grids)."""
+    sc, pos, vel, rng = _tape_scene(seed, n_far)
+    dims, dx, dt = sc.dims, sc.dx, sc.dt
+    x, v = dev_soa(pos), dev_soa(vel)
+    xo, vo, grids = ctx.flip_step_soa(x, v, dims, dx, dt)
+    xr, vr, tape = ctx.flip_step_record(x, v, dims, dx, dt)
+    for got, want in zip([xr, vr] + tape.grids[0], [xo, vo] + grids):
+        assert torch.equal(got, want)
+    a = dev_soa(rng.normal(size=pos.shape))
+    b = dev_soa(rng.normal(size=pos.shape))
+    r = [t.cpu().numpy() for t in ctx.flip_step_backward(x, v, a, b, dims, dx, dt)]
+    t = [t.cpu().numpy() for t in ctx.flip_step_backward(x, v, a, b, dims, dx, dt, tape=tape)]
+    assert np.array_equal(r[0], t[0]) and np.array_equal(r[1], t[1])
+    # the tape outlives later steps on the same context
+    ctx.flip_step_record(xr, vr, dims, dx, dt)
+    t2 = [t.cpu().numpy() for t in ctx.flip_step_backward(x, v, a, b, dims, dx, dt, tape=tape)]
+    assert np.array_equal(r[0], t2[0]) and np.array_equal(r[1], t2[1])
+
+
+def test_rollout_tape_matches_recompute(ctx):
+    """A 6-step autograd rollout with taped steps (the default) and with the
+    recomputing backward: the same loss and bit-identical gradients."""
+    from gridfluidsim3d_amd import autograd as ga
+    sc, pos, vel, rng = _tape_scene(53, 100)
+    dims, dx, dt = sc.dims, sc.dx, sc.dt
+    target = torch.tensor(rng.uniform(0.0, 4.0, size=(dims[2], dims[1], dims[0])),
+                          dtype=torch.float32, device="cuda")
+    grads = []
+    for tape in (True, False):
+        x0 = dev_soa(pos).requires_grad_(True)
+        v0 = dev_soa(vel).requires_grad_(True)
+        x, _v = ga.rollout(ctx, x0, v0, dims, dx, 6, dt, tape=tape)
+        L = ga.density_loss(ctx, x, target, dims, dx)
+        L.backward()
+        grads.append((L.item(), x0.grad.cpu().numpy(), v0.grad.cpu().numpy()))
+    assert grads[0][0] == grads[1][0]
+    assert np.array_equal(grads[0][1], grads[1][1]) and np.array_equal(grads[0][2], grads[1][2])
+
+
+def test_step_tape_rejects_mismatch(ctx):
+    import ctypes
+    from gridfluidsim3d_amd import gfsx
+    from gridfluidsim3d_amd import pybindings as pb
+    dims, dx, dt = (6, 6, 6), 0.1, 0.01
+    pos, vel = small_scene(46, dims, dx, box=((1, 4), (1, 4), (1, 4)))
+    x, v = dev_soa(pos), dev_soa(vel)
+    _xo, _vo, tape = ctx.flip_step_record(x, v, dims, dx, dt)
+    # a tape of another grid or particle count
+    with pytest.raises(ValueError, match="another step"):
+        ctx.flip_step_backward(x, v, x, v, (6, 6, 7), dx, dt, tape=tape)
+    f = pb.lib_func("gfsx_flip_step_record",
+                    [ctypes.c_void_p, ctypes.POINTER(gfsx.Scene), ctypes.POINTER(gfsx.StepParams),
+                     ctypes.POINTER(gfsx.StepTape)])
+    pr = ctx._params(dt, (0.0, -9.8, 0.0), 0.05, gfsx.TRILINEAR, gfsx.ORDER_KEEP)
+    xo, vo = torch.empty_like(x), torch.empty_like(v)
+    sc = ctx._soa_scene(x, v, dims, dx, xo, vo)
+    t = tape.c_tape()
+    with pytest.raises(RuntimeError, match="every grid"):
+        pb.execute_lib_func(f, [ctx(), ctypes.byref(sc), ctypes.byref(pr), ctypes.byref(t)])
+    tape.set_grids(sc)
+    t.starts -= 1
+    with pytest.raises(RuntimeError, match="gfsx_step_tape_starts"):
+        pb.execute_lib_func(f, [ctx(), ctypes.byref(sc), ctypes.byref(pr), ctypes.byref(t)])
+    t.starts += 1
+    t.n -= 1
+    with pytest.raises(RuntimeError, match="particle count"):
+        pb.execute_lib_func(f, [ctx(), ctypes.byref(sc), ctypes.byref(pr), ctypes.byref(t)])
+
+
 def test_step_backward_position_only_and_velocity_only(ctx):
     """Adjoints of x' alone and of v' alone, each against the dense
     reference (covers the RK4 path and the PIC/FLIP path separately)."""

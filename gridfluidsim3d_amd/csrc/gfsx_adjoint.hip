@@ -23,7 +23,10 @@
 //      quotient rule of the normalisation,
 //        dL/dv_p += K_pf lam_f / W_f,
 //        dL/dx_p += lam_f (v_p - n_f) / W_f * dK_pf/dx_p,
-//      with lam_f = dL/dg_f + dL/dn_f (the body force is additive).
+//      with lam_f = dL/dg_f + dL/dn_f (the body force is additive).  A pass
+//      over the faces first forms A_f = lam_f / W_f and B_f = A_f n_f, so
+//      the gather reads two arrays per face: lam_f (v_p - n_f) / W_f =
+//      v_p A_f - B_f.
 //   3. (scalar splat) k_adj_p2g_scalar: the adjoint of the unnormalised
 //      S = sum K val, W = sum K of gfsx_p2g for any radius.
 //
@@ -496,13 +499,51 @@ __global__ __launch_bounds__(kBlock) void k_adj_advect_brick(StepAdjoint a, Adjo
     }
 }
 
-// dL/dg, dL/dn from the fixed-point sums (one array of `count` faces).
-__global__ __launch_bounds__(kBlock) void k_fixed_to_float(const unsigned long long *fx,
-                                                           float *out, int64_t count,
-                                                           const uint32_t *maxbits, int64_t n) {
+struct FaceCount {
+    int64_t nf[3], tot;   // faces per component, all faces
+};
+
+// The two factors the P2G adjoint gathers per face, from the face's grid
+// adjoints lg = dL/dg, ln = dL/dn, weight w and normalised value nv:
+// A = (lg + ln) / w and B = A nv where w > 0, else 0 (the face took no
+// weight and its value was set, not normalised).
+__device__ __forceinline__ void face_coeffs(float lg, float ln, float w, float nv, float &A,
+                                            float &B) {
+    const bool on = w > 0.0f;
+    const float q = (lg + ln) * (1.0f / w);
+    A = on ? q : 0.0f;
+    B = on ? q * nv : 0.0f;
+}
+
+// the face of flat index f < tot (u faces, then v, then w): component, index
+__device__ __forceinline__ int face_comp(int64_t f, const int64_t nf[3], int64_t &l) {
+    const int c = f < nf[0] ? 0 : f < nf[0] + nf[1] ? 1 : 2;
+    l = f - (c == 0 ? 0 : c == 1 ? nf[0] : nf[0] + nf[1]);
+    return c;
+}
+
+// dL/dg, dL/dn from the fixed-point sums, turned into A and B (written over
+// lg and ln) for one thread per face.
+__global__ __launch_bounds__(kBlock) void k_fixed_to_coeffs(const unsigned long long *fx,
+                                                            StepAdjoint a, FaceCount fc,
+                                                            const uint32_t *maxbits) {
     const int64_t f = blockIdx.x * (int64_t)kBlock + threadIdx.x;
-    if (f >= count) return;
-    out[f] = (float)((double)(long long)fx[f] / fixed_scale(*maxbits, n));
+    if (f >= fc.tot) return;
+    const double scale = fixed_scale(*maxbits, a.n);
+    const float lg = (float)((double)(long long)fx[f] / scale);
+    const float ln = (float)((double)(long long)fx[fc.tot + f] / scale);
+    int64_t l;
+    const int c = face_comp(f, fc.nf, l);
+    face_coeffs(lg, ln, a.w[c][l], a.old[c][l], a.lg[c][l], a.ln[c][l]);
+}
+
+// the same from float grid adjoints (the atomic-add variant)
+__global__ __launch_bounds__(kBlock) void k_face_coeffs(StepAdjoint a, FaceCount fc) {
+    const int64_t f = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    if (f >= fc.tot) return;
+    int64_t l;
+    const int c = face_comp(f, fc.nf, l);
+    face_coeffs(a.lg[c][l], a.ln[c][l], a.w[c][l], a.old[c][l], a.lg[c][l], a.ln[c][l]);
 }
 
 // Faces of component c within the kernel radius R = dx of x: the node
@@ -539,10 +580,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_p2g_step(StepAdjoint a) {
             lo[ax] = any ? (int)fmax(s, 0.0) : 0;
             hi[ax] = any ? (int)fmin(s + 1.0, (double)dims[ax] - 1.0) : -1;
         }
-        const float *__restrict__ W = a.w[c];
-        const float *__restrict__ N = a.old[c];
-        const float *__restrict__ LG = a.lg[c];
-        const float *__restrict__ LN = a.ln[c];
+        const float *__restrict__ A = a.lg[c];   // per-face coefficients (face_coeffs)
+        const float *__restrict__ B = a.ln[c];
         for (int k = lo[2]; k <= hi[2]; k++) {
             const float gz = (float)((float)k * dx);
             for (int j = lo[1]; j <= hi[1]; j++) {
@@ -554,13 +593,10 @@ __global__ __launch_bounds__(kBlock) void k_adj_p2g_step(StepAdjoint a) {
                     const float t = d2 * a.inv_r2;
                     if (!(t < 1.0f)) continue;
                     const uint32_t f = (uint32_t)i + (uint32_t)dims[0] * ((uint32_t)j + (uint32_t)dims[1] * (uint32_t)k);
-                    const float wf = W[f];
-                    if (!(wf > 0.0f)) continue;
-                    const float lam = LG[f] + LN[f];
-                    const float iw = 1.0f / wf;
-                    lv[c] += wyvill(t) * lam * iw;
-                    // d(d2)/dx_p = -2 (g - p)
-                    const float coef = lam * (vp[c] - N[f]) * iw * wyvill_dt(t) * a.inv_r2 * -2.0f;
+                    const float af = A[f], bf = B[f];
+                    lv[c] += wyvill(t) * af;
+                    // d(d2)/dx_p = -2 (g - p); lam (v_p - n_f) / W_f = v_p A - B
+                    const float coef = fmaf(vp[c], af, -bf) * wyvill_dt(t) * a.inv_r2 * -2.0f;
                     lx[0] += coef * vx;
                     lx[1] += coef * vy;
                     lx[2] += coef * vz;
@@ -631,6 +667,9 @@ void flip_step_adjoint(const StepAdjoint &a, const AdjointSort *so, hipStream_t 
     int64_t nf[3], tot = 0;
     for (int c = 0; c < 3; c++)
         tot += (nf[c] = (int64_t)(a.mac.ni + (c == 0)) * (a.mac.nj + (c == 1)) * (a.mac.nk + (c == 2)));
+    FaceCount fc;
+    for (int c = 0; c < 3; c++) fc.nf[c] = nf[c];
+    fc.tot = tot;
     if (so) {
         // the grid adjoints are one block: u,v,w of dL/dg, then of dL/dn
         if (a.lg[1] != a.lg[0] + nf[0] || a.lg[2] != a.lg[1] + nf[1] ||
@@ -649,9 +688,9 @@ void flip_step_adjoint(const StepAdjoint &a, const AdjointSort *so, hipStream_t 
                                dim3(kBlock), 0, s, a, *so, nbrick);
             GFSX_LAUNCH_CHECK();
         }
-        hipLaunchKernelGGL(k_fixed_to_float, dim3(blocks_for(2 * tot)), dim3(kBlock), 0, s,
-                           (const unsigned long long *)so->fx, a.lg[0], 2 * tot,
-                           (const uint32_t *)so->maxbits, a.n);
+        hipLaunchKernelGGL(k_fixed_to_coeffs, dim3(blocks_for(tot)), dim3(kBlock), 0, s,
+                           (const unsigned long long *)so->fx, a, fc,
+                           (const uint32_t *)so->maxbits);
         GFSX_LAUNCH_CHECK();
     } else {
         for (int c = 0; c < 3; c++) {
@@ -662,6 +701,8 @@ void flip_step_adjoint(const StepAdjoint &a, const AdjointSort *so, hipStream_t 
             hipLaunchKernelGGL(k_adj_advect, dim3(blocks_for(a.n)), dim3(kBlock), 0, s, a);
             GFSX_LAUNCH_CHECK();
         }
+        hipLaunchKernelGGL(k_face_coeffs, dim3(blocks_for(tot)), dim3(kBlock), 0, s, a, fc);
+        GFSX_LAUNCH_CHECK();
     }
     if (a.n <= 0) return;
     hipLaunchKernelGGL(k_adj_p2g_step, dim3(blocks_for(a.n)), dim3(kBlock), 0, s, a);

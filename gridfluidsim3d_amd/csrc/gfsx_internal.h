@@ -239,7 +239,8 @@ struct StepAdjoint {
     const float *g[3];             // grid after the body force
     const float *old[3];           // normalised P2G result
     const float *w[3];             // P2G weight sums
-    float *lg[3], *ln[3];          // dL/dg, dL/dn (workspace, zeroed here)
+    float *lg[3], *ln[3];          // dL/dg, dL/dn (workspace, zeroed here), then
+                                   // the P2G adjoint's per-face factors A, B
     MacGrid mac;                   // dims, dx, limits (f unused)
     float r_pic, r_flip;
     float h_half, h_full, h_sixth;

@@ -215,8 +215,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_advect(StepAdjoint a) {
 }
 
 // The P2G bricks (4x4x4 cells) of the forward step's brick sort, one
-// work-group each, in two passes over the brick's particles (one
-// contiguous range of sorted slots).  A tile of every face index within M
+// work-group each, over the brick's particles (one contiguous range of
+// sorted slots).  A tile of every face index within M
 // cells of the brick's
 // trilinear footprint holds the grid after the body force and the
 // normalised P2G result (0 for faces outside the arrays, which is what an
@@ -224,16 +224,14 @@ __global__ __launch_bounds__(kBlock) void k_adj_advect(StepAdjoint a) {
 // moved more than M cells) use global memory.  Work-groups past the bricks
 // take the particles of the overflow bin (outside the bin grid).
 //
-//   pass 1 (SCATTER false): every sample's adjoint is computed, dL/dx and
-//     dL/dv are written, and the largest |sample adjoint| is reduced into
-//     maxbits.
-//   pass 2 (SCATTER true): the samples are recomputed and lam * weight is
-//     added as a 64-bit fixed-point integer, scale 2^61 / (5 n max), into an
-//     LDS tile (ds_add_u64; LDS float atomics run at a small fraction of
-//     the integer rate on gfx950) and then once per face into global
-//     accumulators.  Integer sums do not depend on the order of the adds,
-//     so the grid adjoints are the same on every run; no face can overflow:
-//     it receives at most 5n terms of magnitude at most 2^61 / (5n).
+// Every sample's adjoint is computed, dL/dx and dL/dv are written, and
+// lam * weight is added as a 64-bit fixed-point integer, scale
+// 2^61 / (5 n B) with B a bound on |sample adjoint| (adj_bound), into an LDS
+// tile (ds_add_u64; LDS float atomics run at a small fraction of the integer
+// rate on gfx950) and then once per face into global accumulators.  Integer
+// sums do not depend on the order of the adds, so the grid adjoints are the
+// same on every run; no face can overflow: it receives at most 5n terms of
+// magnitude at most 2^61 / (5n).
 namespace abrick {
 constexpr int BX = 4, BY = 4, BZ = 4;   // the P2G bricks of the forward's brick sort
 constexpr int M = 1;
@@ -242,41 +240,57 @@ constexpr int TN = TX * TY * TZ;
 constexpr int NEXTRA = 16;
 }  // namespace abrick
 
-// fixed-point scale for adjoint terms of magnitude at most maxabs, n particles
-__device__ __forceinline__ double fixed_scale(uint32_t maxbits, int64_t n) {
-    const float m = __uint_as_float(maxbits);
-    if (!(m > 0.0f)) return 1.0;
-    return 2305843009213693952.0 / ((double)m * 5.0 * (double)(n > 0 ? n : 1));   // 2^61
+// An upper bound B on |lam| over every sample adjoint of adj_particle, from
+// bound[0] = max |dL/dx'|, bound[1] = max |dL/dv'| (over particles and
+// components) and bound[2] = max |g| over the faces (float bits).  A
+// trilinear sample's derivative along an axis is a convex combination of
+// differences of two grid values, at most 2 max|g| per cell, so a stage's
+// position adjoint is at most kappa = 3 * 2 max|g| / dx times its largest
+// sample adjoint; the walk of adj_particle then gives, with h = dt and
+// A, V the two maxima,
+//   k4: h/6 A,   k3: 2h/6 A + h kappa L4,   k2: 2h/6 A + h/2 kappa L3,
+//   x:  h/6 A + h/2 kappa L2 + |r_pic + r_flip| V,   n: |r_flip| V,
+// and B is their maximum, widened by 1% for the FP32 rounding of the
+// samples.  It needs no pass over the samples themselves.
+__device__ __forceinline__ double adj_bound(const uint32_t *bound, const StepAdjoint &a) {
+    const double A = __uint_as_float(bound[0]), V = __uint_as_float(bound[1]);
+    const double kappa = 6.0 * (double)__uint_as_float(bound[2]) * a.mac.inv_dx;
+    const double h6 = fabs((double)a.h_sixth), hh = fabs((double)a.h_half);
+    const double hf = fabs((double)a.h_full);
+    const double l4 = h6 * A;
+    const double l3 = 2.0 * h6 * A + hf * kappa * l4;
+    const double l2 = 2.0 * h6 * A + hh * kappa * l3;
+    const double l1 = h6 * A + hh * kappa * l2 + fabs((double)(a.r_pic + a.r_flip)) * V;
+    const double ln = fabs((double)a.r_flip) * V;
+    return 1.01 * fmax(fmax(fmax(l4, l3), fmax(l2, l1)), ln);
+}
+
+// fixed-point scale for adjoint terms of magnitude at most B, n particles
+__device__ __forceinline__ double fixed_scale(const uint32_t *bound, const StepAdjoint &a) {
+    const double B = adj_bound(bound, a);
+    if (!(B > 0.0) || !(B < 1.0e300)) return 1.0;
+    return 2305843009213693952.0 / (B * 5.0 * (double)(a.n > 0 ? a.n : 1));   // 2^61
 }
 
 __device__ __forceinline__ unsigned long long to_fixed(float v, double scale) {
     return (unsigned long long)__double2ll_rn((double)v * scale);
 }
 
-template <bool SCATTER>
 __global__ __launch_bounds__(kBlock) void k_adj_advect_brick(StepAdjoint a, AdjointSort so,
                                                              uint32_t nbrick) {
     using namespace abrick;
     __shared__ float gt[6 * TN];                 // g (0-2) and n (3-5) on the tile
-    __shared__ unsigned long long at[SCATTER ? 6 * TN : 1];   // fixed-point dL/dg, dL/dn
-    __shared__ float s_max[kBlock / 64];
+    __shared__ unsigned long long at[6 * TN];    // fixed-point dL/dg, dL/dn
     const int tid = threadIdx.x;
     const uint32_t bid = blockIdx.x;
-    const double scale = SCATTER ? fixed_scale(*so.maxbits, a.n) : 0.0;
-    float mx = 0.0f;   // largest |sample adjoint| of this thread (pass 1)
+    const double scale = fixed_scale(so.bound, a);
     auto fixed_sink = [&](int arr, int, int, int, uint32_t o, float v) {
         atomicAdd(so.fx + (arr < 3 ? a.lg[arr] - a.lg[0] : a.ln[arr - 3] - a.lg[0]) + o,
                   to_fixed(v, scale));
     };
     // global path of one sample (outside the tile, or an overflow particle)
     auto global_adjoint = [&](int base, const float q[3], const float lam[3], float lq[3]) {
-        if (SCATTER)
-            mac_adjoint(a, base == 0 ? a.g : a.old, base, q, lam, lq, fixed_sink);
-        else {
-            for (int c = 0; c < 3; c++) mx = fmaxf(mx, fabsf(lam[c]));
-            mac_adjoint(a, base == 0 ? a.g : a.old, base, q, lam, lq,
-                        [](int, int, int, int, uint32_t, float) {});
-        }
+        mac_adjoint(a, base == 0 ? a.g : a.old, base, q, lam, lq, fixed_sink);
     };
     struct GlobalFixed {
         const GlobalPolicy &gp;
@@ -289,21 +303,10 @@ __global__ __launch_bounds__(kBlock) void k_adj_advect_brick(StepAdjoint a, Adjo
     const GlobalPolicy gpol{a};
     const GlobalFixed gfix{gpol, global_adjoint};
     auto finish = [&](uint32_t q, const float lx[3], const float av[3]) {
-        if (SCATTER) return;
         const uint32_t p = so.perm[q];
         for (int c = 0; c < 3; c++) {
             a.gx[c][p] = lx[c];
             a.gv[c][p] = a.r_flip * av[c];
-        }
-    };
-    auto block_max = [&]() {
-        float m = mx;
-        for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
-        if ((tid & 63) == 0) s_max[tid >> 6] = m;
-        __syncthreads();
-        if (tid == 0) {
-            for (int w = 1; w < kBlock / 64; w++) m = fmaxf(m, s_max[w]);
-            if (m > 0.0f) atomicMax(so.maxbits, __float_as_uint(m));
         }
     };
     if (bid >= nbrick) {
@@ -320,7 +323,6 @@ __global__ __launch_bounds__(kBlock) void k_adj_advect_brick(StepAdjoint a, Adjo
             adj_particle(a, x0, ax, av, lx, gfix);
             finish(q, lx, av);
         }
-        if (!SCATTER) block_max();
         return;
     }
     // brick bid: bin bid of the sort, one contiguous range of slots
@@ -360,7 +362,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_advect_brick(StepAdjoint a, Adjo
             const int e = tid + m * kBlock;
             if (e < 6 * TN) {
                 gt[e] = v[m];
-                if (SCATTER) at[e] = 0ull;
+                at[e] = 0ull;
             }
         }
     }
@@ -403,10 +405,6 @@ __global__ __launch_bounds__(kBlock) void k_adj_advect_brick(StepAdjoint a, Adjo
                         sx * tz * (v011 - v001) + tx * tz * (v111 - v101));
         dt[2] += lam * (sx * sy * (v001 - v000) + tx * sy * (v101 - v100) +
                         sx * ty * (v011 - v010) + tx * ty * (v111 - v110));
-        if (!SCATTER) {
-            mx = fmaxf(mx, fabsf(lam));
-            return;
-        }
         unsigned long long *t = at + arr * TN + l;
         atomicAdd(t, to_fixed(lam * (sx * sy * sz), scale));
         atomicAdd(t + 1, to_fixed(lam * (tx * sy * sz), scale));
@@ -477,10 +475,6 @@ __global__ __launch_bounds__(kBlock) void k_adj_advect_brick(StepAdjoint a, Adjo
         adj_particle(a, x0, ax, av, lx, tpol);
         finish(q, lx, av);
     }
-    if (!SCATTER) {
-        block_max();
-        return;
-    }
     __syncthreads();
     for (int e = tid; e < 6 * TN; e += kBlock) {
         const unsigned long long v = at[e];
@@ -526,15 +520,54 @@ __device__ __forceinline__ int face_comp(int64_t f, const int64_t nf[3], int64_t
 // lg and ln) for one thread per face.
 __global__ __launch_bounds__(kBlock) void k_fixed_to_coeffs(const unsigned long long *fx,
                                                             StepAdjoint a, FaceCount fc,
-                                                            const uint32_t *maxbits) {
+                                                            const uint32_t *bound) {
     const int64_t f = blockIdx.x * (int64_t)kBlock + threadIdx.x;
     if (f >= fc.tot) return;
-    const double scale = fixed_scale(*maxbits, a.n);
+    const double scale = fixed_scale(bound, a);
     const float lg = (float)((double)(long long)fx[f] / scale);
     const float ln = (float)((double)(long long)fx[fc.tot + f] / scale);
     int64_t l;
     const int c = face_comp(f, fc.nf, l);
     face_coeffs(lg, ln, a.w[c][l], a.old[c][l], a.lg[c][l], a.ln[c][l]);
+}
+
+// bound[0..2] = max |dL/dx'|, max |dL/dv'| over the particles and max |g|
+// over the faces (adj_bound's inputs), as float bits: non-negative floats
+// order like their bit patterns.  NaN entries are passed over, as fmaxf
+// does.
+__global__ __launch_bounds__(kBlock) void k_adj_bound(StepAdjoint a, FaceCount fc,
+                                                      uint32_t *bound) {
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    const int64_t i0 = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    float m[3] = {0.0f, 0.0f, 0.0f};
+    for (int64_t p = i0; p < a.n; p += stride)
+        for (int c = 0; c < 3; c++) {
+            m[0] = fmaxf(m[0], fabsf(a.ax[c][p]));
+            m[1] = fmaxf(m[1], fabsf(a.av[c][p]));
+        }
+    for (int64_t f = i0; f < fc.tot; f += stride) {
+        int64_t l;
+        const int c = face_comp(f, fc.nf, l);
+        m[2] = fmaxf(m[2], fabsf(a.g[c][l]));
+    }
+    // one atomic per work-group and word at most, and none once the word
+    // holds as much (same-address atomics serialise in L2)
+    __shared__ float s_m[3][kBlock / 64];
+    for (int k = 0; k < 3; k++) {
+        float v = m[k];
+        for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off));
+        if ((threadIdx.x & 63) == 0) s_m[k][threadIdx.x >> 6] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        const int k = threadIdx.x;
+        float v = s_m[k][0];
+        for (int w = 1; w < kBlock / 64; w++) v = fmaxf(v, s_m[k][w]);
+        const uint32_t bits = __float_as_uint(v);
+        if (v > 0.0f && bits > __hip_atomic_load(bound + k, __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_AGENT))
+            atomicMax(bound + k, bits);
+    }
 }
 
 // the same from float grid adjoints (the atomic-add variant)
@@ -677,20 +710,22 @@ void flip_step_adjoint(const StepAdjoint &a, const AdjointSort *so, hipStream_t 
             a.ln[2] != a.ln[1] + nf[1])
             throw Error("flip_step_adjoint: grid adjoint arrays must be contiguous");
         fill_u32((uint32_t *)so->fx, 0u, 4 * tot, s);
-        fill_u32(so->maxbits, 0u, 1, s);
+        fill_u32(so->bound, 0u, 3, s);
+        // the fixed-point scale's bound, then one work-group per bin of the
+        // brick sort and the overflow bin's
+        hipLaunchKernelGGL(k_adj_bound, dim3((unsigned)std::min<int64_t>(
+                               blocks_for(std::max<int64_t>(a.n, tot)), 1024)),
+                           dim3(kBlock), 0, s, a, fc, so->bound);
+        GFSX_LAUNCH_CHECK();
         if (a.n > 0) {
-            // one work-group per bin of the brick sort, then the overflow bin's
             const uint32_t nbrick = so->nbb;
-            hipLaunchKernelGGL(k_adj_advect_brick<false>, dim3(nbrick + abrick::NEXTRA),
-                               dim3(kBlock), 0, s, a, *so, nbrick);
-            GFSX_LAUNCH_CHECK();
-            hipLaunchKernelGGL(k_adj_advect_brick<true>, dim3(nbrick + abrick::NEXTRA),
-                               dim3(kBlock), 0, s, a, *so, nbrick);
+            hipLaunchKernelGGL(k_adj_advect_brick, dim3(nbrick + abrick::NEXTRA), dim3(kBlock),
+                               0, s, a, *so, nbrick);
             GFSX_LAUNCH_CHECK();
         }
         hipLaunchKernelGGL(k_fixed_to_coeffs, dim3(blocks_for(tot)), dim3(kBlock), 0, s,
                            (const unsigned long long *)so->fx, a, fc,
-                           (const uint32_t *)so->maxbits);
+                           (const uint32_t *)so->bound);
         GFSX_LAUNCH_CHECK();
     } else {
         for (int c = 0; c < 3; c++) {

@@ -725,7 +725,7 @@ void run_adjoint(gfsx_ctx *ctx, const gfsx_scene *sc, const gfsx_step_params *pr
     // fixed-point sums into it in one pass)
     float *lg = ctx->f32(S_B_LG, 2 * tot), *ln = lg + tot;
     so.fx = (unsigned long long *)ctx->get(S_B_FX, sizeof(unsigned long long) * 2 * tot);
-    so.maxbits = ctx->u32(S_B_MAX, 1);
+    so.bound = ctx->u32(S_B_MAX, 3);
     const float *g[3] = {grids.u, grids.v, grids.w};
     const float *old[3] = {grids.u_old, grids.v_old, grids.w_old};
     const float *wgt[3] = {grids.wu, grids.wv, grids.ww};

@@ -255,15 +255,15 @@ struct StepAdjoint {
 // are slots [pstart[lb], pstart[lb + 1]) of the sorted positions xs, and
 // perm maps a slot to its input index.  fx: fixed-point accumulators for
 // the grid adjoints (int64, the face arrays of dL/dg u,v,w then dL/dn u,v,w
-// back to back); maxbits: one word for the largest |sample adjoint| (float
-// bits).
+// back to back); bound: three words for the fixed-point scale's bound
+// (max |dL/dx'|, max |dL/dv'|, max |g|, float bits).
 struct AdjointSort {
     const uint32_t *pstart, *perm;
     const float *xs[3];
     uint32_t nbb;
     int gbx, gby;
     unsigned long long *fx;
-    uint32_t *maxbits;
+    uint32_t *bound;
 };
 
 // FluidSimulation's face fill and extrapolation around the transfer (one

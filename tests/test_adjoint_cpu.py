@@ -129,3 +129,68 @@ def test_dense_rollout_density_gradient_matches_fd(orc64):
         fd = (lp - lm) / (2 * eps)
         an = float((g.numpy() * d).sum())
         assert abs(fd - an) <= 1e-6 * max(abs(fd), 1.0), (fd, an)
+
+
+def _step_samples(x, v, dims, dx, dt, body_force=(0.0, -9.8, 0.0), pic_ratio=0.05):
+    """dense_flip.step, also returning its grids after the body force and its
+    five sample tensors (k1 = I(g, x), k2, k3, k4, I(n, x)) with their
+    gradients retained."""
+    ni, nj, nk = dims
+    masks = dfl.border_masks(x, dims, dx)
+    grids, olds = [], []
+    for c in range(3):
+        off = [0.5 * dx] * 3
+        off[c] = 0.0
+        nd = (ni + (c == 0), nj + (c == 1), nk + (c == 2))
+        S, W = dfl.splat(x, v[:, c], nd, off, dx, dx)
+        n = torch.where(W > 0, S / torch.where(W > 0, W, torch.ones_like(W)), torch.zeros_like(S))
+        bf = float(torch.tensor(body_force[c], dtype=torch.float32))
+        grids.append((n + masks[c].to(torch.float64) * (bf * dt)).detach())
+        olds.append(n.detach())
+    k1 = dfl.sample(grids, x, dims, dx)
+    vold = dfl.sample(olds, x, dims, dx)
+    h = 0.5 * dt
+    k2 = dfl.sample(grids, x + h * k1, dims, dx)
+    k3 = dfl.sample(grids, x + h * k2, dims, dx)
+    k4 = dfl.sample(grids, x + dt * k3, dims, dx)
+    samples = [k1, k2, k3, k4, vold]
+    for t in samples:
+        t.retain_grad()
+    v1 = pic_ratio * k1 + (1.0 - pic_ratio) * (v + k1 - vold)
+    x1 = x + (dt / 6.0) * (k1 + 2.0 * k2 + 2.0 * k3 + k4)
+    return x1, v1, grids, samples
+
+
+@pytest.mark.parametrize("seed,amp,dt", [(61, 1.5, 1.0 / 60.0), (62, 20.0, 1.0 / 30.0),
+                                         (63, 60.0, 1.0 / 20.0)])
+def test_fixed_point_bound_covers_sample_adjoints(seed, amp, dt):
+    """The brick adjoint scales its fixed-point sums by an a-priori bound B on
+    every sample adjoint, from max |dL/dx'|, max |dL/dv'| and max |g| alone
+    (adj_bound, gfsx_adjoint.hip): no face sum can overflow only if B holds.
+    The sample adjoints here are the gradients of <a, x'> + <b, v'> with
+    respect to the step's five sample values (autograd through the dense
+    FP64 step), up to CFL numbers of about 60 (there B is 1e5 times the
+    largest sample adjoint: 17 of the 61 bits)."""
+    dims, dx = (10, 9, 8), 0.1
+    pos, vel = small_scene(seed, dims, dx, amp=amp)
+    rng = np.random.default_rng(seed)
+    x = torch.tensor(pos, dtype=torch.float64, requires_grad=True)
+    v = torch.tensor(vel, dtype=torch.float64)
+    x1, v1, grids, samples = _step_samples(x, v, dims, dx, dt)
+    a = torch.tensor(rng.normal(size=pos.shape) * rng.uniform(0.1, 10.0))
+    b = torch.tensor(rng.normal(size=pos.shape) * rng.uniform(0.1, 10.0))
+    ((a * x1).sum() + (b * v1).sum()).backward()
+    lam = max(float(t.grad.abs().max()) for t in samples)
+    A, V = float(a.abs().max()), float(b.abs().max())
+    G = max(float(g.abs().max()) for g in grids)
+    kappa = 6.0 * G / dx
+    h6, hh, hf = dt / 6.0, 0.5 * dt, dt
+    r_pic, r_flip = 0.05, 0.95
+    l4 = h6 * A
+    l3 = 2.0 * h6 * A + hf * kappa * l4
+    l2 = 2.0 * h6 * A + hh * kappa * l3
+    l1 = h6 * A + hh * kappa * l2 + abs(r_pic + r_flip) * V
+    B = 1.01 * max(l4, l3, l2, l1, abs(r_flip) * V)
+    cfl = float(np.abs(vel).max()) * dt / dx
+    print("CFL %.1f: max |sample adjoint| %.4g, bound %.4g (%.1fx)" % (cfl, lam, B, B / lam))
+    assert lam <= B

@@ -265,15 +265,24 @@ __device__ __forceinline__ double adj_bound(const uint32_t *bound, const StepAdj
     return 1.01 * fmax(fmax(fmax(l4, l3), fmax(l2, l1)), ln);
 }
 
-// fixed-point scale for adjoint terms of magnitude at most B, n particles
+// Fixed-point scale for adjoint terms of magnitude at most B, n particles:
+// min(2^61 / (5n), 2^50) / B, so that a face's sum of at most 5n terms stays
+// below 2^61 and every term below 2^50 (to_fixed's range).
 __device__ __forceinline__ double fixed_scale(const uint32_t *bound, const StepAdjoint &a) {
     const double B = adj_bound(bound, a);
     if (!(B > 0.0) || !(B < 1.0e300)) return 1.0;
-    return 2305843009213693952.0 / (B * 5.0 * (double)(a.n > 0 ? a.n : 1));   // 2^61
+    const double cap = 2305843009213693952.0 / (5.0 * (double)(a.n > 0 ? a.n : 1));   // 2^61
+    return fmin(cap, 1125899906842624.0) / B;   // 2^50
 }
 
+// round(v * scale) as a two's-complement 64-bit integer, for
+// |v * scale| < 2^51: one FP64 FMA onto 1.5 * 2^52 rounds the product to an
+// integer (the sum lies in [2^52, 2^53), where doubles are the integers), and
+// there the bit patterns differ by the integers' difference.
 __device__ __forceinline__ unsigned long long to_fixed(float v, double scale) {
-    return (unsigned long long)__double2ll_rn((double)v * scale);
+    constexpr double kMagic = 6755399441055744.0;   // 1.5 * 2^52
+    const double y = fma((double)v, scale, kMagic);
+    return (unsigned long long)(__double_as_longlong(y) - __double_as_longlong(kMagic));
 }
 
 __global__ __launch_bounds__(kBlock) void k_adj_advect_brick(StepAdjoint a, AdjointSort so,

@@ -796,6 +796,9 @@ __device__ __forceinline__ void bscatter_comp(unsigned long long *acc, uint32_t 
             }
 }
 
+// OFF32: fewer than 2^30 sort slots, so that particle arrays are indexed by
+// 32-bit byte offsets (the array addresses stay in scalar registers).
+template <bool OFF32>
 __global__ __launch_bounds__(bscat::NT) void k_p2g_bscatter(
     const SceneDesc *__restrict__ scenes, int nsc, uint32_t nbin, int per_xcd,
     const uint32_t *__restrict__ pstart, const uint32_t *__restrict__ gperm, SortedOut copy,
@@ -846,15 +849,22 @@ __global__ __launch_bounds__(bscat::NT) void k_p2g_bscatter(
                           gperm ? sd.vin[2] : sd.p2g.vel[2]};
     const uint32_t p0 = (uint32_t)sd.p0;
     auto at = [&](uint32_t q) { return gperm ? gld(gperm, q) - p0 : q; };
+    auto ld = [&](const float *a, uint32_t i) { return OFF32 ? gld_at(a, i * 4u) : gld(a, i); };
+    auto st = [&](float *a, uint32_t i, float v) {
+        if (OFF32)
+            gst_at(a, i * 4u, v);
+        else
+            gst(a, i, v);
+    };
     // with copy.x[0] set, every particle read through gperm is also written
     // to its sort slot (for G2P to read in order)
     const bool copies = copy.x[0] != nullptr;
     auto put = [&](uint32_t q, float x, float y, float z, const float v[3]) {
-        copy.x[0][q] = x;
-        copy.x[1][q] = y;
-        copy.x[2][q] = z;
+        st(copy.x[0], q, x);
+        st(copy.x[1], q, y);
+        st(copy.x[2], q, z);
 #pragma unroll
-        for (int c = 0; c < 3; c++) copy.v[c][q] = v[c];
+        for (int c = 0; c < 3; c++) st(copy.v[c], q, v[c]);
     };
     {
         unsigned long long *z = &s_acc[0][0][0][0];
@@ -871,11 +881,11 @@ __global__ __launch_bounds__(bscat::NT) void k_p2g_bscatter(
     for (int m = 0; m < PRE; m++) {
         const uint32_t s = (uint32_t)(t + m * NT);
         const uint32_t q = at(q0 + (s < N ? s : 0u));
-        px[m] = gld(xp, q);
-        py[m] = gld(yp, q);
-        pz[m] = gld(zp, q);
+        px[m] = ld(xp, q);
+        py[m] = ld(yp, q);
+        pz[m] = ld(zp, q);
 #pragma unroll
-        for (int c = 0; c < 3; c++) pv[m][c] = gld(vp[c], q);
+        for (int c = 0; c < 3; c++) pv[m][c] = ld(vp[c], q);
     }
 #pragma unroll
     for (int m = 0; m < PRE; m++) {
@@ -892,7 +902,7 @@ __global__ __launch_bounds__(bscat::NT) void k_p2g_bscatter(
         const uint32_t q = at(q0 + s);
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            const float av = fabsf(gld(vp[c], q));
+            const float av = fabsf(ld(vp[c], q));
             if (av < kBig) vm[c] = fmaxf(vm[c], av);
         }
     }
@@ -976,8 +986,8 @@ __global__ __launch_bounds__(bscat::NT) void k_p2g_bscatter(
         if ((uint32_t)(t + m * NT) < N) scatter(px[m], py[m], pz[m], pv[m]);
     for (uint32_t s = (uint32_t)(t + PRE * NT); s < N; s += NT) {
         const uint32_t q = at(q0 + s);
-        const float v[3] = {gld(vp[0], q), gld(vp[1], q), gld(vp[2], q)};
-        const float x = gld(xp, q), y = gld(yp, q), z = gld(zp, q);
+        const float v[3] = {ld(vp[0], q), ld(vp[1], q), ld(vp[2], q)};
+        const float x = ld(xp, q), y = ld(yp, q), z = ld(zp, q);
         if (copies) put(q0 + s, x, y, z, v);
         scatter(x, y, z, v);
     }
@@ -1327,10 +1337,11 @@ __device__ __forceinline__ void g2p_bbrick(const StepG2P &a, const MacGrid &g, i
     // reads its particle at q and writes it at q - p0
     auto ld = [&](uint32_t q, float x[3], float v[3]) {
         if (SIMPLE) {
+            const uint32_t off = q * 4u;
 #pragma unroll
             for (int c = 0; c < 3; c++) {
-                x[c] = gld(a.xin[c], q);
-                v[c] = gld(a.vin[c], q);
+                x[c] = gld_at(a.xin[c], off);
+                v[c] = gld_at(a.vin[c], off);
             }
         } else {
             load_particle(a, q, x, v);
@@ -1338,11 +1349,11 @@ __device__ __forceinline__ void g2p_bbrick(const StepG2P &a, const MacGrid &g, i
     };
     auto put = [&](uint32_t q, const float x1[3], const float vn[3]) {
         if (SIMPLE) {
-            const uint32_t o = q - a.p0;
+            const uint32_t o = (q - a.p0) * 4u;
 #pragma unroll
             for (int c = 0; c < 3; c++) {
-                gst(a.xout[c], o, x1[c]);
-                gst(a.vout[c], o, vn[c]);
+                gst_at(a.xout[c], o, x1[c]);
+                gst_at(a.vout[c], o, vn[c]);
             }
         } else {
             store_particle(a, perm, q, x1, vn);
@@ -1610,7 +1621,8 @@ void flip_step_brick(const StepBatch &s, const BrickSort &bs, hipStream_t st, hi
         p2g_copy.v[c] = s.gather && s.xs[0] ? s.vs[c] : nullptr;
     }
     fill_u32((uint32_t *)s.tile_rows, 0u, 2 * std::max<int64_t>(s.tot.nrm, 1), st);
-    hipLaunchKernelGGL(k_p2g_bscatter, dim3((unsigned)(8 * (int64_t)per_xcd)), dim3(bscat::NT), 0, st,
+    hipLaunchKernelGGL(s.tot.n < ((int64_t)1 << 30) ? k_p2g_bscatter<true> : k_p2g_bscatter<false>,
+                       dim3((unsigned)(8 * (int64_t)per_xcd)), dim3(bscat::NT), 0, st,
                        s.scenes_dev, nsc, nbin, per_xcd, (const uint32_t *)bs.pstart,
                        s.gather ? (const uint32_t *)bs.perm : nullptr, p2g_copy, s.tiles,
                        s.tile_rows, bs.fluid);
@@ -1637,7 +1649,7 @@ void flip_step_brick(const StepBatch &s, const BrickSort &bs, hipStream_t st, hi
         so.perm = bs.perm;
     }
     // the brick G2P without the permutation read, particle ids or kept order
-    bool simple = !so.perm && !s.keep_order;
+    bool simple = !so.perm && !s.keep_order && s.tot.n < ((int64_t)1 << 30);
     for (const auto &d : s.scenes) simple = simple && !d.id_out;
     if (s.g2p_variant == 0) {
         const uint32_t nbrick = (uint32_t)s.tot.ng2p;

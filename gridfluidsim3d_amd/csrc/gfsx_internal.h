@@ -219,6 +219,23 @@ __device__ __forceinline__ void gst(T *p, size_t i, T v) {
     ((GT *)p)[i] = v;
 }
 
+// The element at byte offset off of an array in device memory: a 32-bit
+// offset lets the access take the array's address from scalar registers
+// (no 64-bit address arithmetic per lane).
+template <class T>
+__device__ __forceinline__ T gld_at(const T *p, uint32_t off) {
+    typedef const __attribute__((address_space(1))) char GC;
+    typedef const __attribute__((address_space(1))) T GT;
+    return *(GT *)((GC *)p + off);
+}
+
+template <class T>
+__device__ __forceinline__ void gst_at(T *p, uint32_t off, T v) {
+    typedef __attribute__((address_space(1))) char GC;
+    typedef __attribute__((address_space(1))) T GT;
+    *(GT *)((GC *)p + off) = v;
+}
+
 // Wyvill kernel for t = r^2/R^2 (scalarfield.cpp:569-571 rewritten as the
 // factorisation K = (1-t)^2 (1 - 4t/9) = a^2 (5/9 + 4a/9), a = 1-t):
 // exact zero for t >= 1 without a compare, and no cancellation near t = 1.

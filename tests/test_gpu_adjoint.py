@@ -184,6 +184,20 @@ def test_rollout_tape_matches_recompute(ctx):
     assert np.array_equal(grads[0][1], grads[1][1]) and np.array_equal(grads[0][2], grads[1][2])
 
 
+def test_step_tape_empty_step(ctx):
+    """A recorded step without particles: empty outputs, zero grids, empty
+    gradients, no error."""
+    dims, dx, dt = (6, 5, 4), 0.1, 0.01
+    x = torch.empty((3, 0), dtype=torch.float32, device="cuda")
+    v = torch.empty((3, 0), dtype=torch.float32, device="cuda")
+    xo, vo, tape = ctx.flip_step_record(x, v, dims, dx, dt)
+    assert xo.shape == (3, 0) and vo.shape == (3, 0)
+    for g in tape.grids[2]:
+        assert not bool(g.any())
+    gx, gv = ctx.flip_step_backward(x, v, x, v, dims, dx, dt, tape=tape)
+    assert gx.shape == (3, 0) and gv.shape == (3, 0)
+
+
 def test_step_tape_rejects_mismatch(ctx):
     import ctypes
     from gridfluidsim3d_amd import gfsx

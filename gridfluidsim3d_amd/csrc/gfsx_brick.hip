@@ -1235,7 +1235,11 @@ __global__ __launch_bounds__(kBlock) void k_p2g_bcombine(
 // stage within 0.99 cells of the start.  Returns false, computing nothing,
 // when the start position lies in the grid but outside the brick's cells:
 // then a stage may leave the boxes.  P2: inv_dx is a power of two.
-template <int ST, int GBE, bool P2>
+// INTERIOR: the brick lies at least two cells inside the grid on every
+// side, so a start cell in the brick and stages within 0.99 cells of the
+// start keep every sample position in the grid (no grid tests); a start
+// outside the grid is then left to the caller as well.
+template <int ST, int GBE, bool P2, bool INTERIOR = false>
 __device__ __forceinline__ bool g2p_particle_in(const StepG2P &a, const MacGrid &g,
                                                 const float *const cur[3],
                                                 const float *const old[3], int lx, int ly,
@@ -1253,17 +1257,19 @@ __device__ __forceinline__ bool g2p_particle_in(const StepG2P &a, const MacGrid 
         if (!((unsigned)ix < (unsigned)GBE && (unsigned)iy < (unsigned)GBE &&
               (unsigned)iz < (unsigned)GBE))
             return false;
+    } else if (INTERIOR) {
+        return false;
     }
     // outside the grid every sample is 0 and the stages stay at x0
     float k1[3], vo[3];
-    brick_sample_in<ST, true, GBE, P2>(g, cur[0], cur[1], cur[2], old[0], old[1], old[2], lx, ly,
+    brick_sample_in<ST, true, GBE, P2, INTERIOR>(g, cur[0], cur[1], cur[2], old[0], old[1], old[2], lx, ly,
                                        lz, lox, loy, loz, x0[0], x0[1], x0[2], k1, vo);
     flip_blend(a, v0, k1, vo, vnew);
     float xq[3], k2[3], k3[3], k4[3], unused[3];
     auto stage = [&](const float h, const float k[3], float out[3]) {
 #pragma unroll
         for (int c = 0; c < 3; c++) xq[c] = x0[c] + h * k[c];
-        brick_sample_in<ST, false, GBE, P2>(g, cur[0], cur[1], cur[2], nullptr, nullptr, nullptr,
+        brick_sample_in<ST, false, GBE, P2, INTERIOR>(g, cur[0], cur[1], cur[2], nullptr, nullptr, nullptr,
                                             lx, ly, lz, 0, 0, 0, xq[0], xq[1], xq[2], out, unused);
     };
     stage(a.rk.half, k1, k2);
@@ -1405,27 +1411,38 @@ __device__ __forceinline__ void g2p_bbrick(const StepG2P &a, const MacGrid &g, i
         return;
     }
     // the next particle is loaded while this one is computed
-    float xn[3] = {0.0f, 0.0f, 0.0f}, vn[3] = {0.0f, 0.0f, 0.0f};
-    if (q0 + (uint32_t)t < q1) ld(q0 + t, xn, vn);
-    for (uint32_t q = q0 + (uint32_t)t; q < q1; q += NT) {
-        const float x0[3] = {xn[0], xn[1], xn[2]};
-        const float v0[3] = {vn[0], vn[1], vn[2]};
-        if (q + NT < q1) ld(q + NT, xn, vn);
-        float x1[3], vnew[3];
-        const bool done = g2p_particle_in<ST, GBE, P2>(a, g, cur, old, lx, ly, lz, lox, loy, loz,
-                                                       c0x, c0y, c0z, x0, v0, x1, vnew);
-        if (done) put(q, x1, vnew);
-        // the rest to the slow list, one atomic per wave (by the lowest lane
-        // that has one: lanes past the end have left the loop)
-        const uint64_t mk = __ballot(!done);
-        if (mk) {
-            const int lead = __ffsll((unsigned long long)mk) - 1;
-            uint32_t base = 0;
-            if (lane == lead) base = atomicAdd(slow.count, (uint32_t)__popcll(mk));
-            base = __shfl(base, lead, 64);
-            if (!done) slow.slots[base + (uint32_t)__popcll(mk & ((1ull << lane) - 1ull))] = q;
+    auto run = [&](auto interior) {
+        constexpr bool INTERIOR = decltype(interior)::value;
+        float xn[3] = {0.0f, 0.0f, 0.0f}, vn[3] = {0.0f, 0.0f, 0.0f};
+        if (q0 + (uint32_t)t < q1) ld(q0 + t, xn, vn);
+        for (uint32_t q = q0 + (uint32_t)t; q < q1; q += NT) {
+            const float x0[3] = {xn[0], xn[1], xn[2]};
+            const float v0[3] = {vn[0], vn[1], vn[2]};
+            if (q + NT < q1) ld(q + NT, xn, vn);
+            float x1[3], vnew[3];
+            const bool done = g2p_particle_in<ST, GBE, P2, INTERIOR>(
+                a, g, cur, old, lx, ly, lz, lox, loy, loz, c0x, c0y, c0z, x0, v0, x1, vnew);
+            if (done) put(q, x1, vnew);
+            // the rest to the slow list, one atomic per wave (by the lowest
+            // lane that has one: lanes past the end have left the loop)
+            const uint64_t mk = __ballot(!done);
+            if (mk) {
+                const int lead = __ffsll((unsigned long long)mk) - 1;
+                uint32_t base = 0;
+                if (lane == lead) base = atomicAdd(slow.count, (uint32_t)__popcll(mk));
+                base = __shfl(base, lead, 64);
+                if (!done) slow.slots[base + (uint32_t)__popcll(mk & ((1ull << lane) - 1ull))] = q;
+            }
         }
-    }
+    };
+    // two cells of margin to every grid face (a stage moves under a cell;
+    // the margin also covers the rounding of positions)
+    const bool interior = c0x >= 2 && c0y >= 2 && c0z >= 2 && c0x + GBE + 2 <= g.ni &&
+                          c0y + GBE + 2 <= g.nj && c0z + GBE + 2 <= g.nk;
+    if (interior)
+        run(std::true_type{});
+    else
+        run(std::false_type{});
 }
 
 // One work-group per G2P brick of 8^3 cells of any scene.

@@ -417,7 +417,8 @@ __device__ __forceinline__ void brick_sample(const MacGrid &g, const float *o0,
 // boxes: one base index per box from the half-shifted cells, the unshifted
 // axis of each component one step further.  The same arithmetic as the
 // in-box branch of brick_sample, so the results are the same bits.
-template <int ST, bool OLD, int GBE = gbrick::GB, bool P2 = false>
+// INGRID: the position is known to lie in the grid (no test).
+template <int ST, bool OLD, int GBE = gbrick::GB, bool P2 = false, bool INGRID = false>
 __device__ __forceinline__ void brick_sample_in(const MacGrid &g,
                                                 const float *__restrict__ cur0,
                                                 const float *__restrict__ cur1,
@@ -430,8 +431,8 @@ __device__ __forceinline__ void brick_sample_in(const MacGrid &g,
                                                 float x, float y, float z,
                                                 float out[3], float out_old[3]) {
     using B = gbrick::Box<ST, GBE>;
-    if (!(x >= 0.0f && y >= 0.0f && z >= 0.0f && x < g.limf[0] && y < g.limf[1] &&
-          z < g.limf[2])) {
+    if (!INGRID && !(x >= 0.0f && y >= 0.0f && z >= 0.0f && x < g.limf[0] && y < g.limf[1] &&
+                     z < g.limf[2])) {
         out[0] = out[1] = out[2] = 0.0f;
         if (OLD) out_old[0] = out_old[1] = out_old[2] = 0.0f;
         return;

@@ -457,6 +457,11 @@ def _g2p_variant_scene(kind):
     if kind == "blobs":
         sc = scenes.random_blobs(22, (27, 18, 13), 0.1, amplitude=2.0)
         return sc.pos, sc.vel, sc.dims, 0.1, sc.dt
+    if kind == "interior":
+        # 8^3-cell bricks two cells or more inside the grid on every side,
+        # dx a power of two (FP32 cell coordinates)
+        sc = scenes.random_blobs(23, (40, 36, 44), 1.0 / 32.0, amplitude=2.0)
+        return sc.pos, sc.vel, sc.dims, 1.0 / 32.0, sc.dt
     if kind == "tiny":
         # one brick per axis, holding both pad layers
         dims = (5, 6, 7)
@@ -472,13 +477,14 @@ def _g2p_variant_scene(kind):
     return pos, vel, dims, 0.1, 0.01
 
 
-@pytest.mark.parametrize("kind", ["blobs", "tiny", "wild"])
+@pytest.mark.parametrize("kind", ["blobs", "tiny", "wild", "interior"])
 @pytest.mark.parametrize("stencil", [TRILINEAR, TRICUBIC])
 @pytest.mark.parametrize("keep", [False, True])
 def test_step_g2p_variants_bit_identical(ctx, kind, stencil, keep):
     """The LDS-staged G2P (0) over bricks of 8^3 and of 4^3 cells and the
     direct one (1) give identical bits, including samples that fall back to
-    global memory and particles outside the bin grid."""
+    global memory, particles outside the bin grid and bricks far enough
+    inside the grid that their samples skip the grid tests."""
     pos, vel, dims, dx, dt = _g2p_variant_scene(kind)
     old = ctx.get_option("g2p_variant"), ctx.get_option("g2p_brick")
     out = []

@@ -1319,8 +1319,8 @@ struct G2PSlow {
     uint32_t *count;      // number of slots
 };
 
-// The largest |value| of three per-thread maxima over the work-group (NaN
-// if any is NaN), in s_max[w][c] per wave first.
+// max(a, b), NaN if either is NaN (a wave's largest |value| per component;
+// a NaN value makes its brick take the checked path).
 __device__ __forceinline__ float nan_max(float a, float b) {
     return (a != a || b != b) ? __builtin_nanf("") : fmaxf(a, b);
 }
@@ -1367,7 +1367,7 @@ __device__ __forceinline__ void g2p_bbrick(const StepG2P &a, const MacGrid &g, i
     };
     __shared__ float s_cur[3][B::E * B::E * B::E];
     __shared__ float s_old[3][B::EO * B::EO * B::EO];
-    __shared__ float s_max[NT / 64][3];
+    __shared__ uint8_t s_fast[NT / 64];   // per wave: its stages stay near their starts
     if (q1 == q0) return;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int lx = c0x - B::LO, ly = c0y - B::LO, lz = c0z - B::LO;
@@ -1379,23 +1379,20 @@ __device__ __forceinline__ void g2p_bbrick(const StepG2P &a, const MacGrid &g, i
         stage_mac_boxes<B::E>(bc, g.f, g.ni, g.nj, g.nk, lx, ly, lz, m);
         stage_mac_boxes<B::EO>(bo, a.old, g.ni, g.nj, g.nk, lox, loy, loz, mo);
     }
+    const float reach = fabsf(a.rk.full) * (float)g.inv_dx * (ST == 0 ? 1.0f : 2.0f);
+    bool wfast = true;
 #pragma unroll
     for (int c = 0; c < 3; c++) {
         float v = m[c];
 #pragma unroll
         for (int d = 32; d > 0; d >>= 1) v = nan_max(v, __shfl_xor(v, d, 64));
-        if (lane == 0) s_max[wave][c] = v;
+        wfast = wfast && reach * v <= 0.99f;
     }
+    if (lane == 0) s_fast[wave] = wfast ? 1 : 0;
     __syncthreads();
-    const float reach = fabsf(a.rk.full) * (float)g.inv_dx * (ST == 0 ? 1.0f : 2.0f);
     bool fast = true;
 #pragma unroll
-    for (int c = 0; c < 3; c++) {
-        float v = s_max[0][c];
-#pragma unroll
-        for (int w = 1; w < NT / 64; w++) v = nan_max(v, s_max[w][c]);
-        fast = fast && reach * v <= 0.99f;
-    }
+    for (int w = 0; w < NT / 64; w++) fast = fast && s_fast[w] != 0;
     const float *const cur[3] = {s_cur[0], s_cur[1], s_cur[2]};
     const float *const old[3] = {s_old[0], s_old[1], s_old[2]};
     if (!fast) {
@@ -1483,9 +1480,9 @@ __device__ __forceinline__ void g2p_b4_kernel(const SceneDesc *__restrict__ scen
                                   slow);
 }
 
-// Trilinear: four work-groups per CU fit in LDS; four waves per SIMD.
+// Trilinear: five work-groups per CU fit in LDS; five waves per SIMD.
 template <bool P2, bool SIMPLE>
-__global__ __launch_bounds__(gbrick::NT, 4) void k_flip_g2p_bb_tl(
+__global__ __launch_bounds__(gbrick::NT, 5) void k_flip_g2p_bb_tl(
     const SceneDesc *__restrict__ scenes, int nsc, Sorted so, int keep_order, uint32_t nbrick,
     int per_xcd, const uint32_t *__restrict__ pstart, const uint32_t *__restrict__ perm,
     G2PSlow slow) {
